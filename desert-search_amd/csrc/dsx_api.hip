@@ -1,0 +1,264 @@
+// C ABI of libdsx.so (include/dsx.h): context management and the seed-gate host loop.
+#include "../../include/dsx.h"
+#include "dsx_internal.h"
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace {
+thread_local std::string g_last_error;
+
+int hip_fail(hipError_t e, const char *what, int line) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s: %s (dsx_api.hip:%d)", what, hipGetErrorString(e), line);
+    g_last_error = buf;
+    return DSX_E_HIP;
+}
+#define HIPCK(x)                                                   \
+    do {                                                           \
+        hipError_t e_ = (x);                                       \
+        if (e_ != hipSuccess) return hip_fail(e_, #x, __LINE__);   \
+    } while (0)
+
+// Gate blur weights: expf(-0.5 d^2 / sig^2) in float, as gate.cuh:216 forms them, evaluated here
+// with the host libm; ksum is their float sum in index order and norm = 1 / (ksum * ksum).
+dsx::GateKernel gate_kernel(int R, float sig) {
+    dsx::GateKernel K{};
+    for (int t = 0; t <= 2 * R; t++) {
+        const float d = float(t - R);
+        K.k[t] = expf(-0.5f * d * d / (sig * sig));
+    }
+    float ks = 0;
+    for (int t = 0; t <= 2 * R; t++) ks += K.k[t];
+    K.norm = 1.f / (ks * ks);
+    K.w = float(1.25 * (32.0 / 63.0) * 1.5);
+    return K;
+}
+}  // namespace
+
+struct dsx_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    // gate buffers, sized for one batch
+    uint32_t cap1 = 0;
+    uint32_t *d_k1 = nullptr, *d_i2 = nullptr, *d_k3 = nullptr, *d_cnt = nullptr;
+    float *d_a3 = nullptr, *d_all = nullptr;
+    dsx::SideParams *d_sides = nullptr;
+    uint64_t *d_seeds = nullptr;
+    uint32_t *h_cnt = nullptr, *h_k3 = nullptr;
+    float *h_a3 = nullptr;
+    hipEvent_t ev[8] = {};
+    double prof_ms[4] = {0, 0, 0, 0};
+    uint64_t prof_work[4] = {0, 0, 0, 0};
+};
+
+static int ctx_alloc(dsx_ctx *c);
+
+extern "C" {
+
+const char *dsx_strerror(int code) {
+    switch (code) {
+    case DSX_OK: return "ok";
+    case DSX_E_ARG: return "invalid argument";
+    case DSX_E_HIP: return "HIP runtime error";
+    case DSX_E_CAPACITY: return "capacity exceeded";
+    case DSX_E_NODEVICE: return "no usable gfx950 device";
+    default: return "unknown error";
+    }
+}
+const char *dsx_last_error(void) { return g_last_error.c_str(); }
+
+int dsx_create(int device, dsx_ctx **out) {
+    if (!out) return DSX_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+        g_last_error = "no HIP device";
+        return DSX_E_NODEVICE;
+    }
+    HIPCK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCK(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        g_last_error = std::string("device is ") + prop.gcnArchName + ", library built for gfx950";
+        return DSX_E_NODEVICE;
+    }
+    dsx_ctx *c = new dsx_ctx();
+    c->device = device;
+    const int rc = ctx_alloc(c);
+    if (rc != DSX_OK) {
+        dsx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return DSX_OK;
+}
+
+static int ctx_alloc(dsx_ctx *c) {
+    HIPCK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    c->cap1 = 1u << 18;
+    const size_t n1 = c->cap1;
+    HIPCK(hipMalloc(&c->d_k1, n1 * 4));
+    HIPCK(hipMalloc(&c->d_i2, n1 * 4));
+    HIPCK(hipMalloc(&c->d_k3, n1 * 4));
+    HIPCK(hipMalloc(&c->d_a3, n1 * 4));
+    HIPCK(hipMalloc(&c->d_all, n1 * 4));
+    HIPCK(hipMalloc(&c->d_cnt, 16));
+    HIPCK(hipMalloc(&c->d_sides, 2 * n1 * sizeof(dsx::SideParams)));
+    HIPCK(hipMalloc(&c->d_seeds, n1 * 8));
+    HIPCK(hipHostMalloc(&c->h_cnt, 16, hipHostMallocDefault));
+    HIPCK(hipHostMalloc(&c->h_k3, n1 * 4, hipHostMallocDefault));
+    HIPCK(hipHostMalloc(&c->h_a3, n1 * 4, hipHostMallocDefault));
+    for (auto &e : c->ev) HIPCK(hipEventCreate(&e));
+    return DSX_OK;
+}
+
+int dsx_destroy(dsx_ctx *c) {
+    if (!c) return DSX_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (void *p : {(void *)c->d_k1, (void *)c->d_i2, (void *)c->d_k3, (void *)c->d_a3, (void *)c->d_all, (void *)c->d_cnt,
+                    (void *)c->d_sides, (void *)c->d_seeds})
+        if (p) (void)hipFree(p);
+    for (void *p : {(void *)c->h_cnt, (void *)c->h_k3, (void *)c->h_a3})
+        if (p) (void)hipHostFree(p);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return DSX_OK;
+}
+
+static bool valid_gate_params(const dsx_gate_params *p) {
+    return p && p->box >= 0 && p->box <= 1 && std::isfinite(p->max_phase) && std::isfinite(p->amp) &&
+           std::isfinite(p->pthr) && std::isfinite(p->margin) && p->sig > 0 && std::isfinite(p->sig);
+}
+
+int dsx_gate(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *p, uint64_t *seeds, float *amps,
+             uint64_t *kidx, uint64_t cap, uint64_t *n_out, dsx_gate_counts *counts) {
+    if (!c || !valid_gate_params(p) || !n_out) return DSX_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    const dsx::GateKernel kpre = gate_kernel(4, p->box ? p->sig : 1.0f);
+    const dsx::GateKernel kfull = gate_kernel(7, 2.0f);
+    const float thr_pre = p->box ? p->pthr : p->amp - p->margin;
+    const int mode_pre = p->box ? 0 : 2;
+    const double maxph = double(p->max_phase);
+    std::vector<std::pair<uint64_t, float>> pass;
+    uint64_t tot[3] = {0, 0, 0};
+    for (int i = 0; i < 4; i++) { c->prof_ms[i] = 0; c->prof_work[i] = 0; }
+    uint64_t B = 1ull << 24;   // seed indices per batch (expected phase survivors ~0.72% << cap1)
+    for (uint64_t base = 0; base < count;) {
+        const uint32_t nb = uint32_t(std::min<uint64_t>(B, count - base));
+        HIPCK(hipMemsetAsync(c->d_cnt, 0, 16, c->st));
+        HIPCK(hipEventRecord(c->ev[0], c->st));
+        dsx::launch_phase(c->st, start + base, nb, maxph, c->d_k1, c->d_cnt, c->cap1);
+        HIPCK(hipGetLastError());
+        HIPCK(hipEventRecord(c->ev[1], c->st));
+        HIPCK(hipMemcpyAsync(c->h_cnt, c->d_cnt, 4, hipMemcpyDeviceToHost, c->st));
+        HIPCK(hipStreamSynchronize(c->st));
+        const uint32_t n1 = c->h_cnt[0];
+        if (n1 > c->cap1) {                // phase survivors overflow the batch buffers: smaller batch
+            B = std::max<uint64_t>(1, B / 2);
+            continue;
+        }
+        float ms;
+        HIPCK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        c->prof_ms[0] += ms;
+        c->prof_work[0] += nb;
+        uint32_t n2 = 0, n3 = 0;
+        if (n1) {
+            HIPCK(hipEventRecord(c->ev[2], c->st));
+            dsx::launch_init(c->st, start + base, c->d_k1, nullptr, n1, c->d_sides);
+            HIPCK(hipGetLastError());
+            HIPCK(hipEventRecord(c->ev[3], c->st));
+            dsx::launch_amp(c->st, mode_pre, c->d_sides, nullptr, n1, kpre, thr_pre, nullptr, c->d_i2, nullptr,
+                            c->d_cnt + 1, c->cap1, nullptr);
+            HIPCK(hipGetLastError());
+            HIPCK(hipEventRecord(c->ev[4], c->st));
+            HIPCK(hipMemcpyAsync(c->h_cnt + 1, c->d_cnt + 1, 4, hipMemcpyDeviceToHost, c->st));
+            HIPCK(hipStreamSynchronize(c->st));
+            n2 = std::min(c->h_cnt[1], c->cap1);
+            HIPCK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+            c->prof_ms[1] += ms;
+            c->prof_work[1] += 2ull * n1;
+            HIPCK(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
+            c->prof_ms[2] += ms;
+            c->prof_work[2] += n1;
+            if (n2) {
+                HIPCK(hipEventRecord(c->ev[5], c->st));
+                dsx::launch_amp(c->st, 1, c->d_sides, c->d_i2, n2, kfull, p->amp, c->d_k1, c->d_k3, c->d_a3, c->d_cnt + 2,
+                                c->cap1, nullptr);
+                HIPCK(hipGetLastError());
+                HIPCK(hipEventRecord(c->ev[6], c->st));
+                HIPCK(hipMemcpyAsync(c->h_cnt + 2, c->d_cnt + 2, 4, hipMemcpyDeviceToHost, c->st));
+                HIPCK(hipStreamSynchronize(c->st));
+                n3 = std::min(c->h_cnt[2], c->cap1);
+                HIPCK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
+                c->prof_ms[3] += ms;
+                c->prof_work[3] += n2;
+                if (n3) {
+                    HIPCK(hipMemcpyAsync(c->h_k3, c->d_k3, size_t(n3) * 4, hipMemcpyDeviceToHost, c->st));
+                    HIPCK(hipMemcpyAsync(c->h_a3, c->d_a3, size_t(n3) * 4, hipMemcpyDeviceToHost, c->st));
+                    HIPCK(hipStreamSynchronize(c->st));
+                    for (uint32_t i = 0; i < n3; i++) pass.push_back({base + c->h_k3[i], c->h_a3[i]});
+                }
+            }
+        }
+        tot[0] += n1;
+        tot[1] += n2;
+        tot[2] += n3;
+        base += nb;
+    }
+    std::sort(pass.begin(), pass.end());
+    if (counts) { counts->phase = tot[0]; counts->pre = tot[1]; counts->full = tot[2]; }
+    *n_out = pass.size();
+    const uint64_t nw = std::min<uint64_t>(cap, pass.size());
+    for (uint64_t i = 0; i < nw; i++) {
+        if (seeds) seeds[i] = dsx::mix64(start + pass[i].first);
+        if (amps) amps[i] = pass[i].second;
+        if (kidx) kidx[i] = pass[i].first;
+    }
+    return pass.size() > cap ? DSX_E_CAPACITY : DSX_OK;
+}
+
+int dsx_gate_amps(dsx_ctx *c, const uint64_t *seeds, uint64_t n, const dsx_gate_params *p, float *pre, float *full) {
+    if (!c || !valid_gate_params(p) || (n && !seeds)) return DSX_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    const dsx::GateKernel kpre = gate_kernel(4, p->box ? p->sig : 1.0f);
+    const dsx::GateKernel kfull = gate_kernel(7, 2.0f);
+    const int mode_pre = p->box ? 0 : 2;
+    for (uint64_t b = 0; b < n; b += c->cap1) {
+        const uint32_t m = uint32_t(std::min<uint64_t>(c->cap1, n - b));
+        HIPCK(hipMemcpyAsync(c->d_seeds, seeds + b, size_t(m) * 8, hipMemcpyHostToDevice, c->st));
+        HIPCK(hipMemsetAsync(c->d_cnt, 0, 16, c->st));
+        dsx::launch_init(c->st, 0, nullptr, c->d_seeds, m, c->d_sides);
+        HIPCK(hipGetLastError());
+        if (pre) {
+            dsx::launch_amp(c->st, mode_pre, c->d_sides, nullptr, m, kpre, 1e30f, nullptr, c->d_i2, nullptr, c->d_cnt,
+                            c->cap1, c->d_all);
+            HIPCK(hipGetLastError());
+            HIPCK(hipMemcpyAsync(pre + b, c->d_all, size_t(m) * 4, hipMemcpyDeviceToHost, c->st));
+        }
+        if (full) {
+            dsx::launch_amp(c->st, 1, c->d_sides, nullptr, m, kfull, 1e30f, nullptr, c->d_i2, nullptr, c->d_cnt + 1,
+                            c->cap1, c->d_a3);
+            HIPCK(hipGetLastError());
+            HIPCK(hipMemcpyAsync(full + b, c->d_a3, size_t(m) * 4, hipMemcpyDeviceToHost, c->st));
+        }
+        HIPCK(hipStreamSynchronize(c->st));
+    }
+    return DSX_OK;
+}
+
+int dsx_gate_profile(dsx_ctx *c, double ms[4], uint64_t work[4]) {
+    if (!c || !ms || !work) return DSX_E_ARG;
+    for (int i = 0; i < 4; i++) { ms[i] = c->prof_ms[i]; work[i] = c->prof_work[i]; }
+    return DSX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// Internal declarations shared by the HIP translation units of libdsx.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "dsx_rng.h"
+
+namespace dsx {
+
+// Temperature octave-0 Perlin of one noise side (the reference's tg::PerlinParams, gate.cuh:49).
+struct SideParams {
+    uint8_t perm[256];
+    double ox, oz, fy, vy;
+    int iy, pad_;
+};
+
+// Gate blur kernel as the kernels consume it: 2R+1 unnormalised float weights, the float
+// normalisation 1/(ksum*ksum) and the octave weight w = float(1.25 * 32/63 * 1.5).
+struct GateKernel {
+    float k[15];
+    float norm;
+    float w;
+};
+
+void launch_phase(hipStream_t st, uint64_t base, uint32_t n, double maxph, uint32_t *out, uint32_t *count, uint32_t cap);
+void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64_t *seeds, uint32_t n, SideParams *out);
+void launch_amp(hipStream_t st, int mode, const SideParams *sides, const uint32_t *sel, uint32_t n, const GateKernel &K,
+                float thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count, uint32_t cap,
+                float *all_amp);
+
+}  // namespace dsx

@@ -1,0 +1,129 @@
+"""Python binding of libdsx.so (include/dsx.h) - the host-side mirror of the reference's GPU engine.
+
+The reference exposes its GPU path as C++ functions in ``src/gpu/gproxy.h`` and ``src/gpu/gate.cuh``
+(``tg::run_gate``, ``gproxy::start_gate``, the ``tgate`` tool's ``calib`` mode). This module binds the
+C ABI that replaces them, with the same argument meaning, via ctypes. It only loads the in-tree
+``libdsx.so``: if the library is missing or no gfx950 device is present, it raises - there is no CPU
+fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdsx.so")
+
+DSX_OK, DSX_E_ARG, DSX_E_HIP, DSX_E_CAPACITY, DSX_E_NODEVICE = 0, 1, 2, 3, 4
+
+
+class DsxError(RuntimeError):
+    def __init__(self, code, what, detail=""):
+        self.code = code
+        super().__init__(f"{what}: {detail or 'error'} (code {code})")
+
+
+class GateParams(ctypes.Structure):
+    """dsx_gate_params; defaults are the reference's production GPU gate (gproxy.cu:409, gate.cuh:349-355
+    with dsearch_desert.cpp:76 gate_phase/gate_amp/gate_pthr)."""
+    _fields_ = [("max_phase", ctypes.c_float), ("amp", ctypes.c_float), ("pthr", ctypes.c_float),
+                ("margin", ctypes.c_float), ("sig", ctypes.c_float), ("box", ctypes.c_int)]
+
+    def __init__(self, max_phase=0.06, amp=0.70, pthr=0.645, margin=0.01, sig=1.0, box=1):
+        super().__init__(max_phase, amp, pthr, margin, sig, box)
+
+
+class GateCounts(ctypes.Structure):
+    _fields_ = [("phase", ctypes.c_uint64), ("pre", ctypes.c_uint64), ("full", ctypes.c_uint64)]
+
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libdsx.so (once). Raises if the built library is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise DsxError(DSX_E_NODEVICE, "libdsx.so not built", path)
+    lib = ctypes.CDLL(path)
+    lib.dsx_strerror.restype = ctypes.c_char_p
+    lib.dsx_strerror.argtypes = [ctypes.c_int]
+    lib.dsx_last_error.restype = ctypes.c_char_p
+    lib.dsx_last_error.argtypes = []
+    lib.dsx_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    lib.dsx_destroy.argtypes = [ctypes.c_void_p]
+    lib.dsx_gate.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(GateParams), _u64p, _f32p,
+                             _u64p, ctypes.c_uint64, _u64p, ctypes.POINTER(GateCounts)]
+    lib.dsx_gate_amps.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, ctypes.POINTER(GateParams), _f32p, _f32p]
+    lib.dsx_gate_profile.argtypes = [ctypes.c_void_p, _f64p, _u64p]
+    for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile"):
+        getattr(lib, f).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct)) if a is not None else None
+
+
+class Engine:
+    """One device context (dsx_ctx). Not shared between threads."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        self._check(self.lib.dsx_create(device, ctypes.byref(h)), "dsx_create")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != DSX_OK:
+            raise DsxError(rc, what, (self.lib.dsx_strerror(rc) + b" " + self.lib.dsx_last_error()).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dsx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def gate(self, start, count, params=None, cap=1 << 20):
+        """Seeds passing the GPU gate over indices [start, start+count): (seeds u64, amps f32, k u64, counts)."""
+        p = params or GateParams()
+        seeds = np.zeros(cap, np.uint64)
+        amps = np.zeros(cap, np.float32)
+        kidx = np.zeros(cap, np.uint64)
+        n = ctypes.c_uint64()
+        cnt = GateCounts()
+        rc = self.lib.dsx_gate(self.h, start, count, ctypes.byref(p), _ptr(seeds, ctypes.c_uint64),
+                               _ptr(amps, ctypes.c_float), _ptr(kidx, ctypes.c_uint64), cap, ctypes.byref(n),
+                               ctypes.byref(cnt))
+        self._check(rc, "dsx_gate")
+        m = n.value
+        return seeds[:m], amps[:m], kidx[:m], (cnt.phase, cnt.pre, cnt.full)
+
+    def gate_amps(self, seeds, params=None):
+        """(pre-gate amplitude, full amplitude) per seed, no thresholds (tgate calib mode)."""
+        p = params or GateParams()
+        s = np.ascontiguousarray(seeds, dtype=np.uint64)
+        pre = np.zeros(len(s), np.float32)
+        full = np.zeros(len(s), np.float32)
+        self._check(self.lib.dsx_gate_amps(self.h, _ptr(s, ctypes.c_uint64), len(s), ctypes.byref(p),
+                                           _ptr(pre, ctypes.c_float), _ptr(full, ctypes.c_float)), "dsx_gate_amps")
+        return pre, full
+
+    def gate_profile(self):
+        ms = np.zeros(4, np.float64)
+        work = np.zeros(4, np.uint64)
+        self._check(self.lib.dsx_gate_profile(self.h, _ptr(ms, ctypes.c_double), _ptr(work, ctypes.c_uint64)),
+                    "dsx_gate_profile")
+        return ms, work

@@ -1,0 +1,79 @@
+/*
+ * dsx.h - C ABI of the MI355X desert-search engine (libdsx.so, gfx950 HIP kernels).
+ *
+ * This is the drop-in boundary for desert-search's per-seed pipeline. Each entry point replaces a
+ * function of the reference's GPU engine, whose C++ interface lives in src/gpu/gproxy.h and
+ * src/gpu/gate.cuh; the reference file:line is cited on each declaration. Unlike the reference:
+ *   - every call takes an explicit per-device context (the reference keeps file-scope statics);
+ *   - errors are integer codes (DSX_OK == 0), never exit(); dsx_strerror() names them;
+ *   - outputs go to caller-owned buffers with explicit capacities; overflow is DSX_E_CAPACITY with the
+ *     full count returned, never a silent truncation (gate.cuh:387,395,405 truncate silently);
+ *   - results are canonical: survivor lists are sorted by seed index, so they do not depend on GPU
+ *     scheduling (the reference's atomic appends do).
+ * A context may be used by one host thread at a time; distinct contexts are independent.
+ * No torch types, no STL: plain pointers and sizes only.
+ */
+#ifndef DSX_H
+#define DSX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+enum {
+    DSX_OK = 0,
+    DSX_E_ARG = 1,       /* invalid argument (null pointer, bad size or mode) */
+    DSX_E_HIP = 2,       /* a HIP runtime call failed (message via dsx_last_error) */
+    DSX_E_CAPACITY = 3,  /* caller buffer or internal queue too small; *n_out holds the true count */
+    DSX_E_NODEVICE = 4,  /* no usable gfx950 device */
+};
+const char *dsx_strerror(int code);
+/* Text of the last HIP failure on this thread (empty if none). */
+const char *dsx_last_error(void);
+
+/* ---------------------------------------------------------------- context */
+typedef struct dsx_ctx dsx_ctx;
+/* Creates the engine on HIP device `device` and uploads the constant tables.
+ * Replaces gproxy::init (gproxy.h:35, gproxy.cu:391-401) minus its detached thread. */
+int dsx_create(int device, dsx_ctx **out);
+int dsx_destroy(dsx_ctx *ctx);
+
+/* ---------------------------------------------------------------- seed gate (S0) */
+typedef struct {
+    float max_phase; /* y-phase cut, pass if phase <= (double)max_phase   (GateCfg::maxph, 0.06 in production) */
+    float amp;       /* full 512^2 amplitude gate threshold               (GateCfg::amp, 0.70) */
+    float pthr;      /* anti-aliased 256^2 pre-gate threshold, box=1      (GateCfg::pthr, 0.645) */
+    float margin;    /* legacy pre-gate threshold amp - margin, box=0     (GateCfg::margin, 0.01) */
+    float sig;       /* pre-gate blur sigma in grid cells                 (GateCfg::sig, 1.0) */
+    int box;         /* 1: 4-parity anti-aliased pre-gate; 0: legacy even-parity pre-gate */
+} dsx_gate_params;
+
+typedef struct {
+    uint64_t phase; /* seeds passing the y-phase filter */
+    uint64_t pre;   /* ... and the pre-gate */
+    uint64_t full;  /* ... and the full amplitude gate */
+} dsx_gate_counts;
+
+/* The GPU seed gate over seed indices k in [start, start+count), seed = splitmix64(start+k):
+ * y-phase filter -> pre-gate -> full 512^2 amplitude gate.
+ * Replaces tg::run_gate (gate.cuh:361-419) and gproxy::start_gate (gproxy.h:40-41).
+ * Survivors are written sorted by k: seeds[i] (the seed), amps[i] (full amplitude A+B), kidx[i] (k).
+ * Any of seeds/amps/kidx may be null. *n_out = number of survivors; DSX_E_CAPACITY if > cap. */
+int dsx_gate(dsx_ctx *ctx, uint64_t start, uint64_t count, const dsx_gate_params *p, uint64_t *seeds, float *amps,
+             uint64_t *kidx, uint64_t cap, uint64_t *n_out, dsx_gate_counts *counts);
+
+/* Pre-gate and full-gate amplitudes (A+B) of an explicit seed list, no thresholds applied.
+ * Replaces the calibration mode of the reference's gate tool (tgate.cu:53-85, `calib=N`). */
+int dsx_gate_amps(dsx_ctx *ctx, const uint64_t *seeds, uint64_t n, const dsx_gate_params *p, float *pre, float *full);
+
+/* Device time (ms) of the kernels of the last dsx_gate call, summed over its batches:
+ * ms[0] phase filter, ms[1] Perlin init, ms[2] pre-gate, ms[3] full gate; work[0..3] the number of
+ * items each processed (seed indices, sides, pre-gate seeds, full-gate seeds). */
+int dsx_gate_profile(dsx_ctx *ctx, double ms[4], uint64_t work[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSX_H */

@@ -1,0 +1,79 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures under tests/golden/ by running the reference's own CPU tools.
+
+The tools are built from the reference sources by ``make -f oracle/Makefile.ref`` into oracle/_ref/
+(this container only; /root/reference is not needed afterwards). Every fixture written here is data:
+inputs and the reference's outputs.
+
+  c1.*            dsearch_desert 0 200000 P 8           (config 1, CPU-default gate)
+  r3m.*           dsearch_desert 0 3000000 P 8          (larger range: exact step-8 floods)
+  rg1m.*          dsearch_desert 0 0 P 8 stdin=1 < gated_1e12_1m.txt (seeds gated with the GPU-gate
+                  rules over k in [1e12, 1e12 + 2^20); the list itself is an input fixture)
+  dflood.jsonl    dflood SEED X Z STEP at the verified desert finds, steps 8 and 4 (all rows) and
+                  step 1 (the first rows)
+Sorted CSV rows are stored (the reference writes rows in thread completion order).
+"""
+import concurrent.futures as cf
+import hashlib
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(REPO, "oracle", "_ref")
+
+
+def run_search(tag, args, stdin=None):
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, tag)
+        subprocess.run([os.path.join(REF, "dsearch_desert")] + [args[0], args[1], pre] + args[2:], check=True,
+                       stdin=open(stdin) if stdin else None, stderr=subprocess.DEVNULL)
+        rows = sorted(open(pre + ".csv").read().splitlines()[1:])
+        srows = sorted(open(pre + ".seeds.csv").read().splitlines()[1:])
+        js = json.load(open(pre + ".json"))
+    keep = {k: js[k] for k in ("start", "done", "passed", "scanned", "sites")}
+    keep["seeds_rows"] = len(srows)
+    keep["seeds_sha256"] = hashlib.sha256("\n".join(srows).encode()).hexdigest()
+    with open(os.path.join(HERE, tag + ".csv"), "w") as f:
+        f.write("\n".join(rows) + "\n")
+    if len(srows) <= 1000:
+        with open(os.path.join(HERE, tag + ".seeds.csv"), "w") as f:
+            f.write("\n".join(srows) + "\n")
+    with open(os.path.join(HERE, tag + ".json"), "w") as f:
+        json.dump(keep, f, indent=1)
+    print(tag, keep)
+
+
+def dflood(seed, x, z, step):
+    out = subprocess.run([os.path.join(REF, "dflood"), str(seed), str(x), str(z), str(step)], capture_output=True,
+                         text=True)
+    m = re.search(r"step (\d+): (\d+) cells", out.stdout)
+    if not m:
+        return {"seed": seed, "x": x, "z": z, "step": step, "cells": None, "edge": None}
+    return {"seed": seed, "x": x, "z": z, "step": step, "cells": int(m.group(2)) // (step * step),
+            "edge": "window edge" in out.stdout}
+
+
+def main():
+    run_search("c1", ["0", "200000", "8"])
+    run_search("r3m", ["0", "3000000", "8"])
+    run_search("rg1m", ["0", "0", "8", "stdin=1"], stdin=os.path.join(HERE, "gated_1e12_1m.txt"))
+    finds = [json.loads(l) for l in open(os.path.join(HERE, "desert_finds.jsonl"))]
+    pts = [(int(r["seed"]), r["x"], r["z"]) for r in finds if "x" in r and "seed" in r]
+    jobs = [(s, x, z, st) for st in (8, 4) for (s, x, z) in pts] + [(s, x, z, 1) for (s, x, z) in pts[:12]]
+    res = []
+    with cf.ThreadPoolExecutor(int(sys.argv[1]) if len(sys.argv) > 1 else 6) as ex:
+        for r in ex.map(lambda j: dflood(*j), jobs):
+            res.append(r)
+    with open(os.path.join(HERE, "dflood.jsonl"), "w") as f:
+        for r in res:
+            f.write(json.dumps(r) + "\n")
+    print("dflood", len(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,106 @@
+"""The CPU oracle against the reference's own outputs (golden fixtures written by
+tests/golden/make_goldens.py from the reference binaries) and the MC-Measure-verified finds."""
+import concurrent.futures as cf
+import ctypes
+import hashlib
+import json
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+
+def _run(oracle, tmp, tag, start=0, count=0, seeds=None):
+    pre = os.path.join(tmp, tag)
+    cnt = np.zeros(4, np.uint64)
+    sec = ctypes.c_double()
+    if seeds is not None:
+        arr = np.ascontiguousarray(seeds, dtype=np.uint64)
+        rc = oracle.or_run(0, 0, arr.ctypes.data, len(arr), 1, 8, pre.encode(), cnt.ctypes.data, ctypes.byref(sec),
+                           None, 0)
+    else:
+        rc = oracle.or_run(start, count, None, 0, 0, 8, pre.encode(), cnt.ctypes.data, ctypes.byref(sec), None, 0)
+    assert rc == 0
+    rows = sorted(open(pre + ".csv").read().splitlines()[1:])
+    srows = sorted(open(pre + ".seeds.csv").read().splitlines()[1:])
+    return rows, srows, [int(c) for c in cnt]
+
+
+def _check(tag, rows, srows, cnt):
+    gold = json.load(open(os.path.join(GOLDEN, tag + ".json")))
+    grows = open(os.path.join(GOLDEN, tag + ".csv")).read().splitlines()
+    assert rows == grows
+    assert len(srows) == gold["seeds_rows"]
+    assert hashlib.sha256("\n".join(srows).encode()).hexdigest() == gold["seeds_sha256"]
+    assert cnt == [gold["done"], gold["passed"], gold["scanned"], gold["sites"]]
+
+
+def test_config1_matches_reference(oracle):
+    """BASELINE config 1 (seeds mix(k), k < 200000, production Params): CSV, seeds CSV and counters."""
+    with tempfile.TemporaryDirectory() as td:
+        rows, srows, cnt = _run(oracle, td, "c1", 0, 200000)
+    _check("c1", rows, srows, cnt)
+    assert srows == open(os.path.join(GOLDEN, "c1.seeds.csv")).read().splitlines()
+
+
+def test_gated_list_matches_reference(oracle):
+    """stdin=1 mode on seeds gated with the GPU-gate rules (92 seeds, k in [1e12, 1e12 + 2^20))."""
+    seeds = [int(l.split()[0]) % (1 << 64) for l in open(os.path.join(GOLDEN, "gated_1e12_1m.txt"))]
+    with tempfile.TemporaryDirectory() as td:
+        rows, srows, cnt = _run(oracle, td, "rg1m", seeds=seeds)
+    _check("rg1m", rows, srows, cnt)
+
+
+def test_3m_range_matches_reference(oracle):
+    """3M seed indices: 392 scanned seeds including exact step-8 floods."""
+    with tempfile.TemporaryDirectory() as td:
+        rows, srows, cnt = _run(oracle, td, "r3m", 0, 3000000)
+    _check("r3m", rows, srows, cnt)
+
+
+def test_fixture_rows_consistent():
+    """The verified-finds fixture: blocks^2 = 16 * cells, CANDIDATE coordinates 4q + 2."""
+    n = 0
+    for line in open(os.path.join(GOLDEN, "desert_finds.jsonl")):
+        r = json.loads(line)
+        if "cells" not in r or "x" not in r:
+            continue
+        n += 1
+        assert r["cells"] * 16 == r["blocks2"]
+        assert r["x"] % 4 == 2 and r["z"] % 4 == 2
+    assert n == 220
+
+
+class FloodOut(ctypes.Structure):
+    _fields_ = [("cells", ctypes.c_uint64), ("sampled", ctypes.c_uint64), ("minx", ctypes.c_int32),
+                ("maxx", ctypes.c_int32), ("minz", ctypes.c_int32), ("maxz", ctypes.c_int32),
+                ("edge", ctypes.c_int32), ("border", ctypes.c_int32)]
+
+
+def _flood(oracle, seed, x, z, step):
+    out = FloodOut()
+    oracle.or_flood(seed % (1 << 64), 0, 0.35, x >> 2, z >> 2, step, 20000 // step, 0, ctypes.byref(out), None)
+    return out
+
+
+@pytest.mark.parametrize("step,every", [(8, 1), (4, 6)])
+def test_dflood_matches_reference(oracle, step, every):
+    """dflood SEED X Z STEP (bridged desert_state flood, 4-connected, half = 20000/step) at the finds."""
+    gold = [json.loads(l) for l in open(os.path.join(GOLDEN, "dflood.jsonl"))]
+    gold = [g for g in gold if g["step"] == step][::every]
+    with cf.ThreadPoolExecutor(8) as ex:
+        res = list(ex.map(lambda g: _flood(oracle, g["seed"], g["x"], g["z"], step), gold))
+    for g, r in zip(gold, res):
+        assert r.cells == g["cells"], g
+        assert bool(r.edge) == g["edge"]
+
+
+def test_dflood_step1_matches_mcmeasure(oracle):
+    """The exact step-1 flood at one verified find equals MC-Measure's cell count."""
+    finds = [json.loads(l) for l in open(os.path.join(GOLDEN, "desert_finds.jsonl"))]
+    r = finds[0]
+    out = _flood(oracle, int(r["seed"]), r["x"], r["z"], 1)
+    assert out.cells == r["cells"] and not out.edge and not out.border
