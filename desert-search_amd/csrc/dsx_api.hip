@@ -1,6 +1,5 @@
 // C ABI of libdsx.so (include/dsx.h): context management and the seed-gate host loop.
-#include "../../include/dsx.h"
-#include "dsx_internal.h"
+#include "ctx.h"
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -10,21 +9,18 @@
 #include <utility>
 #include <vector>
 
-namespace {
+namespace dsx {
 thread_local std::string g_last_error;
-
-int hip_fail(hipError_t e, const char *what, int line) {
-    char buf[256];
-    std::snprintf(buf, sizeof buf, "%s: %s (dsx_api.hip:%d)", what, hipGetErrorString(e), line);
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    char buf[320];
+    std::snprintf(buf, sizeof buf, "%s: %s (%s:%d)", what, hipGetErrorString(e), file, line);
     g_last_error = buf;
     return DSX_E_HIP;
 }
-#define HIPCK(x)                                                   \
-    do {                                                           \
-        hipError_t e_ = (x);                                       \
-        if (e_ != hipSuccess) return hip_fail(e_, #x, __LINE__);   \
-    } while (0)
+}  // namespace dsx
+using dsx::g_last_error;
 
+namespace {
 // Gate blur weights: expf(-0.5 d^2 / sig^2) in float, as gate.cuh:216 forms them, evaluated here
 // with the host libm; ksum is their float sum in index order and norm = 1 / (ksum * ksum).
 dsx::GateKernel gate_kernel(int R, float sig) {
@@ -40,22 +36,6 @@ dsx::GateKernel gate_kernel(int R, float sig) {
     return K;
 }
 }  // namespace
-
-struct dsx_ctx {
-    int device = 0;
-    hipStream_t st = nullptr;
-    // gate buffers, sized for one batch
-    uint32_t cap1 = 0;
-    uint32_t *d_k1 = nullptr, *d_i2 = nullptr, *d_k3 = nullptr, *d_cnt = nullptr;
-    float *d_a3 = nullptr, *d_all = nullptr;
-    dsx::SideParams *d_sides = nullptr;
-    uint64_t *d_seeds = nullptr;
-    uint32_t *h_cnt = nullptr, *h_k3 = nullptr;
-    float *h_a3 = nullptr;
-    hipEvent_t ev[8] = {};
-    double prof_ms[4] = {0, 0, 0, 0};
-    uint64_t prof_work[4] = {0, 0, 0, 0};
-};
 
 static int ctx_alloc(dsx_ctx *c);
 
@@ -123,7 +103,9 @@ int dsx_destroy(dsx_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (void *p : {(void *)c->d_k1, (void *)c->d_i2, (void *)c->d_k3, (void *)c->d_a3, (void *)c->d_all, (void *)c->d_cnt,
-                    (void *)c->d_sides, (void *)c->d_seeds})
+                    (void *)c->d_sides, (void *)c->d_seeds, (void *)c->d_tsp, (void *)c->d_tabs, (void *)c->d_bandmax,
+                    (void *)c->d_act, (void *)c->d_ncand, (void *)c->d_cand, (void *)c->d_ibT, (void *)c->d_ibC,
+                    (void *)c->d_loT, (void *)c->d_hiT})
         if (p) (void)hipFree(p);
     for (void *p : {(void *)c->h_cnt, (void *)c->h_k3, (void *)c->h_a3})
         if (p) (void)hipHostFree(p);

@@ -1,0 +1,35 @@
+// Period tables and translation scan: shared declarations (tables.hip, scan host code).
+#pragma once
+#include "dsx_internal.h"
+
+namespace dsx {
+
+constexpr int TAB_BAND = 64;          // output rows per k_tab wave (8 bands per 512-row table)
+constexpr int NT_TRANS = 60;          // world translations per axis (dsearch_desert.cpp:46)
+constexpr int SCAN_KMIN = -30;        // first translation (dsearch_desert.cpp:47)
+constexpr int SCAN_CELL = 512;        // quarts per T-table cell (dsearch_desert.cpp:48)
+constexpr int SCAN_PERIOD = 262144;   // side-A period of temperature octave 0 in quarts (dsearch_desert.cpp:45)
+constexpr int SCAN_WORLD = 7499996;   // world border in quarts (dsearch_desert.cpp:44)
+
+struct TabArgs {
+    float kT[17], kC[17];   // normalised blur weights float(exp(-i^2/2s^2) / sum), 2r+1 used
+    double w[3];            // octave-0 weight factor * weight of T, C, E
+};
+struct ScanArgs {
+    float ta, tt, wc, we;
+};
+struct ScanCand {
+    float s, t, c, e;
+    int x, z;
+    uint32_t pos, pad_;
+};
+
+void launch_octinit(hipStream_t st, const uint64_t *seeds, uint32_t n, SideParams *out);
+bool launch_tab(hipStream_t st, const SideParams *sp, uint32_t n, int t0, int nt, int R, const TabArgs &A, float *tabs,
+                float *bandmax);
+void launch_active(hipStream_t st, const float *tabs, uint32_t n, float ta, uint32_t *act, uint32_t *count, uint32_t cap);
+void launch_scan(hipStream_t st, const float *tabs, const uint32_t *act, uint32_t nact, const int16_t *ibT,
+                 const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A, ScanCand *cand,
+                 uint32_t *ncand, uint32_t cap);
+
+}  // namespace dsx

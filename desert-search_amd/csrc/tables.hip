@@ -1,0 +1,202 @@
+// Period tables, Gaussian blur and the world translation scan (stages S1-S2) for gfx950.
+//
+// Reference semantics:
+//   octave-0 Perlin of T, C, E per side           standalone.hpp:123-139 (NormalNoise::init)
+//   6 period tables (TA,TB,CA,CB,EA,EB), 512^2    lowfield.hpp:14-55 (tab_setup, tabulate_octave)
+//   periodic separable blur, normalised weights   lowfield.hpp:105-139 (Blur)
+//   maxta / maxtb                                 dsearch_desert.cpp:247
+//   active side-A cells (TA >= ta) and the scan   dsearch_desert.cpp:256-280 (gproxy.cu:180-209)
+// Each table is tabulated and blurred by one wave per row band (grid_blur.h, MODE 1: the period table
+// is exactly the full gate's 512^2 grid with the channel's own octave weight), written once to HBM.
+// The scan runs one wave per active side-A cell with lanes on the 60 x translations, so the side-B
+// maps of a lane are loop invariant and each tz step is one gather per table; every candidate carries
+// its scan position ((j*512 + i)*64 + tz)*64 + tx, from which the host restores the CPU's order.
+#include "grid_blur.h"
+#include "tables.h"
+
+namespace dsx {
+
+// Octave 0 of channel T/C/E (table t: channel t/2, side t&1) for every seed: one lane per table,
+// permutation shuffled in a lane-private LDS slot (same scheme as k_init in gate.hip).
+constexpr int PSTRIDE = 260;
+__global__ __launch_bounds__(64) void k_octinit(const uint64_t *seeds, uint32_t n, SideParams *out) {
+    __shared__ uint32_t slot32[64 * PSTRIDE / 4];
+    uint8_t *slots = reinterpret_cast<uint8_t *>(slot32);
+    const int lane = threadIdx.x;
+    const uint32_t g = blockIdx.x * 64u + lane;
+    const bool active = g < 6u * n;
+    uint8_t *my = slots + lane * PSTRIDE;
+    double ox = 0, oy = 0, oz = 0;
+    if (active) {
+        const uint32_t s = g / 6, t = g % 6;
+        const int ch = (t / 2 == 0) ? DSX_CH_T : (t / 2 == 1) ? DSX_CH_C : DSX_CH_E;
+        Xoro w = Xoro::seeded(seeds[s]);
+        const uint64_t a = w.next(), b = w.next();
+        Xoro c = Xoro::raw(a ^ DSX_NOISE[ch].fork_a, b ^ DSX_NOISE[ch].fork_b);
+        uint64_t ra = c.next(), rb = c.next();
+        if (t & 1) { ra = c.next(); rb = c.next(); }
+        const uint64_t *salt = DSX_OCT_SALT[DSX_NOISE[ch].first + 10];
+        Xoro r = Xoro::raw(ra ^ salt[0], rb ^ salt[1]);
+        ox = r.uniform() * 256;
+        oy = r.uniform() * 256;
+        oz = r.uniform() * 256;
+        uint32_t *my32 = reinterpret_cast<uint32_t *>(my);
+        for (int q = 0; q < 64; q++) my32[q] = 0x03020100u + 0x04040404u * uint32_t(q);
+        for (int i = 0; i < 256; i++) {
+            const int j = i + int(r.bounded(uint32_t(256 - i)));
+            const uint8_t tmp = my[i];
+            my[i] = my[j];
+            my[j] = tmp;
+        }
+    }
+    __syncthreads();
+    const uint32_t g0 = blockIdx.x * 64u;
+    for (int q = 0; q < 64; q++) {
+        if (g0 + q >= 6u * n) break;
+        reinterpret_cast<uint32_t *>(out[g0 + q].perm)[lane] = slot32[q * (PSTRIDE / 4) + lane];
+    }
+    if (active) {
+        SideParams &p = out[g];
+        p.ox = ox;
+        p.oz = oz;
+        p.fy = oy - floor(oy);
+        p.vy = fade(p.fy);
+        p.iy = int(floor(oy));
+    }
+}
+
+// One wave per (seed, table, band of TAB_BAND output rows). first_table/ntab select T (0..1) or C/E
+// (2..5) so that T and C/E may use different blur radii. bandmax gets the band's maximum.
+template <int R>
+__global__ __launch_bounds__(64) void k_tab(const SideParams *sp, uint32_t n, int first_table, int ntab, TabArgs A,
+                                            float *tabs, float *bandmax) {
+    __shared__ WaveLds L;
+    constexpr int NB = 512 / TAB_BAND;
+    const uint32_t blk = blockIdx.x;
+    const uint32_t band = blk % NB, tq = (blk / NB) % ntab, s = blk / (NB * ntab);
+    if (s >= n) return;
+    const int t = first_table + int(tq);
+    const SideParams &p = sp[size_t(s) * 6 + t];
+    grid_setup<1>(p, L, A.w[t / 2]);
+    const int lane = threadIdx.x;
+    float *dst = tabs + (size_t(s) * 6 + t) * (512 * 512);
+    float best = -1e30f;
+    const float *kern = (t < 2) ? A.kT : A.kC;
+    grid_blur_rows<1, R>(p, L, kern, int(band) * TAB_BAND, int(band) * TAB_BAND + TAB_BAND - 1,
+                         [&](int o, const float (&v)[8]) {
+                             float4 *row = reinterpret_cast<float4 *>(dst + size_t(o) * 512 + 8 * lane);
+                             row[0] = make_float4(v[0], v[1], v[2], v[3]);
+                             row[1] = make_float4(v[4], v[5], v[6], v[7]);
+#pragma unroll
+                             for (int q = 0; q < 8; q++) best = fmaxf(best, v[q]);
+                         });
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
+    if (lane == 0) bandmax[(size_t(s) * 6 + t) * NB + band] = best;
+}
+
+// Side-A cells with TA >= ta, as (seed, cell) pairs; one atomic per wave.
+__global__ __launch_bounds__(256) void k_active(const float *tabs, uint32_t n, float ta, uint32_t *act, uint32_t *count,
+                                                uint32_t cap) {
+    const uint32_t s = blockIdx.y;
+    const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+    const bool on = s < n && tabs[size_t(s) * 6 * 512 * 512 + cell] >= ta;
+    const uint64_t mask = __ballot(on);
+    if (mask == 0) return;
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, uint32_t(__popcll(mask)));
+    base = __shfl(base, 0);
+    if (on) {
+        const uint32_t k = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+        if (k < cap) {
+            act[2 * k] = s;
+            act[2 * k + 1] = cell;
+        }
+    }
+}
+
+// One wave per active cell; lane = x translation. Candidates (score >= tt) are appended per seed.
+__global__ __launch_bounds__(64) void k_scan(const float *tabs, const uint32_t *act, uint32_t nact, const int16_t *ibT,
+                                             const int8_t *loT, const int8_t *hiT, const int16_t *ibC, ScanArgs A,
+                                             ScanCand *cand, uint32_t *ncand, uint32_t cap) {
+    const uint32_t a = blockIdx.x;
+    if (a >= nact) return;
+    const uint32_t s = act[2 * a], cell = act[2 * a + 1];
+    const int j = int(cell >> 9), i = int(cell & 511), lane = threadIdx.x;
+    const float *TA = tabs + size_t(s) * 6 * 512 * 512;
+    const float *TB = TA + 512 * 512, *CA = TB + 512 * 512, *CB = CA + 512 * 512, *EA = CB + 512 * 512,
+                *EB = EA + 512 * 512;
+    const float sa = TA[cell];
+    const int ca = ((2 * j) & 511) * 512 + ((2 * i) & 511);
+    const float cA = CA[ca], eA = EA[ca];
+    const bool lane_ok = lane < NT_TRANS && lane >= loT[i] && lane <= hiT[i];
+    const int cT = lane_ok ? ibT[i * NT_TRANS + lane] : 0, cC = lane_ok ? ibC[i * NT_TRANS + lane] : 0;
+    const float tt = A.tt, pre = A.tt - 0.3f, wc = A.wc, we = A.we;
+    const int tz0 = loT[j], tz1 = hiT[j];
+    for (int tz = tz0; tz <= tz1; tz++) {
+        const int r = ibT[j * NT_TRANS + tz], rc = ibC[j * NT_TRANS + tz];
+        bool hit = false;
+        float tv = 0, cv = 0, ev = 0, sc = 0;
+        if (lane_ok) {
+            tv = sa + TB[r * 512 + cT];
+            if (!(tv < pre)) {
+                cv = cA + CB[rc * 512 + cC];
+                ev = eA + EB[rc * 512 + cC];
+                sc = tv + wc * cv + we * ev;
+                hit = !(sc < tt);
+            }
+        }
+        const uint64_t mask = __ballot(hit);
+        if (mask == 0) continue;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&ncand[s], uint32_t(__popcll(mask)));
+        base = __shfl(base, 0);
+        if (hit) {
+            const uint32_t k = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+            if (k < cap) {
+                ScanCand &o = cand[size_t(s) * cap + k];
+                o.s = sc;
+                o.t = tv;
+                o.c = cv;
+                o.e = ev;
+                o.x = i * SCAN_CELL + (lane + SCAN_KMIN) * SCAN_PERIOD;
+                o.z = j * SCAN_CELL + (tz + SCAN_KMIN) * SCAN_PERIOD;
+                o.pos = ((uint32_t(j) * 512u + uint32_t(i)) * 64u + uint32_t(tz)) * 64u + uint32_t(lane);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+void launch_octinit(hipStream_t st, const uint64_t *seeds, uint32_t n, SideParams *out) {
+    if (n) hipLaunchKernelGGL(k_octinit, dim3((6 * n + 63) / 64), dim3(64), 0, st, seeds, n, out);
+}
+template <int R>
+static void tab_r(hipStream_t st, const SideParams *sp, uint32_t n, int t0, int nt, const TabArgs &A, float *tabs,
+                  float *bandmax) {
+    hipLaunchKernelGGL(k_tab<R>, dim3(n * nt * (512 / TAB_BAND)), dim3(64), 0, st, sp, n, t0, nt, A, tabs, bandmax);
+}
+bool launch_tab(hipStream_t st, const SideParams *sp, uint32_t n, int t0, int nt, int R, const TabArgs &A, float *tabs,
+                float *bandmax) {
+    if (!n) return true;
+    switch (R) {
+    case 4: tab_r<4>(st, sp, n, t0, nt, A, tabs, bandmax); return true;
+    case 5: tab_r<5>(st, sp, n, t0, nt, A, tabs, bandmax); return true;
+    case 6: tab_r<6>(st, sp, n, t0, nt, A, tabs, bandmax); return true;
+    case 7: tab_r<7>(st, sp, n, t0, nt, A, tabs, bandmax); return true;
+    case 8: tab_r<8>(st, sp, n, t0, nt, A, tabs, bandmax); return true;
+    default: return false;
+    }
+}
+void launch_active(hipStream_t st, const float *tabs, uint32_t n, float ta, uint32_t *act, uint32_t *count, uint32_t cap) {
+    if (n) hipLaunchKernelGGL(k_active, dim3(512 * 512 / 256, n), dim3(256), 0, st, tabs, n, ta, act, count, cap);
+}
+void launch_scan(hipStream_t st, const float *tabs, const uint32_t *act, uint32_t nact, const int16_t *ibT,
+                 const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A, ScanCand *cand,
+                 uint32_t *ncand, uint32_t cap) {
+    if (nact)
+        hipLaunchKernelGGL(k_scan, dim3(nact), dim3(64), 0, st, tabs, act, nact, ibT, loT, hiT, ibC, A, cand, ncand, cap);
+}
+
+}  // namespace dsx

@@ -33,6 +33,18 @@ class GateParams(ctypes.Structure):
         super().__init__(max_phase, amp, pthr, margin, sig, box)
 
 
+class ScanParams(ctypes.Structure):
+    """dsx_scan_params; defaults are the reference's Params (dsearch_desert.cpp:53-58)."""
+    _fields_ = [("ta", ctypes.c_float), ("tt", ctypes.c_float), ("wc", ctypes.c_float), ("we", ctypes.c_float),
+                ("sigma", ctypes.c_double), ("sigmac", ctypes.c_double)]
+
+    def __init__(self, ta=0.25, tt=0.55, wc=0.25, we=0.41, sigma=2.0, sigmac=2.0):
+        super().__init__(ta, tt, wc, we, sigma, sigmac)
+
+
+RAW_CAND = np.dtype([("s", "<f4"), ("t", "<f4"), ("c", "<f4"), ("e", "<f4"), ("x", "<i4"), ("z", "<i4")])
+
+
 class GateCounts(ctypes.Structure):
     _fields_ = [("phase", ctypes.c_uint64), ("pre", ctypes.c_uint64), ("full", ctypes.c_uint64)]
 
@@ -62,7 +74,10 @@ def load(path=LIB_PATH):
                              _u64p, ctypes.c_uint64, _u64p, ctypes.POINTER(GateCounts)]
     lib.dsx_gate_amps.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, ctypes.POINTER(GateParams), _f32p, _f32p]
     lib.dsx_gate_profile.argtypes = [ctypes.c_void_p, _f64p, _u64p]
-    for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile"):
+    lib.dsx_scan.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, ctypes.POINTER(ScanParams), _f32p, _f32p,
+                             ctypes.c_void_p, ctypes.c_uint64, _u64p]
+    lib.dsx_tables.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ScanParams), _f32p]
+    for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile", "dsx_scan", "dsx_tables"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -127,3 +142,23 @@ class Engine:
         self._check(self.lib.dsx_gate_profile(self.h, _ptr(ms, ctypes.c_double), _ptr(work, ctypes.c_uint64)),
                     "dsx_gate_profile")
         return ms, work
+
+    def scan(self, seeds, params=None, cap=1 << 16):
+        """Per seed: maxta, maxtb and the raw candidates (RAW_CAND records) in CPU scan order."""
+        p = params or ScanParams()
+        s = np.ascontiguousarray(seeds, dtype=np.uint64)
+        n = len(s)
+        maxta = np.zeros(n, np.float32)
+        maxtb = np.zeros(n, np.float32)
+        cands = np.zeros(max(n, 1) * cap, RAW_CAND)
+        ncand = np.zeros(n, np.uint64)
+        rc = self.lib.dsx_scan(self.h, _ptr(s, ctypes.c_uint64), n, ctypes.byref(p), _ptr(maxta, ctypes.c_float),
+                               _ptr(maxtb, ctypes.c_float), cands.ctypes.data, cap, _ptr(ncand, ctypes.c_uint64))
+        self._check(rc, "dsx_scan")
+        return maxta, maxtb, [cands[i * cap:i * cap + int(ncand[i])] for i in range(n)]
+
+    def tables(self, seed, params=None):
+        p = params or ScanParams()
+        out = np.zeros((6, 512, 512), np.float32)
+        self._check(self.lib.dsx_tables(self.h, seed, ctypes.byref(p), _ptr(out, ctypes.c_float)), "dsx_tables")
+        return out

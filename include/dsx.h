@@ -73,6 +73,35 @@ int dsx_gate_amps(dsx_ctx *ctx, const uint64_t *seeds, uint64_t n, const dsx_gat
  * items each processed (seed indices, sides, pre-gate seeds, full-gate seeds). */
 int dsx_gate_profile(dsx_ctx *ctx, double ms[4], uint64_t work[4]);
 
+/* ---------------------------------------------------------------- period tables + scan (S1-S2) */
+typedef struct {
+    float ta;      /* side-A T prune: scan cells with TA >= ta        (Params::ta, 0.25) */
+    float tt;      /* candidate score threshold                        (Params::t, 0.55) */
+    float wc, we;  /* score weights of smoothed C and E                (Params::wc 0.25, we 0.41) */
+    double sigma;  /* T blur sigma in table cells                      (Params::sigma, 2.0) */
+    double sigmac; /* C/E blur sigma in table cells                    (Params::sigmac, 2.0) */
+} dsx_scan_params;
+
+/* One raw candidate: score, its T/C/E parts and the world quart position (gproxy::RawCand, gproxy.h:28). */
+typedef struct {
+    float s, t, c, e;
+    int32_t x, z;
+} dsx_raw_cand;
+
+/* Octave-0 period tables, blur and translation scan of each seed (seeds from the gate).
+ * Replaces gproxy::scan_init + gproxy::scan (gproxy.h:30-33, gproxy.cu:221-307); the translation maps
+ * and blur weights are derived internally from the parameters (dsearch_desert.cpp:98-113,
+ * lowfield.hpp:105-112). Per seed i: maxta[i], maxtb[i] (max of the blurred side-A/side-B T tables),
+ * ncand[i] = number of raw candidates (score >= tt), written to cands[i*cap .. i*cap + min(ncand, cap))
+ * in the CPU scan order (j, i, tz, tx) of dsearch_desert.cpp:257-279, so the caller's bucket
+ * reduction reproduces the CPU's candidate list. DSX_E_CAPACITY if any ncand[i] > cap. */
+int dsx_scan(dsx_ctx *ctx, const uint64_t *seeds, uint64_t n, const dsx_scan_params *p, float *maxta, float *maxtb,
+             dsx_raw_cand *cands, uint64_t cap, uint64_t *ncand);
+
+/* The 6 blurred period tables {TA, TB, CA, CB, EA, EB} of one seed, 512*512 floats each, row-major
+ * (lowfield.hpp tabulate_octave + Blur; the `gscancheck` comparison, dsearch_desert.cpp:292-296). */
+int dsx_tables(dsx_ctx *ctx, uint64_t seed, const dsx_scan_params *p, float *out);
+
 #ifdef __cplusplus
 }
 #endif
