@@ -1,0 +1,315 @@
+// FP64 climate sampler for gfx950: improved Perlin noise, 2-sided normal noise, terrain spline,
+// surface-biome grid and parameter-tree fallback, and the flood predicates built on them.
+//
+// Reference semantics, restated as device code with the same double/float operation order:
+//   Perlin init / sample         standalone.hpp:55-82
+//   NormalNoise init / sample    standalone.hpp:123-151
+//   spline                       standalone.hpp:154-169
+//   classify (nearest leaf)      standalone.hpp:170-196 (iterative here: explicit frame stack)
+//   climate / finish             standalone.hpp:224-238
+//   SurfaceGrid::lookup          surface.hpp:58-70 (grid built on the host, dsx_api)
+//   desert / desert_state        surface.hpp:85-124
+//   DesertProxy                  desert_proxy.hpp:14-32
+//   hot predicate                dsearch_desert.cpp:334
+// A seed's 46 octaves (T 2, H 2, C 9, E 4, Shift 3, W 3 per side) are initialised once per seed on
+// the device (k_climinit) and read from LDS by the flood kernels.
+#pragma once
+#include "dsx_internal.h"
+#include "../data/mc_tables.inc"
+
+namespace dsx {
+
+constexpr int CLIM_NOCT = 46;
+// octave slots per channel and side, and the first slot of each channel (side 0, then side 1)
+__host__ __device__ constexpr int clim_n(int ch) { return ch == 0 ? 2 : ch == 1 ? 2 : ch == 2 ? 9 : ch == 3 ? 4 : 3; }
+__host__ __device__ constexpr int clim_base(int ch) {
+    return ch == 0 ? 0 : ch == 1 ? 4 : ch == 2 ? 8 : ch == 3 ? 26 : ch == 4 ? 34 : 40;
+}
+
+struct ClimOct {
+    uint8_t perm[256];
+    double ox, oy, oz, freq, weight, fy, vy;
+    int iy, pad_;
+};
+struct ClimSeed {
+    ClimOct oct[CLIM_NOCT];
+    double factor[6];
+};
+
+// Surface-biome grid (surface.hpp:12-71): cut positions per axis T,H,C,E,W and the cell table.
+struct SurfGrid {
+    const int32_t *cuts[5];
+    int ncut[5];
+    int dim[5];
+    const int16_t *cell;
+};
+
+// ------------------------------------------------------------------ noise
+__device__ __forceinline__ double lerpd(double t, double a, double b) { return a + t * (b - a); }
+
+__device__ __forceinline__ double perlin_sample(const ClimOct &o, double x, double y, double z) {
+    x += o.ox;
+    z += o.oz;
+    const double bx = floor(x), bz = floor(z);
+    const int ix = int(bx), iz = int(bz);
+    x -= bx;
+    z -= bz;
+    int jy = o.iy;
+    double dy = o.fy, ty = o.vy;
+    if (y != 0) {
+        y += o.oy;
+        const double by = floor(y);
+        jy = int(by);
+        dy = y - by;
+        ty = fade(dy);
+    }
+    const uint8_t *P = o.perm;
+    const int a0 = P[ix & 255], a1 = P[(ix + 1) & 255];
+    const int b00 = P[(a0 + jy) & 255], b01 = P[(a0 + jy + 1) & 255];
+    const int b10 = P[(a1 + jy) & 255], b11 = P[(a1 + jy + 1) & 255];
+    const double tx = fade(x), tz = fade(z);
+    const double x1 = x - 1, y1 = dy - 1, z1 = z - 1;
+    const double l00 = lerpd(tx, grad(P[(b00 + iz) & 255], x, dy, z), grad(P[(b10 + iz) & 255], x1, dy, z));
+    const double l10 = lerpd(tx, grad(P[(b01 + iz) & 255], x, y1, z), grad(P[(b11 + iz) & 255], x1, y1, z));
+    const double l01 = lerpd(tx, grad(P[(b00 + iz + 1) & 255], x, dy, z1), grad(P[(b10 + iz + 1) & 255], x1, dy, z1));
+    const double l11 = lerpd(tx, grad(P[(b01 + iz + 1) & 255], x, y1, z1), grad(P[(b11 + iz + 1) & 255], x1, y1, z1));
+    return lerpd(tz, lerpd(ty, l00, l10), lerpd(ty, l01, l11));
+}
+
+// NormalNoise::sample: per-side sums, then (A + B) * factor.
+__device__ __forceinline__ double noise_sample(const ClimSeed &C, int ch, double x, double y, double z) {
+    const int n = clim_n(ch), b = clim_base(ch);
+    double sum0 = 0, sum1 = 0;
+    {
+        const double px = x, py = y, pz = z;
+#pragma unroll 1
+        for (int j = 0; j < n; j++) {
+            const ClimOct &o = C.oct[b + j];
+            sum0 += o.weight * perlin_sample(o, px * o.freq, py * o.freq, pz * o.freq);
+        }
+    }
+    {
+        const double sc = 337. / 331.;
+        const double px = x * sc, py = y * sc, pz = z * sc;
+#pragma unroll 1
+        for (int j = 0; j < n; j++) {
+            const ClimOct &o = C.oct[b + n + j];
+            sum1 += o.weight * perlin_sample(o, px * o.freq, py * o.freq, pz * o.freq);
+        }
+    }
+    return (sum0 + sum1) * C.factor[ch];
+}
+// DesertProxy::octaves: one running sum over both sides, octaves [0, hi).
+__device__ __forceinline__ double noise_octaves(const ClimSeed &C, int ch, double x, double z, int hi) {
+    const int n = clim_n(ch), b = clim_base(ch);
+    double s = 0;
+    for (int side = 0; side < 2; side++) {
+        const double sc = side ? 337.0 / 331.0 : 1.0;
+#pragma unroll 1
+        for (int k = 0; k < hi && k < n; k++) {
+            const ClimOct &o = C.oct[b + side * n + k];
+            s += o.weight * perlin_sample(o, x * sc * o.freq, 0, z * sc * o.freq);
+        }
+    }
+    return s * C.factor[ch];
+}
+
+// ------------------------------------------------------------------ terrain spline (depth 4: unrolled by template)
+template <int D>
+__device__ __noinline__ float spline_at(int index, const float *v) {
+    const dsx_spline_node &s = DSX_SPLINE[index];
+    if (s.n == 0) return s.value;
+    if constexpr (D == 0) {
+        return s.value;   // unreachable: the spline tree is 4 levels deep
+    } else {
+        const float x = v[s.axis];
+        int hi = 0;
+        while (hi < s.n && x > s.x[hi]) hi++;
+        if (hi == 0 || hi == s.n) {
+            const int j = hi == 0 ? 0 : s.n - 1;
+            return spline_at<D - 1>(s.child[j], v) + s.slope[j] * (x - s.x[j]);
+        }
+        const int lo = hi - 1;
+        const float width = s.x[hi] - s.x[lo], t = (x - s.x[lo]) / width;
+        const float a = spline_at<D - 1>(s.child[lo], v), b = spline_at<D - 1>(s.child[hi], v);
+        const float p = s.slope[lo] * width - (b - a), q = -s.slope[hi] * width + (b - a);
+        return float(lerpd(double(t), double(a), double(b)) + double(t * (1.f - t)) * lerpd(double(t), double(p), double(q)));
+    }
+}
+__device__ __noinline__ float spline0(const float *v) { return spline_at<5>(0, v); }
+
+// ------------------------------------------------------------------ biome classification
+__device__ __forceinline__ uint64_t box_dist(int node, const int64_t *v) {
+    const uint64_t enc = DSX_TREE[node];
+    uint64_t sum = 0;
+    for (int i = 0; i < 6; i++) {
+        const int idx = int((enc >> (8 * i)) & 255);
+        const int64_t lo = DSX_PARAM[idx][0], hi = DSX_PARAM[idx][1];
+        int64_t d = lo - v[i];
+        const int64_t d2 = v[i] - hi;
+        if (d2 > d) d = d2;
+        if (d < 0) d = 0;
+        sum += uint64_t(d * d);
+    }
+    return sum;
+}
+// Nearest leaf by branch and bound, visiting children in the reference's order so that ties resolve
+// to the same leaf (first found wins: strict < everywhere).
+__device__ __noinline__ int classify(const int64_t *v) {
+    struct Frame { int child, step, level, i; };
+    Frame st[8];
+    int sp = 0;
+    uint64_t bestd = ~0ull;
+    int best = 0;
+    auto open = [&](int node, int level) {
+        int step;
+        do { step = DSX_STEPS[level++]; } while (node + step >= DSX_NTREE);
+        st[sp++] = Frame{int(DSX_TREE[node] >> 48), step, level, 0};
+    };
+    if ((DSX_TREE[0] >> 56) == 255) {
+        best = 0;
+    } else {
+        open(0, 0);
+    }
+    while (sp > 0) {
+        Frame &f = st[sp - 1];
+        if (f.i >= 6 || f.child >= DSX_NTREE) { sp--; continue; }
+        const int c = f.child;
+        f.child += f.step;
+        f.i++;
+        const uint64_t d = box_dist(c, v);
+        if (!(d < bestd)) continue;
+        if ((DSX_TREE[c] >> 56) == 255) {
+            bestd = d;
+            best = c;
+        } else {
+            open(c, f.level);
+        }
+    }
+    return int((DSX_TREE[best] >> 48) & 255);
+}
+
+__device__ __forceinline__ int grid_lookup(const SurfGrid &g, const int64_t *q) {
+    constexpr int AX[5] = {0, 1, 2, 3, 5};
+    int idx[5];
+#pragma unroll
+    for (int a = 0; a < 5; a++) {
+        const int64_t v = q[AX[a]];
+        const int32_t *c = g.cuts[a];
+        const int nc = g.ncut[a];
+        if (v < c[0] || v >= c[nc - 1]) return -1;
+        // upper_bound - 1
+        int lo = 0, hi = nc;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (int64_t(c[mid]) <= v) lo = mid + 1;
+            else hi = mid;
+        }
+        idx[a] = lo - 1;
+        if (v == c[idx[a]]) return -1;
+    }
+    const size_t k = ((((size_t(idx[0]) * g.dim[1] + idx[1]) * g.dim[2] + idx[2]) * g.dim[3] + idx[3]) * g.dim[4] + idx[4]);
+    return g.cell[k];
+}
+
+// ------------------------------------------------------------------ predicates
+__device__ __forceinline__ float ridges(float w) { return -(fabsf(fabsf(w) - 0.6666667f) - 0.33333334f) * 3.f; }
+
+__device__ __forceinline__ void shifted(const ClimSeed &C, int x, int z, double &px, double &pz) {
+    px = x + 4. * noise_sample(C, DSX_CH_SHIFT, x, 0, z);
+    pz = z + 4. * noise_sample(C, DSX_CH_SHIFT, z, x, 0);
+}
+
+// Sampler::climate at quart (x, y, z) (standalone.hpp:224-238).
+__device__ void climate_q(const ClimSeed &C, int x, int y, int z, int64_t *q) {
+    double px, pz;
+    shifted(C, x, z, px, pz);
+    const float c = float(noise_sample(C, DSX_CH_C, px, 0, pz));
+    const float e = float(noise_sample(C, DSX_CH_E, px, 0, pz)), w = float(noise_sample(C, DSX_CH_W, px, 0, pz));
+    const float v[4] = {c, e, ridges(w), w};
+    const double off = double(spline0(v) + 0.015f);
+    const float d = float(1. - (y * 4) / 128. - 83. / 160. + off);
+    const float t = float(noise_sample(C, DSX_CH_T, px, 0, pz)), h = float(noise_sample(C, DSX_CH_H, px, 0, pz));
+    const float vals[6] = {t, h, c, e, d, w};
+    for (int i = 0; i < 6; i++) q[i] = int64_t(10000.f * vals[i]);
+}
+
+// surface.hpp:85-93
+__device__ bool is_desert(const ClimSeed &C, const SurfGrid &g, int x, int z) {
+    double px, pz;
+    shifted(C, x, z, px, pz);
+    const float t = float(noise_sample(C, DSX_CH_T, px, 0, pz));
+    if (int64_t(10000.f * t) < 5500) return false;
+    const float c = float(noise_sample(C, DSX_CH_C, px, 0, pz));
+    const float e = float(noise_sample(C, DSX_CH_E, px, 0, pz)), w = float(noise_sample(C, DSX_CH_W, px, 0, pz));
+    const float v[4] = {c, e, ridges(w), w};
+    const double off = double(spline0(v) + 0.015f);
+    const float d = float(1. - (64 * 4) / 128. - 83. / 160. + off);
+    const float h = float(noise_sample(C, DSX_CH_H, px, 0, pz));   // T is deterministic: reuse t
+    int64_t q[6] = {int64_t(10000.f * t), int64_t(10000.f * h), int64_t(10000.f * c),
+                    int64_t(10000.f * e),  int64_t(10000.f * d), int64_t(10000.f * w)};
+    const int b = grid_lookup(g, q);
+    return (b >= 0 ? b : classify(q)) == 2;
+}
+
+// surface.hpp:98-124: 2 desert, 1 river that would be desert with weirdness +-0.0501, 0 other.
+__device__ int desert_state(const ClimSeed &C, const SurfGrid &g, int x, int z) {
+    double px, pz;
+    shifted(C, x, z, px, pz);
+    const float t = float(noise_sample(C, DSX_CH_T, px, 0, pz));
+    if (int64_t(10000.f * t) < 5500) return 0;
+    const float c = float(noise_sample(C, DSX_CH_C, px, 0, pz)), e = float(noise_sample(C, DSX_CH_E, px, 0, pz));
+    const float w = float(noise_sample(C, DSX_CH_W, px, 0, pz)), h = float(noise_sample(C, DSX_CH_H, px, 0, pz));
+    int64_t q[6] = {int64_t(10000.f * t), int64_t(10000.f * h), int64_t(10000.f * c),
+                    int64_t(10000.f * e), -10000,               int64_t(10000.f * w)};
+    bool exact_depth = false;
+    auto pick = [&]() {
+        const int b = grid_lookup(g, q);
+        if (b >= 0) return b;
+        if (!exact_depth) {
+            const float v[4] = {c, e, ridges(w), w};
+            const double off = double(spline0(v) + 0.015f);
+            q[4] = int64_t(10000.f * float(1. - (64 * 4) / 128. - 83. / 160. + off));
+            exact_depth = true;
+        }
+        return classify(q);
+    };
+    const int b = pick();
+    if (b == 2) return 2;
+    if (b != 7 && b != 11) return 0;
+    if (q[5] > 500 || q[5] < -500) return 0;
+    q[5] = q[5] >= 0 ? 501 : -501;
+    return pick() == 2 ? 1 : 0;
+}
+
+// dsearch_desert.cpp:334
+__device__ __forceinline__ int hot_state(const ClimSeed &C, int x, int z) {
+    return noise_sample(C, DSX_CH_T, x, 0, z) >= 0.55 ? 2 : 0;
+}
+// desert_proxy.hpp:25-32
+__device__ __forceinline__ float proxy_prob(const ClimSeed &C, int x, int z) {
+    if (noise_sample(C, DSX_CH_T, x, 0, z) < 0.55) return 0;
+    const double c = noise_octaves(C, DSX_CH_C, x, z, 3), e = noise_octaves(C, DSX_CH_E, x, z, 2);
+    int i = int(floor((c + 0.8) / 0.05)), j = int(floor((e + 1.0) / 0.05));
+    i = i < 0 ? 0 : (i > 39 ? 39 : i);
+    j = j < 0 ? 0 : (j > 39 ? 39 : j);
+    return DSX_DPROB[i][j];
+}
+__device__ __forceinline__ int proxy_state(const ClimSeed &C, int x, int z, float cut) {
+    return proxy_prob(C, x, z) >= cut ? 2 : 0;
+}
+
+enum PredKind { PRED_DESERT_STATE = 0, PRED_HOT = 1, PRED_PROXY = 2, PRED_DESERT = 3 };
+
+__device__ __forceinline__ int predicate(int kind, const ClimSeed &C, const SurfGrid &g, int x, int z, float cut) {
+    switch (kind) {
+    case PRED_DESERT_STATE: return desert_state(C, g, x, z);
+    case PRED_HOT: return hot_state(C, x, z);
+    case PRED_PROXY: return proxy_state(C, x, z, cut);
+    default: return is_desert(C, g, x, z) ? 2 : 0;
+    }
+}
+
+void launch_climinit(hipStream_t st, const uint64_t *seeds, uint32_t n, ClimSeed *out);
+
+}  // namespace dsx

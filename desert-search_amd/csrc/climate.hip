@@ -1,0 +1,126 @@
+// Per-seed climate initialisation and batched evaluation kernels (noise, climate vector, predicates).
+#include "climate.h"
+#include "climate_api.h"
+
+namespace dsx {
+
+// One lane per octave slot (46 per seed): NormalNoise::init (standalone.hpp:123-139) for the slot's
+// channel and side, then Perlin::init (standalone.hpp:55-61) with the permutation shuffled in a
+// lane-private LDS slot.
+constexpr int CSTRIDE = 260;
+__global__ __launch_bounds__(64) void k_climinit(const uint64_t *seeds, uint32_t n, ClimSeed *out) {
+    __shared__ uint32_t slot32[64 * CSTRIDE / 4];
+    uint8_t *slots = reinterpret_cast<uint8_t *>(slot32);
+    const int lane = threadIdx.x;
+    const uint32_t g = blockIdx.x * 64u + lane;
+    const bool active = g < CLIM_NOCT * n;
+    uint8_t *my = slots + lane * CSTRIDE;
+    double ox = 0, oy = 0, oz = 0, freq = 0, weight = 0;
+    if (active) {
+        const uint32_t s = g / CLIM_NOCT, slot = g % CLIM_NOCT;
+        int ch = 5;
+        while (clim_base(ch) > int(slot)) ch--;
+        const int n_ch = clim_n(ch), rel = int(slot) - clim_base(ch), side = rel / n_ch, j = rel % n_ch;
+        const dsx_noise_cfg &cfg = DSX_NOISE[ch];
+        Xoro w = Xoro::seeded(seeds[s]);
+        const uint64_t a = w.next(), b = w.next();
+        Xoro r = Xoro::raw(a ^ cfg.fork_a, b ^ cfg.fork_b);
+        uint64_t ra = r.next(), rb = r.next();
+        if (side) { ra = r.next(); rb = r.next(); }
+        double f = ldexp(1., cfg.first);
+        double wt = double(1 << (cfg.len - 1)) / ((1 << cfg.len) - 1);
+        int i = 0, cnt = -1;
+        for (; i < cfg.len; i++, f *= 2, wt *= .5)
+            if (cfg.amp[i] != 0 && ++cnt == j) break;
+        const uint64_t *salt = DSX_OCT_SALT[cfg.first + i + 10];
+        Xoro q = Xoro::raw(ra ^ salt[0], rb ^ salt[1]);
+        ox = q.uniform() * 256;
+        oy = q.uniform() * 256;
+        oz = q.uniform() * 256;
+        freq = f;
+        weight = wt * cfg.amp[i];
+        uint32_t *my32 = reinterpret_cast<uint32_t *>(my);
+        for (int k = 0; k < 64; k++) my32[k] = 0x03020100u + 0x04040404u * uint32_t(k);
+        for (int k = 0; k < 256; k++) {
+            const int m = k + int(q.bounded(uint32_t(256 - k)));
+            const uint8_t t = my[k];
+            my[k] = my[m];
+            my[m] = t;
+        }
+    }
+    __syncthreads();
+    const uint32_t g0 = blockIdx.x * 64u;
+    for (int k = 0; k < 64; k++) {
+        const uint32_t gk = g0 + k;
+        if (gk >= CLIM_NOCT * n) break;
+        reinterpret_cast<uint32_t *>(out[gk / CLIM_NOCT].oct[gk % CLIM_NOCT].perm)[lane] = slot32[k * (CSTRIDE / 4) + lane];
+    }
+    if (active) {
+        ClimOct &o = out[g / CLIM_NOCT].oct[g % CLIM_NOCT];
+        o.ox = ox;
+        o.oy = oy;
+        o.oz = oz;
+        o.freq = freq;
+        o.weight = weight;
+        o.iy = int(floor(oy));
+        o.fy = oy - floor(oy);
+        o.vy = fade(o.fy);
+        if (g % CLIM_NOCT == 0) {
+            for (int ch = 0; ch < 6; ch++) {
+                const dsx_noise_cfg &c = DSX_NOISE[ch];
+                int lo = 0, hi = c.len - 1;
+                while (c.amp[lo] == 0) lo++;
+                while (c.amp[hi] == 0) hi--;
+                const int span = hi - lo + 1;
+                out[g / CLIM_NOCT].factor[ch] = (5. * span) / (3. * (span + 1));
+            }
+        }
+    }
+}
+
+// Copy one seed's climate into LDS (all threads of the block participate).
+__device__ __forceinline__ void load_climate(ClimSeed &dst, const ClimSeed &src) {
+    const uint4 *s = reinterpret_cast<const uint4 *>(&src);
+    uint4 *d = reinterpret_cast<uint4 *>(&dst);
+    for (int k = threadIdx.x; k < int(sizeof(ClimSeed) / 16); k += blockDim.x) d[k] = s[k];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_noise(const ClimSeed *cs, int ch, const double *pts, uint32_t n, double *out) {
+    __shared__ ClimSeed C;
+    load_climate(C, *cs);
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k < n) out[k] = noise_sample(C, ch, pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]);
+}
+
+__global__ __launch_bounds__(256) void k_climate(const ClimSeed *cs, const int32_t *xs, const int32_t *zs, uint32_t n,
+                                                 int64_t *q) {
+    __shared__ ClimSeed C;
+    load_climate(C, *cs);
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k < n) climate_q(C, xs[k], 64, zs[k], q + 6 * size_t(k));
+}
+
+__global__ __launch_bounds__(256) void k_pred(const ClimSeed *cs, SurfGrid g, int kind, float cut, const int32_t *xs,
+                                              const int32_t *zs, uint32_t n, int8_t *out) {
+    __shared__ ClimSeed C;
+    load_climate(C, *cs);
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k < n) out[k] = int8_t(predicate(kind, C, g, xs[k], zs[k], cut));
+}
+
+void launch_climinit(hipStream_t st, const uint64_t *seeds, uint32_t n, ClimSeed *out) {
+    if (n) hipLaunchKernelGGL(k_climinit, dim3((CLIM_NOCT * n + 63) / 64), dim3(64), 0, st, seeds, n, out);
+}
+void launch_noise(hipStream_t st, const ClimSeed *cs, int ch, const double *pts, uint32_t n, double *out) {
+    if (n) hipLaunchKernelGGL(k_noise, dim3((n + 255) / 256), dim3(256), 0, st, cs, ch, pts, n, out);
+}
+void launch_climate(hipStream_t st, const ClimSeed *cs, const int32_t *xs, const int32_t *zs, uint32_t n, int64_t *q) {
+    if (n) hipLaunchKernelGGL(k_climate, dim3((n + 255) / 256), dim3(256), 0, st, cs, xs, zs, n, q);
+}
+void launch_pred(hipStream_t st, const ClimSeed *cs, const SurfGrid &g, int kind, float cut, const int32_t *xs,
+                 const int32_t *zs, uint32_t n, int8_t *out) {
+    if (n) hipLaunchKernelGGL(k_pred, dim3((n + 255) / 256), dim3(256), 0, st, cs, g, kind, cut, xs, zs, n, out);
+}
+
+}  // namespace dsx

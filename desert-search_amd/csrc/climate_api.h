@@ -1,0 +1,10 @@
+// Launchers of the climate kernels (climate.hip), used by the API translation units.
+#pragma once
+#include "climate.h"
+
+namespace dsx {
+void launch_noise(hipStream_t st, const ClimSeed *cs, int ch, const double *pts, uint32_t n, double *out);
+void launch_climate(hipStream_t st, const ClimSeed *cs, const int32_t *xs, const int32_t *zs, uint32_t n, int64_t *q);
+void launch_pred(hipStream_t st, const ClimSeed *cs, const SurfGrid &g, int kind, float cut, const int32_t *xs,
+                 const int32_t *zs, uint32_t n, int8_t *out);
+}  // namespace dsx

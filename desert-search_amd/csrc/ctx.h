@@ -17,6 +17,8 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
         if (e_ != hipSuccess) return dsx::hip_fail(e_, #x, __FILE__, __LINE__);      \
     } while (0)
 
+void dsx_ctx_free_ext(struct dsx_ctx *c);   // climate / flood state (floods_api.hip)
+
 struct dsx_ctx {
     int device = 0;
     hipStream_t st = nullptr;

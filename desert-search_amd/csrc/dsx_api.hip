@@ -91,7 +91,7 @@ static int ctx_alloc(dsx_ctx *c) {
     HIPCK(hipMalloc(&c->d_cnt, 16));
     HIPCK(hipMalloc(&c->d_sides, 2 * n1 * sizeof(dsx::SideParams)));
     HIPCK(hipMalloc(&c->d_seeds, n1 * 8));
-    HIPCK(hipHostMalloc(&c->h_cnt, 16, hipHostMallocDefault));
+    HIPCK(hipHostMalloc(&c->h_cnt, 256, hipHostMallocDefault));
     HIPCK(hipHostMalloc(&c->h_k3, n1 * 4, hipHostMallocDefault));
     HIPCK(hipHostMalloc(&c->h_a3, n1 * 4, hipHostMallocDefault));
     for (auto &e : c->ev) HIPCK(hipEventCreate(&e));
@@ -102,6 +102,7 @@ int dsx_destroy(dsx_ctx *c) {
     if (!c) return DSX_E_ARG;
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
+    dsx_ctx_free_ext(c);
     for (void *p : {(void *)c->d_k1, (void *)c->d_i2, (void *)c->d_k3, (void *)c->d_a3, (void *)c->d_all, (void *)c->d_cnt,
                     (void *)c->d_sides, (void *)c->d_seeds, (void *)c->d_tsp, (void *)c->d_tabs, (void *)c->d_bandmax,
                     (void *)c->d_act, (void *)c->d_ncand, (void *)c->d_cand, (void *)c->d_ibT, (void *)c->d_ibC,

@@ -6,6 +6,8 @@
 
 namespace dsx {
 
+constexpr int WORLD_Q = 7499996;   // world border in quarts (dense_flood.hpp:33, dsearch_desert.cpp:44)
+
 // Temperature octave-0 Perlin of one noise side (the reference's tg::PerlinParams, gate.cuh:49).
 struct SideParams {
     uint8_t perm[256];

@@ -1,0 +1,52 @@
+// Tiled frontier flood engine: shared declarations.
+#pragma once
+#include "climate.h"
+
+namespace dsx {
+
+constexpr int FT = 32;   // tile edge in lattice cells; one 32-bit mask per tile row
+
+// One bounded flood (dense_flood.hpp:52-86): window of (2*half+1)^2 lattice points, spacing `step`
+// quarts, centred on (qx, qz); pred kind (PredKind), proxy cut, 4- or 8-connectivity.
+struct FloodJob {
+    int32_t seed;   // index into the batch's ClimSeed array
+    int32_t kind;
+    float cut;
+    int32_t qx, qz, step, half, conn8;
+};
+struct FloodRes {
+    unsigned long long cells, sampled;
+    int minx, maxx, minz, maxz;
+    int edge, border;
+};
+// A 32x32 block of one flood's window: window cell (i, j) = (tx*32 + c, tz*32 + r) <-> bit c of row r.
+struct Tile {
+    uint32_t job;
+    int32_t tx, tz;
+    uint32_t queued;        // 1 while listed for the next propagation round
+    uint32_t pass[FT];      // predicate != 0 (inside window and world)
+    uint32_t cnt[FT];       // predicate == 2
+    uint32_t reach[FT];     // cells the flood has entered (pushed)
+    uint32_t in[FT];        // cells touched from a neighbouring tile (and the start cell)
+};
+
+struct FloodBufs {
+    Tile *tiles;
+    uint32_t *ntiles;             // allocated tiles
+    uint32_t tile_cap;
+    unsigned long long *hkeys;    // hash: (job, tx, tz) -> tile index, packed with the key
+    uint32_t hmask;               // table size - 1
+    uint32_t *lists[4];           // eval / prop lists, current and next
+    uint32_t *lcount;             // [eval, prop, next_eval, next_prop, overflow]
+    uint32_t list_cap;
+};
+
+void launch_flood_start(hipStream_t st, const FloodJob *jobs, uint32_t n, FloodBufs B, uint32_t *eval, uint32_t *prop);
+void launch_flood_eval(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
+                       const uint32_t *list, uint32_t n);
+void launch_flood_prop(hipStream_t st, const FloodJob *jobs, FloodBufs B, const uint32_t *list, uint32_t n,
+                       uint32_t *next_eval, uint32_t *next_prop);
+void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res);
+void launch_flood_resinit(hipStream_t st, const FloodJob *jobs, uint32_t n, FloodRes *res);
+
+}  // namespace dsx

@@ -1,0 +1,321 @@
+// C ABI for the climate sampler and the flood engine (include/dsx.h: dsx_noise, dsx_climate,
+// dsx_predicates, dsx_flood). Replaces the CPU floods of dsearch_desert.cpp:326-533 and tools/dflood.cpp.
+#include "climate_api.h"
+#include "ctx.h"
+#include "floods.h"
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+// Surface-biome grid (surface.hpp:12-55): the depth-0 leaves of the parameter tree are boxes that
+// tile (T, H, C, E, W) space; index them on the grid of all box boundaries. A cell covered by one
+// biome maps to it; cells covered by several biomes or none map to -1 (the tree decides).
+struct HostGrid {
+    std::vector<int32_t> cuts[5];
+    std::vector<int16_t> cell;
+    int dim[5] = {};
+    HostGrid() {
+        static const int AX[5] = {0, 1, 2, 3, 5};
+        struct Box { int32_t lo[5], hi[5]; int biome; };
+        std::vector<Box> boxes;
+        for (int n = 0; n < DSX_NTREE; n++) {
+            const uint64_t e = DSX_TREE[n];
+            if ((e >> 56) != 255) continue;
+            const int32_t *d = DSX_PARAM[(e >> 32) & 255];
+            if (d[0] != 0 || d[1] != 0) continue;
+            Box b;
+            b.biome = int((e >> 48) & 255);
+            for (int a = 0; a < 5; a++) {
+                const int32_t *r = DSX_PARAM[(e >> (8 * AX[a])) & 255];
+                b.lo[a] = r[0];
+                b.hi[a] = r[1];
+            }
+            boxes.push_back(b);
+        }
+        for (int a = 0; a < 5; a++) {
+            for (const Box &b : boxes) { cuts[a].push_back(b.lo[a]); cuts[a].push_back(b.hi[a] + 1); }
+            std::sort(cuts[a].begin(), cuts[a].end());
+            cuts[a].erase(std::unique(cuts[a].begin(), cuts[a].end()), cuts[a].end());
+            dim[a] = int(cuts[a].size()) - 1;
+        }
+        size_t total = 1;
+        for (int a = 0; a < 5; a++) total *= size_t(dim[a]);
+        cell.assign(total, -2);
+        for (const Box &b : boxes) {
+            int lo[5], hi[5];
+            for (int a = 0; a < 5; a++) {
+                lo[a] = int(std::lower_bound(cuts[a].begin(), cuts[a].end(), b.lo[a]) - cuts[a].begin());
+                hi[a] = int(std::lower_bound(cuts[a].begin(), cuts[a].end(), b.hi[a] + 1) - cuts[a].begin());
+            }
+            for (int i0 = lo[0]; i0 < hi[0]; i0++)
+                for (int i1 = lo[1]; i1 < hi[1]; i1++)
+                    for (int i2 = lo[2]; i2 < hi[2]; i2++)
+                        for (int i3 = lo[3]; i3 < hi[3]; i3++)
+                            for (int i4 = lo[4]; i4 < hi[4]; i4++) {
+                                const size_t k = ((((size_t(i0) * dim[1] + i1) * dim[2] + i2) * dim[3] + i3) * dim[4] + i4);
+                                int16_t &v = cell[k];
+                                v = (v == -2) ? int16_t(b.biome) : (v == b.biome ? v : int16_t(-1));
+                            }
+        }
+        for (int16_t &v : cell)
+            if (v == -2) v = -1;
+    }
+};
+
+}  // namespace
+
+// Device-resident climate state of a context (allocated on first use).
+struct ClimState {
+    dsx::SurfGrid grid{};
+    int32_t *d_cuts = nullptr;
+    int16_t *d_cell = nullptr;
+    dsx::ClimSeed *d_clim = nullptr;
+    uint32_t clim_cap = 0;
+    uint64_t *d_useeds = nullptr;
+    // flood buffers
+    dsx::FloodBufs fb{};
+    dsx::FloodJob *d_jobs = nullptr;
+    dsx::FloodRes *d_res = nullptr;
+    uint32_t job_cap = 0;
+    // batch I/O staging
+    void *d_io = nullptr;
+    size_t io_cap = 0;
+};
+
+static ClimState *clim_state(dsx_ctx *c);
+
+static int clim_alloc(dsx_ctx *c, ClimState *&S) {
+    S = clim_state(c);
+    if (S->d_cell) return DSX_OK;
+    static const HostGrid G;
+    size_t ncut = 0;
+    for (int a = 0; a < 5; a++) ncut += G.cuts[a].size();
+    HIPCK(hipMalloc(&S->d_cuts, ncut * 4));
+    HIPCK(hipMalloc(&S->d_cell, G.cell.size() * 2));
+    size_t off = 0;
+    for (int a = 0; a < 5; a++) {
+        HIPCK(hipMemcpy(S->d_cuts + off, G.cuts[a].data(), G.cuts[a].size() * 4, hipMemcpyHostToDevice));
+        S->grid.cuts[a] = S->d_cuts + off;
+        S->grid.ncut[a] = int(G.cuts[a].size());
+        S->grid.dim[a] = G.dim[a];
+        off += G.cuts[a].size();
+    }
+    HIPCK(hipMemcpy(S->d_cell, G.cell.data(), G.cell.size() * 2, hipMemcpyHostToDevice));
+    S->grid.cell = S->d_cell;
+    S->clim_cap = 4096;
+    HIPCK(hipMalloc(&S->d_clim, size_t(S->clim_cap) * sizeof(dsx::ClimSeed)));
+    HIPCK(hipMalloc(&S->d_useeds, size_t(S->clim_cap) * 8));
+    // flood engine
+    dsx::FloodBufs &B = S->fb;
+    B.tile_cap = 1u << 21;                    // 2M tiles x 528 B
+    B.list_cap = B.tile_cap;
+    B.hmask = (1u << 23) - 1;                 // 8M hash slots
+    HIPCK(hipMalloc(&B.tiles, size_t(B.tile_cap) * sizeof(dsx::Tile)));
+    HIPCK(hipMemset(B.tiles, 0, size_t(B.tile_cap) * sizeof(dsx::Tile)));
+    HIPCK(hipMalloc(&B.ntiles, 4));
+    HIPCK(hipMalloc(&B.hkeys, (size_t(B.hmask) + 1) * 8));
+    HIPCK(hipMemset(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8));
+    for (auto &l : B.lists) HIPCK(hipMalloc(&l, size_t(B.list_cap) * 4));
+    HIPCK(hipMalloc(&B.lcount, 8 * 4));
+    S->job_cap = 1u << 20;
+    HIPCK(hipMalloc(&S->d_jobs, size_t(S->job_cap) * sizeof(dsx::FloodJob)));
+    HIPCK(hipMalloc(&S->d_res, size_t(S->job_cap) * sizeof(dsx::FloodRes)));
+    return DSX_OK;
+}
+
+static int io_reserve(dsx_ctx *c, ClimState *S, size_t bytes) {
+    if (S->io_cap >= bytes) return DSX_OK;
+    if (S->d_io) HIPCK(hipFree(S->d_io));
+    S->d_io = nullptr;
+    S->io_cap = std::max<size_t>(bytes, 1 << 20);
+    HIPCK(hipMalloc(&S->d_io, S->io_cap));
+    return DSX_OK;
+}
+
+// Climate of `n` distinct seeds into S->d_clim[0..n).
+static int clim_seeds(dsx_ctx *c, ClimState *S, const uint64_t *seeds, uint32_t n) {
+    if (n > S->clim_cap) {
+        dsx::g_last_error = "too many distinct seeds in one call";
+        return DSX_E_CAPACITY;
+    }
+    HIPCK(hipMemcpyAsync(S->d_useeds, seeds, size_t(n) * 8, hipMemcpyHostToDevice, c->st));
+    dsx::launch_climinit(c->st, S->d_useeds, n, S->d_clim);
+    HIPCK(hipGetLastError());
+    return DSX_OK;
+}
+
+// All per-context extension state lives here; freed by dsx_destroy via ctx_free_ext.
+#include <mutex>
+static std::mutex g_ext_mu;
+static std::unordered_map<dsx_ctx *, ClimState *> g_ext;
+static ClimState *clim_state(dsx_ctx *c) {
+    std::lock_guard<std::mutex> l(g_ext_mu);
+    ClimState *&s = g_ext[c];
+    if (!s) s = new ClimState();
+    return s;
+}
+void dsx_ctx_free_ext(dsx_ctx *c) {
+    ClimState *s = nullptr;
+    {
+        std::lock_guard<std::mutex> l(g_ext_mu);
+        auto it = g_ext.find(c);
+        if (it == g_ext.end()) return;
+        s = it->second;
+        g_ext.erase(it);
+    }
+    for (void *p : {(void *)s->d_cuts, (void *)s->d_cell, (void *)s->d_clim, (void *)s->d_useeds, (void *)s->fb.tiles,
+                    (void *)s->fb.ntiles, (void *)s->fb.hkeys, (void *)s->fb.lists[0], (void *)s->fb.lists[1],
+                    (void *)s->fb.lists[2], (void *)s->fb.lists[3], (void *)s->fb.lcount, (void *)s->d_jobs,
+                    (void *)s->d_res, s->d_io})
+        if (p) (void)hipFree(p);
+    delete s;
+}
+
+extern "C" {
+
+int dsx_noise(dsx_ctx *c, uint64_t seed, int channel, const double *xyz, uint64_t n, double *out) {
+    if (!c || channel < 0 || channel > 5 || (n && (!xyz || !out))) return DSX_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    ClimState *S;
+    int rc = clim_alloc(c, S);
+    if (rc) return rc;
+    if ((rc = clim_seeds(c, S, &seed, 1))) return rc;
+    if ((rc = io_reserve(c, S, n * 32))) return rc;
+    double *d_pts = static_cast<double *>(S->d_io), *d_out = d_pts + 3 * n;
+    HIPCK(hipMemcpyAsync(d_pts, xyz, n * 24, hipMemcpyHostToDevice, c->st));
+    dsx::launch_noise(c->st, S->d_clim, channel, d_pts, uint32_t(n), d_out);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(out, d_out, n * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCK(hipStreamSynchronize(c->st));
+    return DSX_OK;
+}
+
+int dsx_climate(dsx_ctx *c, uint64_t seed, const int32_t *xs, const int32_t *zs, uint64_t n, int64_t *q) {
+    if (!c || (n && (!xs || !zs || !q))) return DSX_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    ClimState *S;
+    int rc = clim_alloc(c, S);
+    if (rc) return rc;
+    if ((rc = clim_seeds(c, S, &seed, 1))) return rc;
+    if ((rc = io_reserve(c, S, n * 56))) return rc;
+    int32_t *d_x = static_cast<int32_t *>(S->d_io), *d_z = d_x + n;
+    int64_t *d_q = reinterpret_cast<int64_t *>(d_z + n + (n & 1));
+    HIPCK(hipMemcpyAsync(d_x, xs, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCK(hipMemcpyAsync(d_z, zs, n * 4, hipMemcpyHostToDevice, c->st));
+    dsx::launch_climate(c->st, S->d_clim, d_x, d_z, uint32_t(n), d_q);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(q, d_q, n * 48, hipMemcpyDeviceToHost, c->st));
+    HIPCK(hipStreamSynchronize(c->st));
+    return DSX_OK;
+}
+
+int dsx_predicates(dsx_ctx *c, uint64_t seed, int kind, float cut, const int32_t *xs, const int32_t *zs, uint64_t n,
+                   int8_t *out) {
+    if (!c || kind < 0 || kind > 3 || (n && (!xs || !zs || !out))) return DSX_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    ClimState *S;
+    int rc = clim_alloc(c, S);
+    if (rc) return rc;
+    if ((rc = clim_seeds(c, S, &seed, 1))) return rc;
+    if ((rc = io_reserve(c, S, n * 9))) return rc;
+    int32_t *d_x = static_cast<int32_t *>(S->d_io), *d_z = d_x + n;
+    int8_t *d_o = reinterpret_cast<int8_t *>(d_z + n);
+    HIPCK(hipMemcpyAsync(d_x, xs, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCK(hipMemcpyAsync(d_z, zs, n * 4, hipMemcpyHostToDevice, c->st));
+    dsx::launch_pred(c->st, S->d_clim, S->grid, kind, cut, d_x, d_z, uint32_t(n), d_o);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(out, d_o, n, hipMemcpyDeviceToHost, c->st));
+    HIPCK(hipStreamSynchronize(c->st));
+    return DSX_OK;
+}
+
+int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) {
+    if (!c || (n && (!jobs || !out))) return DSX_E_ARG;
+    for (uint64_t k = 0; k < n; k++) {
+        const dsx_flood_job &j = jobs[k];
+        if (j.kind < 0 || j.kind > 3 || j.step < 1 || j.half < 0 || 2 * int64_t(j.half) + 1 > 65535 ||
+            int64_t(j.half) * j.step > 1000000)
+            return DSX_E_ARG;
+    }
+    if (n == 0) return DSX_OK;
+    HIPCK(hipSetDevice(c->device));
+    ClimState *S;
+    int rc = clim_alloc(c, S);
+    if (rc) return rc;
+    if (n > S->job_cap) {
+        dsx::g_last_error = "too many floods in one call";
+        return DSX_E_CAPACITY;
+    }
+    // distinct seeds -> climate slots
+    std::vector<uint64_t> useeds;
+    std::unordered_map<uint64_t, int32_t> slot;
+    std::vector<dsx::FloodJob> hj(n);
+    for (uint64_t k = 0; k < n; k++) {
+        auto it = slot.find(jobs[k].seed);
+        int32_t s;
+        if (it == slot.end()) {
+            s = int32_t(useeds.size());
+            slot.emplace(jobs[k].seed, s);
+            useeds.push_back(jobs[k].seed);
+        } else {
+            s = it->second;
+        }
+        hj[k] = dsx::FloodJob{s, jobs[k].kind, jobs[k].cut, jobs[k].qx, jobs[k].qz, jobs[k].step, jobs[k].half,
+                              jobs[k].conn8};
+    }
+    if ((rc = clim_seeds(c, S, useeds.data(), uint32_t(useeds.size())))) return rc;
+    dsx::FloodBufs &B = S->fb;
+    HIPCK(hipMemcpyAsync(S->d_jobs, hj.data(), n * sizeof(dsx::FloodJob), hipMemcpyHostToDevice, c->st));
+    HIPCK(hipMemsetAsync(B.ntiles, 0, 4, c->st));
+    HIPCK(hipMemsetAsync(B.lcount, 0, 32, c->st));
+    dsx::launch_flood_resinit(c->st, S->d_jobs, uint32_t(n), S->d_res);
+    dsx::launch_flood_start(c->st, S->d_jobs, uint32_t(n), B, B.lists[0], B.lists[1]);
+    HIPCK(hipGetLastError());
+    int cur = 0;   // lists[2*cur] eval, lists[2*cur+1] prop
+    uint32_t *h = c->h_cnt;
+    int status = DSX_OK;
+    for (int round = 0;; round++) {
+        HIPCK(hipMemcpyAsync(h, B.lcount, 20, hipMemcpyDeviceToHost, c->st));
+        HIPCK(hipStreamSynchronize(c->st));
+        if (h[4]) { status = DSX_E_CAPACITY; break; }
+        const uint32_t ne = h[0], np = h[1];
+        if (np == 0 || round > 4000000) break;
+        uint32_t *le = B.lists[2 * cur], *lp = B.lists[2 * cur + 1];
+        uint32_t *nle = B.lists[2 * (1 - cur)], *nlp = B.lists[2 * (1 - cur) + 1];
+        // next-round counters start at zero; this round's lists are consumed
+        HIPCK(hipMemsetAsync(B.lcount + 2, 0, 8, c->st));
+        dsx::launch_flood_eval(c->st, S->d_jobs, S->d_clim, S->grid, B, le, ne);
+        dsx::launch_flood_prop(c->st, S->d_jobs, B, lp, np, nle, nlp);
+        HIPCK(hipGetLastError());
+        // move next counters into place: [0,1] <- [2,3]
+        HIPCK(hipMemcpyAsync(B.lcount, B.lcount + 2, 8, hipMemcpyDeviceToDevice, c->st));
+        cur = 1 - cur;
+    }
+    uint32_t ntiles = 0;
+    HIPCK(hipMemcpyAsync(&ntiles, B.ntiles, 4, hipMemcpyDeviceToHost, c->st));
+    HIPCK(hipStreamSynchronize(c->st));
+    ntiles = std::min(ntiles, B.tile_cap);
+    if (status == DSX_OK) {
+        dsx::launch_flood_acc(c->st, S->d_jobs, B, ntiles, S->d_res);
+        HIPCK(hipGetLastError());
+        std::vector<dsx::FloodRes> res(n);
+        HIPCK(hipMemcpyAsync(res.data(), S->d_res, n * sizeof(dsx::FloodRes), hipMemcpyDeviceToHost, c->st));
+        HIPCK(hipStreamSynchronize(c->st));
+        for (uint64_t k = 0; k < n; k++)
+            out[k] = dsx_area{res[k].cells, res[k].sampled, res[k].minx, res[k].maxx, res[k].minz, res[k].maxz,
+                              res[k].edge, res[k].border};
+    } else {
+        dsx::g_last_error = "flood tile pool exhausted: split the batch";
+    }
+    // leave the pool and hash clean for the next call
+    HIPCK(hipMemsetAsync(B.tiles, 0, size_t(ntiles) * sizeof(dsx::Tile), c->st));
+    HIPCK(hipMemsetAsync(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8, c->st));
+    HIPCK(hipStreamSynchronize(c->st));
+    return status;
+}
+
+}  // extern "C"

@@ -45,6 +45,21 @@ class ScanParams(ctypes.Structure):
 RAW_CAND = np.dtype([("s", "<f4"), ("t", "<f4"), ("c", "<f4"), ("e", "<f4"), ("x", "<i4"), ("z", "<i4")])
 
 
+FLOOD_JOB = np.dtype([("seed", "<u8"), ("kind", "<i4"), ("cut", "<f4"), ("qx", "<i4"), ("qz", "<i4"), ("step", "<i4"),
+                      ("half", "<i4"), ("conn8", "<i4"), ("pad", "<i4")])
+AREA = np.dtype([("cells", "<u8"), ("sampled", "<u8"), ("minx", "<i4"), ("maxx", "<i4"), ("minz", "<i4"),
+                 ("maxz", "<i4"), ("edge", "<i4"), ("border", "<i4")])
+PRED_DESERT_STATE, PRED_HOT, PRED_PROXY, PRED_DESERT = 0, 1, 2, 3
+
+
+def flood_jobs(rows):
+    """rows of (seed, kind, cut, qx, qz, step, half, conn8) -> FLOOD_JOB array"""
+    a = np.zeros(len(rows), FLOOD_JOB)
+    for k, r in enumerate(rows):
+        a[k] = tuple(r) + (0,)
+    return a
+
+
 class GateCounts(ctypes.Structure):
     _fields_ = [("phase", ctypes.c_uint64), ("pre", ctypes.c_uint64), ("full", ctypes.c_uint64)]
 
@@ -77,7 +92,15 @@ def load(path=LIB_PATH):
     lib.dsx_scan.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, ctypes.POINTER(ScanParams), _f32p, _f32p,
                              ctypes.c_void_p, ctypes.c_uint64, _u64p]
     lib.dsx_tables.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ScanParams), _f32p]
-    for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile", "dsx_scan", "dsx_tables"):
+    lib.dsx_noise.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
+                              ctypes.c_void_p]
+    lib.dsx_climate.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                ctypes.c_void_p]
+    lib.dsx_predicates.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+    lib.dsx_flood.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+    for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile", "dsx_scan", "dsx_tables",
+              "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -161,4 +184,33 @@ class Engine:
         p = params or ScanParams()
         out = np.zeros((6, 512, 512), np.float32)
         self._check(self.lib.dsx_tables(self.h, seed, ctypes.byref(p), _ptr(out, ctypes.c_float)), "dsx_tables")
+        return out
+
+    def noise(self, seed, channel, xyz):
+        p = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(p), np.float64)
+        self._check(self.lib.dsx_noise(self.h, seed, channel, p.ctypes.data, len(p), out.ctypes.data), "dsx_noise")
+        return out
+
+    def climate(self, seed, xs, zs):
+        x = np.ascontiguousarray(xs, dtype=np.int32)
+        z = np.ascontiguousarray(zs, dtype=np.int32)
+        q = np.zeros((len(x), 6), np.int64)
+        self._check(self.lib.dsx_climate(self.h, seed, x.ctypes.data, z.ctypes.data, len(x), q.ctypes.data),
+                    "dsx_climate")
+        return q
+
+    def predicates(self, seed, kind, xs, zs, cut=0.35):
+        x = np.ascontiguousarray(xs, dtype=np.int32)
+        z = np.ascontiguousarray(zs, dtype=np.int32)
+        out = np.zeros(len(x), np.int8)
+        self._check(self.lib.dsx_predicates(self.h, seed, kind, cut, x.ctypes.data, z.ctypes.data, len(x),
+                                            out.ctypes.data), "dsx_predicates")
+        return out
+
+    def flood(self, jobs):
+        """jobs: FLOOD_JOB array (see flood_jobs) -> AREA array"""
+        j = np.ascontiguousarray(jobs, dtype=FLOOD_JOB)
+        out = np.zeros(len(j), AREA)
+        self._check(self.lib.dsx_flood(self.h, j.ctypes.data, len(j), out.ctypes.data), "dsx_flood")
         return out

@@ -102,6 +102,51 @@ int dsx_scan(dsx_ctx *ctx, const uint64_t *seeds, uint64_t n, const dsx_scan_par
  * (lowfield.hpp tabulate_octave + Blur; the `gscancheck` comparison, dsearch_desert.cpp:292-296). */
 int dsx_tables(dsx_ctx *ctx, uint64_t seed, const dsx_scan_params *p, float *out);
 
+/* ---------------------------------------------------------------- FP64 climate sampler */
+/* Noise channels (standalone.hpp:93) and flood predicates. */
+enum { DSX_NOISE_T = 0, DSX_NOISE_H = 1, DSX_NOISE_C = 2, DSX_NOISE_E = 3, DSX_NOISE_SHIFT = 4, DSX_NOISE_W = 5 };
+enum {
+    DSX_PRED_DESERT_STATE = 0, /* SurfaceSampler::desert_state: 2 desert, 1 bridgeable river, 0 other (surface.hpp:98-124) */
+    DSX_PRED_HOT = 1,          /* T(x, 0, z) >= 0.55 -> 2 else 0 (dsearch_desert.cpp:334) */
+    DSX_PRED_PROXY = 2,        /* DesertProxy(cut): 2 or 0 (desert_proxy.hpp:32) */
+    DSX_PRED_DESERT = 3        /* SurfaceSampler::desert -> 2 or 0 (surface.hpp:85-93) */
+};
+
+/* NormalNoise::sample (standalone.hpp:140-151) of `channel` for seed at n points xyz[3k..3k+2], FP64. */
+int dsx_noise(dsx_ctx *ctx, uint64_t seed, int channel, const double *xyz, uint64_t n, double *out);
+/* Sampler::climate (standalone.hpp:224-238) at quarts (xs[k], 64, zs[k]): q[6k..6k+5] = 10000 x {T,H,C,E,D,W}. */
+int dsx_climate(dsx_ctx *ctx, uint64_t seed, const int32_t *xs, const int32_t *zs, uint64_t n, int64_t *q);
+/* Flood predicate `kind` (DSX_PRED_*) at quarts (xs[k], zs[k]); cut is the proxy cut (Params::pcut). */
+int dsx_predicates(dsx_ctx *ctx, uint64_t seed, int kind, float cut, const int32_t *xs, const int32_t *zs, uint64_t n,
+                   int8_t *out);
+
+/* ---------------------------------------------------------------- bounded floods (S3-S4, area queries) */
+/* One flood: window of (2*half+1)^2 lattice points of spacing `step` quarts centred on (qx, qz). */
+typedef struct {
+    uint64_t seed;
+    int32_t kind;     /* DSX_PRED_* */
+    float cut;        /* proxy cut (kind DSX_PRED_PROXY) */
+    int32_t qx, qz;   /* centre, quarts */
+    int32_t step, half;
+    int32_t conn8;    /* 1: also step to diagonal neighbours */
+} dsx_flood_job;
+
+/* standalone::DenseArea (dense_flood.hpp:9-13). */
+typedef struct {
+    uint64_t cells;   /* cells with predicate 2 reached from the centre */
+    uint64_t sampled; /* cells whose predicate was evaluated */
+    int32_t minx, maxx, minz, maxz; /* quart bounds of the counted cells, starting at the centre */
+    int32_t edge;     /* the flood reached the window edge */
+    int32_t border;   /* the flood touched the world border (|x| or |z| > 7,499,996) */
+} dsx_area;
+
+/* A batch of floods, evaluated together on the device. Replaces dense_flood_bridged
+ * (dense_flood.hpp:52-86) at every call site: hot zones (dsearch_desert.cpp:337, 352), proxy floods
+ * (:387), exact floods (:512, :521) and tools/dflood.cpp:20. Results are identical to the reference's
+ * depth-first flood (they do not depend on visiting order). DSX_E_CAPACITY if the batch needs more
+ * tiles than the engine holds (split the batch). */
+int dsx_flood(dsx_ctx *ctx, const dsx_flood_job *jobs, uint64_t n, dsx_area *out);
+
 #ifdef __cplusplus
 }
 #endif

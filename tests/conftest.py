@@ -27,6 +27,15 @@ def load_package():
     return mod
 
 
+class FloodOut(ctypes.Structure):
+    _fields_ = [("cells", ctypes.c_uint64), ("sampled", ctypes.c_uint64), ("minx", ctypes.c_int32),
+                ("maxx", ctypes.c_int32), ("minz", ctypes.c_int32), ("maxz", ctypes.c_int32),
+                ("edge", ctypes.c_int32), ("border", ctypes.c_int32)]
+
+    def tuple(self):
+        return (self.cells, self.sampled, self.minx, self.maxx, self.minz, self.maxz, bool(self.edge), bool(self.border))
+
+
 class Oracle:
     """ctypes view of oracle/liboracle.so (the CPU checker)."""
 
@@ -86,6 +95,18 @@ class Oracle:
         V = np.zeros((G, G), np.float32)
         self.L.or_gate_grid(seed, side, G, box, V.ctypes.data)
         return V
+
+    def flood(self, seed, kind, cut, qx, qz, step, half, conn8):
+        out = FloodOut()
+        self.L.or_flood(seed, kind, cut, qx, qz, step, half, conn8, ctypes.byref(out), None)
+        return out.tuple()
+
+    def pred_batch(self, seed, kind, xs, zs, cut=0.35):
+        x = np.ascontiguousarray(xs, dtype=np.int32)
+        z = np.ascontiguousarray(zs, dtype=np.int32)
+        out = np.zeros(len(x), np.int8)
+        self.L.or_pred_batch(seed, kind, cut, x.ctypes.data, z.ctypes.data, len(x), out.ctypes.data)
+        return out
 
     def phase_list(self, start, count, maxph, cap=1 << 22):
         out = np.zeros(cap, np.uint64)

@@ -1,0 +1,124 @@
+"""FP64 climate sampler and flood engine on the GPU against the CPU oracle (bitwise / exact), and the
+exact step-1 area at verified finds against MC-Measure's cell counts."""
+import concurrent.futures as cf
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_package
+
+pytestmark = pytest.mark.gpu
+
+FINDS = [json.loads(l) for l in open(os.path.join(GOLDEN, "desert_finds.jsonl"))]
+FINDS = [r for r in FINDS if "x" in r and "cells" in r]
+SEEDS = [int(r["seed"]) % (1 << 64) for r in FINDS[:3]] + [12345]
+
+
+def u64(s):
+    return int(s) % (1 << 64)
+
+
+@pytest.mark.parametrize("ch", range(6))
+def test_noise_bitexact(engine, oracle, ch):
+    rng = np.random.default_rng(ch)
+    for seed in SEEDS[:2]:
+        xyz = np.column_stack([rng.uniform(-8e6, 8e6, 300), np.zeros(300), rng.uniform(-8e6, 8e6, 300)])
+        xyz[:100, 1] = rng.uniform(-8e6, 8e6, 100)          # y != 0 branch (Shift's second sample)
+        xyz[100:150] = np.round(xyz[100:150])                # integer quarts
+        got = engine.noise(seed, ch, xyz)
+        want = np.array([oracle.or_noise(seed, ch, *p) for p in xyz])
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
+def test_climate_vector_bitexact(engine, oracle):
+    r = FINDS[0]
+    seed = u64(r["seed"])
+    rng = np.random.default_rng(1)
+    xs = (r["x"] >> 2) + rng.integers(-3000, 3000, 400)
+    zs = (r["z"] >> 2) + rng.integers(-3000, 3000, 400)
+    got = engine.climate(seed, xs, zs)
+    for k in range(len(xs)):
+        q = np.zeros(6, np.int64)
+        oracle.or_climate(seed, int(xs[k]), int(zs[k]), q.ctypes.data)
+        assert np.array_equal(got[k], q), k
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_predicates_exact(engine, oracle, kind):
+    """Predicates around verified deserts (many desert / river / hot cells) and at random places."""
+    rng = np.random.default_rng(kind)
+    for r in FINDS[:3]:
+        seed = u64(r["seed"])
+        xs = np.concatenate([(r["x"] >> 2) + rng.integers(-2500, 2500, 6000), rng.integers(-7_500_000, 7_500_000, 1000)])
+        zs = np.concatenate([(r["z"] >> 2) + rng.integers(-2500, 2500, 6000), rng.integers(-7_500_000, 7_500_000, 1000)])
+        got = engine.predicates(seed, kind, xs, zs)
+        want = oracle.pred_batch(seed, kind, xs, zs)
+        assert np.array_equal(got, want)
+        if kind == 0:
+            assert (got == 2).sum() > 500 and (got == 1).sum() > 0
+
+
+def _area(a):
+    return (int(a["cells"]), int(a["sampled"]), int(a["minx"]), int(a["maxx"]), int(a["minz"]), int(a["maxz"]),
+            bool(a["edge"]), bool(a["border"]))
+
+
+def _compare(engine, oracle, rows):
+    dsx = load_package().dsx
+    got = engine.flood(dsx.flood_jobs(rows))
+    with cf.ThreadPoolExecutor(8) as ex:
+        want = list(ex.map(lambda r: oracle.flood(*r), rows))
+    for r, g, w in zip(rows, got, want):
+        assert _area(g) == w, (r, _area(g), w)
+
+
+def test_flood_desert_state_steps(engine, oracle):
+    """Bridged desert_state floods at the verified finds, steps 8 (4- and 8-connected) and 4."""
+    rows = []
+    for r in FINDS[:6]:
+        seed, qx, qz = u64(r["seed"]), r["x"] >> 2, r["z"] >> 2
+        rows += [(seed, 0, 0.35, qx, qz, 8, 2500, 0), (seed, 0, 0.35, qx, qz, 8, 2000, 1), (seed, 0, 0.35, qx, qz, 4, 4000, 0)]
+    _compare(engine, oracle, rows)
+
+
+def test_flood_hot_and_proxy(engine, oracle):
+    """Hot-zone floods (step 256 and 128, 4-connected) and proxy floods (step 32, 8-connected)."""
+    rows = []
+    for r in FINDS[:8]:
+        seed, qx, qz = u64(r["seed"]), r["x"] >> 2, r["z"] >> 2
+        rows += [(seed, 1, 0.35, qx, qz, 256, 125, 0), (seed, 1, 0.35, qx, qz, 128, 250, 0),
+                 (seed, 2, 0.35, qx, qz, 32, 500, 1), (seed, 2, 0.5, qx + 640, qz - 320, 32, 500, 1)]
+    _compare(engine, oracle, rows)
+
+
+def test_flood_edges_and_borders(engine, oracle):
+    """Window edge hits, world border, a start that is not passable, a start outside the world."""
+    r = FINDS[0]
+    seed, qx, qz = u64(r["seed"]), r["x"] >> 2, r["z"] >> 2
+    rows = [(seed, 0, 0.35, qx, qz, 8, 60, 0), (seed, 0, 0.35, qx, qz, 4, 33, 1), (seed, 0, 0.35, qx, qz, 1, 0, 0),
+            (seed, 1, 0.35, 7_499_000, 100, 256, 125, 0), (seed, 1, 0.35, -7_499_990, 7_499_990, 64, 200, 1),
+            (seed, 1, 0.35, 7_499_998, 0, 64, 40, 0), (seed, 0, 0.35, qx + 20000, qz + 20000, 4, 500, 0),
+            (seed, 2, 0.35, 0, 0, 32, 31, 1), (seed, 1, 0.35, 5, 5, 16, 16, 0)]
+    _compare(engine, oracle, rows)
+
+
+def test_area_queries_step1_match_mcmeasure(engine):
+    """The exact step-1 flood (tools/dflood.cpp: half 20000) at verified finds = MC-Measure's cells."""
+    dsx = load_package().dsx
+    rows = [(u64(r["seed"]), 0, 0.35, r["x"] >> 2, r["z"] >> 2, 1, 20000, 0) for r in FINDS[:4]]
+    got = engine.flood(dsx.flood_jobs(rows))
+    for r, g in zip(FINDS[:4], got):
+        assert int(g["cells"]) == r["cells"] and not g["edge"] and not g["border"]
+
+
+def test_flood_golden_step4_all_finds(engine):
+    """Step-4 floods at all 220 finds in one batch against the reference dflood readings."""
+    dsx = load_package().dsx
+    gold = [json.loads(l) for l in open(os.path.join(GOLDEN, "dflood.jsonl"))]
+    gold = [g for g in gold if g["step"] == 4]
+    rows = [(u64(g["seed"]), 0, 0.35, g["x"] >> 2, g["z"] >> 2, 4, 5000, 0) for g in gold]
+    got = engine.flood(dsx.flood_jobs(rows))
+    assert [int(a["cells"]) for a in got] == [g["cells"] for g in gold]
