@@ -14,11 +14,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off: no FMA contraction, results bit-identical to the reference's CPU arithmetic.
 # -packed-fp32-ops off: no v_pk_{add,mul,fma}_f32 anywhere in the device code.
-HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=off",
              "-fno-gpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
 HIP_SOURCES = ["csrc/gate.hip", "csrc/tables.hip", "csrc/climate.hip", "csrc/floods.hip", "csrc/dsx_api.hip",
-               "csrc/scan_api.hip", "csrc/floods_api.hip"]
+               "csrc/scan_api.hip", "csrc/floods_api.hip", "host/search.cpp"]
+HOST_TOOLS = ["host/dsearch_desert.cpp"]
 
 
 def build_lib(verbose=False):
@@ -31,6 +32,13 @@ def build_lib(verbose=False):
     if verbose and r.stderr.strip():
         print(r.stderr)
     os.replace(tmp, out)
+    for src in HOST_TOOLS:
+        exe = os.path.join(HERE, os.path.splitext(os.path.basename(src))[0])
+        r = subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(HERE, src), "-L" + HERE, "-ldsx",
+                            "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("g++ failed:\n" + r.stdout + r.stderr)
+        os.replace(exe + ".tmp", exe)
     return out
 
 

@@ -109,6 +109,18 @@ __global__ __launch_bounds__(256) void k_pred(const ClimSeed *cs, SurfGrid g, in
     if (k < n) out[k] = int8_t(predicate(kind, C, g, xs[k], zs[k], cut));
 }
 
+// Points of many seeds: the climate stays in global memory (L2), one thread per point.
+__global__ __launch_bounds__(256) void k_points(const ClimSeed *cs, SurfGrid g, const uint32_t *sidx, const int32_t *xs,
+                                                const int32_t *zs, uint32_t n, int what, float cut, int8_t *out8,
+                                                double *outd) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= n) return;
+    const ClimSeed &C = cs[sidx[k]];
+    if (what == 16) outd[k] = noise_sample(C, DSX_CH_T, xs[k], 0, zs[k]);
+    else if (what == 17) outd[k] = proxy_prob(C, xs[k], zs[k]);
+    else out8[k] = int8_t(predicate(what, C, g, xs[k], zs[k], cut));
+}
+
 void launch_climinit(hipStream_t st, const uint64_t *seeds, uint32_t n, ClimSeed *out) {
     if (n) hipLaunchKernelGGL(k_climinit, dim3((CLIM_NOCT * n + 63) / 64), dim3(64), 0, st, seeds, n, out);
 }
@@ -117,6 +129,10 @@ void launch_noise(hipStream_t st, const ClimSeed *cs, int ch, const double *pts,
 }
 void launch_climate(hipStream_t st, const ClimSeed *cs, const int32_t *xs, const int32_t *zs, uint32_t n, int64_t *q) {
     if (n) hipLaunchKernelGGL(k_climate, dim3((n + 255) / 256), dim3(256), 0, st, cs, xs, zs, n, q);
+}
+void launch_points(hipStream_t st, const ClimSeed *cs, const SurfGrid &g, const uint32_t *sidx, const int32_t *xs,
+                   const int32_t *zs, uint32_t n, int what, float cut, int8_t *out8, double *outd) {
+    if (n) hipLaunchKernelGGL(k_points, dim3((n + 255) / 256), dim3(256), 0, st, cs, g, sidx, xs, zs, n, what, cut, out8, outd);
 }
 void launch_pred(hipStream_t st, const ClimSeed *cs, const SurfGrid &g, int kind, float cut, const int32_t *xs,
                  const int32_t *zs, uint32_t n, int8_t *out) {

@@ -32,7 +32,28 @@ dsx::GateKernel gate_kernel(int R, float sig) {
     float ks = 0;
     for (int t = 0; t <= 2 * R; t++) ks += K.k[t];
     K.norm = 1.f / (ks * ks);
-    K.w = float(1.25 * (32.0 / 63.0) * 1.5);
+    K.wd = double(float(1.25 * (32.0 / 63.0) * 1.5));
+    return K;
+}
+// CPU-default pre-gate kernel (lowfield.hpp:92-96): r = 4, float(exp(-d^2/2)) normalised by their
+// double sum; octave weight factor * weight of temperature octave 0 (dsearch_desert.cpp:217).
+dsx::GateKernel cpu_gate_kernel() {
+    dsx::GateKernel K{};
+    constexpr int r = 4;
+    double s = 0;
+    for (int d = -r; d <= r; d++) {
+        K.k[d + r] = float(std::exp(-0.5 * d * d));
+        s += K.k[d + r];
+    }
+    for (int d = 0; d <= 2 * r; d++) K.k[d] = float(K.k[d] / s);
+    K.norm = 1.f;
+    const dsx_noise_cfg &c = DSX_NOISE[DSX_CH_T];
+    const double w0 = double(1 << (c.len - 1)) / ((1 << c.len) - 1);
+    int lo = 0, hi = c.len - 1;
+    while (c.amp[lo] == 0) lo++;
+    while (c.amp[hi] == 0) hi--;
+    const double factor = (5. * (hi - lo + 1)) / (3. * (hi - lo + 2));
+    K.wd = factor * (w0 * c.amp[0]);
     return K;
 }
 }  // namespace
@@ -128,7 +149,7 @@ int dsx_gate(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *
     HIPCK(hipSetDevice(c->device));
     const dsx::GateKernel kpre = gate_kernel(4, p->box ? p->sig : 1.0f);
     const dsx::GateKernel kfull = gate_kernel(7, 2.0f);
-    const float thr_pre = p->box ? p->pthr : p->amp - p->margin;
+    const double thr_pre = p->box ? double(p->pthr) : double(p->amp - p->margin);
     const int mode_pre = p->box ? 0 : 2;
     const double maxph = double(p->max_phase);
     std::vector<std::pair<uint64_t, float>> pass;
@@ -174,7 +195,7 @@ int dsx_gate(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *
             c->prof_work[2] += n1;
             if (n2) {
                 HIPCK(hipEventRecord(c->ev[5], c->st));
-                dsx::launch_amp(c->st, 1, c->d_sides, c->d_i2, n2, kfull, p->amp, c->d_k1, c->d_k3, c->d_a3, c->d_cnt + 2,
+                dsx::launch_amp(c->st, 1, c->d_sides, c->d_i2, n2, kfull, double(p->amp), c->d_k1, c->d_k3, c->d_a3, c->d_cnt + 2,
                                 c->cap1, nullptr);
                 HIPCK(hipGetLastError());
                 HIPCK(hipEventRecord(c->ev[6], c->st));
@@ -209,6 +230,57 @@ int dsx_gate(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *
     return pass.size() > cap ? DSX_E_CAPACITY : DSX_OK;
 }
 
+int dsx_gate_cpu(dsx_ctx *c, uint64_t start, uint64_t count, double max_phase, double amp, double margin,
+                 uint64_t *seeds, float *est, uint64_t *kidx, uint64_t cap, uint64_t *n_out, uint64_t *n_phase) {
+    if (!c || !n_out || !std::isfinite(max_phase) || !std::isfinite(amp) || !std::isfinite(margin)) return DSX_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    const dsx::GateKernel K = cpu_gate_kernel();
+    const double thr = amp - margin;
+    std::vector<std::pair<uint64_t, float>> pass;
+    uint64_t nph = 0;
+    uint64_t B = 1ull << 24;
+    for (uint64_t base = 0; base < count;) {
+        const uint32_t nb = uint32_t(std::min<uint64_t>(B, count - base));
+        HIPCK(hipMemsetAsync(c->d_cnt, 0, 16, c->st));
+        dsx::launch_phase(c->st, start + base, nb, max_phase, c->d_k1, c->d_cnt, c->cap1);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(c->h_cnt, c->d_cnt, 4, hipMemcpyDeviceToHost, c->st));
+        HIPCK(hipStreamSynchronize(c->st));
+        const uint32_t n1 = c->h_cnt[0];
+        if (n1 > c->cap1) {
+            B = std::max<uint64_t>(1, B / 2);
+            continue;
+        }
+        if (n1) {
+            dsx::launch_init(c->st, start + base, c->d_k1, nullptr, n1, c->d_sides);
+            dsx::launch_amp(c->st, 2, c->d_sides, nullptr, n1, K, thr, c->d_k1, c->d_k3, c->d_a3, c->d_cnt + 1, c->cap1,
+                            nullptr);
+            HIPCK(hipGetLastError());
+            HIPCK(hipMemcpyAsync(c->h_cnt + 1, c->d_cnt + 1, 4, hipMemcpyDeviceToHost, c->st));
+            HIPCK(hipStreamSynchronize(c->st));
+            const uint32_t n2 = std::min(c->h_cnt[1], c->cap1);
+            if (n2) {
+                HIPCK(hipMemcpyAsync(c->h_k3, c->d_k3, size_t(n2) * 4, hipMemcpyDeviceToHost, c->st));
+                HIPCK(hipMemcpyAsync(c->h_a3, c->d_a3, size_t(n2) * 4, hipMemcpyDeviceToHost, c->st));
+                HIPCK(hipStreamSynchronize(c->st));
+                for (uint32_t i = 0; i < n2; i++) pass.push_back({base + c->h_k3[i], c->h_a3[i]});
+            }
+        }
+        nph += n1;
+        base += nb;
+    }
+    std::sort(pass.begin(), pass.end());
+    if (n_phase) *n_phase = nph;
+    *n_out = pass.size();
+    const uint64_t nw = std::min<uint64_t>(cap, pass.size());
+    for (uint64_t i = 0; i < nw; i++) {
+        if (seeds) seeds[i] = dsx::mix64(start + pass[i].first);
+        if (est) est[i] = pass[i].second;
+        if (kidx) kidx[i] = pass[i].first;
+    }
+    return pass.size() > cap ? DSX_E_CAPACITY : DSX_OK;
+}
+
 int dsx_gate_amps(dsx_ctx *c, const uint64_t *seeds, uint64_t n, const dsx_gate_params *p, float *pre, float *full) {
     if (!c || !valid_gate_params(p) || (n && !seeds)) return DSX_E_ARG;
     HIPCK(hipSetDevice(c->device));
@@ -222,13 +294,13 @@ int dsx_gate_amps(dsx_ctx *c, const uint64_t *seeds, uint64_t n, const dsx_gate_
         dsx::launch_init(c->st, 0, nullptr, c->d_seeds, m, c->d_sides);
         HIPCK(hipGetLastError());
         if (pre) {
-            dsx::launch_amp(c->st, mode_pre, c->d_sides, nullptr, m, kpre, 1e30f, nullptr, c->d_i2, nullptr, c->d_cnt,
+            dsx::launch_amp(c->st, mode_pre, c->d_sides, nullptr, m, kpre, 1e300, nullptr, c->d_i2, nullptr, c->d_cnt,
                             c->cap1, c->d_all);
             HIPCK(hipGetLastError());
             HIPCK(hipMemcpyAsync(pre + b, c->d_all, size_t(m) * 4, hipMemcpyDeviceToHost, c->st));
         }
         if (full) {
-            dsx::launch_amp(c->st, 1, c->d_sides, nullptr, m, kfull, 1e30f, nullptr, c->d_i2, nullptr, c->d_cnt + 1,
+            dsx::launch_amp(c->st, 1, c->d_sides, nullptr, m, kfull, 1e300, nullptr, c->d_i2, nullptr, c->d_cnt + 1,
                             c->cap1, c->d_a3);
             HIPCK(hipGetLastError());
             HIPCK(hipMemcpyAsync(full + b, c->d_a3, size_t(m) * 4, hipMemcpyDeviceToHost, c->st));

@@ -15,18 +15,20 @@ struct SideParams {
     int iy, pad_;
 };
 
-// Gate blur kernel as the kernels consume it: 2R+1 unnormalised float weights, the float
-// normalisation 1/(ksum*ksum) and the octave weight w = float(1.25 * 32/63 * 1.5).
+// Gate blur kernel as the kernels consume it: 2R+1 float weights, the float factor applied to the
+// maximum, and the octave weight. GPU gate (gate.cuh:216, 248): unnormalised expf weights, factor
+// 1/(ksum*ksum), weight float(1.25 * 32/63 * 1.5). CPU-default pre-gate (lowfield.hpp:92-96):
+// normalised float(exp / sum) weights, factor 1, weight factor * octave weight in double.
 struct GateKernel {
     float k[15];
     float norm;
-    float w;
+    double wd;   // octave weight the corner tables are scaled by
 };
 
 void launch_phase(hipStream_t st, uint64_t base, uint32_t n, double maxph, uint32_t *out, uint32_t *count, uint32_t cap);
 void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64_t *seeds, uint32_t n, SideParams *out);
 void launch_amp(hipStream_t st, int mode, const SideParams *sides, const uint32_t *sel, uint32_t n, const GateKernel &K,
-                float thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count, uint32_t cap,
+                double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count, uint32_t cap,
                 float *all_amp);
 
 }  // namespace dsx

@@ -47,6 +47,12 @@ void launch_flood_eval(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs,
 void launch_flood_prop(hipStream_t st, const FloodJob *jobs, FloodBufs B, const uint32_t *list, uint32_t n,
                        uint32_t *next_eval, uint32_t *next_prop);
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res);
+struct TileMask {
+    uint32_t job;
+    int32_t tx, tz;
+    uint32_t rows[FT];
+};
+void launch_flood_export(hipStream_t st, FloodBufs B, uint32_t ntiles, TileMask *out, uint32_t *count, uint32_t cap);
 void launch_flood_resinit(hipStream_t st, const FloodJob *jobs, uint32_t n, FloodRes *res);
 
 }  // namespace dsx

@@ -284,7 +284,31 @@ __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBuf
     }
 }
 
+// Tiles with counted reached cells -> (job, tx, tz, reach & cnt rows).
+__global__ __launch_bounds__(64) void k_flood_export(FloodBufs B, uint32_t ntiles, TileMask *out, uint32_t *count,
+                                                     uint32_t cap) {
+    const uint32_t b = blockIdx.x;
+    if (b >= ntiles) return;
+    const Tile &T = B.tiles[b];
+    const int lane = threadIdx.x;
+    const uint32_t m = lane < FT ? (T.reach[lane] & T.cnt[lane]) : 0u;
+    if (!__any(m != 0)) return;
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(count, 1u);
+    k = __shfl(k, 0);
+    if (k >= cap) return;
+    if (lane < FT) out[k].rows[lane] = m;
+    if (lane == 0) {
+        out[k].job = T.job;
+        out[k].tx = T.tx;
+        out[k].tz = T.tz;
+    }
+}
+
 // ------------------------------------------------------------------ launchers
+void launch_flood_export(hipStream_t st, FloodBufs B, uint32_t ntiles, TileMask *out, uint32_t *count, uint32_t cap) {
+    if (ntiles) hipLaunchKernelGGL(k_flood_export, dim3(ntiles), dim3(64), 0, st, B, ntiles, out, count, cap);
+}
 void launch_flood_resinit(hipStream_t st, const FloodJob *jobs, uint32_t n, FloodRes *res) {
     if (n) hipLaunchKernelGGL(k_flood_resinit, dim3((n + 255) / 256), dim3(256), 0, st, jobs, n, res);
 }

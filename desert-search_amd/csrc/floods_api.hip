@@ -84,6 +84,10 @@ struct ClimState {
     // batch I/O staging
     void *d_io = nullptr;
     size_t io_cap = 0;
+    // counted-cell tiles of the last dsx_flood call
+    dsx::TileMask *d_masks = nullptr;
+    uint32_t mask_cap = 0;
+    std::vector<dsx::TileMask> masks;
 };
 
 static ClimState *clim_state(dsx_ctx *c);
@@ -124,6 +128,8 @@ static int clim_alloc(dsx_ctx *c, ClimState *&S) {
     S->job_cap = 1u << 20;
     HIPCK(hipMalloc(&S->d_jobs, size_t(S->job_cap) * sizeof(dsx::FloodJob)));
     HIPCK(hipMalloc(&S->d_res, size_t(S->job_cap) * sizeof(dsx::FloodRes)));
+    S->mask_cap = 1u << 20;
+    HIPCK(hipMalloc(&S->d_masks, size_t(S->mask_cap) * sizeof(dsx::TileMask)));
     return DSX_OK;
 }
 
@@ -170,7 +176,7 @@ void dsx_ctx_free_ext(dsx_ctx *c) {
     for (void *p : {(void *)s->d_cuts, (void *)s->d_cell, (void *)s->d_clim, (void *)s->d_useeds, (void *)s->fb.tiles,
                     (void *)s->fb.ntiles, (void *)s->fb.hkeys, (void *)s->fb.lists[0], (void *)s->fb.lists[1],
                     (void *)s->fb.lists[2], (void *)s->fb.lists[3], (void *)s->fb.lcount, (void *)s->d_jobs,
-                    (void *)s->d_res, s->d_io})
+                    (void *)s->d_res, s->d_io, (void *)s->d_masks})
         if (p) (void)hipFree(p);
     delete s;
 }
@@ -301,10 +307,24 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     ntiles = std::min(ntiles, B.tile_cap);
     if (status == DSX_OK) {
         dsx::launch_flood_acc(c->st, S->d_jobs, B, ntiles, S->d_res);
+        HIPCK(hipMemsetAsync(B.lcount + 5, 0, 4, c->st));
+        dsx::launch_flood_export(c->st, B, ntiles, S->d_masks, B.lcount + 5, S->mask_cap);
         HIPCK(hipGetLastError());
         std::vector<dsx::FloodRes> res(n);
         HIPCK(hipMemcpyAsync(res.data(), S->d_res, n * sizeof(dsx::FloodRes), hipMemcpyDeviceToHost, c->st));
+        HIPCK(hipMemcpyAsync(h + 5, B.lcount + 5, 4, hipMemcpyDeviceToHost, c->st));
         HIPCK(hipStreamSynchronize(c->st));
+        const uint32_t nm = h[5];
+        if (nm > S->mask_cap) {
+            status = DSX_E_CAPACITY;
+            dsx::g_last_error = "flood mask export overflow";
+        }
+        S->masks.resize(std::min(nm, S->mask_cap));
+        if (!S->masks.empty()) {
+            HIPCK(hipMemcpyAsync(S->masks.data(), S->d_masks, S->masks.size() * sizeof(dsx::TileMask),
+                                 hipMemcpyDeviceToHost, c->st));
+            HIPCK(hipStreamSynchronize(c->st));
+        }
         for (uint64_t k = 0; k < n; k++)
             out[k] = dsx_area{res[k].cells, res[k].sampled, res[k].minx, res[k].maxx, res[k].minz, res[k].maxz,
                               res[k].edge, res[k].border};
@@ -316,6 +336,46 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     HIPCK(hipMemsetAsync(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8, c->st));
     HIPCK(hipStreamSynchronize(c->st));
     return status;
+}
+
+int dsx_flood_masks(dsx_ctx *c, dsx_tile_mask *out, uint64_t cap, uint64_t *n_out) {
+    if (!c || !n_out || (cap && !out)) return DSX_E_ARG;
+    ClimState *S = clim_state(c);
+    *n_out = S->masks.size();
+    const uint64_t w = std::min<uint64_t>(cap, S->masks.size());
+    static_assert(sizeof(dsx_tile_mask) == sizeof(dsx::TileMask), "mask layout");
+    if (w) std::memcpy(out, S->masks.data(), w * sizeof(dsx_tile_mask));
+    return S->masks.size() > cap ? DSX_E_CAPACITY : DSX_OK;
+}
+
+int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_t *sidx, const int32_t *xs,
+               const int32_t *zs, uint64_t n, int what, float cut, int8_t *out8, double *outd) {
+    const bool dbl = what == DSX_POINT_T || what == DSX_POINT_PROXYPROB;
+    if (!c || (n && (!sidx || !xs || !zs)) || (!dbl && (what < 0 || what > 3)) || (dbl ? (n && !outd) : (n && !out8)) ||
+        (nseeds && !seeds))
+        return DSX_E_ARG;
+    for (uint64_t k = 0; k < n; k++)
+        if (sidx[k] >= nseeds) return DSX_E_ARG;
+    if (n == 0) return DSX_OK;
+    HIPCK(hipSetDevice(c->device));
+    ClimState *S;
+    int rc = clim_alloc(c, S);
+    if (rc) return rc;
+    if ((rc = clim_seeds(c, S, seeds, uint32_t(nseeds)))) return rc;
+    if ((rc = io_reserve(c, S, n * 24))) return rc;
+    uint32_t *d_s = static_cast<uint32_t *>(S->d_io);
+    int32_t *d_x = reinterpret_cast<int32_t *>(d_s + n), *d_z = d_x + n;
+    double *d_d = reinterpret_cast<double *>(d_z + n + (n & 1));
+    int8_t *d_8 = reinterpret_cast<int8_t *>(d_d);
+    HIPCK(hipMemcpyAsync(d_s, sidx, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCK(hipMemcpyAsync(d_x, xs, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCK(hipMemcpyAsync(d_z, zs, n * 4, hipMemcpyHostToDevice, c->st));
+    dsx::launch_points(c->st, S->d_clim, S->grid, d_s, d_x, d_z, uint32_t(n), what, cut, d_8, d_d);
+    HIPCK(hipGetLastError());
+    if (dbl) HIPCK(hipMemcpyAsync(outd, d_d, n * 8, hipMemcpyDeviceToHost, c->st));
+    else HIPCK(hipMemcpyAsync(out8, d_8, n, hipMemcpyDeviceToHost, c->st));
+    HIPCK(hipStreamSynchronize(c->st));
+    return DSX_OK;
 }
 
 }  // extern "C"

@@ -91,7 +91,7 @@ template <int MODE> struct GateR { static constexpr int R = (MODE == 1) ? 7 : 4;
 
 template <int MODE>
 __device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K) {
-    grid_setup<MODE>(sp, L, double(K.w));
+    grid_setup<MODE>(sp, L, K.wd);
     float best = -1e9f;
     constexpr int NC = Shape<MODE>::G / 64;
     grid_blur_rows<MODE, GateR<MODE>::R>(sp, L, K.k, 0, Shape<MODE>::G - 1, [&](int, const float (&o)[NC]) {
@@ -109,7 +109,7 @@ __device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K)
 // gets every amplitude, indexed by block.
 template <int MODE>
 __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint32_t *sel, uint32_t n, GateKernel K,
-                                             float thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp,
+                                             double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp,
                                              uint32_t *count, uint32_t cap, float *all_amp) {
     __shared__ WaveLds L[2];
     __shared__ float red[2];
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
         amp += red[0];
         amp += red[1];
         if (all_amp) all_amp[item] = amp;
-        if (amp >= thr) {
+        if (double(amp) >= thr) {
             const uint32_t i = atomicAdd(count, 1u);
             if (i < cap) {
                 out_idx[i] = kmap ? kmap[pidx] : pidx;
@@ -144,7 +144,7 @@ void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64
     hipLaunchKernelGGL(k_init, dim3((2 * n + 63) / 64), dim3(64), 0, st, base, ks, seeds, n, out);
 }
 void launch_amp(hipStream_t st, int mode, const SideParams *sides, const uint32_t *sel, uint32_t n, const GateKernel &K,
-                float thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count, uint32_t cap,
+                double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count, uint32_t cap,
                 float *all_amp) {
     if (n == 0) return;
     if (mode == 0)

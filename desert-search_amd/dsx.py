@@ -52,6 +52,25 @@ AREA = np.dtype([("cells", "<u8"), ("sampled", "<u8"), ("minx", "<i4"), ("maxx",
 PRED_DESERT_STATE, PRED_HOT, PRED_PROXY, PRED_DESERT = 0, 1, 2, 3
 
 
+class SearchParams(ctypes.Structure):
+    """dsx_search_params = the reference's struct Params (dsearch_desert.cpp:50-78), same defaults."""
+    _fields_ = [(n, ctypes.c_double) for n in ("max_phase", "amp", "amp_margin", "sigma", "sigmac", "wc", "we", "t", "ta")] + \
+        [("max_sites", ctypes.c_int32), ("pcut", ctypes.c_double), ("hot_min", ctypes.c_double), ("multi", ctypes.c_int32),
+         ("exact_n", ctypes.c_int32), ("exact_min", ctypes.c_double), ("hot_pre", ctypes.c_double),
+         ("csv_min", ctypes.c_double), ("nms", ctypes.c_int32), ("refine_mb", ctypes.c_double),
+         ("verify_mb", ctypes.c_double)] + \
+        [(n, ctypes.c_int32) for n in ("study", "gpu", "gpucheck", "gscan", "gscancheck", "gate")] + \
+        [(n, ctypes.c_double) for n in ("gate_phase", "gate_amp", "gate_pthr")] + [("from_stdin", ctypes.c_int32)]
+
+
+class SearchCounts(ctypes.Structure):
+    _fields_ = [("done", ctypes.c_uint64), ("passed", ctypes.c_uint64), ("scanned", ctypes.c_uint64),
+                ("sites", ctypes.c_uint64)]
+
+
+SEARCH_ENUM, SEARCH_GATE, SEARCH_LIST = 0, 1, 2
+
+
 def flood_jobs(rows):
     """rows of (seed, kind, cut, qx, qz, step, half, conn8) -> FLOOD_JOB array"""
     a = np.zeros(len(rows), FLOOD_JOB)
@@ -99,8 +118,13 @@ def load(path=LIB_PATH):
     lib.dsx_predicates.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
     lib.dsx_flood.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+    lib.dsx_search_defaults.argtypes = [ctypes.POINTER(SearchParams)]
+    lib.dsx_search_defaults.restype = None
+    lib.dsx_search.argtypes = [ctypes.c_void_p, ctypes.POINTER(SearchParams), ctypes.c_int, ctypes.c_uint64,
+                               ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_char_p,
+                               ctypes.POINTER(SearchCounts)]
     for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile", "dsx_scan", "dsx_tables",
-              "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood"):
+              "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood", "dsx_search"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -214,3 +238,23 @@ class Engine:
         out = np.zeros(len(j), AREA)
         self._check(self.lib.dsx_flood(self.h, j.ctypes.data, len(j), out.ctypes.data), "dsx_flood")
         return out
+
+    def search_params(self, **kw):
+        p = SearchParams()
+        self.lib.dsx_search_defaults(ctypes.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+    def search(self, prefix, mode, start=0, count=0, seeds=None, params=None):
+        """Run the whole pipeline; writes PREFIX.csv / PREFIX.seeds.csv (with headers) and returns the counts."""
+        p = params or self.search_params()
+        with open(prefix + ".csv", "w") as f:
+            f.write("seed,fyA,fyB,maxta,maxtb,rank,x,z,score,t0,c0,e0,proxy,s8,s4,edge,hot\n")
+        with open(prefix + ".seeds.csv", "w") as f:
+            f.write("seed,fyA,fyB,maxta,maxtb,candidates,sites,best\n")
+        s = np.ascontiguousarray(seeds if seeds is not None else [], dtype=np.uint64)
+        c = SearchCounts()
+        self._check(self.lib.dsx_search(self.h, ctypes.byref(p), mode, start, count, s.ctypes.data if len(s) else None,
+                                        len(s), prefix.encode(), ctypes.byref(c)), "dsx_search")
+        return (c.done, c.passed, c.scanned, c.sites)

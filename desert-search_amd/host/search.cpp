@@ -1,0 +1,633 @@
+// The per-seed search of dsearch_desert.cpp:173-557 on top of the C ABI (include/dsx.h).
+//
+// Each seed runs as a C++20 coroutine that follows the reference worker's control flow statement by
+// statement: sites -> hot zones -> proxy floods -> exact floods -> rows. Whenever it needs the GPU
+// (points or floods) it co_awaits a request; the scheduler collects the requests of every seed in
+// flight and executes them as one dsx_points / dsx_flood batch, then resumes the seeds. Where the
+// reference's later steps depend only on coordinates (the hottest point of every site, the step-256
+// pre-flood of every hot point, the proxy test of every start), the coroutine requests them all up
+// front in one batch, so a seed needs only a handful of scheduler rounds. Order-dependent host state
+// (hotcovered, covered, seen, std::sort of sites and proxies) is kept exactly as in the reference.
+#include "../../include/dsx.h"
+#include "../csrc/dsx_rng.h"
+#include <algorithm>
+#include <cmath>
+#include <coroutine>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <utility>
+#include <vector>
+
+namespace {
+
+struct Site { float s, t, c, e; int x, z; };
+struct Eval {
+    size_t rank;
+    int x, z;
+    double proxy, s8 = -1, s4 = 0;
+    bool edge = false;
+    double hot = 0;
+};
+
+// ------------------------------------------------------------------ coroutine plumbing
+struct Task {
+    struct promise_type {
+        Task get_return_object() { return Task{std::coroutine_handle<promise_type>::from_promise(*this)}; }
+        std::suspend_never initial_suspend() noexcept { return {}; }
+        std::suspend_always final_suspend() noexcept { return {}; }
+        void return_void() {}
+        void unhandled_exception() { std::terminate(); }
+    };
+    std::coroutine_handle<promise_type> h;
+};
+
+struct FloodOut {
+    dsx_area a{};
+    std::vector<std::pair<int, int>> cells;   // counted window cells (i, j), ordered by j then i
+};
+
+struct Sched;
+
+struct PointsReq {
+    Sched *s;
+    uint64_t seed;
+    int what;
+    float cut;
+    std::vector<int32_t> xs, zs;
+    std::vector<double> vd;
+    std::vector<int8_t> v8;
+    std::coroutine_handle<> h;
+    bool await_ready() const noexcept { return xs.empty(); }
+    void await_suspend(std::coroutine_handle<> hh);
+    void await_resume() const noexcept {}
+};
+struct FloodReq {
+    Sched *s;
+    std::vector<dsx_flood_job> jobs;
+    bool want_cells;
+    std::vector<FloodOut> out;
+    std::coroutine_handle<> h;
+    bool await_ready() const noexcept { return jobs.empty(); }
+    void await_suspend(std::coroutine_handle<> hh);
+    void await_resume() const noexcept {}
+};
+
+struct Sched {
+    dsx_ctx *ctx;
+    std::vector<PointsReq *> points;
+    std::vector<FloodReq *> floods;
+    int err = DSX_OK;
+    std::vector<dsx_tile_mask> masks;
+
+    int run_points(std::vector<PointsReq *> &rs) {
+        // group by (what, cut); each group one dsx_points call over many seeds
+        std::map<std::pair<int, float>, std::vector<PointsReq *>> groups;
+        for (auto *r : rs) groups[{r->what, r->cut}].push_back(r);
+        for (auto &[key, g] : groups) {
+            const bool dbl = key.first == DSX_POINT_T || key.first == DSX_POINT_PROXYPROB;
+            size_t i0 = 0;
+            while (i0 < g.size()) {
+                std::vector<uint64_t> seeds;
+                std::unordered_map<uint64_t, uint32_t> slot;
+                std::vector<uint32_t> sidx;
+                std::vector<int32_t> xs, zs;
+                size_t i1 = i0;
+                for (; i1 < g.size(); i1++) {
+                    PointsReq *r = g[i1];
+                    if (!slot.count(r->seed)) {
+                        if (seeds.size() >= 4000) break;
+                        slot.emplace(r->seed, uint32_t(seeds.size()));
+                        seeds.push_back(r->seed);
+                    }
+                    const uint32_t sl = slot[r->seed];
+                    sidx.insert(sidx.end(), r->xs.size(), sl);
+                    xs.insert(xs.end(), r->xs.begin(), r->xs.end());
+                    zs.insert(zs.end(), r->zs.begin(), r->zs.end());
+                }
+                std::vector<double> vd(dbl ? xs.size() : 0);
+                std::vector<int8_t> v8(dbl ? 0 : xs.size());
+                const int rc = dsx_points(ctx, seeds.data(), seeds.size(), sidx.data(), xs.data(), zs.data(), xs.size(),
+                                          key.first, key.second, v8.data(), vd.data());
+                if (rc) return rc;
+                size_t off = 0;
+                for (size_t i = i0; i < i1; i++) {
+                    PointsReq *r = g[i];
+                    const size_t m = r->xs.size();
+                    if (dbl) r->vd.assign(vd.begin() + off, vd.begin() + off + m);
+                    else r->v8.assign(v8.begin() + off, v8.begin() + off + m);
+                    off += m;
+                }
+                i0 = i1;
+            }
+        }
+        return DSX_OK;
+    }
+
+    // One dsx_flood call over jobs [lo, hi) of the flattened list; split in halves on capacity errors.
+    int flood_range(std::vector<dsx_flood_job> &jobs, std::vector<std::pair<FloodReq *, size_t>> &who, size_t lo,
+                    size_t hi, bool cells) {
+        if (lo >= hi) return DSX_OK;
+        std::vector<dsx_area> res(hi - lo);
+        int rc = dsx_flood(ctx, jobs.data() + lo, hi - lo, res.data());
+        if (rc == DSX_E_CAPACITY && hi - lo > 1) {
+            const size_t mid = lo + (hi - lo) / 2;
+            if ((rc = flood_range(jobs, who, lo, mid, cells))) return rc;
+            return flood_range(jobs, who, mid, hi, cells);
+        }
+        if (rc) return rc;
+        for (size_t k = lo; k < hi; k++) who[k].first->out[who[k].second].a = res[k - lo];
+        if (cells) {
+            uint64_t nm = 0;
+            dsx_flood_masks(ctx, nullptr, 0, &nm);
+            masks.resize(nm);
+            if (nm && (rc = dsx_flood_masks(ctx, masks.data(), nm, &nm))) return rc;
+            for (const dsx_tile_mask &m : masks) {
+                auto [r, k] = who[lo + m.job];
+                if (!r->want_cells) continue;
+                auto &cl = r->out[k].cells;
+                for (int row = 0; row < 32; row++)
+                    for (uint32_t b = m.rows[row]; b; b &= b - 1) cl.push_back({m.tx * 32 + __builtin_ctz(b), m.tz * 32 + row});
+            }
+            for (size_t k = lo; k < hi; k++) {
+                auto [r, q] = who[k];
+                if (!r->want_cells) continue;
+                auto &cl = r->out[q].cells;
+                std::sort(cl.begin(), cl.end(), [](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+                    return a.second != b.second ? a.second < b.second : a.first < b.first;
+                });
+            }
+        }
+        return DSX_OK;
+    }
+
+    int run_floods(std::vector<FloodReq *> &rs) {
+        std::vector<dsx_flood_job> jobs;
+        std::vector<std::pair<FloodReq *, size_t>> who;
+        bool cells = false;
+        for (auto *r : rs) {
+            r->out.assign(r->jobs.size(), FloodOut{});
+            cells |= r->want_cells;
+            for (size_t k = 0; k < r->jobs.size(); k++) {
+                jobs.push_back(r->jobs[k]);
+                who.push_back({r, k});
+            }
+        }
+        // chunks: at most 4000 distinct seeds and 2^18 floods per call
+        size_t lo = 0;
+        while (lo < jobs.size()) {
+            std::unordered_set<uint64_t> seen;
+            size_t hi = lo;
+            while (hi < jobs.size() && hi - lo < (1u << 18)) {
+                if (!seen.count(jobs[hi].seed)) {
+                    if (seen.size() >= 4000) break;
+                    seen.insert(jobs[hi].seed);
+                }
+                hi++;
+            }
+            const int rc = flood_range(jobs, who, lo, hi, cells);
+            if (rc) return rc;
+            lo = hi;
+        }
+        return DSX_OK;
+    }
+
+    // Execute all pending requests, then resume their coroutines. Returns false when nothing was
+    // pending or a request failed (the caller then destroys the suspended coroutines).
+    bool step() {
+        if (err || (points.empty() && floods.empty())) return false;
+        std::vector<PointsReq *> ps;
+        std::vector<FloodReq *> fs;
+        ps.swap(points);
+        fs.swap(floods);
+        if (!ps.empty()) err = run_points(ps);
+        if (!err && !fs.empty()) err = run_floods(fs);
+        if (err) return false;
+        for (auto *r : ps) r->h.resume();
+        for (auto *r : fs) r->h.resume();
+        return true;
+    }
+};
+void PointsReq::await_suspend(std::coroutine_handle<> hh) {
+    h = hh;
+    s->points.push_back(this);
+}
+void FloodReq::await_suspend(std::coroutine_handle<> hh) {
+    h = hh;
+    s->floods.push_back(this);
+}
+
+// ------------------------------------------------------------------ per-seed work
+struct SeedOut {
+    uint64_t seed = 0;
+    double fy[2] = {0, 0};
+    float maxta = 0, maxtb = 0;
+    bool scanned = false;
+    size_t ncand = 0;
+    std::vector<Site> sites;
+    std::vector<Eval> ev;
+    double best = 0;
+    std::vector<std::string> lines;   // stderr lines (CANDIDATE)
+    std::vector<double> bests;        // per exact flood value v, in order (for the global "best" lines)
+    std::vector<std::pair<int, int>> best_xy;
+    std::vector<double> best_proxy;
+    std::vector<int> best_step;
+    bool done = false;
+};
+
+inline uint64_t hotkey(int x, int z) {
+    return (uint64_t(uint32_t((x + 8000000) >> 7)) << 32) | uint32_t((z + 8000000) >> 7);
+}
+inline uint64_t cellkey(int x, int z) {
+    return (uint64_t(uint32_t((x + 8000000) >> 5)) << 32) | uint32_t((z + 8000000) >> 5);
+}
+inline dsx_flood_job fjob(uint64_t seed, int kind, float cut, int qx, int qz, int step, int half, int conn8) {
+    dsx_flood_job j;
+    std::memset(&j, 0, sizeof j);
+    j.seed = seed;
+    j.kind = kind;
+    j.cut = cut;
+    j.qx = qx;
+    j.qz = qz;
+    j.step = step;
+    j.half = half;
+    j.conn8 = conn8;
+    return j;
+}
+
+// dsearch_desert.cpp:309-533 for one seed whose sites are known.
+Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
+    const uint64_t seed = O.seed;
+    const float pcut = float(P.pcut);
+    const auto &sites = O.sites;
+    const size_t ns = sites.size();
+    std::vector<Eval> &ev = O.ev;
+    std::vector<std::pair<int, int>> seen;
+    std::unordered_set<uint64_t> covered, hotcovered;
+
+    // ---- hottest of the 7x7 T samples of every site (dsearch_desert.cpp:328-332), one batch
+    std::vector<int> hx(ns), hz(ns);
+    std::vector<double> bt(ns, -9);
+    // proxy-probability search of every site for the hot_min <= 0, multi <= 1 mode (:374-378)
+    std::vector<int> px_x(ns), px_z(ns);
+    std::vector<float> px_b(ns, -1);
+    if (P.hot_min > 0) {
+        PointsReq q{&S, seed, DSX_POINT_T, 0.f, {}, {}, {}, {}, {}};
+        for (size_t r = 0; r < ns; r++)
+            for (int dz = -3; dz <= 3; dz++)
+                for (int dx = -3; dx <= 3; dx++) {
+                    q.xs.push_back(sites[r].x + dx * 256);
+                    q.zs.push_back(sites[r].z + dz * 256);
+                }
+        co_await q;
+        size_t k = 0;
+        for (size_t r = 0; r < ns; r++) {
+            hx[r] = sites[r].x;
+            hz[r] = sites[r].z;
+            for (int dz = -3; dz <= 3; dz++)
+                for (int dx = -3; dx <= 3; dx++, k++)
+                    if (q.vd[k] > bt[r]) {
+                        bt[r] = q.vd[k];
+                        hx[r] = sites[r].x + dx * 256;
+                        hz[r] = sites[r].z + dz * 256;
+                    }
+        }
+    } else if (P.multi <= 1) {
+        PointsReq q{&S, seed, DSX_POINT_PROXYPROB, pcut, {}, {}, {}, {}, {}};
+        for (size_t r = 0; r < ns; r++)
+            for (int dz = -4; dz <= 4; dz++)
+                for (int dx = -4; dx <= 4; dx++) {
+                    q.xs.push_back(sites[r].x + dx * 64);
+                    q.zs.push_back(sites[r].z + dz * 64);
+                }
+        co_await q;
+        size_t k = 0;
+        for (size_t r = 0; r < ns; r++) {
+            px_x[r] = sites[r].x;
+            px_z[r] = sites[r].z;
+            for (int dz = -4; dz <= 4; dz++)
+                for (int dx = -4; dx <= 4; dx++, k++) {
+                    const float pr = float(q.vd[k]);
+                    if (pr > px_b[r]) {
+                        px_b[r] = pr;
+                        px_x[r] = sites[r].x + dx * 64;
+                        px_z[r] = sites[r].z + dz * 64;
+                    }
+                }
+        }
+    }
+    // ---- step-256 pre-floods of every distinct hot point (dsearch_desert.cpp:337), one batch
+    std::map<std::pair<int, int>, size_t> pre_idx;
+    FloodReq pre{&S, {}, true, {}, {}};
+    if (P.hot_min > 0 && P.hot_pre > 0) {
+        for (size_t r = 0; r < ns; r++) {
+            if (bt[r] < 0.55) continue;
+            auto key = std::make_pair(hx[r], hz[r]);
+            if (pre_idx.count(key)) continue;
+            pre_idx[key] = pre.jobs.size();
+            pre.jobs.push_back(fjob(seed, DSX_PRED_HOT, 0.f, hx[r], hz[r], 256, 32000 / 256, 0));
+        }
+        co_await pre;
+    }
+    std::map<std::pair<int, int>, FloodOut> h128;   // step-128 floods done so far, by hot point
+
+    for (size_t rank = 0; rank < ns; rank++) {
+        const Site &st = sites[rank];
+        std::vector<std::pair<int, int>> starts;
+        double hot_area = 0;
+        if (P.hot_min > 0) {
+            if (bt[rank] < 0.55 || hotcovered.count(hotkey(hx[rank], hz[rank]))) continue;
+            const int x0 = hx[rank], z0 = hz[rank];
+            if (P.hot_pre > 0) {
+                const FloodOut &h2 = pre.out[pre_idx[{x0, z0}]];
+                if (h2.a.cells * 256.0 * 256 * 16 / 1e6 < P.hot_pre * P.hot_min) {
+                    const int half = 32000 / 256;
+                    for (auto [i, j] : h2.cells) {
+                        const int x = x0 + (i - half) * 256, z = z0 + (j - half) * 256;
+                        for (int d = 0; d < 4; d++) hotcovered.insert(hotkey(x + (d & 1) * 128, z + (d >> 1) * 128));
+                    }
+                    continue;
+                }
+            }
+            auto it = h128.find({x0, z0});
+            if (it == h128.end()) {
+                FloodReq f{&S, {fjob(seed, DSX_PRED_HOT, 0.f, x0, z0, 128, 32000 / 128, 0)}, true, {}, {}};
+                co_await f;
+                it = h128.emplace(std::make_pair(x0, z0), std::move(f.out[0])).first;
+            }
+            const FloodOut &hzn = it->second;
+            hot_area = hzn.a.cells * 128.0 * 128 * 16 / 1e6;
+            const int half = 32000 / 128;
+            for (auto [i, j] : hzn.cells) {
+                const int x = x0 + (i - half) * 128, z = z0 + (j - half) * 128;
+                hotcovered.insert(hotkey(x, z));
+                if (hot_area >= P.hot_min && ((i - half) & 3) == 0 && ((j - half) & 3) == 0) starts.push_back({x, z});
+            }
+            if (hot_area < P.hot_min) continue;
+        } else if (P.multi <= 1) {
+            if (px_b[rank] >= pcut) starts.push_back({px_x[rank], px_z[rank]});
+        } else {
+            for (int dz = -P.multi; dz <= P.multi; dz++)
+                for (int dx = -P.multi; dx <= P.multi; dx++) starts.push_back({st.x + dx * 512, st.z + dz * 512});
+        }
+        if (starts.empty()) continue;
+        // ---- proxy floods (dsearch_desert.cpp:384-400): PX of every start in one batch, then the floods
+        PointsReq pxq{&S, seed, DSX_PRED_PROXY, pcut, {}, {}, {}, {}, {}};
+        for (auto [bx, bz] : starts) {
+            pxq.xs.push_back(bx);
+            pxq.zs.push_back(bz);
+        }
+        co_await pxq;
+        for (size_t k = 0; k < starts.size(); k++) {
+            const int bx = starts[k].first, bz = starts[k].second;
+            if (pxq.v8[k] != 2) continue;
+            if (covered.count(cellkey(bx, bz))) continue;
+            FloodReq f{&S, {fjob(seed, DSX_PRED_PROXY, pcut, bx, bz, 32, 16000 / 32, 1)}, true, {}, {}};
+            co_await f;
+            const FloodOut &a = f.out[0];
+            const int half = 16000 / 32;
+            for (auto [i, j] : a.cells) covered.insert(cellkey(bx + (i - half) * 32, bz + (j - half) * 32));
+            const std::pair<int, int> sig(a.a.minx, a.a.minz);
+            if (std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;
+            seen.push_back(sig);
+            Eval e{rank, bx, bz, a.a.cells * 32.0 * 32 * 16 / 1e6};
+            e.hot = hot_area;
+            ev.push_back(e);
+        }
+    }
+    // ---- exact floods of the best proxies (dsearch_desert.cpp:494-533)
+    std::sort(ev.begin(), ev.end(), [](const Eval &a, const Eval &b) { return a.proxy > b.proxy; });
+    double seed_best = 0;
+    for (size_t k2 = 0; !P.study && k2 < ev.size() && int(k2) < P.exact_n && ev[k2].proxy >= P.exact_min; k2++) {
+        Eval &e = ev[k2];
+        const int sx = e.x, sz = e.z;
+        PointsReq ring{&S, seed, DSX_PRED_DESERT, 0.f, {}, {}, {}, {}, {}};
+        for (int r = 0; r <= 8; r++)
+            for (int dz = -r; dz <= r; dz++)
+                for (int dx = -r; dx <= r; dx++) {
+                    if (std::max(std::abs(dx), std::abs(dz)) != r) continue;
+                    ring.xs.push_back(sx + dx * 16);
+                    ring.zs.push_back(sz + dz * 16);
+                }
+        co_await ring;
+        int bx = sx, bz = sz;
+        bool found = false;
+        for (size_t k = 0; k < ring.xs.size() && !found; k++)
+            if (ring.v8[k] == 2) {
+                bx = ring.xs[k];
+                bz = ring.zs[k];
+                found = true;
+            }
+        if (!found) {
+            e.s8 = 0;
+            continue;
+        }
+        FloodReq f8{&S, {fjob(seed, DSX_PRED_DESERT_STATE, 0.f, bx, bz, 8, 16000 / 8, 1)}, false, {}, {}};
+        co_await f8;
+        e.x = bx;
+        e.z = bz;
+        e.s8 = f8.out[0].a.cells * 64.0 * 16 / 1e6;
+        e.edge = f8.out[0].a.edge != 0;
+        if (e.s8 >= P.refine_mb) {
+            FloodReq f4{&S, {fjob(seed, DSX_PRED_DESERT_STATE, 0.f, bx, bz, 4, 16000 / 4, 0)}, false, {}, {}};
+            co_await f4;
+            e.s4 = f4.out[0].a.cells * 16.0 * 16 / 1e6;
+            e.edge = e.edge || f4.out[0].a.edge != 0;
+        }
+        const double v = e.s4 > 0 ? e.s4 : e.s8;
+        seed_best = std::max(seed_best, v);
+        if (e.s4 >= P.verify_mb) {
+            char line[256];
+            std::snprintf(line, sizeof line, "CANDIDATE step4=%.2fM seed=%lld x=%d z=%d -> verify with exact flood (Y=256)",
+                          e.s4, (long long)seed, bx * 4 + 2, bz * 4 + 2);
+            O.lines.push_back(line);
+        }
+        O.bests.push_back(v);
+        O.best_xy.push_back({bx * 4 + 2, bz * 4 + 2});
+        O.best_proxy.push_back(e.proxy);
+        O.best_step.push_back(e.s4 > 0 ? 4 : 8);
+    }
+    O.best = seed_best;
+    O.done = true;
+}
+
+// ------------------------------------------------------------------ seeds -> sites (S1-S2 on the GPU)
+// dsearch_desert.cpp:227, 274-277, 301-308
+void make_sites(const std::vector<dsx_raw_cand> &raw, const dsx_search_params &P, SeedOut &O) {
+    std::vector<Site> cand;
+    std::unordered_map<uint64_t, uint32_t> bucket;
+    for (const dsx_raw_cand &r : raw) {
+        const uint64_t key = (uint64_t(uint32_t((r.x + 8000000) / P.nms)) << 32) | uint32_t((r.z + 8000000) / P.nms);
+        auto [it, fresh] = bucket.try_emplace(key, uint32_t(cand.size()));
+        if (fresh) cand.push_back({r.s, r.t, r.c, r.e, r.x, r.z});
+        else if (r.s > cand[it->second].s) cand[it->second] = {r.s, r.t, r.c, r.e, r.x, r.z};
+    }
+    O.ncand = cand.size();
+    std::sort(cand.begin(), cand.end(), [](const Site &a, const Site &b) { return a.s > b.s; });
+    for (auto &c : cand) {
+        bool near = false;
+        for (auto &q : O.sites)
+            if (std::abs(q.x - c.x) < P.nms && std::abs(q.z - c.z) < P.nms) { near = true; break; }
+        if (!near) O.sites.push_back(c);
+        if (int(O.sites.size()) >= P.max_sites) break;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void dsx_search_defaults(dsx_search_params *p) {
+    dsx_search_params d{};
+    d.max_phase = 0.10; d.amp = 0.70; d.amp_margin = 0.01; d.sigma = 2.0; d.sigmac = 2.0;
+    d.wc = 0.25; d.we = 0.41; d.t = 0.55; d.ta = 0.25; d.max_sites = 300; d.pcut = 0.35; d.hot_min = 180;
+    d.multi = 1; d.exact_n = 2; d.exact_min = 75; d.hot_pre = 0.8; d.csv_min = 50; d.nms = 5000;
+    d.refine_mb = 70; d.verify_mb = 80; d.study = 0; d.gpu = 0; d.gpucheck = 0; d.gscan = 0; d.gscancheck = 0;
+    d.gate = 0; d.gate_phase = 0.06; d.gate_amp = 0.70; d.gate_pthr = 0.645; d.from_stdin = 0;
+    *p = d;
+}
+
+int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t start, uint64_t count, const uint64_t *seeds,
+               uint64_t nseeds, const char *prefix, dsx_search_counts *counts) {
+    if (!ctx || !pp || !prefix || !counts || mode < 0 || mode > 2 || (mode == DSX_SEARCH_LIST && nseeds && !seeds))
+        return DSX_E_ARG;
+    const dsx_search_params P = *pp;
+    if (P.study || P.nms <= 0 || P.max_sites < 0) return DSX_E_ARG;
+    const std::string pre(prefix);
+    FILE *out = std::fopen((pre + ".csv").c_str(), "a");
+    FILE *sout = std::fopen((pre + ".seeds.csv").c_str(), "a");
+    if (!out || !sout) {
+        if (out) std::fclose(out);
+        if (sout) std::fclose(sout);
+        return DSX_E_ARG;
+    }
+    int rc = DSX_OK;
+    static double global_best = 0;
+    const dsx_scan_params sp{float(P.ta), float(P.t), float(P.wc), float(P.we), P.sigma, P.sigmac};
+    // seed source in chunks: (seed list, CPU-default gate applies?)
+    const uint64_t CHUNK = 1ull << 24;
+    uint64_t pos = 0;
+    const uint64_t total = mode == DSX_SEARCH_LIST ? nseeds : count;
+    std::vector<uint64_t> batch;
+    while (pos < total && rc == DSX_OK) {
+        const uint64_t m = std::min(CHUNK, total - pos);
+        batch.clear();
+        if (mode == DSX_SEARCH_LIST) {
+            batch.assign(seeds + pos, seeds + pos + m);
+            counts->done += m;
+            counts->passed += m;
+        } else if (mode == DSX_SEARCH_GATE) {
+            dsx_gate_params gp{float(P.gate_phase), float(P.gate_amp), float(P.gate_pthr), 0.01f, 1.0f, 1};
+            uint64_t n = 0;
+            dsx_gate_counts gc;
+            batch.resize(1 << 16);
+            rc = dsx_gate(ctx, start + pos, m, &gp, batch.data(), nullptr, nullptr, batch.size(), &n, &gc);
+            if (rc == DSX_E_CAPACITY) {
+                batch.resize(n);
+                rc = dsx_gate(ctx, start + pos, m, &gp, batch.data(), nullptr, nullptr, batch.size(), &n, &gc);
+            }
+            if (rc) break;
+            batch.resize(n);
+            counts->done += n;
+            counts->passed += n;
+        } else {
+            uint64_t n = 0, nph = 0;
+            batch.resize(1 << 18);
+            rc = dsx_gate_cpu(ctx, start + pos, m, P.max_phase, P.amp, P.amp_margin, batch.data(), nullptr, nullptr,
+                              batch.size(), &n, &nph);
+            if (rc == DSX_E_CAPACITY) {
+                batch.resize(n);
+                rc = dsx_gate_cpu(ctx, start + pos, m, P.max_phase, P.amp, P.amp_margin, batch.data(), nullptr, nullptr,
+                                  batch.size(), &n, &nph);
+            }
+            if (rc) break;
+            batch.resize(n);
+            counts->done += m;
+            counts->passed += nph;
+        }
+        pos += m;
+        // ---- tables + scan, in sub-batches of seeds processed together downstream
+        const size_t SUB = 512;
+        for (size_t b0 = 0; b0 < batch.size() && rc == DSX_OK; b0 += SUB) {
+            const size_t nb = std::min(SUB, batch.size() - b0);
+            std::vector<float> mta(nb), mtb(nb);
+            std::vector<uint64_t> ncand(nb);
+            uint64_t cap = 1 << 12;
+            std::vector<dsx_raw_cand> raw(nb * cap);
+            rc = dsx_scan(ctx, batch.data() + b0, nb, &sp, mta.data(), mtb.data(), raw.data(), cap, ncand.data());
+            if (rc == DSX_E_CAPACITY) {
+                cap = *std::max_element(ncand.begin(), ncand.end());
+                raw.assign(nb * cap, dsx_raw_cand{});
+                rc = dsx_scan(ctx, batch.data() + b0, nb, &sp, mta.data(), mtb.data(), raw.data(), cap, ncand.data());
+            }
+            if (rc) break;
+            std::vector<std::unique_ptr<SeedOut>> outs;
+            for (size_t k = 0; k < nb; k++) {
+                auto O = std::make_unique<SeedOut>();
+                O->seed = batch[b0 + k];
+                dsx::temperature_phase(O->seed, O->fy);   // fyA, fyB (dsearch_desert.cpp:80-94)
+                O->maxta = mta[k];
+                O->maxtb = mtb[k];
+                if (mode == DSX_SEARCH_ENUM && O->maxta + O->maxtb < P.amp) {
+                    outs.push_back(std::move(O));
+                    continue;
+                }
+                O->scanned = true;
+                std::vector<dsx_raw_cand> r(raw.begin() + k * cap, raw.begin() + k * cap + ncand[k]);
+                make_sites(r, P, *O);
+                outs.push_back(std::move(O));
+            }
+            // ---- hot zones, proxy floods, exact floods: all scanned seeds as coroutines
+            Sched S{ctx, {}, {}, DSX_OK, {}};
+            std::vector<Task> tasks;
+            for (auto &O : outs)
+                if (O->scanned) tasks.push_back(seed_task(S, P, *O));
+            while (S.step()) {
+            }
+            rc = S.err;
+            for (auto &t : tasks) t.h.destroy();
+            if (rc) break;
+            // ---- rows (dsearch_desert.cpp:251, 526-548)
+            for (auto &O : outs) {
+                char line[512];
+                if (!O->scanned) {
+                    std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,0,0,0\n", (long long)O->seed, O->fy[0], O->fy[1],
+                                  O->maxta, O->maxtb);
+                    std::fputs(line, sout);
+                    continue;
+                }
+                counts->scanned++;
+                counts->sites += O->sites.size();
+                for (auto &l : O->lines) std::fprintf(stderr, "%s\n", l.c_str());
+                for (size_t k = 0; k < O->bests.size(); k++)
+                    if (O->bests[k] > global_best) {
+                        global_best = O->bests[k];
+                        std::fprintf(stderr, "best %.2fM (step %d) seed=%lld x=%d z=%d proxy=%.1fM\n", O->bests[k],
+                                     O->best_step[k], (long long)O->seed, O->best_xy[k].first, O->best_xy[k].second,
+                                     O->best_proxy[k]);
+                    }
+                for (auto &e : O->ev) {
+                    if (e.proxy < P.csv_min && e.s8 <= 0 && e.s4 <= 0) continue;
+                    const Site &st = O->sites[e.rank];
+                    std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,%zu,%d,%d,%.4f,%.4f,%.4f,%.4f,%.3f,%.3f,%.3f,%d,%.1f\n",
+                                  (long long)O->seed, O->fy[0], O->fy[1], O->maxta, O->maxtb, e.rank, e.x * 4 + 2, e.z * 4 + 2,
+                                  st.s, st.t, st.c, st.e, e.proxy, e.s8, e.s4, int(e.edge), e.hot);
+                    std::fputs(line, out);
+                }
+                std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,%zu,%zu,%.3f\n", (long long)O->seed, O->fy[0],
+                              O->fy[1], O->maxta, O->maxtb, O->ncand, O->sites.size(), O->best);
+                std::fputs(line, sout);
+            }
+        }
+    }
+    std::fclose(out);
+    std::fclose(sout);
+    return rc;
+}
+
+}  // extern "C"

@@ -64,6 +64,14 @@ typedef struct {
 int dsx_gate(dsx_ctx *ctx, uint64_t start, uint64_t count, const dsx_gate_params *p, uint64_t *seeds, float *amps,
              uint64_t *kidx, uint64_t cap, uint64_t *n_out, dsx_gate_counts *counts);
 
+/* The CPU-default gate of the reference's seed enumeration (dsearch_desert.cpp:202-223, gpu=0 or
+ * gate=0) on the GPU: seed = splitmix64(start+k), keep k if phase <= max_phase (double compare,
+ * dsearch_desert.cpp:206) and amplitude_sub A + B >= amp - margin (lowfield.hpp:60-102,
+ * dsearch_desert.cpp:217-219). Survivors sorted by k: seeds[i], est[i] (the float estimate A + B),
+ * kidx[i]; *n_phase = phase survivors (the `passed` counter). DSX_E_CAPACITY if more than cap. */
+int dsx_gate_cpu(dsx_ctx *ctx, uint64_t start, uint64_t count, double max_phase, double amp, double margin,
+                 uint64_t *seeds, float *est, uint64_t *kidx, uint64_t cap, uint64_t *n_out, uint64_t *n_phase);
+
 /* Pre-gate and full-gate amplitudes (A+B) of an explicit seed list, no thresholds applied.
  * Replaces the calibration mode of the reference's gate tool (tgate.cu:53-85, `calib=N`). */
 int dsx_gate_amps(dsx_ctx *ctx, const uint64_t *seeds, uint64_t n, const dsx_gate_params *p, float *pre, float *full);
@@ -120,6 +128,13 @@ int dsx_climate(dsx_ctx *ctx, uint64_t seed, const int32_t *xs, const int32_t *z
 int dsx_predicates(dsx_ctx *ctx, uint64_t seed, int kind, float cut, const int32_t *xs, const int32_t *zs, uint64_t n,
                    int8_t *out);
 
+/* Many seeds at once: point k belongs to seeds[sidx[k]] and sits at quart (xs[k], zs[k]).
+ * what = DSX_PRED_* gives the predicate in out8[k]; what = DSX_POINT_T gives T(x, 0, z) in outd[k]
+ * (the hot-zone search of dsearch_desert.cpp:328-332). */
+enum { DSX_POINT_T = 16, DSX_POINT_PROXYPROB = 17 };   /* 17: DesertProxy::prob (desert_proxy.hpp:25-30) in outd */
+int dsx_points(dsx_ctx *ctx, const uint64_t *seeds, uint64_t nseeds, const uint32_t *sidx, const int32_t *xs,
+               const int32_t *zs, uint64_t n, int what, float cut, int8_t *out8, double *outd);
+
 /* ---------------------------------------------------------------- bounded floods (S3-S4, area queries) */
 /* One flood: window of (2*half+1)^2 lattice points of spacing `step` quarts centred on (qx, qz). */
 typedef struct {
@@ -146,6 +161,50 @@ typedef struct {
  * depth-first flood (they do not depend on visiting order). DSX_E_CAPACITY if the batch needs more
  * tiles than the engine holds (split the batch). */
 int dsx_flood(dsx_ctx *ctx, const dsx_flood_job *jobs, uint64_t n, dsx_area *out);
+
+/* The counted cells (state 3 in dense_flood.hpp's window) of the floods of the last dsx_flood call, as
+ * 32x32 tile masks: window cell (i, j) = (tx*32 + c, tz*32 + r) is counted iff bit c of rows[r].
+ * Only tiles with at least one counted cell; any order. The hot-zone and proxy stages need them for
+ * their covered sets (dsearch_desert.cpp:342-346, 358-363, 388-395). */
+typedef struct {
+    uint32_t job;
+    int32_t tx, tz;
+    uint32_t rows[32];
+} dsx_tile_mask;
+int dsx_flood_masks(dsx_ctx *ctx, dsx_tile_mask *out, uint64_t cap, uint64_t *n_out);
+
+/* ---------------------------------------------------------------- the whole per-seed search */
+/* struct Params of the reference (dsearch_desert.cpp:50-78), same names and defaults. */
+typedef struct {
+    double max_phase, amp, amp_margin, sigma, sigmac, wc, we, t, ta;
+    int32_t max_sites;
+    double pcut, hot_min;
+    int32_t multi, exact_n;
+    double exact_min, hot_pre, csv_min;
+    int32_t nms;
+    double refine_mb, verify_mb;
+    int32_t study, gpu, gpucheck, gscan, gscancheck, gate;
+    double gate_phase, gate_amp, gate_pthr;
+    int32_t from_stdin;
+} dsx_search_params;
+void dsx_search_defaults(dsx_search_params *p);
+
+enum {
+    DSX_SEARCH_ENUM = 0, /* seeds splitmix64(start+k), k < count, CPU-default gate (phase, amplitude_sub, maxta+maxtb) */
+    DSX_SEARCH_GATE = 1, /* GPU gate over [start, start+count) (gate=1: gate_phase, gate_pthr, gate_amp) */
+    DSX_SEARCH_LIST = 2  /* explicit gated seeds (stdin=1) */
+};
+typedef struct {
+    uint64_t done, passed, scanned, sites;
+} dsx_search_counts;
+
+/* The per-seed pipeline of dsearch_desert.cpp:173-557 (gate -> tables/scan -> sites -> hot zones ->
+ * proxy floods -> exact floods) with every numeric stage on the GPU. Appends rows to PREFIX.csv and
+ * PREFIX.seeds.csv in the reference's formats (opened in append mode; the caller writes the headers),
+ * prints CANDIDATE / best lines to stderr as the reference does, and accumulates counts.
+ * Rows of one seed are contiguous; seeds appear in batch order. study=1 is not supported (DSX_E_ARG). */
+int dsx_search(dsx_ctx *ctx, const dsx_search_params *p, int mode, uint64_t start, uint64_t count, const uint64_t *seeds,
+               uint64_t nseeds, const char *prefix, dsx_search_counts *counts);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,76 @@
+"""The whole per-seed pipeline on the GPU (dsx_search, and the dsearch_desert command line) against
+the reference's own outputs: sorted CSV rows, seeds CSV and JSON counters, byte for byte."""
+import hashlib
+import json
+import os
+import subprocess
+import tempfile
+
+import pytest
+
+from conftest import GOLDEN, REPO, load_package
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(tag, prefix, counts=None):
+    gold = json.load(open(os.path.join(GOLDEN, tag + ".json")))
+    rows = sorted(open(prefix + ".csv").read().splitlines()[1:])
+    srows = sorted(open(prefix + ".seeds.csv").read().splitlines()[1:])
+    assert rows == open(os.path.join(GOLDEN, tag + ".csv")).read().splitlines()
+    assert len(srows) == gold["seeds_rows"]
+    assert hashlib.sha256("\n".join(srows).encode()).hexdigest() == gold["seeds_sha256"]
+    if counts is not None:
+        assert list(counts) == [gold["done"], gold["passed"], gold["scanned"], gold["sites"]]
+
+
+def test_config1_pipeline(engine):
+    """seeds mix(k), k < 200000, CPU-default gate semantics, production Params."""
+    dsx = load_package().dsx
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "c1")
+        counts = engine.search(pre, dsx.SEARCH_ENUM, 0, 200000)
+        _check("c1", pre, counts)
+
+
+def test_gated_list_pipeline(engine):
+    """stdin=1 semantics on 92 GPU-gated seeds (exact step-8 floods included)."""
+    dsx = load_package().dsx
+    seeds = [int(l.split()[0]) % (1 << 64) for l in open(os.path.join(GOLDEN, "gated_1e12_1m.txt"))]
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "rg1m")
+        counts = engine.search(pre, dsx.SEARCH_LIST, seeds=seeds)
+        _check("rg1m", pre, counts)
+
+
+def test_3m_pipeline(engine):
+    dsx = load_package().dsx
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "r3m")
+        counts = engine.search(pre, dsx.SEARCH_ENUM, 0, 3000000)
+        _check("r3m", pre, counts)
+
+
+def test_gate_mode_equals_gated_list(engine):
+    """gate=1 over [1e12, 1e12 + 2^20) gives the same rows as stdin=1 on the gate's survivors."""
+    dsx = load_package().dsx
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "g")
+        counts = engine.search(pre, dsx.SEARCH_GATE, 10 ** 12, 1 << 20)
+        _check("rg1m", pre, counts)
+
+
+def test_cli_config1(engine):
+    """The dsearch_desert command line: same outputs and JSON counters as the reference binary."""
+    exe = os.path.join(REPO, "desert-search_amd", "dsearch_desert")
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "cli")
+        r = subprocess.run([exe, "0", "200000", pre, "8"], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        assert "done: 200000 seeds (3891 passed)" in r.stderr
+        js = json.load(open(pre + ".json"))
+        _check("c1", pre, (js["done"], js["passed"], js["scanned"], js["sites"]))
+        r2 = subprocess.run([exe, "0", "10", pre, "8"], capture_output=True, text=True, timeout=60)
+        assert r2.returncode == 2 and "exist?" in r2.stderr
+        r3 = subprocess.run([exe, "0", "10", pre + "x", "8", "bogus=1"], capture_output=True, text=True, timeout=60)
+        assert r3.returncode == 2 and "unknown key bogus" in r3.stderr
