@@ -76,8 +76,15 @@ __device__ __forceinline__ double perlin_sample(const ClimOct &o, double x, doub
     return lerpd(tz, lerpd(ty, l00, l10), lerpd(ty, l01, l11));
 }
 
-// NormalNoise::sample: per-side sums, then (A + B) * factor.
-__device__ __forceinline__ double noise_sample(const ClimSeed &C, int ch, double x, double y, double z) {
+// The temperature octaves alone (slots 0-3), enough for the hot predicate.
+struct ClimT {
+    ClimOct oct[4];
+    double factor[6];
+};
+
+// NormalNoise::sample: per-side sums, then (A + B) * factor. CS: ClimSeed, or ClimT for channel T.
+template <class CS>
+__device__ __forceinline__ double noise_sample(const CS &C, int ch, double x, double y, double z) {
     const int n = clim_n(ch), b = clim_base(ch);
     double sum0 = 0, sum1 = 0;
     {
@@ -100,7 +107,8 @@ __device__ __forceinline__ double noise_sample(const ClimSeed &C, int ch, double
     return (sum0 + sum1) * C.factor[ch];
 }
 // DesertProxy::octaves: one running sum over both sides, octaves [0, hi).
-__device__ __forceinline__ double noise_octaves(const ClimSeed &C, int ch, double x, double z, int hi) {
+template <class CS>
+__device__ __forceinline__ double noise_octaves(const CS &C, int ch, double x, double z, int hi) {
     const int n = clim_n(ch), b = clim_base(ch);
     double s = 0;
     for (int side = 0; side < 2; side++) {
@@ -283,11 +291,13 @@ __device__ int desert_state(const ClimSeed &C, const SurfGrid &g, int x, int z) 
 }
 
 // dsearch_desert.cpp:334
-__device__ __forceinline__ int hot_state(const ClimSeed &C, int x, int z) {
+template <class CS>
+__device__ __forceinline__ int hot_state(const CS &C, int x, int z) {
     return noise_sample(C, DSX_CH_T, x, 0, z) >= 0.55 ? 2 : 0;
 }
 // desert_proxy.hpp:25-32
-__device__ __forceinline__ float proxy_prob(const ClimSeed &C, int x, int z) {
+template <class CS>
+__device__ __forceinline__ float proxy_prob(const CS &C, int x, int z) {
     if (noise_sample(C, DSX_CH_T, x, 0, z) < 0.55) return 0;
     const double c = noise_octaves(C, DSX_CH_C, x, z, 3), e = noise_octaves(C, DSX_CH_E, x, z, 2);
     int i = int(floor((c + 0.8) / 0.05)), j = int(floor((e + 1.0) / 0.05));
@@ -295,7 +305,8 @@ __device__ __forceinline__ float proxy_prob(const ClimSeed &C, int x, int z) {
     j = j < 0 ? 0 : (j > 39 ? 39 : j);
     return DSX_DPROB[i][j];
 }
-__device__ __forceinline__ int proxy_state(const ClimSeed &C, int x, int z, float cut) {
+template <class CS>
+__device__ __forceinline__ int proxy_state(const CS &C, int x, int z, float cut) {
     return proxy_prob(C, x, z) >= cut ? 2 : 0;
 }
 

@@ -1,0 +1,198 @@
+// Small-window floods in one wave with the whole window state in LDS (dense_flood.hpp:52-86).
+//
+// For windows up to 255 x 255 lattice points (the step-256 hot-zone pre-floods of
+// dsearch_desert.cpp:337, the most numerous floods of the pipeline) the tiled engine would evaluate
+// whole 32x32 tiles around a zone of a few dozen cells. Here one wave floods one window and evaluates
+// exactly the cells the reference touches:
+//   * touched and counted bitmaps (1 bit per cell each) and a ring of pushed cells live in LDS;
+//   * a step pops up to 64 pushed cells (one per lane), claims their untouched in-window neighbours
+//     with LDS atomicOr on the touched bitmap and lists the claimed cells; the listed cells are then
+//     evaluated 64 at a time (all lanes busy on the FP64 predicate), and passable ones are pushed;
+//   * the outputs (cells, sampled, bbox, edge, border) are order independent, so the visiting order
+//     (wave-parallel, ring order) gives exactly the reference's depth-first results.
+// A window whose ring would overflow is reported (res.border = -1) and handed to the tiled engine.
+#include "floods.h"
+#include <climits>
+#include <type_traits>
+
+namespace dsx {
+
+constexpr int LDS_N = 255;                                    // largest window edge
+constexpr int LDS_WORDS = (LDS_N * LDS_N + 31) / 32;           // 2033 words per bitmap
+constexpr int LDS_RING = 4096;                                 // pushed cells in flight
+
+template <class CS>
+__device__ __forceinline__ int lds_pred(int kind, const CS &C, const SurfGrid &g, int x, int z, float cut) {
+    if constexpr (std::is_same<CS, ClimT>::value) {
+        return hot_state(C, x, z);
+    } else {
+        return predicate(kind, C, g, x, z, cut);
+    }
+}
+
+template <class CS>
+__global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
+                                                  SurfGrid g, FloodRes *res, TileMask *masks, uint32_t *mcount,
+                                                  uint32_t mcap) {
+    __shared__ CS C;
+    __shared__ uint32_t touched[LDS_WORDS], counted[LDS_WORDS];
+    __shared__ uint32_t ring[LDS_RING];
+    __shared__ uint32_t evl[512];
+    __shared__ uint32_t qhead, qtail, nev, ovf;
+    const uint32_t b = blockIdx.x;
+    if (b >= n) return;
+    const uint32_t job = list[b];
+    const FloodJob J = jobs[job];
+    const int lane = threadIdx.x;
+    const int nw = 2 * J.half + 1, ncell = nw * nw;
+    {   // climate (only the octaves the predicate reads when CS is ClimT) and cleared bitmaps
+        const uint4 *src = reinterpret_cast<const uint4 *>(&cs[J.seed]);
+        uint4 *dst = reinterpret_cast<uint4 *>(&C);
+        constexpr int noct = sizeof(C.oct) / 16;
+        for (int k = lane; k < noct; k += 64) dst[k] = src[k];
+        if (lane < 6) C.factor[lane] = cs[J.seed].factor[lane];
+        const int words = (ncell + 31) / 32;
+        for (int k = lane; k < words; k += 64) {
+            touched[k] = 0;
+            counted[k] = 0;
+        }
+        if (lane == 0) {
+            qhead = 0;
+            qtail = 0;
+            nev = 0;
+            ovf = 0;
+        }
+    }
+    __syncthreads();
+    unsigned long long cells = 0, sampled = 0;
+    int minx = J.qx, maxx = J.qx, minz = J.qz, maxz = J.qz;
+    bool edge = false, border = false;
+    // the centre is touched first (dense_flood.hpp:80)
+    if (lane == 0) {
+        const int c = J.half * nw + J.half;
+        touched[c >> 5] |= 1u << (c & 31);
+        evl[0] = uint32_t(c);
+        nev = 1;
+    }
+    __syncthreads();
+    for (;;) {
+        // ---- evaluate the claimed cells, push the passable ones
+        const uint32_t ne = nev;
+        for (uint32_t e = lane; e < ne; e += 64) {
+            const int c = int(evl[e]), i = c % nw, j = c / nw;
+            const int x = J.qx + (i - J.half) * J.step, z = J.qz + (j - J.half) * J.step;
+            if (x < -WORLD_Q || x > WORLD_Q || z < -WORLD_Q || z > WORLD_Q) {
+                border = true;
+                continue;
+            }
+            sampled++;
+            const int v = lds_pred<CS>(J.kind, C, g, x, z, J.cut);
+            if (v == 0) continue;
+            if (v == 2) {
+                atomicOr(&counted[c >> 5], 1u << (c & 31));
+                cells++;
+                minx = min(minx, x);
+                maxx = max(maxx, x);
+                minz = min(minz, z);
+                maxz = max(maxz, z);
+            }
+            const uint32_t t = atomicAdd(&qtail, 1u);
+            if (t - qhead >= LDS_RING) ovf = 1;
+            else ring[t % LDS_RING] = uint32_t(c);
+        }
+        __syncthreads();
+        if (lane == 0) nev = 0;
+        const uint32_t h = qhead, tl = qtail;
+        __syncthreads();
+        if (ovf || h == tl) break;
+        // ---- expand up to 64 pushed cells: claim their untouched neighbours
+        const uint32_t np = min(tl - h, 64u);
+        if (uint32_t(lane) < np) {
+            const int c = int(ring[(h + lane) % LDS_RING]), i = c % nw, j = c / nw;
+            const int nn = J.conn8 ? 8 : 4;
+            for (int d = 0; d < nn; d++) {
+                const int di = d == 0 ? -1 : d == 1 ? 1 : d == 2 ? 0 : d == 3 ? 0 : d == 4 ? -1 : d == 5 ? 1 : d == 6 ? -1 : 1;
+                const int dj = d == 0 ? 0 : d == 1 ? 0 : d == 2 ? -1 : d == 3 ? 1 : d == 4 ? -1 : d == 5 ? -1 : d == 6 ? 1 : 1;
+                const int ni = i + di, nj = j + dj;
+                if (ni < 0 || nj < 0 || ni >= nw || nj >= nw) {
+                    edge = true;
+                    continue;
+                }
+                const int q = nj * nw + ni;
+                const uint32_t bit = 1u << (q & 31);
+                if (atomicOr(&touched[q >> 5], bit) & bit) continue;
+                evl[atomicAdd(&nev, 1u)] = uint32_t(q);
+            }
+        }
+        __syncthreads();
+        if (lane == 0) qhead = h + np;
+        __syncthreads();
+    }
+    // ---- reduce
+    for (int o = 32; o > 0; o >>= 1) {
+        cells += __shfl_xor(cells, o);
+        sampled += __shfl_xor(sampled, o);
+        minx = min(minx, __shfl_xor(minx, o));
+        maxx = max(maxx, __shfl_xor(maxx, o));
+        minz = min(minz, __shfl_xor(minz, o));
+        maxz = max(maxz, __shfl_xor(maxz, o));
+    }
+    const bool any_edge = __any(edge), any_border = __any(border);
+    if (ovf) {
+        if (lane == 0) res[job].border = -1;   // not done: the host reruns it on the tiled engine
+        return;
+    }
+    if (lane == 0) {
+        FloodRes r;
+        r.cells = cells;
+        r.sampled = sampled;
+        r.minx = minx;
+        r.maxx = maxx;
+        r.minz = minz;
+        r.maxz = maxz;
+        r.edge = any_edge;
+        r.border = any_border;
+        res[job] = r;
+    }
+    if (cells == 0) return;
+    // ---- counted cells as 32x32 tile masks over the counted bounding box
+    const int i0 = (minx - J.qx) / J.step + J.half, i1 = (maxx - J.qx) / J.step + J.half;
+    const int j0 = (minz - J.qz) / J.step + J.half, j1 = (maxz - J.qz) / J.step + J.half;
+    for (int tz = j0 >> 5; tz <= (j1 >> 5); tz++)
+        for (int tx = i0 >> 5; tx <= (i1 >> 5); tx++) {
+            uint32_t row = 0;
+            if (lane < 32) {
+                const int j = tz * 32 + lane;
+                if (j < nw) {
+                    const int base = j * nw + tx * 32;                  // first cell of this tile row
+                    const int w0 = base >> 5, sh = base & 31;
+                    const uint64_t two = uint64_t(counted[w0]) | (uint64_t(w0 + 1 < LDS_WORDS ? counted[w0 + 1] : 0u) << 32);
+                    row = uint32_t(two >> sh);
+                    const int k = nw - tx * 32;                          // valid columns of this row
+                    if (k < 32) row &= (1u << k) - 1u;
+                }
+            }
+            if (!__any(row != 0)) continue;
+            uint32_t k = 0;
+            if (lane == 0) k = atomicAdd(mcount, 1u);
+            k = __shfl(k, 0);
+            if (k >= mcap) continue;
+            if (lane < 32) masks[k].rows[lane] = row;
+            if (lane == 0) {
+                masks[k].job = job;
+                masks[k].tx = tx;
+                masks[k].tz = tz;
+            }
+        }
+}
+
+void launch_flood_lds(hipStream_t st, bool hot, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
+                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t *mcount, uint32_t mcap) {
+    if (!n) return;
+    if (hot)
+        hipLaunchKernelGGL(k_flood_lds<ClimT>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcount, mcap);
+    else
+        hipLaunchKernelGGL(k_flood_lds<ClimSeed>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcount, mcap);
+}
+
+}  // namespace dsx

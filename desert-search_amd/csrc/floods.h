@@ -24,10 +24,18 @@ struct Tile {
     uint32_t job;
     int32_t tx, tz;
     uint32_t queued;        // 1 while listed for the next propagation round
+    uint32_t slot1;         // hash slot + 1 of this tile's entry (0: not in the table)
+    uint32_t pad_[3];
     uint32_t pass[FT];      // predicate != 0 (inside window and world)
     uint32_t cnt[FT];       // predicate == 2
     uint32_t reach[FT];     // cells the flood has entered (pushed)
     uint32_t in[FT];        // cells touched from a neighbouring tile (and the start cell)
+};
+
+struct TileMask {
+    uint32_t job;
+    int32_t tx, tz;
+    uint32_t rows[FT];
 };
 
 struct FloodBufs {
@@ -41,18 +49,22 @@ struct FloodBufs {
     uint32_t list_cap;
 };
 
-void launch_flood_start(hipStream_t st, const FloodJob *jobs, uint32_t n, FloodBufs B, uint32_t *eval, uint32_t *prop);
+// jl[0..n): indices of the jobs to start (their hash keys and result slots use the original index)
+void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B, uint32_t *eval,
+                        uint32_t *prop);
 void launch_flood_eval(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
                        const uint32_t *list, uint32_t n);
 void launch_flood_prop(hipStream_t st, const FloodJob *jobs, FloodBufs B, const uint32_t *list, uint32_t n,
                        uint32_t *next_eval, uint32_t *next_prop);
-void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res);
-struct TileMask {
-    uint32_t job;
-    int32_t tx, tz;
-    uint32_t rows[FT];
-};
-void launch_flood_export(hipStream_t st, FloodBufs B, uint32_t ntiles, TileMask *out, uint32_t *count, uint32_t cap);
-void launch_flood_resinit(hipStream_t st, const FloodJob *jobs, uint32_t n, FloodRes *res);
+// Reduce, export the counted masks, and return every touched tile and hash slot to the zero / empty
+// state the next call expects (no pool-wide memsets).
+void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res,
+                      struct TileMask *masks, uint32_t *mcount, uint32_t mcap);
+// res[k] = empty area at the centre, for k in jl[0..n) (jl null: k in [0, n))
+void launch_flood_resinit(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodRes *res);
+// One wave per small-window flood (2*half+1 <= 255), state in LDS; hot: predicate is the T test.
+constexpr int LDS_MAX_N = 255;
+void launch_flood_lds(hipStream_t st, bool hot, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
+                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t *mcount, uint32_t mcap);
 
 }  // namespace dsx

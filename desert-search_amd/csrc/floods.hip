@@ -55,6 +55,7 @@ __device__ uint32_t tile_get(const FloodBufs &B, uint32_t job, int tx, int tz, b
             const unsigned long long want = key | mine;
             cur = atomicCAS(&B.hkeys[h], HEMPTY, want);
             if (cur == HEMPTY) {
+                B.tiles[mine].slot1 = h + 1;
                 *fresh = true;
                 return mine;
             }
@@ -63,33 +64,63 @@ __device__ uint32_t tile_get(const FloodBufs &B, uint32_t job, int tx, int tz, b
     }
     return 0xffffffffu;
 }
+// Octave slots read by the hot test (first 4: T of both sides) and the proxy (all 14: plus C octaves
+// 0-2 and E octaves 0-1 of both sides).
+__device__ __forceinline__ int proxy_slot(int k) {
+    constexpr int slots[14] = {0, 1, 2, 3, 8, 9, 10, 17, 18, 19, 26, 27, 30, 31};
+    return slots[k];
+}
 __device__ __forceinline__ void list_push(uint32_t *list, uint32_t *count, uint32_t cap, uint32_t *ovf, uint32_t v) {
     const uint32_t k = atomicAdd(count, 1u);
     if (k < cap) list[k] = v;
     else atomicOr(ovf, 1u);
 }
 
-__global__ void k_flood_resinit(const FloodJob *jobs, uint32_t n, FloodRes *res) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+__global__ void k_flood_resinit(const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodRes *res) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t k = jl ? jl[q] : q;
     FloodRes r{};
     r.minx = r.maxx = jobs[k].qx;
     r.minz = r.maxz = jobs[k].qz;
     res[k] = r;
 }
 
+// Appends v to list for every lane with on set: one atomic per wave.
+__device__ __forceinline__ void wave_push(uint32_t *list, uint32_t *count, uint32_t cap, uint32_t *ovf, bool on, uint32_t v) {
+    const uint64_t mask = __ballot(on);
+    if (mask == 0) return;
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, uint32_t(__popcll(mask)));
+    base = __shfl(base, 0);
+    if (on) {
+        const uint32_t k = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+        if (k < cap) list[k] = v;
+        else atomicOr(ovf, 1u);
+    }
+}
+
 // The start cell of every flood: touched ("in") and queued.
-__global__ void k_flood_start(const FloodJob *jobs, uint32_t n, FloodBufs B, uint32_t *eval, uint32_t *prop) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const FloodJob J = jobs[k];
-    const int c = J.half % FT, t = J.half / FT;
-    bool fresh;
-    const uint32_t ti = tile_get(B, k, t, t, &fresh);
-    if (ti == 0xffffffffu) { atomicOr(&B.lcount[4], 2u); return; }
-    atomicOr(&B.tiles[ti].in[c], 1u << c);
-    if (fresh) list_push(eval, &B.lcount[0], B.list_cap, &B.lcount[4], ti);
-    if (atomicCAS(&B.tiles[ti].queued, 0u, 1u) == 0u) list_push(prop, &B.lcount[1], B.list_cap, &B.lcount[4], ti);
+__global__ __launch_bounds__(256) void k_flood_start(const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B,
+                                                     uint32_t *eval, uint32_t *prop) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    bool fresh = false, queue = false;
+    uint32_t ti = 0xffffffffu;
+    if (q < n) {
+        const uint32_t k = jl[q];
+        const FloodJob J = jobs[k];
+        const int c = J.half % FT, t = J.half / FT;   // the centre cell (half, half)
+        ti = tile_get(B, k, t, t, &fresh);
+        if (ti == 0xffffffffu) {
+            atomicOr(&B.lcount[4], 2u);
+        } else {
+            atomicOr(&B.tiles[ti].in[c], 1u << c);
+            queue = atomicCAS(&B.tiles[ti].queued, 0u, 1u) == 0u;
+        }
+    }
+    wave_push(eval, &B.lcount[0], B.list_cap, &B.lcount[4], fresh && ti != 0xffffffffu, ti);
+    wave_push(prop, &B.lcount[1], B.list_cap, &B.lcount[4], queue, ti);
 }
 
 // Predicate of the 1024 cells of each listed tile: pass = v != 0, cnt = v == 2 (cells outside the window
@@ -102,9 +133,22 @@ __global__ __launch_bounds__(256) void k_flood_eval(const FloodJob *jobs, const 
     if (b >= n) return;
     Tile &T = B.tiles[list[b]];
     const FloodJob J = jobs[T.job];
-    const uint4 *src = reinterpret_cast<const uint4 *>(&cs[J.seed]);
-    uint4 *dst = reinterpret_cast<uint4 *>(&C);
-    for (int k = threadIdx.x; k < int(sizeof(ClimSeed) / 16); k += 256) dst[k] = src[k];
+    // copy only the octaves the predicate reads: T for the hot test, T + C octaves 0-2 + E octaves
+    // 0-1 for the proxy, everything for the biome predicates
+    const ClimSeed &G = cs[J.seed];
+    constexpr int OW = sizeof(ClimOct) / 16;
+    if (J.kind == PRED_HOT || J.kind == PRED_PROXY) {
+        const int nslot = J.kind == PRED_HOT ? 4 : 14;
+        for (int k = threadIdx.x; k < nslot * OW; k += 256) {
+            const int slot = proxy_slot(k / OW);
+            reinterpret_cast<uint4 *>(&C.oct[slot])[k % OW] = reinterpret_cast<const uint4 *>(&G.oct[slot])[k % OW];
+        }
+        if (threadIdx.x < 6) C.factor[threadIdx.x] = G.factor[threadIdx.x];
+    } else {
+        const uint4 *src = reinterpret_cast<const uint4 *>(&G);
+        uint4 *dst = reinterpret_cast<uint4 *>(&C);
+        for (int k = threadIdx.x; k < int(sizeof(ClimSeed) / 16); k += 256) dst[k] = src[k];
+    }
     if (threadIdx.x < FT) { sp[threadIdx.x] = 0; sc[threadIdx.x] = 0; }
     __syncthreads();
     const int nw = int(nwin(J));
@@ -171,7 +215,9 @@ __global__ __launch_bounds__(64) void k_flood_prop(const FloodJob *jobs, FloodBu
         vert_up |= (up0 << 1) | (up0 >> 1);
         vert_dn |= (dn31 << 1) | (dn31 >> 1);
     }
-    // 8 directions: dx, dz, and this lane's contribution (row, mask)
+    // 8 directions: dx, dz, and this lane's contribution (row, mask); lane 0 collects the tiles to list
+    uint32_t push_e[8], push_p[8];
+    int ne = 0, np = 0;
     for (int d = 0; d < 8; d++) {
         const int dxs[8] = {-1, 1, 0, 0, -1, 1, -1, 1}, dzs[8] = {0, 0, -1, 1, -1, -1, 1, 1};
         const int dx = dxs[d], dz = dzs[d];
@@ -207,19 +253,36 @@ __global__ __launch_bounds__(64) void k_flood_prop(const FloodJob *jobs, FloodBu
             const uint32_t o = atomicOr(&N.in[row], mask);
             grew = (o | mask) != o;
         }
-        if (lane == 0 && fresh) list_push(next_eval, &B.lcount[2], B.list_cap, &B.lcount[4], nti);
+        if (lane == 0 && fresh) push_e[ne++] = nti;
         if (__any(grew) && lane == 0) {
             __threadfence();
-            if (atomicCAS(&N.queued, 0u, 1u) == 0u) list_push(next_prop, &B.lcount[3], B.list_cap, &B.lcount[4], nti);
+            if (atomicCAS(&N.queued, 0u, 1u) == 0u) push_p[np++] = nti;
+        }
+    }
+    if (lane == 0) {
+        if (ne) {
+            const uint32_t k = atomicAdd(&B.lcount[2], uint32_t(ne));
+            for (int q = 0; q < ne; q++) {
+                if (k + q < B.list_cap) next_eval[k + q] = push_e[q];
+                else atomicOr(&B.lcount[4], 1u);
+            }
+        }
+        if (np) {
+            const uint32_t k = atomicAdd(&B.lcount[3], uint32_t(np));
+            for (int q = 0; q < np; q++) {
+                if (k + q < B.list_cap) next_prop[k + q] = push_p[q];
+                else atomicOr(&B.lcount[4], 1u);
+            }
         }
     }
 }
 
 // Per-tile reduction of the flood outputs into res[job].
-__global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res) {
+__global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res,
+                                                  TileMask *masks, uint32_t *mcount, uint32_t mcap) {
     const uint32_t b = blockIdx.x;
     if (b >= ntiles) return;
-    const Tile &T = B.tiles[b];
+    Tile &T = B.tiles[b];
     const FloodJob J = jobs[T.job];
     const int lane = threadIdx.x, r = lane & 31;
     const bool c8 = J.conn8 != 0;
@@ -282,38 +345,41 @@ __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBuf
         if (any_border) atomicOr(&o.border, 1);
         if (any_edge) atomicOr(&o.edge, 1);
     }
-}
-
-// Tiles with counted reached cells -> (job, tx, tz, reach & cnt rows).
-__global__ __launch_bounds__(64) void k_flood_export(FloodBufs B, uint32_t ntiles, TileMask *out, uint32_t *count,
-                                                     uint32_t cap) {
-    const uint32_t b = blockIdx.x;
-    if (b >= ntiles) return;
-    const Tile &T = B.tiles[b];
-    const int lane = threadIdx.x;
-    const uint32_t m = lane < FT ? (T.reach[lane] & T.cnt[lane]) : 0u;
-    if (!__any(m != 0)) return;
-    uint32_t k = 0;
-    if (lane == 0) k = atomicAdd(count, 1u);
-    k = __shfl(k, 0);
-    if (k >= cap) return;
-    if (lane < FT) out[k].rows[lane] = m;
+    // counted cells of this tile -> mask export
+    if (__any(RC != 0)) {
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(mcount, 1u);
+        k = __shfl(k, 0);
+        if (k < mcap) {
+            if (lane < FT) masks[k].rows[lane] = RC;
+            if (lane == 0) {
+                masks[k].job = T.job;
+                masks[k].tx = T.tx;
+                masks[k].tz = T.tz;
+            }
+        }
+    }
+    // leave the tile and its hash slot clean for the next call (everything above has read them)
+    __builtin_amdgcn_wave_barrier();
+    if (lane < FT) {
+        T.reach[lane] = 0;
+        T.in[lane] = 0;
+    }
     if (lane == 0) {
-        out[k].job = T.job;
-        out[k].tx = T.tx;
-        out[k].tz = T.tz;
+        if (T.slot1) B.hkeys[T.slot1 - 1] = HEMPTY;
+        T.slot1 = 0;
+        T.queued = 0;
     }
 }
 
 // ------------------------------------------------------------------ launchers
-void launch_flood_export(hipStream_t st, FloodBufs B, uint32_t ntiles, TileMask *out, uint32_t *count, uint32_t cap) {
-    if (ntiles) hipLaunchKernelGGL(k_flood_export, dim3(ntiles), dim3(64), 0, st, B, ntiles, out, count, cap);
+
+void launch_flood_resinit(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodRes *res) {
+    if (n) hipLaunchKernelGGL(k_flood_resinit, dim3((n + 255) / 256), dim3(256), 0, st, jobs, jl, n, res);
 }
-void launch_flood_resinit(hipStream_t st, const FloodJob *jobs, uint32_t n, FloodRes *res) {
-    if (n) hipLaunchKernelGGL(k_flood_resinit, dim3((n + 255) / 256), dim3(256), 0, st, jobs, n, res);
-}
-void launch_flood_start(hipStream_t st, const FloodJob *jobs, uint32_t n, FloodBufs B, uint32_t *eval, uint32_t *prop) {
-    if (n) hipLaunchKernelGGL(k_flood_start, dim3((n + 255) / 256), dim3(256), 0, st, jobs, n, B, eval, prop);
+void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B, uint32_t *eval,
+                        uint32_t *prop) {
+    if (n) hipLaunchKernelGGL(k_flood_start, dim3((n + 255) / 256), dim3(256), 0, st, jobs, jl, n, B, eval, prop);
 }
 void launch_flood_eval(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
                        const uint32_t *list, uint32_t n) {
@@ -323,8 +389,9 @@ void launch_flood_prop(hipStream_t st, const FloodJob *jobs, FloodBufs B, const 
                        uint32_t *next_eval, uint32_t *next_prop) {
     if (n) hipLaunchKernelGGL(k_flood_prop, dim3(n), dim3(64), 0, st, jobs, B, list, n, next_eval, next_prop);
 }
-void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res) {
-    if (ntiles) hipLaunchKernelGGL(k_flood_acc, dim3(ntiles), dim3(64), 0, st, jobs, B, ntiles, res);
+void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res, TileMask *masks,
+                      uint32_t *mcount, uint32_t mcap) {
+    if (ntiles) hipLaunchKernelGGL(k_flood_acc, dim3(ntiles), dim3(64), 0, st, jobs, B, ntiles, res, masks, mcount, mcap);
 }
 
 }  // namespace dsx

@@ -4,7 +4,10 @@
 #include "ctx.h"
 #include "floods.h"
 #include <algorithm>
+#include <chrono>
 #include <climits>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <unordered_map>
 #include <vector>
@@ -88,6 +91,7 @@ struct ClimState {
     dsx::TileMask *d_masks = nullptr;
     uint32_t mask_cap = 0;
     std::vector<dsx::TileMask> masks;
+    uint32_t *d_jl = nullptr;   // job index lists: small hot + small other (<= job_cap), then tiled (<= job_cap)
 };
 
 static ClimState *clim_state(dsx_ctx *c);
@@ -130,6 +134,7 @@ static int clim_alloc(dsx_ctx *c, ClimState *&S) {
     HIPCK(hipMalloc(&S->d_res, size_t(S->job_cap) * sizeof(dsx::FloodRes)));
     S->mask_cap = 1u << 20;
     HIPCK(hipMalloc(&S->d_masks, size_t(S->mask_cap) * sizeof(dsx::TileMask)));
+    HIPCK(hipMalloc(&S->d_jl, size_t(S->job_cap) * 8));
     return DSX_OK;
 }
 
@@ -176,7 +181,7 @@ void dsx_ctx_free_ext(dsx_ctx *c) {
     for (void *p : {(void *)s->d_cuts, (void *)s->d_cell, (void *)s->d_clim, (void *)s->d_useeds, (void *)s->fb.tiles,
                     (void *)s->fb.ntiles, (void *)s->fb.hkeys, (void *)s->fb.lists[0], (void *)s->fb.lists[1],
                     (void *)s->fb.lists[2], (void *)s->fb.lists[3], (void *)s->fb.lcount, (void *)s->d_jobs,
-                    (void *)s->d_res, s->d_io, (void *)s->d_masks})
+                    (void *)s->d_res, s->d_io, (void *)s->d_masks, (void *)s->d_jl})
         if (p) (void)hipFree(p);
     delete s;
 }
@@ -278,9 +283,47 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     HIPCK(hipMemcpyAsync(S->d_jobs, hj.data(), n * sizeof(dsx::FloodJob), hipMemcpyHostToDevice, c->st));
     HIPCK(hipMemsetAsync(B.ntiles, 0, 4, c->st));
     HIPCK(hipMemsetAsync(B.lcount, 0, 32, c->st));
-    dsx::launch_flood_resinit(c->st, S->d_jobs, uint32_t(n), S->d_res);
-    dsx::launch_flood_start(c->st, S->d_jobs, uint32_t(n), B, B.lists[0], B.lists[1]);
+    dsx::launch_flood_resinit(c->st, S->d_jobs, nullptr, uint32_t(n), S->d_res);
+    // small windows: one wave each with the window in LDS; the rest: the tiled engine
+    std::vector<uint32_t> jl_hot, jl_oth, jl_big;
+    for (uint64_t k = 0; k < n; k++) {
+        if (2 * hj[k].half + 1 <= dsx::LDS_MAX_N) (hj[k].kind == dsx::PRED_HOT ? jl_hot : jl_oth).push_back(uint32_t(k));
+        else jl_big.push_back(uint32_t(k));
+    }
+    uint32_t *d_hot = S->d_jl, *d_oth = d_hot + jl_hot.size(), *d_big = d_hot + S->job_cap;
+    if (!jl_hot.empty()) HIPCK(hipMemcpyAsync(d_hot, jl_hot.data(), jl_hot.size() * 4, hipMemcpyHostToDevice, c->st));
+    if (!jl_oth.empty()) HIPCK(hipMemcpyAsync(d_oth, jl_oth.data(), jl_oth.size() * 4, hipMemcpyHostToDevice, c->st));
+    dsx::launch_flood_lds(c->st, true, S->d_jobs, d_hot, uint32_t(jl_hot.size()), S->d_clim, S->grid, S->d_res, S->d_masks,
+                          B.lcount + 5, S->mask_cap);
+    dsx::launch_flood_lds(c->st, false, S->d_jobs, d_oth, uint32_t(jl_oth.size()), S->d_clim, S->grid, S->d_res,
+                          S->d_masks, B.lcount + 5, S->mask_cap);
     HIPCK(hipGetLastError());
+    std::vector<dsx::FloodRes> res(n);
+    if (!jl_hot.empty() || !jl_oth.empty()) {
+        // windows whose LDS ring overflowed go to the tiled engine
+        HIPCK(hipMemcpyAsync(res.data(), S->d_res, n * sizeof(dsx::FloodRes), hipMemcpyDeviceToHost, c->st));
+        HIPCK(hipStreamSynchronize(c->st));
+        for (uint32_t k : jl_hot) if (res[k].border < 0) jl_big.push_back(k);
+        for (uint32_t k : jl_oth) if (res[k].border < 0) jl_big.push_back(k);
+    }
+    if (!jl_big.empty()) {
+        HIPCK(hipMemcpyAsync(d_big, jl_big.data(), jl_big.size() * 4, hipMemcpyHostToDevice, c->st));
+        dsx::launch_flood_resinit(c->st, S->d_jobs, d_big, uint32_t(jl_big.size()), S->d_res);
+        dsx::launch_flood_start(c->st, S->d_jobs, d_big, uint32_t(jl_big.size()), B, B.lists[0], B.lists[1]);
+        HIPCK(hipGetLastError());
+    }
+    static double prof_t = 0, prof_rounds = 0, prof_calls = 0, prof_tiles = 0, prof_sampled = 0;
+    struct ProfOut {
+        double *t, *r, *c, *n, *s;
+        ~ProfOut() {
+            if (std::getenv("DSX_PROFILE"))
+                std::fprintf(stderr, "dsx_flood profile: %.0f calls, %.0f rounds, %.0f tiles (%.0f cells evaluated), %.0f sampled, %.3f s\n",
+                             *c, *r, *n, *n * 1024, *s, *t);
+        }
+    };
+    static ProfOut prof_out{&prof_t, &prof_rounds, &prof_calls, &prof_tiles, &prof_sampled};
+    const auto prof_t0 = std::chrono::steady_clock::now();
+    prof_calls++;
     int cur = 0;   // lists[2*cur] eval, lists[2*cur+1] prop
     uint32_t *h = c->h_cnt;
     int status = DSX_OK;
@@ -290,6 +333,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         if (h[4]) { status = DSX_E_CAPACITY; break; }
         const uint32_t ne = h[0], np = h[1];
         if (np == 0 || round > 4000000) break;
+        prof_rounds++;
         uint32_t *le = B.lists[2 * cur], *lp = B.lists[2 * cur + 1];
         uint32_t *nle = B.lists[2 * (1 - cur)], *nlp = B.lists[2 * (1 - cur) + 1];
         // next-round counters start at zero; this round's lists are consumed
@@ -305,12 +349,10 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     HIPCK(hipMemcpyAsync(&ntiles, B.ntiles, 4, hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
     ntiles = std::min(ntiles, B.tile_cap);
+    prof_tiles += ntiles;
     if (status == DSX_OK) {
-        dsx::launch_flood_acc(c->st, S->d_jobs, B, ntiles, S->d_res);
-        HIPCK(hipMemsetAsync(B.lcount + 5, 0, 4, c->st));
-        dsx::launch_flood_export(c->st, B, ntiles, S->d_masks, B.lcount + 5, S->mask_cap);
+        dsx::launch_flood_acc(c->st, S->d_jobs, B, ntiles, S->d_res, S->d_masks, B.lcount + 5, S->mask_cap);
         HIPCK(hipGetLastError());
-        std::vector<dsx::FloodRes> res(n);
         HIPCK(hipMemcpyAsync(res.data(), S->d_res, n * sizeof(dsx::FloodRes), hipMemcpyDeviceToHost, c->st));
         HIPCK(hipMemcpyAsync(h + 5, B.lcount + 5, 4, hipMemcpyDeviceToHost, c->st));
         HIPCK(hipStreamSynchronize(c->st));
@@ -325,16 +367,21 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
                                  hipMemcpyDeviceToHost, c->st));
             HIPCK(hipStreamSynchronize(c->st));
         }
+        for (uint64_t k = 0; k < n; k++) prof_sampled += double(res[k].sampled);
         for (uint64_t k = 0; k < n; k++)
             out[k] = dsx_area{res[k].cells, res[k].sampled, res[k].minx, res[k].maxx, res[k].minz, res[k].maxz,
                               res[k].edge, res[k].border};
     } else {
         dsx::g_last_error = "flood tile pool exhausted: split the batch";
     }
-    // leave the pool and hash clean for the next call
-    HIPCK(hipMemsetAsync(B.tiles, 0, size_t(ntiles) * sizeof(dsx::Tile), c->st));
-    HIPCK(hipMemsetAsync(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8, c->st));
+    // k_flood_acc has returned every tile and hash slot of this call to the clean state; after an
+    // overflow (acc not run) reset them wholesale
+    if (status != DSX_OK) {
+        HIPCK(hipMemsetAsync(B.tiles, 0, size_t(ntiles) * sizeof(dsx::Tile), c->st));
+        HIPCK(hipMemsetAsync(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8, c->st));
+    }
     HIPCK(hipStreamSynchronize(c->st));
+    prof_t += std::chrono::duration<double>(std::chrono::steady_clock::now() - prof_t0).count();
     return status;
 }
 

@@ -11,8 +11,10 @@
 #include "../../include/dsx.h"
 #include "../csrc/dsx_rng.h"
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <coroutine>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -47,9 +49,56 @@ struct Task {
     std::coroutine_handle<promise_type> h;
 };
 
+// A flood's outputs: the DenseArea and its counted cells (state 3 of dense_flood.hpp's window) as
+// 32x32 tile masks sorted by (tz, tx).
 struct FloodOut {
     dsx_area a{};
-    std::vector<std::pair<int, int>> cells;   // counted window cells (i, j), ordered by j then i
+    int qx = 0, qz = 0, step = 1, half = 0;
+    std::vector<dsx_tile_mask> tiles;
+    bool counted(int i, int j) const {
+        if (i < 0 || j < 0 || i > 2 * half || j > 2 * half) return false;
+        const int tx = i >> 5, tz = j >> 5;
+        auto it = std::lower_bound(tiles.begin(), tiles.end(), std::make_pair(tz, tx),
+                                   [](const dsx_tile_mask &m, const std::pair<int, int> &k) {
+                                       return m.tz != k.first ? m.tz < k.first : m.tx < k.second;
+                                   });
+        if (it == tiles.end() || it->tz != tz || it->tx != tx) return false;
+        return (it->rows[j & 31] >> (i & 31)) & 1u;
+    }
+    // counted cells in window row-major order (j, then i), as the reference scans its bbox
+    template <class F> void each_counted(F &&f) const {
+        size_t t0 = 0;
+        while (t0 < tiles.size()) {
+            size_t t1 = t0;
+            while (t1 < tiles.size() && tiles[t1].tz == tiles[t0].tz) t1++;
+            for (int r = 0; r < 32; r++)
+                for (size_t t = t0; t < t1; t++)
+                    for (uint32_t b = tiles[t].rows[r]; b; b &= b - 1) f(tiles[t].tx * 32 + __builtin_ctz(b), tiles[t].tz * 32 + r);
+            t0 = t1;
+        }
+    }
+};
+
+// The reference's key sets (hotcovered: 128-quart keys, covered: 32-quart keys) as the list of floods
+// whose counted cells were inserted. A flood's keys are an affine image of its window lattice, so a
+// key is in the set iff the cell it maps back to is counted in one of the floods.
+struct KeySet {
+    struct Cover { const FloodOut *f; int shift, sub, bx, bz; };
+    std::vector<Cover> covers;
+    // insert every counted cell of f, lattice point x0 + (i - half) * step plus `sub` sub-points per axis
+    // spaced 2^shift (sub = 2: the step-256 pre-flood's 4 keys per cell, dsearch_desert.cpp:345)
+    void add(const FloodOut &f, int shift, int sub) {
+        covers.push_back({&f, shift, sub, (f.qx - f.half * f.step + 8000000) >> shift,
+                          (f.qz - f.half * f.step + 8000000) >> shift});
+    }
+    bool has(int x, int z) const {
+        for (const Cover &c : covers) {
+            const int mx = ((x + 8000000) >> c.shift) - c.bx, mz = ((z + 8000000) >> c.shift) - c.bz;
+            if (mx < 0 || mz < 0) continue;
+            if (c.f->counted(mx / c.sub, mz / c.sub)) return true;
+        }
+        return false;
+    }
 };
 
 struct Sched;
@@ -76,6 +125,25 @@ struct FloodReq {
     bool await_ready() const noexcept { return jobs.empty(); }
     void await_suspend(std::coroutine_handle<> hh);
     void await_resume() const noexcept {}
+};
+
+// Wall-time accounting of the search stages, printed to stderr when DSX_PROFILE is set.
+struct Prof {
+    double t[8] = {};
+    uint64_t n[8] = {};
+    ~Prof() {
+        if (!std::getenv("DSX_PROFILE")) return;
+        static const char *name[8] = {"gate", "scan", "sites", "points", "floods", "rows", "steps", "flood jobs"};
+        for (int i = 0; i < 8; i++)
+            if (t[i] > 0 || n[i]) std::fprintf(stderr, "dsx_search profile: %-10s %8.3f s  %llu\n", name[i], t[i], (unsigned long long)n[i]);
+    }
+};
+Prof g_prof;
+struct Timer {
+    int k;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit Timer(int k_) : k(k_) {}
+    ~Timer() { g_prof.t[k] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); g_prof.n[k]++; }
 };
 
 struct Sched {
@@ -141,7 +209,14 @@ struct Sched {
             return flood_range(jobs, who, mid, hi, cells);
         }
         if (rc) return rc;
-        for (size_t k = lo; k < hi; k++) who[k].first->out[who[k].second].a = res[k - lo];
+        for (size_t k = lo; k < hi; k++) {
+            FloodOut &o = who[k].first->out[who[k].second];
+            o.a = res[k - lo];
+            o.qx = jobs[k].qx;
+            o.qz = jobs[k].qz;
+            o.step = jobs[k].step;
+            o.half = jobs[k].half;
+        }
         if (cells) {
             uint64_t nm = 0;
             dsx_flood_masks(ctx, nullptr, 0, &nm);
@@ -149,18 +224,15 @@ struct Sched {
             if (nm && (rc = dsx_flood_masks(ctx, masks.data(), nm, &nm))) return rc;
             for (const dsx_tile_mask &m : masks) {
                 auto [r, k] = who[lo + m.job];
-                if (!r->want_cells) continue;
-                auto &cl = r->out[k].cells;
-                for (int row = 0; row < 32; row++)
-                    for (uint32_t b = m.rows[row]; b; b &= b - 1) cl.push_back({m.tx * 32 + __builtin_ctz(b), m.tz * 32 + row});
+                if (r->want_cells) r->out[k].tiles.push_back(m);
             }
             for (size_t k = lo; k < hi; k++) {
                 auto [r, q] = who[k];
-                if (!r->want_cells) continue;
-                auto &cl = r->out[q].cells;
-                std::sort(cl.begin(), cl.end(), [](const std::pair<int, int> &a, const std::pair<int, int> &b) {
-                    return a.second != b.second ? a.second < b.second : a.first < b.first;
-                });
+                auto &t = r->out[q].tiles;
+                if (t.size() > 1)
+                    std::sort(t.begin(), t.end(), [](const dsx_tile_mask &a, const dsx_tile_mask &b) {
+                        return a.tz != b.tz ? a.tz < b.tz : a.tx < b.tx;
+                    });
             }
         }
         return DSX_OK;
@@ -205,8 +277,16 @@ struct Sched {
         std::vector<FloodReq *> fs;
         ps.swap(points);
         fs.swap(floods);
-        if (!ps.empty()) err = run_points(ps);
-        if (!err && !fs.empty()) err = run_floods(fs);
+        g_prof.n[6]++;
+        if (!ps.empty()) {
+            Timer t(3);
+            err = run_points(ps);
+        }
+        if (!err && !fs.empty()) {
+            Timer t(4);
+            for (auto *r : fs) g_prof.n[7] += r->jobs.size();
+            err = run_floods(fs);
+        }
         if (err) return false;
         for (auto *r : ps) r->h.resume();
         for (auto *r : fs) r->h.resume();
@@ -268,7 +348,8 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
     const size_t ns = sites.size();
     std::vector<Eval> &ev = O.ev;
     std::vector<std::pair<int, int>> seen;
-    std::unordered_set<uint64_t> covered, hotcovered;
+    KeySet covered, hotcovered;
+    std::vector<std::unique_ptr<FloodOut>> keep;   // floods referenced by the key sets
 
     // ---- hottest of the 7x7 T samples of every site (dsearch_desert.cpp:328-332), one batch
     std::vector<int> hx(ns), hz(ns);
@@ -278,6 +359,8 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
     std::vector<float> px_b(ns, -1);
     if (P.hot_min > 0) {
         PointsReq q{&S, seed, DSX_POINT_T, 0.f, {}, {}, {}, {}, {}};
+        q.xs.reserve(ns * 49);
+        q.zs.reserve(ns * 49);
         for (size_t r = 0; r < ns; r++)
             for (int dz = -3; dz <= 3; dz++)
                 for (int dx = -3; dx <= 3; dx++) {
@@ -321,123 +404,191 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
                 }
         }
     }
-    // ---- step-256 pre-floods of every distinct hot point (dsearch_desert.cpp:337), one batch
-    std::map<std::pair<int, int>, size_t> pre_idx;
-    FloodReq pre{&S, {}, true, {}, {}};
-    if (P.hot_min > 0 && P.hot_pre > 0) {
+    // ---- speculative batches. Every flood below is a pure function of its start point, so results
+    // computed ahead are exactly what the sequential loop would compute; the loop decides which to use.
+    using XY = std::pair<int, int>;
+    std::map<XY, size_t> pre_idx, h128_idx;
+    FloodReq pre{&S, {}, true, {}, {}};          // step-256 pre-floods of every distinct hot point
+    FloodReq h128{&S, {}, true, {}, {}};         // step-128 floods of every big one
+    if (P.hot_min > 0) {
         for (size_t r = 0; r < ns; r++) {
-            if (bt[r] < 0.55) continue;
-            auto key = std::make_pair(hx[r], hz[r]);
-            if (pre_idx.count(key)) continue;
-            pre_idx[key] = pre.jobs.size();
+            if (bt[r] < 0.55 || pre_idx.count({hx[r], hz[r]})) continue;
+            pre_idx[{hx[r], hz[r]}] = pre.jobs.size();
             pre.jobs.push_back(fjob(seed, DSX_PRED_HOT, 0.f, hx[r], hz[r], 256, 32000 / 256, 0));
         }
-        co_await pre;
+        if (P.hot_pre > 0) co_await pre;
+        for (auto &[xy, k] : pre_idx)
+            if (P.hot_pre <= 0 || pre.out[k].a.cells * 256.0 * 256 * 16 / 1e6 >= P.hot_pre * P.hot_min) {
+                h128_idx[xy] = h128.jobs.size();
+                h128.jobs.push_back(fjob(seed, DSX_PRED_HOT, 0.f, xy.first, xy.second, 128, 32000 / 128, 0));
+            }
+        co_await h128;
     }
-    std::map<std::pair<int, int>, FloodOut> h128;   // step-128 floods done so far, by hot point
+    // starts of every big step-128 zone and their proxy test, one batch; then the proxy flood of each
+    // zone's first passable start, one batch
+    std::vector<std::vector<XY>> zone_starts(h128.jobs.size());
+    PointsReq pxq{&S, seed, DSX_PRED_PROXY, pcut, {}, {}, {}, {}, {}};
+    std::vector<size_t> px_off(h128.jobs.size() + 1, 0);
+    for (size_t z = 0; z < h128.jobs.size(); z++) {
+        const FloodOut &hzn = h128.out[z];
+        const double area = hzn.a.cells * 128.0 * 128 * 16 / 1e6;
+        px_off[z] = pxq.xs.size();
+        if (area >= P.hot_min) {
+            const int x0 = hzn.qx, z0 = hzn.qz, half = hzn.half;
+            hzn.each_counted([&](int i, int j) {
+                if (((i - half) & 3) == 0 && ((j - half) & 3) == 0) {
+                    zone_starts[z].push_back({x0 + (i - half) * 128, z0 + (j - half) * 128});
+                    pxq.xs.push_back(x0 + (i - half) * 128);
+                    pxq.zs.push_back(z0 + (j - half) * 128);
+                }
+            });
+        }
+    }
+    px_off[h128.jobs.size()] = pxq.xs.size();
+    co_await pxq;
+    // Proxy floods ahead of the loop, in rounds: each round floods, per zone, the first passable start
+    // that no flood so far covers. A start the loop will flood is one that no earlier flood covers, so
+    // after the rounds the loop almost never has to wait (the covered set it sees is a subset).
+    std::map<XY, const FloodOut *> prox_idx;
+    std::vector<std::unique_ptr<FloodReq>> prox_rounds;
+    KeySet spec_cov;
+    for (;;) {
+        auto req = std::make_unique<FloodReq>(FloodReq{&S, {}, true, {}, {}});
+        for (size_t z = 0; z < zone_starts.size(); z++)
+            for (size_t k = 0; k < zone_starts[z].size(); k++) {
+                const XY st = zone_starts[z][k];
+                if (pxq.v8[px_off[z] + k] != 2 || prox_idx.count(st) || spec_cov.has(st.first, st.second)) continue;
+                bool dup = false;
+                for (auto &j : req->jobs) dup |= (j.qx == st.first && j.qz == st.second);
+                if (!dup) req->jobs.push_back(fjob(seed, DSX_PRED_PROXY, pcut, st.first, st.second, 32, 16000 / 32, 1));
+                break;
+            }
+        if (req->jobs.empty()) break;
+        co_await *req;
+        for (size_t k = 0; k < req->jobs.size(); k++) {
+            prox_idx[{req->jobs[k].qx, req->jobs[k].qz}] = &req->out[k];
+            spec_cov.add(req->out[k], 5, 1);
+        }
+        prox_rounds.push_back(std::move(req));
+    }
 
     for (size_t rank = 0; rank < ns; rank++) {
         const Site &st = sites[rank];
-        std::vector<std::pair<int, int>> starts;
+        std::vector<XY> starts;
+        std::vector<int8_t> start_px;
         double hot_area = 0;
         if (P.hot_min > 0) {
-            if (bt[rank] < 0.55 || hotcovered.count(hotkey(hx[rank], hz[rank]))) continue;
-            const int x0 = hx[rank], z0 = hz[rank];
+            if (bt[rank] < 0.55 || hotcovered.has(hx[rank], hz[rank])) continue;
+            const XY hp{hx[rank], hz[rank]};
             if (P.hot_pre > 0) {
-                const FloodOut &h2 = pre.out[pre_idx[{x0, z0}]];
+                const FloodOut &h2 = pre.out[pre_idx[hp]];
                 if (h2.a.cells * 256.0 * 256 * 16 / 1e6 < P.hot_pre * P.hot_min) {
-                    const int half = 32000 / 256;
-                    for (auto [i, j] : h2.cells) {
-                        const int x = x0 + (i - half) * 256, z = z0 + (j - half) * 256;
-                        for (int d = 0; d < 4; d++) hotcovered.insert(hotkey(x + (d & 1) * 128, z + (d >> 1) * 128));
-                    }
+                    hotcovered.add(h2, 7, 2);
                     continue;
                 }
             }
-            auto it = h128.find({x0, z0});
-            if (it == h128.end()) {
-                FloodReq f{&S, {fjob(seed, DSX_PRED_HOT, 0.f, x0, z0, 128, 32000 / 128, 0)}, true, {}, {}};
-                co_await f;
-                it = h128.emplace(std::make_pair(x0, z0), std::move(f.out[0])).first;
-            }
-            const FloodOut &hzn = it->second;
+            const size_t z = h128_idx[hp];
+            const FloodOut &hzn = h128.out[z];
             hot_area = hzn.a.cells * 128.0 * 128 * 16 / 1e6;
-            const int half = 32000 / 128;
-            for (auto [i, j] : hzn.cells) {
-                const int x = x0 + (i - half) * 128, z = z0 + (j - half) * 128;
-                hotcovered.insert(hotkey(x, z));
-                if (hot_area >= P.hot_min && ((i - half) & 3) == 0 && ((j - half) & 3) == 0) starts.push_back({x, z});
-            }
+            hotcovered.add(hzn, 7, 1);
             if (hot_area < P.hot_min) continue;
-        } else if (P.multi <= 1) {
-            if (px_b[rank] >= pcut) starts.push_back({px_x[rank], px_z[rank]});
+            starts = zone_starts[z];
+            start_px.assign(pxq.v8.begin() + px_off[z], pxq.v8.begin() + px_off[z + 1]);
         } else {
-            for (int dz = -P.multi; dz <= P.multi; dz++)
-                for (int dx = -P.multi; dx <= P.multi; dx++) starts.push_back({st.x + dx * 512, st.z + dz * 512});
+            if (P.multi <= 1) {
+                if (px_b[rank] >= pcut) starts.push_back({px_x[rank], px_z[rank]});
+            } else {
+                for (int dz = -P.multi; dz <= P.multi; dz++)
+                    for (int dx = -P.multi; dx <= P.multi; dx++) starts.push_back({st.x + dx * 512, st.z + dz * 512});
+            }
+            if (starts.empty()) continue;
+            PointsReq q{&S, seed, DSX_PRED_PROXY, pcut, {}, {}, {}, {}, {}};
+            for (auto [bx, bz] : starts) {
+                q.xs.push_back(bx);
+                q.zs.push_back(bz);
+            }
+            co_await q;
+            start_px = q.v8;
         }
-        if (starts.empty()) continue;
-        // ---- proxy floods (dsearch_desert.cpp:384-400): PX of every start in one batch, then the floods
-        PointsReq pxq{&S, seed, DSX_PRED_PROXY, pcut, {}, {}, {}, {}, {}};
-        for (auto [bx, bz] : starts) {
-            pxq.xs.push_back(bx);
-            pxq.zs.push_back(bz);
-        }
-        co_await pxq;
+        // ---- proxy floods (dsearch_desert.cpp:384-400)
         for (size_t k = 0; k < starts.size(); k++) {
             const int bx = starts[k].first, bz = starts[k].second;
-            if (pxq.v8[k] != 2) continue;
-            if (covered.count(cellkey(bx, bz))) continue;
-            FloodReq f{&S, {fjob(seed, DSX_PRED_PROXY, pcut, bx, bz, 32, 16000 / 32, 1)}, true, {}, {}};
-            co_await f;
-            const FloodOut &a = f.out[0];
-            const int half = 16000 / 32;
-            for (auto [i, j] : a.cells) covered.insert(cellkey(bx + (i - half) * 32, bz + (j - half) * 32));
-            const std::pair<int, int> sig(a.a.minx, a.a.minz);
+            if (start_px[k] != 2) continue;
+            if (covered.has(bx, bz)) continue;
+            const FloodOut *a;
+            auto it = prox_idx.find(starts[k]);
+            if (it != prox_idx.end()) {
+                a = it->second;
+            } else {
+                FloodReq f{&S, {fjob(seed, DSX_PRED_PROXY, pcut, bx, bz, 32, 16000 / 32, 1)}, true, {}, {}};
+                co_await f;
+                keep.push_back(std::make_unique<FloodOut>(std::move(f.out[0])));
+                a = keep.back().get();
+            }
+            covered.add(*a, 5, 1);
+            const std::pair<int, int> sig(a->a.minx, a->a.minz);
             if (std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;
             seen.push_back(sig);
-            Eval e{rank, bx, bz, a.a.cells * 32.0 * 32 * 16 / 1e6};
+            Eval e{rank, bx, bz, a->a.cells * 32.0 * 32 * 16 / 1e6};
             e.hot = hot_area;
             ev.push_back(e);
         }
     }
-    // ---- exact floods of the best proxies (dsearch_desert.cpp:494-533)
+    // ---- exact floods of the best proxies (dsearch_desert.cpp:494-533). The ev entries are
+    // independent: ring searches of all of them in one batch, then their step-8 and step-4 floods.
     std::sort(ev.begin(), ev.end(), [](const Eval &a, const Eval &b) { return a.proxy > b.proxy; });
-    double seed_best = 0;
-    for (size_t k2 = 0; !P.study && k2 < ev.size() && int(k2) < P.exact_n && ev[k2].proxy >= P.exact_min; k2++) {
-        Eval &e = ev[k2];
-        const int sx = e.x, sz = e.z;
-        PointsReq ring{&S, seed, DSX_PRED_DESERT, 0.f, {}, {}, {}, {}, {}};
+    size_t nx = 0;
+    while (!P.study && nx < ev.size() && int(nx) < P.exact_n && ev[nx].proxy >= P.exact_min) nx++;
+    PointsReq ring{&S, seed, DSX_PRED_DESERT, 0.f, {}, {}, {}, {}, {}};
+    for (size_t k2 = 0; k2 < nx; k2++)
         for (int r = 0; r <= 8; r++)
             for (int dz = -r; dz <= r; dz++)
                 for (int dx = -r; dx <= r; dx++) {
                     if (std::max(std::abs(dx), std::abs(dz)) != r) continue;
-                    ring.xs.push_back(sx + dx * 16);
-                    ring.zs.push_back(sz + dz * 16);
+                    ring.xs.push_back(ev[k2].x + dx * 16);
+                    ring.zs.push_back(ev[k2].z + dz * 16);
                 }
-        co_await ring;
-        int bx = sx, bz = sz;
-        bool found = false;
-        for (size_t k = 0; k < ring.xs.size() && !found; k++)
+    co_await ring;
+    const size_t per = ring.xs.size() / std::max<size_t>(nx, 1);
+    FloodReq f8{&S, {}, false, {}, {}};
+    std::vector<int> f8_of(nx, -1);
+    for (size_t k2 = 0; k2 < nx; k2++) {
+        for (size_t k = k2 * per; k < (k2 + 1) * per; k++)
             if (ring.v8[k] == 2) {
-                bx = ring.xs[k];
-                bz = ring.zs[k];
-                found = true;
+                f8_of[k2] = int(f8.jobs.size());
+                f8.jobs.push_back(fjob(seed, DSX_PRED_DESERT_STATE, 0.f, ring.xs[k], ring.zs[k], 8, 16000 / 8, 1));
+                break;
             }
-        if (!found) {
+    }
+    co_await f8;
+    FloodReq f4{&S, {}, false, {}, {}};
+    std::vector<int> f4_of(nx, -1);
+    for (size_t k2 = 0; k2 < nx; k2++) {
+        if (f8_of[k2] < 0) continue;
+        const FloodOut &a8 = f8.out[f8_of[k2]];
+        if (a8.a.cells * 64.0 * 16 / 1e6 >= P.refine_mb) {
+            f4_of[k2] = int(f4.jobs.size());
+            f4.jobs.push_back(fjob(seed, DSX_PRED_DESERT_STATE, 0.f, a8.qx, a8.qz, 4, 16000 / 4, 0));
+        }
+    }
+    co_await f4;
+    double seed_best = 0;
+    for (size_t k2 = 0; k2 < nx; k2++) {
+        Eval &e = ev[k2];
+        if (f8_of[k2] < 0) {
             e.s8 = 0;
             continue;
         }
-        FloodReq f8{&S, {fjob(seed, DSX_PRED_DESERT_STATE, 0.f, bx, bz, 8, 16000 / 8, 1)}, false, {}, {}};
-        co_await f8;
+        const FloodOut &a8 = f8.out[f8_of[k2]];
+        const int bx = a8.qx, bz = a8.qz;
         e.x = bx;
         e.z = bz;
-        e.s8 = f8.out[0].a.cells * 64.0 * 16 / 1e6;
-        e.edge = f8.out[0].a.edge != 0;
-        if (e.s8 >= P.refine_mb) {
-            FloodReq f4{&S, {fjob(seed, DSX_PRED_DESERT_STATE, 0.f, bx, bz, 4, 16000 / 4, 0)}, false, {}, {}};
-            co_await f4;
-            e.s4 = f4.out[0].a.cells * 16.0 * 16 / 1e6;
-            e.edge = e.edge || f4.out[0].a.edge != 0;
+        e.s8 = a8.a.cells * 64.0 * 16 / 1e6;
+        e.edge = a8.a.edge != 0;
+        if (f4_of[k2] >= 0) {
+            const FloodOut &a4 = f4.out[f4_of[k2]];
+            e.s4 = a4.a.cells * 16.0 * 16 / 1e6;
+            e.edge = e.edge || a4.a.edge != 0;
         }
         const double v = e.s4 > 0 ? e.s4 : e.s8;
         seed_best = std::max(seed_best, v);
@@ -517,6 +668,7 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
     while (pos < total && rc == DSX_OK) {
         const uint64_t m = std::min(CHUNK, total - pos);
         batch.clear();
+        auto tgate = std::chrono::steady_clock::now();
         if (mode == DSX_SEARCH_LIST) {
             batch.assign(seeds + pos, seeds + pos + m);
             counts->done += m;
@@ -551,10 +703,12 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
             counts->passed += nph;
         }
         pos += m;
+        g_prof.t[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() - tgate).count();
         // ---- tables + scan, in sub-batches of seeds processed together downstream
         const size_t SUB = 512;
         for (size_t b0 = 0; b0 < batch.size() && rc == DSX_OK; b0 += SUB) {
             const size_t nb = std::min(SUB, batch.size() - b0);
+            auto tscan = std::chrono::steady_clock::now();
             std::vector<float> mta(nb), mtb(nb);
             std::vector<uint64_t> ncand(nb);
             uint64_t cap = 1 << 12;
@@ -582,6 +736,8 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
                 make_sites(r, P, *O);
                 outs.push_back(std::move(O));
             }
+            auto tsites = std::chrono::steady_clock::now();
+            g_prof.t[1] += std::chrono::duration<double>(tsites - tscan).count();
             // ---- hot zones, proxy floods, exact floods: all scanned seeds as coroutines
             Sched S{ctx, {}, {}, DSX_OK, {}};
             std::vector<Task> tasks;
@@ -591,6 +747,7 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
             }
             rc = S.err;
             for (auto &t : tasks) t.h.destroy();
+            g_prof.t[2] += std::chrono::duration<double>(std::chrono::steady_clock::now() - tsites).count();
             if (rc) break;
             // ---- rows (dsearch_desert.cpp:251, 526-548)
             for (auto &O : outs) {

@@ -105,6 +105,18 @@ def test_flood_edges_and_borders(engine, oracle):
     _compare(engine, oracle, rows)
 
 
+def test_flood_small_windows_mixed_batch(engine, oracle):
+    """Windows up to 255 x 255 (single-wave floods) of every kind, mixed with larger ones in one batch."""
+    rows = []
+    for r in FINDS[6:14]:
+        seed, qx, qz = u64(r["seed"]), r["x"] >> 2, r["z"] >> 2
+        rows += [(seed, 0, 0.35, qx, qz, 8, 127, 1), (seed, 0, 0.35, qx, qz, 16, 127, 0),
+                 (seed, 3, 0.35, qx, qz, 8, 100, 0), (seed, 1, 0.35, qx, qz, 256, 127, 0),
+                 (seed, 1, 0.35, qx, qz, 64, 128, 0), (seed, 2, 0.35, qx, qz, 32, 127, 1),
+                 (seed, 2, 0.6, qx - 96, qz + 64, 16, 90, 1), (seed, 0, 0.35, qx, qz, 2, 1, 0)]
+    _compare(engine, oracle, rows)
+
+
 def test_area_queries_step1_match_mcmeasure(engine):
     """The exact step-1 flood (tools/dflood.cpp: half 20000) at verified finds = MC-Measure's cells."""
     dsx = load_package().dsx
