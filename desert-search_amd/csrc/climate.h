@@ -26,12 +26,19 @@ __host__ __device__ constexpr int clim_base(int ch) {
     return ch == 0 ? 0 : ch == 1 ? 4 : ch == 2 ? 8 : ch == 3 ? 26 : ch == 4 ? 34 : 40;
 }
 
+// Minecraft's 16 gradient vectors (standalone.hpp:40-49) as doubles: (gx, gy) pairs and gz in two
+// arrays, so that the hash-indexed 16-byte and 8-byte LDS reads of a wave hit distinct banks.
+struct GradTab {
+    double2 xy[16];
+    double z[16];
+};
 struct ClimOct {
     uint8_t perm[256];
     double ox, oy, oz, freq, weight, fy, vy;
     int iy, pad_;
 };
 struct ClimSeed {
+    GradTab g;
     ClimOct oct[CLIM_NOCT];
     double factor[6];
 };
@@ -47,7 +54,13 @@ struct SurfGrid {
 // ------------------------------------------------------------------ noise
 __device__ __forceinline__ double lerpd(double t, double a, double b) { return a + t * (b - a); }
 
-__device__ __forceinline__ double perlin_sample(const ClimOct &o, double x, double y, double z) {
+// grad() of dsx_rng.h with the components read from the table: the same products and sum order.
+__device__ __forceinline__ double grad_t(const GradTab &G, int h, double x, double y, double z) {
+    const double2 g = G.xy[h & 15];
+    return g.x * x + g.y * y + G.z[h & 15] * z;
+}
+
+__device__ __forceinline__ double perlin_sample(const GradTab &G, const ClimOct &o, double x, double y, double z) {
     x += o.ox;
     z += o.oz;
     const double bx = floor(x), bz = floor(z);
@@ -69,18 +82,33 @@ __device__ __forceinline__ double perlin_sample(const ClimOct &o, double x, doub
     const int b10 = P[(a1 + jy) & 255], b11 = P[(a1 + jy + 1) & 255];
     const double tx = fade(x), tz = fade(z);
     const double x1 = x - 1, y1 = dy - 1, z1 = z - 1;
-    const double l00 = lerpd(tx, grad(P[(b00 + iz) & 255], x, dy, z), grad(P[(b10 + iz) & 255], x1, dy, z));
-    const double l10 = lerpd(tx, grad(P[(b01 + iz) & 255], x, y1, z), grad(P[(b11 + iz) & 255], x1, y1, z));
-    const double l01 = lerpd(tx, grad(P[(b00 + iz + 1) & 255], x, dy, z1), grad(P[(b10 + iz + 1) & 255], x1, dy, z1));
-    const double l11 = lerpd(tx, grad(P[(b01 + iz + 1) & 255], x, y1, z1), grad(P[(b11 + iz + 1) & 255], x1, y1, z1));
+    const double l00 = lerpd(tx, grad_t(G, P[(b00 + iz) & 255], x, dy, z), grad_t(G, P[(b10 + iz) & 255], x1, dy, z));
+    const double l10 = lerpd(tx, grad_t(G, P[(b01 + iz) & 255], x, y1, z), grad_t(G, P[(b11 + iz) & 255], x1, y1, z));
+    const double l01 = lerpd(tx, grad_t(G, P[(b00 + iz + 1) & 255], x, dy, z1), grad_t(G, P[(b10 + iz + 1) & 255], x1, dy, z1));
+    const double l11 = lerpd(tx, grad_t(G, P[(b01 + iz + 1) & 255], x, y1, z1), grad_t(G, P[(b11 + iz + 1) & 255], x1, y1, z1));
     return lerpd(tz, lerpd(ty, l00, l10), lerpd(ty, l01, l11));
 }
 
 // The temperature octaves alone (slots 0-3), enough for the hot predicate.
 struct ClimT {
+    GradTab g;
     ClimOct oct[4];
     double factor[6];
 };
+
+static_assert(offsetof(ClimT, oct) == offsetof(ClimSeed, oct), "ClimT is a prefix of ClimSeed");
+static_assert(sizeof(ClimSeed) % 16 == 0 && offsetof(ClimT, factor) % 16 == 0, "uint4 copies");
+
+// Copy the part of a seed's climate that CS holds (all of it, or ClimT's prefix and the factors);
+// threads lane, lane + nt, ... of the block participate.
+template <class CS>
+__device__ __forceinline__ void copy_climate(CS &dst, const ClimSeed &src, int lane, int nt) {
+    const uint4 *s = reinterpret_cast<const uint4 *>(&src);
+    uint4 *d = reinterpret_cast<uint4 *>(&dst);
+    constexpr int pre = int(offsetof(CS, factor) / 16);
+    for (int k = lane; k < pre; k += nt) d[k] = s[k];
+    if (lane < 6) dst.factor[lane] = src.factor[lane];
+}
 
 // NormalNoise::sample: per-side sums, then (A + B) * factor. CS: ClimSeed, or ClimT for channel T.
 template <class CS>
@@ -92,7 +120,7 @@ __device__ __forceinline__ double noise_sample(const CS &C, int ch, double x, do
 #pragma unroll 1
         for (int j = 0; j < n; j++) {
             const ClimOct &o = C.oct[b + j];
-            sum0 += o.weight * perlin_sample(o, px * o.freq, py * o.freq, pz * o.freq);
+            sum0 += o.weight * perlin_sample(C.g, o, px * o.freq, py * o.freq, pz * o.freq);
         }
     }
     {
@@ -101,7 +129,7 @@ __device__ __forceinline__ double noise_sample(const CS &C, int ch, double x, do
 #pragma unroll 1
         for (int j = 0; j < n; j++) {
             const ClimOct &o = C.oct[b + n + j];
-            sum1 += o.weight * perlin_sample(o, px * o.freq, py * o.freq, pz * o.freq);
+            sum1 += o.weight * perlin_sample(C.g, o, px * o.freq, py * o.freq, pz * o.freq);
         }
     }
     return (sum0 + sum1) * C.factor[ch];
@@ -116,7 +144,7 @@ __device__ __forceinline__ double noise_octaves(const CS &C, int ch, double x, d
 #pragma unroll 1
         for (int k = 0; k < hi && k < n; k++) {
             const ClimOct &o = C.oct[b + side * n + k];
-            s += o.weight * perlin_sample(o, x * sc * o.freq, 0, z * sc * o.freq);
+            s += o.weight * perlin_sample(C.g, o, x * sc * o.freq, 0, z * sc * o.freq);
         }
     }
     return s * C.factor[ch];

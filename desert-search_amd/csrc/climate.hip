@@ -66,6 +66,11 @@ __global__ __launch_bounds__(64) void k_climinit(const uint64_t *seeds, uint32_t
         o.fy = oy - floor(oy);
         o.vy = fade(o.fy);
         if (g % CLIM_NOCT == 0) {
+            GradTab &G = out[g / CLIM_NOCT].g;
+            for (int h = 0; h < 16; h++) {
+                G.xy[h] = make_double2(grad_component(0, h), grad_component(1, h));
+                G.z[h] = grad_component(2, h);
+            }
             for (int ch = 0; ch < 6; ch++) {
                 const dsx_noise_cfg &c = DSX_NOISE[ch];
                 int lo = 0, hi = c.len - 1;

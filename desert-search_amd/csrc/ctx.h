@@ -17,7 +17,9 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
         if (e_ != hipSuccess) return dsx::hip_fail(e_, #x, __FILE__, __LINE__);      \
     } while (0)
 
-void dsx_ctx_free_ext(struct dsx_ctx *c);   // climate / flood state (floods_api.hip)
+void dsx_ctx_free_ext(struct dsx_ctx *c);      // climate / flood state (floods_api.hip)
+void dsx_ctx_free_workers(struct dsx_ctx *c);  // pipeline worker contexts (host/search.cpp)
+int dsx_ctx_device(const struct dsx_ctx *c);
 
 struct dsx_ctx {
     int device = 0;

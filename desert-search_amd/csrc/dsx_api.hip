@@ -60,6 +60,8 @@ dsx::GateKernel cpu_gate_kernel() {
 
 static int ctx_alloc(dsx_ctx *c);
 
+int dsx_ctx_device(const dsx_ctx *c) { return c->device; }
+
 extern "C" {
 
 const char *dsx_strerror(int code) {
@@ -121,6 +123,7 @@ static int ctx_alloc(dsx_ctx *c) {
 
 int dsx_destroy(dsx_ctx *c) {
     if (!c) return DSX_E_ARG;
+    dsx_ctx_free_workers(c);
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
     dsx_ctx_free_ext(c);

@@ -83,12 +83,14 @@ DSX_HD double fade(double t) { return t * t * t * (t * (t * 6. - 15.) + 10.); }
 
 // Minecraft's 16 gradient vectors as (gx*x + gy*y) + gz*z with components in {0, +-1}
 // (standalone.hpp:40-49). The components are decoded from bit masks so no local array is indexed.
+// Component a (0 x, 1 y, 2 z) of gradient h: bit h of a +1 mask and of a -1 mask.
+DSX_HD double grad_component(int a, unsigned h) {
+    constexpr unsigned pos[3] = {0x1055u, 0x5503u, 0x2330u}, neg[3] = {0x40AAu, 0xAA0Cu, 0x8CC0u};
+    return ((pos[a] >> h) & 1) ? 1.0 : (((neg[a] >> h) & 1) ? -1.0 : 0.0);
+}
 DSX_HD double grad(unsigned h, double x, double y, double z) {
     h &= 15;
-    // per h: gx = +1 mask / -1 mask, gy likewise, gz likewise
-    const double gx = ((0x1055u >> h) & 1) ? 1.0 : (((0x40AAu >> h) & 1) ? -1.0 : 0.0);
-    const double gy = ((0x5503u >> h) & 1) ? 1.0 : (((0xAA0Cu >> h) & 1) ? -1.0 : 0.0);
-    const double gz = ((0x2330u >> h) & 1) ? 1.0 : (((0x8CC0u >> h) & 1) ? -1.0 : 0.0);
+    const double gx = grad_component(0, h), gy = grad_component(1, h), gz = grad_component(2, h);
     return gx * x + gy * y + gz * z;
 }
 

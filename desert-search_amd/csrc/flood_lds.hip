@@ -46,11 +46,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
     const int lane = threadIdx.x;
     const int nw = 2 * J.half + 1, ncell = nw * nw;
     {   // climate (only the octaves the predicate reads when CS is ClimT) and cleared bitmaps
-        const uint4 *src = reinterpret_cast<const uint4 *>(&cs[J.seed]);
-        uint4 *dst = reinterpret_cast<uint4 *>(&C);
-        constexpr int noct = sizeof(C.oct) / 16;
-        for (int k = lane; k < noct; k += 64) dst[k] = src[k];
-        if (lane < 6) C.factor[lane] = cs[J.seed].factor[lane];
+        copy_climate(C, cs[J.seed], lane, 64);
         const int words = (ncell + 31) / 32;
         for (int k = lane; k < words; k += 64) {
             touched[k] = 0;

@@ -5,8 +5,8 @@
 // cells reachable from the start through predicate != 0 cells and on their neighbourhood, not on the
 // visiting order. This engine computes the same sets tile-parallel:
 //   * the window is split into 32x32 tiles, allocated on first touch through a device hash table;
-//   * k_flood_eval evaluates the FP64 predicate for all 1024 cells of each new tile (one 256-thread
-//     block per tile, the seed's 46 octaves in LDS);
+//   * k_flood_eval evaluates the FP64 predicate for all 1024 cells of each new tile (one 1024-thread
+//     block per tile, one cell per thread, the octaves the predicate reads in LDS);
 //   * k_flood_prop (one wave per tile, one lane per 32-bit row mask) grows the reached set inside the
 //     tile to a fixpoint with bit-parallel dilation, then ORs the touched cells it adds across the
 //     tile edges into the neighbours' `in` masks and queues those tiles;
@@ -125,10 +125,9 @@ __global__ __launch_bounds__(256) void k_flood_start(const FloodJob *jobs, const
 
 // Predicate of the 1024 cells of each listed tile: pass = v != 0, cnt = v == 2 (cells outside the window
 // or the world are neither).
-__global__ __launch_bounds__(256) void k_flood_eval(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B,
-                                                    const uint32_t *list, uint32_t n) {
+__global__ __launch_bounds__(FT * FT) void k_flood_eval(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B,
+                                                        const uint32_t *list, uint32_t n) {
     __shared__ ClimSeed C;
-    __shared__ uint32_t sp[FT], sc[FT];
     const uint32_t b = blockIdx.x;
     if (b >= n) return;
     Tile &T = B.tiles[list[b]];
@@ -137,35 +136,37 @@ __global__ __launch_bounds__(256) void k_flood_eval(const FloodJob *jobs, const 
     // 0-1 for the proxy, everything for the biome predicates
     const ClimSeed &G = cs[J.seed];
     constexpr int OW = sizeof(ClimOct) / 16;
+    const int tid = threadIdx.x;
     if (J.kind == PRED_HOT || J.kind == PRED_PROXY) {
         const int nslot = J.kind == PRED_HOT ? 4 : 14;
-        for (int k = threadIdx.x; k < nslot * OW; k += 256) {
+        for (int k = tid; k < nslot * OW; k += FT * FT) {
             const int slot = proxy_slot(k / OW);
             reinterpret_cast<uint4 *>(&C.oct[slot])[k % OW] = reinterpret_cast<const uint4 *>(&G.oct[slot])[k % OW];
         }
-        if (threadIdx.x < 6) C.factor[threadIdx.x] = G.factor[threadIdx.x];
+        constexpr int GW = sizeof(GradTab) / 16;
+        if (tid < GW) reinterpret_cast<uint4 *>(&C.g)[tid] = reinterpret_cast<const uint4 *>(&G.g)[tid];
+        if (tid < 6) C.factor[tid] = G.factor[tid];
     } else {
         const uint4 *src = reinterpret_cast<const uint4 *>(&G);
         uint4 *dst = reinterpret_cast<uint4 *>(&C);
-        for (int k = threadIdx.x; k < int(sizeof(ClimSeed) / 16); k += 256) dst[k] = src[k];
+        for (int k = tid; k < int(sizeof(ClimSeed) / 16); k += FT * FT) dst[k] = src[k];
     }
-    if (threadIdx.x < FT) { sp[threadIdx.x] = 0; sc[threadIdx.x] = 0; }
     __syncthreads();
+    // one cell per thread; wave w holds rows 2w (lanes 0-31) and 2w + 1 (lanes 32-63)
     const int nw = int(nwin(J));
-    for (int e = threadIdx.x; e < FT * FT; e += 256) {
-        const int r = e / FT, col = e % FT;
-        const int i = T.tx * FT + col, j = T.tz * FT + r;
-        if (i >= nw || j >= nw) continue;
+    const int r = tid / FT, col = tid % FT;
+    const int i = T.tx * FT + col, j = T.tz * FT + r;
+    int v = 0;
+    if (i < nw && j < nw) {
         const int x = J.qx + (i - J.half) * J.step, z = J.qz + (j - J.half) * J.step;
-        if (x < -WORLD_Q || x > WORLD_Q || z < -WORLD_Q || z > WORLD_Q) continue;
-        const int v = predicate(J.kind, C, g, x, z, J.cut);
-        if (v != 0) atomicOr(&sp[r], 1u << col);
-        if (v == 2) atomicOr(&sc[r], 1u << col);
+        if (x >= -WORLD_Q && x <= WORLD_Q && z >= -WORLD_Q && z <= WORLD_Q) v = predicate(J.kind, C, g, x, z, J.cut);
     }
-    __syncthreads();
-    if (threadIdx.x < FT) {
-        T.pass[threadIdx.x] = sp[threadIdx.x];
-        T.cnt[threadIdx.x] = sc[threadIdx.x];
+    const uint64_t mp = __ballot(v != 0), mc = __ballot(v == 2);
+    if ((tid & 63) == 0) {
+        T.pass[r] = uint32_t(mp);
+        T.pass[r + 1] = uint32_t(mp >> 32);
+        T.cnt[r] = uint32_t(mc);
+        T.cnt[r + 1] = uint32_t(mc >> 32);
     }
 }
 
@@ -383,7 +384,7 @@ void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl
 }
 void launch_flood_eval(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
                        const uint32_t *list, uint32_t n) {
-    if (n) hipLaunchKernelGGL(k_flood_eval, dim3(n), dim3(256), 0, st, jobs, cs, g, B, list, n);
+    if (n) hipLaunchKernelGGL(k_flood_eval, dim3(n), dim3(FT * FT), 0, st, jobs, cs, g, B, list, n);
 }
 void launch_flood_prop(hipStream_t st, const FloodJob *jobs, FloodBufs B, const uint32_t *list, uint32_t n,
                        uint32_t *next_eval, uint32_t *next_prop) {

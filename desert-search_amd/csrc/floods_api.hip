@@ -322,8 +322,9 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         }
     };
     static ProfOut prof_out{&prof_t, &prof_rounds, &prof_calls, &prof_tiles, &prof_sampled};
+    static std::mutex prof_mu;
+    double my_rounds = 0, my_tiles = 0, my_sampled = 0;
     const auto prof_t0 = std::chrono::steady_clock::now();
-    prof_calls++;
     int cur = 0;   // lists[2*cur] eval, lists[2*cur+1] prop
     uint32_t *h = c->h_cnt;
     int status = DSX_OK;
@@ -333,7 +334,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         if (h[4]) { status = DSX_E_CAPACITY; break; }
         const uint32_t ne = h[0], np = h[1];
         if (np == 0 || round > 4000000) break;
-        prof_rounds++;
+        my_rounds++;
         uint32_t *le = B.lists[2 * cur], *lp = B.lists[2 * cur + 1];
         uint32_t *nle = B.lists[2 * (1 - cur)], *nlp = B.lists[2 * (1 - cur) + 1];
         // next-round counters start at zero; this round's lists are consumed
@@ -349,7 +350,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     HIPCK(hipMemcpyAsync(&ntiles, B.ntiles, 4, hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
     ntiles = std::min(ntiles, B.tile_cap);
-    prof_tiles += ntiles;
+    my_tiles += ntiles;
     if (status == DSX_OK) {
         dsx::launch_flood_acc(c->st, S->d_jobs, B, ntiles, S->d_res, S->d_masks, B.lcount + 5, S->mask_cap);
         HIPCK(hipGetLastError());
@@ -367,7 +368,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
                                  hipMemcpyDeviceToHost, c->st));
             HIPCK(hipStreamSynchronize(c->st));
         }
-        for (uint64_t k = 0; k < n; k++) prof_sampled += double(res[k].sampled);
+        for (uint64_t k = 0; k < n; k++) my_sampled += double(res[k].sampled);
         for (uint64_t k = 0; k < n; k++)
             out[k] = dsx_area{res[k].cells, res[k].sampled, res[k].minx, res[k].maxx, res[k].minz, res[k].maxz,
                               res[k].edge, res[k].border};
@@ -381,7 +382,14 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         HIPCK(hipMemsetAsync(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8, c->st));
     }
     HIPCK(hipStreamSynchronize(c->st));
-    prof_t += std::chrono::duration<double>(std::chrono::steady_clock::now() - prof_t0).count();
+    {
+        std::lock_guard<std::mutex> l(prof_mu);
+        prof_t += std::chrono::duration<double>(std::chrono::steady_clock::now() - prof_t0).count();
+        prof_calls++;
+        prof_rounds += my_rounds;
+        prof_tiles += my_tiles;
+        prof_sampled += my_sampled;
+    }
     return status;
 }
 

@@ -60,7 +60,8 @@ class SearchParams(ctypes.Structure):
          ("csv_min", ctypes.c_double), ("nms", ctypes.c_int32), ("refine_mb", ctypes.c_double),
          ("verify_mb", ctypes.c_double)] + \
         [(n, ctypes.c_int32) for n in ("study", "gpu", "gpucheck", "gscan", "gscancheck", "gate")] + \
-        [(n, ctypes.c_double) for n in ("gate_phase", "gate_amp", "gate_pthr")] + [("from_stdin", ctypes.c_int32)]
+        [(n, ctypes.c_double) for n in ("gate_phase", "gate_amp", "gate_pthr")] + [("from_stdin", ctypes.c_int32),
+                                                                               ("workers", ctypes.c_int32)]
 
 
 class SearchCounts(ctypes.Structure):

@@ -13,7 +13,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <coroutine>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -25,6 +29,12 @@
 #include <unordered_set>
 #include <utility>
 #include <vector>
+
+namespace dsx {
+extern thread_local std::string g_last_error;
+}
+int dsx_ctx_device(const dsx_ctx *c);
+void dsx_ctx_free_workers(dsx_ctx *c);
 
 namespace {
 
@@ -139,11 +149,17 @@ struct Prof {
     }
 };
 Prof g_prof;
+std::mutex g_prof_mu;
+void prof_add(int k, double t, uint64_t n) {
+    std::lock_guard<std::mutex> l(g_prof_mu);
+    g_prof.t[k] += t;
+    g_prof.n[k] += n;
+}
 struct Timer {
     int k;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     explicit Timer(int k_) : k(k_) {}
-    ~Timer() { g_prof.t[k] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); g_prof.n[k]++; }
+    ~Timer() { prof_add(k, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), 1); }
 };
 
 struct Sched {
@@ -277,14 +293,16 @@ struct Sched {
         std::vector<FloodReq *> fs;
         ps.swap(points);
         fs.swap(floods);
-        g_prof.n[6]++;
+        prof_add(6, 0, 1);
         if (!ps.empty()) {
             Timer t(3);
             err = run_points(ps);
         }
         if (!err && !fs.empty()) {
             Timer t(4);
-            for (auto *r : fs) g_prof.n[7] += r->jobs.size();
+            uint64_t nj = 0;
+            for (auto *r : fs) nj += r->jobs.size();
+            prof_add(7, 0, nj);
             err = run_floods(fs);
         }
         if (err) return false;
@@ -629,6 +647,181 @@ void make_sites(const std::vector<dsx_raw_cand> &raw, const dsx_search_params &P
     }
 }
 
+// ------------------------------------------------------------------ groups of seeds on worker contexts
+// A group: consecutive seeds of the batch, processed by one worker (tables + scan, sites, hot zones,
+// proxy and exact floods) on its own context; the rows are written by the caller in group order.
+struct Group {
+    std::vector<uint64_t> seeds;
+    std::vector<std::unique_ptr<SeedOut>> outs;
+    int rc = DSX_OK;
+    std::string err;
+    bool done = false;
+};
+
+int run_group(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g) {
+    const dsx_scan_params sp{float(P.ta), float(P.t), float(P.wc), float(P.we), P.sigma, P.sigmac};
+    const size_t nb = g.seeds.size();
+    auto tscan = std::chrono::steady_clock::now();
+    std::vector<float> mta(nb), mtb(nb);
+    std::vector<uint64_t> ncand(nb);
+    uint64_t cap = 1 << 12;
+    std::vector<dsx_raw_cand> raw(nb * cap);
+    int rc = dsx_scan(w, g.seeds.data(), nb, &sp, mta.data(), mtb.data(), raw.data(), cap, ncand.data());
+    if (rc == DSX_E_CAPACITY) {
+        cap = *std::max_element(ncand.begin(), ncand.end());
+        raw.assign(nb * cap, dsx_raw_cand{});
+        rc = dsx_scan(w, g.seeds.data(), nb, &sp, mta.data(), mtb.data(), raw.data(), cap, ncand.data());
+    }
+    if (rc) return rc;
+    for (size_t k = 0; k < nb; k++) {
+        auto O = std::make_unique<SeedOut>();
+        O->seed = g.seeds[k];
+        dsx::temperature_phase(O->seed, O->fy);   // fyA, fyB (dsearch_desert.cpp:80-94)
+        O->maxta = mta[k];
+        O->maxtb = mtb[k];
+        if (!(mode == DSX_SEARCH_ENUM && O->maxta + O->maxtb < P.amp)) {
+            O->scanned = true;
+            std::vector<dsx_raw_cand> r(raw.begin() + k * cap, raw.begin() + k * cap + ncand[k]);
+            make_sites(r, P, *O);
+        }
+        g.outs.push_back(std::move(O));
+    }
+    auto tsites = std::chrono::steady_clock::now();
+    prof_add(1, std::chrono::duration<double>(tsites - tscan).count(), 0);
+    // hot zones, proxy floods, exact floods: all scanned seeds of the group as coroutines
+    Sched S{w, {}, {}, DSX_OK, {}};
+    std::vector<Task> tasks;
+    for (auto &O : g.outs)
+        if (O->scanned) tasks.push_back(seed_task(S, P, *O));
+    while (S.step()) {
+    }
+    for (auto &t : tasks) t.h.destroy();
+    prof_add(2, std::chrono::duration<double>(std::chrono::steady_clock::now() - tsites).count(), 0);
+    return S.err;
+}
+
+// Rows of one group (dsearch_desert.cpp:251, 526-548), in seed order.
+void write_group(const Group &g, const dsx_search_params &P, FILE *out, FILE *sout, dsx_search_counts *counts,
+                 double &global_best) {
+    for (auto &O : g.outs) {
+        char line[512];
+        if (!O->scanned) {
+            std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,0,0,0\n", (long long)O->seed, O->fy[0], O->fy[1],
+                          O->maxta, O->maxtb);
+            std::fputs(line, sout);
+            continue;
+        }
+        counts->scanned++;
+        counts->sites += O->sites.size();
+        for (auto &l : O->lines) std::fprintf(stderr, "%s\n", l.c_str());
+        for (size_t k = 0; k < O->bests.size(); k++)
+            if (O->bests[k] > global_best) {
+                global_best = O->bests[k];
+                std::fprintf(stderr, "best %.2fM (step %d) seed=%lld x=%d z=%d proxy=%.1fM\n", O->bests[k], O->best_step[k],
+                             (long long)O->seed, O->best_xy[k].first, O->best_xy[k].second, O->best_proxy[k]);
+            }
+        for (auto &e : O->ev) {
+            if (e.proxy < P.csv_min && e.s8 <= 0 && e.s4 <= 0) continue;
+            const Site &st = O->sites[e.rank];
+            std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,%zu,%d,%d,%.4f,%.4f,%.4f,%.4f,%.3f,%.3f,%.3f,%d,%.1f\n",
+                          (long long)O->seed, O->fy[0], O->fy[1], O->maxta, O->maxtb, e.rank, e.x * 4 + 2, e.z * 4 + 2, st.s,
+                          st.t, st.c, st.e, e.proxy, e.s8, e.s4, int(e.edge), e.hot);
+            std::fputs(line, out);
+        }
+        std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,%zu,%zu,%.3f\n", (long long)O->seed, O->fy[0], O->fy[1],
+                      O->maxta, O->maxtb, O->ncand, O->sites.size(), O->best);
+        std::fputs(line, sout);
+    }
+}
+
+// Worker contexts of a caller's context, created on first use and kept until dsx_destroy.
+std::mutex g_workers_mu;
+std::unordered_map<dsx_ctx *, std::vector<dsx_ctx *>> g_workers;
+
+int worker_contexts(dsx_ctx *ctx, int n, std::vector<dsx_ctx *> &out) {
+    std::lock_guard<std::mutex> l(g_workers_mu);
+    auto &v = g_workers[ctx];
+    while (int(v.size()) < n) {
+        dsx_ctx *w = nullptr;
+        const int rc = dsx_create(dsx_ctx_device(ctx), &w);
+        if (rc) return rc;
+        v.push_back(w);
+    }
+    out.assign(v.begin(), v.begin() + n);
+    return DSX_OK;
+}
+
+// Runs groups on worker threads; the caller submits groups and collects them in submission order.
+class Pool {
+  public:
+    Pool(const std::vector<dsx_ctx *> &ws, const dsx_search_params &P, int mode) : P_(P), mode_(mode) {
+        for (dsx_ctx *w : ws) th_.emplace_back([this, w] { loop(w); });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            closing_ = true;
+        }
+        cv_work_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void submit(Group *g) {
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            q_.push_back(g);
+        }
+        cv_work_.notify_one();
+    }
+    void wait(Group *g) {
+        std::unique_lock<std::mutex> l(mu_);
+        cv_done_.wait(l, [g] { return g->done; });
+    }
+    bool ready(Group *g) {
+        std::lock_guard<std::mutex> l(mu_);
+        return g->done;
+    }
+    void stop() {   // groups not yet started finish at once with an error
+        std::lock_guard<std::mutex> l(mu_);
+        stop_ = true;
+    }
+
+  private:
+    void loop(dsx_ctx *w) {
+        for (;;) {
+            Group *g;
+            bool skip;
+            {
+                std::unique_lock<std::mutex> l(mu_);
+                cv_work_.wait(l, [this] { return closing_ || !q_.empty(); });
+                if (q_.empty()) return;
+                g = q_.front();
+                q_.pop_front();
+                skip = stop_;
+            }
+            int rc = DSX_E_ARG;
+            std::string err = "search stopped after an earlier error";
+            if (!skip) {
+                rc = run_group(w, P_, mode_, *g);
+                err = rc ? dsx_last_error() : "";
+            }
+            {
+                std::lock_guard<std::mutex> l(mu_);
+                g->rc = rc;
+                g->err = err;
+                g->done = true;
+            }
+            cv_done_.notify_all();
+        }
+    }
+    const dsx_search_params P_;
+    const int mode_;
+    std::mutex mu_;
+    std::condition_variable cv_work_, cv_done_;
+    std::deque<Group *> q_;
+    bool closing_ = false, stop_ = false;
+    std::vector<std::thread> th_;
+};
+
 }  // namespace
 
 extern "C" {
@@ -648,143 +841,111 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
     if (!ctx || !pp || !prefix || !counts || mode < 0 || mode > 2 || (mode == DSX_SEARCH_LIST && nseeds && !seeds))
         return DSX_E_ARG;
     const dsx_search_params P = *pp;
-    if (P.study || P.nms <= 0 || P.max_sites < 0) return DSX_E_ARG;
+    if (P.study || P.nms <= 0 || P.max_sites < 0 || P.workers < 0 || P.workers > 64) return DSX_E_ARG;
+    const int nw = P.workers ? P.workers : 4;
+    std::vector<dsx_ctx *> wctx;
+    int rc = worker_contexts(ctx, nw, wctx);
+    if (rc) return rc;
     const std::string pre(prefix);
     FILE *out = std::fopen((pre + ".csv").c_str(), "a");
     FILE *sout = std::fopen((pre + ".seeds.csv").c_str(), "a");
     if (!out || !sout) {
         if (out) std::fclose(out);
         if (sout) std::fclose(sout);
+        dsx::g_last_error = "cannot open " + pre + ".csv / .seeds.csv";
         return DSX_E_ARG;
     }
-    int rc = DSX_OK;
     static double global_best = 0;
-    const dsx_scan_params sp{float(P.ta), float(P.t), float(P.wc), float(P.we), P.sigma, P.sigmac};
-    // seed source in chunks: (seed list, CPU-default gate applies?)
-    const uint64_t CHUNK = 1ull << 24;
-    uint64_t pos = 0;
-    const uint64_t total = mode == DSX_SEARCH_LIST ? nseeds : count;
-    std::vector<uint64_t> batch;
-    while (pos < total && rc == DSX_OK) {
-        const uint64_t m = std::min(CHUNK, total - pos);
-        batch.clear();
-        auto tgate = std::chrono::steady_clock::now();
-        if (mode == DSX_SEARCH_LIST) {
-            batch.assign(seeds + pos, seeds + pos + m);
-            counts->done += m;
-            counts->passed += m;
-        } else if (mode == DSX_SEARCH_GATE) {
-            dsx_gate_params gp{float(P.gate_phase), float(P.gate_amp), float(P.gate_pthr), 0.01f, 1.0f, 1};
-            uint64_t n = 0;
-            dsx_gate_counts gc;
-            batch.resize(1 << 16);
-            rc = dsx_gate(ctx, start + pos, m, &gp, batch.data(), nullptr, nullptr, batch.size(), &n, &gc);
-            if (rc == DSX_E_CAPACITY) {
+    std::deque<std::unique_ptr<Group>> pending;   // submitted, in seed order
+    std::string first_err;
+    {
+        Pool pool(wctx, P, mode);
+        // write the finished groups at the front, waiting for them while more than `keep` are pending
+        auto drain = [&](size_t keep) {
+            while (!pending.empty()) {
+                Group *g = pending.front().get();
+                if (pending.size() > keep) pool.wait(g);
+                else if (!pool.ready(g)) break;
+                if (g->rc && rc == DSX_OK) {
+                    rc = g->rc;
+                    first_err = g->err;
+                    pool.stop();
+                }
+                if (rc == DSX_OK) write_group(*g, P, out, sout, counts, global_best);
+                pending.pop_front();
+            }
+        };
+        // seed source in chunks
+        const uint64_t CHUNK = 1ull << 24;
+        const uint64_t total = mode == DSX_SEARCH_LIST ? nseeds : count;
+        std::vector<uint64_t> batch;
+        for (uint64_t pos = 0; pos < total && rc == DSX_OK;) {
+            const uint64_t m = std::min(CHUNK, total - pos);
+            batch.clear();
+            auto tgate = std::chrono::steady_clock::now();
+            if (mode == DSX_SEARCH_LIST) {
+                batch.assign(seeds + pos, seeds + pos + m);
+                counts->done += m;
+                counts->passed += m;
+            } else if (mode == DSX_SEARCH_GATE) {
+                dsx_gate_params gp{float(P.gate_phase), float(P.gate_amp), float(P.gate_pthr), 0.01f, 1.0f, 1};
+                uint64_t n = 0;
+                dsx_gate_counts gc;
+                batch.resize(1 << 16);
+                int r = dsx_gate(ctx, start + pos, m, &gp, batch.data(), nullptr, nullptr, batch.size(), &n, &gc);
+                if (r == DSX_E_CAPACITY) {
+                    batch.resize(n);
+                    r = dsx_gate(ctx, start + pos, m, &gp, batch.data(), nullptr, nullptr, batch.size(), &n, &gc);
+                }
+                if (r) { rc = r; first_err = dsx_last_error(); break; }
                 batch.resize(n);
-                rc = dsx_gate(ctx, start + pos, m, &gp, batch.data(), nullptr, nullptr, batch.size(), &n, &gc);
-            }
-            if (rc) break;
-            batch.resize(n);
-            counts->done += n;
-            counts->passed += n;
-        } else {
-            uint64_t n = 0, nph = 0;
-            batch.resize(1 << 18);
-            rc = dsx_gate_cpu(ctx, start + pos, m, P.max_phase, P.amp, P.amp_margin, batch.data(), nullptr, nullptr,
-                              batch.size(), &n, &nph);
-            if (rc == DSX_E_CAPACITY) {
+                counts->done += n;
+                counts->passed += n;
+            } else {
+                uint64_t n = 0, nph = 0;
+                batch.resize(1 << 18);
+                int r = dsx_gate_cpu(ctx, start + pos, m, P.max_phase, P.amp, P.amp_margin, batch.data(), nullptr, nullptr,
+                                     batch.size(), &n, &nph);
+                if (r == DSX_E_CAPACITY) {
+                    batch.resize(n);
+                    r = dsx_gate_cpu(ctx, start + pos, m, P.max_phase, P.amp, P.amp_margin, batch.data(), nullptr, nullptr,
+                                     batch.size(), &n, &nph);
+                }
+                if (r) { rc = r; first_err = dsx_last_error(); break; }
                 batch.resize(n);
-                rc = dsx_gate_cpu(ctx, start + pos, m, P.max_phase, P.amp, P.amp_margin, batch.data(), nullptr, nullptr,
-                                  batch.size(), &n, &nph);
+                counts->done += m;
+                counts->passed += nph;
             }
-            if (rc) break;
-            batch.resize(n);
-            counts->done += m;
-            counts->passed += nph;
+            pos += m;
+            prof_add(0, std::chrono::duration<double>(std::chrono::steady_clock::now() - tgate).count(), 0);
+            // groups: about two per worker for this chunk, 32 to 256 seeds each
+            const size_t sub = std::clamp<size_t>((batch.size() + 2 * nw - 1) / (2 * nw), 32, 256);
+            for (size_t b0 = 0; b0 < batch.size(); b0 += sub) {
+                auto g = std::make_unique<Group>();
+                g->seeds.assign(batch.begin() + b0, batch.begin() + std::min(batch.size(), b0 + sub));
+                pool.submit(g.get());
+                pending.push_back(std::move(g));
+            }
+            drain(size_t(8 * nw));   // bounded backlog; rows of finished groups go out early
         }
-        pos += m;
-        g_prof.t[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() - tgate).count();
-        // ---- tables + scan, in sub-batches of seeds processed together downstream
-        const size_t SUB = 512;
-        for (size_t b0 = 0; b0 < batch.size() && rc == DSX_OK; b0 += SUB) {
-            const size_t nb = std::min(SUB, batch.size() - b0);
-            auto tscan = std::chrono::steady_clock::now();
-            std::vector<float> mta(nb), mtb(nb);
-            std::vector<uint64_t> ncand(nb);
-            uint64_t cap = 1 << 12;
-            std::vector<dsx_raw_cand> raw(nb * cap);
-            rc = dsx_scan(ctx, batch.data() + b0, nb, &sp, mta.data(), mtb.data(), raw.data(), cap, ncand.data());
-            if (rc == DSX_E_CAPACITY) {
-                cap = *std::max_element(ncand.begin(), ncand.end());
-                raw.assign(nb * cap, dsx_raw_cand{});
-                rc = dsx_scan(ctx, batch.data() + b0, nb, &sp, mta.data(), mtb.data(), raw.data(), cap, ncand.data());
-            }
-            if (rc) break;
-            std::vector<std::unique_ptr<SeedOut>> outs;
-            for (size_t k = 0; k < nb; k++) {
-                auto O = std::make_unique<SeedOut>();
-                O->seed = batch[b0 + k];
-                dsx::temperature_phase(O->seed, O->fy);   // fyA, fyB (dsearch_desert.cpp:80-94)
-                O->maxta = mta[k];
-                O->maxtb = mtb[k];
-                if (mode == DSX_SEARCH_ENUM && O->maxta + O->maxtb < P.amp) {
-                    outs.push_back(std::move(O));
-                    continue;
-                }
-                O->scanned = true;
-                std::vector<dsx_raw_cand> r(raw.begin() + k * cap, raw.begin() + k * cap + ncand[k]);
-                make_sites(r, P, *O);
-                outs.push_back(std::move(O));
-            }
-            auto tsites = std::chrono::steady_clock::now();
-            g_prof.t[1] += std::chrono::duration<double>(tsites - tscan).count();
-            // ---- hot zones, proxy floods, exact floods: all scanned seeds as coroutines
-            Sched S{ctx, {}, {}, DSX_OK, {}};
-            std::vector<Task> tasks;
-            for (auto &O : outs)
-                if (O->scanned) tasks.push_back(seed_task(S, P, *O));
-            while (S.step()) {
-            }
-            rc = S.err;
-            for (auto &t : tasks) t.h.destroy();
-            g_prof.t[2] += std::chrono::duration<double>(std::chrono::steady_clock::now() - tsites).count();
-            if (rc) break;
-            // ---- rows (dsearch_desert.cpp:251, 526-548)
-            for (auto &O : outs) {
-                char line[512];
-                if (!O->scanned) {
-                    std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,0,0,0\n", (long long)O->seed, O->fy[0], O->fy[1],
-                                  O->maxta, O->maxtb);
-                    std::fputs(line, sout);
-                    continue;
-                }
-                counts->scanned++;
-                counts->sites += O->sites.size();
-                for (auto &l : O->lines) std::fprintf(stderr, "%s\n", l.c_str());
-                for (size_t k = 0; k < O->bests.size(); k++)
-                    if (O->bests[k] > global_best) {
-                        global_best = O->bests[k];
-                        std::fprintf(stderr, "best %.2fM (step %d) seed=%lld x=%d z=%d proxy=%.1fM\n", O->bests[k],
-                                     O->best_step[k], (long long)O->seed, O->best_xy[k].first, O->best_xy[k].second,
-                                     O->best_proxy[k]);
-                    }
-                for (auto &e : O->ev) {
-                    if (e.proxy < P.csv_min && e.s8 <= 0 && e.s4 <= 0) continue;
-                    const Site &st = O->sites[e.rank];
-                    std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,%zu,%d,%d,%.4f,%.4f,%.4f,%.4f,%.3f,%.3f,%.3f,%d,%.1f\n",
-                                  (long long)O->seed, O->fy[0], O->fy[1], O->maxta, O->maxtb, e.rank, e.x * 4 + 2, e.z * 4 + 2,
-                                  st.s, st.t, st.c, st.e, e.proxy, e.s8, e.s4, int(e.edge), e.hot);
-                    std::fputs(line, out);
-                }
-                std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,%zu,%zu,%.3f\n", (long long)O->seed, O->fy[0],
-                              O->fy[1], O->maxta, O->maxtb, O->ncand, O->sites.size(), O->best);
-                std::fputs(line, sout);
-            }
-        }
+        drain(0);
     }
     std::fclose(out);
     std::fclose(sout);
+    if (rc) dsx::g_last_error = first_err;
     return rc;
 }
 
 }  // extern "C"
+
+void dsx_ctx_free_workers(dsx_ctx *c) {
+    std::vector<dsx_ctx *> v;
+    {
+        std::lock_guard<std::mutex> l(g_workers_mu);
+        auto it = g_workers.find(c);
+        if (it == g_workers.end()) return;
+        v.swap(it->second);
+        g_workers.erase(it);
+    }
+    for (dsx_ctx *w : v) dsx_destroy(w);
+}
