@@ -19,7 +19,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared", "
              "-Wno-unused-result"]
 HIP_SOURCES = ["csrc/gate.hip", "csrc/tables.hip", "csrc/climate.hip", "csrc/floods.hip", "csrc/flood_lds.hip",
                "csrc/dsx_api.hip",
-               "csrc/scan_api.hip", "csrc/floods_api.hip", "host/search.cpp"]
+               "csrc/scan_api.hip", "csrc/floods_api.hip", "host/search.cpp", "host/serve.cpp"]
 HOST_TOOLS = ["host/dsearch_desert.cpp"]
 
 

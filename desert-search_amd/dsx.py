@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdsx.so")
 
-DSX_OK, DSX_E_ARG, DSX_E_HIP, DSX_E_CAPACITY, DSX_E_NODEVICE = 0, 1, 2, 3, 4
+DSX_OK, DSX_E_ARG, DSX_E_HIP, DSX_E_CAPACITY, DSX_E_NODEVICE, DSX_E_BUSY = 0, 1, 2, 3, 4, 5
 
 
 class DsxError(RuntimeError):
@@ -50,6 +50,7 @@ FLOOD_JOB = np.dtype([("seed", "<u8"), ("kind", "<i4"), ("cut", "<f4"), ("qx", "
 AREA = np.dtype([("cells", "<u8"), ("sampled", "<u8"), ("minx", "<i4"), ("maxx", "<i4"), ("minz", "<i4"),
                  ("maxz", "<i4"), ("edge", "<i4"), ("border", "<i4")])
 PRED_DESERT_STATE, PRED_HOT, PRED_PROXY, PRED_DESERT = 0, 1, 2, 3
+AREA_QUERY = np.dtype([("seed", "<u8"), ("x", "<i4"), ("z", "<i4"), ("half", "<i4"), ("pad", "<i4")])
 
 
 class SearchParams(ctypes.Structure):
@@ -70,6 +71,23 @@ class SearchCounts(ctypes.Structure):
 
 
 SEARCH_ENUM, SEARCH_GATE, SEARCH_LIST = 0, 1, 2
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("workers", ctypes.c_int32), ("area_batch", ctypes.c_int32), ("hbm_budget", ctypes.c_uint64),
+                ("params", SearchParams)]
+
+
+class Area(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("cells", "sampled")] + \
+        [(n, ctypes.c_int32) for n in ("minx", "maxx", "minz", "maxz", "edge", "border")]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("kind", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("area", Area), ("counts", SearchCounts), ("csv_bytes", ctypes.c_uint64),
+                ("seeds_bytes", ctypes.c_uint64)]
 
 
 def flood_jobs(rows):
@@ -124,8 +142,23 @@ def load(path=LIB_PATH):
     lib.dsx_search.argtypes = [ctypes.c_void_p, ctypes.POINTER(SearchParams), ctypes.c_int, ctypes.c_uint64,
                                ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_char_p,
                                ctypes.POINTER(SearchCounts)]
+    lib.dsx_area_queries.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+    lib.dsx_area_workload.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]
+    lib.dsx_area_workload.restype = None
+    lib.dsx_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerConfig), ctypes.POINTER(ctypes.c_void_p)]
+    lib.dsx_server_destroy.argtypes = [ctypes.c_void_p]
+    lib.dsx_server_workers.argtypes = [ctypes.c_void_p]
+    lib.dsx_submit_scan.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, _u64p]
+    lib.dsx_submit_area.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p]
+    lib.dsx_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(Result)]
+    lib.dsx_poll.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(Result)]
+    lib.dsx_result_text.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p,
+                                    ctypes.c_uint64]
+    lib.dsx_release.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
     for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile", "dsx_scan", "dsx_tables",
-              "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood", "dsx_search"):
+              "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood", "dsx_search", "dsx_area_queries",
+              "dsx_server_create", "dsx_server_destroy", "dsx_server_workers", "dsx_submit_scan", "dsx_submit_area",
+              "dsx_wait", "dsx_poll", "dsx_result_text", "dsx_release"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -233,6 +266,13 @@ class Engine:
                                             out.ctypes.data), "dsx_predicates")
         return out
 
+    def area_queries(self, queries):
+        """queries: AREA_QUERY array -> AREA array (dsx_area_queries)"""
+        q = np.ascontiguousarray(queries, dtype=AREA_QUERY)
+        out = np.zeros(len(q), AREA)
+        self._check(self.lib.dsx_area_queries(self.h, q.ctypes.data, len(q), out.ctypes.data), "dsx_area_queries")
+        return out
+
     def flood(self, jobs):
         """jobs: FLOOD_JOB array (see flood_jobs) -> AREA array"""
         j = np.ascontiguousarray(jobs, dtype=FLOOD_JOB)
@@ -259,3 +299,81 @@ class Engine:
         self._check(self.lib.dsx_search(self.h, ctypes.byref(p), mode, start, count, s.ctypes.data if len(s) else None,
                                         len(s), prefix.encode(), ctypes.byref(c)), "dsx_search")
         return (c.done, c.passed, c.scanned, c.sites)
+
+
+def area_workload(n=4096, gen_seed=0xD5E27C4):
+    """dsx_area_workload: (AREA_QUERY array, caps) of the synthetic area-query workload."""
+    lib = load()
+    q = np.zeros(n, AREA_QUERY)
+    caps = np.zeros(n, np.uint64)
+    lib.dsx_area_workload(gen_seed, n, q.ctypes.data, caps.ctypes.data)
+    return q, caps
+
+
+class Server:
+    """dsx_server: scans and area queries submitted at any time, served by worker contexts with
+    continuous batching. Thread-safe."""
+
+    def __init__(self, device=0, workers=0, area_batch=0, hbm_budget=0, params=None):
+        self.lib = load()
+        cfg = ServerConfig()
+        cfg.workers, cfg.area_batch, cfg.hbm_budget = workers, area_batch, hbm_budget
+        if params is None:
+            params = SearchParams()
+            self.lib.dsx_search_defaults(ctypes.byref(params))
+        cfg.params = params
+        h = ctypes.c_void_p()
+        self._check(self.lib.dsx_server_create(device, ctypes.byref(cfg), ctypes.byref(h)), "dsx_server_create")
+        self.h = h
+        self.workers = self.lib.dsx_server_workers(h)
+
+    def _check(self, rc, what):
+        if rc != DSX_OK:
+            raise DsxError(rc, what, (self.lib.dsx_strerror(rc) + b" " + self.lib.dsx_last_error()).decode())
+
+    def submit_scan(self, start, count):
+        i = ctypes.c_uint64()
+        self._check(self.lib.dsx_submit_scan(self.h, start, count, ctypes.byref(i)), "dsx_submit_scan")
+        return i.value
+
+    def submit_area(self, seed, x, z, half):
+        q = np.zeros(1, AREA_QUERY)
+        q[0] = (seed, x, z, half, 0)
+        i = ctypes.c_uint64()
+        self._check(self.lib.dsx_submit_area(self.h, q.ctypes.data, ctypes.byref(i)), "dsx_submit_area")
+        return i.value
+
+    def wait(self, rid):
+        r = Result()
+        self._check(self.lib.dsx_wait(self.h, rid, ctypes.byref(r)), "dsx_wait")
+        return r
+
+    def poll(self, rid):
+        r = Result()
+        rc = self.lib.dsx_poll(self.h, rid, ctypes.byref(r))
+        if rc == DSX_E_BUSY:
+            return None
+        self._check(rc, "dsx_poll")
+        return r
+
+    def text(self, rid):
+        """(csv rows, seeds rows) of a finished scan"""
+        r = self.wait(rid)
+        csv = ctypes.create_string_buffer(max(1, r.csv_bytes))
+        seeds = ctypes.create_string_buffer(max(1, r.seeds_bytes))
+        self._check(self.lib.dsx_result_text(self.h, rid, csv, r.csv_bytes, seeds, r.seeds_bytes), "dsx_result_text")
+        return csv.raw[:r.csv_bytes].decode(), seeds.raw[:r.seeds_bytes].decode()
+
+    def release(self, rid):
+        self._check(self.lib.dsx_release(self.h, rid), "dsx_release")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dsx_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -28,6 +28,7 @@ int main(int argc, char **argv) {
     const int threads = argc > 4 ? std::stoi(argv[4]) : 11;
     dsx_search_params P;
     dsx_search_defaults(&P);
+    P.workers = threads < 1 ? 1 : (threads > 64 ? 64 : threads);   // THREADS -> pipeline workers
     for (int i = 5; i < argc; i++) {
         const std::string kv = argv[i];
         const auto eq = kv.find('=');

@@ -1,0 +1,36 @@
+// Host pipeline internals shared by dsx_search (search.cpp) and the request server (serve.cpp).
+#pragma once
+#include "../../include/dsx.h"
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace dsxh {
+
+struct SeedOut;
+
+// Consecutive gated seeds processed together by one worker context: tables + scan, sites, hot zones,
+// proxy and exact floods (dsearch_desert.cpp:173-557).
+struct Group {
+    std::vector<uint64_t> seeds;
+    std::vector<std::unique_ptr<SeedOut>> outs;
+    int rc = DSX_OK;
+    std::string err;
+    bool done = false;
+    Group();
+    ~Group();
+};
+int run_group(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g);
+
+// The group's output in the reference's formats: PREFIX.csv rows, PREFIX.seeds.csv rows, and the
+// stderr lines (CANDIDATE; "best" lines when `best` is given: it holds the running best).
+struct GroupText {
+    std::string csv, seeds, log;
+    uint64_t scanned = 0, sites = 0;
+};
+void format_group(const Group &g, const dsx_search_params &P, GroupText &t, double *best);
+
+int worker_contexts(dsx_ctx *ctx, int n, std::vector<dsx_ctx *> &out);
+uint64_t env_u64(const char *name, uint64_t def);
+
+}  // namespace dsxh

@@ -10,6 +10,7 @@
 // (hotcovered, covered, seen, std::sort of sites and proxies) is kept exactly as in the reference.
 #include "../../include/dsx.h"
 #include "../csrc/dsx_rng.h"
+#include "pipeline.h"
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -36,7 +37,7 @@ extern thread_local std::string g_last_error;
 int dsx_ctx_device(const dsx_ctx *c);
 void dsx_ctx_free_workers(dsx_ctx *c);
 
-namespace {
+namespace dsxh {
 
 struct Site { float s, t, c, e; int x, z; };
 struct Eval {
@@ -648,15 +649,8 @@ void make_sites(const std::vector<dsx_raw_cand> &raw, const dsx_search_params &P
 }
 
 // ------------------------------------------------------------------ groups of seeds on worker contexts
-// A group: consecutive seeds of the batch, processed by one worker (tables + scan, sites, hot zones,
-// proxy and exact floods) on its own context; the rows are written by the caller in group order.
-struct Group {
-    std::vector<uint64_t> seeds;
-    std::vector<std::unique_ptr<SeedOut>> outs;
-    int rc = DSX_OK;
-    std::string err;
-    bool done = false;
-};
+Group::Group() = default;
+Group::~Group() = default;
 
 int run_group(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g) {
     const dsx_scan_params sp{float(P.ta), float(P.t), float(P.wc), float(P.we), P.sigma, P.sigmac};
@@ -700,25 +694,25 @@ int run_group(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g) {
     return S.err;
 }
 
-// Rows of one group (dsearch_desert.cpp:251, 526-548), in seed order.
-void write_group(const Group &g, const dsx_search_params &P, FILE *out, FILE *sout, dsx_search_counts *counts,
-                 double &global_best) {
+void format_group(const Group &g, const dsx_search_params &P, GroupText &t, double *best) {
+    char line[512];
     for (auto &O : g.outs) {
-        char line[512];
         if (!O->scanned) {
             std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,0,0,0\n", (long long)O->seed, O->fy[0], O->fy[1],
                           O->maxta, O->maxtb);
-            std::fputs(line, sout);
+            t.seeds += line;
             continue;
         }
-        counts->scanned++;
-        counts->sites += O->sites.size();
-        for (auto &l : O->lines) std::fprintf(stderr, "%s\n", l.c_str());
+        t.scanned++;
+        t.sites += O->sites.size();
+        for (auto &l : O->lines) t.log += l + "\n";
         for (size_t k = 0; k < O->bests.size(); k++)
-            if (O->bests[k] > global_best) {
-                global_best = O->bests[k];
-                std::fprintf(stderr, "best %.2fM (step %d) seed=%lld x=%d z=%d proxy=%.1fM\n", O->bests[k], O->best_step[k],
-                             (long long)O->seed, O->best_xy[k].first, O->best_xy[k].second, O->best_proxy[k]);
+            if (best && O->bests[k] > *best) {
+                *best = O->bests[k];
+                std::snprintf(line, sizeof line, "best %.2fM (step %d) seed=%lld x=%d z=%d proxy=%.1fM\n", O->bests[k],
+                              O->best_step[k], (long long)O->seed, O->best_xy[k].first, O->best_xy[k].second,
+                              O->best_proxy[k]);
+                t.log += line;
             }
         for (auto &e : O->ev) {
             if (e.proxy < P.csv_min && e.s8 <= 0 && e.s4 <= 0) continue;
@@ -726,12 +720,19 @@ void write_group(const Group &g, const dsx_search_params &P, FILE *out, FILE *so
             std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,%zu,%d,%d,%.4f,%.4f,%.4f,%.4f,%.3f,%.3f,%.3f,%d,%.1f\n",
                           (long long)O->seed, O->fy[0], O->fy[1], O->maxta, O->maxtb, e.rank, e.x * 4 + 2, e.z * 4 + 2, st.s,
                           st.t, st.c, st.e, e.proxy, e.s8, e.s4, int(e.edge), e.hot);
-            std::fputs(line, out);
+            t.csv += line;
         }
         std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,%zu,%zu,%.3f\n", (long long)O->seed, O->fy[0], O->fy[1],
                       O->maxta, O->maxtb, O->ncand, O->sites.size(), O->best);
-        std::fputs(line, sout);
+        t.seeds += line;
     }
+}
+
+// Tuning knobs from the environment (DSX_CHUNK: seed indices gated per chunk, DSX_GROUP: seeds per
+// worker group).
+uint64_t env_u64(const char *name, uint64_t def) {
+    const char *v = std::getenv(name);
+    return v && *v ? std::strtoull(v, nullptr, 10) : def;
 }
 
 // Worker contexts of a caller's context, created on first use and kept until dsx_destroy.
@@ -822,7 +823,9 @@ class Pool {
     std::vector<std::thread> th_;
 };
 
-}  // namespace
+}  // namespace dsxh
+
+using namespace dsxh;
 
 extern "C" {
 
@@ -842,7 +845,7 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
         return DSX_E_ARG;
     const dsx_search_params P = *pp;
     if (P.study || P.nms <= 0 || P.max_sites < 0 || P.workers < 0 || P.workers > 64) return DSX_E_ARG;
-    const int nw = P.workers ? P.workers : 4;
+    const int nw = P.workers ? P.workers : 16;
     std::vector<dsx_ctx *> wctx;
     int rc = worker_contexts(ctx, nw, wctx);
     if (rc) return rc;
@@ -871,12 +874,21 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
                     first_err = g->err;
                     pool.stop();
                 }
-                if (rc == DSX_OK) write_group(*g, P, out, sout, counts, global_best);
+                if (rc == DSX_OK) {
+                    GroupText t;
+                    format_group(*g, P, t, &global_best);
+                    std::fputs(t.csv.c_str(), out);
+                    std::fputs(t.seeds.c_str(), sout);
+                    std::fputs(t.log.c_str(), stderr);
+                    counts->scanned += t.scanned;
+                    counts->sites += t.sites;
+                }
                 pending.pop_front();
             }
         };
-        // seed source in chunks
-        const uint64_t CHUNK = 1ull << 24;
+        // seed source in chunks; the gate of the next chunk overlaps the workers' groups of this one
+        const uint64_t CHUNK = env_u64("DSX_CHUNK", 1ull << 22);
+        const size_t group = size_t(std::clamp<uint64_t>(env_u64("DSX_GROUP", 64), 1, 4096));
         const uint64_t total = mode == DSX_SEARCH_LIST ? nseeds : count;
         std::vector<uint64_t> batch;
         for (uint64_t pos = 0; pos < total && rc == DSX_OK;) {
@@ -918,11 +930,9 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
             }
             pos += m;
             prof_add(0, std::chrono::duration<double>(std::chrono::steady_clock::now() - tgate).count(), 0);
-            // groups: about two per worker for this chunk, 32 to 256 seeds each
-            const size_t sub = std::clamp<size_t>((batch.size() + 2 * nw - 1) / (2 * nw), 32, 256);
-            for (size_t b0 = 0; b0 < batch.size(); b0 += sub) {
+            for (size_t b0 = 0; b0 < batch.size(); b0 += group) {
                 auto g = std::make_unique<Group>();
-                g->seeds.assign(batch.begin() + b0, batch.begin() + std::min(batch.size(), b0 + sub));
+                g->seeds.assign(batch.begin() + b0, batch.begin() + std::min(batch.size(), b0 + group));
                 pool.submit(g.get());
                 pending.push_back(std::move(g));
             }

@@ -28,6 +28,7 @@ enum {
     DSX_E_HIP = 2,       /* a HIP runtime call failed (message via dsx_last_error) */
     DSX_E_CAPACITY = 3,  /* caller buffer or internal queue too small; *n_out holds the true count */
     DSX_E_NODEVICE = 4,  /* no usable gfx950 device */
+    DSX_E_BUSY = 5,      /* the request has not finished yet (dsx_poll) */
 };
 const char *dsx_strerror(int code);
 /* Text of the last HIP failure on this thread (empty if none). */
@@ -186,7 +187,7 @@ typedef struct {
     int32_t study, gpu, gpucheck, gscan, gscancheck, gate;
     double gate_phase, gate_amp, gate_pthr;
     int32_t from_stdin;
-    int32_t workers; /* pipeline workers, each with its own stream and device buffers (0: default 4) */
+    int32_t workers; /* pipeline workers, each with its own stream and device buffers (0: default 16) */
 } dsx_search_params;
 void dsx_search_defaults(dsx_search_params *p);
 
@@ -206,6 +207,64 @@ typedef struct {
  * Rows of one seed are contiguous; seeds appear in batch order. study=1 is not supported (DSX_E_ARG). */
 int dsx_search(dsx_ctx *ctx, const dsx_search_params *p, int mode, uint64_t start, uint64_t count, const uint64_t *seeds,
                uint64_t nseeds, const char *prefix, dsx_search_counts *counts);
+
+/* ---------------------------------------------------------------- area queries (tools/dflood.cpp) */
+typedef struct {
+    uint64_t seed;
+    int32_t x, z;   /* block coordinates; the query quart is (x >> 2, z >> 2) (dflood.cpp:14) */
+    int32_t half;   /* window half-size in quarts: (2*half+1)^2 cells of step 1 (dflood.cpp: 20000) */
+    int32_t pad_;
+} dsx_area_query;
+
+/* Exact area queries, batched: for each query, desert_state at the query quart (dflood.cpp:17); a
+ * start that is not desert (state != 2) gives cells = 0, sampled = 1 and no flood; otherwise the
+ * step-1, 4-connected, river-bridged flood dense_flood_bridged(desert_state, qx, qz, 1, half)
+ * (dflood.cpp:20). blocks^2 = 16 * cells (MC-Measure's "Blocks^2 Size"). */
+int dsx_area_queries(dsx_ctx *ctx, const dsx_area_query *q, uint64_t n, dsx_area *out);
+
+/* Synthetic area-query workload (no counterpart in the reference): query j has seed splitmix64(j);
+ * then, from one xoroshiro128++ seeded like standalone::Random(gen_seed): X = bounded(60000001) - 3e7,
+ * Z = bounded(60000001) - 3e7, u = uniform(), cap = floor(10^(3 + 4u)) cells, half =
+ * floor((sqrt(cap) - 1) / 2). caps may be null. */
+void dsx_area_workload(uint64_t gen_seed, uint64_t n, dsx_area_query *out, uint64_t *caps);
+
+/* ---------------------------------------------------------------- request server
+ * A request loop on device-resident state (no counterpart in the reference, which runs one search per
+ * process): seed-range scans (the dsx_search pipeline with the GPU gate) and area queries are
+ * submitted at any time from any thread; worker threads, each owning a context (stream, tables,
+ * flood tiles), pick the oldest pending work: a group of gated seeds of a scan, or every pending area
+ * query that fits the worker's flood tiles (continuous batching), so requests that arrive while others
+ * run are admitted as workers and memory free up and overlap on the device. Results do not depend on
+ * the mix: a request's output equals that of running it alone. */
+typedef struct dsx_server dsx_server;
+typedef struct {
+    int32_t workers;          /* worker contexts; 0: 16 */
+    int32_t area_batch;       /* most area queries per flood batch; 0: 4096 */
+    uint64_t hbm_budget;      /* device bytes the worker contexts may hold (fewer workers if needed); 0: no limit */
+    dsx_search_params params; /* pipeline parameters of scan requests */
+} dsx_server_config;
+typedef struct {
+    uint64_t id;
+    int32_t kind;                     /* 0 scan, 1 area query */
+    int32_t status;                   /* DSX_OK or the error code of the request */
+    double t_submit, t_start, t_done; /* seconds on the server's clock */
+    dsx_area area;                    /* area query result */
+    dsx_search_counts counts;         /* scan counts (done, passed, scanned, sites) */
+    uint64_t csv_bytes, seeds_bytes;  /* scan rows: PREFIX.csv and PREFIX.seeds.csv text sizes */
+} dsx_result;
+int dsx_server_create(int device, const dsx_server_config *cfg, dsx_server **out);
+/* Finishes the requests in flight, then frees the workers. */
+int dsx_server_destroy(dsx_server *s);
+int dsx_server_workers(const dsx_server *s);
+/* GPU-gate scan of seeds splitmix64(k), k in [start, start + count). */
+int dsx_submit_scan(dsx_server *s, uint64_t start, uint64_t count, uint64_t *id);
+int dsx_submit_area(dsx_server *s, const dsx_area_query *q, uint64_t *id);
+int dsx_wait(dsx_server *s, uint64_t id, dsx_result *r);   /* blocks until the request is done */
+int dsx_poll(dsx_server *s, uint64_t id, dsx_result *r);   /* DSX_E_BUSY while it runs */
+/* Rows of a finished scan (CSV text without headers); either buffer may be null. */
+int dsx_result_text(dsx_server *s, uint64_t id, char *csv, uint64_t csv_cap, char *seeds, uint64_t seeds_cap);
+/* Forget a finished request. */
+int dsx_release(dsx_server *s, uint64_t id);
 
 #ifdef __cplusplus
 }
