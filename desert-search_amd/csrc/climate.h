@@ -349,6 +349,6 @@ __device__ __forceinline__ int predicate(int kind, const ClimSeed &C, const Surf
     }
 }
 
-void launch_climinit(hipStream_t st, const uint64_t *seeds, uint32_t n, ClimSeed *out);
+void launch_climinit(hipStream_t st, const uint64_t *seeds, const uint32_t *slots, uint32_t n, ClimSeed *out);
 
 }  // namespace dsx

@@ -4,11 +4,11 @@
 
 namespace dsx {
 
-// One lane per octave slot (46 per seed): NormalNoise::init (standalone.hpp:123-139) for the slot's
+// One lane per octave slot (46 per seed; seed s goes to out[dst[s]]): NormalNoise::init (standalone.hpp:123-139) for the slot's
 // channel and side, then Perlin::init (standalone.hpp:55-61) with the permutation shuffled in a
 // lane-private LDS slot.
 constexpr int CSTRIDE = 260;
-__global__ __launch_bounds__(64) void k_climinit(const uint64_t *seeds, uint32_t n, ClimSeed *out) {
+__global__ __launch_bounds__(64) void k_climinit(const uint64_t *seeds, const uint32_t *dst, uint32_t n, ClimSeed *out) {
     __shared__ uint32_t slot32[64 * CSTRIDE / 4];
     uint8_t *slots = reinterpret_cast<uint8_t *>(slot32);
     const int lane = threadIdx.x;
@@ -53,10 +53,11 @@ __global__ __launch_bounds__(64) void k_climinit(const uint64_t *seeds, uint32_t
     for (int k = 0; k < 64; k++) {
         const uint32_t gk = g0 + k;
         if (gk >= CLIM_NOCT * n) break;
-        reinterpret_cast<uint32_t *>(out[gk / CLIM_NOCT].oct[gk % CLIM_NOCT].perm)[lane] = slot32[k * (CSTRIDE / 4) + lane];
+        reinterpret_cast<uint32_t *>(out[dst[gk / CLIM_NOCT]].oct[gk % CLIM_NOCT].perm)[lane] = slot32[k * (CSTRIDE / 4) + lane];
     }
     if (active) {
-        ClimOct &o = out[g / CLIM_NOCT].oct[g % CLIM_NOCT];
+        ClimSeed &cs = out[dst[g / CLIM_NOCT]];
+        ClimOct &o = cs.oct[g % CLIM_NOCT];
         o.ox = ox;
         o.oy = oy;
         o.oz = oz;
@@ -66,7 +67,7 @@ __global__ __launch_bounds__(64) void k_climinit(const uint64_t *seeds, uint32_t
         o.fy = oy - floor(oy);
         o.vy = fade(o.fy);
         if (g % CLIM_NOCT == 0) {
-            GradTab &G = out[g / CLIM_NOCT].g;
+            GradTab &G = cs.g;
             for (int h = 0; h < 16; h++) {
                 G.xy[h] = make_double2(grad_component(0, h), grad_component(1, h));
                 G.z[h] = grad_component(2, h);
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(64) void k_climinit(const uint64_t *seeds, uint32_t
                 while (c.amp[lo] == 0) lo++;
                 while (c.amp[hi] == 0) hi--;
                 const int span = hi - lo + 1;
-                out[g / CLIM_NOCT].factor[ch] = (5. * span) / (3. * (span + 1));
+                cs.factor[ch] = (5. * span) / (3. * (span + 1));
             }
         }
     }
@@ -126,8 +127,8 @@ __global__ __launch_bounds__(256) void k_points(const ClimSeed *cs, SurfGrid g, 
     else out8[k] = int8_t(predicate(what, C, g, xs[k], zs[k], cut));
 }
 
-void launch_climinit(hipStream_t st, const uint64_t *seeds, uint32_t n, ClimSeed *out) {
-    if (n) hipLaunchKernelGGL(k_climinit, dim3((CLIM_NOCT * n + 63) / 64), dim3(64), 0, st, seeds, n, out);
+void launch_climinit(hipStream_t st, const uint64_t *seeds, const uint32_t *slots, uint32_t n, ClimSeed *out) {
+    if (n) hipLaunchKernelGGL(k_climinit, dim3((CLIM_NOCT * n + 63) / 64), dim3(64), 0, st, seeds, slots, n, out);
 }
 void launch_noise(hipStream_t st, const ClimSeed *cs, int ch, const double *pts, uint32_t n, double *out) {
     if (n) hipLaunchKernelGGL(k_noise, dim3((n + 255) / 256), dim3(256), 0, st, cs, ch, pts, n, out);

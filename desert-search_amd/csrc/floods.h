@@ -44,18 +44,23 @@ struct FloodBufs {
     uint32_t tile_cap;
     unsigned long long *hkeys;    // hash: (job, tx, tz) -> tile index, packed with the key
     uint32_t hmask;               // table size - 1
-    uint32_t *lists[4];           // eval / prop lists, current and next
-    uint32_t *lcount;             // [eval, prop, next_eval, next_prop, overflow]
+    uint32_t *lists[6];           // eval / prop lists of rounds r with r % 3 = 0, 1, 2
+    uint32_t *lcount;             // [0..5] their lengths; [FLOOD_OVF] overflow flags; [FLOOD_MASKS] exported masks
     uint32_t list_cap;
 };
 
-// jl[0..n): indices of the jobs to start (their hash keys and result slots use the original index)
+constexpr int FLOOD_OVF = 6, FLOOD_MASKS = 7;
+
+// jl[0..n): indices of the jobs to start (their hash keys and result slots use the original index);
+// the start tiles go to the lists of round 0
 void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B, uint32_t *eval,
                         uint32_t *prop);
-void launch_flood_eval(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
-                       const uint32_t *list, uint32_t n);
-void launch_flood_prop(hipStream_t st, const FloodJob *jobs, FloodBufs B, const uint32_t *list, uint32_t n,
-                       uint32_t *next_eval, uint32_t *next_prop);
+// Round `round`: evaluate the new tiles of its eval list, then propagate the tiles of its prop list
+// into the next round's lists. `width`: the tiles the host expects per list (a grid hint: any grid
+// covers every listed tile).
+constexpr uint32_t EVAL_GRID = 512, PROP_GRID = 2048;
+void launch_flood_round(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B, int round,
+                        uint32_t width);
 // Reduce, export the counted masks, and return every touched tile and hash slot to the zero / empty
 // state the next call expects (no pool-wide memsets).
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res,

@@ -10,7 +10,7 @@
 //   * k_flood_prop (one wave per tile, one lane per 32-bit row mask) grows the reached set inside the
 //     tile to a fixpoint with bit-parallel dilation, then ORs the touched cells it adds across the
 //     tile edges into the neighbours' `in` masks and queues those tiles;
-//   * the host repeats eval + prop until no tile is queued; k_flood_acc then reduces each tile:
+//   * the host queues rounds of eval + prop until no tile is queued; k_flood_acc then reduces each tile:
 //       reached = pushed cells, touched = reached | in | dilate_inside(reached),
 //       cells = |reached & pred==2|, sampled = |touched inside the world|, border = touched outside
 //       the world, edge = reached cell on the window boundary, bbox over counted reached cells
@@ -113,69 +113,82 @@ __global__ __launch_bounds__(256) void k_flood_start(const FloodJob *jobs, const
         const int c = J.half % FT, t = J.half / FT;   // the centre cell (half, half)
         ti = tile_get(B, k, t, t, &fresh);
         if (ti == 0xffffffffu) {
-            atomicOr(&B.lcount[4], 2u);
+            atomicOr(&B.lcount[FLOOD_OVF], 2u);
         } else {
             atomicOr(&B.tiles[ti].in[c], 1u << c);
             queue = atomicCAS(&B.tiles[ti].queued, 0u, 1u) == 0u;
         }
     }
-    wave_push(eval, &B.lcount[0], B.list_cap, &B.lcount[4], fresh && ti != 0xffffffffu, ti);
-    wave_push(prop, &B.lcount[1], B.list_cap, &B.lcount[4], queue, ti);
+    wave_push(eval, &B.lcount[0], B.list_cap, &B.lcount[FLOOD_OVF], fresh && ti != 0xffffffffu, ti);
+    wave_push(prop, &B.lcount[1], B.list_cap, &B.lcount[FLOOD_OVF], queue, ti);
 }
 
+// Rounds: round r evaluates the new tiles of eval list r % 3 and propagates the tiles of prop list
+// r % 3, listing the next round's tiles in lists (r + 1) % 3; its eval kernel first empties lists
+// (r + 2) % 3 (read by the previous round, written by the next). The grids are fixed (grid-stride over
+// the device-side counts), so the host queues several rounds per synchronisation.
+//
 // Predicate of the 1024 cells of each listed tile: pass = v != 0, cnt = v == 2 (cells outside the window
 // or the world are neither).
 __global__ __launch_bounds__(FT * FT) void k_flood_eval(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B,
-                                                        const uint32_t *list, uint32_t n) {
+                                                        int round) {
     __shared__ ClimSeed C;
-    const uint32_t b = blockIdx.x;
-    if (b >= n) return;
-    Tile &T = B.tiles[list[b]];
-    const FloodJob J = jobs[T.job];
-    // copy only the octaves the predicate reads: T for the hot test, T + C octaves 0-2 + E octaves
-    // 0-1 for the proxy, everything for the biome predicates
-    const ClimSeed &G = cs[J.seed];
-    constexpr int OW = sizeof(ClimOct) / 16;
     const int tid = threadIdx.x;
-    if (J.kind == PRED_HOT || J.kind == PRED_PROXY) {
-        const int nslot = J.kind == PRED_HOT ? 4 : 14;
-        for (int k = tid; k < nslot * OW; k += FT * FT) {
-            const int slot = proxy_slot(k / OW);
-            reinterpret_cast<uint4 *>(&C.oct[slot])[k % OW] = reinterpret_cast<const uint4 *>(&G.oct[slot])[k % OW];
+    const int cur = round % 3, clr = (round + 2) % 3;
+    const uint32_t n = min(__atomic_load_n(&B.lcount[2 * cur], __ATOMIC_RELAXED), B.list_cap);
+    const uint32_t *list = B.lists[2 * cur];
+    if (blockIdx.x == 0 && tid < 2) B.lcount[2 * clr + tid] = 0u;
+    int loaded_seed = -1, loaded_cls = -1;
+    for (uint32_t b = blockIdx.x; b < n; b += gridDim.x) {
+        Tile &T = B.tiles[list[b]];
+        const FloodJob J = jobs[T.job];
+        // octaves the predicate reads: T for the hot test (class 0), T + C octaves 0-2 + E octaves 0-1
+        // for the proxy (class 1), everything for the biome predicates (class 2)
+        const int cls = J.kind == PRED_HOT ? 0 : (J.kind == PRED_PROXY ? 1 : 2);
+        if (J.seed != loaded_seed || cls > loaded_cls) {
+            __syncthreads();
+            const ClimSeed &G = cs[J.seed];
+            constexpr int OW = sizeof(ClimOct) / 16;
+            if (cls < 2) {
+                const int nslot = cls == 0 ? 4 : 14;
+                for (int k = tid; k < nslot * OW; k += FT * FT) {
+                    const int slot = proxy_slot(k / OW);
+                    reinterpret_cast<uint4 *>(&C.oct[slot])[k % OW] = reinterpret_cast<const uint4 *>(&G.oct[slot])[k % OW];
+                }
+                constexpr int GW = sizeof(GradTab) / 16;
+                if (tid < GW) reinterpret_cast<uint4 *>(&C.g)[tid] = reinterpret_cast<const uint4 *>(&G.g)[tid];
+                if (tid < 6) C.factor[tid] = G.factor[tid];
+            } else {
+                const uint4 *src = reinterpret_cast<const uint4 *>(&G);
+                uint4 *dst = reinterpret_cast<uint4 *>(&C);
+                for (int k = tid; k < int(sizeof(ClimSeed) / 16); k += FT * FT) dst[k] = src[k];
+            }
+            __syncthreads();
+            loaded_seed = J.seed;
+            loaded_cls = cls;
         }
-        constexpr int GW = sizeof(GradTab) / 16;
-        if (tid < GW) reinterpret_cast<uint4 *>(&C.g)[tid] = reinterpret_cast<const uint4 *>(&G.g)[tid];
-        if (tid < 6) C.factor[tid] = G.factor[tid];
-    } else {
-        const uint4 *src = reinterpret_cast<const uint4 *>(&G);
-        uint4 *dst = reinterpret_cast<uint4 *>(&C);
-        for (int k = tid; k < int(sizeof(ClimSeed) / 16); k += FT * FT) dst[k] = src[k];
-    }
-    __syncthreads();
-    // one cell per thread; wave w holds rows 2w (lanes 0-31) and 2w + 1 (lanes 32-63)
-    const int nw = int(nwin(J));
-    const int r = tid / FT, col = tid % FT;
-    const int i = T.tx * FT + col, j = T.tz * FT + r;
-    int v = 0;
-    if (i < nw && j < nw) {
-        const int x = J.qx + (i - J.half) * J.step, z = J.qz + (j - J.half) * J.step;
-        if (x >= -WORLD_Q && x <= WORLD_Q && z >= -WORLD_Q && z <= WORLD_Q) v = predicate(J.kind, C, g, x, z, J.cut);
-    }
-    const uint64_t mp = __ballot(v != 0), mc = __ballot(v == 2);
-    if ((tid & 63) == 0) {
-        T.pass[r] = uint32_t(mp);
-        T.pass[r + 1] = uint32_t(mp >> 32);
-        T.cnt[r] = uint32_t(mc);
-        T.cnt[r + 1] = uint32_t(mc >> 32);
+        // one cell per thread; wave w holds rows 2w (lanes 0-31) and 2w + 1 (lanes 32-63)
+        const int nw = int(nwin(J));
+        const int r = tid / FT, col = tid % FT;
+        const int i = T.tx * FT + col, j = T.tz * FT + r;
+        int v = 0;
+        if (i < nw && j < nw) {
+            const int x = J.qx + (i - J.half) * J.step, z = J.qz + (j - J.half) * J.step;
+            if (x >= -WORLD_Q && x <= WORLD_Q && z >= -WORLD_Q && z <= WORLD_Q) v = predicate(J.kind, C, g, x, z, J.cut);
+        }
+        const uint64_t mp = __ballot(v != 0), mc = __ballot(v == 2);
+        if ((tid & 63) == 0) {
+            T.pass[r] = uint32_t(mp);
+            T.pass[r + 1] = uint32_t(mp >> 32);
+            T.cnt[r] = uint32_t(mc);
+            T.cnt[r + 1] = uint32_t(mc >> 32);
+        }
     }
 }
 
-// Grow the reached set of each listed tile to a fixpoint and hand the new frontier to the neighbours.
-__global__ __launch_bounds__(64) void k_flood_prop(const FloodJob *jobs, FloodBufs B, const uint32_t *list, uint32_t n,
-                                                   uint32_t *next_eval, uint32_t *next_prop) {
-    const uint32_t b = blockIdx.x;
-    if (b >= n) return;
-    const uint32_t ti = list[b];
+// Grow the reached set of tile ti to a fixpoint and hand the new frontier to the neighbours (one wave).
+__device__ __forceinline__ void prop_tile(const FloodJob *jobs, FloodBufs &B, uint32_t ti, uint32_t *next_eval,
+                                          uint32_t *next_prop, uint32_t *cnt_e, uint32_t *cnt_p) {
     Tile &T = B.tiles[ti];
     const int lane = threadIdx.x;
     const int r = lane & 31;
@@ -245,7 +258,7 @@ __global__ __launch_bounds__(64) void k_flood_prop(const FloodJob *jobs, FloodBu
         nti = __shfl(nti, 0);
         fresh = __shfl(int(fresh), 0) != 0;
         if (nti == 0xffffffffu) {
-            if (lane == 0) atomicOr(&B.lcount[4], 2u);
+            if (lane == 0) atomicOr(&B.lcount[FLOOD_OVF], 2u);
             continue;
         }
         Tile &N = B.tiles[nti];
@@ -262,20 +275,29 @@ __global__ __launch_bounds__(64) void k_flood_prop(const FloodJob *jobs, FloodBu
     }
     if (lane == 0) {
         if (ne) {
-            const uint32_t k = atomicAdd(&B.lcount[2], uint32_t(ne));
+            const uint32_t k = atomicAdd(cnt_e, uint32_t(ne));
             for (int q = 0; q < ne; q++) {
                 if (k + q < B.list_cap) next_eval[k + q] = push_e[q];
-                else atomicOr(&B.lcount[4], 1u);
+                else atomicOr(&B.lcount[FLOOD_OVF], 1u);
             }
         }
         if (np) {
-            const uint32_t k = atomicAdd(&B.lcount[3], uint32_t(np));
+            const uint32_t k = atomicAdd(cnt_p, uint32_t(np));
             for (int q = 0; q < np; q++) {
                 if (k + q < B.list_cap) next_prop[k + q] = push_p[q];
-                else atomicOr(&B.lcount[4], 1u);
+                else atomicOr(&B.lcount[FLOOD_OVF], 1u);
             }
         }
     }
+}
+
+__global__ __launch_bounds__(64) void k_flood_prop(const FloodJob *jobs, FloodBufs B, int round) {
+    const int cur = round % 3, nxt = (round + 1) % 3;
+    const uint32_t n = min(__atomic_load_n(&B.lcount[2 * cur + 1], __ATOMIC_RELAXED), B.list_cap);
+    const uint32_t *list = B.lists[2 * cur + 1];
+    uint32_t *next_eval = B.lists[2 * nxt], *next_prop = B.lists[2 * nxt + 1];
+    uint32_t *cnt_e = &B.lcount[2 * nxt], *cnt_p = &B.lcount[2 * nxt + 1];
+    for (uint32_t b = blockIdx.x; b < n; b += gridDim.x) prop_tile(jobs, B, list[b], next_eval, next_prop, cnt_e, cnt_p);
 }
 
 // Per-tile reduction of the flood outputs into res[job].
@@ -382,13 +404,11 @@ void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl
                         uint32_t *prop) {
     if (n) hipLaunchKernelGGL(k_flood_start, dim3((n + 255) / 256), dim3(256), 0, st, jobs, jl, n, B, eval, prop);
 }
-void launch_flood_eval(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
-                       const uint32_t *list, uint32_t n) {
-    if (n) hipLaunchKernelGGL(k_flood_eval, dim3(n), dim3(FT * FT), 0, st, jobs, cs, g, B, list, n);
-}
-void launch_flood_prop(hipStream_t st, const FloodJob *jobs, FloodBufs B, const uint32_t *list, uint32_t n,
-                       uint32_t *next_eval, uint32_t *next_prop) {
-    if (n) hipLaunchKernelGGL(k_flood_prop, dim3(n), dim3(64), 0, st, jobs, B, list, n, next_eval, next_prop);
+void launch_flood_round(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B, int round,
+                        uint32_t width) {
+    const uint32_t w = width < 8 ? 8 : width;
+    hipLaunchKernelGGL(k_flood_eval, dim3(w < EVAL_GRID ? w : EVAL_GRID), dim3(FT * FT), 0, st, jobs, cs, g, B, round);
+    hipLaunchKernelGGL(k_flood_prop, dim3(w < PROP_GRID ? w : PROP_GRID), dim3(64), 0, st, jobs, B, round);
 }
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res, TileMask *masks,
                       uint32_t *mcount, uint32_t mcap) {

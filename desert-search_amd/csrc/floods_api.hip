@@ -78,7 +78,10 @@ struct ClimState {
     int16_t *d_cell = nullptr;
     dsx::ClimSeed *d_clim = nullptr;
     uint32_t clim_cap = 0;
-    uint64_t *d_useeds = nullptr;
+    uint64_t *d_useeds = nullptr;   // seeds to initialise, then their slots (u32)
+    std::unordered_map<uint64_t, uint32_t> cslot;   // seed -> climate slot (initialised on the device)
+    std::vector<uint64_t> miss;                      // upload sources of the last k_climinit (kept alive
+    std::vector<uint32_t> miss_slot;                 // until the call's stream synchronises)
     // flood buffers
     dsx::FloodBufs fb{};
     dsx::FloodJob *d_jobs = nullptr;
@@ -116,7 +119,7 @@ static int clim_alloc(dsx_ctx *c, ClimState *&S) {
     S->grid.cell = S->d_cell;
     S->clim_cap = 4096;
     HIPCK(hipMalloc(&S->d_clim, size_t(S->clim_cap) * sizeof(dsx::ClimSeed)));
-    HIPCK(hipMalloc(&S->d_useeds, size_t(S->clim_cap) * 8));
+    HIPCK(hipMalloc(&S->d_useeds, size_t(S->clim_cap) * 12));
     // flood engine
     dsx::FloodBufs &B = S->fb;
     B.tile_cap = 1u << 21;                    // 2M tiles x 528 B
@@ -147,15 +150,45 @@ static int io_reserve(dsx_ctx *c, ClimState *S, size_t bytes) {
     return DSX_OK;
 }
 
-// Climate of `n` distinct seeds into S->d_clim[0..n).
-static int clim_seeds(dsx_ctx *c, ClimState *S, const uint64_t *seeds, uint32_t n) {
-    if (n > S->clim_cap) {
-        dsx::g_last_error = "too many distinct seeds in one call";
-        return DSX_E_CAPACITY;
+// Climate slots of `n` seeds (repeats allowed): slot[k] = index into S->d_clim. A seed's climate stays
+// on the device across calls; the seeds not cached are initialised in one k_climinit launch (when they
+// do not fit the free slots, the cache restarts empty).
+static int clim_slots(dsx_ctx *c, ClimState *S, const uint64_t *seeds, uint32_t n, uint32_t *slot) {
+    std::vector<uint64_t> &miss = S->miss;
+    auto assign = [&]() {
+        miss.clear();
+        for (uint32_t k = 0; k < n; k++) {
+            auto it = S->cslot.find(seeds[k]);
+            if (it != S->cslot.end()) {
+                slot[k] = it->second;
+                continue;
+            }
+            if (S->cslot.size() >= S->clim_cap) return false;
+            slot[k] = uint32_t(S->cslot.size());
+            S->cslot.emplace(seeds[k], slot[k]);
+            miss.push_back(seeds[k]);
+        }
+        return true;
+    };
+    if (!assign()) {
+        S->cslot.clear();   // restart the cache with this call's seeds
+        if (!assign()) {
+            S->cslot.clear();
+            dsx::g_last_error = "too many distinct seeds in one call";
+            return DSX_E_CAPACITY;
+        }
     }
-    HIPCK(hipMemcpyAsync(S->d_useeds, seeds, size_t(n) * 8, hipMemcpyHostToDevice, c->st));
-    dsx::launch_climinit(c->st, S->d_useeds, n, S->d_clim);
-    HIPCK(hipGetLastError());
+    if (!miss.empty()) {
+        const uint32_t m = uint32_t(miss.size());
+        std::vector<uint32_t> &ms = S->miss_slot;
+        ms.resize(m);
+        for (uint32_t k = 0; k < m; k++) ms[k] = S->cslot[miss[k]];
+        uint32_t *d_sl = reinterpret_cast<uint32_t *>(S->d_useeds + S->clim_cap);
+        HIPCK(hipMemcpyAsync(S->d_useeds, miss.data(), size_t(m) * 8, hipMemcpyHostToDevice, c->st));
+        HIPCK(hipMemcpyAsync(d_sl, ms.data(), size_t(m) * 4, hipMemcpyHostToDevice, c->st));
+        dsx::launch_climinit(c->st, S->d_useeds, d_sl, m, S->d_clim);
+        HIPCK(hipGetLastError());
+    }
     return DSX_OK;
 }
 
@@ -180,7 +213,8 @@ void dsx_ctx_free_ext(dsx_ctx *c) {
     }
     for (void *p : {(void *)s->d_cuts, (void *)s->d_cell, (void *)s->d_clim, (void *)s->d_useeds, (void *)s->fb.tiles,
                     (void *)s->fb.ntiles, (void *)s->fb.hkeys, (void *)s->fb.lists[0], (void *)s->fb.lists[1],
-                    (void *)s->fb.lists[2], (void *)s->fb.lists[3], (void *)s->fb.lcount, (void *)s->d_jobs,
+                    (void *)s->fb.lists[2], (void *)s->fb.lists[3], (void *)s->fb.lists[4], (void *)s->fb.lists[5],
+                    (void *)s->fb.lcount, (void *)s->d_jobs,
                     (void *)s->d_res, s->d_io, (void *)s->d_masks, (void *)s->d_jl})
         if (p) (void)hipFree(p);
     delete s;
@@ -194,11 +228,12 @@ int dsx_noise(dsx_ctx *c, uint64_t seed, int channel, const double *xyz, uint64_
     ClimState *S;
     int rc = clim_alloc(c, S);
     if (rc) return rc;
-    if ((rc = clim_seeds(c, S, &seed, 1))) return rc;
+    uint32_t sl = 0;
+    if ((rc = clim_slots(c, S, &seed, 1, &sl))) return rc;
     if ((rc = io_reserve(c, S, n * 32))) return rc;
     double *d_pts = static_cast<double *>(S->d_io), *d_out = d_pts + 3 * n;
     HIPCK(hipMemcpyAsync(d_pts, xyz, n * 24, hipMemcpyHostToDevice, c->st));
-    dsx::launch_noise(c->st, S->d_clim, channel, d_pts, uint32_t(n), d_out);
+    dsx::launch_noise(c->st, S->d_clim + sl, channel, d_pts, uint32_t(n), d_out);
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(out, d_out, n * 8, hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
@@ -211,13 +246,14 @@ int dsx_climate(dsx_ctx *c, uint64_t seed, const int32_t *xs, const int32_t *zs,
     ClimState *S;
     int rc = clim_alloc(c, S);
     if (rc) return rc;
-    if ((rc = clim_seeds(c, S, &seed, 1))) return rc;
+    uint32_t sl = 0;
+    if ((rc = clim_slots(c, S, &seed, 1, &sl))) return rc;
     if ((rc = io_reserve(c, S, n * 56))) return rc;
     int32_t *d_x = static_cast<int32_t *>(S->d_io), *d_z = d_x + n;
     int64_t *d_q = reinterpret_cast<int64_t *>(d_z + n + (n & 1));
     HIPCK(hipMemcpyAsync(d_x, xs, n * 4, hipMemcpyHostToDevice, c->st));
     HIPCK(hipMemcpyAsync(d_z, zs, n * 4, hipMemcpyHostToDevice, c->st));
-    dsx::launch_climate(c->st, S->d_clim, d_x, d_z, uint32_t(n), d_q);
+    dsx::launch_climate(c->st, S->d_clim + sl, d_x, d_z, uint32_t(n), d_q);
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(q, d_q, n * 48, hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
@@ -231,13 +267,14 @@ int dsx_predicates(dsx_ctx *c, uint64_t seed, int kind, float cut, const int32_t
     ClimState *S;
     int rc = clim_alloc(c, S);
     if (rc) return rc;
-    if ((rc = clim_seeds(c, S, &seed, 1))) return rc;
+    uint32_t sl = 0;
+    if ((rc = clim_slots(c, S, &seed, 1, &sl))) return rc;
     if ((rc = io_reserve(c, S, n * 9))) return rc;
     int32_t *d_x = static_cast<int32_t *>(S->d_io), *d_z = d_x + n;
     int8_t *d_o = reinterpret_cast<int8_t *>(d_z + n);
     HIPCK(hipMemcpyAsync(d_x, xs, n * 4, hipMemcpyHostToDevice, c->st));
     HIPCK(hipMemcpyAsync(d_z, zs, n * 4, hipMemcpyHostToDevice, c->st));
-    dsx::launch_pred(c->st, S->d_clim, S->grid, kind, cut, d_x, d_z, uint32_t(n), d_o);
+    dsx::launch_pred(c->st, S->d_clim + sl, S->grid, kind, cut, d_x, d_z, uint32_t(n), d_o);
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(out, d_o, n, hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
@@ -278,7 +315,9 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         hj[k] = dsx::FloodJob{s, jobs[k].kind, jobs[k].cut, jobs[k].qx, jobs[k].qz, jobs[k].step, jobs[k].half,
                               jobs[k].conn8};
     }
-    if ((rc = clim_seeds(c, S, useeds.data(), uint32_t(useeds.size())))) return rc;
+    std::vector<uint32_t> uslot(useeds.size());
+    if ((rc = clim_slots(c, S, useeds.data(), uint32_t(useeds.size()), uslot.data()))) return rc;
+    for (uint64_t k = 0; k < n; k++) hj[k].seed = int32_t(uslot[hj[k].seed]);
     dsx::FloodBufs &B = S->fb;
     HIPCK(hipMemcpyAsync(S->d_jobs, hj.data(), n * sizeof(dsx::FloodJob), hipMemcpyHostToDevice, c->st));
     HIPCK(hipMemsetAsync(B.ntiles, 0, 4, c->st));
@@ -294,9 +333,9 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     if (!jl_hot.empty()) HIPCK(hipMemcpyAsync(d_hot, jl_hot.data(), jl_hot.size() * 4, hipMemcpyHostToDevice, c->st));
     if (!jl_oth.empty()) HIPCK(hipMemcpyAsync(d_oth, jl_oth.data(), jl_oth.size() * 4, hipMemcpyHostToDevice, c->st));
     dsx::launch_flood_lds(c->st, true, S->d_jobs, d_hot, uint32_t(jl_hot.size()), S->d_clim, S->grid, S->d_res, S->d_masks,
-                          B.lcount + 5, S->mask_cap);
+                          B.lcount + dsx::FLOOD_MASKS, S->mask_cap);
     dsx::launch_flood_lds(c->st, false, S->d_jobs, d_oth, uint32_t(jl_oth.size()), S->d_clim, S->grid, S->d_res,
-                          S->d_masks, B.lcount + 5, S->mask_cap);
+                          S->d_masks, B.lcount + dsx::FLOOD_MASKS, S->mask_cap);
     HIPCK(hipGetLastError());
     std::vector<dsx::FloodRes> res(n);
     if (!jl_hot.empty() || !jl_oth.empty()) {
@@ -312,39 +351,45 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         dsx::launch_flood_start(c->st, S->d_jobs, d_big, uint32_t(jl_big.size()), B, B.lists[0], B.lists[1]);
         HIPCK(hipGetLastError());
     }
-    static double prof_t = 0, prof_rounds = 0, prof_calls = 0, prof_tiles = 0, prof_sampled = 0;
+    static double prof_t = 0, prof_rounds = 0, prof_calls = 0, prof_tiles = 0, prof_sampled = 0, prof_bsampled = 0,
+                  prof_evald = 0;
     struct ProfOut {
-        double *t, *r, *c, *n, *s;
+        double *t, *r, *c, *n, *s, *bs, *ev;
         ~ProfOut() {
             if (std::getenv("DSX_PROFILE"))
-                std::fprintf(stderr, "dsx_flood profile: %.0f calls, %.0f rounds, %.0f tiles (%.0f cells evaluated), %.0f sampled, %.3f s\n",
-                             *c, *r, *n, *n * 1024, *s, *t);
+                std::fprintf(stderr,
+                             "dsx_flood profile: %.0f calls, %.0f rounds, %.0f tiles (%.0f cells evaluated, %.0f sampled by "
+                             "tiled floods), %.0f sampled in all, %.3f s\n",
+                             *c, *r, *n, *ev, *bs, *s, *t);
         }
     };
-    static ProfOut prof_out{&prof_t, &prof_rounds, &prof_calls, &prof_tiles, &prof_sampled};
+    static ProfOut prof_out{&prof_t, &prof_rounds, &prof_calls, &prof_tiles, &prof_sampled, &prof_bsampled, &prof_evald};
     static std::mutex prof_mu;
-    double my_rounds = 0, my_tiles = 0, my_sampled = 0;
+    double my_rounds = 0, my_tiles = 0, my_sampled = 0, my_bsampled = 0;
     const auto prof_t0 = std::chrono::steady_clock::now();
-    int cur = 0;   // lists[2*cur] eval, lists[2*cur+1] prop
+    // rounds, queued several at a time between synchronisations (a finished flood leaves an empty
+    // list, so extra rounds do nothing); grids sized from the list length seen at the last check
     uint32_t *h = c->h_cnt;
     int status = DSX_OK;
-    for (int round = 0;; round++) {
-        HIPCK(hipMemcpyAsync(h, B.lcount, 20, hipMemcpyDeviceToHost, c->st));
-        HIPCK(hipStreamSynchronize(c->st));
-        if (h[4]) { status = DSX_E_CAPACITY; break; }
-        const uint32_t ne = h[0], np = h[1];
-        if (np == 0 || round > 4000000) break;
-        my_rounds++;
-        uint32_t *le = B.lists[2 * cur], *lp = B.lists[2 * cur + 1];
-        uint32_t *nle = B.lists[2 * (1 - cur)], *nlp = B.lists[2 * (1 - cur) + 1];
-        // next-round counters start at zero; this round's lists are consumed
-        HIPCK(hipMemsetAsync(B.lcount + 2, 0, 8, c->st));
-        dsx::launch_flood_eval(c->st, S->d_jobs, S->d_clim, S->grid, B, le, ne);
-        dsx::launch_flood_prop(c->st, S->d_jobs, B, lp, np, nle, nlp);
-        HIPCK(hipGetLastError());
-        // move next counters into place: [0,1] <- [2,3]
-        HIPCK(hipMemcpyAsync(B.lcount, B.lcount + 2, 8, hipMemcpyDeviceToDevice, c->st));
-        cur = 1 - cur;
+    if (!jl_big.empty()) {
+        int round = 0, burst = 2;
+        uint32_t width = uint32_t(jl_big.size());
+        for (;;) {
+            for (int k = 0; k < burst; k++, round++)
+                dsx::launch_flood_round(c->st, S->d_jobs, S->d_clim, S->grid, B, round, 2 * width + 16 * uint32_t(k + 1));
+            HIPCK(hipGetLastError());
+            my_rounds += burst;
+            HIPCK(hipMemcpyAsync(h, B.lcount, 32, hipMemcpyDeviceToHost, c->st));
+            HIPCK(hipStreamSynchronize(c->st));
+            if (h[dsx::FLOOD_OVF]) {
+                status = DSX_E_CAPACITY;
+                break;
+            }
+            const uint32_t ne = h[2 * (round % 3)], np = h[2 * (round % 3) + 1];
+            if (np == 0 || round > 4000000) break;
+            width = std::max(ne, np);
+            burst = std::min(burst * 2, 8);
+        }
     }
     uint32_t ntiles = 0;
     HIPCK(hipMemcpyAsync(&ntiles, B.ntiles, 4, hipMemcpyDeviceToHost, c->st));
@@ -352,12 +397,12 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     ntiles = std::min(ntiles, B.tile_cap);
     my_tiles += ntiles;
     if (status == DSX_OK) {
-        dsx::launch_flood_acc(c->st, S->d_jobs, B, ntiles, S->d_res, S->d_masks, B.lcount + 5, S->mask_cap);
+        dsx::launch_flood_acc(c->st, S->d_jobs, B, ntiles, S->d_res, S->d_masks, B.lcount + dsx::FLOOD_MASKS, S->mask_cap);
         HIPCK(hipGetLastError());
         HIPCK(hipMemcpyAsync(res.data(), S->d_res, n * sizeof(dsx::FloodRes), hipMemcpyDeviceToHost, c->st));
-        HIPCK(hipMemcpyAsync(h + 5, B.lcount + 5, 4, hipMemcpyDeviceToHost, c->st));
+        HIPCK(hipMemcpyAsync(h + dsx::FLOOD_MASKS, B.lcount + dsx::FLOOD_MASKS, 4, hipMemcpyDeviceToHost, c->st));
         HIPCK(hipStreamSynchronize(c->st));
-        const uint32_t nm = h[5];
+        const uint32_t nm = h[dsx::FLOOD_MASKS];
         if (nm > S->mask_cap) {
             status = DSX_E_CAPACITY;
             dsx::g_last_error = "flood mask export overflow";
@@ -369,6 +414,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
             HIPCK(hipStreamSynchronize(c->st));
         }
         for (uint64_t k = 0; k < n; k++) my_sampled += double(res[k].sampled);
+        for (uint32_t k : jl_big) my_bsampled += double(res[k].sampled);
         for (uint64_t k = 0; k < n; k++)
             out[k] = dsx_area{res[k].cells, res[k].sampled, res[k].minx, res[k].maxx, res[k].minz, res[k].maxz,
                               res[k].edge, res[k].border};
@@ -389,6 +435,8 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         prof_rounds += my_rounds;
         prof_tiles += my_tiles;
         prof_sampled += my_sampled;
+        prof_bsampled += my_bsampled;
+        prof_evald += my_tiles * 1024;
     }
     return status;
 }
@@ -416,13 +464,19 @@ int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_
     ClimState *S;
     int rc = clim_alloc(c, S);
     if (rc) return rc;
-    if ((rc = clim_seeds(c, S, seeds, uint32_t(nseeds)))) return rc;
+    if (nseeds > S->clim_cap) {
+        dsx::g_last_error = "too many distinct seeds in one call";
+        return DSX_E_CAPACITY;
+    }
+    std::vector<uint32_t> sslot(nseeds), sidx2(n);
+    if ((rc = clim_slots(c, S, seeds, uint32_t(nseeds), sslot.data()))) return rc;
+    for (uint64_t k = 0; k < n; k++) sidx2[k] = sslot[sidx[k]];
     if ((rc = io_reserve(c, S, n * 24))) return rc;
     uint32_t *d_s = static_cast<uint32_t *>(S->d_io);
     int32_t *d_x = reinterpret_cast<int32_t *>(d_s + n), *d_z = d_x + n;
     double *d_d = reinterpret_cast<double *>(d_z + n + (n & 1));
     int8_t *d_8 = reinterpret_cast<int8_t *>(d_d);
-    HIPCK(hipMemcpyAsync(d_s, sidx, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCK(hipMemcpyAsync(d_s, sidx2.data(), n * 4, hipMemcpyHostToDevice, c->st));
     HIPCK(hipMemcpyAsync(d_x, xs, n * 4, hipMemcpyHostToDevice, c->st));
     HIPCK(hipMemcpyAsync(d_z, zs, n * 4, hipMemcpyHostToDevice, c->st));
     dsx::launch_points(c->st, S->d_clim, S->grid, d_s, d_x, d_z, uint32_t(n), what, cut, d_8, d_d);

@@ -845,7 +845,7 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
         return DSX_E_ARG;
     const dsx_search_params P = *pp;
     if (P.study || P.nms <= 0 || P.max_sites < 0 || P.workers < 0 || P.workers > 64) return DSX_E_ARG;
-    const int nw = P.workers ? P.workers : 16;
+    const int nw = P.workers ? P.workers : 24;
     std::vector<dsx_ctx *> wctx;
     int rc = worker_contexts(ctx, nw, wctx);
     if (rc) return rc;

@@ -187,7 +187,7 @@ typedef struct {
     int32_t study, gpu, gpucheck, gscan, gscancheck, gate;
     double gate_phase, gate_amp, gate_pthr;
     int32_t from_stdin;
-    int32_t workers; /* pipeline workers, each with its own stream and device buffers (0: default 16) */
+    int32_t workers; /* pipeline workers, each with its own stream and device buffers (0: default 24) */
 } dsx_search_params;
 void dsx_search_defaults(dsx_search_params *p);
 
