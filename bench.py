@@ -1,31 +1,59 @@
 #!/usr/bin/env python3
-"""Benchmark: gated seeds/s of the GPU seed gate on one MI355X.
+"""Benchmark: gated seeds/s of the whole per-seed pipeline on one MI355X, and area-query latency.
 
-Workload (one step): seed indices k in [0, 2^24), seeds splitmix64(k), through the GPU gate - y-phase
-<= 0.06f, 4-parity anti-aliased 256^2 pre-gate >= 0.645, full 512^2 amplitude gate >= 0.70 - with
-the survivor list sorted and copied back. value = gated seeds (full-gate survivors) per second.
-Inputs are generated on the device (seed index ranges), so nothing crosses PCIe before a step.
+Workload (one step): seed indices k in [1e12 + s*1e8, 1e12 + (s+1)*1e8), seeds splitmix64(k), through
+the GPU pipeline exactly as `dsearch_desert START COUNT PREFIX THREADS gpu=1 gate=1` runs it: GPU gate
+(y-phase <= 0.06f, 4-parity pre-gate >= 0.645, full 512^2 gate >= 0.70) -> octave-0 period tables,
+blur and translation scan -> sites -> hot-zone floods -> proxy floods -> exact step-8/step-4 floods ->
+CSV rows. Step 0 starts at the 1e7-seed pipeline range from 1e12 and runs ten times its length, so
+that the ~780 gated seeds per 1e7 indices keep every pipeline worker busy (steady state). value =
+gated seeds (seeds that pass the gate and run through the pipeline: `scanned` of PREFIX.json) per
+second over the timed steps. Inputs are seed index ranges generated on the device; outputs are the
+CSV files the pipeline writes.
 
 Also reported:
-  roofline      the dominant kernel (the pre-gate k_amp<0>): algorithmic FP32 operations per launch
-                (18 adds of the value grid + 2 x 9 multiply-adds of the blur per cell, 2 x 65,536
-                cells per phase survivor) / its average launch time, from HIP events recorded by the
-                library on the stream the kernel runs on, against MI355X's published FP32 vector peak.
-  cpu_baseline  the same gate computed by the CPU oracle (a C++ port of the identical arithmetic) on a
-                bounded sample of the same workload, on this host's cores.
+  roofline      the dominant kernel, the tiled flood engine's tile evaluation (k_flood_eval, FP64
+                VALU): algorithmic FP64 operations = the reference's sampled cells of the floods it ran
+                x octave samples per predicate call of their kind and cost class x 89 FP64 operations
+                per octave sample, divided by its device time from HIP events recorded around each
+                launch on the stream it runs on (all launches in the timed region).
+  cpu_baseline  the reference's own CPU dsearch_desert (built from its sources into oracle/_ref, gpu=0)
+                over a bounded sample of seed indices on this host's cores: scanned / seconds. Falls
+                back to the oracle's C++ restatement of the same pipeline when oracle/_ref is absent.
+  gate_only     the GPU gate alone over 2^24 seed indices (the gate kernel workload).
+  area_queries  4096 exact step-1 area queries (seed splitmix64(j), X, Z uniform in +-3e7, window
+                half = floor((sqrt(cap)-1)/2) for caps log-uniform over 1e3..1e7 cells) submitted at
+                once to the request server: p50/p99 latency per query and queries/s.
+  serving       the request server under Poisson arrivals mixing seed-range scans (70%, length
+                floor(10^(3+4u)), random 64-bit start) and area queries (30%): p50/p99 latency per class
+                and gated seeds/s.
 """
 import argparse
 import ctypes
 import importlib.util
 import json
 import os
+import subprocess
 import sys
+import tempfile
+import threading
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
-FLOPS_PER_PRE_SEED = 2 * 65536 * (18 + 2 * 9 * 2)   # 7,077,888 FP32 ops per phase survivor
-FP32_PEAK_TFLOPS = 157.3
-N_SEEDS = 1 << 24
+START = 10 ** 12
+N_STEP = 10 ** 8
+FP64_PEAK_TFLOPS = 78.6
+FLOPS_PER_SAMPLE = 89   # FP64 operations of one octave sample (Perlin::sample + NormalNoise weighting)
+KINDS = ("desert_state", "hot", "proxy", "desert")
+
+
+def pred_samples(kind, hot):
+    """Octave samples of one predicate call (kind index as DSX_PRED_*, cost class hot/cold)."""
+    if kind == 1:
+        return 4
+    if kind == 2:
+        return 14 if hot else 4
+    return 52 if hot else 16
 
 
 def load_package():
@@ -38,73 +66,199 @@ def load_package():
     return mod
 
 
-def cpu_baseline(sample=1 << 21):
-    """Oracle GPU-gate restatement over seed indices [0, sample) on this host's cores."""
-    L = ctypes.CDLL(os.path.join(REPO, "oracle", "liboracle.so"))
-    L.or_gate_run.restype = ctypes.c_int64
-    L.or_gate_run.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
-                              ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
-    cores = min(16, len(os.sched_getaffinity(0)))
-    cap = 1 << 14
-    idx = (ctypes.c_uint64 * cap)()
-    amp = (ctypes.c_float * cap)()
-    cnt = (ctypes.c_uint64 * 3)()
-    t = time.perf_counter()
-    L.or_gate_run(0, sample, 0.06, 0.645, 0.70, cores, idx, amp, cap, cnt)
-    sec = time.perf_counter() - t
-    return {"value": cnt[2] / sec, "unit": "gated seeds/s", "cores": cores, "kind": "port",
-            "sample": f"seed indices [0, 2^{sample.bit_length() - 1}) through the oracle's C++ restatement of the "
-                      f"same gate ({cnt[0]} phase, {cnt[1]} pre-gate, {cnt[2]} gated seeds in {sec:.1f} s)"}
+def host_cores():
+    """CPU cores this process may use: affinity, capped by the cgroup quota (as deploy/prun.sh:6-11)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if quota != "max":
+            n = min(n, max(1, int(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def cpu_baseline(sample=10 ** 7):
+    """The reference CPU pipeline (gpu=0) over seed indices [0, sample) on this host's cores."""
+    cores = host_cores()
+    ref = os.path.join(REPO, "oracle", "_ref", "dsearch_desert")
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "cpu")
+        if os.access(ref, os.X_OK):
+            subprocess.run([ref, "0", str(sample), pre, str(cores)], check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL)
+            m = json.load(open(pre + ".json"))
+            kind, what = "reference", "the reference's dsearch_desert built from its sources"
+            scanned, sec, passed = m["scanned"], m["seconds"], m["passed"]
+        else:
+            L = ctypes.CDLL(os.path.join(REPO, "oracle", "liboracle.so"))
+            L.or_run.restype = ctypes.c_int
+            L.or_run.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_int]
+            cnt = (ctypes.c_uint64 * 4)()
+            secs = ctypes.c_double()
+            L.or_run(0, sample, None, 0, 0, cores, pre.encode(), cnt, ctypes.byref(secs), None, 0)
+            kind, what = "port", "the oracle's C++ restatement of the reference pipeline"
+            scanned, sec, passed = cnt[2], secs.value, cnt[1]
+    return {"value": scanned / sec, "unit": "gated seeds/s", "cores": cores, "kind": kind,
+            "sample": f"seed indices [0, {sample:.0e}) through {what} with its CPU gate (gpu=0, production "
+                      f"Params): {passed} phase survivors, {scanned} gated seeds through the pipeline in "
+                      f"{sec:.1f} s on {cores} threads"}
+
+
+def percentiles(v):
+    import numpy as np
+    if len(v) == 0:
+        return None, None
+    a = np.asarray(v) * 1e3
+    return float(np.percentile(a, 50)), float(np.percentile(a, 99))
+
+
+def area_burst(dsx, n=4096, workers=16):
+    """The 4096 area queries submitted at once to the request server."""
+    q, caps = dsx.area_workload(n)
+    s = dsx.Server(0, workers=workers)
+    try:
+        t0 = time.perf_counter()
+        ids = [s.submit_area(int(r["seed"]), int(r["x"]), int(r["z"]), int(r["half"])) for r in q]
+        lat, cells = [], 0
+        for i in ids:
+            r = s.wait(i)
+            if r.status != 0:
+                raise RuntimeError("area query failed")
+            lat.append(r.t_done - r.t_submit)
+            cells += r.area.cells
+            s.release(i)
+        sec = time.perf_counter() - t0
+    finally:
+        s.close()
+    p50, p99 = percentiles(lat)
+    return {"queries": n, "seconds": sec, "queries_per_s": n / sec, "p50_ms": p50, "p99_ms": p99,
+            "desert_cells": int(cells), "workers": workers}
+
+
+def serving(dsx, n_req=240, rate=24.0, seed=20261203, workers=16):
+    """Poisson arrivals (rate requests/s), 70% scans / 30% area queries, on one request server."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    q, caps = dsx.area_workload(n_req)
+    gaps = -np.log(1.0 - rng.random(n_req)) / rate
+    kinds = rng.random(n_req) < 0.7
+    lens = np.floor(10.0 ** (3 + 4 * rng.random(n_req))).astype(np.int64)
+    starts = rng.integers(0, 2 ** 63, n_req, dtype=np.int64).astype(np.uint64) * 2 + rng.integers(0, 2, n_req).astype(np.uint64)
+    s = dsx.Server(0, workers=workers)
+    ids = []
+    try:
+        t0 = time.perf_counter()
+        t = t0
+        for k in range(n_req):
+            t += gaps[k]
+            d = t - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            if kinds[k]:
+                ids.append(("scan", s.submit_scan(int(starts[k]), int(lens[k]))))
+            else:
+                ids.append(("area", s.submit_area(int(q[k]["seed"]), int(q[k]["x"]), int(q[k]["z"]), int(q[k]["half"]))))
+        lat = {"scan": [], "area": []}
+        gated = 0
+        t_end = t0
+        for kind, i in ids:
+            r = s.wait(i)
+            if r.status != 0:
+                raise RuntimeError(f"{kind} request failed")
+            lat[kind].append(r.t_done - r.t_submit)
+            if kind == "scan":
+                gated += r.counts.scanned
+            s.release(i)
+        sec = time.perf_counter() - t0
+    finally:
+        s.close()
+    out = {"requests": n_req, "rate_per_s": rate, "seconds": sec, "gated_seeds_per_s": gated / sec,
+           "workers": workers}
+    for kind in ("scan", "area"):
+        p50, p99 = percentiles(lat[kind])
+        out[kind] = {"n": len(lat[kind]), "p50_ms": p50, "p99_ms": p99}
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workers", type=int, default=0, help="pipeline workers (0: the library default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the gate-only, area-query and serving legs")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU only (the seed gate shards by seed range; no multi-GPU "
-              "run is part of it)", file=sys.stderr)
+        print("bench.py: this build measures one GPU only (seeds shard by index range with no communication; "
+              "no multi-GPU run is part of it)", file=sys.stderr)
         sys.exit(2)
     import torch
     pkg = load_package()
-    eng = pkg.dsx.Engine(0)
-    for _ in range(a.warmup):
-        eng.gate(0, N_SEEDS)
+    dsx = pkg.dsx
+    eng = dsx.Engine(0)
+    lib = eng.lib
+    params = eng.search_params(gpu=1, gate=1, workers=a.workers)
+    td = tempfile.mkdtemp(prefix="dsx_bench_")
+    for w in range(a.warmup):
+        eng.search(os.path.join(td, f"w{w}"), dsx.SEARCH_GATE, START - (w + 1) * N_STEP, N_STEP, params=params)
     torch.cuda.synchronize()
+    lib.dsx_profile(1)
+    st = dsx.FloodStats()
+    lib.dsx_flood_stats_get(ctypes.byref(st), 1)
     t0 = time.perf_counter()
     gated = 0
-    pre_ms = 0.0
-    pre_work = 0
-    for _ in range(a.steps):
-        seeds, amps, k, cnt = eng.gate(0, N_SEEDS)
-        gated += cnt[2]
-        ms, work = eng.gate_profile()
-        pre_ms += ms[2]
-        pre_work += int(work[2])
+    seeds_done = 0
+    for s in range(a.steps):
+        c = eng.search(os.path.join(td, f"s{s}"), dsx.SEARCH_GATE, START + s * N_STEP, N_STEP, params=params)
+        seeds_done += N_STEP
+        gated += c[2]
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
+    lib.dsx_flood_stats_get(ctypes.byref(st), 1)
+    lib.dsx_profile(0)
     value = gated / sec
-    avg_launch_ms = pre_ms / a.steps
-    achieved = FLOPS_PER_PRE_SEED * (pre_work / a.steps) / (avg_launch_ms * 1e-3) / 1e12
+    # roofline of the tile evaluation kernel
+    samples = sum(st.sampled_cold[k] * pred_samples(k, False) + st.sampled_hot[k] * pred_samples(k, True)
+                  for k in range(4))
+    flops = samples * FLOPS_PER_SAMPLE
+    launches = max(1, st.eval_launches)
+    avg_ms = st.eval_ms / launches
+    achieved = flops / (st.eval_ms * 1e-3) / 1e12 if st.eval_ms > 0 else 0.0
+    cells_eval = sum(st.tiles[k] for k in range(4)) * 1024
+    sampled_big = sum(st.sampled_cold[k] + st.sampled_hot[k] for k in range(4))
     out = {
         "metric": "gated seeds/s on 1 MI355X vs host-CPU ref; p50/p99 area-query latency",
         "value": value, "unit": "gated seeds/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": 1e3 * sec / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (seed indices; seeds = splitmix64(k))",
-        "config": {"workload": "seed gate, BASELINE config 2: k in [0, 2^24), y-phase <= 0.06f, 4-parity "
-                               "pre-gate >= 0.645, full 512^2 gate >= 0.70",
-                   "seeds_per_step": N_SEEDS, "gated_per_step": cnt[2], "phase_survivors_per_step": cnt[0],
-                   "pregate_survivors_per_step": cnt[1]},
-        "roofline": {"bound": "valu", "kernel": "k_amp<0> (4-parity pre-gate)", "achieved": achieved,
-                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": None, "avg_launch_ms": avg_launch_ms,
-                     "flops_per_launch": FLOPS_PER_PRE_SEED * (pre_work / a.steps)},
-        "cpu_baseline": None if a.no_cpu else cpu_baseline(),
+        "dtype": "f64", "data": "synthetic (seed index ranges; seeds = splitmix64(k))",
+        "config": {"workload": "whole per-seed pipeline, GPU gate (gate=1) -> tables -> scan -> hot zones -> proxy "
+                               "floods -> exact floods; step s = seed indices [1e12 + s*1e8, +1e8)",
+                   "seed_indices_per_step": N_STEP, "gated_seeds": int(gated), "workers": int(params.workers) or None},
+        "roofline": {"bound": "fp64-valu", "kernel": "k_flood_eval (tiled flood engine, tile evaluation)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "avg_launch_ms": avg_ms, "launches": int(st.eval_launches),
+                     "flops_per_launch": flops / launches,
+                     "octave_samples": int(samples), "sampled_cells": int(sampled_big),
+                     "cells_evaluated": int(cells_eval)},
     }
     eng.close()
+    if not a.no_extra:
+        g = dsx.Engine(0)
+        g.gate(0, 1 << 24)
+        t1 = time.perf_counter()
+        _, _, _, cnt = g.gate(0, 1 << 24)
+        gsec = time.perf_counter() - t1
+        g.close()
+        out["gate_only"] = {"seed_indices": 1 << 24, "seconds": gsec, "gated_seeds_per_s": cnt[2] / gsec,
+                            "counts": list(cnt)}
+        out["area_queries"] = area_burst(dsx)
+        out["serving"] = serving(dsx)
+    out["cpu_baseline"] = None if a.no_cpu else cpu_baseline()
     print(json.dumps(out))
 
 

@@ -7,6 +7,7 @@
 import os
 import shutil
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
@@ -23,15 +24,33 @@ HIP_SOURCES = ["csrc/gate.hip", "csrc/tables.hip", "csrc/climate.hip", "csrc/flo
 HOST_TOOLS = ["host/dsearch_desert.cpp"]
 
 
+def _compile(src, obj):
+    """One translation unit to an object (every kernel is launched from the unit that defines it, so
+    the units link without relocatable device code)."""
+    flags = [f for f in HIP_FLAGS if f != "-shared"]
+    r = subprocess.run([HIPCC] + flags + ["-c", os.path.join(HERE, src), "-o", obj], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed on %s:\n%s%s" % (src, r.stdout, r.stderr))
+    return r.stderr
+
+
 def build_lib(verbose=False):
     out = os.path.join(HERE, "libdsx.so")
     tmp = out + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + [os.path.join(HERE, s) for s in HIP_SOURCES] + ["-o", tmp]
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    odir = os.path.join(HERE, "_build")
+    os.makedirs(odir, exist_ok=True)
+    objs = [os.path.join(odir, os.path.basename(s) + ".o") for s in HIP_SOURCES]
+    jobs = max(1, min(len(HIP_SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        logs = list(ex.map(_compile, HIP_SOURCES, objs))
+    r = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp], capture_output=True,
+                       text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
-    if verbose and r.stderr.strip():
-        print(r.stderr)
+        raise RuntimeError("hipcc link failed:\n" + r.stdout + r.stderr)
+    if verbose:
+        for log in logs:
+            if log.strip():
+                print(log)
     os.replace(tmp, out)
     for src in HOST_TOOLS:
         exe = os.path.join(HERE, os.path.splitext(os.path.basename(src))[0])

@@ -271,11 +271,14 @@ __device__ void climate_q(const ClimSeed &C, int x, int y, int z, int64_t *q) {
 }
 
 // surface.hpp:85-93
-__device__ bool is_desert(const ClimSeed &C, const SurfGrid &g, int x, int z) {
+// *hot (when given) is set when the temperature test passes, i.e. when the 36 further octave samples
+// of C, E, W and H are taken: the predicate's cost class.
+__device__ bool is_desert(const ClimSeed &C, const SurfGrid &g, int x, int z, bool *hot = nullptr) {
     double px, pz;
     shifted(C, x, z, px, pz);
     const float t = float(noise_sample(C, DSX_CH_T, px, 0, pz));
     if (int64_t(10000.f * t) < 5500) return false;
+    if (hot) *hot = true;
     const float c = float(noise_sample(C, DSX_CH_C, px, 0, pz));
     const float e = float(noise_sample(C, DSX_CH_E, px, 0, pz)), w = float(noise_sample(C, DSX_CH_W, px, 0, pz));
     const float v[4] = {c, e, ridges(w), w};
@@ -289,11 +292,12 @@ __device__ bool is_desert(const ClimSeed &C, const SurfGrid &g, int x, int z) {
 }
 
 // surface.hpp:98-124: 2 desert, 1 river that would be desert with weirdness +-0.0501, 0 other.
-__device__ int desert_state(const ClimSeed &C, const SurfGrid &g, int x, int z) {
+__device__ int desert_state(const ClimSeed &C, const SurfGrid &g, int x, int z, bool *hot = nullptr) {
     double px, pz;
     shifted(C, x, z, px, pz);
     const float t = float(noise_sample(C, DSX_CH_T, px, 0, pz));
     if (int64_t(10000.f * t) < 5500) return 0;
+    if (hot) *hot = true;
     const float c = float(noise_sample(C, DSX_CH_C, px, 0, pz)), e = float(noise_sample(C, DSX_CH_E, px, 0, pz));
     const float w = float(noise_sample(C, DSX_CH_W, px, 0, pz)), h = float(noise_sample(C, DSX_CH_H, px, 0, pz));
     int64_t q[6] = {int64_t(10000.f * t), int64_t(10000.f * h), int64_t(10000.f * c),
@@ -325,8 +329,9 @@ __device__ __forceinline__ int hot_state(const CS &C, int x, int z) {
 }
 // desert_proxy.hpp:25-32
 template <class CS>
-__device__ __forceinline__ float proxy_prob(const CS &C, int x, int z) {
+__device__ __forceinline__ float proxy_prob(const CS &C, int x, int z, bool *hot = nullptr) {
     if (noise_sample(C, DSX_CH_T, x, 0, z) < 0.55) return 0;
+    if (hot) *hot = true;
     const double c = noise_octaves(C, DSX_CH_C, x, z, 3), e = noise_octaves(C, DSX_CH_E, x, z, 2);
     int i = int(floor((c + 0.8) / 0.05)), j = int(floor((e + 1.0) / 0.05));
     i = i < 0 ? 0 : (i > 39 ? 39 : i);
@@ -334,20 +339,33 @@ __device__ __forceinline__ float proxy_prob(const CS &C, int x, int z) {
     return DSX_DPROB[i][j];
 }
 template <class CS>
-__device__ __forceinline__ int proxy_state(const CS &C, int x, int z, float cut) {
-    return proxy_prob(C, x, z) >= cut ? 2 : 0;
+__device__ __forceinline__ int proxy_state(const CS &C, int x, int z, float cut, bool *hot = nullptr) {
+    return proxy_prob(C, x, z, hot) >= cut ? 2 : 0;
 }
 
 enum PredKind { PRED_DESERT_STATE = 0, PRED_HOT = 1, PRED_PROXY = 2, PRED_DESERT = 3 };
 
-__device__ __forceinline__ int predicate(int kind, const ClimSeed &C, const SurfGrid &g, int x, int z, float cut) {
+__device__ __forceinline__ int predicate(int kind, const ClimSeed &C, const SurfGrid &g, int x, int z, float cut,
+                                         bool *hot = nullptr) {
     switch (kind) {
-    case PRED_DESERT_STATE: return desert_state(C, g, x, z);
+    case PRED_DESERT_STATE: return desert_state(C, g, x, z, hot);
     case PRED_HOT: return hot_state(C, x, z);
-    case PRED_PROXY: return proxy_state(C, x, z, cut);
-    default: return is_desert(C, g, x, z) ? 2 : 0;
+    case PRED_PROXY: return proxy_state(C, x, z, cut, hot);
+    default: return is_desert(C, g, x, z, hot) ? 2 : 0;
     }
 }
+
+// Octave samples (Perlin::sample calls) one predicate evaluation takes, by kind and cost class
+// (cold: the temperature test failed). Shift 2 x 6, T 4, C 18, E 8, W 6, H 4 (standalone.hpp:94-100);
+// the proxy reads C octaves 0-2 and E octaves 0-1 of both sides (desert_proxy.hpp:14-24).
+__host__ __device__ constexpr int pred_samples(int kind, bool hot) {
+    return kind == PRED_HOT ? 4 : kind == PRED_PROXY ? (hot ? 14 : 4) : (hot ? 52 : 16);
+}
+// FP64 operations of one octave sample as Perlin::sample writes them (standalone.hpp:62-82) with
+// NormalNoise's weighting (standalone.hpp:146-149), y = 0: 3 coordinate scalings, 2 offsets, 2 floors,
+// 2 fraction subtractions, 2 fades x 7, 3 corner offsets, 8 gradient dots x 5, 7 lerps x 3, weight
+// and accumulate 2.
+constexpr int FLOPS_PER_SAMPLE = 89;
 
 void launch_climinit(hipStream_t st, const uint64_t *seeds, const uint32_t *slots, uint32_t n, ClimSeed *out);
 

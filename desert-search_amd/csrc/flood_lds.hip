@@ -22,11 +22,11 @@ constexpr int LDS_WORDS = (LDS_N * LDS_N + 31) / 32;           // 2033 words per
 constexpr int LDS_RING = 4096;                                 // pushed cells in flight
 
 template <class CS>
-__device__ __forceinline__ int lds_pred(int kind, const CS &C, const SurfGrid &g, int x, int z, float cut) {
+__device__ __forceinline__ int lds_pred(int kind, const CS &C, const SurfGrid &g, int x, int z, float cut, bool *hot) {
     if constexpr (std::is_same<CS, ClimT>::value) {
         return hot_state(C, x, z);
     } else {
-        return predicate(kind, C, g, x, z, cut);
+        return predicate(kind, C, g, x, z, cut, hot);
     }
 }
 
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         }
     }
     __syncthreads();
-    unsigned long long cells = 0, sampled = 0;
+    unsigned long long cells = 0, sampled = 0, hotc = 0;
     int minx = J.qx, maxx = J.qx, minz = J.qz, maxz = J.qz;
     bool edge = false, border = false;
     // the centre is touched first (dense_flood.hpp:80)
@@ -82,7 +82,9 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
                 continue;
             }
             sampled++;
-            const int v = lds_pred<CS>(J.kind, C, g, x, z, J.cut);
+            bool hot = false;
+            const int v = lds_pred<CS>(J.kind, C, g, x, z, J.cut, &hot);
+            hotc += hot;
             if (v == 0) continue;
             if (v == 2) {
                 atomicOr(&counted[c >> 5], 1u << (c & 31));
@@ -128,6 +130,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
     for (int o = 32; o > 0; o >>= 1) {
         cells += __shfl_xor(cells, o);
         sampled += __shfl_xor(sampled, o);
+        hotc += __shfl_xor(hotc, o);
         minx = min(minx, __shfl_xor(minx, o));
         maxx = max(maxx, __shfl_xor(maxx, o));
         minz = min(minz, __shfl_xor(minz, o));
@@ -148,6 +151,8 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         r.maxz = maxz;
         r.edge = any_edge;
         r.border = any_border;
+        r.hot = hotc;
+        r.tiles = 0;
         res[job] = r;
     }
     if (cells == 0) return;

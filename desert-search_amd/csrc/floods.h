@@ -18,6 +18,8 @@ struct FloodRes {
     unsigned long long cells, sampled;
     int minx, maxx, minz, maxz;
     int edge, border;
+    unsigned long long hot;     // sampled cells that passed the predicate's temperature test
+    unsigned long long tiles;   // tiles the engine allocated for this flood (0: small-window engine)
 };
 // A 32x32 block of one flood's window: window cell (i, j) = (tx*32 + c, tz*32 + r) <-> bit c of row r.
 struct Tile {
@@ -30,6 +32,7 @@ struct Tile {
     uint32_t cnt[FT];       // predicate == 2
     uint32_t reach[FT];     // cells the flood has entered (pushed)
     uint32_t in[FT];        // cells touched from a neighbouring tile (and the start cell)
+    uint32_t hot[FT];       // the predicate's temperature test passed (its costlier branch ran)
 };
 
 struct TileMask {
@@ -59,8 +62,9 @@ void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl
 // into the next round's lists. `width`: the tiles the host expects per list (a grid hint: any grid
 // covers every listed tile).
 constexpr uint32_t EVAL_GRID = 512, PROP_GRID = 2048;
+// ev0/ev1 (optional): HIP events recorded when the evaluation kernel starts and ends.
 void launch_flood_round(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B, int round,
-                        uint32_t width);
+                        uint32_t width, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // Reduce, export the counted masks, and return every touched tile and hash slot to the zero / empty
 // state the next call expects (no pool-wide memsets).
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res,

@@ -16,6 +16,7 @@
 //       the world, edge = reached cell on the window boundary, bbox over counted reached cells
 //       (starting at the window centre, as dense_flood.hpp:57).
 #include "floods.h"
+#include <hip/hip_ext.h>
 #include <climits>
 
 namespace dsx {
@@ -172,16 +173,19 @@ __global__ __launch_bounds__(FT * FT) void k_flood_eval(const FloodJob *jobs, co
         const int r = tid / FT, col = tid % FT;
         const int i = T.tx * FT + col, j = T.tz * FT + r;
         int v = 0;
+        bool hot = false;
         if (i < nw && j < nw) {
             const int x = J.qx + (i - J.half) * J.step, z = J.qz + (j - J.half) * J.step;
-            if (x >= -WORLD_Q && x <= WORLD_Q && z >= -WORLD_Q && z <= WORLD_Q) v = predicate(J.kind, C, g, x, z, J.cut);
+            if (x >= -WORLD_Q && x <= WORLD_Q && z >= -WORLD_Q && z <= WORLD_Q) v = predicate(J.kind, C, g, x, z, J.cut, &hot);
         }
-        const uint64_t mp = __ballot(v != 0), mc = __ballot(v == 2);
+        const uint64_t mp = __ballot(v != 0), mc = __ballot(v == 2), mh = __ballot(hot);
         if ((tid & 63) == 0) {
             T.pass[r] = uint32_t(mp);
             T.pass[r + 1] = uint32_t(mp >> 32);
             T.cnt[r] = uint32_t(mc);
             T.cnt[r + 1] = uint32_t(mc >> 32);
+            T.hot[r] = uint32_t(mh);
+            T.hot[r + 1] = uint32_t(mh >> 32);
         }
     }
 }
@@ -328,6 +332,7 @@ __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBuf
         }
     }
     unsigned long long sampled = __popc(touched & world);
+    unsigned long long hotc = __popc(touched & world & (lane < FT ? T.hot[r] : 0u));
     const bool border = (touched & ~world) != 0;
     const uint32_t RC = R & (lane < FT ? T.cnt[r] : 0u);
     unsigned long long cells = __popc(RC);
@@ -348,6 +353,7 @@ __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBuf
     }
     for (int o = 32; o > 0; o >>= 1) {
         sampled += __shfl_xor(sampled, o);
+        hotc += __shfl_xor(hotc, o);
         cells += __shfl_xor(cells, o);
         minx = min(minx, __shfl_xor(minx, o));
         maxx = max(maxx, __shfl_xor(maxx, o));
@@ -357,7 +363,9 @@ __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBuf
     const bool any_border = __any(border), any_edge = __any(edge);
     if (lane == 0) {
         FloodRes &o = res[T.job];
+        atomicAdd(&o.tiles, 1ull);
         if (sampled) atomicAdd(&o.sampled, sampled);
+        if (hotc) atomicAdd(&o.hot, hotc);
         if (cells) {
             atomicAdd(&o.cells, cells);
             atomicMin(&o.minx, minx);
@@ -405,9 +413,11 @@ void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl
     if (n) hipLaunchKernelGGL(k_flood_start, dim3((n + 255) / 256), dim3(256), 0, st, jobs, jl, n, B, eval, prop);
 }
 void launch_flood_round(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B, int round,
-                        uint32_t width) {
+                        uint32_t width, hipEvent_t ev0, hipEvent_t ev1) {
     const uint32_t w = width < 8 ? 8 : width;
-    hipLaunchKernelGGL(k_flood_eval, dim3(w < EVAL_GRID ? w : EVAL_GRID), dim3(FT * FT), 0, st, jobs, cs, g, B, round);
+    const dim3 eg(w < EVAL_GRID ? w : EVAL_GRID);
+    if (ev0) hipExtLaunchKernelGGL(k_flood_eval, eg, dim3(FT * FT), 0, st, ev0, ev1, 0u, jobs, cs, g, B, round);
+    else hipLaunchKernelGGL(k_flood_eval, eg, dim3(FT * FT), 0, st, jobs, cs, g, B, round);
     hipLaunchKernelGGL(k_flood_prop, dim3(w < PROP_GRID ? w : PROP_GRID), dim3(64), 0, st, jobs, B, round);
 }
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res, TileMask *masks,

@@ -4,6 +4,7 @@
 #include "ctx.h"
 #include "floods.h"
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cstdio>
@@ -95,6 +96,7 @@ struct ClimState {
     uint32_t mask_cap = 0;
     std::vector<dsx::TileMask> masks;
     uint32_t *d_jl = nullptr;   // job index lists: small hot + small other (<= job_cap), then tiled (<= job_cap)
+    std::vector<hipEvent_t> events;   // evaluation-kernel event pairs (profiling on)
 };
 
 static ClimState *clim_state(dsx_ctx *c);
@@ -192,6 +194,26 @@ static int clim_slots(dsx_ctx *c, ClimState *S, const uint64_t *seeds, uint32_t 
     return DSX_OK;
 }
 
+// Process-wide flood-engine statistics (dsx_flood_stats_get) and the profiling switch (dsx_profile).
+static std::atomic<int> g_prof_on{0};
+static std::mutex g_stats_mu;
+static dsx_flood_stats g_stats{};
+static void stats_add(const dsx_flood_stats &a) {
+    std::lock_guard<std::mutex> l(g_stats_mu);
+    g_stats.calls += a.calls;
+    g_stats.rounds += a.rounds;
+    for (int k = 0; k < 4; k++) {
+        g_stats.floods[k] += a.floods[k];
+        g_stats.tiles[k] += a.tiles[k];
+        g_stats.sampled_cold[k] += a.sampled_cold[k];
+        g_stats.sampled_hot[k] += a.sampled_hot[k];
+    }
+    g_stats.small_floods += a.small_floods;
+    g_stats.small_sampled += a.small_sampled;
+    g_stats.eval_ms += a.eval_ms;
+    g_stats.eval_launches += a.eval_launches;
+}
+
 // All per-context extension state lives here; freed by dsx_destroy via ctx_free_ext.
 #include <mutex>
 static std::mutex g_ext_mu;
@@ -217,6 +239,7 @@ void dsx_ctx_free_ext(dsx_ctx *c) {
                     (void *)s->fb.lcount, (void *)s->d_jobs,
                     (void *)s->d_res, s->d_io, (void *)s->d_masks, (void *)s->d_jl})
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
     delete s;
 }
 
@@ -351,22 +374,9 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         dsx::launch_flood_start(c->st, S->d_jobs, d_big, uint32_t(jl_big.size()), B, B.lists[0], B.lists[1]);
         HIPCK(hipGetLastError());
     }
-    static double prof_t = 0, prof_rounds = 0, prof_calls = 0, prof_tiles = 0, prof_sampled = 0, prof_bsampled = 0,
-                  prof_evald = 0;
-    struct ProfOut {
-        double *t, *r, *c, *n, *s, *bs, *ev;
-        ~ProfOut() {
-            if (std::getenv("DSX_PROFILE"))
-                std::fprintf(stderr,
-                             "dsx_flood profile: %.0f calls, %.0f rounds, %.0f tiles (%.0f cells evaluated, %.0f sampled by "
-                             "tiled floods), %.0f sampled in all, %.3f s\n",
-                             *c, *r, *n, *ev, *bs, *s, *t);
-        }
-    };
-    static ProfOut prof_out{&prof_t, &prof_rounds, &prof_calls, &prof_tiles, &prof_sampled, &prof_bsampled, &prof_evald};
-    static std::mutex prof_mu;
-    double my_rounds = 0, my_tiles = 0, my_sampled = 0, my_bsampled = 0;
-    const auto prof_t0 = std::chrono::steady_clock::now();
+    const bool prof = g_prof_on.load(std::memory_order_relaxed) != 0;
+    uint64_t my_rounds = 0;
+    size_t nev = 0;   // events used by this call (pairs around the evaluation launches)
     // rounds, queued several at a time between synchronisations (a finished flood leaves an empty
     // list, so extra rounds do nothing); grids sized from the list length seen at the last check
     uint32_t *h = c->h_cnt;
@@ -375,8 +385,21 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         int round = 0, burst = 2;
         uint32_t width = uint32_t(jl_big.size());
         for (;;) {
-            for (int k = 0; k < burst; k++, round++)
-                dsx::launch_flood_round(c->st, S->d_jobs, S->d_clim, S->grid, B, round, 2 * width + 16 * uint32_t(k + 1));
+            for (int k = 0; k < burst; k++, round++) {
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (prof) {
+                    while (S->events.size() < nev + 2) {
+                        hipEvent_t e;
+                        HIPCK(hipEventCreate(&e));
+                        S->events.push_back(e);
+                    }
+                    e0 = S->events[nev];
+                    e1 = S->events[nev + 1];
+                    nev += 2;
+                }
+                dsx::launch_flood_round(c->st, S->d_jobs, S->d_clim, S->grid, B, round, 2 * width + 16 * uint32_t(k + 1), e0,
+                                        e1);
+            }
             HIPCK(hipGetLastError());
             my_rounds += burst;
             HIPCK(hipMemcpyAsync(h, B.lcount, 32, hipMemcpyDeviceToHost, c->st));
@@ -395,7 +418,6 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     HIPCK(hipMemcpyAsync(&ntiles, B.ntiles, 4, hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
     ntiles = std::min(ntiles, B.tile_cap);
-    my_tiles += ntiles;
     if (status == DSX_OK) {
         dsx::launch_flood_acc(c->st, S->d_jobs, B, ntiles, S->d_res, S->d_masks, B.lcount + dsx::FLOOD_MASKS, S->mask_cap);
         HIPCK(hipGetLastError());
@@ -413,8 +435,30 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
                                  hipMemcpyDeviceToHost, c->st));
             HIPCK(hipStreamSynchronize(c->st));
         }
-        for (uint64_t k = 0; k < n; k++) my_sampled += double(res[k].sampled);
-        for (uint32_t k : jl_big) my_bsampled += double(res[k].sampled);
+        dsx_flood_stats st{};
+        st.calls = 1;
+        st.rounds = my_rounds;
+        std::vector<char> big(n, 0);
+        for (uint32_t k : jl_big) big[k] = 1;
+        for (uint64_t k = 0; k < n; k++) {
+            const int kind = hj[k].kind;
+            if (big[k]) {
+                st.floods[kind]++;
+                st.tiles[kind] += res[k].tiles;
+                st.sampled_hot[kind] += res[k].hot;
+                st.sampled_cold[kind] += res[k].sampled - res[k].hot;
+            } else {
+                st.small_floods++;
+                st.small_sampled += res[k].sampled;
+            }
+        }
+        for (size_t e = 0; e < nev; e += 2) {
+            float ms = 0;
+            HIPCK(hipEventElapsedTime(&ms, S->events[e], S->events[e + 1]));
+            st.eval_ms += ms;
+            st.eval_launches++;
+        }
+        stats_add(st);
         for (uint64_t k = 0; k < n; k++)
             out[k] = dsx_area{res[k].cells, res[k].sampled, res[k].minx, res[k].maxx, res[k].minz, res[k].maxz,
                               res[k].edge, res[k].border};
@@ -428,17 +472,20 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         HIPCK(hipMemsetAsync(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8, c->st));
     }
     HIPCK(hipStreamSynchronize(c->st));
-    {
-        std::lock_guard<std::mutex> l(prof_mu);
-        prof_t += std::chrono::duration<double>(std::chrono::steady_clock::now() - prof_t0).count();
-        prof_calls++;
-        prof_rounds += my_rounds;
-        prof_tiles += my_tiles;
-        prof_sampled += my_sampled;
-        prof_bsampled += my_bsampled;
-        prof_evald += my_tiles * 1024;
-    }
     return status;
+}
+
+int dsx_profile(int on) {
+    g_prof_on.store(on ? 1 : 0);
+    return DSX_OK;
+}
+
+int dsx_flood_stats_get(dsx_flood_stats *out, int reset) {
+    if (!out) return DSX_E_ARG;
+    std::lock_guard<std::mutex> l(g_stats_mu);
+    *out = g_stats;
+    if (reset) g_stats = dsx_flood_stats{};
+    return DSX_OK;
 }
 
 int dsx_flood_masks(dsx_ctx *c, dsx_tile_mask *out, uint64_t cap, uint64_t *n_out) {

@@ -98,6 +98,14 @@ def flood_jobs(rows):
     return a
 
 
+class FloodStats(ctypes.Structure):
+    """dsx_flood_stats: process-wide flood-engine counters (per kind index DSX_PRED_*)."""
+    _fields_ = [("calls", ctypes.c_uint64), ("rounds", ctypes.c_uint64), ("floods", ctypes.c_uint64 * 4),
+                ("tiles", ctypes.c_uint64 * 4), ("sampled_cold", ctypes.c_uint64 * 4),
+                ("sampled_hot", ctypes.c_uint64 * 4), ("small_floods", ctypes.c_uint64),
+                ("small_sampled", ctypes.c_uint64), ("eval_ms", ctypes.c_double), ("eval_launches", ctypes.c_uint64)]
+
+
 class GateCounts(ctypes.Structure):
     _fields_ = [("phase", ctypes.c_uint64), ("pre", ctypes.c_uint64), ("full", ctypes.c_uint64)]
 
@@ -155,10 +163,12 @@ def load(path=LIB_PATH):
     lib.dsx_result_text.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p,
                                     ctypes.c_uint64]
     lib.dsx_release.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+    lib.dsx_profile.argtypes = [ctypes.c_int]
+    lib.dsx_flood_stats_get.argtypes = [ctypes.POINTER(FloodStats), ctypes.c_int]
     for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile", "dsx_scan", "dsx_tables",
               "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood", "dsx_search", "dsx_area_queries",
               "dsx_server_create", "dsx_server_destroy", "dsx_server_workers", "dsx_submit_scan", "dsx_submit_area",
-              "dsx_wait", "dsx_poll", "dsx_result_text", "dsx_release"):
+              "dsx_wait", "dsx_poll", "dsx_result_text", "dsx_release", "dsx_profile", "dsx_flood_stats_get"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib

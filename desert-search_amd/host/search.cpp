@@ -147,6 +147,17 @@ struct Prof {
         static const char *name[8] = {"gate", "scan", "sites", "points", "floods", "rows", "steps", "flood jobs"};
         for (int i = 0; i < 8; i++)
             if (t[i] > 0 || n[i]) std::fprintf(stderr, "dsx_search profile: %-10s %8.3f s  %llu\n", name[i], t[i], (unsigned long long)n[i]);
+        dsx_flood_stats f;
+        dsx_flood_stats_get(&f, 0);
+        std::fprintf(stderr, "dsx_flood profile: %llu calls, %llu rounds; small-window floods %llu (%llu sampled)\n",
+                     (unsigned long long)f.calls, (unsigned long long)f.rounds, (unsigned long long)f.small_floods,
+                     (unsigned long long)f.small_sampled);
+        static const char *kind[4] = {"desert_state", "hot", "proxy", "desert"};
+        for (int k = 0; k < 4; k++)
+            if (f.floods[k])
+                std::fprintf(stderr, "dsx_flood profile: %-12s %8llu floods %9llu tiles, sampled %llu cold + %llu hot\n", kind[k],
+                             (unsigned long long)f.floods[k], (unsigned long long)f.tiles[k],
+                             (unsigned long long)f.sampled_cold[k], (unsigned long long)f.sampled_hot[k]);
     }
 };
 Prof g_prof;
@@ -845,7 +856,7 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
         return DSX_E_ARG;
     const dsx_search_params P = *pp;
     if (P.study || P.nms <= 0 || P.max_sites < 0 || P.workers < 0 || P.workers > 64) return DSX_E_ARG;
-    const int nw = P.workers ? P.workers : 24;
+    const int nw = P.workers ? P.workers : 16;
     std::vector<dsx_ctx *> wctx;
     int rc = worker_contexts(ctx, nw, wctx);
     if (rc) return rc;

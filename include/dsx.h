@@ -174,6 +174,24 @@ typedef struct {
 } dsx_tile_mask;
 int dsx_flood_masks(dsx_ctx *ctx, dsx_tile_mask *out, uint64_t cap, uint64_t *n_out);
 
+/* Flood-engine statistics, process-wide over every context (no counterpart in the reference, which
+ * times its stages with wall clocks, dsearch_desert.cpp:590-593). Counted per dsx_flood call that
+ * succeeds. "Sampled" cells are those the reference's flood evaluates (DenseArea::sampled), split by
+ * cost class: hot = the predicate's temperature test passed, so its longer branch ran. */
+typedef struct {
+    uint64_t calls, rounds;        /* dsx_flood calls; evaluate + propagate rounds of the tiled engine */
+    uint64_t floods[4];            /* floods run by the tiled engine, by kind (DSX_PRED_*) */
+    uint64_t tiles[4];             /* 32x32 tiles it evaluated, by kind (x 1024 = cells evaluated) */
+    uint64_t sampled_cold[4];      /* sampled cells of those floods, by kind and cost class */
+    uint64_t sampled_hot[4];
+    uint64_t small_floods, small_sampled; /* floods of the small-window engine and their sampled cells */
+    double eval_ms;                /* device time of the tile evaluation kernel (HIP events; profiling on) */
+    uint64_t eval_launches;        /* its launches timed */
+} dsx_flood_stats;
+/* 1: bracket every launch of the tile evaluation kernel with HIP events on its stream (eval_ms). */
+int dsx_profile(int on);
+int dsx_flood_stats_get(dsx_flood_stats *out, int reset);
+
 /* ---------------------------------------------------------------- the whole per-seed search */
 /* struct Params of the reference (dsearch_desert.cpp:50-78), same names and defaults. */
 typedef struct {
@@ -187,7 +205,7 @@ typedef struct {
     int32_t study, gpu, gpucheck, gscan, gscancheck, gate;
     double gate_phase, gate_amp, gate_pthr;
     int32_t from_stdin;
-    int32_t workers; /* pipeline workers, each with its own stream and device buffers (0: default 24) */
+    int32_t workers; /* pipeline workers, each with its own stream and device buffers (0: default 16) */
 } dsx_search_params;
 void dsx_search_defaults(dsx_search_params *p);
 
