@@ -40,6 +40,7 @@ import threading
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")   # one hardware queue per pipeline worker stream
 START = 10 ** 12
 N_STEP = 10 ** 8
 FP64_PEAK_TFLOPS = 78.6
@@ -244,7 +245,13 @@ def main():
                      "avg_launch_ms": avg_ms, "launches": int(st.eval_launches),
                      "flops_per_launch": flops / launches,
                      "octave_samples": int(samples), "sampled_cells": int(sampled_big),
-                     "cells_evaluated": int(cells_eval)},
+                     "cells_evaluated": int(cells_eval),
+                     "by_kind": {KINDS[k]: {"floods": st.floods[k], "tiles": st.tiles[k],
+                                            "sampled_cold": st.sampled_cold[k], "sampled_hot": st.sampled_hot[k],
+                                            "by_size": [[st.size_floods[k][b], st.size_tiles[k][b],
+                                                         st.size_sampled[k][b]] for b in range(6)]}
+                                 for k in range(4) if st.floods[k]},
+                     "small_window_floods": st.small_floods, "small_window_sampled": st.small_sampled},
     }
     eng.close()
     if not a.no_extra:

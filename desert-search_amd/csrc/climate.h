@@ -15,6 +15,7 @@
 // the device (k_climinit) and read from LDS by the flood kernels.
 #pragma once
 #include "dsx_internal.h"
+#include <type_traits>
 #include "../data/mc_tables.inc"
 
 namespace dsx {
@@ -96,8 +97,31 @@ struct ClimT {
     double factor[6];
 };
 
+// The 14 octaves the proxy predicate reads (T both sides, C octaves 0-2 and E octaves 0-1 of both
+// sides; desert_proxy.hpp:14-32): the slots of ClimSeed in this order.
+constexpr int PROXY_SLOTS[14] = {0, 1, 2, 3, 8, 9, 10, 17, 18, 19, 26, 27, 30, 31};
+struct ClimP {
+    GradTab g;
+    ClimOct oct[14];
+    double factor[6];
+};
+
 static_assert(offsetof(ClimT, oct) == offsetof(ClimSeed, oct), "ClimT is a prefix of ClimSeed");
-static_assert(sizeof(ClimSeed) % 16 == 0 && offsetof(ClimT, factor) % 16 == 0, "uint4 copies");
+static_assert(offsetof(ClimP, oct) == offsetof(ClimSeed, oct), "ClimP starts like ClimSeed");
+static_assert(sizeof(ClimSeed) % 16 == 0 && offsetof(ClimT, factor) % 16 == 0 && offsetof(ClimP, factor) % 16 == 0 &&
+                  sizeof(ClimOct) % 16 == 0,
+              "uint4 copies");
+
+// Slot of octave k of `side` of channel ch in a climate block CS. ClimSeed and ClimT hold the slots in
+// ClimSeed order; ClimP holds PROXY_SLOTS (only the octaves the proxy reads are valid).
+template <class CS>
+__host__ __device__ constexpr int oct_slot(int ch, int side, int k) {
+    if constexpr (std::is_same<CS, ClimP>::value) {
+        return ch == DSX_CH_T ? side * 2 + k : ch == DSX_CH_C ? 4 + side * 3 + k : 10 + side * 2 + k;
+    } else {
+        return clim_base(ch) + side * clim_n(ch) + k;
+    }
+}
 
 // Copy the part of a seed's climate that CS holds (all of it, or ClimT's prefix and the factors);
 // threads lane, lane + nt, ... of the block participate.
@@ -105,21 +129,29 @@ template <class CS>
 __device__ __forceinline__ void copy_climate(CS &dst, const ClimSeed &src, int lane, int nt) {
     const uint4 *s = reinterpret_cast<const uint4 *>(&src);
     uint4 *d = reinterpret_cast<uint4 *>(&dst);
-    constexpr int pre = int(offsetof(CS, factor) / 16);
-    for (int k = lane; k < pre; k += nt) d[k] = s[k];
+    if constexpr (std::is_same<CS, ClimP>::value) {
+        constexpr int GW = int(sizeof(GradTab) / 16), OW = int(sizeof(ClimOct) / 16);
+        for (int k = lane; k < GW + 14 * OW; k += nt) {
+            const int o = (k - GW) / OW, w = (k - GW) % OW;
+            d[k] = k < GW ? s[k] : s[GW + PROXY_SLOTS[o] * OW + w];
+        }
+    } else {
+        constexpr int pre = int(offsetof(CS, factor) / 16);
+        for (int k = lane; k < pre; k += nt) d[k] = s[k];
+    }
     if (lane < 6) dst.factor[lane] = src.factor[lane];
 }
 
 // NormalNoise::sample: per-side sums, then (A + B) * factor. CS: ClimSeed, or ClimT for channel T.
 template <class CS>
 __device__ __forceinline__ double noise_sample(const CS &C, int ch, double x, double y, double z) {
-    const int n = clim_n(ch), b = clim_base(ch);
+    const int n = clim_n(ch);
     double sum0 = 0, sum1 = 0;
     {
         const double px = x, py = y, pz = z;
 #pragma unroll 1
         for (int j = 0; j < n; j++) {
-            const ClimOct &o = C.oct[b + j];
+            const ClimOct &o = C.oct[oct_slot<CS>(ch, 0, j)];
             sum0 += o.weight * perlin_sample(C.g, o, px * o.freq, py * o.freq, pz * o.freq);
         }
     }
@@ -128,7 +160,7 @@ __device__ __forceinline__ double noise_sample(const CS &C, int ch, double x, do
         const double px = x * sc, py = y * sc, pz = z * sc;
 #pragma unroll 1
         for (int j = 0; j < n; j++) {
-            const ClimOct &o = C.oct[b + n + j];
+            const ClimOct &o = C.oct[oct_slot<CS>(ch, 1, j)];
             sum1 += o.weight * perlin_sample(C.g, o, px * o.freq, py * o.freq, pz * o.freq);
         }
     }
@@ -137,13 +169,13 @@ __device__ __forceinline__ double noise_sample(const CS &C, int ch, double x, do
 // DesertProxy::octaves: one running sum over both sides, octaves [0, hi).
 template <class CS>
 __device__ __forceinline__ double noise_octaves(const CS &C, int ch, double x, double z, int hi) {
-    const int n = clim_n(ch), b = clim_base(ch);
+    const int n = clim_n(ch);
     double s = 0;
     for (int side = 0; side < 2; side++) {
         const double sc = side ? 337.0 / 331.0 : 1.0;
 #pragma unroll 1
         for (int k = 0; k < hi && k < n; k++) {
-            const ClimOct &o = C.oct[b + side * n + k];
+            const ClimOct &o = C.oct[oct_slot<CS>(ch, side, k)];
             s += o.weight * perlin_sample(C.g, o, x * sc * o.freq, 0, z * sc * o.freq);
         }
     }

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -91,6 +92,13 @@ int dsx_create(int device, dsx_ctx **out) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
         g_last_error = std::string("device is ") + prop.gcnArchName + ", library built for gfx950";
         return DSX_E_NODEVICE;
+    }
+    // host wait mode of stream synchronisations (DSX_SYNC=spin|yield|block; default: the runtime's)
+    if (const char *m = std::getenv("DSX_SYNC")) {
+        const unsigned f = !std::strcmp(m, "spin") ? hipDeviceScheduleSpin
+                           : !std::strcmp(m, "yield") ? hipDeviceScheduleYield
+                           : !std::strcmp(m, "block") ? hipDeviceScheduleBlockingSync : hipDeviceScheduleAuto;
+        (void)hipSetDeviceFlags(f);
     }
     dsx_ctx *c = new dsx_ctx();
     c->device = device;

@@ -1,33 +1,52 @@
-// Small-window floods in one wave with the whole window state in LDS (dense_flood.hpp:52-86).
+// Floods in one wave with their state in LDS (dense_flood.hpp:52-86), evaluating exactly the cells
+// the reference touches.
 //
-// For windows up to 255 x 255 lattice points (the step-256 hot-zone pre-floods of
-// dsearch_desert.cpp:337, the most numerous floods of the pipeline) the tiled engine would evaluate
-// whole 32x32 tiles around a zone of a few dozen cells. Here one wave floods one window and evaluates
-// exactly the cells the reference touches:
-//   * touched and counted bitmaps (1 bit per cell each) and a ring of pushed cells live in LDS;
+// The wave keeps a local sub-window of at most 255 x 255 lattice points around the flood's start: the
+// whole window when it is that small (the step-256 hot-zone pre-floods of dsearch_desert.cpp:337),
+// else the 255 x 255 points centred on the start (step-128 hot zones :352 and step-32 proxy floods
+// :387, whose components are almost always far smaller than their windows). The tiled engine would
+// evaluate whole 32x32 tiles around such components; here:
+//   * touched and counted bitmaps (1 bit per local cell each) and a ring of pushed cells live in LDS;
 //   * a step pops up to 64 pushed cells (one per lane), claims their untouched in-window neighbours
 //     with LDS atomicOr on the touched bitmap and lists the claimed cells; the listed cells are then
 //     evaluated 64 at a time (all lanes busy on the FP64 predicate), and passable ones are pushed;
 //   * the outputs (cells, sampled, bbox, edge, border) are order independent, so the visiting order
 //     (wave-parallel, ring order) gives exactly the reference's depth-first results.
-// A window whose ring would overflow is reported (res.border = -1) and handed to the tiled engine.
+// A flood that reaches the edge of its local sub-window inside the real window, or whose ring would
+// overflow, is reported unfinished (res.border = -1); the host reruns it on the tiled engine.
 #include "floods.h"
 #include <climits>
 #include <type_traits>
 
 namespace dsx {
 
-constexpr int LDS_N = 255;                                    // largest window edge
+constexpr int LDS_N = 255;                                    // largest local sub-window edge
 constexpr int LDS_WORDS = (LDS_N * LDS_N + 31) / 32;           // 2033 words per bitmap
-constexpr int LDS_RING = 4096;                                 // pushed cells in flight
+constexpr int LDS_RING = 2048;                                 // pushed cells in flight
 
 template <class CS>
 __device__ __forceinline__ int lds_pred(int kind, const CS &C, const SurfGrid &g, int x, int z, float cut, bool *hot) {
     if constexpr (std::is_same<CS, ClimT>::value) {
         return hot_state(C, x, z);
+    } else if constexpr (std::is_same<CS, ClimP>::value) {
+        return proxy_state(C, x, z, cut, hot);
     } else {
         return predicate(kind, C, g, x, z, cut, hot);
     }
+}
+
+// 32 bits of row lj of a local bitmap (row length L bits) starting at local column c0 (c0 may be
+// negative or run past L: those columns read as 0).
+__device__ __forceinline__ uint32_t bitmap_row32(const uint32_t *bm, int L, int lj, int c0) {
+    const int lo = c0 < 0 ? 0 : c0, hi = c0 + 32 > L ? L : c0 + 32;   // columns [lo, hi) exist
+    if (hi <= lo) return 0u;
+    const int b = lj * L + lo;
+    const int w0 = b >> 5, sh = b & 31;
+    const uint64_t two = uint64_t(bm[w0]) | (uint64_t(w0 + 1 < LDS_WORDS ? bm[w0 + 1] : 0u) << 32);
+    uint32_t v = uint32_t(two >> sh);
+    const int n = hi - lo;
+    if (n < 32) v &= (1u << n) - 1u;
+    return v << (lo - c0);
 }
 
 template <class CS>
@@ -44,8 +63,11 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
     const uint32_t job = list[b];
     const FloodJob J = jobs[job];
     const int lane = threadIdx.x;
-    const int nw = 2 * J.half + 1, ncell = nw * nw;
-    {   // climate (only the octaves the predicate reads when CS is ClimT) and cleared bitmaps
+    const int nw = 2 * J.half + 1;
+    // local sub-window: columns/rows [o, o + L) of the window
+    const int L = nw < LDS_N ? nw : LDS_N, o = J.half - (L - 1) / 2;
+    const int ncell = L * L;
+    {   // climate (only the octaves the predicate reads unless CS is ClimSeed) and cleared bitmaps
         copy_climate(C, cs[J.seed], lane, 64);
         const int words = (ncell + 31) / 32;
         for (int k = lane; k < words; k += 64) {
@@ -65,7 +87,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
     bool edge = false, border = false;
     // the centre is touched first (dense_flood.hpp:80)
     if (lane == 0) {
-        const int c = J.half * nw + J.half;
+        const int c = (J.half - o) * L + (J.half - o);
         touched[c >> 5] |= 1u << (c & 31);
         evl[0] = uint32_t(c);
         nev = 1;
@@ -75,7 +97,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         // ---- evaluate the claimed cells, push the passable ones
         const uint32_t ne = nev;
         for (uint32_t e = lane; e < ne; e += 64) {
-            const int c = int(evl[e]), i = c % nw, j = c / nw;
+            const int c = int(evl[e]), i = c % L + o, j = c / L + o;
             const int x = J.qx + (i - J.half) * J.step, z = J.qz + (j - J.half) * J.step;
             if (x < -WORLD_Q || x > WORLD_Q || z < -WORLD_Q || z > WORLD_Q) {
                 border = true;
@@ -106,17 +128,22 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         // ---- expand up to 64 pushed cells: claim their untouched neighbours
         const uint32_t np = min(tl - h, 64u);
         if (uint32_t(lane) < np) {
-            const int c = int(ring[(h + lane) % LDS_RING]), i = c % nw, j = c / nw;
+            const int c = int(ring[(h + lane) % LDS_RING]), li = c % L, lj = c / L;
             const int nn = J.conn8 ? 8 : 4;
             for (int d = 0; d < nn; d++) {
                 const int di = d == 0 ? -1 : d == 1 ? 1 : d == 2 ? 0 : d == 3 ? 0 : d == 4 ? -1 : d == 5 ? 1 : d == 6 ? -1 : 1;
                 const int dj = d == 0 ? 0 : d == 1 ? 0 : d == 2 ? -1 : d == 3 ? 1 : d == 4 ? -1 : d == 5 ? -1 : d == 6 ? 1 : 1;
-                const int ni = i + di, nj = j + dj;
+                const int ni = li + di + o, nj = lj + dj + o;   // window coordinates
                 if (ni < 0 || nj < 0 || ni >= nw || nj >= nw) {
                     edge = true;
                     continue;
                 }
-                const int q = nj * nw + ni;
+                const int qi = ni - o, qj = nj - o;
+                if (qi < 0 || qj < 0 || qi >= L || qj >= L) {   // beyond the local sub-window
+                    ovf = 1;
+                    continue;
+                }
+                const int q = qj * L + qi;
                 const uint32_t bit = 1u << (q & 31);
                 if (atomicOr(&touched[q >> 5], bit) & bit) continue;
                 evl[atomicAdd(&nev, 1u)] = uint32_t(q);
@@ -125,16 +152,17 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         __syncthreads();
         if (lane == 0) qhead = h + np;
         __syncthreads();
+        if (ovf) break;
     }
     // ---- reduce
-    for (int o = 32; o > 0; o >>= 1) {
-        cells += __shfl_xor(cells, o);
-        sampled += __shfl_xor(sampled, o);
-        hotc += __shfl_xor(hotc, o);
-        minx = min(minx, __shfl_xor(minx, o));
-        maxx = max(maxx, __shfl_xor(maxx, o));
-        minz = min(minz, __shfl_xor(minz, o));
-        maxz = max(maxz, __shfl_xor(maxz, o));
+    for (int o2 = 32; o2 > 0; o2 >>= 1) {
+        cells += __shfl_xor(cells, o2);
+        sampled += __shfl_xor(sampled, o2);
+        hotc += __shfl_xor(hotc, o2);
+        minx = min(minx, __shfl_xor(minx, o2));
+        maxx = max(maxx, __shfl_xor(maxx, o2));
+        minz = min(minz, __shfl_xor(minz, o2));
+        maxz = max(maxz, __shfl_xor(maxz, o2));
     }
     const bool any_edge = __any(edge), any_border = __any(border);
     if (ovf) {
@@ -156,20 +184,17 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         res[job] = r;
     }
     if (cells == 0) return;
-    // ---- counted cells as 32x32 tile masks over the counted bounding box
+    // ---- counted cells as 32x32 tile masks (window tile coordinates) over the counted bounding box
     const int i0 = (minx - J.qx) / J.step + J.half, i1 = (maxx - J.qx) / J.step + J.half;
     const int j0 = (minz - J.qz) / J.step + J.half, j1 = (maxz - J.qz) / J.step + J.half;
     for (int tz = j0 >> 5; tz <= (j1 >> 5); tz++)
         for (int tx = i0 >> 5; tx <= (i1 >> 5); tx++) {
             uint32_t row = 0;
             if (lane < 32) {
-                const int j = tz * 32 + lane;
-                if (j < nw) {
-                    const int base = j * nw + tx * 32;                  // first cell of this tile row
-                    const int w0 = base >> 5, sh = base & 31;
-                    const uint64_t two = uint64_t(counted[w0]) | (uint64_t(w0 + 1 < LDS_WORDS ? counted[w0 + 1] : 0u) << 32);
-                    row = uint32_t(two >> sh);
-                    const int k = nw - tx * 32;                          // valid columns of this row
+                const int j = tz * 32 + lane;                 // window row
+                if (j < nw && j - o >= 0 && j - o < L) {
+                    row = bitmap_row32(counted, L, j - o, tx * 32 - o);
+                    const int k = nw - tx * 32;               // valid columns of this row
                     if (k < 32) row &= (1u << k) - 1u;
                 }
             }
@@ -187,11 +212,13 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         }
 }
 
-void launch_flood_lds(hipStream_t st, bool hot, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
+void launch_flood_lds(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
                       const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t *mcount, uint32_t mcap) {
     if (!n) return;
-    if (hot)
+    if (cls == LDS_HOT)
         hipLaunchKernelGGL(k_flood_lds<ClimT>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcount, mcap);
+    else if (cls == LDS_PROXY)
+        hipLaunchKernelGGL(k_flood_lds<ClimP>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcount, mcap);
     else
         hipLaunchKernelGGL(k_flood_lds<ClimSeed>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcount, mcap);
 }

@@ -25,9 +25,10 @@ struct FloodRes {
 struct Tile {
     uint32_t job;
     int32_t tx, tz;
-    uint32_t queued;        // 1 while listed for the next propagation round
+    uint32_t qstate;        // 0 idle, 1 queued or being processed, 2 being processed and touched again
     uint32_t slot1;         // hash slot + 1 of this tile's entry (0: not in the table)
-    uint32_t pad_[3];
+    uint32_t evald;         // 1 once its 1024 cells are evaluated
+    uint32_t pad_[2];
     uint32_t pass[FT];      // predicate != 0 (inside window and world)
     uint32_t cnt[FT];       // predicate == 2
     uint32_t reach[FT];     // cells the flood has entered (pushed)
@@ -41,39 +42,45 @@ struct TileMask {
     uint32_t rows[FT];
 };
 
+// Device state of the tiled engine: the tile pool, the (job, tx, tz) -> tile hash, and the work
+// queue of tiles to process (indices; Q_EMPTY where unused, which every call leaves behind).
 struct FloodBufs {
     Tile *tiles;
     uint32_t *ntiles;             // allocated tiles
     uint32_t tile_cap;
     unsigned long long *hkeys;    // hash: (job, tx, tz) -> tile index, packed with the key
     uint32_t hmask;               // table size - 1
-    uint32_t *lists[6];           // eval / prop lists of rounds r with r % 3 = 0, 1, 2
-    uint32_t *lcount;             // [0..5] their lengths; [FLOOD_OVF] overflow flags; [FLOOD_MASKS] exported masks
-    uint32_t list_cap;
+    uint32_t *wq;                 // work queue slots
+    uint32_t wq_cap;
+    uint32_t *ctl;                // [Q_HEAD] [Q_TAIL] [Q_PENDING] [FLOOD_OVF] [FLOOD_MASKS] [Q_ITEMS]
 };
 
-constexpr int FLOOD_OVF = 6, FLOOD_MASKS = 7;
+constexpr int Q_HEAD = 0, Q_TAIL = 1, Q_PENDING = 2, FLOOD_OVF = 3, FLOOD_MASKS = 4, Q_ITEMS = 5, CTL_WORDS = 8;
+constexpr uint32_t Q_EMPTY = 0xffffffffu;
+// FLOOD_OVF bits: 1 work queue full, 2 tile pool or hash table full, 4 time limit reached
+constexpr uint32_t OVF_QUEUE = 1, OVF_TILES = 2, OVF_TIME = 4;
 
 // jl[0..n): indices of the jobs to start (their hash keys and result slots use the original index);
-// the start tiles go to the lists of round 0
-void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B, uint32_t *eval,
-                        uint32_t *prop);
-// Round `round`: evaluate the new tiles of its eval list, then propagate the tiles of its prop list
-// into the next round's lists. `width`: the tiles the host expects per list (a grid hint: any grid
-// covers every listed tile).
-constexpr uint32_t EVAL_GRID = 512, PROP_GRID = 2048;
-// ev0/ev1 (optional): HIP events recorded when the evaluation kernel starts and ends.
-void launch_flood_round(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B, int round,
-                        uint32_t width, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// their start tiles are queued
+void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B);
+// The persistent flood kernel: `blocks` workgroups take queued tiles until no work is pending. A tile
+// is evaluated (all 1024 cells) the first time it is processed; every processing grows its reached
+// set to a fixpoint and queues the neighbours it touches. ev0/ev1 (optional): HIP events around it.
+void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
+                      uint32_t blocks, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // Reduce, export the counted masks, and return every touched tile and hash slot to the zero / empty
 // state the next call expects (no pool-wide memsets).
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res,
                       struct TileMask *masks, uint32_t *mcount, uint32_t mcap);
 // res[k] = empty area at the centre, for k in jl[0..n) (jl null: k in [0, n))
 void launch_flood_resinit(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodRes *res);
-// One wave per small-window flood (2*half+1 <= 255), state in LDS; hot: predicate is the T test.
+// One wave per flood with its state in LDS over a local sub-window of at most LDS_MAX_N^2 points
+// around the start; floods that outgrow it come back with res.border = -1 (rerun them on the tiled
+// engine). cls: LDS_HOT (the T test, any window), LDS_PROXY (proxy predicate, any window), LDS_FULL
+// (any kind; the host sends only windows that fit whole).
 constexpr int LDS_MAX_N = 255;
-void launch_flood_lds(hipStream_t st, bool hot, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
+enum { LDS_HOT = 0, LDS_PROXY = 1, LDS_FULL = 2 };
+void launch_flood_lds(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
                       const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t *mcount, uint32_t mcap);
 
 }  // namespace dsx

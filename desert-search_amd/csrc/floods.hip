@@ -1,20 +1,34 @@
-// Tiled frontier flood engine for gfx950 (bounded bridged floods, dense_flood.hpp:52-86).
+// Tiled flood engine for gfx950 (bounded bridged floods, dense_flood.hpp:52-86), one persistent
+// kernel per batch of floods.
 //
 // The reference floods depth-first through a dense (2*half+1)^2 byte window. Every output of that
 // flood (cells, sampled, bbox, edge, border, and the final window state) depends only on the set of
 // cells reachable from the start through predicate != 0 cells and on their neighbourhood, not on the
 // visiting order. This engine computes the same sets tile-parallel:
 //   * the window is split into 32x32 tiles, allocated on first touch through a device hash table;
-//   * k_flood_eval evaluates the FP64 predicate for all 1024 cells of each new tile (one 1024-thread
-//     block per tile, one cell per thread, the octaves the predicate reads in LDS);
-//   * k_flood_prop (one wave per tile, one lane per 32-bit row mask) grows the reached set inside the
-//     tile to a fixpoint with bit-parallel dilation, then ORs the touched cells it adds across the
-//     tile edges into the neighbours' `in` masks and queues those tiles;
-//   * the host queues rounds of eval + prop until no tile is queued; k_flood_acc then reduces each tile:
+//   * k_flood_run is a persistent kernel over a device work queue of tiles. A workgroup takes a tile,
+//     evaluates the FP64 predicate of its 1024 cells the first time (the octaves the predicate reads
+//     staged in LDS), grows the reached set inside the tile to a fixpoint with bit-parallel dilation
+//     (one wave, one lane per 32-bit row mask), ORs the touched cells across the tile edges into the
+//     neighbours' `in` masks and queues the neighbours it touched. It runs until no work is pending:
+//     no per-round kernel launches and no host round trips;
+//   * k_flood_acc then reduces each tile:
 //       reached = pushed cells, touched = reached | in | dilate_inside(reached),
 //       cells = |reached & pred==2|, sampled = |touched inside the world|, border = touched outside
 //       the world, edge = reached cell on the window boundary, bbox over counted reached cells
 //       (starting at the window centre, as dense_flood.hpp:57).
+//
+// Ownership and visibility. A tile is processed by one workgroup at a time: its qstate word goes
+// 0 -> 1 when it is queued, and a workgroup that touches a tile in state 1 (queued or in process)
+// moves it to 2, which makes the processing workgroup run it again before it returns it to 0. Tile
+// state shared between workgroups is only read and written with agent-scope accesses (sc1) and
+// atomics, which are coherent across the XCDs' L2 caches; at every hand-over (queue slot, qstate,
+// hash publication) the wave first waits until its earlier accesses have completed.
+//
+// Termination. Q_PENDING counts queued items not yet finished (an item's own pushes are counted
+// before it finishes), so it reaches 0 exactly when no more work can appear. A workgroup waiting on
+// an empty queue slot leaves when Q_PENDING is 0, when an overflow was flagged, or after a time limit
+// (flagged as an error), so every wave exits.
 #include "floods.h"
 #include <hip/hip_ext.h>
 #include <climits>
@@ -30,6 +44,19 @@ __device__ __forceinline__ uint32_t col_mask(const FloodJob &J, int tx) {
     return k >= FT ? 0xffffffffu : (k <= 0 ? 0u : ((1u << k) - 1u));
 }
 
+// agent-scope (all XCDs) relaxed loads and stores of tile state shared inside a kernel
+template <class T> __device__ __forceinline__ T ald(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ void ast(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Hand-over points: every earlier memory operation of this wave has completed (its agent-scope
+// stores and atomics are performed at the coherence point) before the next one issues. Shared state
+// is only ever touched through agent-scope accesses, so no cache writeback or invalidation is needed
+// (an agent-scope release fence would write back the whole L2 at every hand-over).
+__device__ __forceinline__ void mem_done() { __builtin_amdgcn_s_waitcnt(0); }
+
 constexpr unsigned long long HEMPTY = ~0ull;
 __device__ __forceinline__ unsigned long long hkey(uint32_t job, int tx, int tz) {
     return (uint64_t(job) << 44) | (uint64_t(uint32_t(tx) & 2047u) << 33) | (uint64_t(uint32_t(tz) & 2047u) << 22);
@@ -42,21 +69,21 @@ __device__ uint32_t tile_get(const FloodBufs &B, uint32_t job, int tx, int tz, b
     uint32_t mine = 0xffffffffu;
     *fresh = false;
     for (uint32_t probe = 0; probe <= B.hmask; probe++, h = (h + 1) & B.hmask) {
-        unsigned long long cur = __atomic_load_n(&B.hkeys[h], __ATOMIC_RELAXED);
+        unsigned long long cur = ald(&B.hkeys[h]);
         if (cur == HEMPTY) {
             if (mine == 0xffffffffu) {
                 mine = atomicAdd(B.ntiles, 1u);
-                if (mine >= B.tile_cap) return 0xffffffffu;        // pool exhausted (host reports it)
+                if (mine >= B.tile_cap) return 0xffffffffu;        // pool exhausted (reported as OVF_TILES)
                 Tile &t = B.tiles[mine];
-                t.job = job;
-                t.tx = tx;
-                t.tz = tz;
-                __threadfence();
+                ast(&t.job, job);
+                ast(&t.tx, tx);
+                ast(&t.tz, tz);
+                mem_done();
             }
             const unsigned long long want = key | mine;
             cur = atomicCAS(&B.hkeys[h], HEMPTY, want);
             if (cur == HEMPTY) {
-                B.tiles[mine].slot1 = h + 1;
+                ast(&B.tiles[mine].slot1, h + 1);
                 *fresh = true;
                 return mine;
             }
@@ -65,16 +92,33 @@ __device__ uint32_t tile_get(const FloodBufs &B, uint32_t job, int tx, int tz, b
     }
     return 0xffffffffu;
 }
-// Octave slots read by the hot test (first 4: T of both sides) and the proxy (all 14: plus C octaves
-// 0-2 and E octaves 0-1 of both sides).
-__device__ __forceinline__ int proxy_slot(int k) {
-    constexpr int slots[14] = {0, 1, 2, 3, 8, 9, 10, 17, 18, 19, 26, 27, 30, 31};
-    return slots[k];
+
+// Append tile ti to the work queue (counted as pending first).
+__device__ __forceinline__ void wq_push(const FloodBufs &B, uint32_t ti) {
+    atomicAdd(&B.ctl[Q_PENDING], 1u);
+    const uint32_t k = atomicAdd(&B.ctl[Q_TAIL], 1u);
+    if (k >= B.wq_cap) {
+        atomicOr(&B.ctl[FLOOD_OVF], OVF_QUEUE);
+        return;
+    }
+    mem_done();
+    __hip_atomic_store(&B.wq[k], ti, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
-__device__ __forceinline__ void list_push(uint32_t *list, uint32_t *count, uint32_t cap, uint32_t *ovf, uint32_t v) {
-    const uint32_t k = atomicAdd(count, 1u);
-    if (k < cap) list[k] = v;
-    else atomicOr(ovf, 1u);
+
+// After new `in` bits of tile ti are published: make sure the tile is (re)processed.
+__device__ __forceinline__ void tile_touch(const FloodBufs &B, uint32_t ti) {
+    uint32_t *q = &B.tiles[ti].qstate;
+    mem_done();
+    for (;;) {
+        uint32_t old = atomicCAS(q, 0u, 1u);
+        if (old == 0u) {
+            wq_push(B, ti);
+            return;
+        }
+        if (old == 2u) return;
+        old = atomicCAS(q, 1u, 2u);   // queued or in process: have it run again
+        if (old != 0u) return;        // now 2; if it went idle meanwhile (0), queue it ourselves
+    }
 }
 
 __global__ void k_flood_resinit(const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodRes *res) {
@@ -87,91 +131,34 @@ __global__ void k_flood_resinit(const FloodJob *jobs, const uint32_t *jl, uint32
     res[k] = r;
 }
 
-// Appends v to list for every lane with on set: one atomic per wave.
-__device__ __forceinline__ void wave_push(uint32_t *list, uint32_t *count, uint32_t cap, uint32_t *ovf, bool on, uint32_t v) {
-    const uint64_t mask = __ballot(on);
-    if (mask == 0) return;
-    const int lane = threadIdx.x & 63;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(count, uint32_t(__popcll(mask)));
-    base = __shfl(base, 0);
-    if (on) {
-        const uint32_t k = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-        if (k < cap) list[k] = v;
-        else atomicOr(ovf, 1u);
-    }
-}
-
-// The start cell of every flood: touched ("in") and queued.
-__global__ __launch_bounds__(256) void k_flood_start(const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B,
-                                                     uint32_t *eval, uint32_t *prop) {
+// The start cell of every flood: touched ("in") and its tile queued.
+__global__ __launch_bounds__(256) void k_flood_start(const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    bool fresh = false, queue = false;
-    uint32_t ti = 0xffffffffu;
-    if (q < n) {
-        const uint32_t k = jl[q];
-        const FloodJob J = jobs[k];
-        const int c = J.half % FT, t = J.half / FT;   // the centre cell (half, half)
-        ti = tile_get(B, k, t, t, &fresh);
-        if (ti == 0xffffffffu) {
-            atomicOr(&B.lcount[FLOOD_OVF], 2u);
-        } else {
-            atomicOr(&B.tiles[ti].in[c], 1u << c);
-            queue = atomicCAS(&B.tiles[ti].queued, 0u, 1u) == 0u;
-        }
+    if (q >= n) return;
+    const uint32_t k = jl[q];
+    const FloodJob J = jobs[k];
+    const int c = J.half % FT, t = J.half / FT;   // the centre cell (half, half)
+    bool fresh = false;
+    const uint32_t ti = tile_get(B, k, t, t, &fresh);
+    if (ti == 0xffffffffu) {
+        atomicOr(&B.ctl[FLOOD_OVF], OVF_TILES);
+        return;
     }
-    wave_push(eval, &B.lcount[0], B.list_cap, &B.lcount[FLOOD_OVF], fresh && ti != 0xffffffffu, ti);
-    wave_push(prop, &B.lcount[1], B.list_cap, &B.lcount[FLOOD_OVF], queue, ti);
+    atomicOr(&B.tiles[ti].in[c], 1u << c);
+    tile_touch(B, ti);
 }
 
-// Rounds: round r evaluates the new tiles of eval list r % 3 and propagates the tiles of prop list
-// r % 3, listing the next round's tiles in lists (r + 1) % 3; its eval kernel first empties lists
-// (r + 2) % 3 (read by the previous round, written by the next). The grids are fixed (grid-stride over
-// the device-side counts), so the host queues several rounds per synchronisation.
-//
-// Predicate of the 1024 cells of each listed tile: pass = v != 0, cnt = v == 2 (cells outside the window
-// or the world are neither).
-__global__ __launch_bounds__(FT * FT) void k_flood_eval(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B,
-                                                        int round) {
-    __shared__ ClimSeed C;
+// Evaluate the 1024 cells of tile T with NT threads (pass = v != 0, cnt = v == 2, hot = the
+// predicate's temperature test passed; cells outside the window or the world are none of them).
+template <int NT>
+__device__ __forceinline__ void eval_tile(Tile &T, const FloodJob &J, const ClimSeed &C, const SurfGrid &g) {
     const int tid = threadIdx.x;
-    const int cur = round % 3, clr = (round + 2) % 3;
-    const uint32_t n = min(__atomic_load_n(&B.lcount[2 * cur], __ATOMIC_RELAXED), B.list_cap);
-    const uint32_t *list = B.lists[2 * cur];
-    if (blockIdx.x == 0 && tid < 2) B.lcount[2 * clr + tid] = 0u;
-    int loaded_seed = -1, loaded_cls = -1;
-    for (uint32_t b = blockIdx.x; b < n; b += gridDim.x) {
-        Tile &T = B.tiles[list[b]];
-        const FloodJob J = jobs[T.job];
-        // octaves the predicate reads: T for the hot test (class 0), T + C octaves 0-2 + E octaves 0-1
-        // for the proxy (class 1), everything for the biome predicates (class 2)
-        const int cls = J.kind == PRED_HOT ? 0 : (J.kind == PRED_PROXY ? 1 : 2);
-        if (J.seed != loaded_seed || cls > loaded_cls) {
-            __syncthreads();
-            const ClimSeed &G = cs[J.seed];
-            constexpr int OW = sizeof(ClimOct) / 16;
-            if (cls < 2) {
-                const int nslot = cls == 0 ? 4 : 14;
-                for (int k = tid; k < nslot * OW; k += FT * FT) {
-                    const int slot = proxy_slot(k / OW);
-                    reinterpret_cast<uint4 *>(&C.oct[slot])[k % OW] = reinterpret_cast<const uint4 *>(&G.oct[slot])[k % OW];
-                }
-                constexpr int GW = sizeof(GradTab) / 16;
-                if (tid < GW) reinterpret_cast<uint4 *>(&C.g)[tid] = reinterpret_cast<const uint4 *>(&G.g)[tid];
-                if (tid < 6) C.factor[tid] = G.factor[tid];
-            } else {
-                const uint4 *src = reinterpret_cast<const uint4 *>(&G);
-                uint4 *dst = reinterpret_cast<uint4 *>(&C);
-                for (int k = tid; k < int(sizeof(ClimSeed) / 16); k += FT * FT) dst[k] = src[k];
-            }
-            __syncthreads();
-            loaded_seed = J.seed;
-            loaded_cls = cls;
-        }
-        // one cell per thread; wave w holds rows 2w (lanes 0-31) and 2w + 1 (lanes 32-63)
-        const int nw = int(nwin(J));
-        const int r = tid / FT, col = tid % FT;
-        const int i = T.tx * FT + col, j = T.tz * FT + r;
+    const int nw = int(nwin(J));
+    const int tx = ald(&T.tx), tz = ald(&T.tz);
+#pragma unroll 1
+    for (int base = 0; base < FT * FT; base += NT) {
+        const int cell = base + tid, r = cell / FT, col = cell % FT;   // a wave covers rows r, r + 1
+        const int i = tx * FT + col, j = tz * FT + r;
         int v = 0;
         bool hot = false;
         if (i < nw && j < nw) {
@@ -180,29 +167,27 @@ __global__ __launch_bounds__(FT * FT) void k_flood_eval(const FloodJob *jobs, co
         }
         const uint64_t mp = __ballot(v != 0), mc = __ballot(v == 2), mh = __ballot(hot);
         if ((tid & 63) == 0) {
-            T.pass[r] = uint32_t(mp);
-            T.pass[r + 1] = uint32_t(mp >> 32);
-            T.cnt[r] = uint32_t(mc);
-            T.cnt[r + 1] = uint32_t(mc >> 32);
-            T.hot[r] = uint32_t(mh);
-            T.hot[r + 1] = uint32_t(mh >> 32);
+            ast(&T.pass[r], uint32_t(mp));
+            ast(&T.pass[r + 1], uint32_t(mp >> 32));
+            ast(&T.cnt[r], uint32_t(mc));
+            ast(&T.cnt[r + 1], uint32_t(mc >> 32));
+            ast(&T.hot[r], uint32_t(mh));
+            ast(&T.hot[r + 1], uint32_t(mh >> 32));
         }
     }
+    mem_done();   // each wave's stores are performed before the tile is handed on
 }
 
 // Grow the reached set of tile ti to a fixpoint and hand the new frontier to the neighbours (one wave).
-__device__ __forceinline__ void prop_tile(const FloodJob *jobs, FloodBufs &B, uint32_t ti, uint32_t *next_eval,
-                                          uint32_t *next_prop, uint32_t *cnt_e, uint32_t *cnt_p) {
+__device__ void prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J) {
     Tile &T = B.tiles[ti];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int r = lane & 31;
-    if (lane == 0) atomicExch(&T.queued, 0u);
-    __threadfence();
-    __builtin_amdgcn_wave_barrier();
-    const FloodJob J = jobs[T.job];
     const bool c8 = J.conn8 != 0;
-    const uint32_t in = __atomic_load_n(&T.in[r], __ATOMIC_RELAXED);
-    const uint32_t pass = T.pass[r], old = T.reach[r];
+    const uint32_t job = ald(&T.job);
+    const int tx = ald(&T.tx), tz = ald(&T.tz);
+    const uint32_t in = lane < FT ? ald(&T.in[r]) : 0u;
+    const uint32_t pass = lane < FT ? ald(&T.pass[r]) : 0u, old = lane < FT ? ald(&T.reach[r]) : 0u;
     uint32_t R = lane < FT ? (old | (in & pass)) : 0u;
     for (;;) {
         const uint32_t up = __shfl(R, (r + 31) & 31, 32), dn = __shfl(R, (r + 1) & 31, 32);
@@ -215,7 +200,7 @@ __device__ __forceinline__ void prop_tile(const FloodJob *jobs, FloodBufs &B, ui
         if (!__any(ch)) break;
     }
     const uint32_t NR = R & ~old;
-    if (lane < FT && NR) T.reach[r] = R;
+    if (lane < FT && NR) ast(&T.reach[r], R);
     if (!__any(NR != 0)) return;
     // ---- frontier across the tile edges
     const int nta = int(ntile_axis(J));
@@ -233,9 +218,6 @@ __device__ __forceinline__ void prop_tile(const FloodJob *jobs, FloodBufs &B, ui
         vert_up |= (up0 << 1) | (up0 >> 1);
         vert_dn |= (dn31 << 1) | (dn31 >> 1);
     }
-    // 8 directions: dx, dz, and this lane's contribution (row, mask); lane 0 collects the tiles to list
-    uint32_t push_e[8], push_p[8];
-    int ne = 0, np = 0;
     for (int d = 0; d < 8; d++) {
         const int dxs[8] = {-1, 1, 0, 0, -1, 1, -1, 1}, dzs[8] = {0, 0, -1, 1, -1, -1, 1, 1};
         const int dx = dxs[d], dz = dzs[d];
@@ -251,18 +233,17 @@ __device__ __forceinline__ void prop_tile(const FloodJob *jobs, FloodBufs &B, ui
         else if (d == 6) { row = 0; mask = (lane == 0 && (dn31 & 1u)) ? (1u << 31) : 0u; }
         else { row = 0; mask = (lane == 0 && (dn31 >> 31)) ? 1u : 0u; }
         if (lane >= FT) mask = 0;
-        const int ntx = T.tx + dx, ntz = T.tz + dz;
+        const int ntx = tx + dx, ntz = tz + dz;
         if (ntx < 0 || ntz < 0 || ntx >= nta || ntz >= nta) continue;     // outside the window: edge (acc)
         mask &= col_mask(J, ntx);
         if (ntz * FT + row >= int(nwin(J))) mask = 0;
         if (!__any(mask != 0)) continue;
         uint32_t nti = 0;
         bool fresh = false;
-        if (lane == 0) nti = tile_get(B, T.job, ntx, ntz, &fresh);
+        if (lane == 0) nti = tile_get(B, job, ntx, ntz, &fresh);
         nti = __shfl(nti, 0);
-        fresh = __shfl(int(fresh), 0) != 0;
         if (nti == 0xffffffffu) {
-            if (lane == 0) atomicOr(&B.lcount[FLOOD_OVF], 2u);
+            if (lane == 0) atomicOr(&B.ctl[FLOOD_OVF], OVF_TILES);
             continue;
         }
         Tile &N = B.tiles[nti];
@@ -271,40 +252,109 @@ __device__ __forceinline__ void prop_tile(const FloodJob *jobs, FloodBufs &B, ui
             const uint32_t o = atomicOr(&N.in[row], mask);
             grew = (o | mask) != o;
         }
-        if (lane == 0 && fresh) push_e[ne++] = nti;
-        if (__any(grew) && lane == 0) {
-            __threadfence();
-            if (atomicCAS(&N.queued, 0u, 1u) == 0u) push_p[np++] = nti;
-        }
+        if (__any(grew) && lane == 0) tile_touch(B, nti);
     }
-    if (lane == 0) {
-        if (ne) {
-            const uint32_t k = atomicAdd(cnt_e, uint32_t(ne));
-            for (int q = 0; q < ne; q++) {
-                if (k + q < B.list_cap) next_eval[k + q] = push_e[q];
-                else atomicOr(&B.lcount[FLOOD_OVF], 1u);
+}
+
+// Persistent workgroups over the work queue (NT threads each; wave 0 propagates).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_flood_run(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B) {
+    __shared__ ClimSeed C;
+    __shared__ uint32_t s_item, s_again;
+    __shared__ int s_stop;
+    const int tid = threadIdx.x;
+    int loaded_seed = -1, loaded_cls = -1;
+    for (;;) {
+        // ---- take the next queue slot and wait for its tile
+        if (tid == 0) {
+            uint32_t item = Q_EMPTY;
+            int stop = 0;
+            const uint32_t k = atomicAdd(&B.ctl[Q_HEAD], 1u);
+            if (k >= B.wq_cap) {
+                stop = 1;
+            } else {
+                const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+                for (uint32_t it = 0;; it++) {
+                    item = __hip_atomic_load(&B.wq[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    if (item != Q_EMPTY) break;
+                    if ((it & 7u) == 7u) {   // the exit conditions, every 8th poll
+                        if (ald(&B.ctl[Q_PENDING]) == 0u || ald(&B.ctl[FLOOD_OVF]) != 0u) {
+                            stop = 1;
+                            break;
+                        }
+                        if (__builtin_amdgcn_s_memrealtime() - t0 > 2000000000ull) {   // 20 s at 100 MHz
+                            atomicOr(&B.ctl[FLOOD_OVF], OVF_TIME);
+                            stop = 1;
+                            break;
+                        }
+                    }
+                    __builtin_amdgcn_s_sleep(4);
+                }
+                if (item != Q_EMPTY) ast(&B.wq[k], Q_EMPTY);   // leave the slot clean for the next call
             }
+            s_item = item;
+            s_stop = stop;
         }
-        if (np) {
-            const uint32_t k = atomicAdd(cnt_p, uint32_t(np));
-            for (int q = 0; q < np; q++) {
-                if (k + q < B.list_cap) next_prop[k + q] = push_p[q];
-                else atomicOr(&B.lcount[FLOOD_OVF], 1u);
+        __syncthreads();
+        if (s_stop) break;
+        const uint32_t ti = s_item;
+        Tile &T = B.tiles[ti];
+        if (tid == 0) {
+            atomicExch(&T.qstate, 1u);   // in process: touches from now on ask for another pass
+            atomicAdd(&B.ctl[Q_ITEMS], 1u);
+        }
+        __syncthreads();
+        const FloodJob J = jobs[ald(&T.job)];
+        // ---- first processing: evaluate the tile (octaves the predicate reads in LDS)
+        if (ald(&T.evald) == 0u) {
+            const int cls = J.kind == PRED_HOT ? 0 : (J.kind == PRED_PROXY ? 1 : 2);
+            if (J.seed != loaded_seed || cls > loaded_cls) {
+                const ClimSeed &G = cs[J.seed];
+                constexpr int OW = sizeof(ClimOct) / 16;
+                if (cls < 2) {
+                    const int nslot = cls == 0 ? 4 : 14;
+                    for (int k = tid; k < nslot * OW; k += NT) {
+                        const int slot = PROXY_SLOTS[k / OW];
+                        reinterpret_cast<uint4 *>(&C.oct[slot])[k % OW] = reinterpret_cast<const uint4 *>(&G.oct[slot])[k % OW];
+                    }
+                    constexpr int GW = sizeof(GradTab) / 16;
+                    if (tid < GW) reinterpret_cast<uint4 *>(&C.g)[tid] = reinterpret_cast<const uint4 *>(&G.g)[tid];
+                    if (tid < 6) C.factor[tid] = G.factor[tid];
+                } else {
+                    const uint4 *src = reinterpret_cast<const uint4 *>(&G);
+                    uint4 *dst = reinterpret_cast<uint4 *>(&C);
+                    for (int k = tid; k < int(sizeof(ClimSeed) / 16); k += NT) dst[k] = src[k];
+                }
+                __syncthreads();
+                loaded_seed = J.seed;
+                loaded_cls = cls;
             }
+            eval_tile<NT>(T, J, C, g);
+            __syncthreads();
+            if (tid == 0) ast(&T.evald, 1u);
+        }
+        // ---- propagate, as many passes as touches arrive meanwhile
+        for (;;) {
+            if (tid < 64) prop_tile(B, ti, J);
+            __syncthreads();
+            if (tid == 0) {
+                mem_done();
+                uint32_t again = 0;
+                if (atomicCAS(&T.qstate, 1u, 0u) == 2u) {   // touched while in process
+                    atomicExch(&T.qstate, 1u);
+                    again = 1;
+                } else {
+                    atomicSub(&B.ctl[Q_PENDING], 1u);        // this item is done (its pushes are counted)
+                }
+                s_again = again;
+            }
+            __syncthreads();
+            if (!s_again) break;
         }
     }
 }
 
-__global__ __launch_bounds__(64) void k_flood_prop(const FloodJob *jobs, FloodBufs B, int round) {
-    const int cur = round % 3, nxt = (round + 1) % 3;
-    const uint32_t n = min(__atomic_load_n(&B.lcount[2 * cur + 1], __ATOMIC_RELAXED), B.list_cap);
-    const uint32_t *list = B.lists[2 * cur + 1];
-    uint32_t *next_eval = B.lists[2 * nxt], *next_prop = B.lists[2 * nxt + 1];
-    uint32_t *cnt_e = &B.lcount[2 * nxt], *cnt_p = &B.lcount[2 * nxt + 1];
-    for (uint32_t b = blockIdx.x; b < n; b += gridDim.x) prop_tile(jobs, B, list[b], next_eval, next_prop, cnt_e, cnt_p);
-}
-
-// Per-tile reduction of the flood outputs into res[job].
+// Per-tile reduction of the flood outputs into res[job]; leaves the tile and its hash slot clean.
 __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res,
                                                   TileMask *masks, uint32_t *mcount, uint32_t mcap) {
     const uint32_t b = blockIdx.x;
@@ -399,26 +449,26 @@ __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBuf
     if (lane == 0) {
         if (T.slot1) B.hkeys[T.slot1 - 1] = HEMPTY;
         T.slot1 = 0;
-        T.queued = 0;
+        T.qstate = 0;
+        T.evald = 0;
     }
 }
 
 // ------------------------------------------------------------------ launchers
 
+constexpr int RUN_NT = 256;
+
 void launch_flood_resinit(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodRes *res) {
     if (n) hipLaunchKernelGGL(k_flood_resinit, dim3((n + 255) / 256), dim3(256), 0, st, jobs, jl, n, res);
 }
-void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B, uint32_t *eval,
-                        uint32_t *prop) {
-    if (n) hipLaunchKernelGGL(k_flood_start, dim3((n + 255) / 256), dim3(256), 0, st, jobs, jl, n, B, eval, prop);
+void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B) {
+    if (n) hipLaunchKernelGGL(k_flood_start, dim3((n + 255) / 256), dim3(256), 0, st, jobs, jl, n, B);
 }
-void launch_flood_round(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B, int round,
-                        uint32_t width, hipEvent_t ev0, hipEvent_t ev1) {
-    const uint32_t w = width < 8 ? 8 : width;
-    const dim3 eg(w < EVAL_GRID ? w : EVAL_GRID);
-    if (ev0) hipExtLaunchKernelGGL(k_flood_eval, eg, dim3(FT * FT), 0, st, ev0, ev1, 0u, jobs, cs, g, B, round);
-    else hipLaunchKernelGGL(k_flood_eval, eg, dim3(FT * FT), 0, st, jobs, cs, g, B, round);
-    hipLaunchKernelGGL(k_flood_prop, dim3(w < PROP_GRID ? w : PROP_GRID), dim3(64), 0, st, jobs, B, round);
+void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
+                      uint32_t blocks, hipEvent_t ev0, hipEvent_t ev1) {
+    const dim3 grid(blocks ? blocks : 1);
+    if (ev0) hipExtLaunchKernelGGL(k_flood_run<RUN_NT>, grid, dim3(RUN_NT), 0, st, ev0, ev1, 0u, jobs, cs, g, B);
+    else hipLaunchKernelGGL(k_flood_run<RUN_NT>, grid, dim3(RUN_NT), 0, st, jobs, cs, g, B);
 }
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res, TileMask *masks,
                       uint32_t *mcount, uint32_t mcap) {

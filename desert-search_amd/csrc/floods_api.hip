@@ -125,15 +125,16 @@ static int clim_alloc(dsx_ctx *c, ClimState *&S) {
     // flood engine
     dsx::FloodBufs &B = S->fb;
     B.tile_cap = 1u << 21;                    // 2M tiles x 528 B
-    B.list_cap = B.tile_cap;
     B.hmask = (1u << 23) - 1;                 // 8M hash slots
     HIPCK(hipMalloc(&B.tiles, size_t(B.tile_cap) * sizeof(dsx::Tile)));
     HIPCK(hipMemset(B.tiles, 0, size_t(B.tile_cap) * sizeof(dsx::Tile)));
     HIPCK(hipMalloc(&B.ntiles, 4));
     HIPCK(hipMalloc(&B.hkeys, (size_t(B.hmask) + 1) * 8));
     HIPCK(hipMemset(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8));
-    for (auto &l : B.lists) HIPCK(hipMalloc(&l, size_t(B.list_cap) * 4));
-    HIPCK(hipMalloc(&B.lcount, 8 * 4));
+    B.wq_cap = 4 * B.tile_cap;               // queue slots (a tile is queued again when touched again)
+    HIPCK(hipMalloc(&B.wq, size_t(B.wq_cap) * 4));
+    HIPCK(hipMemset(B.wq, 0xff, size_t(B.wq_cap) * 4));
+    HIPCK(hipMalloc(&B.ctl, dsx::CTL_WORDS * 4));
     S->job_cap = 1u << 20;
     HIPCK(hipMalloc(&S->d_jobs, size_t(S->job_cap) * sizeof(dsx::FloodJob)));
     HIPCK(hipMalloc(&S->d_res, size_t(S->job_cap) * sizeof(dsx::FloodRes)));
@@ -194,6 +195,15 @@ static int clim_slots(dsx_ctx *c, ClimState *S, const uint64_t *seeds, uint32_t 
     return DSX_OK;
 }
 
+static uint64_t env_u32(const char *name, uint64_t def) {
+    const char *v = std::getenv(name);
+    return v && *v ? std::strtoull(v, nullptr, 10) : def;
+}
+static bool env_flag(const char *name, bool def) {
+    const char *v = std::getenv(name);
+    return v && *v ? std::atoi(v) != 0 : def;
+}
+
 // Process-wide flood-engine statistics (dsx_flood_stats_get) and the profiling switch (dsx_profile).
 static std::atomic<int> g_prof_on{0};
 static std::mutex g_stats_mu;
@@ -212,6 +222,12 @@ static void stats_add(const dsx_flood_stats &a) {
     g_stats.small_sampled += a.small_sampled;
     g_stats.eval_ms += a.eval_ms;
     g_stats.eval_launches += a.eval_launches;
+    for (int k = 0; k < 4; k++)
+        for (int b = 0; b < 6; b++) {
+            g_stats.size_floods[k][b] += a.size_floods[k][b];
+            g_stats.size_tiles[k][b] += a.size_tiles[k][b];
+            g_stats.size_sampled[k][b] += a.size_sampled[k][b];
+        }
 }
 
 // All per-context extension state lives here; freed by dsx_destroy via ctx_free_ext.
@@ -234,9 +250,7 @@ void dsx_ctx_free_ext(dsx_ctx *c) {
         g_ext.erase(it);
     }
     for (void *p : {(void *)s->d_cuts, (void *)s->d_cell, (void *)s->d_clim, (void *)s->d_useeds, (void *)s->fb.tiles,
-                    (void *)s->fb.ntiles, (void *)s->fb.hkeys, (void *)s->fb.lists[0], (void *)s->fb.lists[1],
-                    (void *)s->fb.lists[2], (void *)s->fb.lists[3], (void *)s->fb.lists[4], (void *)s->fb.lists[5],
-                    (void *)s->fb.lcount, (void *)s->d_jobs,
+                    (void *)s->fb.ntiles, (void *)s->fb.hkeys, (void *)s->fb.wq, (void *)s->fb.ctl, (void *)s->d_jobs,
                     (void *)s->d_res, s->d_io, (void *)s->d_masks, (void *)s->d_jl})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
@@ -344,85 +358,92 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     dsx::FloodBufs &B = S->fb;
     HIPCK(hipMemcpyAsync(S->d_jobs, hj.data(), n * sizeof(dsx::FloodJob), hipMemcpyHostToDevice, c->st));
     HIPCK(hipMemsetAsync(B.ntiles, 0, 4, c->st));
-    HIPCK(hipMemsetAsync(B.lcount, 0, 32, c->st));
+    HIPCK(hipMemsetAsync(B.ctl, 0, dsx::CTL_WORDS * 4, c->st));
     dsx::launch_flood_resinit(c->st, S->d_jobs, nullptr, uint32_t(n), S->d_res);
-    // small windows: one wave each with the window in LDS; the rest: the tiled engine
-    std::vector<uint32_t> jl_hot, jl_oth, jl_big;
+    // The one-wave LDS engine for every flood whose window fits it whole and, with DSX_LDS_ROUTE=1,
+    // first for hot-zone and proxy floods of any window (on a local sub-window; exact work, but one
+    // BFS layer per step, which measured slower in the pipeline than the tiled engine's 32-cell
+    // bit-parallel rounds); the rest, and the floods that outgrow the sub-window, on the tiled engine.
+    static const bool lds_route = env_flag("DSX_LDS_ROUTE", false);
+    std::vector<uint32_t> jl_small, jl_big, by_cls[3];
     for (uint64_t k = 0; k < n; k++) {
-        if (2 * hj[k].half + 1 <= dsx::LDS_MAX_N) (hj[k].kind == dsx::PRED_HOT ? jl_hot : jl_oth).push_back(uint32_t(k));
+        const bool fits = 2 * hj[k].half + 1 <= dsx::LDS_MAX_N;
+        if (hj[k].kind == dsx::PRED_HOT && (fits || lds_route)) by_cls[dsx::LDS_HOT].push_back(uint32_t(k));
+        else if (hj[k].kind == dsx::PRED_PROXY && (fits || lds_route)) by_cls[dsx::LDS_PROXY].push_back(uint32_t(k));
+        else if (fits) by_cls[dsx::LDS_FULL].push_back(uint32_t(k));
         else jl_big.push_back(uint32_t(k));
     }
-    uint32_t *d_hot = S->d_jl, *d_oth = d_hot + jl_hot.size(), *d_big = d_hot + S->job_cap;
-    if (!jl_hot.empty()) HIPCK(hipMemcpyAsync(d_hot, jl_hot.data(), jl_hot.size() * 4, hipMemcpyHostToDevice, c->st));
-    if (!jl_oth.empty()) HIPCK(hipMemcpyAsync(d_oth, jl_oth.data(), jl_oth.size() * 4, hipMemcpyHostToDevice, c->st));
-    dsx::launch_flood_lds(c->st, true, S->d_jobs, d_hot, uint32_t(jl_hot.size()), S->d_clim, S->grid, S->d_res, S->d_masks,
-                          B.lcount + dsx::FLOOD_MASKS, S->mask_cap);
-    dsx::launch_flood_lds(c->st, false, S->d_jobs, d_oth, uint32_t(jl_oth.size()), S->d_clim, S->grid, S->d_res,
-                          S->d_masks, B.lcount + dsx::FLOOD_MASKS, S->mask_cap);
+    uint32_t n_cls[3];
+    for (int cls = 0; cls < 3; cls++) {
+        n_cls[cls] = uint32_t(by_cls[cls].size());
+        jl_small.insert(jl_small.end(), by_cls[cls].begin(), by_cls[cls].end());
+    }
+    uint32_t *d_small = S->d_jl, *d_big = d_small + S->job_cap;
+    if (!jl_small.empty()) {
+        HIPCK(hipMemcpyAsync(d_small, jl_small.data(), jl_small.size() * 4, hipMemcpyHostToDevice, c->st));
+        uint32_t off = 0;
+        for (int cls = 0; cls < 3; cls++) {
+            dsx::launch_flood_lds(c->st, cls, S->d_jobs, d_small + off, n_cls[cls], S->d_clim, S->grid, S->d_res,
+                                  S->d_masks, B.ctl + dsx::FLOOD_MASKS, S->mask_cap);
+            off += n_cls[cls];
+        }
+    }
     HIPCK(hipGetLastError());
     std::vector<dsx::FloodRes> res(n);
-    if (!jl_hot.empty() || !jl_oth.empty()) {
-        // windows whose LDS ring overflowed go to the tiled engine
+    if (!jl_small.empty()) {
+        // floods that outgrew the LDS engine go to the tiled engine
         HIPCK(hipMemcpyAsync(res.data(), S->d_res, n * sizeof(dsx::FloodRes), hipMemcpyDeviceToHost, c->st));
         HIPCK(hipStreamSynchronize(c->st));
-        for (uint32_t k : jl_hot) if (res[k].border < 0) jl_big.push_back(k);
-        for (uint32_t k : jl_oth) if (res[k].border < 0) jl_big.push_back(k);
+        for (uint32_t k : jl_small)
+            if (res[k].border < 0) jl_big.push_back(k);
     }
     if (!jl_big.empty()) {
         HIPCK(hipMemcpyAsync(d_big, jl_big.data(), jl_big.size() * 4, hipMemcpyHostToDevice, c->st));
         dsx::launch_flood_resinit(c->st, S->d_jobs, d_big, uint32_t(jl_big.size()), S->d_res);
-        dsx::launch_flood_start(c->st, S->d_jobs, d_big, uint32_t(jl_big.size()), B, B.lists[0], B.lists[1]);
+        dsx::launch_flood_start(c->st, S->d_jobs, d_big, uint32_t(jl_big.size()), B);
         HIPCK(hipGetLastError());
     }
     const bool prof = g_prof_on.load(std::memory_order_relaxed) != 0;
-    uint64_t my_rounds = 0;
-    size_t nev = 0;   // events used by this call (pairs around the evaluation launches)
-    // rounds, queued several at a time between synchronisations (a finished flood leaves an empty
-    // list, so extra rounds do nothing); grids sized from the list length seen at the last check
+    size_t nev = 0;   // events used by this call (a pair around the persistent kernel)
     uint32_t *h = c->h_cnt;
     int status = DSX_OK;
     if (!jl_big.empty()) {
-        int round = 0, burst = 2;
-        uint32_t width = uint32_t(jl_big.size());
-        for (;;) {
-            for (int k = 0; k < burst; k++, round++) {
-                hipEvent_t e0 = nullptr, e1 = nullptr;
-                if (prof) {
-                    while (S->events.size() < nev + 2) {
-                        hipEvent_t e;
-                        HIPCK(hipEventCreate(&e));
-                        S->events.push_back(e);
-                    }
-                    e0 = S->events[nev];
-                    e1 = S->events[nev + 1];
-                    nev += 2;
-                }
-                dsx::launch_flood_round(c->st, S->d_jobs, S->d_clim, S->grid, B, round, 2 * width + 16 * uint32_t(k + 1), e0,
-                                        e1);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (prof) {
+            while (S->events.size() < 2) {
+                hipEvent_t e;
+                HIPCK(hipEventCreate(&e));
+                S->events.push_back(e);
             }
-            HIPCK(hipGetLastError());
-            my_rounds += burst;
-            HIPCK(hipMemcpyAsync(h, B.lcount, 32, hipMemcpyDeviceToHost, c->st));
-            HIPCK(hipStreamSynchronize(c->st));
-            if (h[dsx::FLOOD_OVF]) {
-                status = DSX_E_CAPACITY;
-                break;
-            }
-            const uint32_t ne = h[2 * (round % 3)], np = h[2 * (round % 3) + 1];
-            if (np == 0 || round > 4000000) break;
-            width = std::max(ne, np);
-            burst = std::min(burst * 2, 8);
+            e0 = S->events[0];
+            e1 = S->events[1];
+            nev = 2;
         }
+        // workgroups: enough for the tiles in flight (every flood's frontier), at most a full chip
+        static const uint32_t per_job = uint32_t(env_u32("DSX_FLOOD_BLOCKS_PER_JOB", 2)),
+                              max_blocks = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS", 256));
+        const uint32_t blocks = uint32_t(std::clamp<size_t>(per_job * jl_big.size(), 16, max_blocks));
+        dsx::launch_flood_run(c->st, S->d_jobs, S->d_clim, S->grid, B, blocks, e0, e1);
+        HIPCK(hipGetLastError());
     }
+    HIPCK(hipMemcpyAsync(h, B.ctl, dsx::CTL_WORDS * 4, hipMemcpyDeviceToHost, c->st));
     uint32_t ntiles = 0;
     HIPCK(hipMemcpyAsync(&ntiles, B.ntiles, 4, hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
     ntiles = std::min(ntiles, B.tile_cap);
+    const uint32_t qtail = std::min(h[dsx::Q_TAIL], B.wq_cap), items = h[dsx::Q_ITEMS];
+    if (h[dsx::FLOOD_OVF]) {
+        status = DSX_E_CAPACITY;
+        if (h[dsx::FLOOD_OVF] & dsx::OVF_TIME) {
+            status = DSX_E_HIP;
+            dsx::g_last_error = "flood engine time limit reached";
+        }
+    }
     if (status == DSX_OK) {
-        dsx::launch_flood_acc(c->st, S->d_jobs, B, ntiles, S->d_res, S->d_masks, B.lcount + dsx::FLOOD_MASKS, S->mask_cap);
+        dsx::launch_flood_acc(c->st, S->d_jobs, B, ntiles, S->d_res, S->d_masks, B.ctl + dsx::FLOOD_MASKS, S->mask_cap);
         HIPCK(hipGetLastError());
         HIPCK(hipMemcpyAsync(res.data(), S->d_res, n * sizeof(dsx::FloodRes), hipMemcpyDeviceToHost, c->st));
-        HIPCK(hipMemcpyAsync(h + dsx::FLOOD_MASKS, B.lcount + dsx::FLOOD_MASKS, 4, hipMemcpyDeviceToHost, c->st));
+        HIPCK(hipMemcpyAsync(h + dsx::FLOOD_MASKS, B.ctl + dsx::FLOOD_MASKS, 4, hipMemcpyDeviceToHost, c->st));
         HIPCK(hipStreamSynchronize(c->st));
         const uint32_t nm = h[dsx::FLOOD_MASKS];
         if (nm > S->mask_cap) {
@@ -437,7 +458,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         }
         dsx_flood_stats st{};
         st.calls = 1;
-        st.rounds = my_rounds;
+        st.rounds = items;
         std::vector<char> big(n, 0);
         for (uint32_t k : jl_big) big[k] = 1;
         for (uint64_t k = 0; k < n; k++) {
@@ -447,6 +468,11 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
                 st.tiles[kind] += res[k].tiles;
                 st.sampled_hot[kind] += res[k].hot;
                 st.sampled_cold[kind] += res[k].sampled - res[k].hot;
+                int b = 0;
+                while (b < 5 && res[k].tiles > (1ull << (2 * b))) b++;
+                st.size_floods[kind][b]++;
+                st.size_tiles[kind][b] += res[k].tiles;
+                st.size_sampled[kind][b] += res[k].sampled;
             } else {
                 st.small_floods++;
                 st.small_sampled += res[k].sampled;
@@ -462,14 +488,15 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         for (uint64_t k = 0; k < n; k++)
             out[k] = dsx_area{res[k].cells, res[k].sampled, res[k].minx, res[k].maxx, res[k].minz, res[k].maxz,
                               res[k].edge, res[k].border};
-    } else {
-        dsx::g_last_error = "flood tile pool exhausted: split the batch";
+    } else if (status == DSX_E_CAPACITY) {
+        dsx::g_last_error = "flood tile pool or work queue exhausted: split the batch";
     }
     // k_flood_acc has returned every tile and hash slot of this call to the clean state; after an
     // overflow (acc not run) reset them wholesale
     if (status != DSX_OK) {
         HIPCK(hipMemsetAsync(B.tiles, 0, size_t(ntiles) * sizeof(dsx::Tile), c->st));
         HIPCK(hipMemsetAsync(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8, c->st));
+        if (qtail) HIPCK(hipMemsetAsync(B.wq, 0xff, size_t(qtail) * 4, c->st));   // slots left unconsumed
     }
     HIPCK(hipStreamSynchronize(c->st));
     return status;

@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdsx.so")
 
 DSX_OK, DSX_E_ARG, DSX_E_HIP, DSX_E_CAPACITY, DSX_E_NODEVICE, DSX_E_BUSY = 0, 1, 2, 3, 4, 5
+HW_QUEUES = 16   # hardware queues requested from the HIP runtime: one per default pipeline worker
 
 
 class DsxError(RuntimeError):
@@ -50,6 +51,7 @@ FLOOD_JOB = np.dtype([("seed", "<u8"), ("kind", "<i4"), ("cut", "<f4"), ("qx", "
 AREA = np.dtype([("cells", "<u8"), ("sampled", "<u8"), ("minx", "<i4"), ("maxx", "<i4"), ("minz", "<i4"),
                  ("maxz", "<i4"), ("edge", "<i4"), ("border", "<i4")])
 PRED_DESERT_STATE, PRED_HOT, PRED_PROXY, PRED_DESERT = 0, 1, 2, 3
+TILE_MASK = np.dtype([("job", "<u4"), ("tx", "<i4"), ("tz", "<i4"), ("rows", "<u4", (32,))])
 AREA_QUERY = np.dtype([("seed", "<u8"), ("x", "<i4"), ("z", "<i4"), ("half", "<i4"), ("pad", "<i4")])
 
 
@@ -103,7 +105,9 @@ class FloodStats(ctypes.Structure):
     _fields_ = [("calls", ctypes.c_uint64), ("rounds", ctypes.c_uint64), ("floods", ctypes.c_uint64 * 4),
                 ("tiles", ctypes.c_uint64 * 4), ("sampled_cold", ctypes.c_uint64 * 4),
                 ("sampled_hot", ctypes.c_uint64 * 4), ("small_floods", ctypes.c_uint64),
-                ("small_sampled", ctypes.c_uint64), ("eval_ms", ctypes.c_double), ("eval_launches", ctypes.c_uint64)]
+                ("small_sampled", ctypes.c_uint64), ("eval_ms", ctypes.c_double), ("eval_launches", ctypes.c_uint64),
+                ("size_floods", (ctypes.c_uint64 * 6) * 4), ("size_tiles", (ctypes.c_uint64 * 6) * 4),
+                ("size_sampled", (ctypes.c_uint64 * 6) * 4)]
 
 
 class GateCounts(ctypes.Structure):
@@ -124,6 +128,9 @@ def load(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise DsxError(DSX_E_NODEVICE, "libdsx.so not built", path)
+    # one hardware queue per pipeline worker stream (read by the HIP runtime when it starts; a value
+    # the caller set is kept)
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(HW_QUEUES))
     lib = ctypes.CDLL(path)
     lib.dsx_strerror.restype = ctypes.c_char_p
     lib.dsx_strerror.argtypes = [ctypes.c_int]
@@ -145,6 +152,7 @@ def load(path=LIB_PATH):
     lib.dsx_predicates.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
     lib.dsx_flood.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+    lib.dsx_flood_masks.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, _u64p]
     lib.dsx_search_defaults.argtypes = [ctypes.POINTER(SearchParams)]
     lib.dsx_search_defaults.restype = None
     lib.dsx_search.argtypes = [ctypes.c_void_p, ctypes.POINTER(SearchParams), ctypes.c_int, ctypes.c_uint64,
@@ -166,7 +174,7 @@ def load(path=LIB_PATH):
     lib.dsx_profile.argtypes = [ctypes.c_int]
     lib.dsx_flood_stats_get.argtypes = [ctypes.POINTER(FloodStats), ctypes.c_int]
     for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile", "dsx_scan", "dsx_tables",
-              "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood", "dsx_search", "dsx_area_queries",
+              "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood", "dsx_flood_masks", "dsx_search", "dsx_area_queries",
               "dsx_server_create", "dsx_server_destroy", "dsx_server_workers", "dsx_submit_scan", "dsx_submit_area",
               "dsx_wait", "dsx_poll", "dsx_result_text", "dsx_release", "dsx_profile", "dsx_flood_stats_get"):
         getattr(lib, f).restype = ctypes.c_int
@@ -288,6 +296,15 @@ class Engine:
         j = np.ascontiguousarray(jobs, dtype=FLOOD_JOB)
         out = np.zeros(len(j), AREA)
         self._check(self.lib.dsx_flood(self.h, j.ctypes.data, len(j), out.ctypes.data), "dsx_flood")
+        return out
+
+    def flood_masks(self):
+        """Counted cells of the last flood() call: TILE_MASK array (job, tx, tz, rows[32])."""
+        n = ctypes.c_uint64()
+        self.lib.dsx_flood_masks(self.h, None, 0, ctypes.byref(n))
+        out = np.zeros(n.value, TILE_MASK)
+        if n.value:
+            self._check(self.lib.dsx_flood_masks(self.h, out.ctypes.data, n.value, ctypes.byref(n)), "dsx_flood_masks")
         return out
 
     def search_params(self, **kw):

@@ -19,6 +19,9 @@ static volatile sig_atomic_t interrupted = 0;
 static void on_signal(int) { interrupted = 1; }
 
 int main(int argc, char **argv) {
+    // one hardware queue per pipeline worker stream; read by the HIP runtime when it starts (a value
+    // set by the caller is kept)
+    setenv("GPU_MAX_HW_QUEUES", "16", 0);
     if (argc < 4) {
         std::fprintf(stderr, "dsearch_desert START COUNT OUT_PREFIX [threads] [key=value...]\n");
         return 2;

@@ -179,16 +179,19 @@ int dsx_flood_masks(dsx_ctx *ctx, dsx_tile_mask *out, uint64_t cap, uint64_t *n_
  * succeeds. "Sampled" cells are those the reference's flood evaluates (DenseArea::sampled), split by
  * cost class: hot = the predicate's temperature test passed, so its longer branch ran. */
 typedef struct {
-    uint64_t calls, rounds;        /* dsx_flood calls; evaluate + propagate rounds of the tiled engine */
+    uint64_t calls, rounds;        /* dsx_flood calls; tile work items the tiled engine processed */
     uint64_t floods[4];            /* floods run by the tiled engine, by kind (DSX_PRED_*) */
     uint64_t tiles[4];             /* 32x32 tiles it evaluated, by kind (x 1024 = cells evaluated) */
     uint64_t sampled_cold[4];      /* sampled cells of those floods, by kind and cost class */
     uint64_t sampled_hot[4];
     uint64_t small_floods, small_sampled; /* floods of the small-window engine and their sampled cells */
-    double eval_ms;                /* device time of the tile evaluation kernel (HIP events; profiling on) */
+    double eval_ms;                /* device time of the tiled engine's kernel (HIP events; profiling on) */
     uint64_t eval_launches;        /* its launches timed */
+    /* tiled-engine floods by kind and size class (tiles per flood <= 1, 4, 16, 64, 256, more):
+     * floods, tiles and sampled cells */
+    uint64_t size_floods[4][6], size_tiles[4][6], size_sampled[4][6];
 } dsx_flood_stats;
-/* 1: bracket every launch of the tile evaluation kernel with HIP events on its stream (eval_ms). */
+/* 1: bracket every launch of the tiled engine's kernel with HIP events on its stream (eval_ms). */
 int dsx_profile(int on);
 int dsx_flood_stats_get(dsx_flood_stats *out, int reset);
 

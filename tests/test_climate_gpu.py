@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, load_package
+from conftest import GOLDEN, FloodOut, load_package
 
 pytestmark = pytest.mark.gpu
 
@@ -115,6 +115,35 @@ def test_flood_small_windows_mixed_batch(engine, oracle):
                  (seed, 1, 0.35, qx, qz, 64, 128, 0), (seed, 2, 0.35, qx, qz, 32, 127, 1),
                  (seed, 2, 0.6, qx - 96, qz + 64, 16, 90, 1), (seed, 0, 0.35, qx, qz, 2, 1, 0)]
     _compare(engine, oracle, rows)
+
+
+def test_flood_counted_masks(engine, oracle):
+    """The counted cells each flood exports (state 3 of dense_flood.hpp's window) equal the oracle's
+    window state, for every engine: small windows, proxy and hot-zone floods that run on a local
+    sub-window of the one-wave engine, those that outgrow it (tiled engine), and exact floods."""
+    dsx = load_package().dsx
+    rows = []
+    for r in FINDS[14:20]:
+        seed, qx, qz = u64(r["seed"]), r["x"] >> 2, r["z"] >> 2
+        rows += [(seed, 2, 0.35, qx + 2048, qz - 1024, 32, 500, 1), (seed, 2, 0.35, qx, qz, 32, 500, 1),
+                 (seed, 1, 0.35, qx - 4096, qz, 128, 250, 0), (seed, 1, 0.35, qx, qz, 256, 125, 0),
+                 (seed, 0, 0.35, qx, qz, 8, 300, 1), (seed, 2, 0.5, qx + 512, qz + 512, 32, 100, 1)]
+    got = engine.flood(dsx.flood_jobs(rows))
+    masks = engine.flood_masks()
+    for k, row in enumerate(rows):
+        seed, kind, cut, qx, qz, step, half, c8 = row
+        n = 2 * half + 1
+        st = np.zeros(n * n, np.uint8)
+        out = FloodOut()
+        oracle.L.or_flood(seed, kind, cut, qx, qz, step, half, c8, ctypes.byref(out), st.ctypes.data)
+        want = st.reshape(n, n) == 3
+        have = np.zeros((((n + 31) // 32) * 32,) * 2, bool)
+        for m in masks[masks["job"] == k]:
+            bits = (m["rows"][:, None] >> np.arange(32, dtype=np.uint32)[None, :]) & 1
+            have[m["tz"] * 32:m["tz"] * 32 + 32, m["tx"] * 32:m["tx"] * 32 + 32] |= bits.astype(bool)
+        assert np.array_equal(have[:n, :n], want), row
+        assert not have[n:, :].any() and not have[:, n:].any()
+        assert int(got[k]["cells"]) == int(want.sum()) == out.cells
 
 
 def test_area_queries_step1_match_mcmeasure(engine):
