@@ -40,7 +40,8 @@ import threading
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")   # one hardware queue per pipeline worker stream
+# hardware queues for the 16 pipeline worker streams, the gate and spares (DSX_HW_QUEUES overrides)
+os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("DSX_HW_QUEUES", "24")
 START = 10 ** 12
 N_STEP = 10 ** 8
 FP64_PEAK_TFLOPS = 78.6
@@ -229,7 +230,7 @@ def main():
     launches = max(1, st.eval_launches)
     avg_ms = st.eval_ms / launches
     achieved = flops / (st.eval_ms * 1e-3) / 1e12 if st.eval_ms > 0 else 0.0
-    cells_eval = sum(st.tiles[k] for k in range(4)) * 1024
+    cells_eval = sum(st.evaluated[k] for k in range(4))
     sampled_big = sum(st.sampled_cold[k] + st.sampled_hot[k] for k in range(4))
     out = {
         "metric": "gated seeds/s on 1 MI355X vs host-CPU ref; p50/p99 area-query latency",
@@ -246,7 +247,7 @@ def main():
                      "flops_per_launch": flops / launches,
                      "octave_samples": int(samples), "sampled_cells": int(sampled_big),
                      "cells_evaluated": int(cells_eval),
-                     "by_kind": {KINDS[k]: {"floods": st.floods[k], "tiles": st.tiles[k],
+                     "by_kind": {KINDS[k]: {"floods": st.floods[k], "tiles": st.tiles[k], "evaluated": st.evaluated[k],
                                             "sampled_cold": st.sampled_cold[k], "sampled_hot": st.sampled_hot[k],
                                             "by_size": [[st.size_floods[k][b], st.size_tiles[k][b],
                                                          st.size_sampled[k][b]] for b in range(6)]}

@@ -46,4 +46,9 @@ struct dsx_ctx {
     uint64_t cand_cap = 0;               // candidates per seed the buffer holds
     int16_t *d_ibT = nullptr, *d_ibC = nullptr;
     int8_t *d_loT = nullptr, *d_hiT = nullptr;
+    // pinned scan I/O: seeds up; per-seed maxima, counts and candidates down (one synchronisation)
+    uint64_t *h_seeds = nullptr;
+    float *h_smax = nullptr;
+    uint32_t *h_sn = nullptr;
+    dsx::ScanCand *h_cand = nullptr;
 };

@@ -141,7 +141,8 @@ int dsx_destroy(dsx_ctx *c) {
                     (void *)c->d_act, (void *)c->d_ncand, (void *)c->d_cand, (void *)c->d_ibT, (void *)c->d_ibC,
                     (void *)c->d_loT, (void *)c->d_hiT})
         if (p) (void)hipFree(p);
-    for (void *p : {(void *)c->h_cnt, (void *)c->h_k3, (void *)c->h_a3})
+    for (void *p : {(void *)c->h_cnt, (void *)c->h_k3, (void *)c->h_a3, (void *)c->h_seeds, (void *)c->h_smax,
+                    (void *)c->h_sn, (void *)c->h_cand})
         if (p) (void)hipHostFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);

@@ -13,7 +13,7 @@
 //   * the outputs (cells, sampled, bbox, edge, border) are order independent, so the visiting order
 //     (wave-parallel, ring order) gives exactly the reference's depth-first results.
 // A flood that reaches the edge of its local sub-window inside the real window, or whose ring would
-// overflow, is reported unfinished (res.border = -1); the host reruns it on the tiled engine.
+// overflow, is handed to the tiled engine (appended to its job list, which runs next on the stream).
 #include "floods.h"
 #include <climits>
 #include <type_traits>
@@ -51,8 +51,9 @@ __device__ __forceinline__ uint32_t bitmap_row32(const uint32_t *bm, int L, int 
 
 template <class CS>
 __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
-                                                  SurfGrid g, FloodRes *res, TileMask *masks, uint32_t *mcount,
-                                                  uint32_t mcap) {
+                                                  SurfGrid g, FloodRes *res, TileMask *masks, uint32_t mcap, uint32_t *big,
+                                                  uint32_t *ctl) {
+    uint32_t *mcount = ctl + FLOOD_MASKS;
     __shared__ CS C;
     __shared__ uint32_t touched[LDS_WORDS], counted[LDS_WORDS];
     __shared__ uint32_t ring[LDS_RING];
@@ -165,8 +166,14 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         maxz = max(maxz, __shfl_xor(maxz, o2));
     }
     const bool any_edge = __any(edge), any_border = __any(border);
-    if (ovf) {
-        if (lane == 0) res[job].border = -1;   // not done: the host reruns it on the tiled engine
+    if (ovf) {   // not done here: hand it to the tiled engine
+        if (lane == 0) {
+            FloodRes r{};
+            r.minx = r.maxx = J.qx;
+            r.minz = r.maxz = J.qz;
+            res[job] = r;
+            big[atomicAdd(&ctl[BIG_N], 1u)] = job;
+        }
         return;
     }
     if (lane == 0) {
@@ -181,6 +188,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         r.border = any_border;
         r.hot = hotc;
         r.tiles = 0;
+        r.evald = sampled;
         res[job] = r;
     }
     if (cells == 0) return;
@@ -213,14 +221,14 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
 }
 
 void launch_flood_lds(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
-                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t *mcount, uint32_t mcap) {
+                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, uint32_t *big, uint32_t *ctl) {
     if (!n) return;
     if (cls == LDS_HOT)
-        hipLaunchKernelGGL(k_flood_lds<ClimT>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcount, mcap);
+        hipLaunchKernelGGL(k_flood_lds<ClimT>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, big, ctl);
     else if (cls == LDS_PROXY)
-        hipLaunchKernelGGL(k_flood_lds<ClimP>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcount, mcap);
+        hipLaunchKernelGGL(k_flood_lds<ClimP>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, big, ctl);
     else
-        hipLaunchKernelGGL(k_flood_lds<ClimSeed>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcount, mcap);
+        hipLaunchKernelGGL(k_flood_lds<ClimSeed>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, big, ctl);
 }
 
 }  // namespace dsx

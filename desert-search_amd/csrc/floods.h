@@ -20,6 +20,7 @@ struct FloodRes {
     int edge, border;
     unsigned long long hot;     // sampled cells that passed the predicate's temperature test
     unsigned long long tiles;   // tiles the engine allocated for this flood (0: small-window engine)
+    unsigned long long evald;   // cells it evaluated (8x8 sub-blocks x 64)
 };
 // A 32x32 block of one flood's window: window cell (i, j) = (tx*32 + c, tz*32 + r) <-> bit c of row r.
 struct Tile {
@@ -27,7 +28,7 @@ struct Tile {
     int32_t tx, tz;
     uint32_t qstate;        // 0 idle, 1 queued or being processed, 2 being processed and touched again
     uint32_t slot1;         // hash slot + 1 of this tile's entry (0: not in the table)
-    uint32_t evald;         // 1 once its 1024 cells are evaluated
+    uint32_t evald;         // 8x8 sub-blocks evaluated so far (bit bz*4 + bx)
     uint32_t pad_[2];
     uint32_t pass[FT];      // predicate != 0 (inside window and world)
     uint32_t cnt[FT];       // predicate == 2
@@ -55,32 +56,41 @@ struct FloodBufs {
     uint32_t *ctl;                // [Q_HEAD] [Q_TAIL] [Q_PENDING] [FLOOD_OVF] [FLOOD_MASKS] [Q_ITEMS]
 };
 
-constexpr int Q_HEAD = 0, Q_TAIL = 1, Q_PENDING = 2, FLOOD_OVF = 3, FLOOD_MASKS = 4, Q_ITEMS = 5, CTL_WORDS = 8;
+// ctl words: queue head/tail, pending items, overflow flags, exported masks, items processed, the
+// length of the tiled engine's job list (grows when a small-window flood hands over), allocated tiles
+// (published)
+constexpr int Q_HEAD = 0, Q_TAIL = 1, Q_PENDING = 2, FLOOD_OVF = 3, FLOOD_MASKS = 4, Q_ITEMS = 5, BIG_N = 6,
+              CTL_NTILES = 7, CTL_WORDS = 8;
 constexpr uint32_t Q_EMPTY = 0xffffffffu;
 // FLOOD_OVF bits: 1 work queue full, 2 tile pool or hash table full, 4 time limit reached
 constexpr uint32_t OVF_QUEUE = 1, OVF_TILES = 2, OVF_TIME = 4;
 
-// jl[0..n): indices of the jobs to start (their hash keys and result slots use the original index);
-// their start tiles are queued
-void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B);
+// Before a batch of n floods: zero the control words and the tile count, set the tiled engine's job
+// list length to nbig, and initialise every result (an empty area at the centre).
+void launch_flood_prep(hipStream_t st, const FloodJob *jobs, uint32_t n, FloodRes *res, FloodBufs B, uint32_t nbig);
+// The start tile of every job of the tiled engine's list big[0 .. ctl[BIG_N]) is touched and queued
+// (n: an upper bound of the list length).
+void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *big, uint32_t n, FloodBufs B);
 // The persistent flood kernel: `blocks` workgroups take queued tiles until no work is pending. A tile
-// is evaluated (all 1024 cells) the first time it is processed; every processing grows its reached
-// set to a fixpoint and queues the neighbours it touches. ev0/ev1 (optional): HIP events around it.
+// is evaluated lazily, by 8x8 sub-blocks the flood touches; every processing grows its reached set to
+// a fixpoint and queues the neighbours it touched. ev0/ev1 (optional): HIP events around it.
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
                       uint32_t blocks, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-// Reduce, export the counted masks, and return every touched tile and hash slot to the zero / empty
-// state the next call expects (no pool-wide memsets).
-void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res,
-                      struct TileMask *masks, uint32_t *mcount, uint32_t mcap);
-// res[k] = empty area at the centre, for k in jl[0..n) (jl null: k in [0, n))
-void launch_flood_resinit(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodRes *res);
+// Reduce every allocated tile (count read on the device) into res, export counted masks, and return
+// the tiles and hash slots to the clean state.
+void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRes *res, struct TileMask *masks,
+                      uint32_t mcap, uint32_t blocks);
+// Copy the results, the control words (ctl[CTL_NTILES] = allocated tiles) and up to hmcap exported
+// masks into host memory, so that one stream synchronisation ends the batch.
+void launch_flood_publish(hipStream_t st, const FloodRes *res, uint32_t n, FloodBufs B, const struct TileMask *masks,
+                          FloodRes *h_res, uint32_t *h_ctl, struct TileMask *h_masks, uint32_t hmcap);
 // One wave per flood with its state in LDS over a local sub-window of at most LDS_MAX_N^2 points
-// around the start; floods that outgrow it come back with res.border = -1 (rerun them on the tiled
-// engine). cls: LDS_HOT (the T test, any window), LDS_PROXY (proxy predicate, any window), LDS_FULL
-// (any kind; the host sends only windows that fit whole).
+// around the start; a flood that outgrows it is handed to the tiled engine (its result reset and its
+// index appended to big[], counted in ctl[BIG_N]). cls: LDS_HOT (the T test, any window), LDS_PROXY
+// (proxy predicate, any window), LDS_FULL (any kind; the host sends only windows that fit whole).
 constexpr int LDS_MAX_N = 255;
 enum { LDS_HOT = 0, LDS_PROXY = 1, LDS_FULL = 2 };
 void launch_flood_lds(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
-                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t *mcount, uint32_t mcap);
+                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, uint32_t *big, uint32_t *ctl);
 
 }  // namespace dsx

@@ -6,12 +6,14 @@
 // cells reachable from the start through predicate != 0 cells and on their neighbourhood, not on the
 // visiting order. This engine computes the same sets tile-parallel:
 //   * the window is split into 32x32 tiles, allocated on first touch through a device hash table;
-//   * k_flood_run is a persistent kernel over a device work queue of tiles. A workgroup takes a tile,
-//     evaluates the FP64 predicate of its 1024 cells the first time (the octaves the predicate reads
-//     staged in LDS), grows the reached set inside the tile to a fixpoint with bit-parallel dilation
-//     (one wave, one lane per 32-bit row mask), ORs the touched cells across the tile edges into the
-//     neighbours' `in` masks and queues the neighbours it touched. It runs until no work is pending:
-//     no per-round kernel launches and no host round trips;
+//   * k_flood_run is a persistent kernel over a device work queue of tiles. A workgroup takes a tile
+//     and grows its reached set to a fixpoint with bit-parallel dilation (one wave, one lane per 32-bit
+//     row mask) through the cells evaluated so far; whenever the touched cells (the reached set's
+//     neighbours and the cells touched from neighbouring tiles) reach an 8x8 sub-block not evaluated
+//     yet, its waves evaluate the FP64 predicate of those sub-blocks (one cell per lane, the octaves
+//     the predicate reads staged in LDS) and the fixpoint continues. Then it ORs the new frontier
+//     across the tile edges into the neighbours' `in` masks and queues the neighbours it touched. It
+//     runs until no work is pending: no per-round kernel launches and no host round trips;
 //   * k_flood_acc then reduces each tile:
 //       reached = pushed cells, touched = reached | in | dilate_inside(reached),
 //       cells = |reached & pred==2|, sampled = |touched inside the world|, border = touched outside
@@ -26,12 +28,13 @@
 // hash publication) the wave first waits until its earlier accesses have completed.
 //
 // Termination. Q_PENDING counts queued items not yet finished (an item's own pushes are counted
-// before it finishes), so it reaches 0 exactly when no more work can appear. A workgroup waiting on
-// an empty queue slot leaves when Q_PENDING is 0, when an overflow was flagged, or after a time limit
-// (flagged as an error), so every wave exits.
+// before it finishes), so it reaches 0 exactly when no more work can appear. A workgroup claims the
+// next queue slot and polls it (each waiter on its own address); it leaves when Q_PENDING is 0, when
+// an overflow was flagged, or after a time limit (flagged as an error), so every wave exits.
 #include "floods.h"
 #include <hip/hip_ext.h>
 #include <climits>
+#include <cstdlib>
 
 namespace dsx {
 
@@ -121,21 +124,23 @@ __device__ __forceinline__ void tile_touch(const FloodBufs &B, uint32_t ti) {
     }
 }
 
-__global__ void k_flood_resinit(const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodRes *res) {
+__global__ __launch_bounds__(256) void k_flood_prep(const FloodJob *jobs, uint32_t n, FloodRes *res, FloodBufs B,
+                                                    uint32_t nbig) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < CTL_WORDS) B.ctl[q] = q == BIG_N ? nbig : 0u;
+    if (q == 0) *B.ntiles = 0;
     if (q >= n) return;
-    const uint32_t k = jl ? jl[q] : q;
     FloodRes r{};
-    r.minx = r.maxx = jobs[k].qx;
-    r.minz = r.maxz = jobs[k].qz;
-    res[k] = r;
+    r.minx = r.maxx = jobs[q].qx;
+    r.minz = r.maxz = jobs[q].qz;
+    res[q] = r;
 }
 
-// The start cell of every flood: touched ("in") and its tile queued.
-__global__ __launch_bounds__(256) void k_flood_start(const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B) {
+// The start cell of every flood of the tiled engine's list: touched ("in") and its tile queued.
+__global__ __launch_bounds__(256) void k_flood_start(const FloodJob *jobs, const uint32_t *big, uint32_t n, FloodBufs B) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const uint32_t k = jl[q];
+    if (q >= min(n, B.ctl[BIG_N])) return;
+    const uint32_t k = big[q];
     const FloodJob J = jobs[k];
     const int c = J.half % FT, t = J.half / FT;   // the centre cell (half, half)
     bool fresh = false;
@@ -148,38 +153,43 @@ __global__ __launch_bounds__(256) void k_flood_start(const FloodJob *jobs, const
     tile_touch(B, ti);
 }
 
-// Evaluate the 1024 cells of tile T with NT threads (pass = v != 0, cnt = v == 2, hot = the
-// predicate's temperature test passed; cells outside the window or the world are none of them).
-template <int NT>
-__device__ __forceinline__ void eval_tile(Tile &T, const FloodJob &J, const ClimSeed &C, const SurfGrid &g) {
-    const int tid = threadIdx.x;
+// Sub-blocks: a tile's 32x32 cells are evaluated lazily in 16 blocks of 8x8 (bit bz*4 + bx of the
+// tile's `evald` mask), only those holding cells the flood touches.
+constexpr int SB = 8, NSB = FT / SB;
+
+// Evaluate sub-block sb of the tile at (tx, tz) with one wave (one cell per lane) and OR its bits into
+// the LDS row masks (pass = v != 0, cnt = v == 2, hot = the predicate's temperature test passed; cells
+// outside the window or the world are none of them).
+__device__ __forceinline__ void eval_subblock(int sb, int tx, int tz, const FloodJob &J, const ClimSeed &C,
+                                              const SurfGrid &g, uint32_t *s_pass, uint32_t *s_cnt, uint32_t *s_hot) {
+    const int lane = threadIdx.x & 63;
+    const int bx = sb % NSB, bz = sb / NSB;
+    const int r = bz * SB + (lane >> 3), col = bx * SB + (lane & 7);
+    const int i = tx * FT + col, j = tz * FT + r;
     const int nw = int(nwin(J));
-    const int tx = ald(&T.tx), tz = ald(&T.tz);
-#pragma unroll 1
-    for (int base = 0; base < FT * FT; base += NT) {
-        const int cell = base + tid, r = cell / FT, col = cell % FT;   // a wave covers rows r, r + 1
-        const int i = tx * FT + col, j = tz * FT + r;
-        int v = 0;
-        bool hot = false;
-        if (i < nw && j < nw) {
-            const int x = J.qx + (i - J.half) * J.step, z = J.qz + (j - J.half) * J.step;
-            if (x >= -WORLD_Q && x <= WORLD_Q && z >= -WORLD_Q && z <= WORLD_Q) v = predicate(J.kind, C, g, x, z, J.cut, &hot);
-        }
-        const uint64_t mp = __ballot(v != 0), mc = __ballot(v == 2), mh = __ballot(hot);
-        if ((tid & 63) == 0) {
-            ast(&T.pass[r], uint32_t(mp));
-            ast(&T.pass[r + 1], uint32_t(mp >> 32));
-            ast(&T.cnt[r], uint32_t(mc));
-            ast(&T.cnt[r + 1], uint32_t(mc >> 32));
-            ast(&T.hot[r], uint32_t(mh));
-            ast(&T.hot[r + 1], uint32_t(mh >> 32));
-        }
+    int v = 0;
+    bool hot = false;
+    if (i < nw && j < nw) {
+        const int x = J.qx + (i - J.half) * J.step, z = J.qz + (j - J.half) * J.step;
+        if (x >= -WORLD_Q && x <= WORLD_Q && z >= -WORLD_Q && z <= WORLD_Q) v = predicate(J.kind, C, g, x, z, J.cut, &hot);
     }
-    mem_done();   // each wave's stores are performed before the tile is handed on
+    const uint64_t mp = __ballot(v != 0), mc = __ballot(v == 2), mh = __ballot(hot);
+    if (lane < SB) {   // lane rr: row bz*8 + rr of the sub-block
+        const int sh = lane * SB;
+        const int row = bz * SB + lane;
+        const uint32_t p = uint32_t((mp >> sh) & 0xffu) << (bx * SB), c = uint32_t((mc >> sh) & 0xffu) << (bx * SB),
+                       h = uint32_t((mh >> sh) & 0xffu) << (bx * SB);
+        if (p) atomicOr(&s_pass[row], p);
+        if (c) atomicOr(&s_cnt[row], c);
+        if (h) atomicOr(&s_hot[row], h);
+    }
 }
 
-// Grow the reached set of tile ti to a fixpoint and hand the new frontier to the neighbours (one wave).
-__device__ void prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J) {
+// One wave: grow the reached set of tile T through its evaluated passable cells to a fixpoint. Returns
+// the sub-blocks that hold touched cells not yet evaluated (reached cells' neighbours, and cells
+// touched from neighbouring tiles). When none are left (0), the reached set is final: it is stored
+// and its new frontier is handed to the neighbouring tiles.
+__device__ uint32_t prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J, const uint32_t *s_pass, uint32_t evald) {
     Tile &T = B.tiles[ti];
     const int lane = threadIdx.x & 63;
     const int r = lane & 31;
@@ -187,21 +197,36 @@ __device__ void prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J) {
     const uint32_t job = ald(&T.job);
     const int tx = ald(&T.tx), tz = ald(&T.tz);
     const uint32_t in = lane < FT ? ald(&T.in[r]) : 0u;
-    const uint32_t pass = lane < FT ? ald(&T.pass[r]) : 0u, old = lane < FT ? ald(&T.reach[r]) : 0u;
+    const uint32_t pass = lane < FT ? s_pass[r] : 0u, old = lane < FT ? ald(&T.reach[r]) : 0u;
     uint32_t R = lane < FT ? (old | (in & pass)) : 0u;
+    uint32_t D;
     for (;;) {
         const uint32_t up = __shfl(R, (r + 31) & 31, 32), dn = __shfl(R, (r + 1) & 31, 32);
         const uint32_t upv = r > 0 ? up : 0u, dnv = r < FT - 1 ? dn : 0u;
-        uint32_t D = R | (R << 1) | (R >> 1) | upv | dnv;
+        D = R | (R << 1) | (R >> 1) | upv | dnv;
         if (c8) D |= (upv << 1) | (upv >> 1) | (dnv << 1) | (dnv >> 1);
         const uint32_t Rn = lane < FT ? (R | (D & pass)) : 0u;
         const bool ch = Rn != R;
         R = Rn;
         if (!__any(ch)) break;
     }
+    // touched cells of this tile (inside the window) and the sub-blocks they need
+    const int nw = int(nwin(J));
+    const uint32_t win = (lane < FT && tz * FT + r < nw) ? col_mask(J, tx) : 0u;
+    const uint32_t touched = (D | in) & win;
+    uint32_t need = 0;
+#pragma unroll
+    for (int bx = 0; bx < NSB; bx++) {
+        const uint64_t rows = __ballot(((touched >> (bx * SB)) & 0xffu) != 0u);
+#pragma unroll
+        for (int bz = 0; bz < NSB; bz++)
+            if ((rows >> (bz * SB)) & 0xffu) need |= 1u << (bz * NSB + bx);
+    }
+    need &= ~evald;
+    if (need) return need;
     const uint32_t NR = R & ~old;
     if (lane < FT && NR) ast(&T.reach[r], R);
-    if (!__any(NR != 0)) return;
+    if (!__any(NR != 0)) return 0u;
     // ---- frontier across the tile edges
     const int nta = int(ntile_axis(J));
     const uint32_t up0 = __shfl(NR, 0, 32), dn31 = __shfl(NR, 31, 32);
@@ -236,7 +261,7 @@ __device__ void prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J) {
         const int ntx = tx + dx, ntz = tz + dz;
         if (ntx < 0 || ntz < 0 || ntx >= nta || ntz >= nta) continue;     // outside the window: edge (acc)
         mask &= col_mask(J, ntx);
-        if (ntz * FT + row >= int(nwin(J))) mask = 0;
+        if (ntz * FT + row >= nw) mask = 0;
         if (!__any(mask != 0)) continue;
         uint32_t nti = 0;
         bool fresh = false;
@@ -254,15 +279,19 @@ __device__ void prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J) {
         }
         if (__any(grew) && lane == 0) tile_touch(B, nti);
     }
+    return 0u;
 }
 
-// Persistent workgroups over the work queue (NT threads each; wave 0 propagates).
-template <int NT>
-__global__ __launch_bounds__(NT) void k_flood_run(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B) {
+// Persistent workgroups over the work queue (NT threads each; wave 0 propagates, all waves evaluate).
+// OCC: waves per SIMD the register allocation must allow.
+template <int NT, int OCC>
+__global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B) {
+    constexpr int NWAVE = NT / 64;
     __shared__ ClimSeed C;
-    __shared__ uint32_t s_item, s_again;
+    __shared__ uint32_t s_pass[FT], s_cnt[FT], s_hot[FT];
+    __shared__ uint32_t s_item, s_again, s_need, s_evald;
     __shared__ int s_stop;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6;
     int loaded_seed = -1, loaded_cls = -1;
     for (;;) {
         // ---- take the next queue slot and wait for its tile
@@ -302,40 +331,67 @@ __global__ __launch_bounds__(NT) void k_flood_run(const FloodJob *jobs, const Cl
         if (tid == 0) {
             atomicExch(&T.qstate, 1u);   // in process: touches from now on ask for another pass
             atomicAdd(&B.ctl[Q_ITEMS], 1u);
+            s_evald = ald(&T.evald);
+        }
+        if (tid < FT) {
+            s_pass[tid] = ald(&T.pass[tid]);
+            s_cnt[tid] = ald(&T.cnt[tid]);
+            s_hot[tid] = ald(&T.hot[tid]);
         }
         __syncthreads();
         const FloodJob J = jobs[ald(&T.job)];
-        // ---- first processing: evaluate the tile (octaves the predicate reads in LDS)
-        if (ald(&T.evald) == 0u) {
-            const int cls = J.kind == PRED_HOT ? 0 : (J.kind == PRED_PROXY ? 1 : 2);
-            if (J.seed != loaded_seed || cls > loaded_cls) {
-                const ClimSeed &G = cs[J.seed];
-                constexpr int OW = sizeof(ClimOct) / 16;
-                if (cls < 2) {
-                    const int nslot = cls == 0 ? 4 : 14;
-                    for (int k = tid; k < nslot * OW; k += NT) {
-                        const int slot = PROXY_SLOTS[k / OW];
-                        reinterpret_cast<uint4 *>(&C.oct[slot])[k % OW] = reinterpret_cast<const uint4 *>(&G.oct[slot])[k % OW];
-                    }
-                    constexpr int GW = sizeof(GradTab) / 16;
-                    if (tid < GW) reinterpret_cast<uint4 *>(&C.g)[tid] = reinterpret_cast<const uint4 *>(&G.g)[tid];
-                    if (tid < 6) C.factor[tid] = G.factor[tid];
-                } else {
-                    const uint4 *src = reinterpret_cast<const uint4 *>(&G);
-                    uint4 *dst = reinterpret_cast<uint4 *>(&C);
-                    for (int k = tid; k < int(sizeof(ClimSeed) / 16); k += NT) dst[k] = src[k];
+        const int tx = ald(&T.tx), tz = ald(&T.tz);
+        const uint32_t evald0 = s_evald;
+        for (;;) {
+            // ---- evaluate the touched sub-blocks until the reached set is final, then propagate
+            for (;;) {
+                if (tid < 64) {
+                    const uint32_t need = prop_tile(B, ti, J, s_pass, s_evald);
+                    if (tid == 0) s_need = need;
                 }
                 __syncthreads();
-                loaded_seed = J.seed;
-                loaded_cls = cls;
+                const uint32_t need = s_need;
+                if (!need) break;
+                const int cls = J.kind == PRED_HOT ? 0 : (J.kind == PRED_PROXY ? 1 : 2);
+                if (J.seed != loaded_seed || cls > loaded_cls) {   // the octaves the predicate reads, in LDS
+                    const ClimSeed &G = cs[J.seed];
+                    constexpr int OW = sizeof(ClimOct) / 16;
+                    if (cls < 2) {
+                        const int nslot = cls == 0 ? 4 : 14;
+                        for (int k = tid; k < nslot * OW; k += NT) {
+                            const int slot = PROXY_SLOTS[k / OW];
+                            reinterpret_cast<uint4 *>(&C.oct[slot])[k % OW] =
+                                reinterpret_cast<const uint4 *>(&G.oct[slot])[k % OW];
+                        }
+                        constexpr int GW = sizeof(GradTab) / 16;
+                        if (tid < GW) reinterpret_cast<uint4 *>(&C.g)[tid] = reinterpret_cast<const uint4 *>(&G.g)[tid];
+                        if (tid < 6) C.factor[tid] = G.factor[tid];
+                    } else {
+                        const uint4 *src = reinterpret_cast<const uint4 *>(&G);
+                        uint4 *dst = reinterpret_cast<uint4 *>(&C);
+                        for (int k = tid; k < int(sizeof(ClimSeed) / 16); k += NT) dst[k] = src[k];
+                    }
+                    __syncthreads();
+                    loaded_seed = J.seed;
+                    loaded_cls = cls;
+                }
+                // the k-th needed sub-block goes to wave k % NWAVE
+                int k = 0;
+                for (uint32_t m = need; m; m &= m - 1, k++)
+                    if (k % NWAVE == wave) eval_subblock(__ffs(m) - 1, tx, tz, J, C, g, s_pass, s_cnt, s_hot);
+                __syncthreads();
+                if (tid == 0) s_evald |= need;
+                __syncthreads();
             }
-            eval_tile<NT>(T, J, C, g);
-            __syncthreads();
-            if (tid == 0) ast(&T.evald, 1u);
-        }
-        // ---- propagate, as many passes as touches arrive meanwhile
-        for (;;) {
-            if (tid < 64) prop_tile(B, ti, J);
+            if (tid < FT && s_evald != evald0) {   // newly evaluated cells: store the tile's masks
+                ast(&T.pass[tid], s_pass[tid]);
+                ast(&T.cnt[tid], s_cnt[tid]);
+                ast(&T.hot[tid], s_hot[tid]);
+            }
+            if (tid == 0) {
+                if (s_evald != evald0) ast(&T.evald, s_evald);
+                mem_done();
+            }
             __syncthreads();
             if (tid == 0) {
                 mem_done();
@@ -355,10 +411,8 @@ __global__ __launch_bounds__(NT) void k_flood_run(const FloodJob *jobs, const Cl
 }
 
 // Per-tile reduction of the flood outputs into res[job]; leaves the tile and its hash slot clean.
-__global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res,
-                                                  TileMask *masks, uint32_t *mcount, uint32_t mcap) {
-    const uint32_t b = blockIdx.x;
-    if (b >= ntiles) return;
+__device__ void acc_tile(const FloodJob *jobs, FloodBufs B, uint32_t b, FloodRes *res, TileMask *masks, uint32_t *mcount,
+                         uint32_t mcap) {
     Tile &T = B.tiles[b];
     const FloodJob J = jobs[T.job];
     const int lane = threadIdx.x, r = lane & 31;
@@ -414,6 +468,7 @@ __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBuf
     if (lane == 0) {
         FloodRes &o = res[T.job];
         atomicAdd(&o.tiles, 1ull);
+        atomicAdd(&o.evald, 64ull * __popc(T.evald));
         if (sampled) atomicAdd(&o.sampled, sampled);
         if (hotc) atomicAdd(&o.hot, hotc);
         if (cells) {
@@ -445,6 +500,9 @@ __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBuf
     if (lane < FT) {
         T.reach[lane] = 0;
         T.in[lane] = 0;
+        T.pass[lane] = 0;
+        T.cnt[lane] = 0;
+        T.hot[lane] = 0;
     }
     if (lane == 0) {
         if (T.slot1) B.hkeys[T.slot1 - 1] = HEMPTY;
@@ -454,25 +512,53 @@ __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBuf
     }
 }
 
+__global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBufs B, FloodRes *res, TileMask *masks,
+                                                  uint32_t mcap) {
+    const uint32_t ntiles = min(*B.ntiles, B.tile_cap);
+    for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) acc_tile(jobs, B, b, res, masks, &B.ctl[FLOOD_MASKS], mcap);
+}
+
+__global__ __launch_bounds__(256) void k_flood_publish(const FloodRes *res, uint32_t n, FloodBufs B, const TileMask *masks,
+                                                       FloodRes *h_res, uint32_t *h_ctl, TileMask *h_masks, uint32_t hmcap) {
+    const uint32_t nt = blockDim.x * gridDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 < CTL_WORDS) h_ctl[t0] = t0 == CTL_NTILES ? *B.ntiles : B.ctl[t0];
+    for (uint32_t q = t0; q < n; q += nt) h_res[q] = res[q];
+    const uint32_t nm = min(min(B.ctl[FLOOD_MASKS], hmcap), 0xffffffffu);
+    constexpr uint32_t W = sizeof(TileMask) / 4;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(masks);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(h_masks);
+    for (uint32_t q = t0; q < nm * W; q += nt) dst[q] = src[q];
+}
+
 // ------------------------------------------------------------------ launchers
 
 constexpr int RUN_NT = 256;
 
-void launch_flood_resinit(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodRes *res) {
-    if (n) hipLaunchKernelGGL(k_flood_resinit, dim3((n + 255) / 256), dim3(256), 0, st, jobs, jl, n, res);
+void launch_flood_prep(hipStream_t st, const FloodJob *jobs, uint32_t n, FloodRes *res, FloodBufs B, uint32_t nbig) {
+    const uint32_t m = n > uint32_t(CTL_WORDS) ? n : uint32_t(CTL_WORDS);
+    hipLaunchKernelGGL(k_flood_prep, dim3((m + 255) / 256), dim3(256), 0, st, jobs, n, res, B, nbig);
 }
-void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *jl, uint32_t n, FloodBufs B) {
-    if (n) hipLaunchKernelGGL(k_flood_start, dim3((n + 255) / 256), dim3(256), 0, st, jobs, jl, n, B);
+void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *big, uint32_t n, FloodBufs B) {
+    if (n) hipLaunchKernelGGL(k_flood_start, dim3((n + 255) / 256), dim3(256), 0, st, jobs, big, n, B);
 }
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
                       uint32_t blocks, hipEvent_t ev0, hipEvent_t ev1) {
     const dim3 grid(blocks ? blocks : 1);
-    if (ev0) hipExtLaunchKernelGGL(k_flood_run<RUN_NT>, grid, dim3(RUN_NT), 0, st, ev0, ev1, 0u, jobs, cs, g, B);
-    else hipLaunchKernelGGL(k_flood_run<RUN_NT>, grid, dim3(RUN_NT), 0, st, jobs, cs, g, B);
+    static const bool occ4 = [] {
+        const char *v = std::getenv("DSX_FLOOD_OCC");
+        return v && std::atoi(v) >= 4;
+    }();
+    auto kern = occ4 ? k_flood_run<RUN_NT, 4> : k_flood_run<RUN_NT, 1>;
+    if (ev0) hipExtLaunchKernelGGL(kern, grid, dim3(RUN_NT), 0, st, ev0, ev1, 0u, jobs, cs, g, B);
+    else hipLaunchKernelGGL(kern, grid, dim3(RUN_NT), 0, st, jobs, cs, g, B);
 }
-void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, uint32_t ntiles, FloodRes *res, TileMask *masks,
-                      uint32_t *mcount, uint32_t mcap) {
-    if (ntiles) hipLaunchKernelGGL(k_flood_acc, dim3(ntiles), dim3(64), 0, st, jobs, B, ntiles, res, masks, mcount, mcap);
+void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRes *res, TileMask *masks, uint32_t mcap,
+                      uint32_t blocks) {
+    hipLaunchKernelGGL(k_flood_acc, dim3(blocks ? blocks : 1), dim3(64), 0, st, jobs, B, res, masks, mcap);
+}
+void launch_flood_publish(hipStream_t st, const FloodRes *res, uint32_t n, FloodBufs B, const TileMask *masks,
+                          FloodRes *h_res, uint32_t *h_ctl, TileMask *h_masks, uint32_t hmcap) {
+    hipLaunchKernelGGL(k_flood_publish, dim3(64), dim3(256), 0, st, res, n, B, masks, h_res, h_ctl, h_masks, hmcap);
 }
 
 }  // namespace dsx

@@ -85,18 +85,27 @@ struct ClimState {
     std::vector<uint32_t> miss_slot;                 // until the call's stream synchronises)
     // flood buffers
     dsx::FloodBufs fb{};
-    dsx::FloodJob *d_jobs = nullptr;
     dsx::FloodRes *d_res = nullptr;
     uint32_t job_cap = 0;
     // batch I/O staging
     void *d_io = nullptr;
+    void *h_io = nullptr;
     size_t io_cap = 0;
     // counted-cell tiles of the last dsx_flood call
     dsx::TileMask *d_masks = nullptr;
     uint32_t mask_cap = 0;
     std::vector<dsx::TileMask> masks;
-    uint32_t *d_jl = nullptr;   // job index lists: small hot + small other (<= job_cap), then tiled (<= job_cap)
-    std::vector<hipEvent_t> events;   // evaluation-kernel event pairs (profiling on)
+    std::vector<hipEvent_t> events;   // event pair around the tiled engine's kernel (profiling on)
+    // flood batch I/O: jobs and job lists go up in one copy from pinned staging (device layout: n jobs,
+    // n small-window indices, n tiled-engine indices); results come down through pinned memory the
+    // publish kernel writes
+    uint8_t *d_up = nullptr;
+    uint8_t *h_up = nullptr;
+    size_t h_up_cap = 0;
+    dsx::FloodRes *h_res = nullptr;
+    uint32_t *h_ctl = nullptr;
+    dsx::TileMask *h_masks = nullptr;
+    uint32_t h_res_cap = 0, h_mask_cap = 0;
 };
 
 static ClimState *clim_state(dsx_ctx *c);
@@ -136,20 +145,46 @@ static int clim_alloc(dsx_ctx *c, ClimState *&S) {
     HIPCK(hipMemset(B.wq, 0xff, size_t(B.wq_cap) * 4));
     HIPCK(hipMalloc(&B.ctl, dsx::CTL_WORDS * 4));
     S->job_cap = 1u << 20;
-    HIPCK(hipMalloc(&S->d_jobs, size_t(S->job_cap) * sizeof(dsx::FloodJob)));
+    HIPCK(hipMalloc(&S->d_up, size_t(S->job_cap) * (sizeof(dsx::FloodJob) + 8)));
     HIPCK(hipMalloc(&S->d_res, size_t(S->job_cap) * sizeof(dsx::FloodRes)));
     S->mask_cap = 1u << 20;
     HIPCK(hipMalloc(&S->d_masks, size_t(S->mask_cap) * sizeof(dsx::TileMask)));
-    HIPCK(hipMalloc(&S->d_jl, size_t(S->job_cap) * 8));
+    HIPCK(hipHostMalloc(&S->h_ctl, dsx::CTL_WORDS * 4, hipHostMallocDefault));
     return DSX_OK;
 }
 
+// Pinned staging for a batch of n floods (grown between calls, when the stream is idle).
+static int flood_io_reserve(ClimState *S, uint64_t n) {
+    const size_t up = size_t(n) * (sizeof(dsx::FloodJob) + 8);
+    if (S->h_up_cap < up) {
+        if (S->h_up) HIPCK(hipHostFree(S->h_up));
+        S->h_up = nullptr;
+        S->h_up_cap = std::max<size_t>(up, size_t(4096) * (sizeof(dsx::FloodJob) + 8));
+        HIPCK(hipHostMalloc(&S->h_up, S->h_up_cap, hipHostMallocDefault));
+    }
+    if (S->h_res_cap < n) {
+        if (S->h_res) HIPCK(hipHostFree(S->h_res));
+        S->h_res = nullptr;
+        S->h_res_cap = uint32_t(std::max<uint64_t>(n, 4096));
+        HIPCK(hipHostMalloc(&S->h_res, size_t(S->h_res_cap) * sizeof(dsx::FloodRes), hipHostMallocDefault));
+    }
+    if (!S->h_masks) {
+        S->h_mask_cap = 1u << 15;
+        HIPCK(hipHostMalloc(&S->h_masks, size_t(S->h_mask_cap) * sizeof(dsx::TileMask), hipHostMallocDefault));
+    }
+    return DSX_OK;
+}
+
+// Device I/O buffer and its pinned host twin (one upload and one download per call).
 static int io_reserve(dsx_ctx *c, ClimState *S, size_t bytes) {
     if (S->io_cap >= bytes) return DSX_OK;
     if (S->d_io) HIPCK(hipFree(S->d_io));
+    if (S->h_io) HIPCK(hipHostFree(S->h_io));
     S->d_io = nullptr;
+    S->h_io = nullptr;
     S->io_cap = std::max<size_t>(bytes, 1 << 20);
     HIPCK(hipMalloc(&S->d_io, S->io_cap));
+    HIPCK(hipHostMalloc(&S->h_io, S->io_cap, hipHostMallocDefault));
     return DSX_OK;
 }
 
@@ -215,6 +250,7 @@ static void stats_add(const dsx_flood_stats &a) {
     for (int k = 0; k < 4; k++) {
         g_stats.floods[k] += a.floods[k];
         g_stats.tiles[k] += a.tiles[k];
+        g_stats.evaluated[k] += a.evaluated[k];
         g_stats.sampled_cold[k] += a.sampled_cold[k];
         g_stats.sampled_hot[k] += a.sampled_hot[k];
     }
@@ -250,9 +286,11 @@ void dsx_ctx_free_ext(dsx_ctx *c) {
         g_ext.erase(it);
     }
     for (void *p : {(void *)s->d_cuts, (void *)s->d_cell, (void *)s->d_clim, (void *)s->d_useeds, (void *)s->fb.tiles,
-                    (void *)s->fb.ntiles, (void *)s->fb.hkeys, (void *)s->fb.wq, (void *)s->fb.ctl, (void *)s->d_jobs,
-                    (void *)s->d_res, s->d_io, (void *)s->d_masks, (void *)s->d_jl})
+                    (void *)s->fb.ntiles, (void *)s->fb.hkeys, (void *)s->fb.wq, (void *)s->fb.ctl, (void *)s->d_up,
+                    (void *)s->d_res, s->d_io, (void *)s->d_masks})
         if (p) (void)hipFree(p);
+    for (void *p : {(void *)s->h_up, (void *)s->h_res, (void *)s->h_ctl, (void *)s->h_masks, s->h_io})
+        if (p) (void)hipHostFree(p);
     for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
     delete s;
 }
@@ -335,79 +373,57 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         dsx::g_last_error = "too many floods in one call";
         return DSX_E_CAPACITY;
     }
-    // distinct seeds -> climate slots
+    if ((rc = flood_io_reserve(S, n))) return rc;
+    // jobs (seed -> climate slot) and job lists straight into pinned staging, in the device layout
+    dsx::FloodJob *hj = reinterpret_cast<dsx::FloodJob *>(S->h_up);
+    uint32_t *h_small = reinterpret_cast<uint32_t *>(hj + n), *h_big = h_small + n;
     std::vector<uint64_t> useeds;
     std::unordered_map<uint64_t, int32_t> slot;
-    std::vector<dsx::FloodJob> hj(n);
     for (uint64_t k = 0; k < n; k++) {
-        auto it = slot.find(jobs[k].seed);
-        int32_t s;
-        if (it == slot.end()) {
-            s = int32_t(useeds.size());
-            slot.emplace(jobs[k].seed, s);
-            useeds.push_back(jobs[k].seed);
-        } else {
-            s = it->second;
-        }
-        hj[k] = dsx::FloodJob{s, jobs[k].kind, jobs[k].cut, jobs[k].qx, jobs[k].qz, jobs[k].step, jobs[k].half,
+        auto [it, fresh] = slot.try_emplace(jobs[k].seed, int32_t(useeds.size()));
+        if (fresh) useeds.push_back(jobs[k].seed);
+        hj[k] = dsx::FloodJob{it->second, jobs[k].kind, jobs[k].cut, jobs[k].qx, jobs[k].qz, jobs[k].step, jobs[k].half,
                               jobs[k].conn8};
     }
     std::vector<uint32_t> uslot(useeds.size());
     if ((rc = clim_slots(c, S, useeds.data(), uint32_t(useeds.size()), uslot.data()))) return rc;
     for (uint64_t k = 0; k < n; k++) hj[k].seed = int32_t(uslot[hj[k].seed]);
-    dsx::FloodBufs &B = S->fb;
-    HIPCK(hipMemcpyAsync(S->d_jobs, hj.data(), n * sizeof(dsx::FloodJob), hipMemcpyHostToDevice, c->st));
-    HIPCK(hipMemsetAsync(B.ntiles, 0, 4, c->st));
-    HIPCK(hipMemsetAsync(B.ctl, 0, dsx::CTL_WORDS * 4, c->st));
-    dsx::launch_flood_resinit(c->st, S->d_jobs, nullptr, uint32_t(n), S->d_res);
     // The one-wave LDS engine for every flood whose window fits it whole and, with DSX_LDS_ROUTE=1,
     // first for hot-zone and proxy floods of any window (on a local sub-window; exact work, but one
-    // BFS layer per step, which measured slower in the pipeline than the tiled engine's 32-cell
-    // bit-parallel rounds); the rest, and the floods that outgrow the sub-window, on the tiled engine.
+    // BFS layer per step, which measured slower in the pipeline than the tiled engine); the rest, and
+    // the floods that outgrow the sub-window (handed over on the device), on the tiled engine.
     static const bool lds_route = env_flag("DSX_LDS_ROUTE", false);
-    std::vector<uint32_t> jl_small, jl_big, by_cls[3];
-    for (uint64_t k = 0; k < n; k++) {
-        const bool fits = 2 * hj[k].half + 1 <= dsx::LDS_MAX_N;
-        if (hj[k].kind == dsx::PRED_HOT && (fits || lds_route)) by_cls[dsx::LDS_HOT].push_back(uint32_t(k));
-        else if (hj[k].kind == dsx::PRED_PROXY && (fits || lds_route)) by_cls[dsx::LDS_PROXY].push_back(uint32_t(k));
-        else if (fits) by_cls[dsx::LDS_FULL].push_back(uint32_t(k));
-        else jl_big.push_back(uint32_t(k));
-    }
-    uint32_t n_cls[3];
+    uint32_t n_cls[3] = {0, 0, 0}, n_small = 0, n_big = 0;
+    auto cls_of = [&](const dsx::FloodJob &j) {
+        const bool fits = 2 * j.half + 1 <= dsx::LDS_MAX_N;
+        if (j.kind == dsx::PRED_HOT && (fits || lds_route)) return int(dsx::LDS_HOT);
+        if (j.kind == dsx::PRED_PROXY && (fits || lds_route)) return int(dsx::LDS_PROXY);
+        return fits ? int(dsx::LDS_FULL) : -1;
+    };
+    for (int cls = 0; cls < 3; cls++)
+        for (uint64_t k = 0; k < n; k++)
+            if (cls_of(hj[k]) == cls) {
+                h_small[n_small++] = uint32_t(k);
+                n_cls[cls]++;
+            }
+    for (uint64_t k = 0; k < n; k++)
+        if (cls_of(hj[k]) < 0) h_big[n_big++] = uint32_t(k);
+    uint8_t *d_up = S->d_up;
+    const dsx::FloodJob *d_jobs = reinterpret_cast<const dsx::FloodJob *>(d_up);
+    uint32_t *d_small = reinterpret_cast<uint32_t *>(d_up + n * sizeof(dsx::FloodJob)), *d_big = d_small + n;
+    dsx::FloodBufs &B = S->fb;
+    HIPCK(hipMemcpyAsync(d_up, S->h_up, n * sizeof(dsx::FloodJob) + (size_t(n) + n_big) * 4, hipMemcpyHostToDevice, c->st));
+    dsx::launch_flood_prep(c->st, d_jobs, uint32_t(n), S->d_res, B, n_big);
+    uint32_t off = 0;
     for (int cls = 0; cls < 3; cls++) {
-        n_cls[cls] = uint32_t(by_cls[cls].size());
-        jl_small.insert(jl_small.end(), by_cls[cls].begin(), by_cls[cls].end());
+        dsx::launch_flood_lds(c->st, cls, d_jobs, d_small + off, n_cls[cls], S->d_clim, S->grid, S->d_res, S->d_masks,
+                              S->mask_cap, d_big, B.ctl);
+        off += n_cls[cls];
     }
-    uint32_t *d_small = S->d_jl, *d_big = d_small + S->job_cap;
-    if (!jl_small.empty()) {
-        HIPCK(hipMemcpyAsync(d_small, jl_small.data(), jl_small.size() * 4, hipMemcpyHostToDevice, c->st));
-        uint32_t off = 0;
-        for (int cls = 0; cls < 3; cls++) {
-            dsx::launch_flood_lds(c->st, cls, S->d_jobs, d_small + off, n_cls[cls], S->d_clim, S->grid, S->d_res,
-                                  S->d_masks, B.ctl + dsx::FLOOD_MASKS, S->mask_cap);
-            off += n_cls[cls];
-        }
-    }
-    HIPCK(hipGetLastError());
-    std::vector<dsx::FloodRes> res(n);
-    if (!jl_small.empty()) {
-        // floods that outgrew the LDS engine go to the tiled engine
-        HIPCK(hipMemcpyAsync(res.data(), S->d_res, n * sizeof(dsx::FloodRes), hipMemcpyDeviceToHost, c->st));
-        HIPCK(hipStreamSynchronize(c->st));
-        for (uint32_t k : jl_small)
-            if (res[k].border < 0) jl_big.push_back(k);
-    }
-    if (!jl_big.empty()) {
-        HIPCK(hipMemcpyAsync(d_big, jl_big.data(), jl_big.size() * 4, hipMemcpyHostToDevice, c->st));
-        dsx::launch_flood_resinit(c->st, S->d_jobs, d_big, uint32_t(jl_big.size()), S->d_res);
-        dsx::launch_flood_start(c->st, S->d_jobs, d_big, uint32_t(jl_big.size()), B);
-        HIPCK(hipGetLastError());
-    }
+    // the tiled engine: the jobs listed now plus the small-window floods handed over on the device
     const bool prof = g_prof_on.load(std::memory_order_relaxed) != 0;
-    size_t nev = 0;   // events used by this call (a pair around the persistent kernel)
-    uint32_t *h = c->h_cnt;
-    int status = DSX_OK;
-    if (!jl_big.empty()) {
+    const bool tiled = n_big > 0 || n_small > 0;
+    if (tiled) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (prof) {
             while (S->events.size() < 2) {
@@ -417,89 +433,82 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
             }
             e0 = S->events[0];
             e1 = S->events[1];
-            nev = 2;
         }
-        // workgroups: enough for the tiles in flight (every flood's frontier), at most a full chip
-        static const uint32_t per_job = uint32_t(env_u32("DSX_FLOOD_BLOCKS_PER_JOB", 2)),
-                              max_blocks = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS", 256));
-        const uint32_t blocks = uint32_t(std::clamp<size_t>(per_job * jl_big.size(), 16, max_blocks));
-        dsx::launch_flood_run(c->st, S->d_jobs, S->d_clim, S->grid, B, blocks, e0, e1);
-        HIPCK(hipGetLastError());
+        dsx::launch_flood_start(c->st, d_jobs, d_big, uint32_t(n), B);
+        // workgroups: enough for the tiles in flight (every flood's frontier)
+        static const uint32_t per_job = uint32_t(env_u32("DSX_FLOOD_BLOCKS_PER_JOB", 4)),
+                              max_blocks = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS", 128));
+        const uint32_t blocks = uint32_t(std::clamp<size_t>(size_t(per_job) * n_big, 16, max_blocks));
+        dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, e0, e1);
+        dsx::launch_flood_acc(c->st, d_jobs, B, S->d_res, S->d_masks, S->mask_cap, n_big ? 1024 : 64);
     }
-    HIPCK(hipMemcpyAsync(h, B.ctl, dsx::CTL_WORDS * 4, hipMemcpyDeviceToHost, c->st));
-    uint32_t ntiles = 0;
-    HIPCK(hipMemcpyAsync(&ntiles, B.ntiles, 4, hipMemcpyDeviceToHost, c->st));
+    dsx::launch_flood_publish(c->st, S->d_res, uint32_t(n), B, S->d_masks, S->h_res, S->h_ctl, S->h_masks, S->h_mask_cap);
+    HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(c->st));
-    ntiles = std::min(ntiles, B.tile_cap);
-    const uint32_t qtail = std::min(h[dsx::Q_TAIL], B.wq_cap), items = h[dsx::Q_ITEMS];
+    const uint32_t *h = S->h_ctl;
+    const dsx::FloodRes *res = S->h_res;
+    int status = DSX_OK;
     if (h[dsx::FLOOD_OVF]) {
+        // acc ran on an incomplete flood state: results are void, and the tiles, hash slots and queue
+        // slots this call used are reset wholesale
         status = DSX_E_CAPACITY;
+        dsx::g_last_error = "flood tile pool or work queue exhausted: split the batch";
         if (h[dsx::FLOOD_OVF] & dsx::OVF_TIME) {
             status = DSX_E_HIP;
             dsx::g_last_error = "flood engine time limit reached";
         }
-    }
-    if (status == DSX_OK) {
-        dsx::launch_flood_acc(c->st, S->d_jobs, B, ntiles, S->d_res, S->d_masks, B.ctl + dsx::FLOOD_MASKS, S->mask_cap);
-        HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(res.data(), S->d_res, n * sizeof(dsx::FloodRes), hipMemcpyDeviceToHost, c->st));
-        HIPCK(hipMemcpyAsync(h + dsx::FLOOD_MASKS, B.ctl + dsx::FLOOD_MASKS, 4, hipMemcpyDeviceToHost, c->st));
-        HIPCK(hipStreamSynchronize(c->st));
-        const uint32_t nm = h[dsx::FLOOD_MASKS];
-        if (nm > S->mask_cap) {
-            status = DSX_E_CAPACITY;
-            dsx::g_last_error = "flood mask export overflow";
-        }
-        S->masks.resize(std::min(nm, S->mask_cap));
-        if (!S->masks.empty()) {
-            HIPCK(hipMemcpyAsync(S->masks.data(), S->d_masks, S->masks.size() * sizeof(dsx::TileMask),
-                                 hipMemcpyDeviceToHost, c->st));
-            HIPCK(hipStreamSynchronize(c->st));
-        }
-        dsx_flood_stats st{};
-        st.calls = 1;
-        st.rounds = items;
-        std::vector<char> big(n, 0);
-        for (uint32_t k : jl_big) big[k] = 1;
-        for (uint64_t k = 0; k < n; k++) {
-            const int kind = hj[k].kind;
-            if (big[k]) {
-                st.floods[kind]++;
-                st.tiles[kind] += res[k].tiles;
-                st.sampled_hot[kind] += res[k].hot;
-                st.sampled_cold[kind] += res[k].sampled - res[k].hot;
-                int b = 0;
-                while (b < 5 && res[k].tiles > (1ull << (2 * b))) b++;
-                st.size_floods[kind][b]++;
-                st.size_tiles[kind][b] += res[k].tiles;
-                st.size_sampled[kind][b] += res[k].sampled;
-            } else {
-                st.small_floods++;
-                st.small_sampled += res[k].sampled;
-            }
-        }
-        for (size_t e = 0; e < nev; e += 2) {
-            float ms = 0;
-            HIPCK(hipEventElapsedTime(&ms, S->events[e], S->events[e + 1]));
-            st.eval_ms += ms;
-            st.eval_launches++;
-        }
-        stats_add(st);
-        for (uint64_t k = 0; k < n; k++)
-            out[k] = dsx_area{res[k].cells, res[k].sampled, res[k].minx, res[k].maxx, res[k].minz, res[k].maxz,
-                              res[k].edge, res[k].border};
-    } else if (status == DSX_E_CAPACITY) {
-        dsx::g_last_error = "flood tile pool or work queue exhausted: split the batch";
-    }
-    // k_flood_acc has returned every tile and hash slot of this call to the clean state; after an
-    // overflow (acc not run) reset them wholesale
-    if (status != DSX_OK) {
+        const uint32_t ntiles = std::min(h[dsx::CTL_NTILES], B.tile_cap), qtail = std::min(h[dsx::Q_TAIL], B.wq_cap);
         HIPCK(hipMemsetAsync(B.tiles, 0, size_t(ntiles) * sizeof(dsx::Tile), c->st));
         HIPCK(hipMemsetAsync(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8, c->st));
-        if (qtail) HIPCK(hipMemsetAsync(B.wq, 0xff, size_t(qtail) * 4, c->st));   // slots left unconsumed
+        if (qtail) HIPCK(hipMemsetAsync(B.wq, 0xff, size_t(qtail) * 4, c->st));
+        HIPCK(hipStreamSynchronize(c->st));
+        return status;
     }
-    HIPCK(hipStreamSynchronize(c->st));
-    return status;
+    const uint32_t nm = h[dsx::FLOOD_MASKS];
+    if (nm > S->mask_cap) {
+        dsx::g_last_error = "flood mask export overflow";
+        return DSX_E_CAPACITY;
+    }
+    S->masks.resize(nm);
+    const uint32_t nh = std::min(nm, S->h_mask_cap);
+    if (nh) std::memcpy(S->masks.data(), S->h_masks, size_t(nh) * sizeof(dsx::TileMask));
+    if (nm > nh) {   // more masks than the pinned buffer holds: the rest by copy
+        HIPCK(hipMemcpyAsync(S->masks.data() + nh, S->d_masks + nh, size_t(nm - nh) * sizeof(dsx::TileMask),
+                             hipMemcpyDeviceToHost, c->st));
+        HIPCK(hipStreamSynchronize(c->st));
+    }
+    dsx_flood_stats st{};
+    st.calls = 1;
+    st.rounds = h[dsx::Q_ITEMS];
+    for (uint64_t k = 0; k < n; k++) {
+        const int kind = hj[k].kind;
+        if (res[k].tiles) {   // ran on the tiled engine
+            st.floods[kind]++;
+            st.tiles[kind] += res[k].tiles;
+            st.evaluated[kind] += res[k].evald;
+            st.sampled_hot[kind] += res[k].hot;
+            st.sampled_cold[kind] += res[k].sampled - res[k].hot;
+            int b = 0;
+            while (b < 5 && res[k].tiles > (1ull << (2 * b))) b++;
+            st.size_floods[kind][b]++;
+            st.size_tiles[kind][b] += res[k].tiles;
+            st.size_sampled[kind][b] += res[k].sampled;
+        } else {
+            st.small_floods++;
+            st.small_sampled += res[k].sampled;
+        }
+    }
+    if (prof && tiled) {
+        float ms = 0;
+        HIPCK(hipEventElapsedTime(&ms, S->events[0], S->events[1]));
+        st.eval_ms += ms;
+        st.eval_launches++;
+    }
+    stats_add(st);
+    for (uint64_t k = 0; k < n; k++)
+        out[k] = dsx_area{res[k].cells, res[k].sampled, res[k].minx, res[k].maxx, res[k].minz, res[k].maxz, res[k].edge,
+                          res[k].border};
+    return DSX_OK;
 }
 
 int dsx_profile(int on) {
@@ -546,18 +555,24 @@ int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_
     if ((rc = clim_slots(c, S, seeds, uint32_t(nseeds), sslot.data()))) return rc;
     for (uint64_t k = 0; k < n; k++) sidx2[k] = sslot[sidx[k]];
     if ((rc = io_reserve(c, S, n * 24))) return rc;
+    // inputs packed in pinned memory (one upload), results to their pinned twin (one download)
+    const size_t in_bytes = size_t(n) * 12 + (n & 1) * 4;
+    uint32_t *h_s = static_cast<uint32_t *>(S->h_io);
+    std::memcpy(h_s, sidx2.data(), n * 4);
+    std::memcpy(h_s + n, xs, n * 4);
+    std::memcpy(h_s + 2 * n, zs, n * 4);
     uint32_t *d_s = static_cast<uint32_t *>(S->d_io);
     int32_t *d_x = reinterpret_cast<int32_t *>(d_s + n), *d_z = d_x + n;
     double *d_d = reinterpret_cast<double *>(d_z + n + (n & 1));
     int8_t *d_8 = reinterpret_cast<int8_t *>(d_d);
-    HIPCK(hipMemcpyAsync(d_s, sidx2.data(), n * 4, hipMemcpyHostToDevice, c->st));
-    HIPCK(hipMemcpyAsync(d_x, xs, n * 4, hipMemcpyHostToDevice, c->st));
-    HIPCK(hipMemcpyAsync(d_z, zs, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCK(hipMemcpyAsync(d_s, h_s, in_bytes, hipMemcpyHostToDevice, c->st));
     dsx::launch_points(c->st, S->d_clim, S->grid, d_s, d_x, d_z, uint32_t(n), what, cut, d_8, d_d);
     HIPCK(hipGetLastError());
-    if (dbl) HIPCK(hipMemcpyAsync(outd, d_d, n * 8, hipMemcpyDeviceToHost, c->st));
-    else HIPCK(hipMemcpyAsync(out8, d_8, n, hipMemcpyDeviceToHost, c->st));
+    void *h_out = static_cast<uint8_t *>(S->h_io) + in_bytes;
+    HIPCK(hipMemcpyAsync(h_out, d_d, dbl ? n * 8 : n, hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
+    if (dbl) std::memcpy(outd, h_out, n * 8);
+    else std::memcpy(out8, h_out, n);
     return DSX_OK;
 }
 

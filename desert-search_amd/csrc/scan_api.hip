@@ -66,7 +66,10 @@ int scan_alloc(dsx_ctx *c) {
     HIPCK(hipMalloc(&c->d_bandmax, S * 6 * (512 / dsx::TAB_BAND) * sizeof(float)));
     c->act_cap = 1u << 21;
     HIPCK(hipMalloc(&c->d_act, size_t(c->act_cap) * 8));
-    HIPCK(hipMalloc(&c->d_ncand, S * 4));
+    HIPCK(hipMalloc(&c->d_ncand, (S + 1) * 4));   // per-seed candidate counts, then the active-cell count
+    HIPCK(hipHostMalloc(&c->h_seeds, S * 8, hipHostMallocDefault));
+    HIPCK(hipHostMalloc(&c->h_smax, S * 2 * 4, hipHostMallocDefault));
+    HIPCK(hipHostMalloc(&c->h_sn, (S + 1) * 4, hipHostMallocDefault));
     std::vector<int16_t> ibT(512 * dsx::NT_TRANS), ibC(512 * dsx::NT_TRANS), ibX(512 * dsx::NT_TRANS);
     std::vector<int8_t> loT(512), hiT(512), loC(512), hiC(512);
     trans_map(1024.0, ibT.data(), loT.data(), hiT.data());
@@ -87,7 +90,10 @@ int cand_alloc(dsx_ctx *c, uint64_t per_seed) {
     if (c->d_cand) HIPCK(hipFree(c->d_cand));
     c->d_cand = nullptr;
     c->cand_cap = std::max<uint64_t>(per_seed, 1u << 12);
-    HIPCK(hipMalloc(&c->d_cand, size_t(c->scan_seeds) * c->cand_cap * sizeof(dsx::ScanCand)));
+    HIPCK(hipMalloc(&c->d_cand, size_t(SCAN_CHUNK) * c->cand_cap * sizeof(dsx::ScanCand)));
+    if (c->h_cand) HIPCK(hipHostFree(c->h_cand));
+    c->h_cand = nullptr;
+    HIPCK(hipHostMalloc(&c->h_cand, size_t(SCAN_CHUNK) * c->cand_cap * sizeof(dsx::ScanCand), hipHostMallocDefault));
     return DSX_OK;
 }
 
@@ -107,8 +113,9 @@ int tables_chunk(dsx_ctx *c, const uint64_t *seeds, uint32_t m, const dsx_scan_p
     A.w[0] = octave0_weight(DSX_CH_T);
     A.w[1] = octave0_weight(DSX_CH_C);
     A.w[2] = octave0_weight(DSX_CH_E);
-    HIPCK(hipMemcpyAsync(c->d_seeds, seeds, size_t(m) * 8, hipMemcpyHostToDevice, c->st));
-    dsx::launch_octinit(c->st, c->d_seeds, m, c->d_tsp);
+    std::memcpy(c->h_seeds, seeds, size_t(m) * 8);
+    HIPCK(hipMemcpyAsync(c->d_seeds, c->h_seeds, size_t(m) * 8, hipMemcpyHostToDevice, c->st));
+    dsx::launch_octinit(c->st, c->d_seeds, m, c->d_tsp, c->d_ncand, m + 1);
     HIPCK(hipGetLastError());
     dsx::launch_tab(c->st, c->d_tsp, m, 0, 2, rT, A, c->d_tabs, c->d_bandmax);
     HIPCK(hipGetLastError());
@@ -142,22 +149,22 @@ int dsx_scan(dsx_ctx *c, const uint64_t *seeds, uint64_t n, const dsx_scan_param
     rc = cand_alloc(c, std::max<uint64_t>(cap, 1));
     if (rc) return rc;
     const dsx::ScanArgs A{p->ta, p->tt, p->wc, p->we};
-    const uint32_t NB = 512 / dsx::TAB_BAND;
-    std::vector<float> bmax(size_t(c->scan_seeds) * 6 * NB);
-    std::vector<uint32_t> cnt(c->scan_seeds);
     std::vector<dsx::ScanCand> buf;
     bool overflow = false;
+    const uint32_t kcap = uint32_t(std::min<uint64_t>(c->cand_cap, 0xffffffffu));
     for (uint64_t b = 0; b < n;) {
         uint32_t m = uint32_t(std::min<uint64_t>(c->scan_seeds, n - b));
-        rc = tables_chunk(c, seeds + b, m, p);
+        rc = tables_chunk(c, seeds + b, m, p);   // also clears the chunk's counters
         if (rc) return rc;
-        HIPCK(hipMemsetAsync(c->d_cnt, 0, 4, c->st));
-        HIPCK(hipMemsetAsync(c->d_ncand, 0, size_t(m) * 4, c->st));
-        dsx::launch_active(c->st, c->d_tabs, m, p->ta, c->d_act, c->d_cnt, c->act_cap);
+        uint32_t *d_nact = c->d_ncand + m;
+        dsx::launch_active(c->st, c->d_tabs, m, p->ta, c->d_act, d_nact, c->act_cap);
+        dsx::launch_scan(c->st, c->d_tabs, c->d_act, d_nact, c->act_cap, c->d_ibT, c->d_loT, c->d_hiT, c->d_ibC, A, c->d_cand,
+                         c->d_ncand, kcap);
+        dsx::launch_scan_publish(c->st, m, c->d_bandmax, d_nact, c->d_ncand, c->d_cand, kcap, kcap, c->h_smax, c->h_sn,
+                                 c->h_cand);
         HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(c->h_cnt, c->d_cnt, 4, hipMemcpyDeviceToHost, c->st));
         HIPCK(hipStreamSynchronize(c->st));
-        const uint32_t nact = c->h_cnt[0];
+        const uint32_t nact = c->h_sn[0];
         if (nact > c->act_cap) {            // too many active cells for one chunk: fewer seeds
             if (m == 1) {
                 dsx::g_last_error = "active-cell list overflow for a single seed";
@@ -166,29 +173,14 @@ int dsx_scan(dsx_ctx *c, const uint64_t *seeds, uint64_t n, const dsx_scan_param
             c->scan_seeds = std::max<uint32_t>(1, m / 2);
             continue;
         }
-        const uint32_t kcap = uint32_t(std::min<uint64_t>(c->cand_cap, 0xffffffffu));
-        dsx::launch_scan(c->st, c->d_tabs, c->d_act, nact, c->d_ibT, c->d_loT, c->d_hiT, c->d_ibC, A, c->d_cand,
-                         c->d_ncand, kcap);
-        HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(bmax.data(), c->d_bandmax, size_t(m) * 6 * NB * 4, hipMemcpyDeviceToHost, c->st));
-        HIPCK(hipMemcpyAsync(cnt.data(), c->d_ncand, size_t(m) * 4, hipMemcpyDeviceToHost, c->st));
-        HIPCK(hipStreamSynchronize(c->st));
         for (uint32_t k = 0; k < m; k++) {
-            float ma = -1e30f, mb = -1e30f;
-            for (uint32_t q = 0; q < NB; q++) {
-                ma = std::max(ma, bmax[(size_t(k) * 6 + 0) * NB + q]);
-                mb = std::max(mb, bmax[(size_t(k) * 6 + 1) * NB + q]);
-            }
-            if (maxta) maxta[b + k] = ma;
-            if (maxtb) maxtb[b + k] = mb;
-            ncand[b + k] = cnt[k];
-            const uint32_t have = uint32_t(std::min<uint64_t>(cnt[k], kcap));
-            if (cnt[k] > cap) overflow = true;
-            buf.resize(have);
-            if (have) {
-                HIPCK(hipMemcpy(buf.data(), c->d_cand + size_t(k) * c->cand_cap, size_t(have) * sizeof(dsx::ScanCand),
-                                hipMemcpyDeviceToHost));
-            }
+            if (maxta) maxta[b + k] = c->h_smax[2 * k];
+            if (maxtb) maxtb[b + k] = c->h_smax[2 * k + 1];
+            const uint32_t nc = c->h_sn[1 + k];
+            ncand[b + k] = nc;
+            const uint32_t have = std::min(nc, kcap);
+            if (nc > cap) overflow = true;
+            buf.assign(c->h_cand + size_t(k) * kcap, c->h_cand + size_t(k) * kcap + have);
             std::sort(buf.begin(), buf.end(), [](const dsx::ScanCand &x, const dsx::ScanCand &y) { return x.pos < y.pos; });
             const uint64_t w = std::min<uint64_t>(have, cap);
             dsx_raw_cand *dst = cands + (b + k) * cap;

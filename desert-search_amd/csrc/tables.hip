@@ -19,11 +19,14 @@ namespace dsx {
 // Octave 0 of channel T/C/E (table t: channel t/2, side t&1) for every seed: one lane per table,
 // permutation shuffled in a lane-private LDS slot (same scheme as k_init in gate.hip).
 constexpr int PSTRIDE = 260;
-__global__ __launch_bounds__(64) void k_octinit(const uint64_t *seeds, uint32_t n, SideParams *out) {
+__global__ __launch_bounds__(64) void k_octinit(const uint64_t *seeds, uint32_t n, SideParams *out, uint32_t *zero,
+                                                 uint32_t nzero) {
     __shared__ uint32_t slot32[64 * PSTRIDE / 4];
     uint8_t *slots = reinterpret_cast<uint8_t *>(slot32);
     const int lane = threadIdx.x;
     const uint32_t g = blockIdx.x * 64u + lane;
+    if (blockIdx.x == 0 && zero)   // counters the later kernels of the chunk accumulate into
+        for (uint32_t k = lane; k < nzero; k += 64) zero[k] = 0;
     const bool active = g < 6u * n;
     uint8_t *my = slots + lane * PSTRIDE;
     double ox = 0, oy = 0, oz = 0;
@@ -117,11 +120,46 @@ __global__ __launch_bounds__(256) void k_active(const float *tabs, uint32_t n, f
 }
 
 // One wave per active cell; lane = x translation. Candidates (score >= tt) are appended per seed.
-__global__ __launch_bounds__(64) void k_scan(const float *tabs, const uint32_t *act, uint32_t nact, const int16_t *ibT,
-                                             const int8_t *loT, const int8_t *hiT, const int16_t *ibC, ScanArgs A,
-                                             ScanCand *cand, uint32_t *ncand, uint32_t cap) {
-    const uint32_t a = blockIdx.x;
-    if (a >= nact) return;
+__device__ __forceinline__ void scan_cell(const float *tabs, const uint32_t *act, uint32_t a, const int16_t *ibT,
+                                          const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A,
+                                          ScanCand *cand, uint32_t *ncand, uint32_t cap);
+// Grid-stride over the active cells counted on the device (*count, at most act_cap were listed).
+__global__ __launch_bounds__(64) void k_scan(const float *tabs, const uint32_t *act, const uint32_t *count,
+                                             uint32_t act_cap, const int16_t *ibT, const int8_t *loT, const int8_t *hiT,
+                                             const int16_t *ibC, ScanArgs A, ScanCand *cand, uint32_t *ncand,
+                                             uint32_t cap) {
+    const uint32_t nact = min(*count, act_cap);
+    for (uint32_t a = blockIdx.x; a < nact; a += gridDim.x) scan_cell(tabs, act, a, ibT, loT, hiT, ibC, A, cand, ncand, cap);
+}
+
+// Per seed of the chunk (one block each): max of the blurred TA / TB tables from their band maxima,
+// the candidate count, and up to pcap candidates, into host memory; out_n = the active-cell count.
+__global__ __launch_bounds__(64) void k_scan_publish(const float *bandmax, const uint32_t *count, const uint32_t *ncand,
+                                                     const ScanCand *cand, uint32_t cap, uint32_t pcap, float *h_max,
+                                                     uint32_t *h_n, ScanCand *h_cand) {
+    const uint32_t k = blockIdx.x;
+    const int lane = threadIdx.x;
+    constexpr int NB = 512 / TAB_BAND;
+    if (lane < 2) {
+        float m = -1e30f;
+        for (int q = 0; q < NB; q++) m = fmaxf(m, bandmax[(size_t(k) * 6 + lane) * NB + q]);
+        h_max[2 * k + lane] = m;
+    }
+    const uint32_t nc = ncand[k];
+    if (lane == 0) {
+        h_n[1 + k] = nc;
+        if (k == 0) h_n[0] = *count;
+    }
+    const uint32_t w = min(min(nc, cap), pcap);
+    constexpr uint32_t W = sizeof(ScanCand) / 4;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(cand + size_t(k) * cap);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(h_cand + size_t(k) * pcap);
+    for (uint32_t q = lane; q < w * W; q += 64) dst[q] = src[q];
+}
+
+__device__ __forceinline__ void scan_cell(const float *tabs, const uint32_t *act, uint32_t a, const int16_t *ibT,
+                                          const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A,
+                                          ScanCand *cand, uint32_t *ncand, uint32_t cap) {
     const uint32_t s = act[2 * a], cell = act[2 * a + 1];
     const int j = int(cell >> 9), i = int(cell & 511), lane = threadIdx.x;
     const float *TA = tabs + size_t(s) * 6 * 512 * 512;
@@ -169,8 +207,8 @@ __global__ __launch_bounds__(64) void k_scan(const float *tabs, const uint32_t *
 }
 
 // ------------------------------------------------------------------ launchers
-void launch_octinit(hipStream_t st, const uint64_t *seeds, uint32_t n, SideParams *out) {
-    if (n) hipLaunchKernelGGL(k_octinit, dim3((6 * n + 63) / 64), dim3(64), 0, st, seeds, n, out);
+void launch_octinit(hipStream_t st, const uint64_t *seeds, uint32_t n, SideParams *out, uint32_t *zero, uint32_t nzero) {
+    if (n) hipLaunchKernelGGL(k_octinit, dim3((6 * n + 63) / 64), dim3(64), 0, st, seeds, n, out, zero, nzero);
 }
 template <int R>
 static void tab_r(hipStream_t st, const SideParams *sp, uint32_t n, int t0, int nt, const TabArgs &A, float *tabs,
@@ -192,11 +230,17 @@ bool launch_tab(hipStream_t st, const SideParams *sp, uint32_t n, int t0, int nt
 void launch_active(hipStream_t st, const float *tabs, uint32_t n, float ta, uint32_t *act, uint32_t *count, uint32_t cap) {
     if (n) hipLaunchKernelGGL(k_active, dim3(512 * 512 / 256, n), dim3(256), 0, st, tabs, n, ta, act, count, cap);
 }
-void launch_scan(hipStream_t st, const float *tabs, const uint32_t *act, uint32_t nact, const int16_t *ibT,
-                 const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A, ScanCand *cand,
-                 uint32_t *ncand, uint32_t cap) {
-    if (nact)
-        hipLaunchKernelGGL(k_scan, dim3(nact), dim3(64), 0, st, tabs, act, nact, ibT, loT, hiT, ibC, A, cand, ncand, cap);
+void launch_scan(hipStream_t st, const float *tabs, const uint32_t *act, const uint32_t *count, uint32_t act_cap,
+                 const int16_t *ibT, const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A,
+                 ScanCand *cand, uint32_t *ncand, uint32_t cap) {
+    hipLaunchKernelGGL(k_scan, dim3(8192), dim3(64), 0, st, tabs, act, count, act_cap, ibT, loT, hiT, ibC, A, cand, ncand,
+                       cap);
+}
+void launch_scan_publish(hipStream_t st, uint32_t m, const float *bandmax, const uint32_t *count, const uint32_t *ncand,
+                         const ScanCand *cand, uint32_t cap, uint32_t pcap, float *h_max, uint32_t *h_n, ScanCand *h_cand) {
+    if (m)
+        hipLaunchKernelGGL(k_scan_publish, dim3(m), dim3(64), 0, st, bandmax, count, ncand, cand, cap, pcap, h_max, h_n,
+                           h_cand);
 }
 
 }  // namespace dsx

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdsx.so")
 
 DSX_OK, DSX_E_ARG, DSX_E_HIP, DSX_E_CAPACITY, DSX_E_NODEVICE, DSX_E_BUSY = 0, 1, 2, 3, 4, 5
-HW_QUEUES = 16   # hardware queues requested from the HIP runtime: one per default pipeline worker
+HW_QUEUES = 24   # hardware queues requested from the HIP runtime (16 pipeline workers, the gate, spares)
 
 
 class DsxError(RuntimeError):
@@ -103,7 +103,7 @@ def flood_jobs(rows):
 class FloodStats(ctypes.Structure):
     """dsx_flood_stats: process-wide flood-engine counters (per kind index DSX_PRED_*)."""
     _fields_ = [("calls", ctypes.c_uint64), ("rounds", ctypes.c_uint64), ("floods", ctypes.c_uint64 * 4),
-                ("tiles", ctypes.c_uint64 * 4), ("sampled_cold", ctypes.c_uint64 * 4),
+                ("tiles", ctypes.c_uint64 * 4), ("evaluated", ctypes.c_uint64 * 4), ("sampled_cold", ctypes.c_uint64 * 4),
                 ("sampled_hot", ctypes.c_uint64 * 4), ("small_floods", ctypes.c_uint64),
                 ("small_sampled", ctypes.c_uint64), ("eval_ms", ctypes.c_double), ("eval_launches", ctypes.c_uint64),
                 ("size_floods", (ctypes.c_uint64 * 6) * 4), ("size_tiles", (ctypes.c_uint64 * 6) * 4),
@@ -128,9 +128,9 @@ def load(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise DsxError(DSX_E_NODEVICE, "libdsx.so not built", path)
-    # one hardware queue per pipeline worker stream (read by the HIP runtime when it starts; a value
-    # the caller set is kept)
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(HW_QUEUES))
+    # hardware queues for the pipeline worker streams, the gate and spares, read by the HIP runtime when it starts
+    # (DSX_HW_QUEUES overrides the count)
+    os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("DSX_HW_QUEUES", str(HW_QUEUES))
     lib = ctypes.CDLL(path)
     lib.dsx_strerror.restype = ctypes.c_char_p
     lib.dsx_strerror.argtypes = [ctypes.c_int]

@@ -19,9 +19,10 @@ static volatile sig_atomic_t interrupted = 0;
 static void on_signal(int) { interrupted = 1; }
 
 int main(int argc, char **argv) {
-    // one hardware queue per pipeline worker stream; read by the HIP runtime when it starts (a value
-    // set by the caller is kept)
-    setenv("GPU_MAX_HW_QUEUES", "16", 0);
+    // hardware queues for the pipeline worker streams, the gate and spares, read by the HIP runtime when it starts
+    // (DSX_HW_QUEUES overrides the count)
+    const char *hwq = std::getenv("DSX_HW_QUEUES");
+    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "24", 1);
     if (argc < 4) {
         std::fprintf(stderr, "dsearch_desert START COUNT OUT_PREFIX [threads] [key=value...]\n");
         return 2;

@@ -181,7 +181,8 @@ int dsx_flood_masks(dsx_ctx *ctx, dsx_tile_mask *out, uint64_t cap, uint64_t *n_
 typedef struct {
     uint64_t calls, rounds;        /* dsx_flood calls; tile work items the tiled engine processed */
     uint64_t floods[4];            /* floods run by the tiled engine, by kind (DSX_PRED_*) */
-    uint64_t tiles[4];             /* 32x32 tiles it evaluated, by kind (x 1024 = cells evaluated) */
+    uint64_t tiles[4];             /* 32x32 tiles it allocated, by kind */
+    uint64_t evaluated[4];         /* cells whose predicate it evaluated, by kind */
     uint64_t sampled_cold[4];      /* sampled cells of those floods, by kind and cost class */
     uint64_t sampled_hot[4];
     uint64_t small_floods, small_sampled; /* floods of the small-window engine and their sampled cells */
