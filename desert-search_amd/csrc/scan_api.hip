@@ -66,10 +66,10 @@ int scan_alloc(dsx_ctx *c) {
     HIPCK(hipMalloc(&c->d_bandmax, S * 6 * (512 / dsx::TAB_BAND) * sizeof(float)));
     c->act_cap = 1u << 21;
     HIPCK(hipMalloc(&c->d_act, size_t(c->act_cap) * 8));
-    HIPCK(hipMalloc(&c->d_ncand, (S + 1) * 4));   // per-seed candidate counts, then the active-cell count
+    HIPCK(hipMalloc(&c->d_ncand, 2 * S * 4));   // per-seed candidate counts, then per-seed active-cell counts
     HIPCK(hipHostMalloc(&c->h_seeds, S * 8, hipHostMallocDefault));
     HIPCK(hipHostMalloc(&c->h_smax, S * 2 * 4, hipHostMallocDefault));
-    HIPCK(hipHostMalloc(&c->h_sn, (S + 1) * 4, hipHostMallocDefault));
+    HIPCK(hipHostMalloc(&c->h_sn, (2 * S + 1) * 4, hipHostMallocDefault));
     std::vector<int16_t> ibT(512 * dsx::NT_TRANS), ibC(512 * dsx::NT_TRANS), ibX(512 * dsx::NT_TRANS);
     std::vector<int8_t> loT(512), hiT(512), loC(512), hiC(512);
     trans_map(1024.0, ibT.data(), loT.data(), hiT.data());
@@ -115,7 +115,7 @@ int tables_chunk(dsx_ctx *c, const uint64_t *seeds, uint32_t m, const dsx_scan_p
     A.w[2] = octave0_weight(DSX_CH_E);
     std::memcpy(c->h_seeds, seeds, size_t(m) * 8);
     HIPCK(hipMemcpyAsync(c->d_seeds, c->h_seeds, size_t(m) * 8, hipMemcpyHostToDevice, c->st));
-    dsx::launch_octinit(c->st, c->d_seeds, m, c->d_tsp, c->d_ncand, m + 1);
+    dsx::launch_octinit(c->st, c->d_seeds, m, c->d_tsp, c->d_ncand, 2 * m);
     HIPCK(hipGetLastError());
     dsx::launch_tab(c->st, c->d_tsp, m, 0, 2, rT, A, c->d_tabs, c->d_bandmax);
     HIPCK(hipGetLastError());
@@ -157,15 +157,17 @@ int dsx_scan(dsx_ctx *c, const uint64_t *seeds, uint64_t n, const dsx_scan_param
         rc = tables_chunk(c, seeds + b, m, p);   // also clears the chunk's counters
         if (rc) return rc;
         uint32_t *d_nact = c->d_ncand + m;
-        dsx::launch_active(c->st, c->d_tabs, m, p->ta, c->d_act, d_nact, c->act_cap);
-        dsx::launch_scan(c->st, c->d_tabs, c->d_act, d_nact, c->act_cap, c->d_ibT, c->d_loT, c->d_hiT, c->d_ibC, A, c->d_cand,
+        const uint32_t seg = c->act_cap / m;   // active-cell segment of each seed
+        dsx::launch_active(c->st, c->d_tabs, m, p->ta, c->d_act, d_nact, seg);
+        dsx::launch_scan(c->st, c->d_tabs, c->d_act, d_nact, m, seg, c->d_ibT, c->d_loT, c->d_hiT, c->d_ibC, A, c->d_cand,
                          c->d_ncand, kcap);
         dsx::launch_scan_publish(c->st, m, c->d_bandmax, d_nact, c->d_ncand, c->d_cand, kcap, kcap, c->h_smax, c->h_sn,
                                  c->h_cand);
         HIPCK(hipGetLastError());
         HIPCK(hipStreamSynchronize(c->st));
-        const uint32_t nact = c->h_sn[0];
-        if (nact > c->act_cap) {            // too many active cells for one chunk: fewer seeds
+        uint32_t nact = 0;
+        for (uint32_t k = 0; k < m; k++) nact = std::max(nact, c->h_sn[1 + m + k]);
+        if (nact > seg) {                   // a seed's active cells overflow its segment: fewer seeds
             if (m == 1) {
                 dsx::g_last_error = "active-cell list overflow for a single seed";
                 return DSX_E_CAPACITY;

@@ -29,13 +29,14 @@ void launch_octinit(hipStream_t st, const uint64_t *seeds, uint32_t n, SideParam
                     uint32_t nzero = 0);
 bool launch_tab(hipStream_t st, const SideParams *sp, uint32_t n, int t0, int nt, int R, const TabArgs &A, float *tabs,
                 float *bandmax);
-void launch_active(hipStream_t st, const float *tabs, uint32_t n, float ta, uint32_t *act, uint32_t *count, uint32_t cap);
-// grid-stride over the *count active cells (at most act_cap listed)
-void launch_scan(hipStream_t st, const float *tabs, const uint32_t *act, const uint32_t *count, uint32_t act_cap,
+// active cells of seed s to act[s * seg ..], counted in count[s]
+void launch_active(hipStream_t st, const float *tabs, uint32_t n, float ta, uint32_t *act, uint32_t *count, uint32_t seg);
+// the active cells of the m seeds (XCD x scans seeds x, x + 8, ...)
+void launch_scan(hipStream_t st, const float *tabs, const uint32_t *act, const uint32_t *count, uint32_t m, uint32_t seg,
                  const int16_t *ibT, const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A,
                  ScanCand *cand, uint32_t *ncand, uint32_t cap);
-// per seed: h_max[2k..2k+1] = max TA, max TB; h_n[0] = active cells, h_n[1 + k] = candidates; up to pcap
-// candidates to h_cand[k * pcap ..]
+// per seed k of m: h_max[2k..2k+1] = max TA, max TB; h_n[1 + k] = candidates, h_n[1 + m + k] = active
+// cells; up to pcap candidates to h_cand[k * pcap ..]
 void launch_scan_publish(hipStream_t st, uint32_t m, const float *bandmax, const uint32_t *count, const uint32_t *ncand,
                          const ScanCand *cand, uint32_t cap, uint32_t pcap, float *h_max, uint32_t *h_n, ScanCand *h_cand);
 

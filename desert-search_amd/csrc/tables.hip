@@ -99,8 +99,10 @@ __global__ __launch_bounds__(64) void k_tab(const SideParams *sp, uint32_t n, in
 }
 
 // Side-A cells with TA >= ta, as (seed, cell) pairs; one atomic per wave.
+// Active cells (TA >= ta) of seed s = blockIdx.y into its own segment act[s * seg ..], counted in
+// count[s] (cells past the segment are counted, not stored).
 __global__ __launch_bounds__(256) void k_active(const float *tabs, uint32_t n, float ta, uint32_t *act, uint32_t *count,
-                                                uint32_t cap) {
+                                                uint32_t seg) {
     const uint32_t s = blockIdx.y;
     const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
     const bool on = s < n && tabs[size_t(s) * 6 * 512 * 512 + cell] >= ta;
@@ -108,28 +110,31 @@ __global__ __launch_bounds__(256) void k_active(const float *tabs, uint32_t n, f
     if (mask == 0) return;
     const int lane = threadIdx.x & 63;
     uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(count, uint32_t(__popcll(mask)));
+    if (lane == 0) base = atomicAdd(&count[s], uint32_t(__popcll(mask)));
     base = __shfl(base, 0);
     if (on) {
         const uint32_t k = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-        if (k < cap) {
-            act[2 * k] = s;
-            act[2 * k + 1] = cell;
-        }
+        if (k < seg) act[size_t(s) * seg + k] = cell;
     }
 }
 
-// One wave per active cell; lane = x translation. Candidates (score >= tt) are appended per seed.
-__device__ __forceinline__ void scan_cell(const float *tabs, const uint32_t *act, uint32_t a, const int16_t *ibT,
+__device__ __forceinline__ void scan_cell(const float *tabs, uint32_t s, uint32_t cell, const int16_t *ibT,
                                           const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A,
                                           ScanCand *cand, uint32_t *ncand, uint32_t cap);
-// Grid-stride over the active cells counted on the device (*count, at most act_cap were listed).
-__global__ __launch_bounds__(64) void k_scan(const float *tabs, const uint32_t *act, const uint32_t *count,
-                                             uint32_t act_cap, const int16_t *ibT, const int8_t *loT, const int8_t *hiT,
+// XCD-aware: workgroups are dispatched to the 8 XCDs round-robin (XCD = blockIdx.x % 8), and the
+// workgroups of XCD x scan seeds x, x + 8, ... one after another, striding over each seed's active
+// cells, so each XCD's L2 holds the B-side tables (TB, CB, EB: 3 MB) of one seed at a time instead of
+// the gathers of all seeds of the chunk missing to memory.
+__global__ __launch_bounds__(64) void k_scan(const float *tabs, const uint32_t *act, const uint32_t *count, uint32_t m,
+                                             uint32_t seg, const int16_t *ibT, const int8_t *loT, const int8_t *hiT,
                                              const int16_t *ibC, ScanArgs A, ScanCand *cand, uint32_t *ncand,
                                              uint32_t cap) {
-    const uint32_t nact = min(*count, act_cap);
-    for (uint32_t a = blockIdx.x; a < nact; a += gridDim.x) scan_cell(tabs, act, a, ibT, loT, hiT, ibC, A, cand, ncand, cap);
+    constexpr uint32_t NXCD = 8;
+    const uint32_t xcd = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, per = gridDim.x / NXCD;
+    for (uint32_t s = xcd; s < m; s += NXCD) {
+        const uint32_t na = min(count[s], seg);
+        for (uint32_t a = lb; a < na; a += per) scan_cell(tabs, s, act[size_t(s) * seg + a], ibT, loT, hiT, ibC, A, cand, ncand, cap);
+    }
 }
 
 // Per seed of the chunk (one block each): max of the blurred TA / TB tables from their band maxima,
@@ -137,6 +142,7 @@ __global__ __launch_bounds__(64) void k_scan(const float *tabs, const uint32_t *
 __global__ __launch_bounds__(64) void k_scan_publish(const float *bandmax, const uint32_t *count, const uint32_t *ncand,
                                                      const ScanCand *cand, uint32_t cap, uint32_t pcap, float *h_max,
                                                      uint32_t *h_n, ScanCand *h_cand) {
+    // h_n[1 + k] candidates of seed k; h_n[1 + m + k] its active cells (m = gridDim.x); h_n[0] unused
     const uint32_t k = blockIdx.x;
     const int lane = threadIdx.x;
     constexpr int NB = 512 / TAB_BAND;
@@ -148,7 +154,7 @@ __global__ __launch_bounds__(64) void k_scan_publish(const float *bandmax, const
     const uint32_t nc = ncand[k];
     if (lane == 0) {
         h_n[1 + k] = nc;
-        if (k == 0) h_n[0] = *count;
+        h_n[1 + gridDim.x + k] = count[k];
     }
     const uint32_t w = min(min(nc, cap), pcap);
     constexpr uint32_t W = sizeof(ScanCand) / 4;
@@ -157,10 +163,9 @@ __global__ __launch_bounds__(64) void k_scan_publish(const float *bandmax, const
     for (uint32_t q = lane; q < w * W; q += 64) dst[q] = src[q];
 }
 
-__device__ __forceinline__ void scan_cell(const float *tabs, const uint32_t *act, uint32_t a, const int16_t *ibT,
+__device__ __forceinline__ void scan_cell(const float *tabs, uint32_t s, uint32_t cell, const int16_t *ibT,
                                           const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A,
                                           ScanCand *cand, uint32_t *ncand, uint32_t cap) {
-    const uint32_t s = act[2 * a], cell = act[2 * a + 1];
     const int j = int(cell >> 9), i = int(cell & 511), lane = threadIdx.x;
     const float *TA = tabs + size_t(s) * 6 * 512 * 512;
     const float *TB = TA + 512 * 512, *CA = TB + 512 * 512, *CB = CA + 512 * 512, *EA = CB + 512 * 512,
@@ -227,14 +232,14 @@ bool launch_tab(hipStream_t st, const SideParams *sp, uint32_t n, int t0, int nt
     default: return false;
     }
 }
-void launch_active(hipStream_t st, const float *tabs, uint32_t n, float ta, uint32_t *act, uint32_t *count, uint32_t cap) {
-    if (n) hipLaunchKernelGGL(k_active, dim3(512 * 512 / 256, n), dim3(256), 0, st, tabs, n, ta, act, count, cap);
+void launch_active(hipStream_t st, const float *tabs, uint32_t n, float ta, uint32_t *act, uint32_t *count, uint32_t seg) {
+    if (n) hipLaunchKernelGGL(k_active, dim3(512 * 512 / 256, n), dim3(256), 0, st, tabs, n, ta, act, count, seg);
 }
-void launch_scan(hipStream_t st, const float *tabs, const uint32_t *act, const uint32_t *count, uint32_t act_cap,
+void launch_scan(hipStream_t st, const float *tabs, const uint32_t *act, const uint32_t *count, uint32_t m, uint32_t seg,
                  const int16_t *ibT, const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A,
                  ScanCand *cand, uint32_t *ncand, uint32_t cap) {
-    hipLaunchKernelGGL(k_scan, dim3(8192), dim3(64), 0, st, tabs, act, count, act_cap, ibT, loT, hiT, ibC, A, cand, ncand,
-                       cap);
+    if (m) hipLaunchKernelGGL(k_scan, dim3(8 * 256), dim3(64), 0, st, tabs, act, count, m, seg, ibT, loT, hiT, ibC, A, cand,
+                              ncand, cap);
 }
 void launch_scan_publish(hipStream_t st, uint32_t m, const float *bandmax, const uint32_t *count, const uint32_t *ncand,
                          const ScanCand *cand, uint32_t cap, uint32_t pcap, float *h_max, uint32_t *h_n, ScanCand *h_cand) {
