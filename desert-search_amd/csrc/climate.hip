@@ -127,6 +127,32 @@ __global__ __launch_bounds__(256) void k_points(const ClimSeed *cs, SurfGrid g, 
     else out8[k] = int8_t(predicate(what, C, g, xs[k], zs[k], cut));
 }
 
+// Hottest of the 7 x 7 points around each site (one wave per site, lane l < 49 = (dz + 3) * 7 + dx + 3):
+// T from global memory (L2) as k_points does; the wave keeps the largest T, the lowest index on ties,
+// which is the first maximum of the reference's dz-then-dx scan with a strict greater-than.
+__global__ __launch_bounds__(256) void k_hottest(const ClimSeed *cs, const uint32_t *sidx, const int32_t *xs,
+                                                 const int32_t *zs, uint32_t n, int8_t *out8, double *outd) {
+    const uint32_t site = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (site >= n) return;
+    const ClimSeed &C = cs[sidx[site]];
+    double t = -1e300;
+    int idx = lane;
+    if (lane < 49) t = noise_sample(C, DSX_CH_T, xs[site] + (lane % 7 - 3) * 256, 0, zs[site] + (lane / 7 - 3) * 256);
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ot = __shfl_xor(t, o);
+        const int oi = __shfl_xor(idx, o);
+        if (ot > t || (ot == t && oi < idx)) {
+            t = ot;
+            idx = oi;
+        }
+    }
+    if (lane == 0) {
+        outd[site] = t;
+        out8[site] = int8_t(idx);
+    }
+}
+
 void launch_climinit(hipStream_t st, const uint64_t *seeds, const uint32_t *slots, uint32_t n, ClimSeed *out) {
     if (n) hipLaunchKernelGGL(k_climinit, dim3((CLIM_NOCT * n + 63) / 64), dim3(64), 0, st, seeds, slots, n, out);
 }
@@ -138,7 +164,9 @@ void launch_climate(hipStream_t st, const ClimSeed *cs, const int32_t *xs, const
 }
 void launch_points(hipStream_t st, const ClimSeed *cs, const SurfGrid &g, const uint32_t *sidx, const int32_t *xs,
                    const int32_t *zs, uint32_t n, int what, float cut, int8_t *out8, double *outd) {
-    if (n) hipLaunchKernelGGL(k_points, dim3((n + 255) / 256), dim3(256), 0, st, cs, g, sidx, xs, zs, n, what, cut, out8, outd);
+    if (!n) return;
+    if (what == 18) hipLaunchKernelGGL(k_hottest, dim3((n + 3) / 4), dim3(256), 0, st, cs, sidx, xs, zs, n, out8, outd);
+    else hipLaunchKernelGGL(k_points, dim3((n + 255) / 256), dim3(256), 0, st, cs, g, sidx, xs, zs, n, what, cut, out8, outd);
 }
 void launch_pred(hipStream_t st, const ClimSeed *cs, const SurfGrid &g, int kind, float cut, const int32_t *xs,
                  const int32_t *zs, uint32_t n, int8_t *out) {

@@ -536,9 +536,10 @@ int dsx_flood_masks(dsx_ctx *c, dsx_tile_mask *out, uint64_t cap, uint64_t *n_ou
 
 int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_t *sidx, const int32_t *xs,
                const int32_t *zs, uint64_t n, int what, float cut, int8_t *out8, double *outd) {
-    const bool dbl = what == DSX_POINT_T || what == DSX_POINT_PROXYPROB;
-    if (!c || (n && (!sidx || !xs || !zs)) || (!dbl && (what < 0 || what > 3)) || (dbl ? (n && !outd) : (n && !out8)) ||
-        (nseeds && !seeds))
+    const bool hottest = what == DSX_POINT_HOTTEST;
+    const bool dbl = what == DSX_POINT_T || what == DSX_POINT_PROXYPROB || hottest;
+    if (!c || (n && (!sidx || !xs || !zs)) || (!dbl && (what < 0 || what > 3)) || (dbl && n && !outd) ||
+        ((!dbl || hottest) && n && !out8) || (nseeds && !seeds))
         return DSX_E_ARG;
     for (uint64_t k = 0; k < n; k++)
         if (sidx[k] >= nseeds) return DSX_E_ARG;
@@ -554,8 +555,9 @@ int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_
     std::vector<uint32_t> sslot(nseeds), sidx2(n);
     if ((rc = clim_slots(c, S, seeds, uint32_t(nseeds), sslot.data()))) return rc;
     for (uint64_t k = 0; k < n; k++) sidx2[k] = sslot[sidx[k]];
-    if ((rc = io_reserve(c, S, n * 24))) return rc;
-    // inputs packed in pinned memory (one upload), results to their pinned twin (one download)
+    if ((rc = io_reserve(c, S, n * 24 + 16))) return rc;
+    // inputs packed in pinned memory (one upload), results to their pinned twin (one download):
+    // [sidx | xs | zs | pad] then the doubles (or bytes), then for HOTTEST the index bytes
     const size_t in_bytes = size_t(n) * 12 + (n & 1) * 4;
     uint32_t *h_s = static_cast<uint32_t *>(S->h_io);
     std::memcpy(h_s, sidx2.data(), n * 4);
@@ -564,15 +566,17 @@ int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_
     uint32_t *d_s = static_cast<uint32_t *>(S->d_io);
     int32_t *d_x = reinterpret_cast<int32_t *>(d_s + n), *d_z = d_x + n;
     double *d_d = reinterpret_cast<double *>(d_z + n + (n & 1));
-    int8_t *d_8 = reinterpret_cast<int8_t *>(d_d);
+    int8_t *d_8 = hottest ? reinterpret_cast<int8_t *>(d_d + n) : reinterpret_cast<int8_t *>(d_d);
     HIPCK(hipMemcpyAsync(d_s, h_s, in_bytes, hipMemcpyHostToDevice, c->st));
     dsx::launch_points(c->st, S->d_clim, S->grid, d_s, d_x, d_z, uint32_t(n), what, cut, d_8, d_d);
     HIPCK(hipGetLastError());
-    void *h_out = static_cast<uint8_t *>(S->h_io) + in_bytes;
-    HIPCK(hipMemcpyAsync(h_out, d_d, dbl ? n * 8 : n, hipMemcpyDeviceToHost, c->st));
+    uint8_t *h_out = static_cast<uint8_t *>(S->h_io) + in_bytes;
+    const size_t out_bytes = dbl ? n * 8 + (hottest ? n : 0) : n;
+    HIPCK(hipMemcpyAsync(h_out, d_d, out_bytes, hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
     if (dbl) std::memcpy(outd, h_out, n * 8);
-    else std::memcpy(out8, h_out, n);
+    if (hottest) std::memcpy(out8, h_out + n * 8, n);
+    else if (!dbl) std::memcpy(out8, h_out, n);
     return DSX_OK;
 }
 

@@ -51,6 +51,7 @@ FLOOD_JOB = np.dtype([("seed", "<u8"), ("kind", "<i4"), ("cut", "<f4"), ("qx", "
 AREA = np.dtype([("cells", "<u8"), ("sampled", "<u8"), ("minx", "<i4"), ("maxx", "<i4"), ("minz", "<i4"),
                  ("maxz", "<i4"), ("edge", "<i4"), ("border", "<i4")])
 PRED_DESERT_STATE, PRED_HOT, PRED_PROXY, PRED_DESERT = 0, 1, 2, 3
+POINT_T, POINT_PROXYPROB, POINT_HOTTEST = 16, 17, 18
 TILE_MASK = np.dtype([("job", "<u4"), ("tx", "<i4"), ("tz", "<i4"), ("rows", "<u4", (32,))])
 AREA_QUERY = np.dtype([("seed", "<u8"), ("x", "<i4"), ("z", "<i4"), ("half", "<i4"), ("pad", "<i4")])
 
@@ -153,6 +154,9 @@ def load(path=LIB_PATH):
                                    ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
     lib.dsx_flood.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
     lib.dsx_flood_masks.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, _u64p]
+    lib.dsx_points.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
+                               ctypes.c_void_p]
     lib.dsx_search_defaults.argtypes = [ctypes.POINTER(SearchParams)]
     lib.dsx_search_defaults.restype = None
     lib.dsx_search.argtypes = [ctypes.c_void_p, ctypes.POINTER(SearchParams), ctypes.c_int, ctypes.c_uint64,
@@ -174,7 +178,7 @@ def load(path=LIB_PATH):
     lib.dsx_profile.argtypes = [ctypes.c_int]
     lib.dsx_flood_stats_get.argtypes = [ctypes.POINTER(FloodStats), ctypes.c_int]
     for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile", "dsx_scan", "dsx_tables",
-              "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood", "dsx_flood_masks", "dsx_search", "dsx_area_queries",
+              "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood", "dsx_flood_masks", "dsx_points", "dsx_search", "dsx_area_queries",
               "dsx_server_create", "dsx_server_destroy", "dsx_server_workers", "dsx_submit_scan", "dsx_submit_area",
               "dsx_wait", "dsx_poll", "dsx_result_text", "dsx_release", "dsx_profile", "dsx_flood_stats_get"):
         getattr(lib, f).restype = ctypes.c_int
@@ -297,6 +301,18 @@ class Engine:
         out = np.zeros(len(j), AREA)
         self._check(self.lib.dsx_flood(self.h, j.ctypes.data, len(j), out.ctypes.data), "dsx_flood")
         return out
+
+    def points(self, seeds, sidx, xs, zs, what, cut=0.35):
+        """dsx_points: per point k of seeds[sidx[k]] at quart (xs[k], zs[k]) -> (int8 array, float64 array)."""
+        sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+        si = np.ascontiguousarray(sidx, dtype=np.uint32)
+        x = np.ascontiguousarray(xs, dtype=np.int32)
+        z = np.ascontiguousarray(zs, dtype=np.int32)
+        o8 = np.zeros(len(x), np.int8)
+        od = np.zeros(len(x), np.float64)
+        self._check(self.lib.dsx_points(self.h, sd.ctypes.data, len(sd), si.ctypes.data, x.ctypes.data, z.ctypes.data,
+                                        len(x), what, cut, o8.ctypes.data, od.ctypes.data), "dsx_points")
+        return o8, od
 
     def flood_masks(self):
         """Counted cells of the last flood() call: TILE_MASK array (job, tx, tz, rows[32])."""

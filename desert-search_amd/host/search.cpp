@@ -186,7 +186,8 @@ struct Sched {
         std::map<std::pair<int, float>, std::vector<PointsReq *>> groups;
         for (auto *r : rs) groups[{r->what, r->cut}].push_back(r);
         for (auto &[key, g] : groups) {
-            const bool dbl = key.first == DSX_POINT_T || key.first == DSX_POINT_PROXYPROB;
+            const bool hottest = key.first == DSX_POINT_HOTTEST;
+            const bool dbl = key.first == DSX_POINT_T || key.first == DSX_POINT_PROXYPROB || hottest;
             size_t i0 = 0;
             while (i0 < g.size()) {
                 std::vector<uint64_t> seeds;
@@ -207,7 +208,7 @@ struct Sched {
                     zs.insert(zs.end(), r->zs.begin(), r->zs.end());
                 }
                 std::vector<double> vd(dbl ? xs.size() : 0);
-                std::vector<int8_t> v8(dbl ? 0 : xs.size());
+                std::vector<int8_t> v8(dbl && !hottest ? 0 : xs.size());
                 const int rc = dsx_points(ctx, seeds.data(), seeds.size(), sidx.data(), xs.data(), zs.data(), xs.size(),
                                           key.first, key.second, v8.data(), vd.data());
                 if (rc) return rc;
@@ -216,7 +217,7 @@ struct Sched {
                     PointsReq *r = g[i];
                     const size_t m = r->xs.size();
                     if (dbl) r->vd.assign(vd.begin() + off, vd.begin() + off + m);
-                    else r->v8.assign(v8.begin() + off, v8.begin() + off + m);
+                    if (!dbl || hottest) r->v8.assign(v8.begin() + off, v8.begin() + off + m);
                     off += m;
                 }
                 i0 = i1;
@@ -388,27 +389,20 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
     std::vector<int> px_x(ns), px_z(ns);
     std::vector<float> px_b(ns, -1);
     if (P.hot_min > 0) {
-        PointsReq q{&S, seed, DSX_POINT_T, 0.f, {}, {}, {}, {}, {}};
-        q.xs.reserve(ns * 49);
-        q.zs.reserve(ns * 49);
-        for (size_t r = 0; r < ns; r++)
-            for (int dz = -3; dz <= 3; dz++)
-                for (int dx = -3; dx <= 3; dx++) {
-                    q.xs.push_back(sites[r].x + dx * 256);
-                    q.zs.push_back(sites[r].z + dz * 256);
-                }
-        co_await q;
-        size_t k = 0;
+        // the device keeps the first maximum of the 7 x 7 T samples of each site (dz, then dx)
+        PointsReq q{&S, seed, DSX_POINT_HOTTEST, 0.f, {}, {}, {}, {}, {}};
+        q.xs.reserve(ns);
+        q.zs.reserve(ns);
         for (size_t r = 0; r < ns; r++) {
-            hx[r] = sites[r].x;
-            hz[r] = sites[r].z;
-            for (int dz = -3; dz <= 3; dz++)
-                for (int dx = -3; dx <= 3; dx++, k++)
-                    if (q.vd[k] > bt[r]) {
-                        bt[r] = q.vd[k];
-                        hx[r] = sites[r].x + dx * 256;
-                        hz[r] = sites[r].z + dz * 256;
-                    }
+            q.xs.push_back(sites[r].x);
+            q.zs.push_back(sites[r].z);
+        }
+        co_await q;
+        for (size_t r = 0; r < ns; r++) {
+            const int k = q.v8[r];
+            bt[r] = q.vd[r];
+            hx[r] = sites[r].x + (k % 7 - 3) * 256;
+            hz[r] = sites[r].z + (k / 7 - 3) * 256;
         }
     } else if (P.multi <= 1) {
         PointsReq q{&S, seed, DSX_POINT_PROXYPROB, pcut, {}, {}, {}, {}, {}};

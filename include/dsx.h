@@ -133,6 +133,10 @@ int dsx_predicates(dsx_ctx *ctx, uint64_t seed, int kind, float cut, const int32
  * what = DSX_PRED_* gives the predicate in out8[k]; what = DSX_POINT_T gives T(x, 0, z) in outd[k]
  * (the hot-zone search of dsearch_desert.cpp:328-332). */
 enum { DSX_POINT_T = 16, DSX_POINT_PROXYPROB = 17 };   /* 17: DesertProxy::prob (desert_proxy.hpp:25-30) in outd */
+/* what = DSX_POINT_HOTTEST: (xs[k], zs[k]) is a site; the hottest of the 7 x 7 points site + (dx, dz) * 256
+ * quarts, dz then dx from -3 to 3, by T(x, 0, z) with the first maximum kept (the hot-zone search of
+ * dsearch_desert.cpp:328-332): outd[k] = its T, out8[k] = its index (dz + 3) * 7 + dx + 3. */
+enum { DSX_POINT_HOTTEST = 18 };
 int dsx_points(dsx_ctx *ctx, const uint64_t *seeds, uint64_t nseeds, const uint32_t *sidx, const int32_t *xs,
                const int32_t *zs, uint64_t n, int what, float cut, int8_t *out8, double *outd);
 

@@ -117,6 +117,36 @@ def test_flood_small_windows_mixed_batch(engine, oracle):
     _compare(engine, oracle, rows)
 
 
+def test_hottest_point_search(engine, oracle):
+    """The device search for the hottest of 7 x 7 points around each site (dsearch_desert.cpp:328-332:
+    dz then dx from -3 to 3, step 256, strict greater, so the first maximum wins) equals that scan over
+    T at the 49 points, and T there equals the oracle's NormalNoise."""
+    dsx = load_package().dsx
+    rng = np.random.default_rng(7)
+    seeds = np.array(SEEDS, np.uint64)
+    n = 600
+    sidx = rng.integers(0, len(seeds), n).astype(np.uint32)
+    xs = rng.integers(-7_400_000, 7_400_000, n).astype(np.int32)
+    zs = rng.integers(-7_400_000, 7_400_000, n).astype(np.int32)
+    xs[:8] = 7_499_000   # sites near the world border
+    o8, od = engine.points(seeds, sidx, xs, zs, dsx.POINT_HOTTEST)
+    off = np.arange(49)
+    px = (xs[:, None] + (off % 7 - 3)[None, :] * 256).ravel()
+    pz = (zs[:, None] + (off // 7 - 3)[None, :] * 256).ravel()
+    _, t = engine.points(seeds, np.repeat(sidx, 49), px, pz, dsx.POINT_T)
+    t = t.reshape(n, 49)
+    for k in range(n):
+        best, bi = -9.0, 0
+        for q in range(49):
+            if t[k, q] > best:
+                best, bi = t[k, q], q
+        assert (int(o8[k]), od[k]) == (bi, best), k
+    for k in range(0, n, 97):   # T itself against the oracle (bitwise)
+        for q in (0, 24, 48):
+            w = oracle.or_noise(int(seeds[sidx[k]]), 0, float(px[k * 49 + q]), 0.0, float(pz[k * 49 + q]))
+            assert t[k, q] == w
+
+
 def test_flood_counted_masks(engine, oracle):
     """The counted cells each flood exports (state 3 of dense_flood.hpp's window) equal the oracle's
     window state, for every engine: small windows, proxy and hot-zone floods that run on a local
