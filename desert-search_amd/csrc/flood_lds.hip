@@ -15,6 +15,7 @@
 // A flood that reaches the edge of its local sub-window inside the real window, or whose ring would
 // overflow, is handed to the tiled engine (appended to its job list, which runs next on the stream).
 #include "floods.h"
+#include "flood_dev.h"
 #include <climits>
 #include <type_traits>
 
@@ -51,9 +52,8 @@ __device__ __forceinline__ uint32_t bitmap_row32(const uint32_t *bm, int L, int 
 
 template <class CS>
 __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
-                                                  SurfGrid g, FloodRes *res, TileMask *masks, uint32_t mcap, uint32_t *big,
-                                                  uint32_t *ctl) {
-    uint32_t *mcount = ctl + FLOOD_MASKS;
+                                                  SurfGrid g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B) {
+    uint32_t *mcount = B.ctl + FLOOD_MASKS;
     __shared__ CS C;
     __shared__ uint32_t touched[LDS_WORDS], counted[LDS_WORDS];
     __shared__ uint32_t ring[LDS_RING];
@@ -172,7 +172,8 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
             r.minx = r.maxx = J.qx;
             r.minz = r.maxz = J.qz;
             res[job] = r;
-            big[atomicAdd(&ctl[BIG_N], 1u)] = job;
+            atomicAdd(&B.ctl[HANDED], 1u);
+            flood_start_tile(J, job, B);
         }
         return;
     }
@@ -221,14 +222,14 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
 }
 
 void launch_flood_lds(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
-                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, uint32_t *big, uint32_t *ctl) {
+                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B) {
     if (!n) return;
     if (cls == LDS_HOT)
-        hipLaunchKernelGGL(k_flood_lds<ClimT>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, big, ctl);
+        hipLaunchKernelGGL(k_flood_lds<ClimT>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
     else if (cls == LDS_PROXY)
-        hipLaunchKernelGGL(k_flood_lds<ClimP>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, big, ctl);
+        hipLaunchKernelGGL(k_flood_lds<ClimP>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
     else
-        hipLaunchKernelGGL(k_flood_lds<ClimSeed>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, big, ctl);
+        hipLaunchKernelGGL(k_flood_lds<ClimSeed>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
 }
 
 }  // namespace dsx

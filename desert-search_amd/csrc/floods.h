@@ -53,24 +53,24 @@ struct FloodBufs {
     uint32_t hmask;               // table size - 1
     uint32_t *wq;                 // work queue slots
     uint32_t wq_cap;
-    uint32_t *ctl;                // [Q_HEAD] [Q_TAIL] [Q_PENDING] [FLOOD_OVF] [FLOOD_MASKS] [Q_ITEMS]
+    uint32_t *ctl;                // this batch's control words (zero when it starts)
+    uint32_t *ctl_next;           // the next batch's control words (zeroed at the end of this one)
 };
 
-// ctl words: queue head/tail, pending items, overflow flags, exported masks, items processed, the
-// length of the tiled engine's job list (grows when a small-window flood hands over), allocated tiles
-// (published)
-constexpr int Q_HEAD = 0, Q_TAIL = 1, Q_PENDING = 2, FLOOD_OVF = 3, FLOOD_MASKS = 4, Q_ITEMS = 5, BIG_N = 6,
+// ctl words: queue head/tail, pending items, overflow flags, exported masks, items processed, floods
+// handed over by the small-window engine, allocated tiles (B.ntiles points here)
+constexpr int Q_HEAD = 0, Q_TAIL = 1, Q_PENDING = 2, FLOOD_OVF = 3, FLOOD_MASKS = 4, Q_ITEMS = 5, HANDED = 6,
               CTL_NTILES = 7, CTL_WORDS = 8;
 constexpr uint32_t Q_EMPTY = 0xffffffffu;
 // FLOOD_OVF bits: 1 work queue full, 2 tile pool or hash table full, 4 time limit reached
 constexpr uint32_t OVF_QUEUE = 1, OVF_TILES = 2, OVF_TIME = 4;
 
-// Before a batch of n floods: zero the control words and the tile count, set the tiled engine's job
-// list length to nbig, and initialise every result (an empty area at the centre).
-void launch_flood_prep(hipStream_t st, const FloodJob *jobs, uint32_t n, FloodRes *res, FloodBufs B, uint32_t nbig);
-// The start tile of every job of the tiled engine's list big[0 .. ctl[BIG_N]) is touched and queued
-// (n: an upper bound of the list length).
-void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *big, uint32_t n, FloodBufs B);
+// First kernel of a batch of n floods, reading the host's pinned staging directly: jobs and job lists
+// (layout: n jobs, n small-window indices, then the tiled engine's indices) to the device, every
+// result initialised (an empty area at the centre), and the start tile of each of the n_big tiled
+// floods touched and queued.
+void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big, uint8_t *d_up,
+                        FloodRes *res, FloodBufs B);
 // The persistent flood kernel: `blocks` workgroups take queued tiles until no work is pending. A tile
 // is evaluated lazily, by 8x8 sub-blocks the flood touches; every processing grows its reached set to
 // a fixpoint and queues the neighbours it touched. ev0/ev1 (optional): HIP events around it.
@@ -80,17 +80,18 @@ void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, 
 // the tiles and hash slots to the clean state.
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRes *res, struct TileMask *masks,
                       uint32_t mcap, uint32_t blocks);
-// Copy the results, the control words (ctl[CTL_NTILES] = allocated tiles) and up to hmcap exported
-// masks into host memory, so that one stream synchronisation ends the batch.
+// Copy the results, the control words and up to hmcap exported masks into host memory, so that one
+// stream synchronisation ends the batch, and zero the next batch's control words.
 void launch_flood_publish(hipStream_t st, const FloodRes *res, uint32_t n, FloodBufs B, const struct TileMask *masks,
                           FloodRes *h_res, uint32_t *h_ctl, struct TileMask *h_masks, uint32_t hmcap);
 // One wave per flood with its state in LDS over a local sub-window of at most LDS_MAX_N^2 points
 // around the start; a flood that outgrows it is handed to the tiled engine (its result reset and its
-// index appended to big[], counted in ctl[BIG_N]). cls: LDS_HOT (the T test, any window), LDS_PROXY
-// (proxy predicate, any window), LDS_FULL (any kind; the host sends only windows that fit whole).
+// start tile queued). cls: LDS_HOT (the T test, any window), LDS_PROXY (proxy predicate, any window),
+// LDS_FULL (any kind; the host sends only windows that fit whole).
 constexpr int LDS_MAX_N = 255;
 enum { LDS_HOT = 0, LDS_PROXY = 1, LDS_FULL = 2 };
 void launch_flood_lds(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
-                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, uint32_t *big, uint32_t *ctl);
+                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B);
+
 
 }  // namespace dsx

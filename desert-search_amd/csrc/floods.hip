@@ -32,125 +32,34 @@
 // next queue slot and polls it (each waiter on its own address); it leaves when Q_PENDING is 0, when
 // an overflow was flagged, or after a time limit (flagged as an error), so every wave exits.
 #include "floods.h"
+#include "flood_dev.h"
 #include <hip/hip_ext.h>
 #include <climits>
 #include <cstdlib>
 
 namespace dsx {
 
-__device__ __forceinline__ uint32_t nwin(const FloodJob &J) { return uint32_t(2 * J.half + 1); }
-__device__ __forceinline__ uint32_t ntile_axis(const FloodJob &J) { return (nwin(J) + FT - 1) / FT; }
-// valid-column mask of a tile row: columns whose window index is < n
-__device__ __forceinline__ uint32_t col_mask(const FloodJob &J, int tx) {
-    const int n = int(nwin(J)), c0 = tx * FT;
-    const int k = n - c0;
-    return k >= FT ? 0xffffffffu : (k <= 0 ? 0u : ((1u << k) - 1u));
-}
-
-// agent-scope (all XCDs) relaxed loads and stores of tile state shared inside a kernel
-template <class T> __device__ __forceinline__ T ald(const T *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <class T> __device__ __forceinline__ void ast(T *p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Hand-over points: every earlier memory operation of this wave has completed (its agent-scope
-// stores and atomics are performed at the coherence point) before the next one issues. Shared state
-// is only ever touched through agent-scope accesses, so no cache writeback or invalidation is needed
-// (an agent-scope release fence would write back the whole L2 at every hand-over).
-__device__ __forceinline__ void mem_done() { __builtin_amdgcn_s_waitcnt(0); }
-
-constexpr unsigned long long HEMPTY = ~0ull;
-__device__ __forceinline__ unsigned long long hkey(uint32_t job, int tx, int tz) {
-    return (uint64_t(job) << 44) | (uint64_t(uint32_t(tx) & 2047u) << 33) | (uint64_t(uint32_t(tz) & 2047u) << 22);
-}
-// Returns the tile index of (job, tx, tz), allocating it if absent; *fresh = this call allocated it.
-// Keys and values are published together in one 64-bit CAS, so no reader ever sees a half-made entry.
-__device__ uint32_t tile_get(const FloodBufs &B, uint32_t job, int tx, int tz, bool *fresh) {
-    const unsigned long long key = hkey(job, tx, tz);
-    uint32_t h = uint32_t(mix64(key) & B.hmask);
-    uint32_t mine = 0xffffffffu;
-    *fresh = false;
-    for (uint32_t probe = 0; probe <= B.hmask; probe++, h = (h + 1) & B.hmask) {
-        unsigned long long cur = ald(&B.hkeys[h]);
-        if (cur == HEMPTY) {
-            if (mine == 0xffffffffu) {
-                mine = atomicAdd(B.ntiles, 1u);
-                if (mine >= B.tile_cap) return 0xffffffffu;        // pool exhausted (reported as OVF_TILES)
-                Tile &t = B.tiles[mine];
-                ast(&t.job, job);
-                ast(&t.tx, tx);
-                ast(&t.tz, tz);
-                mem_done();
-            }
-            const unsigned long long want = key | mine;
-            cur = atomicCAS(&B.hkeys[h], HEMPTY, want);
-            if (cur == HEMPTY) {
-                ast(&B.tiles[mine].slot1, h + 1);
-                *fresh = true;
-                return mine;
-            }
-        }
-        if ((cur & ~0x3fffffull) == key) return uint32_t(cur & 0x3fffffull);
-    }
-    return 0xffffffffu;
-}
-
-// Append tile ti to the work queue (counted as pending first).
-__device__ __forceinline__ void wq_push(const FloodBufs &B, uint32_t ti) {
-    atomicAdd(&B.ctl[Q_PENDING], 1u);
-    const uint32_t k = atomicAdd(&B.ctl[Q_TAIL], 1u);
-    if (k >= B.wq_cap) {
-        atomicOr(&B.ctl[FLOOD_OVF], OVF_QUEUE);
-        return;
-    }
-    mem_done();
-    __hip_atomic_store(&B.wq[k], ti, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
-// After new `in` bits of tile ti are published: make sure the tile is (re)processed.
-__device__ __forceinline__ void tile_touch(const FloodBufs &B, uint32_t ti) {
-    uint32_t *q = &B.tiles[ti].qstate;
-    mem_done();
-    for (;;) {
-        uint32_t old = atomicCAS(q, 0u, 1u);
-        if (old == 0u) {
-            wq_push(B, ti);
-            return;
-        }
-        if (old == 2u) return;
-        old = atomicCAS(q, 1u, 2u);   // queued or in process: have it run again
-        if (old != 0u) return;        // now 2; if it went idle meanwhile (0), queue it ourselves
-    }
-}
-
-__global__ __launch_bounds__(256) void k_flood_prep(const FloodJob *jobs, uint32_t n, FloodRes *res, FloodBufs B,
-                                                    uint32_t nbig) {
+__global__ __launch_bounds__(256) void k_flood_stage(const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big,
+                                                     uint8_t *d_up, FloodRes *res, FloodBufs B) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < CTL_WORDS) B.ctl[q] = q == BIG_N ? nbig : 0u;
-    if (q == 0) *B.ntiles = 0;
-    if (q >= n) return;
-    FloodRes r{};
-    r.minx = r.maxx = jobs[q].qx;
-    r.minz = r.maxz = jobs[q].qz;
-    res[q] = r;
-}
-
-// The start cell of every flood of the tiled engine's list: touched ("in") and its tile queued.
-__global__ __launch_bounds__(256) void k_flood_start(const FloodJob *jobs, const uint32_t *big, uint32_t n, FloodBufs B) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= min(n, B.ctl[BIG_N])) return;
-    const uint32_t k = big[q];
-    const FloodJob J = jobs[k];
-    const int c = J.half % FT, t = J.half / FT;   // the centre cell (half, half)
-    bool fresh = false;
-    const uint32_t ti = tile_get(B, k, t, t, &fresh);
-    if (ti == 0xffffffffu) {
-        atomicOr(&B.ctl[FLOOD_OVF], OVF_TILES);
-        return;
+    const FloodJob *hj = reinterpret_cast<const FloodJob *>(h_up);
+    FloodJob *dj = reinterpret_cast<FloodJob *>(d_up);
+    const uint32_t *hl = reinterpret_cast<const uint32_t *>(hj + n);
+    uint32_t *dl = reinterpret_cast<uint32_t *>(dj + n);
+    if (q < n) {
+        const FloodJob J = hj[q];
+        dj[q] = J;
+        FloodRes r{};
+        r.minx = r.maxx = J.qx;
+        r.minz = r.maxz = J.qz;
+        res[q] = r;
+        if (q < n_small) dl[q] = hl[q];
     }
-    atomicOr(&B.tiles[ti].in[c], 1u << c);
-    tile_touch(B, ti);
+    if (q < n_big) {
+        const uint32_t k = hl[n + q];
+        dl[n + q] = k;
+        flood_start_tile(hj[k], k, B);
+    }
 }
 
 // Sub-blocks: a tile's 32x32 cells are evaluated lazily in 16 blocks of 8x8 (bit bz*4 + bx of the
@@ -521,7 +430,10 @@ __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBuf
 __global__ __launch_bounds__(256) void k_flood_publish(const FloodRes *res, uint32_t n, FloodBufs B, const TileMask *masks,
                                                        FloodRes *h_res, uint32_t *h_ctl, TileMask *h_masks, uint32_t hmcap) {
     const uint32_t nt = blockDim.x * gridDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t0 < CTL_WORDS) h_ctl[t0] = t0 == CTL_NTILES ? *B.ntiles : B.ctl[t0];
+    if (t0 < CTL_WORDS) {
+        h_ctl[t0] = B.ctl[t0];
+        B.ctl_next[t0] = 0;
+    }
     for (uint32_t q = t0; q < n; q += nt) h_res[q] = res[q];
     const uint32_t nm = min(min(B.ctl[FLOOD_MASKS], hmcap), 0xffffffffu);
     constexpr uint32_t W = sizeof(TileMask) / 4;
@@ -534,12 +446,9 @@ __global__ __launch_bounds__(256) void k_flood_publish(const FloodRes *res, uint
 
 constexpr int RUN_NT = 256;
 
-void launch_flood_prep(hipStream_t st, const FloodJob *jobs, uint32_t n, FloodRes *res, FloodBufs B, uint32_t nbig) {
-    const uint32_t m = n > uint32_t(CTL_WORDS) ? n : uint32_t(CTL_WORDS);
-    hipLaunchKernelGGL(k_flood_prep, dim3((m + 255) / 256), dim3(256), 0, st, jobs, n, res, B, nbig);
-}
-void launch_flood_start(hipStream_t st, const FloodJob *jobs, const uint32_t *big, uint32_t n, FloodBufs B) {
-    if (n) hipLaunchKernelGGL(k_flood_start, dim3((n + 255) / 256), dim3(256), 0, st, jobs, big, n, B);
+void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big, uint8_t *d_up,
+                        FloodRes *res, FloodBufs B) {
+    if (n) hipLaunchKernelGGL(k_flood_stage, dim3((n + 255) / 256), dim3(256), 0, st, h_up, n, n_small, n_big, d_up, res, B);
 }
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
                       uint32_t blocks, hipEvent_t ev0, hipEvent_t ev1) {

@@ -96,6 +96,8 @@ struct ClimState {
     uint32_t mask_cap = 0;
     std::vector<dsx::TileMask> masks;
     std::vector<hipEvent_t> events;   // event pair around the tiled engine's kernel (profiling on)
+    uint32_t *d_ctl = nullptr;        // two blocks of control words; batch i uses block i % 2
+    uint32_t ctl_phase = 0;
     // flood batch I/O: jobs and job lists go up in one copy from pinned staging (device layout: n jobs,
     // n small-window indices, n tiled-engine indices); results come down through pinned memory the
     // publish kernel writes
@@ -137,13 +139,13 @@ static int clim_alloc(dsx_ctx *c, ClimState *&S) {
     B.hmask = (1u << 23) - 1;                 // 8M hash slots
     HIPCK(hipMalloc(&B.tiles, size_t(B.tile_cap) * sizeof(dsx::Tile)));
     HIPCK(hipMemset(B.tiles, 0, size_t(B.tile_cap) * sizeof(dsx::Tile)));
-    HIPCK(hipMalloc(&B.ntiles, 4));
     HIPCK(hipMalloc(&B.hkeys, (size_t(B.hmask) + 1) * 8));
     HIPCK(hipMemset(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8));
     B.wq_cap = 4 * B.tile_cap;               // queue slots (a tile is queued again when touched again)
     HIPCK(hipMalloc(&B.wq, size_t(B.wq_cap) * 4));
     HIPCK(hipMemset(B.wq, 0xff, size_t(B.wq_cap) * 4));
-    HIPCK(hipMalloc(&B.ctl, dsx::CTL_WORDS * 4));
+    HIPCK(hipMalloc(&S->d_ctl, 2 * dsx::CTL_WORDS * 4));   // two control blocks: batches alternate
+    HIPCK(hipMemset(S->d_ctl, 0, 2 * dsx::CTL_WORDS * 4));
     S->job_cap = 1u << 20;
     HIPCK(hipMalloc(&S->d_up, size_t(S->job_cap) * (sizeof(dsx::FloodJob) + 8)));
     HIPCK(hipMalloc(&S->d_res, size_t(S->job_cap) * sizeof(dsx::FloodRes)));
@@ -286,7 +288,7 @@ void dsx_ctx_free_ext(dsx_ctx *c) {
         g_ext.erase(it);
     }
     for (void *p : {(void *)s->d_cuts, (void *)s->d_cell, (void *)s->d_clim, (void *)s->d_useeds, (void *)s->fb.tiles,
-                    (void *)s->fb.ntiles, (void *)s->fb.hkeys, (void *)s->fb.wq, (void *)s->fb.ctl, (void *)s->d_up,
+                    (void *)s->fb.hkeys, (void *)s->fb.wq, (void *)s->d_ctl, (void *)s->d_up,
                     (void *)s->d_res, s->d_io, (void *)s->d_masks})
         if (p) (void)hipFree(p);
     for (void *p : {(void *)s->h_up, (void *)s->h_res, (void *)s->h_ctl, (void *)s->h_masks, s->h_io})
@@ -410,14 +412,18 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         if (cls_of(hj[k]) < 0) h_big[n_big++] = uint32_t(k);
     uint8_t *d_up = S->d_up;
     const dsx::FloodJob *d_jobs = reinterpret_cast<const dsx::FloodJob *>(d_up);
-    uint32_t *d_small = reinterpret_cast<uint32_t *>(d_up + n * sizeof(dsx::FloodJob)), *d_big = d_small + n;
+    uint32_t *d_small = reinterpret_cast<uint32_t *>(d_up + n * sizeof(dsx::FloodJob));
     dsx::FloodBufs &B = S->fb;
-    HIPCK(hipMemcpyAsync(d_up, S->h_up, n * sizeof(dsx::FloodJob) + (size_t(n) + n_big) * 4, hipMemcpyHostToDevice, c->st));
-    dsx::launch_flood_prep(c->st, d_jobs, uint32_t(n), S->d_res, B, n_big);
+    S->ctl_phase ^= 1u;
+    B.ctl = S->d_ctl + S->ctl_phase * dsx::CTL_WORDS;
+    B.ctl_next = S->d_ctl + (S->ctl_phase ^ 1u) * dsx::CTL_WORDS;
+    B.ntiles = B.ctl + dsx::CTL_NTILES;
+    // jobs and lists read from the pinned staging by the first kernel (no separate copy)
+    dsx::launch_flood_stage(c->st, S->h_up, uint32_t(n), n_small, n_big, d_up, S->d_res, B);
     uint32_t off = 0;
     for (int cls = 0; cls < 3; cls++) {
         dsx::launch_flood_lds(c->st, cls, d_jobs, d_small + off, n_cls[cls], S->d_clim, S->grid, S->d_res, S->d_masks,
-                              S->mask_cap, d_big, B.ctl);
+                              S->mask_cap, B);
         off += n_cls[cls];
     }
     // the tiled engine: the jobs listed now plus the small-window floods handed over on the device
@@ -434,7 +440,6 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
             e0 = S->events[0];
             e1 = S->events[1];
         }
-        dsx::launch_flood_start(c->st, d_jobs, d_big, uint32_t(n), B);
         // workgroups: enough for the tiles in flight (every flood's frontier)
         static const uint32_t per_job = uint32_t(env_u32("DSX_FLOOD_BLOCKS_PER_JOB", 4)),
                               max_blocks = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS", 128));
