@@ -12,11 +12,15 @@ second over the timed steps. Inputs are seed index ranges generated on the devic
 CSV files the pipeline writes.
 
 Also reported:
-  roofline      the dominant kernel, the tiled flood engine's tile evaluation (k_flood_eval, FP64
-                VALU): algorithmic FP64 operations = the reference's sampled cells of the floods it ran
-                x octave samples per predicate call of their kind and cost class x 89 FP64 operations
-                per octave sample, divided by its device time from HIP events recorded around each
-                launch on the stream it runs on (all launches in the timed region).
+  roofline      the dominant kernel, the persistent tiled flood kernel (k_flood_run, FP64 VALU):
+                algorithmic FP64 operations = the reference's sampled cells of the floods it ran x octave
+                samples per predicate call of their kind and cost class x 89 FP64 operations per octave
+                sample, divided by its device time from HIP events recorded around each launch on the
+                stream it runs on (all launches in the timed region). traffic: HBM bytes per launch from
+                the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (calibrated against a
+                plain copy), when that summary is present.
+  roofline_gate the gate's pre-gate kernel (k_amp<0>, FP32 VALU) in the gate-only leg: 7,077,888 FP32
+                operations per phase survivor over its event-timed duration.
   cpu_baseline  the reference's own CPU dsearch_desert (built from its sources into oracle/_ref, gpu=0)
                 over a bounded sample of seed indices on this host's cores: scanned / seconds. Falls
                 back to the oracle's C++ restatement of the same pipeline when oracle/_ref is absent.
@@ -107,6 +111,15 @@ def cpu_baseline(sample=10 ** 7):
             "sample": f"seed indices [0, {sample:.0e}) through {what} with its CPU gate (gpu=0, production "
                       f"Params): {passed} phase survivors, {scanned} gated seeds through the pipeline in "
                       f"{sec:.1f} s on {cores} threads"}
+
+
+def measured_traffic():
+    """HBM bytes per launch of the flood kernel from the committed rocprofv3 --pmc summary, or None."""
+    path = os.path.join(REPO, "profiles", "flood_run_traffic.json")
+    try:
+        return float(json.load(open(path))["hbm_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def percentiles(v):
@@ -240,9 +253,9 @@ def main():
         "config": {"workload": "whole per-seed pipeline, GPU gate (gate=1) -> tables -> scan -> hot zones -> proxy "
                                "floods -> exact floods; step s = seed indices [1e12 + s*1e8, +1e8)",
                    "seed_indices_per_step": N_STEP, "gated_seeds": int(gated), "workers": int(params.workers) or None},
-        "roofline": {"bound": "fp64-valu", "kernel": "k_flood_eval (tiled flood engine, tile evaluation)",
+        "roofline": {"bound": "fp64-valu", "kernel": "k_flood_run (persistent tiled flood engine)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
                      "avg_launch_ms": avg_ms, "launches": int(st.eval_launches),
                      "flops_per_launch": flops / launches,
                      "octave_samples": int(samples), "sampled_cells": int(sampled_big),
@@ -261,9 +274,14 @@ def main():
         t1 = time.perf_counter()
         _, _, _, cnt = g.gate(0, 1 << 24)
         gsec = time.perf_counter() - t1
+        ms, work = g.gate_profile()
         g.close()
         out["gate_only"] = {"seed_indices": 1 << 24, "seconds": gsec, "gated_seeds_per_s": cnt[2] / gsec,
                             "counts": list(cnt)}
+        gach = 7077888 * work[2] / (ms[2] * 1e-3) / 1e12
+        out["roofline_gate"] = {"bound": "fp32-valu", "kernel": "k_amp<0> (4-parity pre-gate)", "achieved": gach,
+                                "peak": 157.3, "unit": "TFLOP/s", "frac": gach / 157.3, "avg_launch_ms": ms[2],
+                                "phase_survivors": int(work[2])}
         out["area_queries"] = area_burst(dsx)
         out["serving"] = serving(dsx)
     out["cpu_baseline"] = None if a.no_cpu else cpu_baseline()
