@@ -21,6 +21,9 @@ Also reported:
                 plain copy), when that summary is present.
   roofline_gate the gate's pre-gate kernel (k_amp<0>, FP32 VALU) in the gate-only leg: 7,077,888 FP32
                 operations per phase survivor over its event-timed duration.
+  prepass       the same timed steps with the FP32 pre-pass + FP64 confirm (dsx_set_precision 1):
+                gated seeds/s, and whether its output rows equal the plain FP64 run's; fp32_error: max
+                |FP32 - FP64| of each noise channel at 20,000 random points of 2 seeds.
   cpu_baseline  the reference's own CPU dsearch_desert (built from its sources into oracle/_ref, gpu=0)
                 over a bounded sample of seed indices on this host's cores: scanned / seconds. Falls
                 back to the oracle's C++ restatement of the same pipeline when oracle/_ref is absent.
@@ -267,6 +270,33 @@ def main():
                                  for k in range(4) if st.floods[k]},
                      "small_window_floods": st.small_floods, "small_window_sampled": st.small_sampled},
     }
+    # the FP32 pre-pass variant over the same steps: throughput and identical rows
+    pp = eng.search_params(gpu=1, gate=1, workers=a.workers, prepass=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gated_pp = 0
+    for s in range(a.steps):
+        gated_pp += eng.search(os.path.join(td, f"p{s}"), dsx.SEARCH_GATE, START + s * N_STEP, N_STEP, params=pp)[2]
+    torch.cuda.synchronize()
+    sec_pp = time.perf_counter() - t0
+    same = all(sorted(open(os.path.join(td, f"{t}{s}.csv")).read().splitlines()) ==
+               sorted(open(os.path.join(td, f"p{s}.csv")).read().splitlines()) and
+               sorted(open(os.path.join(td, f"{t}{s}.seeds.csv")).read().splitlines()) ==
+               sorted(open(os.path.join(td, f"p{s}.seeds.csv")).read().splitlines())
+               for s in range(a.steps) for t in ["s"])
+    import numpy as np
+    rng = np.random.default_rng(3)
+    err = {}
+    for seed in (0x5eed, 3008578943537042204):
+        pts = np.zeros((20000, 3))
+        pts[:, 0] = rng.integers(-7_500_000, 7_500_000, len(pts))
+        pts[:, 2] = rng.integers(-7_500_000, 7_500_000, len(pts))
+        for ch, name in enumerate(("T", "H", "C", "E", "shift", "W")):
+            d = float(np.abs(eng.noise_f32(seed, ch, pts) - eng.noise(seed, ch, pts)).max())
+            err[name] = max(err.get(name, 0.0), d)
+    out["prepass"] = {"value": gated_pp / sec_pp, "ms_per_step": 1e3 * sec_pp / a.steps, "gated_seeds": int(gated_pp),
+                      "outputs_identical": bool(same and gated_pp == gated), "vs_fp64": (gated_pp / sec_pp) / value,
+                      "fp32_error": err, "bound_T": 2e-4}
     eng.close()
     if not a.no_extra:
         g = dsx.Engine(0)

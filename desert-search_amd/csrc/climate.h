@@ -399,6 +399,111 @@ __host__ __device__ constexpr int pred_samples(int kind, bool hot) {
 // and accumulate 2.
 constexpr int FLOPS_PER_SAMPLE = 89;
 
+// ------------------------------------------------------------------ FP32 pre-pass
+// The noise in FP32 where that is safe to decide a predicate, FP64 (above) for the rest. The lattice
+// cell and the fractions are computed in FP64 exactly as perlin_sample; fades, gradient dot products
+// and lerps run in FP32 with FMA. Error bound (u = 2^-24), per octave sample:
+//   fractions f in [0, 1]: |f32 - f| <= u;  fade: |fade'| <= 1.875, evaluation <= 80u  => <= 82u;
+//   gradient dots (components -1, 0, 1, one of them 0; |terms| <= 1): <= 6u;
+//   lerp a + t(b - a) with |b - a| <= 4: error <= max(e_a, e_b) + 4 e_t + 6u, so x-lerps <= 340u,
+//   y-lerps <= 350u, z-lerp <= 684u (< 4.1e-5).
+// A noise value sums its octaves (|weight| sum <= 2 for T) times factor <= 5/3, plus the FP32 sums:
+//   |T32 - T64| <= 5/3 * 2 * 684u + 20u < 1.4e-4   =>  PRE_ERR_T = 2e-4 is used.
+// At shifted coordinates (desert_state) the shift noise in FP32 moves px by at most 4 x 3e-4 quarts;
+// improved noise has |grad| <= 3.75 * 2.45 + 1 < 10.2 per lattice unit and T's octaves scale quarts by
+// at most 2^-9 * 337/331, so T moves by at most 10.2 * 1.2e-3 * 2^-9 * 1.02 * 2 * 5/3 < 1e-4 more:
+// PRE_ERR_TS = 4e-4 is used. A cell is decided in FP32 only when T32 is farther than the bound from
+// the threshold; every other cell is evaluated in FP64, so results are bit-identical to plain FP64.
+constexpr double PRE_ERR_T = 2e-4, PRE_ERR_TS = 4e-4;
+
+__device__ __forceinline__ float fade32(float t) { return t * t * t * fmaf(t, fmaf(t, 6.f, -15.f), 10.f); }
+__device__ __forceinline__ float lerp32(float t, float a, float b) { return fmaf(t, b - a, a); }
+__device__ __forceinline__ float grad32(const GradTab &G, int h, float x, float y, float z) {
+    const double2 g = G.xy[h & 15];
+    return fmaf(float(G.z[h & 15]), z, fmaf(float(g.y), y, float(g.x) * x));
+}
+__device__ __forceinline__ float perlin_sample32(const GradTab &G, const ClimOct &o, double x, double y, double z) {
+    x += o.ox;
+    z += o.oz;
+    const double bx = floor(x), bz = floor(z);
+    const int ix = int(bx), iz = int(bz);
+    const float fx = float(x - bx), fz = float(z - bz);
+    int jy = o.iy;
+    float dy = float(o.fy), ty = float(o.vy);
+    if (y != 0) {
+        y += o.oy;
+        const double by = floor(y);
+        jy = int(by);
+        dy = float(y - by);
+        ty = fade32(dy);
+    }
+    const uint8_t *P = o.perm;
+    const int a0 = P[ix & 255], a1 = P[(ix + 1) & 255];
+    const int b00 = P[(a0 + jy) & 255], b01 = P[(a0 + jy + 1) & 255];
+    const int b10 = P[(a1 + jy) & 255], b11 = P[(a1 + jy + 1) & 255];
+    const float tx = fade32(fx), tz = fade32(fz);
+    const float x1 = fx - 1.f, y1 = dy - 1.f, z1 = fz - 1.f;
+    const float l00 = lerp32(tx, grad32(G, P[(b00 + iz) & 255], fx, dy, fz), grad32(G, P[(b10 + iz) & 255], x1, dy, fz));
+    const float l10 = lerp32(tx, grad32(G, P[(b01 + iz) & 255], fx, y1, fz), grad32(G, P[(b11 + iz) & 255], x1, y1, fz));
+    const float l01 = lerp32(tx, grad32(G, P[(b00 + iz + 1) & 255], fx, dy, z1), grad32(G, P[(b10 + iz + 1) & 255], x1, dy, z1));
+    const float l11 = lerp32(tx, grad32(G, P[(b01 + iz + 1) & 255], fx, y1, z1), grad32(G, P[(b11 + iz + 1) & 255], x1, y1, z1));
+    return lerp32(tz, lerp32(ty, l00, l10), lerp32(ty, l01, l11));
+}
+// NormalNoise::sample with FP32 octaves (coordinates scaled in FP64 as noise_sample does)
+template <class CS>
+__device__ __forceinline__ float noise_sample32(const CS &C, int ch, double x, double y, double z) {
+    const int n = clim_n(ch);
+    float sum0 = 0, sum1 = 0;
+#pragma unroll 1
+    for (int j = 0; j < n; j++) {
+        const ClimOct &o = C.oct[oct_slot<CS>(ch, 0, j)];
+        sum0 = fmaf(float(o.weight), perlin_sample32(C.g, o, x * o.freq, y * o.freq, z * o.freq), sum0);
+    }
+    const double sc = 337. / 331.;
+    const double px = x * sc, py = y * sc, pz = z * sc;
+#pragma unroll 1
+    for (int j = 0; j < n; j++) {
+        const ClimOct &o = C.oct[oct_slot<CS>(ch, 1, j)];
+        sum1 = fmaf(float(o.weight), perlin_sample32(C.g, o, px * o.freq, py * o.freq, pz * o.freq), sum1);
+    }
+    return (sum0 + sum1) * float(C.factor[ch]);
+}
+
+// The predicates with the FP32 pre-pass on their temperature test: the hot test decided both ways,
+// the proxy and the biome predicates' cold cells rejected; the rest in FP64.
+template <class CS>
+__device__ __forceinline__ int hot_state_pre(const CS &C, int x, int z) {
+    const double t = noise_sample32(C, DSX_CH_T, x, 0, z);
+    if (t + PRE_ERR_T < 0.55) return 0;
+    if (t - PRE_ERR_T >= 0.55) return 2;
+    return hot_state(C, x, z);
+}
+template <class CS>
+__device__ __forceinline__ int proxy_state_pre(const CS &C, int x, int z, float cut, bool *hot = nullptr) {
+    if (double(noise_sample32(C, DSX_CH_T, x, 0, z)) + PRE_ERR_T < 0.55) return 0;
+    return proxy_state(C, x, z, cut, hot);
+}
+// desert_state / desert: 0 when T at the FP32-shifted position is certainly below 0.5499 (then
+// int64(10000f * float(T)) <= 5499 < 5500, surface.hpp:102-105)
+__device__ __forceinline__ bool biome_cold_pre(const ClimSeed &C, int x, int z) {
+    const double px = x + 4. * double(noise_sample32(C, DSX_CH_SHIFT, x, 0, z));
+    const double pz = z + 4. * double(noise_sample32(C, DSX_CH_SHIFT, z, x, 0));
+    return double(noise_sample32(C, DSX_CH_T, px, 0, pz)) + PRE_ERR_TS < 0.5499;
+}
+__device__ __forceinline__ int predicate_pre(int kind, const ClimSeed &C, const SurfGrid &g, int x, int z, float cut,
+                                             bool *hot = nullptr) {
+    switch (kind) {
+    case PRED_HOT: return hot_state_pre(C, x, z);
+    case PRED_PROXY: return proxy_state_pre(C, x, z, cut, hot);
+    default: return biome_cold_pre(C, x, z) ? 0 : predicate(kind, C, g, x, z, cut, hot);
+    }
+}
+// the predicate of a flood or point batch: plain FP64 (pre = 0) or with the FP32 pre-pass (pre = 1)
+__device__ __forceinline__ int predicate_mode(int pre, int kind, const ClimSeed &C, const SurfGrid &g, int x, int z,
+                                              float cut, bool *hot = nullptr) {
+    return pre ? predicate_pre(kind, C, g, x, z, cut, hot) : predicate(kind, C, g, x, z, cut, hot);
+}
+
 void launch_climinit(hipStream_t st, const uint64_t *seeds, const uint32_t *slots, uint32_t n, ClimSeed *out);
 
 }  // namespace dsx

@@ -99,6 +99,13 @@ __global__ __launch_bounds__(256) void k_noise(const ClimSeed *cs, int ch, const
     if (k < n) out[k] = noise_sample(C, ch, pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]);
 }
 
+__global__ __launch_bounds__(256) void k_noise32(const ClimSeed *cs, int ch, const double *pts, uint32_t n, double *out) {
+    __shared__ ClimSeed C;
+    load_climate(C, *cs);
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k < n) out[k] = noise_sample32(C, ch, pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]);
+}
+
 __global__ __launch_bounds__(256) void k_climate(const ClimSeed *cs, const int32_t *xs, const int32_t *zs, uint32_t n,
                                                  int64_t *q) {
     __shared__ ClimSeed C;
@@ -108,23 +115,23 @@ __global__ __launch_bounds__(256) void k_climate(const ClimSeed *cs, const int32
 }
 
 __global__ __launch_bounds__(256) void k_pred(const ClimSeed *cs, SurfGrid g, int kind, float cut, const int32_t *xs,
-                                              const int32_t *zs, uint32_t n, int8_t *out) {
+                                              const int32_t *zs, uint32_t n, int8_t *out, int pre) {
     __shared__ ClimSeed C;
     load_climate(C, *cs);
     const uint32_t k = blockIdx.x * 256u + threadIdx.x;
-    if (k < n) out[k] = int8_t(predicate(kind, C, g, xs[k], zs[k], cut));
+    if (k < n) out[k] = int8_t(predicate_mode(pre, kind, C, g, xs[k], zs[k], cut));
 }
 
 // Points of many seeds: the climate stays in global memory (L2), one thread per point.
 __global__ __launch_bounds__(256) void k_points(const ClimSeed *cs, SurfGrid g, const uint32_t *sidx, const int32_t *xs,
                                                 const int32_t *zs, uint32_t n, int what, float cut, int8_t *out8,
-                                                double *outd) {
+                                                double *outd, int pre) {
     const uint32_t k = blockIdx.x * 256u + threadIdx.x;
     if (k >= n) return;
     const ClimSeed &C = cs[sidx[k]];
     if (what == 16) outd[k] = noise_sample(C, DSX_CH_T, xs[k], 0, zs[k]);
     else if (what == 17) outd[k] = proxy_prob(C, xs[k], zs[k]);
-    else out8[k] = int8_t(predicate(what, C, g, xs[k], zs[k], cut));
+    else out8[k] = int8_t(predicate_mode(pre, what, C, g, xs[k], zs[k], cut));
 }
 
 // Hottest of the 7 x 7 points around each site (one wave per site, lane l < 49 = (dz + 3) * 7 + dx + 3):
@@ -156,21 +163,24 @@ __global__ __launch_bounds__(256) void k_hottest(const ClimSeed *cs, const uint3
 void launch_climinit(hipStream_t st, const uint64_t *seeds, const uint32_t *slots, uint32_t n, ClimSeed *out) {
     if (n) hipLaunchKernelGGL(k_climinit, dim3((CLIM_NOCT * n + 63) / 64), dim3(64), 0, st, seeds, slots, n, out);
 }
-void launch_noise(hipStream_t st, const ClimSeed *cs, int ch, const double *pts, uint32_t n, double *out) {
-    if (n) hipLaunchKernelGGL(k_noise, dim3((n + 255) / 256), dim3(256), 0, st, cs, ch, pts, n, out);
+void launch_noise(hipStream_t st, const ClimSeed *cs, int ch, const double *pts, uint32_t n, double *out, int fp32) {
+    if (!n) return;
+    if (fp32) hipLaunchKernelGGL(k_noise32, dim3((n + 255) / 256), dim3(256), 0, st, cs, ch, pts, n, out);
+    else hipLaunchKernelGGL(k_noise, dim3((n + 255) / 256), dim3(256), 0, st, cs, ch, pts, n, out);
 }
 void launch_climate(hipStream_t st, const ClimSeed *cs, const int32_t *xs, const int32_t *zs, uint32_t n, int64_t *q) {
     if (n) hipLaunchKernelGGL(k_climate, dim3((n + 255) / 256), dim3(256), 0, st, cs, xs, zs, n, q);
 }
 void launch_points(hipStream_t st, const ClimSeed *cs, const SurfGrid &g, const uint32_t *sidx, const int32_t *xs,
-                   const int32_t *zs, uint32_t n, int what, float cut, int8_t *out8, double *outd) {
+                   const int32_t *zs, uint32_t n, int what, float cut, int8_t *out8, double *outd, int pre) {
     if (!n) return;
     if (what == 18) hipLaunchKernelGGL(k_hottest, dim3((n + 3) / 4), dim3(256), 0, st, cs, sidx, xs, zs, n, out8, outd);
-    else hipLaunchKernelGGL(k_points, dim3((n + 255) / 256), dim3(256), 0, st, cs, g, sidx, xs, zs, n, what, cut, out8, outd);
+    else hipLaunchKernelGGL(k_points, dim3((n + 255) / 256), dim3(256), 0, st, cs, g, sidx, xs, zs, n, what, cut, out8, outd,
+                            pre);
 }
 void launch_pred(hipStream_t st, const ClimSeed *cs, const SurfGrid &g, int kind, float cut, const int32_t *xs,
-                 const int32_t *zs, uint32_t n, int8_t *out) {
-    if (n) hipLaunchKernelGGL(k_pred, dim3((n + 255) / 256), dim3(256), 0, st, cs, g, kind, cut, xs, zs, n, out);
+                 const int32_t *zs, uint32_t n, int8_t *out, int pre) {
+    if (n) hipLaunchKernelGGL(k_pred, dim3((n + 255) / 256), dim3(256), 0, st, cs, g, kind, cut, xs, zs, n, out, pre);
 }
 
 }  // namespace dsx

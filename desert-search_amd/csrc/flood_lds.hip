@@ -25,14 +25,14 @@ constexpr int LDS_N = 255;                                    // largest local s
 constexpr int LDS_WORDS = (LDS_N * LDS_N + 31) / 32;           // 2033 words per bitmap
 constexpr int LDS_RING = 2048;                                 // pushed cells in flight
 
-template <class CS>
+template <class CS, int PRE>
 __device__ __forceinline__ int lds_pred(int kind, const CS &C, const SurfGrid &g, int x, int z, float cut, bool *hot) {
     if constexpr (std::is_same<CS, ClimT>::value) {
-        return hot_state(C, x, z);
+        return PRE ? hot_state_pre(C, x, z) : hot_state(C, x, z);
     } else if constexpr (std::is_same<CS, ClimP>::value) {
-        return proxy_state(C, x, z, cut, hot);
+        return PRE ? proxy_state_pre(C, x, z, cut, hot) : proxy_state(C, x, z, cut, hot);
     } else {
-        return predicate(kind, C, g, x, z, cut, hot);
+        return PRE ? predicate_pre(kind, C, g, x, z, cut, hot) : predicate(kind, C, g, x, z, cut, hot);
     }
 }
 
@@ -50,7 +50,7 @@ __device__ __forceinline__ uint32_t bitmap_row32(const uint32_t *bm, int L, int 
     return v << (lo - c0);
 }
 
-template <class CS>
+template <class CS, int PRE>
 __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
                                                   SurfGrid g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B) {
     uint32_t *mcount = B.ctl + FLOOD_MASKS;
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
             }
             sampled++;
             bool hot = false;
-            const int v = lds_pred<CS>(J.kind, C, g, x, z, J.cut, &hot);
+            const int v = lds_pred<CS, PRE>(J.kind, C, g, x, z, J.cut, &hot);
             hotc += hot;
             if (v == 0) continue;
             if (v == 2) {
@@ -221,15 +221,22 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         }
 }
 
-void launch_flood_lds(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
-                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B) {
-    if (!n) return;
+template <int PRE>
+static void lds_launch(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
+                       const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B) {
     if (cls == LDS_HOT)
-        hipLaunchKernelGGL(k_flood_lds<ClimT>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
+        hipLaunchKernelGGL((k_flood_lds<ClimT, PRE>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
     else if (cls == LDS_PROXY)
-        hipLaunchKernelGGL(k_flood_lds<ClimP>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
+        hipLaunchKernelGGL((k_flood_lds<ClimP, PRE>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
     else
-        hipLaunchKernelGGL(k_flood_lds<ClimSeed>, dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
+        hipLaunchKernelGGL((k_flood_lds<ClimSeed, PRE>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
+}
+
+void launch_flood_lds(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
+                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B, int pre) {
+    if (!n) return;
+    if (pre) lds_launch<1>(st, cls, jobs, list, n, cs, g, res, masks, mcap, B);
+    else lds_launch<0>(st, cls, jobs, list, n, cs, g, res, masks, mcap, B);
 }
 
 }  // namespace dsx

@@ -74,8 +74,9 @@ void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_
 // The persistent flood kernel: `blocks` workgroups take queued tiles until no work is pending. A tile
 // is evaluated lazily, by 8x8 sub-blocks the flood touches; every processing grows its reached set to
 // a fixpoint and queues the neighbours it touched. ev0/ev1 (optional): HIP events around it.
+// pre: 1 = FP32 pre-pass on the predicates' temperature test (identical results)
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
-                      uint32_t blocks, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                      uint32_t blocks, int pre, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // Reduce every allocated tile (count read on the device) into res, export counted masks, and return
 // the tiles and hash slots to the clean state.
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRes *res, struct TileMask *masks,
@@ -91,7 +92,7 @@ void launch_flood_publish(hipStream_t st, const FloodRes *res, uint32_t n, Flood
 constexpr int LDS_MAX_N = 255;
 enum { LDS_HOT = 0, LDS_PROXY = 1, LDS_FULL = 2 };
 void launch_flood_lds(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
-                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B);
+                      const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B, int pre);
 
 
 }  // namespace dsx

@@ -69,6 +69,7 @@ constexpr int SB = 8, NSB = FT / SB;
 // Evaluate sub-block sb of the tile at (tx, tz) with one wave (one cell per lane) and OR its bits into
 // the LDS row masks (pass = v != 0, cnt = v == 2, hot = the predicate's temperature test passed; cells
 // outside the window or the world are none of them).
+template <int PRE>
 __device__ __forceinline__ void eval_subblock(int sb, int tx, int tz, const FloodJob &J, const ClimSeed &C,
                                               const SurfGrid &g, uint32_t *s_pass, uint32_t *s_cnt, uint32_t *s_hot) {
     const int lane = threadIdx.x & 63;
@@ -80,7 +81,8 @@ __device__ __forceinline__ void eval_subblock(int sb, int tx, int tz, const Floo
     bool hot = false;
     if (i < nw && j < nw) {
         const int x = J.qx + (i - J.half) * J.step, z = J.qz + (j - J.half) * J.step;
-        if (x >= -WORLD_Q && x <= WORLD_Q && z >= -WORLD_Q && z <= WORLD_Q) v = predicate(J.kind, C, g, x, z, J.cut, &hot);
+        if (x >= -WORLD_Q && x <= WORLD_Q && z >= -WORLD_Q && z <= WORLD_Q)
+            v = PRE ? predicate_pre(J.kind, C, g, x, z, J.cut, &hot) : predicate(J.kind, C, g, x, z, J.cut, &hot);
     }
     const uint64_t mp = __ballot(v != 0), mc = __ballot(v == 2), mh = __ballot(hot);
     if (lane < SB) {   // lane rr: row bz*8 + rr of the sub-block
@@ -193,7 +195,7 @@ __device__ uint32_t prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J
 
 // Persistent workgroups over the work queue (NT threads each; wave 0 propagates, all waves evaluate).
 // OCC: waves per SIMD the register allocation must allow.
-template <int NT, int OCC>
+template <int NT, int OCC, int PRE>
 __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B) {
     constexpr int NWAVE = NT / 64;
     __shared__ ClimSeed C;
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                 // the k-th needed sub-block goes to wave k % NWAVE
                 int k = 0;
                 for (uint32_t m = need; m; m &= m - 1, k++)
-                    if (k % NWAVE == wave) eval_subblock(__ffs(m) - 1, tx, tz, J, C, g, s_pass, s_cnt, s_hot);
+                    if (k % NWAVE == wave) eval_subblock<PRE>(__ffs(m) - 1, tx, tz, J, C, g, s_pass, s_cnt, s_hot);
                 __syncthreads();
                 if (tid == 0) s_evald |= need;
                 __syncthreads();
@@ -451,13 +453,14 @@ void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_
     if (n) hipLaunchKernelGGL(k_flood_stage, dim3((n + 255) / 256), dim3(256), 0, st, h_up, n, n_small, n_big, d_up, res, B);
 }
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
-                      uint32_t blocks, hipEvent_t ev0, hipEvent_t ev1) {
+                      uint32_t blocks, int pre, hipEvent_t ev0, hipEvent_t ev1) {
     const dim3 grid(blocks ? blocks : 1);
     static const bool occ4 = [] {
         const char *v = std::getenv("DSX_FLOOD_OCC");
         return v && std::atoi(v) >= 4;
     }();
-    auto kern = occ4 ? k_flood_run<RUN_NT, 4> : k_flood_run<RUN_NT, 1>;
+    auto kern = pre ? (occ4 ? k_flood_run<RUN_NT, 4, 1> : k_flood_run<RUN_NT, 1, 1>)
+                    : (occ4 ? k_flood_run<RUN_NT, 4, 0> : k_flood_run<RUN_NT, 1, 0>);
     if (ev0) hipExtLaunchKernelGGL(kern, grid, dim3(RUN_NT), 0, st, ev0, ev1, 0u, jobs, cs, g, B);
     else hipLaunchKernelGGL(kern, grid, dim3(RUN_NT), 0, st, jobs, cs, g, B);
 }

@@ -96,6 +96,7 @@ struct ClimState {
     uint32_t mask_cap = 0;
     std::vector<dsx::TileMask> masks;
     std::vector<hipEvent_t> events;   // event pair around the tiled engine's kernel (profiling on)
+    int prepass = 0;                  // dsx_set_precision: 1 = FP32 pre-pass in floods and predicates
     uint32_t *d_ctl = nullptr;        // two blocks of control words; batch i uses block i % 2
     uint32_t ctl_phase = 0;
     // flood batch I/O: jobs and job lists go up in one copy from pinned staging (device layout: n jobs,
@@ -297,9 +298,7 @@ void dsx_ctx_free_ext(dsx_ctx *c) {
     delete s;
 }
 
-extern "C" {
-
-int dsx_noise(dsx_ctx *c, uint64_t seed, int channel, const double *xyz, uint64_t n, double *out) {
+static int noise_impl(dsx_ctx *c, uint64_t seed, int channel, const double *xyz, uint64_t n, double *out, int fp32) {
     if (!c || channel < 0 || channel > 5 || (n && (!xyz || !out))) return DSX_E_ARG;
     HIPCK(hipSetDevice(c->device));
     ClimState *S;
@@ -310,10 +309,26 @@ int dsx_noise(dsx_ctx *c, uint64_t seed, int channel, const double *xyz, uint64_
     if ((rc = io_reserve(c, S, n * 32))) return rc;
     double *d_pts = static_cast<double *>(S->d_io), *d_out = d_pts + 3 * n;
     HIPCK(hipMemcpyAsync(d_pts, xyz, n * 24, hipMemcpyHostToDevice, c->st));
-    dsx::launch_noise(c->st, S->d_clim + sl, channel, d_pts, uint32_t(n), d_out);
+    dsx::launch_noise(c->st, S->d_clim + sl, channel, d_pts, uint32_t(n), d_out, fp32);
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(out, d_out, n * 8, hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
+    return DSX_OK;
+}
+
+extern "C" {
+
+int dsx_noise(dsx_ctx *c, uint64_t seed, int channel, const double *xyz, uint64_t n, double *out) {
+    return noise_impl(c, seed, channel, xyz, n, out, 0);
+}
+
+int dsx_noise_f32(dsx_ctx *c, uint64_t seed, int channel, const double *xyz, uint64_t n, double *out) {
+    return noise_impl(c, seed, channel, xyz, n, out, 1);
+}
+
+int dsx_set_precision(dsx_ctx *c, int mode) {
+    if (!c || mode < 0 || mode > 1) return DSX_E_ARG;
+    clim_state(c)->prepass = mode;
     return DSX_OK;
 }
 
@@ -351,7 +366,7 @@ int dsx_predicates(dsx_ctx *c, uint64_t seed, int kind, float cut, const int32_t
     int8_t *d_o = reinterpret_cast<int8_t *>(d_z + n);
     HIPCK(hipMemcpyAsync(d_x, xs, n * 4, hipMemcpyHostToDevice, c->st));
     HIPCK(hipMemcpyAsync(d_z, zs, n * 4, hipMemcpyHostToDevice, c->st));
-    dsx::launch_pred(c->st, S->d_clim + sl, S->grid, kind, cut, d_x, d_z, uint32_t(n), d_o);
+    dsx::launch_pred(c->st, S->d_clim + sl, S->grid, kind, cut, d_x, d_z, uint32_t(n), d_o, S->prepass);
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(out, d_o, n, hipMemcpyDeviceToHost, c->st));
     HIPCK(hipStreamSynchronize(c->st));
@@ -423,7 +438,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     uint32_t off = 0;
     for (int cls = 0; cls < 3; cls++) {
         dsx::launch_flood_lds(c->st, cls, d_jobs, d_small + off, n_cls[cls], S->d_clim, S->grid, S->d_res, S->d_masks,
-                              S->mask_cap, B);
+                              S->mask_cap, B, S->prepass);
         off += n_cls[cls];
     }
     // the tiled engine: the jobs listed now plus the small-window floods handed over on the device
@@ -444,7 +459,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         static const uint32_t per_job = uint32_t(env_u32("DSX_FLOOD_BLOCKS_PER_JOB", 4)),
                               max_blocks = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS", 128));
         const uint32_t blocks = uint32_t(std::clamp<size_t>(size_t(per_job) * n_big, 16, max_blocks));
-        dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, e0, e1);
+        dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, S->prepass, e0, e1);
         dsx::launch_flood_acc(c->st, d_jobs, B, S->d_res, S->d_masks, S->mask_cap, n_big ? 1024 : 64);
     }
     dsx::launch_flood_publish(c->st, S->d_res, uint32_t(n), B, S->d_masks, S->h_res, S->h_ctl, S->h_masks, S->h_mask_cap);
@@ -573,7 +588,7 @@ int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_
     double *d_d = reinterpret_cast<double *>(d_z + n + (n & 1));
     int8_t *d_8 = hottest ? reinterpret_cast<int8_t *>(d_d + n) : reinterpret_cast<int8_t *>(d_d);
     HIPCK(hipMemcpyAsync(d_s, h_s, in_bytes, hipMemcpyHostToDevice, c->st));
-    dsx::launch_points(c->st, S->d_clim, S->grid, d_s, d_x, d_z, uint32_t(n), what, cut, d_8, d_d);
+    dsx::launch_points(c->st, S->d_clim, S->grid, d_s, d_x, d_z, uint32_t(n), what, cut, d_8, d_d, S->prepass);
     HIPCK(hipGetLastError());
     uint8_t *h_out = static_cast<uint8_t *>(S->h_io) + in_bytes;
     const size_t out_bytes = dbl ? n * 8 + (hottest ? n : 0) : n;

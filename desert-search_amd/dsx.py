@@ -65,7 +65,8 @@ class SearchParams(ctypes.Structure):
          ("verify_mb", ctypes.c_double)] + \
         [(n, ctypes.c_int32) for n in ("study", "gpu", "gpucheck", "gscan", "gscancheck", "gate")] + \
         [(n, ctypes.c_double) for n in ("gate_phase", "gate_amp", "gate_pthr")] + [("from_stdin", ctypes.c_int32),
-                                                                               ("workers", ctypes.c_int32)]
+                                                                               ("workers", ctypes.c_int32),
+                                                                               ("prepass", ctypes.c_int32)]
 
 
 class SearchCounts(ctypes.Structure):
@@ -176,11 +177,14 @@ def load(path=LIB_PATH):
                                     ctypes.c_uint64]
     lib.dsx_release.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
     lib.dsx_profile.argtypes = [ctypes.c_int]
+    lib.dsx_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.dsx_noise_f32.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
+                                  ctypes.c_void_p]
     lib.dsx_flood_stats_get.argtypes = [ctypes.POINTER(FloodStats), ctypes.c_int]
     for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile", "dsx_scan", "dsx_tables",
               "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood", "dsx_flood_masks", "dsx_points", "dsx_search", "dsx_area_queries",
               "dsx_server_create", "dsx_server_destroy", "dsx_server_workers", "dsx_submit_scan", "dsx_submit_area",
-              "dsx_wait", "dsx_poll", "dsx_result_text", "dsx_release", "dsx_profile", "dsx_flood_stats_get"):
+              "dsx_wait", "dsx_poll", "dsx_result_text", "dsx_release", "dsx_profile", "dsx_flood_stats_get", "dsx_set_precision", "dsx_noise_f32"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -300,6 +304,17 @@ class Engine:
         j = np.ascontiguousarray(jobs, dtype=FLOOD_JOB)
         out = np.zeros(len(j), AREA)
         self._check(self.lib.dsx_flood(self.h, j.ctypes.data, len(j), out.ctypes.data), "dsx_flood")
+        return out
+
+    def set_precision(self, mode):
+        """dsx_set_precision: 0 plain FP64, 1 FP32 pre-pass + FP64 confirm (identical results)."""
+        self._check(self.lib.dsx_set_precision(self.h, mode), "dsx_set_precision")
+
+    def noise_f32(self, seed, channel, xyz):
+        """dsx_noise_f32: the pre-pass's FP32 octaves at points xyz (n x 3) -> float64 array."""
+        p = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(p), np.float64)
+        self._check(self.lib.dsx_noise_f32(self.h, seed, channel, p.ctypes.data, len(p), out.ctypes.data), "dsx_noise_f32")
         return out
 
     def points(self, seeds, sidx, xs, zs, what, cut=0.35):

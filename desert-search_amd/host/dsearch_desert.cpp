@@ -48,7 +48,8 @@ int main(int argc, char **argv) {
             {"exact_min", &P.exact_min, nullptr}, {"csv_min", &P.csv_min, nullptr}, {"hot_pre", &P.hot_pre, nullptr},
             {"gate", nullptr, &P.gate}, {"gscan", nullptr, &P.gscan}, {"gscancheck", nullptr, &P.gscancheck},
             {"gate_phase", &P.gate_phase, nullptr}, {"gate_amp", &P.gate_amp, nullptr},
-            {"gate_pthr", &P.gate_pthr, nullptr}, {"refine", &P.refine_mb, nullptr}, {"verify", &P.verify_mb, nullptr}};
+            {"gate_pthr", &P.gate_pthr, nullptr}, {"refine", &P.refine_mb, nullptr}, {"verify", &P.verify_mb, nullptr},
+            {"prepass", nullptr, &P.prepass}};
         bool ok = false;
         for (auto &e : keys)
             if (k == e.name) {

@@ -659,6 +659,7 @@ Group::~Group() = default;
 
 int run_group(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g) {
     const dsx_scan_params sp{float(P.ta), float(P.t), float(P.wc), float(P.we), P.sigma, P.sigmac};
+    if (const int rc = dsx_set_precision(w, P.prepass ? 1 : 0)) return rc;
     const size_t nb = g.seeds.size();
     auto tscan = std::chrono::steady_clock::now();
     std::vector<float> mta(nb), mtb(nb);
@@ -840,7 +841,8 @@ void dsx_search_defaults(dsx_search_params *p) {
     d.wc = 0.25; d.we = 0.41; d.t = 0.55; d.ta = 0.25; d.max_sites = 300; d.pcut = 0.35; d.hot_min = 180;
     d.multi = 1; d.exact_n = 2; d.exact_min = 75; d.hot_pre = 0.8; d.csv_min = 50; d.nms = 5000;
     d.refine_mb = 70; d.verify_mb = 80; d.study = 0; d.gpu = 0; d.gpucheck = 0; d.gscan = 0; d.gscancheck = 0;
-    d.gate = 0; d.gate_phase = 0.06; d.gate_amp = 0.70; d.gate_pthr = 0.645; d.from_stdin = 0;
+    d.gate = 0; d.gate_phase = 0.06; d.gate_amp = 0.70; d.gate_pthr = 0.645; d.from_stdin = 0; d.workers = 0;
+    d.prepass = 0;
     *p = d;
 }
 

@@ -123,6 +123,15 @@ enum {
 
 /* NormalNoise::sample (standalone.hpp:140-151) of `channel` for seed at n points xyz[3k..3k+2], FP64. */
 int dsx_noise(dsx_ctx *ctx, uint64_t seed, int channel, const double *xyz, uint64_t n, double *out);
+/* The same with the FP32 octaves of the pre-pass (lattice cells and fractions still FP64); no counterpart
+ * in the reference: it measures the pre-pass error against dsx_noise. */
+int dsx_noise_f32(dsx_ctx *ctx, uint64_t seed, int channel, const double *xyz, uint64_t n, double *out);
+/* Precision of the context's flood and predicate kernels (no counterpart in the reference, which is FP64
+ * throughout): 0 plain FP64; 1 FP32 pre-pass + FP64 confirm — the temperature test of each predicate is
+ * evaluated in FP32 first and decided there only where T32 is farther from its threshold than the proven
+ * FP32 error bound (2e-4, 4e-4 at shifted coordinates); every other cell is evaluated in FP64. Results are
+ * identical in both modes. */
+int dsx_set_precision(dsx_ctx *ctx, int mode);
 /* Sampler::climate (standalone.hpp:224-238) at quarts (xs[k], 64, zs[k]): q[6k..6k+5] = 10000 x {T,H,C,E,D,W}. */
 int dsx_climate(dsx_ctx *ctx, uint64_t seed, const int32_t *xs, const int32_t *zs, uint64_t n, int64_t *q);
 /* Flood predicate `kind` (DSX_PRED_*) at quarts (xs[k], zs[k]); cut is the proxy cut (Params::pcut). */
@@ -214,6 +223,7 @@ typedef struct {
     double gate_phase, gate_amp, gate_pthr;
     int32_t from_stdin;
     int32_t workers; /* pipeline workers, each with its own stream and device buffers (0: default 16) */
+    int32_t prepass; /* 1: FP32 pre-pass + FP64 confirm in the floods and predicates (dsx_set_precision) */
 } dsx_search_params;
 void dsx_search_defaults(dsx_search_params *p);
 
