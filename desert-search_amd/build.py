@@ -21,7 +21,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared", "
 HIP_SOURCES = ["csrc/gate.hip", "csrc/tables.hip", "csrc/climate.hip", "csrc/floods.hip", "csrc/flood_lds.hip",
                "csrc/dsx_api.hip",
                "csrc/scan_api.hip", "csrc/floods_api.hip", "host/search.cpp", "host/serve.cpp"]
-HOST_TOOLS = ["host/dsearch_desert.cpp"]
+HOST_TOOLS = ["host/dsearch_desert.cpp", "host/tgate.cpp", "host/biomemap.cpp", "host/dfunc.cpp"]
 
 
 def _compile(src, obj):
@@ -54,7 +54,9 @@ def build_lib(verbose=False):
     os.replace(tmp, out)
     for src in HOST_TOOLS:
         exe = os.path.join(HERE, os.path.splitext(os.path.basename(src))[0])
-        r = subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(HERE, src), "-L" + HERE, "-ldsx",
+        # host code only; the HIP headers are included for the __host__ __device__ seed arithmetic
+        r = subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
+                            "-I/opt/rocm/include", os.path.join(HERE, src), "-L" + HERE, "-ldsx",
                             "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"], capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError("g++ failed:\n" + r.stdout + r.stderr)

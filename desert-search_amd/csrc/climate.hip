@@ -131,7 +131,12 @@ __global__ __launch_bounds__(256) void k_points(const ClimSeed *cs, SurfGrid g, 
     const ClimSeed &C = cs[sidx[k]];
     if (what == 16) outd[k] = noise_sample(C, DSX_CH_T, xs[k], 0, zs[k]);
     else if (what == 17) outd[k] = proxy_prob(C, xs[k], zs[k]);
-    else out8[k] = int8_t(predicate_mode(pre, what, C, g, xs[k], zs[k], cut));
+    else if (what == 19) {   // surface.hpp:79-83
+        int64_t q[6];
+        climate_q(C, xs[k], 64, zs[k], q);
+        const int b = grid_lookup(g, q);
+        out8[k] = int8_t(b >= 0 ? b : classify(q));
+    } else out8[k] = int8_t(predicate_mode(pre, what, C, g, xs[k], zs[k], cut));
 }
 
 // Hottest of the 7 x 7 points around each site (one wave per site, lane l < 49 = (dz + 3) * 7 + dx + 3):
@@ -158,6 +163,81 @@ __global__ __launch_bounds__(256) void k_hottest(const ClimSeed *cs, const uint3
         outd[site] = t;
         out8[site] = int8_t(idx);
     }
+}
+
+// ------------------------------------------------------------------ proxy-table Monte Carlo (tools/dfunc.cpp)
+// Octaves [lo, hi) of channel ch in one running sum over both sides, times the factor (tools/dfunc.cpp:12-21).
+__device__ __forceinline__ double octave_range(const ClimSeed &C, int ch, double x, double z, int lo, int hi) {
+    const int n = clim_n(ch);
+    double s = 0;
+    for (int side = 0; side < 2; side++) {
+        const double sc = side ? 337.0 / 331.0 : 1.0;
+#pragma unroll 1
+        for (int k = lo; k < hi && k < n; k++) {
+            const ClimOct &o = C.oct[oct_slot<ClimSeed>(ch, side, k)];
+            s += o.weight * perlin_sample(C.g, o, x * sc * o.freq, 0, z * sc * o.freq);
+        }
+    }
+    return s * C.factor[ch];
+}
+
+// tools/dfunc.cpp:52-55: a try is a hot point when T at the shifted position is >= 0.55.
+__global__ __launch_bounds__(256) void k_dfunc_hot(const ClimSeed *cs, const uint32_t *sidx, const int32_t *xs,
+                                                   const int32_t *zs, uint32_t n, uint8_t *hot) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= n) return;
+    const ClimSeed &C = cs[sidx[k]];
+    double px, pz;
+    shifted(C, xs[k], zs[k], px, pz);
+    hot[k] = noise_sample(C, DSX_CH_T, px, 0, pz) >= 0.55;
+}
+
+// tools/dfunc.cpp:56-65 for one hot point: surface biome, MC-Measure bridge to desert, low-octave C/E bins of
+// both sets, counted into hist[((set * 40 + ci) * 40 + ei) * 3 + {total, desert, passable}] (integer atomics,
+// so the histogram does not depend on the order of the points).
+__device__ __forceinline__ int dfunc_bin(double v, double lo) {
+    const int i = int(floor((v + lo) / 0.05));
+    return i < 0 ? 0 : (i > 39 ? 39 : i);
+}
+__global__ __launch_bounds__(256) void k_dfunc_rec(const ClimSeed *cs, SurfGrid g, const uint32_t *sidx,
+                                                   const int32_t *xs, const int32_t *zs, uint32_t n,
+                                                   unsigned long long *hist) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= n) return;
+    const ClimSeed &C = cs[sidx[k]];
+    const int x = xs[k], z = zs[k];
+    double px, pz;
+    shifted(C, x, z, px, pz);
+    int64_t q[6];
+    climate_q(C, x, 64, z, q);
+    int b = grid_lookup(g, q);
+    if (b < 0) b = classify(q);
+    const bool des = b == 2;
+    bool pass = des;
+    if (!des && (b == 7 || b == 11) && q[5] <= 500 && q[5] >= -500) {   // surface.hpp:126-134
+        q[5] = q[5] >= 0 ? 501 : -501;
+        int u = grid_lookup(g, q);
+        if (u < 0) u = classify(q);
+        pass = u == 2;
+    }
+    const double c0 = octave_range(C, DSX_CH_C, px, pz, 0, 1), c02 = c0 + octave_range(C, DSX_CH_C, px, pz, 1, 3);
+    const double e0 = octave_range(C, DSX_CH_E, px, pz, 0, 1), e01 = e0 + octave_range(C, DSX_CH_E, px, pz, 1, 2);
+    for (int set = 0; set < 2; set++) {
+        const int i = dfunc_bin(set ? c02 : c0, 0.8), j = dfunc_bin(set ? e01 : e0, 1.0);
+        unsigned long long *cell = hist + ((set * 40 + i) * 40 + j) * 3;
+        atomicAdd(cell, 1ull);
+        if (des) atomicAdd(cell + 1, 1ull);
+        if (pass) atomicAdd(cell + 2, 1ull);
+    }
+}
+
+void launch_dfunc_hot(hipStream_t st, const ClimSeed *cs, const uint32_t *sidx, const int32_t *xs, const int32_t *zs,
+                      uint32_t n, uint8_t *hot) {
+    if (n) hipLaunchKernelGGL(k_dfunc_hot, dim3((n + 255) / 256), dim3(256), 0, st, cs, sidx, xs, zs, n, hot);
+}
+void launch_dfunc_rec(hipStream_t st, const ClimSeed *cs, const SurfGrid &g, const uint32_t *sidx, const int32_t *xs,
+                      const int32_t *zs, uint32_t n, unsigned long long *hist) {
+    if (n) hipLaunchKernelGGL(k_dfunc_rec, dim3((n + 255) / 256), dim3(256), 0, st, cs, g, sidx, xs, zs, n, hist);
 }
 
 void launch_climinit(hipStream_t st, const uint64_t *seeds, const uint32_t *slots, uint32_t n, ClimSeed *out) {

@@ -11,4 +11,9 @@ void launch_pred(hipStream_t st, const ClimSeed *cs, const SurfGrid &g, int kind
                  const int32_t *zs, uint32_t n, int8_t *out, int pre);
 void launch_points(hipStream_t st, const ClimSeed *cs, const SurfGrid &g, const uint32_t *sidx, const int32_t *xs,
                    const int32_t *zs, uint32_t n, int what, float cut, int8_t *out8, double *outd, int pre);
+// proxy-table Monte Carlo (tools/dfunc.cpp): hot flags of tries, then the histogram of the selected hot points
+void launch_dfunc_hot(hipStream_t st, const ClimSeed *cs, const uint32_t *sidx, const int32_t *xs, const int32_t *zs,
+                      uint32_t n, uint8_t *hot);
+void launch_dfunc_rec(hipStream_t st, const ClimSeed *cs, const SurfGrid &g, const uint32_t *sidx, const int32_t *xs,
+                      const int32_t *zs, uint32_t n, unsigned long long *hist);
 }  // namespace dsx

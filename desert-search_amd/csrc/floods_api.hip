@@ -1,6 +1,7 @@
 // C ABI for the climate sampler and the flood engine (include/dsx.h: dsx_noise, dsx_climate,
 // dsx_predicates, dsx_flood). Replaces the CPU floods of dsearch_desert.cpp:326-533 and tools/dflood.cpp.
 #include "climate_api.h"
+#include "dsx_rng.h"
 #include "ctx.h"
 #include "floods.h"
 #include <algorithm>
@@ -558,7 +559,7 @@ int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_
                const int32_t *zs, uint64_t n, int what, float cut, int8_t *out8, double *outd) {
     const bool hottest = what == DSX_POINT_HOTTEST;
     const bool dbl = what == DSX_POINT_T || what == DSX_POINT_PROXYPROB || hottest;
-    if (!c || (n && (!sidx || !xs || !zs)) || (!dbl && (what < 0 || what > 3)) || (dbl && n && !outd) ||
+    if (!c || (n && (!sidx || !xs || !zs)) || (!dbl && (what < 0 || what > 3) && what != DSX_POINT_BIOME) || (dbl && n && !outd) ||
         ((!dbl || hottest) && n && !out8) || (nseeds && !seeds))
         return DSX_E_ARG;
     for (uint64_t k = 0; k < n; k++)
@@ -597,6 +598,116 @@ int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_
     if (dbl) std::memcpy(outd, h_out, n * 8);
     if (hottest) std::memcpy(out8, h_out + n * 8, n);
     else if (!dbl) std::memcpy(out8, h_out, n);
+    return DSX_OK;
+}
+
+int dsx_dfunc(dsx_ctx *c, uint32_t nseeds, uint32_t pts, uint64_t *hist) {
+    if (!c || !hist || pts > 10000000u) return DSX_E_ARG;
+    std::memset(hist, 0, DSX_DFUNC_CELLS * sizeof(uint64_t));
+    // seeds mix(7k + 99991) that pass the temperature phase filter s <= 0.36 (tools/dfunc.cpp:37-45)
+    std::vector<uint64_t> seeds;
+    for (uint32_t k = 0; k < nseeds; k++) {
+        const uint64_t seed = dsx::mix64(uint64_t(k) * 7 + 99991);
+        if (!(dsx::temperature_phase(seed, nullptr) > 0.36)) seeds.push_back(seed);
+    }
+    if (seeds.empty() || pts == 0) return DSX_OK;
+    HIPCK(hipSetDevice(c->device));
+    ClimState *S;
+    int rc = clim_alloc(c, S);
+    if (rc) return rc;
+    // io layout: histogram (u64 cells) | tries: sidx, x, z (u32 x T each) | hot flags (T bytes) |
+    // selected hot points: sidx, x, z (u32 x T each)
+    const uint32_t T = 1u << 22;
+    const size_t o_try = DSX_DFUNC_CELLS * 8, o_hot = o_try + size_t(T) * 12, o_rec = o_hot + T;
+    if ((rc = io_reserve(c, S, o_rec + size_t(T) * 12))) return rc;
+    uint8_t *d = static_cast<uint8_t *>(S->d_io), *h = static_cast<uint8_t *>(S->h_io);
+    unsigned long long *d_hist = reinterpret_cast<unsigned long long *>(d);
+    HIPCK(hipMemsetAsync(d_hist, 0, DSX_DFUNC_CELLS * 8, c->st));
+    uint32_t *h_try = reinterpret_cast<uint32_t *>(h + o_try), *h_rec = reinterpret_cast<uint32_t *>(h + o_rec);
+    const uint32_t *d_try = reinterpret_cast<const uint32_t *>(d + o_try),
+                   *d_rec = reinterpret_cast<const uint32_t *>(d + o_rec);
+    const uint8_t *h_hot = h + o_hot;
+    const uint64_t max_tries = uint64_t(pts) * 200;   // tools/dfunc.cpp:50
+    struct Run {
+        dsx::Xoro rng;
+        uint64_t tries;
+        uint32_t got, slot;
+        uint32_t t0, m;   // this round's tries in the batch
+    };
+    const uint32_t G = 1024;   // seeds per group (climate slots)
+    for (size_t g0 = 0; g0 < seeds.size(); g0 += G) {
+        const uint32_t ng = uint32_t(std::min<size_t>(G, seeds.size() - g0));
+        std::vector<uint32_t> slot(ng);
+        if ((rc = clim_slots(c, S, seeds.data() + g0, ng, slot.data()))) return rc;
+        std::vector<Run> run(ng);
+        std::vector<uint32_t> active(ng);
+        for (uint32_t i = 0; i < ng; i++) {
+            run[i] = {dsx::Xoro::seeded(seeds[g0 + i] ^ 0x1234), 0, 0, slot[i], 0, 0};
+            active[i] = i;
+        }
+        while (!active.empty()) {
+            // this round: each active seed draws up to 8x the hot points it still needs (at least 256), in
+            // draw order, while the batch has room; the batch is laid out sidx[n] | x[n] | z[n]
+            uint32_t n = 0;
+            std::vector<uint32_t> in_round;
+            for (uint32_t i : active) {
+                Run &r = run[i];
+                const uint64_t want = std::max<uint64_t>(256, uint64_t(pts - r.got) * 8);
+                const uint32_t m = uint32_t(std::min<uint64_t>(std::min(want, max_tries - r.tries), T - n));
+                if (m == 0) break;
+                r.t0 = n;
+                r.m = m;
+                n += m;
+                in_round.push_back(i);
+            }
+            uint32_t *hs = h_try, *hx = h_try + n, *hz = h_try + 2 * size_t(n);
+            for (uint32_t i : in_round) {
+                Run &r = run[i];
+                for (uint32_t t = r.t0; t < r.t0 + r.m; t++) {
+                    hs[t] = r.slot;
+                    hx[t] = uint32_t(int32_t(r.rng.bounded(15000000)) - 7500000);
+                    hz[t] = uint32_t(int32_t(r.rng.bounded(15000000)) - 7500000);
+                }
+            }
+            HIPCK(hipMemcpyAsync(const_cast<uint32_t *>(d_try), h_try, size_t(n) * 12, hipMemcpyHostToDevice, c->st));
+            dsx::launch_dfunc_hot(c->st, S->d_clim, d_try, reinterpret_cast<const int32_t *>(d_try + n),
+                                  reinterpret_cast<const int32_t *>(d_try + 2 * size_t(n)), n, d + o_hot);
+            HIPCK(hipGetLastError());
+            HIPCK(hipMemcpyAsync(h + o_hot, d + o_hot, n, hipMemcpyDeviceToHost, c->st));
+            HIPCK(hipStreamSynchronize(c->st));
+            // the first hot tries of each seed in draw order, up to pts in all (tools/dfunc.cpp:50-56)
+            std::vector<uint32_t> rs, rx, rz;
+            for (uint32_t i : in_round) {
+                Run &r = run[i];
+                for (uint32_t t = r.t0; t < r.t0 + r.m && r.got < pts; t++) {
+                    r.tries++;
+                    if (!h_hot[t]) continue;
+                    r.got++;
+                    rs.push_back(r.slot);
+                    rx.push_back(hx[t]);
+                    rz.push_back(hz[t]);
+                }
+            }
+            std::vector<uint32_t> still;
+            for (uint32_t i : active)
+                if (run[i].got < pts && run[i].tries < max_tries) still.push_back(i);
+            active.swap(still);
+            const uint32_t nr = uint32_t(rs.size());
+            if (nr) {
+                std::memcpy(h_rec, rs.data(), size_t(nr) * 4);
+                std::memcpy(h_rec + nr, rx.data(), size_t(nr) * 4);
+                std::memcpy(h_rec + 2 * size_t(nr), rz.data(), size_t(nr) * 4);
+                HIPCK(hipMemcpyAsync(const_cast<uint32_t *>(d_rec), h_rec, size_t(nr) * 12, hipMemcpyHostToDevice,
+                                     c->st));
+                dsx::launch_dfunc_rec(c->st, S->d_clim, S->grid, d_rec, reinterpret_cast<const int32_t *>(d_rec + nr),
+                                      reinterpret_cast<const int32_t *>(d_rec + 2 * size_t(nr)), nr, d_hist);
+                HIPCK(hipGetLastError());
+            }
+        }
+    }
+    HIPCK(hipMemcpyAsync(h, d_hist, DSX_DFUNC_CELLS * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCK(hipStreamSynchronize(c->st));
+    std::memcpy(hist, h, DSX_DFUNC_CELLS * 8);
     return DSX_OK;
 }
 

@@ -13,14 +13,21 @@ they are carried over as data, re-emitted here in this package's own layout:
 * ``DSX_STEPS``    per-level child strides of the packed tree
 * ``DSX_DPROB``    P(desert | C_lo bin, E_lo bin), bins of 0.05
 
-Usage: python gen_tables.py [/root/reference]   (writes mc_tables.inc beside this script)
+Usage: python gen_tables.py [/root/reference] [--dtable desert_dtable.hpp]   (writes mc_tables.inc beside
+this script; --dtable takes the proxy table from a header written by gen_dtable.py)
 """
 import os
 import re
 import struct
 import sys
 
-REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+ARGS = sys.argv[1:]
+DTABLE = None
+if "--dtable" in ARGS:   # a proxy table regenerated with dfunc + gen_dtable.py instead of the reference's
+    i = ARGS.index("--dtable")
+    DTABLE = ARGS[i + 1]
+    del ARGS[i:i + 2]
+REF = ARGS[0] if ARGS else "/root/reference"
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -69,7 +76,7 @@ def fl(tok):
 
 def main():
     wg = open(os.path.join(REF, "src", "worldgen_data.hpp")).read()
-    dt = open(os.path.join(REF, "src", "desert_dtable.hpp")).read()
+    dt = open(DTABLE or os.path.join(REF, "src", "desert_dtable.hpp")).read()
 
     splines = []
     for it in top_level_items(body(wg, "splines[]")):

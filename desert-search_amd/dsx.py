@@ -51,7 +51,7 @@ FLOOD_JOB = np.dtype([("seed", "<u8"), ("kind", "<i4"), ("cut", "<f4"), ("qx", "
 AREA = np.dtype([("cells", "<u8"), ("sampled", "<u8"), ("minx", "<i4"), ("maxx", "<i4"), ("minz", "<i4"),
                  ("maxz", "<i4"), ("edge", "<i4"), ("border", "<i4")])
 PRED_DESERT_STATE, PRED_HOT, PRED_PROXY, PRED_DESERT = 0, 1, 2, 3
-POINT_T, POINT_PROXYPROB, POINT_HOTTEST = 16, 17, 18
+POINT_T, POINT_PROXYPROB, POINT_HOTTEST, POINT_BIOME = 16, 17, 18, 19
 TILE_MASK = np.dtype([("job", "<u4"), ("tx", "<i4"), ("tz", "<i4"), ("rows", "<u4", (32,))])
 AREA_QUERY = np.dtype([("seed", "<u8"), ("x", "<i4"), ("z", "<i4"), ("half", "<i4"), ("pad", "<i4")])
 
@@ -66,7 +66,8 @@ class SearchParams(ctypes.Structure):
         [(n, ctypes.c_int32) for n in ("study", "gpu", "gpucheck", "gscan", "gscancheck", "gate")] + \
         [(n, ctypes.c_double) for n in ("gate_phase", "gate_amp", "gate_pthr")] + [("from_stdin", ctypes.c_int32),
                                                                                ("workers", ctypes.c_int32),
-                                                                               ("prepass", ctypes.c_int32)]
+                                                                               ("prepass", ctypes.c_int32),
+                                                                               ("mcverify", ctypes.c_int32)]
 
 
 class SearchCounts(ctypes.Structure):
@@ -181,10 +182,11 @@ def load(path=LIB_PATH):
     lib.dsx_noise_f32.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
                                   ctypes.c_void_p]
     lib.dsx_flood_stats_get.argtypes = [ctypes.POINTER(FloodStats), ctypes.c_int]
+    lib.dsx_dfunc.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
     for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile", "dsx_scan", "dsx_tables",
               "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood", "dsx_flood_masks", "dsx_points", "dsx_search", "dsx_area_queries",
               "dsx_server_create", "dsx_server_destroy", "dsx_server_workers", "dsx_submit_scan", "dsx_submit_area",
-              "dsx_wait", "dsx_poll", "dsx_result_text", "dsx_release", "dsx_profile", "dsx_flood_stats_get", "dsx_set_precision", "dsx_noise_f32"):
+              "dsx_wait", "dsx_poll", "dsx_result_text", "dsx_release", "dsx_profile", "dsx_flood_stats_get", "dsx_set_precision", "dsx_noise_f32", "dsx_dfunc"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -291,6 +293,12 @@ class Engine:
         self._check(self.lib.dsx_predicates(self.h, seed, kind, cut, x.ctypes.data, z.ctypes.data, len(x),
                                             out.ctypes.data), "dsx_predicates")
         return out
+
+    def dfunc(self, nseeds, pts):
+        """dsx_dfunc: proxy-table histograms, uint64 array [set][ci][ei][total, desert, passable]."""
+        h = np.zeros((2, 40, 40, 3), np.uint64)
+        self._check(self.lib.dsx_dfunc(self.h, nseeds, pts, h.ctypes.data), "dsx_dfunc")
+        return h
 
     def area_queries(self, queries):
         """queries: AREA_QUERY array -> AREA array (dsx_area_queries)"""

@@ -49,7 +49,7 @@ int main(int argc, char **argv) {
             {"gate", nullptr, &P.gate}, {"gscan", nullptr, &P.gscan}, {"gscancheck", nullptr, &P.gscancheck},
             {"gate_phase", &P.gate_phase, nullptr}, {"gate_amp", &P.gate_amp, nullptr},
             {"gate_pthr", &P.gate_pthr, nullptr}, {"refine", &P.refine_mb, nullptr}, {"verify", &P.verify_mb, nullptr},
-            {"prepass", nullptr, &P.prepass}};
+            {"prepass", nullptr, &P.prepass}, {"mcverify", nullptr, &P.mcverify}};
         bool ok = false;
         for (auto &e : keys)
             if (k == e.name) {

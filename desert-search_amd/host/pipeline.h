@@ -26,6 +26,7 @@ int run_group(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g);
 // stderr lines (CANDIDATE; "best" lines when `best` is given: it holds the running best).
 struct GroupText {
     std::string csv, seeds, log;
+    std::vector<std::string> verified;   // verified CANDIDATE lines (mcverify=1), without time and run
     uint64_t scanned = 0, sites = 0;
 };
 void format_group(const Group &g, const dsx_search_params &P, GroupText &t, double *best);

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
 #include <exception>
 #include <map>
 #include <memory>
@@ -344,6 +345,8 @@ struct SeedOut {
     std::vector<Eval> ev;
     double best = 0;
     std::vector<std::string> lines;   // stderr lines (CANDIDATE)
+    std::vector<std::pair<int, int>> cand_xy;   // block coordinates of each CANDIDATE line
+    std::vector<dsx_area> cand_area;            // their exact step-1 areas (mcverify=1)
     std::vector<double> bests;        // per exact flood value v, in order (for the global "best" lines)
     std::vector<std::pair<int, int>> best_xy;
     std::vector<double> best_proxy;
@@ -621,6 +624,7 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
             std::snprintf(line, sizeof line, "CANDIDATE step4=%.2fM seed=%lld x=%d z=%d -> verify with exact flood (Y=256)",
                           e.s4, (long long)seed, bx * 4 + 2, bz * 4 + 2);
             O.lines.push_back(line);
+            O.cand_xy.push_back({bx * 4 + 2, bz * 4 + 2});
         }
         O.bests.push_back(v);
         O.best_xy.push_back({bx * 4 + 2, bz * 4 + 2});
@@ -697,12 +701,28 @@ int run_group(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g) {
     }
     for (auto &t : tasks) t.h.destroy();
     prof_add(2, std::chrono::duration<double>(std::chrono::steady_clock::now() - tsites).count(), 0);
-    return S.err;
+    if (S.err || !P.mcverify) return S.err;
+    // in-process verification: the exact step-1 area of every CANDIDATE, one batch per group
+    std::vector<dsx_area_query> q;
+    for (auto &O : g.outs)
+        for (auto [x, z] : O->cand_xy) q.push_back(dsx_area_query{O->seed, x, z, 20000, 0});
+    if (q.empty()) return DSX_OK;
+    std::vector<dsx_area> a(q.size());
+    if (const int rc = dsx_area_queries(w, q.data(), q.size(), a.data())) return rc;
+    size_t k = 0;
+    for (auto &O : g.outs)
+        for (size_t c = 0; c < O->cand_xy.size(); c++) O->cand_area.push_back(a[k++]);
+    return DSX_OK;
 }
 
 void format_group(const Group &g, const dsx_search_params &P, GroupText &t, double *best) {
     char line[512];
     for (auto &O : g.outs) {
+        for (size_t c = 0; c < O->cand_area.size(); c++) {   // "<CANDIDATE line> => <MC-Measure result>"
+            std::snprintf(line, sizeof line, " => 1:4 Size: %llu | Blocks^2 Size: %llu\n",
+                          (unsigned long long)O->cand_area[c].cells, (unsigned long long)(16 * O->cand_area[c].cells));
+            t.verified.push_back(O->lines[c] + line);
+        }
         if (!O->scanned) {
             std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,0,0,0\n", (long long)O->seed, O->fy[0], O->fy[1],
                           O->maxta, O->maxtb);
@@ -842,7 +862,7 @@ void dsx_search_defaults(dsx_search_params *p) {
     d.multi = 1; d.exact_n = 2; d.exact_min = 75; d.hot_pre = 0.8; d.csv_min = 50; d.nms = 5000;
     d.refine_mb = 70; d.verify_mb = 80; d.study = 0; d.gpu = 0; d.gpucheck = 0; d.gscan = 0; d.gscancheck = 0;
     d.gate = 0; d.gate_phase = 0.06; d.gate_amp = 0.70; d.gate_pthr = 0.645; d.from_stdin = 0; d.workers = 0;
-    d.prepass = 0;
+    d.prepass = 0; d.mcverify = 0;
     *p = d;
 }
 
@@ -865,6 +885,9 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
         dsx::g_last_error = "cannot open " + pre + ".csv / .seeds.csv";
         return DSX_E_ARG;
     }
+    FILE *vout = P.mcverify ? std::fopen((pre + ".verified").c_str(), "a") : nullptr;
+    const size_t slash = pre.find_last_of('/');
+    const std::string run = slash == std::string::npos ? pre : pre.substr(slash + 1);
     static double global_best = 0;
     std::deque<std::unique_ptr<Group>> pending;   // submitted, in seed order
     std::string first_err;
@@ -887,6 +910,13 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
                     std::fputs(t.csv.c_str(), out);
                     std::fputs(t.seeds.c_str(), sout);
                     std::fputs(t.log.c_str(), stderr);
+                    if (vout && !t.verified.empty()) {
+                        char ts[32];
+                        const std::time_t now = std::time(nullptr);
+                        std::strftime(ts, sizeof ts, "%Y-%m-%dT%H:%M:%SZ", std::gmtime(&now));
+                        for (const std::string &v : t.verified) std::fprintf(vout, "%s %s %s", ts, run.c_str(), v.c_str());
+                        std::fflush(vout);
+                    }
                     counts->scanned += t.scanned;
                     counts->sites += t.sites;
                 }
@@ -949,6 +979,7 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
     }
     std::fclose(out);
     std::fclose(sout);
+    if (vout) std::fclose(vout);
     if (rc) dsx::g_last_error = first_err;
     return rc;
 }

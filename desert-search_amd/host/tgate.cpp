@@ -1,0 +1,123 @@
+// GPU seed gate tool on top of libdsx.so, with the command line of the reference's gate tool
+// (src/gpu/tgate.cu:1-7):
+//   tgate START COUNT [phase=0.10] [amp=0.70] [box=1 pthr=0.645 | box=0 margin=0.01] [sig=1.0] [check=0] [calib=N]
+// Seeds are splitmix64(START + k), k in [0, COUNT). Prints "seed amp" lines (the input of
+// `dsearch_desert ... stdin=1`) for the seeds passing the gate, in seed-index order.
+//   calib=N  "seed pre full phase" for the first N (by seed value) phase-passed seeds of the first 2^26
+//            indices (tgate.cu:53-85; the pre-gate threshold calibration), sorted by seed
+//   check=1  the first 4 passing seeds are re-evaluated from an explicit seed list (dsx_gate_amps), and
+//            both amplitudes are printed to stderr
+#include "../../include/dsx.h"
+#include "../csrc/dsx_rng.h"
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <thread>
+#include <vector>
+
+static int fail(const char *what, int rc) {
+    std::fprintf(stderr, "%s: %s %s\n", what, dsx_strerror(rc), dsx_last_error());
+    return 1;
+}
+
+int main(int argc, char **argv) {
+    const char *hwq = std::getenv("DSX_HW_QUEUES");
+    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "24", 1);
+    if (argc < 3) {
+        std::fprintf(stderr,
+                     "tgate START COUNT [phase=0.10] [amp=0.70] [box=1 pthr=0.645 | box=0 margin=0.01] [check=0] [calib=N]\n");
+        return 2;
+    }
+    const uint64_t start = std::stoull(argv[1]), count = std::stoull(argv[2]);
+    dsx_gate_params g{0.10f, 0.70f, 0.645f, 0.01f, 1.0f, 1};   // GateCfg defaults (gate.cuh:349-355)
+    int check = 0, calib = 0;
+    for (int i = 3; i < argc; i++) {
+        const std::string kv = argv[i];
+        const auto e = kv.find('=');
+        const std::string k = kv.substr(0, e);
+        const double v = std::stod(kv.substr(e + 1));
+        if (k == "phase") g.max_phase = float(v);
+        else if (k == "amp") g.amp = float(v);
+        else if (k == "margin") g.margin = float(v);
+        else if (k == "check") check = int(v);
+        else if (k == "calib") calib = int(v);
+        else if (k == "box") g.box = int(v);
+        else if (k == "sig") g.sig = float(v);
+        else if (k == "pthr") g.pthr = float(v);
+        else if (k == "tpb_pre" || k == "tpb_full" || k == "perseed") continue;   // launch shapes of the CUDA tool
+        else {
+            std::fprintf(stderr, "unknown key %s\n", k.c_str());
+            return 2;
+        }
+    }
+    dsx_ctx *ctx = nullptr;
+    int rc = dsx_create(0, &ctx);
+    if (rc) return fail("dsx_create", rc);
+    const uint64_t B = 1ull << 26;   // seed indices per batch, as tgate.cu:55
+    if (calib) {
+        // phase survivors of the first 2^26 seed indices (count is not used, as tgate.cu:57-60), the first
+        // `calib` of them by seed value, then their pre-gate and full amplitudes from the explicit list
+        const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::vector<uint64_t>> part(nth);
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nth; t++)
+            th.emplace_back([&, t] {
+                for (uint64_t k = B * t / nth; k < B * (t + 1) / nth; k++) {
+                    const uint64_t seed = dsx::mix64(start + k);
+                    if (dsx::temperature_phase(seed, nullptr) <= double(g.max_phase)) part[t].push_back(seed);
+                }
+            });
+        for (auto &x : th) x.join();
+        std::vector<uint64_t> seeds;
+        for (auto &p : part) seeds.insert(seeds.end(), p.begin(), p.end());
+        std::sort(seeds.begin(), seeds.end());
+        seeds.resize(std::min<size_t>(seeds.size(), size_t(calib)));
+        std::vector<float> pre(seeds.size()), full(seeds.size());
+        if ((rc = dsx_gate_amps(ctx, seeds.data(), seeds.size(), &g, pre.data(), full.data())))
+            return fail("dsx_gate_amps", rc);
+        for (size_t i = 0; i < seeds.size(); i++)
+            std::printf("%llu %.5f %.5f %.5f\n", (unsigned long long)seeds[i], pre[i], full[i],
+                        dsx::temperature_phase(seeds[i], nullptr));
+        dsx_destroy(ctx);
+        return 0;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<uint64_t> seeds(1u << 20);
+    std::vector<float> amps(1u << 20);
+    dsx_gate_counts tot{0, 0, 0};
+    int checked = 0;
+    for (uint64_t pos = 0; pos < count; pos += B) {
+        const uint64_t m = std::min(B, count - pos);
+        uint64_t n = 0;
+        dsx_gate_counts c;
+        rc = dsx_gate(ctx, start + pos, m, &g, seeds.data(), amps.data(), nullptr, seeds.size(), &n, &c);
+        if (rc == DSX_E_CAPACITY) {
+            seeds.resize(n);
+            amps.resize(n);
+            rc = dsx_gate(ctx, start + pos, m, &g, seeds.data(), amps.data(), nullptr, seeds.size(), &n, &c);
+        }
+        if (rc) return fail("dsx_gate", rc);
+        for (uint64_t i = 0; i < n; i++) {
+            std::printf("%llu %.5f\n", (unsigned long long)seeds[i], amps[i]);
+            if (check && checked < 4) {
+                float pre = 0, full = 0;
+                if ((rc = dsx_gate_amps(ctx, &seeds[i], 1, &g, &pre, &full))) return fail("dsx_gate_amps", rc);
+                std::fprintf(stderr, "  check seed %llu: gate amp %.5f explicit-list amp %.5f (pre-gate %.5f)\n",
+                             (unsigned long long)seeds[i], amps[i], full, pre);
+                checked++;
+            }
+        }
+        std::fflush(stdout);
+        tot.phase += c.phase;
+        tot.pre += c.pre;
+        tot.full += c.full;
+    }
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "tgate: %llu seed indices, %llu phase, %llu pre-gate, %llu gated in %.2f s\n",
+                 (unsigned long long)count, (unsigned long long)tot.phase, (unsigned long long)tot.pre,
+                 (unsigned long long)tot.full, sec);
+    dsx_destroy(ctx);
+    return 0;
+}

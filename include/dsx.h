@@ -146,6 +146,9 @@ enum { DSX_POINT_T = 16, DSX_POINT_PROXYPROB = 17 };   /* 17: DesertProxy::prob 
  * quarts, dz then dx from -3 to 3, by T(x, 0, z) with the first maximum kept (the hot-zone search of
  * dsearch_desert.cpp:328-332): outd[k] = its T, out8[k] = its index (dz + 3) * 7 + dx + 3. */
 enum { DSX_POINT_HOTTEST = 18 };
+/* what = DSX_POINT_BIOME: SurfaceSampler::biome (surface.hpp:79-83), the surface biome id at quart (x, 64, z)
+ * from the exact climate (grid lookup, else the nearest-box search), in out8[k] read as uint8_t. */
+enum { DSX_POINT_BIOME = 19 };
 int dsx_points(dsx_ctx *ctx, const uint64_t *seeds, uint64_t nseeds, const uint32_t *sidx, const int32_t *xs,
                const int32_t *zs, uint64_t n, int what, float cut, int8_t *out8, double *outd);
 
@@ -224,6 +227,10 @@ typedef struct {
     int32_t from_stdin;
     int32_t workers; /* pipeline workers, each with its own stream and device buffers (0: default 16) */
     int32_t prepass; /* 1: FP32 pre-pass + FP64 confirm in the floods and predicates (dsx_set_precision) */
+    int32_t mcverify; /* 1: verify every CANDIDATE in-process with the exact step-1 area query (half 20000,
+                       * tools/dflood.cpp) and append "<UTC time> <run> <CANDIDATE line> => 1:4 Size: N |
+                       * Blocks^2 Size: M" to PREFIX.verified: the line deploy/gpipe.sh:7-11 writes with
+                       * MC-Measure and scripts/log_finds.py:13 parses */
 } dsx_search_params;
 void dsx_search_defaults(dsx_search_params *p);
 
@@ -239,6 +246,7 @@ typedef struct {
 /* The per-seed pipeline of dsearch_desert.cpp:173-557 (gate -> tables/scan -> sites -> hot zones ->
  * proxy floods -> exact floods) with every numeric stage on the GPU. Appends rows to PREFIX.csv and
  * PREFIX.seeds.csv in the reference's formats (opened in append mode; the caller writes the headers),
+ * and with mcverify=1 the verified CANDIDATE lines to PREFIX.verified,
  * prints CANDIDATE / best lines to stderr as the reference does, and accumulates counts.
  * Rows of one seed are contiguous; seeds appear in batch order. study=1 is not supported (DSX_E_ARG). */
 int dsx_search(dsx_ctx *ctx, const dsx_search_params *p, int mode, uint64_t start, uint64_t count, const uint64_t *seeds,
@@ -263,6 +271,16 @@ int dsx_area_queries(dsx_ctx *ctx, const dsx_area_query *q, uint64_t n, dsx_area
  * Z = bounded(60000001) - 3e7, u = uniform(), cap = floor(10^(3 + 4u)) cells, half =
  * floor((sqrt(cap) - 1) / 2). caps may be null. */
 void dsx_area_workload(uint64_t gen_seed, uint64_t n, dsx_area_query *out, uint64_t *caps);
+
+/* ---------------------------------------------------------------- proxy-table Monte Carlo (tools/dfunc.cpp) */
+/* P(desert-passable | low-octave C, E) histograms at hot points, the input of the proxy table
+ * (tools/dfunc.cpp:26-75, Makefile:58-59 `make dtable`). Seeds mix(7k + 99991), k < nseeds, that pass the
+ * temperature phase filter (sum |fy - 1/2| <= 0.36); per seed, tries (x, z) = (bounded(15e6) - 7.5e6, ...)
+ * from Random(seed ^ 0x1234) until pts hot points (T at the shifted position >= 0.55) or 200 pts tries.
+ * hist[((set * 40 + ci) * 40 + ei) * 3 + {0, 1, 2}] = {hot points, desert, desert or bridgeable river},
+ * set 0: octave 0 of C and E, set 1: C octaves 0-2, E octaves 0-1; bins of 0.05 from -0.8 (C), -1 (E). */
+enum { DSX_DFUNC_CELLS = 2 * 40 * 40 * 3 };
+int dsx_dfunc(dsx_ctx *ctx, uint32_t nseeds, uint32_t pts, uint64_t *hist);
 
 /* ---------------------------------------------------------------- request server
  * A request loop on device-resident state (no counterpart in the reference, which runs one search per

@@ -222,6 +222,42 @@ void or_climate(uint64_t seed, int x, int z, int64_t *q) {
 }
 int or_biome(uint64_t seed, int x, int z) { Climate S(seed); return S.biome(x, z); }
 int or_classify(const int64_t *q) { return classify(q); }
+
+// The biome map of tools/biomemap.cpp:11-45 into out[n * n] (n = 2 * half / step + 1): biome id at quart
+// ((X - half + i * step) >> 2, (Z - half + j * step) >> 2), row j major; 255 on the 4-connected component
+// of the centre over desert_state != 0 where desert_state == 2. Returns n, or -1 if the centre is not
+// desert (out then holds the plain biome map).
+int or_biomemap(uint64_t seed, int X, int Z, int half, int step, uint8_t *out) {
+    const int n = 2 * half / step + 1;
+    Climate S(seed);
+    std::vector<uint8_t> st(size_t(n) * n), seen(size_t(n) * n);
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            const int qx = (X - half + i * step) >> 2, qz = (Z - half + j * step) >> 2;
+            out[size_t(j) * n + i] = uint8_t(S.biome(qx, qz));
+            st[size_t(j) * n + i] = uint8_t(S.desert_state(qx, qz));
+        }
+    const size_t c0 = size_t(n / 2) * n + n / 2;
+    if (st[c0] != 2) return -1;
+    // breadth-first here (the tool walks depth-first): the marked set is the component either way
+    std::vector<size_t> q{c0};
+    seen[c0] = 1;
+    for (size_t h = 0; h < q.size(); h++) {
+        const size_t c = q[h];
+        if (st[c] == 2) out[c] = 255;
+        const int ci = int(c % n), cj = int(c / n);
+        for (int k = 0; k < 4; k++) {
+            const int i = ci + (k == 0) - (k == 1), j = cj + (k == 2) - (k == 3);
+            if (i < 0 || j < 0 || i >= n || j >= n) continue;
+            const size_t d = size_t(j) * n + i;
+            if (!seen[d] && st[d]) {
+                seen[d] = 1;
+                q.push_back(d);
+            }
+        }
+    }
+    return n;
+}
 int or_grid_lookup(const int64_t *q) { return grid().lookup(q); }
 float or_spline(const float *v) { return spline(0, v); }
 

@@ -67,6 +67,8 @@ class Oracle:
             "or_pred_batch": (None, [u64, i32, f32, vp, vp, i64, vp]),
             "or_flood": (None, [u64, i32, f32, i32, i32, i32, i32, i32, vp, vp]),
             "or_run": (i32, [u64, u64, vp, i64, i32, i32, ctypes.c_char_p, vp, vp, vp, i32]),
+            "or_biome": (i32, [u64, i32, i32]),
+            "or_biomemap": (i32, [u64, i32, i32, i32, i32, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -95,6 +97,13 @@ class Oracle:
         V = np.zeros((G, G), np.float32)
         self.L.or_gate_grid(seed, side, G, box, V.ctypes.data)
         return V
+
+    def biomemap(self, seed, X, Z, half, step):
+        """(n, n x n uint8 map) as tools/biomemap.cpp writes it; n < 0 when the centre is not desert."""
+        m = 2 * half // step + 1
+        out = np.zeros((m, m), np.uint8)
+        n = self.L.or_biomemap(seed, X, Z, half, step, out.ctypes.data)
+        return n, out
 
     def flood(self, seed, kind, cut, qx, qz, step, half, conn8):
         out = FloodOut()

@@ -11,6 +11,10 @@ inputs and the reference's outputs.
                   rules over k in [1e12, 1e12 + 2^20); the list itself is an input fixture)
   dflood.jsonl    dflood SEED X Z STEP at the verified desert finds, steps 8 and 4 (all rows) and
                   step 1 (the first rows)
+  biomemap.npz    biomemap SEED X Z HALF STEP at verified desert finds (n x n uint8 maps; `python
+                  make_goldens.py biomemap` rewrites only this file)
+  dfunc_S_P.txt   dfunc S P 8 rows (proxy-table histograms) for (64, 300) and the Makefile's `make dtable`
+                  size (2000, 2000); `python make_goldens.py dfunc` rewrites only these
 Sorted CSV rows are stored (the reference writes rows in thread completion order).
 """
 import concurrent.futures as cf
@@ -58,7 +62,41 @@ def dflood(seed, x, z, step):
             "edge": "window edge" in out.stdout}
 
 
+BIOMEMAP_CASES = [(0, 2000, 16), (1, 2000, 16), (2, 6000, 40), (5, 1200, 8)]   # (find row, half, step)
+
+
+def biomemap_goldens():
+    import numpy as np
+    finds = [json.loads(l) for l in open(os.path.join(HERE, "desert_finds.jsonl"))]
+    arrs = {}
+    for row, half, step in BIOMEMAP_CASES:
+        r = finds[row]
+        args = [r["seed"], str(r["x"]), str(r["z"]), str(half), str(step)]
+        out = subprocess.run([os.path.join(REF, "biomemap")] + args, capture_output=True, check=True).stdout
+        n = int(np.frombuffer(out[:4], np.int32)[0])
+        arrs["map_%d_%d_%d" % (row, half, step)] = np.frombuffer(out[4:], np.uint8).reshape(n, n)
+        arrs["args_%d_%d_%d" % (row, half, step)] = np.array([int(a) for a in args], dtype=object).astype(str)
+    np.savez_compressed(os.path.join(HERE, "biomemap.npz"), **arrs)
+    print("biomemap", sorted(arrs))
+
+
+DFUNC_CASES = [(64, 300), (2000, 2000)]
+
+
+def dfunc_goldens():
+    for seeds, pts in DFUNC_CASES:
+        out = subprocess.run([os.path.join(REF, "dfunc"), str(seeds), str(pts), "8"], capture_output=True, text=True,
+                             check=True).stdout
+        with open(os.path.join(HERE, "dfunc_%d_%d.txt" % (seeds, pts)), "w") as f:
+            f.write(out)
+        print("dfunc", seeds, pts, len(out.splitlines()))
+
+
 def main():
+    if sys.argv[1:2] == ["biomemap"]:
+        return biomemap_goldens()
+    if sys.argv[1:2] == ["dfunc"]:
+        return dfunc_goldens()
     run_search("c1", ["0", "200000", "8"])
     run_search("r3m", ["0", "3000000", "8"])
     run_search("rg1m", ["0", "0", "8", "stdin=1"], stdin=os.path.join(HERE, "gated_1e12_1m.txt"))
@@ -73,6 +111,8 @@ def main():
         for r in res:
             f.write(json.dumps(r) + "\n")
     print("dflood", len(res))
+    biomemap_goldens()
+    dfunc_goldens()
 
 
 if __name__ == "__main__":

@@ -93,3 +93,42 @@ def test_gate_capacity_and_argument_errors(engine):
     assert e.value.code == dsx.DSX_E_ARG
     s, a, kk, c = engine.gate(5, 0)
     assert len(kk) == 0 and c == (0, 0, 0)
+
+
+def test_tgate_tool(engine, oracle):
+    """The tgate command line (tgate.cu:1-7): "seed amp" lines equal to dsx_gate's survivors, check=1
+    re-evaluates from an explicit list, calib=N gives "seed pre full phase" of the first N phase
+    survivors (by seed value) of 2^26 indices."""
+    import ctypes
+    import os
+    import subprocess
+    from conftest import REPO
+    exe = os.path.join(REPO, "desert-search_amd", "tgate")
+    start, count = 10 ** 12, 1 << 20
+    r = subprocess.run([exe, str(start), str(count), "check=1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    from conftest import load_package
+    dsx = load_package().dsx
+    seeds, amps, k, cnt = engine.gate(start, count, dsx.GateParams(max_phase=0.10))   # GateCfg::maxph default
+    assert r.stdout.splitlines() == ["%d %.5f" % (int(s), float(a)) for s, a in zip(seeds, amps)]
+    checks = [l for l in r.stderr.splitlines() if "check seed" in l]
+    assert len(checks) == min(4, len(seeds))
+    for l in checks:
+        w = l.split()
+        assert w[5] == w[8]   # gate amp == explicit-list amp
+    assert "%d seed indices, %d phase, %d pre-gate, %d gated" % ((count,) + tuple(cnt)) in r.stderr
+    n = 12
+    r = subprocess.run([exe, "5", "1", "calib=%d" % n, "phase=0.10"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    ks = oracle.phase_list(5, 1 << 26, 0.10)
+    want = sorted(oracle.or_mix(5 + int(x)) for x in ks)[:n]
+    lines = r.stdout.splitlines()
+    assert [int(l.split()[0]) for l in lines] == want
+    for l in lines:
+        s, p, f, ph = l.split()
+        assert p == "%.5f" % oracle.or_gate_amp(int(s), 256, 4, 1, 1.0)
+        assert f == "%.5f" % oracle.or_gate_amp(int(s), 512, 7, 0, 2.0)
+        fy = (ctypes.c_double * 2)()
+        assert ph == "%.5f" % oracle.or_phase(int(s), ctypes.byref(fy))
+    r = subprocess.run([exe, "0", "10", "bogus=1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "unknown key bogus" in r.stderr

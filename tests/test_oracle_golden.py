@@ -104,3 +104,41 @@ def test_dflood_step1_matches_mcmeasure(oracle):
     r = finds[0]
     out = _flood(oracle, int(r["seed"]), r["x"], r["z"], 1)
     assert out.cells == r["cells"] and not out.edge and not out.border
+
+
+def test_biomemap_matches_reference(oracle):
+    """The biome-map restatement against tools/biomemap.cpp's own output (biome ids, desert component)."""
+    d = np.load(os.path.join(GOLDEN, "biomemap.npz"))
+    for key in [k for k in d.files if k.startswith("map_")]:
+        seed, x, z, half, step = (int(a) for a in d["args_" + key[4:]])
+        n, m = oracle.biomemap(seed % (1 << 64), x, z, half, step)
+        assert n == d[key].shape[0]
+        assert np.array_equal(m, d[key]), key
+    n, _ = oracle.biomemap(1, 0, 0, 400, 16)   # the reference tool: "centre is not desert", exit 1
+    assert n == -1
+
+
+def test_gen_dtable_on_reference_rows():
+    """gen_dtable.py on the reference's dfunc rows: P = passable / total of set 1 with 3 decimals, 0 below
+    20 samples, in the layout gen_tables.py reads (its parser recovers the same 1600 values)."""
+    import importlib.util
+    import re
+    from conftest import REPO
+    spec = importlib.util.spec_from_file_location("gen_dtable", os.path.join(REPO, "desert-search_amd", "data",
+                                                                             "gen_dtable.py"))
+    gd = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gd)
+    rows = np.loadtxt(os.path.join(GOLDEN, "dfunc_2000_2000.txt"), dtype=np.int64)
+    tot = np.zeros((40, 40), np.int64)
+    ok = np.zeros((40, 40), np.int64)
+    r1 = rows[rows[:, 0] == 1]
+    np.add.at(tot, (r1[:, 1], r1[:, 2]), r1[:, 3])
+    np.add.at(ok, (r1[:, 1], r1[:, 2]), r1[:, 5])
+    text = gd.header(open(os.path.join(GOLDEN, "dfunc_2000_2000.txt")).read().splitlines())
+    assert "measured on %.1fM hot points" % (tot.sum() / 1e6) in text
+    assert "inline constexpr float desert_pass_prob[40][40] = {" in text
+    body = text[text.index("= {") + 3:]
+    vals = np.array([float(v) for v in re.findall(r"(-?\d+\.\d+)f", body)]).reshape(40, 40)
+    want = np.where(tot >= 20, ok / np.maximum(tot, 1), 0.0)
+    assert np.all(np.abs(vals - want) <= 0.0005 + 1e-12)
+    assert np.all(vals[tot < 20] == 0)

@@ -74,3 +74,33 @@ def test_cli_config1(engine):
         assert r2.returncode == 2 and "exist?" in r2.stderr
         r3 = subprocess.run([exe, "0", "10", pre + "x", "8", "bogus=1"], capture_output=True, text=True, timeout=60)
         assert r3.returncode == 2 and "unknown key bogus" in r3.stderr
+
+
+def test_mcverify_candidates(engine):
+    """mcverify=1: every CANDIDATE gets the exact step-1 area (half 20000) in PREFIX.verified, in the line
+    format scripts/log_finds.py:13 parses. Thresholds are lowered so that the exact floods run and every
+    step-4 result becomes a CANDIDATE; the areas equal dsx_area_queries at the same points."""
+    import re
+    import numpy as np
+    dsx = load_package().dsx
+    seeds = [int(l.split()[0]) % (1 << 64) for l in open(os.path.join(GOLDEN, "gated_1e12_1m.txt"))][:24]
+    p = engine.search_params(exact_min=0, refine_mb=0, verify_mb=0, mcverify=1)
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "v")
+        engine.search(pre, dsx.SEARCH_LIST, seeds=seeds, params=p)
+        lines = open(pre + ".verified").read().splitlines()
+    pat = re.compile(r'^(\S+) (\S+) CANDIDATE step4=([\d.]+)M seed=(-?\d+) x=(-?\d+) z=(-?\d+).*1:4 Size: (\d+) \| '
+                     r'Blocks\^2 Size: (\d+)')
+    assert len(lines) >= 10
+    q = np.zeros(len(lines), dsx.AREA_QUERY)
+    want = []
+    for k, l in enumerate(lines):
+        m = pat.search(l)
+        assert m, l
+        assert m.group(2) == "v"
+        cells, b2 = int(m.group(7)), int(m.group(8))
+        assert b2 == 16 * cells
+        q[k] = (int(m.group(4)) % (1 << 64), int(m.group(5)), int(m.group(6)), 20000, 0)
+        want.append(cells)
+    got = engine.area_queries(q)
+    assert [int(c) for c in got["cells"]] == want
