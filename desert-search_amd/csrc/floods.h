@@ -58,9 +58,12 @@ struct FloodBufs {
 };
 
 // ctl words: queue head/tail, pending items, overflow flags, exported masks, items processed, floods
-// handed over by the small-window engine, allocated tiles (B.ntiles points here)
+// handed over by the small-window engine, allocated tiles (B.ntiles points here); then what the first
+// workgroup that reached the time limit saw (for the error message): a flag, its slot, the queue head,
+// tail and pending count, and the slot's content
 constexpr int Q_HEAD = 0, Q_TAIL = 1, Q_PENDING = 2, FLOOD_OVF = 3, FLOOD_MASKS = 4, Q_ITEMS = 5, HANDED = 6,
-              CTL_NTILES = 7, CTL_WORDS = 8;
+              CTL_NTILES = 7, TIME_SEEN = 8, TIME_SLOT = 9, TIME_HEAD = 10, TIME_TAIL = 11, TIME_PENDING = 12,
+              TIME_ITEM = 13, CTL_WORDS = 16;
 constexpr uint32_t Q_EMPTY = 0xffffffffu;
 // FLOOD_OVF bits: 1 work queue full, 2 tile pool or hash table full, 4 time limit reached
 constexpr uint32_t OVF_QUEUE = 1, OVF_TILES = 2, OVF_TIME = 4;

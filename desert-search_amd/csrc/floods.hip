@@ -34,6 +34,7 @@
 #include "floods.h"
 #include "flood_dev.h"
 #include <hip/hip_ext.h>
+#include <algorithm>
 #include <climits>
 #include <cstdlib>
 
@@ -196,7 +197,8 @@ __device__ uint32_t prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J
 // Persistent workgroups over the work queue (NT threads each; wave 0 propagates, all waves evaluate).
 // OCC: waves per SIMD the register allocation must allow.
 template <int NT, int OCC, int PRE>
-__global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B) {
+__global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B,
+                                                       int backoff, int spec) {
     constexpr int NWAVE = NT / 64;
     __shared__ ClimSeed C;
     __shared__ uint32_t s_pass[FT], s_cnt[FT], s_hot[FT];
@@ -223,12 +225,24 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                             break;
                         }
                         if (__builtin_amdgcn_s_memrealtime() - t0 > 2000000000ull) {   // 20 s at 100 MHz
+                            if (atomicCAS(&B.ctl[TIME_SEEN], 0u, 1u) == 0u) {
+                                ast(&B.ctl[TIME_SLOT], k);
+                                ast(&B.ctl[TIME_HEAD], ald(&B.ctl[Q_HEAD]));
+                                ast(&B.ctl[TIME_TAIL], ald(&B.ctl[Q_TAIL]));
+                                ast(&B.ctl[TIME_PENDING], ald(&B.ctl[Q_PENDING]));
+                                ast(&B.ctl[TIME_ITEM], __hip_atomic_load(&B.wq[k], __ATOMIC_RELAXED,
+                                                                         __HIP_MEMORY_SCOPE_SYSTEM));
+                            }
                             atomicOr(&B.ctl[FLOOD_OVF], OVF_TIME);
                             stop = 1;
                             break;
                         }
                     }
-                    __builtin_amdgcn_s_sleep(4);
+                    // an idle workgroup polls its slot every ~256 cycles at first; with backoff it then
+                    // sleeps longer, leaving the CU's issue slots and the memory fabric to working waves
+                    if (backoff == 0 || it < 32u) __builtin_amdgcn_s_sleep(4);
+                    else if (backoff == 1 || it < 512u) __builtin_amdgcn_s_sleep(32);
+                    else __builtin_amdgcn_s_sleep(127);
                 }
                 if (item != Q_EMPTY) ast(&B.wq[k], Q_EMPTY);   // leave the slot clean for the next call
             }
@@ -261,8 +275,34 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                     if (tid == 0) s_need = need;
                 }
                 __syncthreads();
-                const uint32_t need = s_need;
+                uint32_t need = s_need;
                 if (!need) break;
+                if (spec) {
+                    // waves left idle by this round (the needed count is not a multiple of the waves) take
+                    // sub-blocks next to the needed ones that are not evaluated yet, inside the window: the
+                    // flood usually grows into them next, and evaluated cells are only a cache of the
+                    // predicate (what is reached and counted still follows the touched cells alone)
+                    int idle = (NWAVE - __popc(need) % NWAVE) % NWAVE;
+                    if (idle) {
+                        const int nw = int(nwin(J));
+                        uint32_t nb = 0;
+                        for (uint32_t m = need; m; m &= m - 1) {
+                            const int b = __ffs(m) - 1, bx = b % NSB, bz = b / NSB;
+                            if (bx > 0) nb |= 1u << (b - 1);
+                            if (bx < NSB - 1) nb |= 1u << (b + 1);
+                            if (bz > 0) nb |= 1u << (b - NSB);
+                            if (bz < NSB - 1) nb |= 1u << (b + NSB);
+                        }
+                        nb &= ~(need | s_evald);
+                        for (; idle && nb; nb &= nb - 1) {
+                            const int b = __ffs(nb) - 1;
+                            if (tx * FT + (b % NSB) * SB < nw && tz * FT + (b / NSB) * SB < nw) {
+                                need |= 1u << b;
+                                idle--;
+                            }
+                        }
+                    }
+                }
                 const int cls = J.kind == PRED_HOT ? 0 : (J.kind == PRED_PROXY ? 1 : 2);
                 if (J.seed != loaded_seed || cls > loaded_cls) {   // the octaves the predicate reads, in LDS
                     const ClimSeed &G = cs[J.seed];
@@ -446,23 +486,41 @@ __global__ __launch_bounds__(256) void k_flood_publish(const FloodRes *res, uint
 
 // ------------------------------------------------------------------ launchers
 
-constexpr int RUN_NT = 256;
-
 void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big, uint8_t *d_up,
                         FloodRes *res, FloodBufs B) {
     if (n) hipLaunchKernelGGL(k_flood_stage, dim3((n + 255) / 256), dim3(256), 0, st, h_up, n, n_small, n_big, d_up, res, B);
 }
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
                       uint32_t blocks, int pre, hipEvent_t ev0, hipEvent_t ev1) {
-    const dim3 grid(blocks ? blocks : 1);
+    // workgroup size (DSX_FLOOD_NT = 64 to 1024 threads); `blocks` is given for 256-thread
+    // workgroups and scaled so that the launch holds the same number of waves
+    static const int nt = [] {
+        const char *v = std::getenv("DSX_FLOOD_NT");
+        const int n = v && *v ? std::atoi(v) : 256;
+        return n == 64 || n == 128 || n == 512 || n == 1024 ? n : 256;
+    }();
     static const bool occ4 = [] {
         const char *v = std::getenv("DSX_FLOOD_OCC");
         return v && std::atoi(v) >= 4;
     }();
-    auto kern = pre ? (occ4 ? k_flood_run<RUN_NT, 4, 1> : k_flood_run<RUN_NT, 1, 1>)
-                    : (occ4 ? k_flood_run<RUN_NT, 4, 0> : k_flood_run<RUN_NT, 1, 0>);
-    if (ev0) hipExtLaunchKernelGGL(kern, grid, dim3(RUN_NT), 0, st, ev0, ev1, 0u, jobs, cs, g, B);
-    else hipLaunchKernelGGL(kern, grid, dim3(RUN_NT), 0, st, jobs, cs, g, B);
+    static const int backoff = [] {
+        const char *v = std::getenv("DSX_FLOOD_BACKOFF");
+        return v && *v ? std::atoi(v) : 0;
+    }();
+    static const int spec = [] {
+        const char *v = std::getenv("DSX_FLOOD_SPEC");   // 0 turns the speculative sub-blocks off
+        return v && *v ? std::atoi(v) : 1;
+    }();
+    decltype(&k_flood_run<256, 1, 0>) kern;
+    if (nt == 64) kern = pre ? k_flood_run<64, 1, 1> : k_flood_run<64, 1, 0>;
+    else if (nt == 128) kern = pre ? k_flood_run<128, 1, 1> : k_flood_run<128, 1, 0>;
+    else if (nt == 512) kern = pre ? k_flood_run<512, 1, 1> : k_flood_run<512, 1, 0>;
+    else if (nt == 1024) kern = pre ? k_flood_run<1024, 1, 1> : k_flood_run<1024, 1, 0>;
+    else kern = pre ? (occ4 ? k_flood_run<256, 4, 1> : k_flood_run<256, 1, 1>)
+                    : (occ4 ? k_flood_run<256, 4, 0> : k_flood_run<256, 1, 0>);
+    const dim3 grid(nt <= 256 ? (blocks ? blocks : 1) * (256 / nt) : std::max(1u, blocks * 256 / nt));
+    if (ev0) hipExtLaunchKernelGGL(kern, grid, dim3(nt), 0, st, ev0, ev1, 0u, jobs, cs, g, B, backoff, spec);
+    else hipLaunchKernelGGL(kern, grid, dim3(nt), 0, st, jobs, cs, g, B, backoff, spec);
 }
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRes *res, TileMask *masks, uint32_t mcap,
                       uint32_t blocks) {

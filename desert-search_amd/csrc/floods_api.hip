@@ -445,6 +445,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     // the tiled engine: the jobs listed now plus the small-window floods handed over on the device
     const bool prof = g_prof_on.load(std::memory_order_relaxed) != 0;
     const bool tiled = n_big > 0 || n_small > 0;
+    uint32_t blocks_used = 0;
     if (tiled) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (prof) {
@@ -460,6 +461,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         static const uint32_t per_job = uint32_t(env_u32("DSX_FLOOD_BLOCKS_PER_JOB", 4)),
                               max_blocks = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS", 128));
         const uint32_t blocks = uint32_t(std::clamp<size_t>(size_t(per_job) * n_big, 16, max_blocks));
+        blocks_used = blocks;
         dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, S->prepass, e0, e1);
         dsx::launch_flood_acc(c->st, d_jobs, B, S->d_res, S->d_masks, S->mask_cap, n_big ? 1024 : 64);
     }
@@ -476,7 +478,13 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         dsx::g_last_error = "flood tile pool or work queue exhausted: split the batch";
         if (h[dsx::FLOOD_OVF] & dsx::OVF_TIME) {
             status = DSX_E_HIP;
-            dsx::g_last_error = "flood engine time limit reached";
+            char msg[256];
+            std::snprintf(msg, sizeof msg,
+                          "flood engine time limit reached (slot %u: head %u tail %u pending %u item %08x; "
+                          "%u floods, %u tiled, %u workgroups)",
+                          h[dsx::TIME_SLOT], h[dsx::TIME_HEAD], h[dsx::TIME_TAIL], h[dsx::TIME_PENDING],
+                          h[dsx::TIME_ITEM], unsigned(n), n_big, blocks_used);
+            dsx::g_last_error = msg;
         }
         const uint32_t ntiles = std::min(h[dsx::CTL_NTILES], B.tile_cap), qtail = std::min(h[dsx::Q_TAIL], B.wq_cap);
         HIPCK(hipMemsetAsync(B.tiles, 0, size_t(ntiles) * sizeof(dsx::Tile), c->st));
