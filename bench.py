@@ -48,7 +48,7 @@ import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 # hardware queues for the 16 pipeline worker streams, the gate and spares (DSX_HW_QUEUES overrides)
-os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("DSX_HW_QUEUES", "24")
+os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("DSX_HW_QUEUES", "20")
 START = 10 ** 12
 N_STEP = 10 ** 8
 FP64_PEAK_TFLOPS = 78.6
@@ -268,7 +268,14 @@ def main():
                                             "by_size": [[st.size_floods[k][b], st.size_tiles[k][b],
                                                          st.size_sampled[k][b]] for b in range(6)]}
                                  for k in range(4) if st.floods[k]},
-                     "small_window_floods": st.small_floods, "small_window_sampled": st.small_sampled},
+                     "small_window_floods": st.small_floods, "small_window_sampled": st.small_sampled,
+                     "eval_rounds": st.eval_rounds,
+                     "needed_per_round": st.eval_needed / max(1, st.eval_rounds),
+                     # where the persistent kernel's workgroups spend their time (fractions of their lifetime)
+                     "workgroup_time": {"seconds": st.wg_life * 1e-8,
+                                        "waiting_for_items": st.wg_wait / max(1, st.wg_life),
+                                        "evaluating": st.wg_eval / max(1, st.wg_life),
+                                        "loading_octaves": st.wg_load / max(1, st.wg_life)}},
     }
     # the FP32 pre-pass variant over the same steps: throughput and identical rows
     pp = eng.search_params(gpu=1, gate=1, workers=a.workers, prepass=1)

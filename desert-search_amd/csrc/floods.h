@@ -60,10 +60,13 @@ struct FloodBufs {
 // ctl words: queue head/tail, pending items, overflow flags, exported masks, items processed, floods
 // handed over by the small-window engine, allocated tiles (B.ntiles points here); then what the first
 // workgroup that reached the time limit saw (for the error message): a flag, its slot, the queue head,
-// tail and pending count, and the slot's content
+// tail and pending count, and the slot's content; evaluation rounds of the tiled engine and the
+// sub-blocks they needed (before speculative ones are added); with profiling on, the workgroups' time
+// (100 MHz ticks) waiting for queue items, evaluating sub-blocks, loading octaves into LDS, and alive
 constexpr int Q_HEAD = 0, Q_TAIL = 1, Q_PENDING = 2, FLOOD_OVF = 3, FLOOD_MASKS = 4, Q_ITEMS = 5, HANDED = 6,
               CTL_NTILES = 7, TIME_SEEN = 8, TIME_SLOT = 9, TIME_HEAD = 10, TIME_TAIL = 11, TIME_PENDING = 12,
-              TIME_ITEM = 13, CTL_WORDS = 16;
+              TIME_ITEM = 13, EVAL_ROUNDS = 14, EVAL_NEED = 15, PROF_WAIT = 16, PROF_EVAL = 17, PROF_LOAD = 18,
+              PROF_LIFE = 19, CTL_WORDS = 24;
 constexpr uint32_t Q_EMPTY = 0xffffffffu;
 // FLOOD_OVF bits: 1 work queue full, 2 tile pool or hash table full, 4 time limit reached
 constexpr uint32_t OVF_QUEUE = 1, OVF_TILES = 2, OVF_TIME = 4;
@@ -79,7 +82,7 @@ void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_
 // a fixpoint and queues the neighbours it touched. ev0/ev1 (optional): HIP events around it.
 // pre: 1 = FP32 pre-pass on the predicates' temperature test (identical results)
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
-                      uint32_t blocks, int pre, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                      uint32_t blocks, int pre, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, int tprof = 0);
 // Reduce every allocated tile (count read on the device) into res, export counted masks, and return
 // the tiles and hash slots to the clean state.
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRes *res, struct TileMask *masks,

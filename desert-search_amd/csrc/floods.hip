@@ -198,8 +198,9 @@ __device__ uint32_t prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J
 // OCC: waves per SIMD the register allocation must allow.
 template <int NT, int OCC, int PRE>
 __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B,
-                                                       int backoff, int spec) {
+                                                       int backoff, int spec, int tprof) {
     constexpr int NWAVE = NT / 64;
+    const uint64_t t_born = tprof ? __builtin_amdgcn_s_memrealtime() : 0;
     __shared__ ClimSeed C;
     __shared__ uint32_t s_pass[FT], s_cnt[FT], s_hot[FT];
     __shared__ uint32_t s_item, s_again, s_need, s_evald;
@@ -212,6 +213,7 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
             uint32_t item = Q_EMPTY;
             int stop = 0;
             const uint32_t k = atomicAdd(&B.ctl[Q_HEAD], 1u);
+            const uint64_t tw0 = tprof ? __builtin_amdgcn_s_memrealtime() : 0;
             if (k >= B.wq_cap) {
                 stop = 1;
             } else {
@@ -248,6 +250,11 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
             }
             s_item = item;
             s_stop = stop;
+            if (tprof) {
+                const uint64_t now = __builtin_amdgcn_s_memrealtime();
+                atomicAdd(&B.ctl[PROF_WAIT], uint32_t(now - tw0));
+                if (stop) atomicAdd(&B.ctl[PROF_LIFE], uint32_t(now - t_born));
+            }
         }
         __syncthreads();
         if (s_stop) break;
@@ -277,6 +284,10 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                 __syncthreads();
                 uint32_t need = s_need;
                 if (!need) break;
+                if (tid == 0) {
+                    atomicAdd(&B.ctl[EVAL_ROUNDS], 1u);
+                    atomicAdd(&B.ctl[EVAL_NEED], uint32_t(__popc(need)));
+                }
                 if (spec) {
                     // waves left idle by this round (the needed count is not a multiple of the waves) take
                     // sub-blocks next to the needed ones that are not evaluated yet, inside the window: the
@@ -303,6 +314,7 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                         }
                     }
                 }
+                const uint64_t te0 = tprof && tid == 0 ? __builtin_amdgcn_s_memrealtime() : 0;
                 const int cls = J.kind == PRED_HOT ? 0 : (J.kind == PRED_PROXY ? 1 : 2);
                 if (J.seed != loaded_seed || cls > loaded_cls) {   // the octaves the predicate reads, in LDS
                     const ClimSeed &G = cs[J.seed];
@@ -325,13 +337,17 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                     __syncthreads();
                     loaded_seed = J.seed;
                     loaded_cls = cls;
+                    if (tprof && tid == 0) atomicAdd(&B.ctl[PROF_LOAD], uint32_t(__builtin_amdgcn_s_memrealtime() - te0));
                 }
                 // the k-th needed sub-block goes to wave k % NWAVE
                 int k = 0;
                 for (uint32_t m = need; m; m &= m - 1, k++)
                     if (k % NWAVE == wave) eval_subblock<PRE>(__ffs(m) - 1, tx, tz, J, C, g, s_pass, s_cnt, s_hot);
                 __syncthreads();
-                if (tid == 0) s_evald |= need;
+                if (tid == 0) {
+                    s_evald |= need;
+                    if (tprof) atomicAdd(&B.ctl[PROF_EVAL], uint32_t(__builtin_amdgcn_s_memrealtime() - te0));
+                }
                 __syncthreads();
             }
             if (tid < FT && s_evald != evald0) {   // newly evaluated cells: store the tile's masks
@@ -491,7 +507,7 @@ void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_
     if (n) hipLaunchKernelGGL(k_flood_stage, dim3((n + 255) / 256), dim3(256), 0, st, h_up, n, n_small, n_big, d_up, res, B);
 }
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
-                      uint32_t blocks, int pre, hipEvent_t ev0, hipEvent_t ev1) {
+                      uint32_t blocks, int pre, hipEvent_t ev0, hipEvent_t ev1, int tprof) {
     // workgroup size (DSX_FLOOD_NT = 64 to 1024 threads); `blocks` is given for 256-thread
     // workgroups and scaled so that the launch holds the same number of waves
     static const int nt = [] {
@@ -519,8 +535,8 @@ void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, 
     else kern = pre ? (occ4 ? k_flood_run<256, 4, 1> : k_flood_run<256, 1, 1>)
                     : (occ4 ? k_flood_run<256, 4, 0> : k_flood_run<256, 1, 0>);
     const dim3 grid(nt <= 256 ? (blocks ? blocks : 1) * (256 / nt) : std::max(1u, blocks * 256 / nt));
-    if (ev0) hipExtLaunchKernelGGL(kern, grid, dim3(nt), 0, st, ev0, ev1, 0u, jobs, cs, g, B, backoff, spec);
-    else hipLaunchKernelGGL(kern, grid, dim3(nt), 0, st, jobs, cs, g, B, backoff, spec);
+    if (ev0) hipExtLaunchKernelGGL(kern, grid, dim3(nt), 0, st, ev0, ev1, 0u, jobs, cs, g, B, backoff, spec, tprof);
+    else hipLaunchKernelGGL(kern, grid, dim3(nt), 0, st, jobs, cs, g, B, backoff, spec, tprof);
 }
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRes *res, TileMask *masks, uint32_t mcap,
                       uint32_t blocks) {

@@ -95,7 +95,8 @@ struct ClimState {
     // counted-cell tiles of the last dsx_flood call
     dsx::TileMask *d_masks = nullptr;
     uint32_t mask_cap = 0;
-    std::vector<dsx::TileMask> masks;
+    std::vector<dsx::TileMask> masks;   // masks beyond the pinned buffer's capacity
+    uint32_t masks_n = 0, masks_h = 0;  // masks of the last call; how many of them are in h_masks
     std::vector<hipEvent_t> events;   // event pair around the tiled engine's kernel (profiling on)
     int prepass = 0;                  // dsx_set_precision: 1 = FP32 pre-pass in floods and predicates
     uint32_t *d_ctl = nullptr;        // two blocks of control words; batch i uses block i % 2
@@ -262,6 +263,12 @@ static void stats_add(const dsx_flood_stats &a) {
     g_stats.small_sampled += a.small_sampled;
     g_stats.eval_ms += a.eval_ms;
     g_stats.eval_launches += a.eval_launches;
+    g_stats.eval_rounds += a.eval_rounds;
+    g_stats.eval_needed += a.eval_needed;
+    g_stats.wg_wait += a.wg_wait;
+    g_stats.wg_eval += a.wg_eval;
+    g_stats.wg_load += a.wg_load;
+    g_stats.wg_life += a.wg_life;
     for (int k = 0; k < 4; k++)
         for (int b = 0; b < 6; b++) {
             g_stats.size_floods[k][b] += a.size_floods[k][b];
@@ -382,11 +389,16 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
             int64_t(j.half) * j.step > 1000000)
             return DSX_E_ARG;
     }
-    if (n == 0) return DSX_OK;
+    if (n == 0) {
+        ClimState *S0 = clim_state(c);
+        S0->masks_n = S0->masks_h = 0;
+        return DSX_OK;
+    }
     HIPCK(hipSetDevice(c->device));
     ClimState *S;
     int rc = clim_alloc(c, S);
     if (rc) return rc;
+    S->masks_n = S->masks_h = 0;   // until this call's masks are published
     if (n > S->job_cap) {
         dsx::g_last_error = "too many floods in one call";
         return DSX_E_CAPACITY;
@@ -462,7 +474,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
                               max_blocks = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS", 128));
         const uint32_t blocks = uint32_t(std::clamp<size_t>(size_t(per_job) * n_big, 16, max_blocks));
         blocks_used = blocks;
-        dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, S->prepass, e0, e1);
+        dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, S->prepass, e0, e1, prof ? 1 : 0);
         dsx::launch_flood_acc(c->st, d_jobs, B, S->d_res, S->d_masks, S->mask_cap, n_big ? 1024 : 64);
     }
     dsx::launch_flood_publish(c->st, S->d_res, uint32_t(n), B, S->d_masks, S->h_res, S->h_ctl, S->h_masks, S->h_mask_cap);
@@ -498,17 +510,26 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         dsx::g_last_error = "flood mask export overflow";
         return DSX_E_CAPACITY;
     }
-    S->masks.resize(nm);
+    // the masks stay in the pinned buffer the publish kernel wrote (dsx_flood_masks copies them out); only
+    // those beyond its capacity are copied here
     const uint32_t nh = std::min(nm, S->h_mask_cap);
-    if (nh) std::memcpy(S->masks.data(), S->h_masks, size_t(nh) * sizeof(dsx::TileMask));
-    if (nm > nh) {   // more masks than the pinned buffer holds: the rest by copy
-        HIPCK(hipMemcpyAsync(S->masks.data() + nh, S->d_masks + nh, size_t(nm - nh) * sizeof(dsx::TileMask),
+    S->masks.resize(nm - nh);
+    if (nm > nh) {
+        HIPCK(hipMemcpyAsync(S->masks.data(), S->d_masks + nh, size_t(nm - nh) * sizeof(dsx::TileMask),
                              hipMemcpyDeviceToHost, c->st));
         HIPCK(hipStreamSynchronize(c->st));
     }
+    S->masks_n = nm;
+    S->masks_h = nh;
     dsx_flood_stats st{};
     st.calls = 1;
     st.rounds = h[dsx::Q_ITEMS];
+    st.eval_rounds = h[dsx::EVAL_ROUNDS];
+    st.eval_needed = h[dsx::EVAL_NEED];
+    st.wg_wait = h[dsx::PROF_WAIT];
+    st.wg_eval = h[dsx::PROF_EVAL];
+    st.wg_load = h[dsx::PROF_LOAD];
+    st.wg_life = h[dsx::PROF_LIFE];
     for (uint64_t k = 0; k < n; k++) {
         const int kind = hj[k].kind;
         if (res[k].tiles) {   // ran on the tiled engine
@@ -556,11 +577,12 @@ int dsx_flood_stats_get(dsx_flood_stats *out, int reset) {
 int dsx_flood_masks(dsx_ctx *c, dsx_tile_mask *out, uint64_t cap, uint64_t *n_out) {
     if (!c || !n_out || (cap && !out)) return DSX_E_ARG;
     ClimState *S = clim_state(c);
-    *n_out = S->masks.size();
-    const uint64_t w = std::min<uint64_t>(cap, S->masks.size());
+    *n_out = S->masks_n;
     static_assert(sizeof(dsx_tile_mask) == sizeof(dsx::TileMask), "mask layout");
-    if (w) std::memcpy(out, S->masks.data(), w * sizeof(dsx_tile_mask));
-    return S->masks.size() > cap ? DSX_E_CAPACITY : DSX_OK;
+    const uint64_t w = std::min<uint64_t>(cap, S->masks_n), wh = std::min<uint64_t>(w, S->masks_h);
+    if (wh) std::memcpy(out, S->h_masks, wh * sizeof(dsx_tile_mask));
+    if (w > wh) std::memcpy(out + wh, S->masks.data(), (w - wh) * sizeof(dsx_tile_mask));
+    return S->masks_n > cap ? DSX_E_CAPACITY : DSX_OK;
 }
 
 int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_t *sidx, const int32_t *xs,

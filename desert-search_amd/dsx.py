@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdsx.so")
 
 DSX_OK, DSX_E_ARG, DSX_E_HIP, DSX_E_CAPACITY, DSX_E_NODEVICE, DSX_E_BUSY = 0, 1, 2, 3, 4, 5
-HW_QUEUES = 24   # hardware queues requested from the HIP runtime (16 pipeline workers, the gate, spares)
+HW_QUEUES = 20   # hardware queues requested from the HIP runtime (16 pipeline workers, the gate, spares)
 
 
 class DsxError(RuntimeError):
@@ -110,7 +110,9 @@ class FloodStats(ctypes.Structure):
                 ("sampled_hot", ctypes.c_uint64 * 4), ("small_floods", ctypes.c_uint64),
                 ("small_sampled", ctypes.c_uint64), ("eval_ms", ctypes.c_double), ("eval_launches", ctypes.c_uint64),
                 ("size_floods", (ctypes.c_uint64 * 6) * 4), ("size_tiles", (ctypes.c_uint64 * 6) * 4),
-                ("size_sampled", (ctypes.c_uint64 * 6) * 4)]
+                ("size_sampled", (ctypes.c_uint64 * 6) * 4), ("eval_rounds", ctypes.c_uint64),
+                ("eval_needed", ctypes.c_uint64), ("wg_wait", ctypes.c_uint64), ("wg_eval", ctypes.c_uint64),
+                ("wg_load", ctypes.c_uint64), ("wg_life", ctypes.c_uint64)]
 
 
 class GateCounts(ctypes.Structure):
@@ -132,7 +134,8 @@ def load(path=LIB_PATH):
     if not os.path.exists(path):
         raise DsxError(DSX_E_NODEVICE, "libdsx.so not built", path)
     # hardware queues for the pipeline worker streams, the gate and spares, read by the HIP runtime when it starts
-    # (DSX_HW_QUEUES overrides the count)
+    # (DSX_HW_QUEUES overrides the count). Streams beyond it share queues; with 32 queues the persistent flood kernel
+    # has run into its time limit; 20 measured as fast as 24
     os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("DSX_HW_QUEUES", str(HW_QUEUES))
     lib = ctypes.CDLL(path)
     lib.dsx_strerror.restype = ctypes.c_char_p

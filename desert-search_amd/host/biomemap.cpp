@@ -20,7 +20,7 @@ static int fail(const char *what, int rc) {
 
 int main(int argc, char **argv) {
     const char *hwq = std::getenv("DSX_HW_QUEUES");
-    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "24", 1);
+    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "20", 1);
     if (argc < 4) {
         std::fprintf(stderr, "biomemap SEED X Z [HALF_BLOCKS] [STEP_BLOCKS]\n");
         return 2;

@@ -12,7 +12,7 @@
 
 int main(int argc, char **argv) {
     const char *hwq = std::getenv("DSX_HW_QUEUES");
-    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "24", 1);
+    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "20", 1);
     if (argc < 4) {
         std::fprintf(stderr, "dfunc SEEDS POINTS THREADS\n");
         return 2;

@@ -20,9 +20,10 @@ static void on_signal(int) { interrupted = 1; }
 
 int main(int argc, char **argv) {
     // hardware queues for the pipeline worker streams, the gate and spares, read by the HIP runtime when it starts
-    // (DSX_HW_QUEUES overrides the count)
+    // (DSX_HW_QUEUES overrides the count). Streams beyond it share queues; with 32 queues the persistent flood
+    // kernel has run into its time limit; 20 measured as fast as 24
     const char *hwq = std::getenv("DSX_HW_QUEUES");
-    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "24", 1);
+    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "20", 1);
     if (argc < 4) {
         std::fprintf(stderr, "dsearch_desert START COUNT OUT_PREFIX [threads] [key=value...]\n");
         return 2;

@@ -25,6 +25,7 @@
 #include <ctime>
 #include <exception>
 #include <map>
+#include <optional>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -95,17 +96,24 @@ struct FloodOut {
 // whose counted cells were inserted. A flood's keys are an affine image of its window lattice, so a
 // key is in the set iff the cell it maps back to is counted in one of the floods.
 struct KeySet {
-    struct Cover { const FloodOut *f; int shift, sub, bx, bz; };
+    // kx0..kx1, kz0..kz1: the key range of the flood's counted cells (its bbox, plus the sub-points)
+    struct Cover { const FloodOut *f; int shift, sub, bx, bz, kx0, kx1, kz0, kz1; };
     std::vector<Cover> covers;
     // insert every counted cell of f, lattice point x0 + (i - half) * step plus `sub` sub-points per axis
     // spaced 2^shift (sub = 2: the step-256 pre-flood's 4 keys per cell, dsearch_desert.cpp:345)
     void add(const FloodOut &f, int shift, int sub) {
         covers.push_back({&f, shift, sub, (f.qx - f.half * f.step + 8000000) >> shift,
-                          (f.qz - f.half * f.step + 8000000) >> shift});
+                          (f.qz - f.half * f.step + 8000000) >> shift, (f.a.minx + 8000000) >> shift,
+                          ((f.a.maxx + 8000000) >> shift) + sub - 1, (f.a.minz + 8000000) >> shift,
+                          ((f.a.maxz + 8000000) >> shift) + sub - 1});
     }
     bool has(int x, int z) const {
+        const int kx = (x + 8000000), kz = (z + 8000000);
         for (const Cover &c : covers) {
-            const int mx = ((x + 8000000) >> c.shift) - c.bx, mz = ((z + 8000000) >> c.shift) - c.bz;
+            if (c.f->a.cells == 0) continue;
+            const int qx = kx >> c.shift, qz = kz >> c.shift;
+            if (qx < c.kx0 || qx > c.kx1 || qz < c.kz0 || qz > c.kz1) continue;
+            const int mx = qx - c.bx, mz = qz - c.bz;
             if (mx < 0 || mz < 0) continue;
             if (c.f->counted(mx / c.sub, mz / c.sub)) return true;
         }
@@ -141,20 +149,28 @@ struct FloodReq {
 
 // Wall-time accounting of the search stages, printed to stderr when DSX_PROFILE is set.
 struct Prof {
-    double t[16] = {};
-    uint64_t n[16] = {};
+    double t[20] = {};
+    uint64_t n[20] = {};
     ~Prof() {
         if (!std::getenv("DSX_PROFILE")) return;
-        static const char *name[16] = {"gate", "scan", "sites", "points", "floods", "rows", "steps", "flood jobs",
+        static const char *name[20] = {"gate", "scan", "sites", "points", "floods", "rows", "steps", "flood jobs",
                                        "make_sites", "resume", "dsx_flood", "masks", "steps<=2", "steps<=8",
-                                       "steps<=32", "steps>32"};
-        for (int i = 0; i < 16; i++)
+                                       "steps<=32", "steps>32", "zone starts", "proxy rounds", "site loop",
+                                       "exact"};
+        for (int i = 0; i < 20; i++)
             if (t[i] > 0 || n[i]) std::fprintf(stderr, "dsx_search profile: %-10s %8.3f s  %llu\n", name[i], t[i], (unsigned long long)n[i]);
         dsx_flood_stats f;
         dsx_flood_stats_get(&f, 0);
         std::fprintf(stderr, "dsx_flood profile: %llu calls, %llu rounds; small-window floods %llu (%llu sampled)\n",
                      (unsigned long long)f.calls, (unsigned long long)f.rounds, (unsigned long long)f.small_floods,
                      (unsigned long long)f.small_sampled);
+        std::fprintf(stderr, "dsx_flood profile: %llu evaluation rounds, %.2f sub-blocks needed per round\n",
+                     (unsigned long long)f.eval_rounds, f.eval_rounds ? double(f.eval_needed) / f.eval_rounds : 0.0);
+        if (f.wg_life)
+            std::fprintf(stderr, "dsx_flood profile: workgroup time %.3f s: waiting %.1f%%, evaluating %.1f%% (octave "
+                         "loads %.1f%%), rest %.1f%%\n", f.wg_life * 1e-8, 100.0 * f.wg_wait / f.wg_life,
+                         100.0 * f.wg_eval / f.wg_life, 100.0 * f.wg_load / f.wg_life,
+                         100.0 * (double(f.wg_life) - f.wg_wait - f.wg_eval) / f.wg_life);
         static const char *kind[4] = {"desert_state", "hot", "proxy", "desert"};
         for (int k = 0; k < 4; k++)
             if (f.floods[k])
@@ -259,6 +275,10 @@ struct Sched {
             dsx_flood_masks(ctx, nullptr, 0, &nm);
             masks.resize(nm);
             if (nm && (rc = dsx_flood_masks(ctx, masks.data(), nm, &nm))) return rc;
+            std::vector<uint32_t> cnt(hi - lo, 0);
+            for (const dsx_tile_mask &m : masks) cnt[m.job]++;
+            for (size_t k = lo; k < hi; k++)
+                if (cnt[k - lo] && who[k].first->want_cells) who[k].first->out[who[k].second].tiles.reserve(cnt[k - lo]);
             for (const dsx_tile_mask &m : masks) {
                 auto [r, k] = who[lo + m.job];
                 if (r->want_cells) r->out[k].tiles.push_back(m);
@@ -468,6 +488,7 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
     std::vector<std::vector<XY>> zone_starts(h128.jobs.size());
     PointsReq pxq{&S, seed, DSX_PRED_PROXY, pcut, {}, {}, {}, {}, {}};
     std::vector<size_t> px_off(h128.jobs.size() + 1, 0);
+    std::optional<Timer> tz(16);
     for (size_t z = 0; z < h128.jobs.size(); z++) {
         const FloodOut &hzn = h128.out[z];
         const double area = hzn.a.cells * 128.0 * 128 * 16 / 1e6;
@@ -484,6 +505,7 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
         }
     }
     px_off[h128.jobs.size()] = pxq.xs.size();
+    tz.reset();
     co_await pxq;
     // Proxy floods ahead of the loop, in rounds: each round floods, per zone, the first passable start
     // that no flood so far covers. A start the loop will flood is one that no earlier flood covers, so
@@ -491,26 +513,76 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
     std::map<XY, const FloodOut *> prox_idx;
     std::vector<std::unique_ptr<FloodReq>> prox_rounds;
     KeySet spec_cov;
+    // Each zone's primary sequence is what the loop below floods: its first passable start that no
+    // primary flood covers, then the next, ... (the covered set only grows, so each zone keeps a cursor).
+    // A round requests every zone's next primary start; a zone still open after r rounds also requests
+    // up to 2^r - 1 further open starts (at most 7, DSX_PROXY_EXTRA), each 1024 quarts (DSX_PROXY_SEP)
+    // from the others, as extras. An extra's result
+    // only fills prox_idx: when the primary sequence reaches a start whose flood is already there, it is
+    // taken without a round, so zones with many components need fewer rounds.
+    std::vector<size_t> cursor(zone_starts.size(), 0);
+    std::vector<int> zrounds(zone_starts.size(), 0);
+    std::vector<uint8_t> is_primary;   // per request of the current round
+    static const int extra_cap = int(env_u64("DSX_PROXY_EXTRA", 7)), extra_sep = int(env_u64("DSX_PROXY_SEP", 1024));
     for (;;) {
+        std::optional<Timer> tr(17);
         auto req = std::make_unique<FloodReq>(FloodReq{&S, {}, true, {}, {}});
-        for (size_t z = 0; z < zone_starts.size(); z++)
-            for (size_t k = 0; k < zone_starts[z].size(); k++) {
-                const XY st = zone_starts[z][k];
-                if (pxq.v8[px_off[z] + k] != 2 || prox_idx.count(st) || spec_cov.has(st.first, st.second)) continue;
-                bool dup = false;
-                for (auto &j : req->jobs) dup |= (j.qx == st.first && j.qz == st.second);
-                if (!dup) req->jobs.push_back(fjob(seed, DSX_PRED_PROXY, pcut, st.first, st.second, 32, 16000 / 32, 1));
-                break;
+        is_primary.clear();
+        auto requested = [&](const XY &st) {
+            for (auto &j : req->jobs)
+                if (j.qx == st.first && j.qz == st.second) return true;
+            return false;
+        };
+        for (size_t z = 0; z < zone_starts.size(); z++) {
+            const size_t nz = zone_starts[z].size();
+            size_t &k = cursor[z];
+            auto open = [&](size_t j) {
+                const XY st = zone_starts[z][j];
+                return pxq.v8[px_off[z] + j] == 2 && !spec_cov.has(st.first, st.second);
+            };
+            // advance the primary sequence through starts already flooded
+            for (; k < nz; k++) {
+                if (!open(k)) continue;
+                auto it = prox_idx.find(zone_starts[z][k]);
+                if (it == prox_idx.end()) break;
+                spec_cov.add(*it->second, 5, 1);
             }
+            if (k >= nz) continue;
+            const XY st = zone_starts[z][k];
+            std::vector<XY> picks{st};
+            if (!requested(st)) {
+                req->jobs.push_back(fjob(seed, DSX_PRED_PROXY, pcut, st.first, st.second, 32, 16000 / 32, 1));
+                is_primary.push_back(1);
+            } else {
+                for (size_t j = 0; j < req->jobs.size(); j++)
+                    if (req->jobs[j].qx == st.first && req->jobs[j].qz == st.second) is_primary[j] = 1;
+            }
+            const int extra = std::min((1 << std::min(zrounds[z], 16)) - 1, extra_cap);
+            for (size_t j = k + 1; j < nz && int(picks.size()) <= extra; j++) {
+                const XY sj = zone_starts[z][j];
+                if (!open(j) || prox_idx.count(sj) || requested(sj)) continue;
+                bool near = false;
+                for (const XY &p : picks)
+                    near |= std::max(std::abs(p.first - sj.first), std::abs(p.second - sj.second)) < extra_sep;
+                if (near) continue;
+                picks.push_back(sj);
+                req->jobs.push_back(fjob(seed, DSX_PRED_PROXY, pcut, sj.first, sj.second, 32, 16000 / 32, 1));
+                is_primary.push_back(0);
+            }
+            zrounds[z]++;
+        }
         if (req->jobs.empty()) break;
+        tr.reset();
         co_await *req;
+        tr.emplace(17);
         for (size_t k = 0; k < req->jobs.size(); k++) {
             prox_idx[{req->jobs[k].qx, req->jobs[k].qz}] = &req->out[k];
-            spec_cov.add(req->out[k], 5, 1);
+            if (is_primary[k]) spec_cov.add(req->out[k], 5, 1);
         }
         prox_rounds.push_back(std::move(req));
     }
 
+    std::optional<Timer> tl(18);
     for (size_t rank = 0; rank < ns; rank++) {
         const Site &st = sites[rank];
         std::vector<XY> starts;
@@ -546,7 +618,9 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
                 q.xs.push_back(bx);
                 q.zs.push_back(bz);
             }
+            tl.reset();
             co_await q;
+            tl.emplace(18);
             start_px = q.v8;
         }
         // ---- proxy floods (dsearch_desert.cpp:384-400)
@@ -560,7 +634,9 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
                 a = it->second;
             } else {
                 FloodReq f{&S, {fjob(seed, DSX_PRED_PROXY, pcut, bx, bz, 32, 16000 / 32, 1)}, true, {}, {}};
+                tl.reset();
                 co_await f;
+                tl.emplace(18);
                 keep.push_back(std::make_unique<FloodOut>(std::move(f.out[0])));
                 a = keep.back().get();
             }
@@ -573,6 +649,7 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
             ev.push_back(e);
         }
     }
+    tl.reset();
     // ---- exact floods of the best proxies (dsearch_desert.cpp:494-533). The ev entries are
     // independent: ring searches of all of them in one batch, then their step-8 and step-4 floods.
     std::sort(ev.begin(), ev.end(), [](const Eval &a, const Eval &b) { return a.proxy > b.proxy; });
@@ -649,10 +726,11 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
 
 // ------------------------------------------------------------------ seeds -> sites (S1-S2 on the GPU)
 // dsearch_desert.cpp:227, 274-277, 301-308
-void make_sites(const std::vector<dsx_raw_cand> &raw, const dsx_search_params &P, SeedOut &O) {
+void make_sites(const dsx_raw_cand *raw, size_t nraw, const dsx_search_params &P, SeedOut &O) {
     std::vector<Site> cand;
     std::unordered_map<uint64_t, uint32_t> bucket;
-    for (const dsx_raw_cand &r : raw) {
+    for (size_t i = 0; i < nraw; i++) {
+        const dsx_raw_cand &r = raw[i];
         const uint64_t key = (uint64_t(uint32_t((r.x + 8000000) / P.nms)) << 32) | uint32_t((r.z + 8000000) / P.nms);
         auto [it, fresh] = bucket.try_emplace(key, uint32_t(cand.size()));
         if (fresh) cand.push_back({r.s, r.t, r.c, r.e, r.x, r.z});
@@ -683,11 +761,12 @@ int group_start(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g, Sche
     std::vector<float> mta(nb), mtb(nb);
     std::vector<uint64_t> ncand(nb);
     uint64_t cap = 1 << 12;
-    std::vector<dsx_raw_cand> raw(nb * cap);
+    thread_local std::vector<dsx_raw_cand> raw;   // reused: dsx_scan writes the ncand[k] entries read below
+    if (raw.size() < nb * cap) raw.resize(nb * cap);
     int rc = dsx_scan(w, g.seeds.data(), nb, &sp, mta.data(), mtb.data(), raw.data(), cap, ncand.data());
     if (rc == DSX_E_CAPACITY) {
         cap = *std::max_element(ncand.begin(), ncand.end());
-        raw.assign(nb * cap, dsx_raw_cand{});
+        if (raw.size() < nb * cap) raw.resize(nb * cap);
         rc = dsx_scan(w, g.seeds.data(), nb, &sp, mta.data(), mtb.data(), raw.data(), cap, ncand.data());
     }
     if (rc) return rc;
@@ -699,9 +778,8 @@ int group_start(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g, Sche
         O->maxtb = mtb[k];
         if (!(mode == DSX_SEARCH_ENUM && O->maxta + O->maxtb < P.amp)) {
             O->scanned = true;
-            std::vector<dsx_raw_cand> r(raw.begin() + k * cap, raw.begin() + k * cap + ncand[k]);
             Timer tm(8);
-            make_sites(r, P, *O);
+            make_sites(raw.data() + k * cap, ncand[k], P, *O);
         }
         g.outs.push_back(std::move(O));
     }

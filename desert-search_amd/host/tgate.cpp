@@ -24,7 +24,7 @@ static int fail(const char *what, int rc) {
 
 int main(int argc, char **argv) {
     const char *hwq = std::getenv("DSX_HW_QUEUES");
-    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "24", 1);
+    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "20", 1);
     if (argc < 3) {
         std::fprintf(stderr,
                      "tgate START COUNT [phase=0.10] [amp=0.70] [box=1 pthr=0.645 | box=0 margin=0.01] [check=0] [calib=N]\n");

@@ -207,6 +207,12 @@ typedef struct {
     /* tiled-engine floods by kind and size class (tiles per flood <= 1, 4, 16, 64, 256, more):
      * floods, tiles and sampled cells */
     uint64_t size_floods[4][6], size_tiles[4][6], size_sampled[4][6];
+    uint64_t eval_rounds;          /* tile evaluation rounds of the tiled engine */
+    uint64_t eval_needed;          /* 8x8 sub-blocks those rounds needed (speculative ones not counted) */
+    /* with profiling on (dsx_profile): workgroup time of the tiled engine in 100 MHz ticks, summed over
+     * workgroups: waiting for a queue item, evaluating sub-blocks (octave loads included), loading octaves
+     * into LDS, and from start to exit */
+    uint64_t wg_wait, wg_eval, wg_load, wg_life;
 } dsx_flood_stats;
 /* 1: bracket every launch of the tiled engine's kernel with HIP events on its stream (eval_ms). */
 int dsx_profile(int on);
