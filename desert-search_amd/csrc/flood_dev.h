@@ -28,6 +28,20 @@ template <class T> __device__ __forceinline__ void ast(T *p, T v) {
 // (an agent-scope release fence would write back the whole L2 at every hand-over).
 __device__ __forceinline__ void mem_done() { __builtin_amdgcn_s_waitcnt(0); }
 
+// Flood results and exported masks are written and read through the same agent-scope accesses as the
+// rest of the engine's shared state, word by word.
+__device__ __forceinline__ void res_store(FloodRes *p, const FloodRes &r) {
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(&r);
+    uint32_t *d = reinterpret_cast<uint32_t *>(p);
+#pragma unroll
+    for (int i = 0; i < int(sizeof(FloodRes) / 4); i++) ast(&d[i], s[i]);
+}
+__device__ __forceinline__ void mask_head_store(TileMask &m, uint32_t job, int32_t tx, int32_t tz) {
+    ast(&m.job, job);
+    ast(&m.tx, tx);
+    ast(&m.tz, tz);
+}
+
 constexpr unsigned long long HEMPTY = ~0ull;
 __device__ __forceinline__ unsigned long long hkey(uint32_t job, int tx, int tz) {
     return (uint64_t(job) << 44) | (uint64_t(uint32_t(tx) & 2047u) << 33) | (uint64_t(uint32_t(tz) & 2047u) << 22);

@@ -171,7 +171,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
             FloodRes r{};
             r.minx = r.maxx = J.qx;
             r.minz = r.maxz = J.qz;
-            res[job] = r;
+            res_store(&res[job], r);
             atomicAdd(&B.ctl[HANDED], 1u);
             flood_start_tile(J, job, B);
         }
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         r.hot = hotc;
         r.tiles = 0;
         r.evald = sampled;
-        res[job] = r;
+        res_store(&res[job], r);
     }
     if (cells == 0) return;
     // ---- counted cells as 32x32 tile masks (window tile coordinates) over the counted bounding box
@@ -212,12 +212,8 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
             if (lane == 0) k = atomicAdd(mcount, 1u);
             k = __shfl(k, 0);
             if (k >= mcap) continue;
-            if (lane < 32) masks[k].rows[lane] = row;
-            if (lane == 0) {
-                masks[k].job = job;
-                masks[k].tx = tx;
-                masks[k].tz = tz;
-            }
+            if (lane < 32) ast(&masks[k].rows[lane], row);
+            if (lane == 0) mask_head_store(masks[k], job, tx, tz);
         }
 }
 

@@ -30,7 +30,8 @@
 // Termination. Q_PENDING counts queued items not yet finished (an item's own pushes are counted
 // before it finishes), so it reaches 0 exactly when no more work can appear. A workgroup claims the
 // next queue slot and polls it (each waiter on its own address); it leaves when Q_PENDING is 0, when
-// an overflow was flagged, or after a time limit (flagged as an error), so every wave exits.
+// an overflow was flagged, or after a time limit of 5 s (flagged as an error; the host then runs the batch
+// once more on a reset engine), so every wave exits.
 #include "floods.h"
 #include "flood_dev.h"
 #include <hip/hip_ext.h>
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(256) void k_flood_stage(const uint8_t *h_up, uint32
         FloodRes r{};
         r.minx = r.maxx = J.qx;
         r.minz = r.maxz = J.qz;
-        res[q] = r;
+        res_store(&res[q], r);
         if (q < n_small) dl[q] = hl[q];
     }
     if (q < n_big) {
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                             stop = 1;
                             break;
                         }
-                        if (__builtin_amdgcn_s_memrealtime() - t0 > 2000000000ull) {   // 20 s at 100 MHz
+                        if (__builtin_amdgcn_s_memrealtime() - t0 > 500000000ull) {   // 5 s at 100 MHz
                             if (atomicCAS(&B.ctl[TIME_SEEN], 0u, 1u) == 0u) {
                                 ast(&B.ctl[TIME_SLOT], k);
                                 ast(&B.ctl[TIME_HEAD], ald(&B.ctl[Q_HEAD]));
@@ -381,45 +382,47 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
 __device__ void acc_tile(const FloodJob *jobs, FloodBufs B, uint32_t b, FloodRes *res, TileMask *masks, uint32_t *mcount,
                          uint32_t mcap) {
     Tile &T = B.tiles[b];
-    const FloodJob J = jobs[T.job];
+    const uint32_t job = ald(&T.job);
+    const int tx = ald(&T.tx), tz = ald(&T.tz);
+    const FloodJob J = jobs[job];
     const int lane = threadIdx.x, r = lane & 31;
     const bool c8 = J.conn8 != 0;
-    const uint32_t R = lane < FT ? T.reach[r] : 0u, I = lane < FT ? T.in[r] : 0u;
+    const uint32_t R = lane < FT ? ald(&T.reach[r]) : 0u, I = lane < FT ? ald(&T.in[r]) : 0u;
     const uint32_t up = __shfl(R, (r + 31) & 31, 32), dn = __shfl(R, (r + 1) & 31, 32);
     const uint32_t upv = r > 0 ? up : 0u, dnv = r < FT - 1 ? dn : 0u;
     uint32_t touched = R | I | (R << 1) | (R >> 1) | upv | dnv;
     if (c8) touched |= (upv << 1) | (upv >> 1) | (dnv << 1) | (dnv >> 1);
     const int nw = int(nwin(J));
-    const int j = T.tz * FT + r;
-    uint32_t win = (lane < FT && j < nw) ? col_mask(J, T.tx) : 0u;
+    const int j = tz * FT + r;
+    uint32_t win = (lane < FT && j < nw) ? col_mask(J, tx) : 0u;
     touched &= win;
     // world mask of this row
     const int z = J.qz + (j - J.half) * J.step;
     uint32_t world = 0;
     if (z >= -WORLD_Q && z <= WORLD_Q) {
         for (int c = 0; c < FT; c++) {
-            const int x = J.qx + (T.tx * FT + c - J.half) * J.step;
+            const int x = J.qx + (tx * FT + c - J.half) * J.step;
             if (x >= -WORLD_Q && x <= WORLD_Q) world |= 1u << c;
         }
     }
     unsigned long long sampled = __popc(touched & world);
-    unsigned long long hotc = __popc(touched & world & (lane < FT ? T.hot[r] : 0u));
+    unsigned long long hotc = __popc(touched & world & (lane < FT ? ald(&T.hot[r]) : 0u));
     const bool border = (touched & ~world) != 0;
-    const uint32_t RC = R & (lane < FT ? T.cnt[r] : 0u);
+    const uint32_t RC = R & (lane < FT ? ald(&T.cnt[r]) : 0u);
     unsigned long long cells = __popc(RC);
     int minx = INT_MAX, maxx = INT_MIN, minz = INT_MAX, maxz = INT_MIN;
     if (RC) {
         const int c0 = __ffs(RC) - 1, c1 = 31 - __clz(RC);
-        minx = J.qx + (T.tx * FT + c0 - J.half) * J.step;
-        maxx = J.qx + (T.tx * FT + c1 - J.half) * J.step;
+        minx = J.qx + (tx * FT + c0 - J.half) * J.step;
+        maxx = J.qx + (tx * FT + c1 - J.half) * J.step;
         minz = maxz = z;
     }
     // a reached cell on the window boundary has a neighbour outside the window
     bool edge = false;
     if (R) {
         if (j == 0 || j == nw - 1) edge = true;
-        if (T.tx == 0 && (R & 1u)) edge = true;
-        const int last = nw - 1 - T.tx * FT;
+        if (tx == 0 && (R & 1u)) edge = true;
+        const int last = nw - 1 - tx * FT;
         if (last >= 0 && last < FT && ((R >> last) & 1u)) edge = true;
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -433,9 +436,9 @@ __device__ void acc_tile(const FloodJob *jobs, FloodBufs B, uint32_t b, FloodRes
     }
     const bool any_border = __any(border), any_edge = __any(edge);
     if (lane == 0) {
-        FloodRes &o = res[T.job];
+        FloodRes &o = res[job];
         atomicAdd(&o.tiles, 1ull);
-        atomicAdd(&o.evald, 64ull * __popc(T.evald));
+        atomicAdd(&o.evald, 64ull * __popc(ald(&T.evald)));
         if (sampled) atomicAdd(&o.sampled, sampled);
         if (hotc) atomicAdd(&o.hot, hotc);
         if (cells) {
@@ -454,34 +457,31 @@ __device__ void acc_tile(const FloodJob *jobs, FloodBufs B, uint32_t b, FloodRes
         if (lane == 0) k = atomicAdd(mcount, 1u);
         k = __shfl(k, 0);
         if (k < mcap) {
-            if (lane < FT) masks[k].rows[lane] = RC;
-            if (lane == 0) {
-                masks[k].job = T.job;
-                masks[k].tx = T.tx;
-                masks[k].tz = T.tz;
-            }
+            if (lane < FT) ast(&masks[k].rows[lane], RC);
+            if (lane == 0) mask_head_store(masks[k], job, tx, tz);
         }
     }
     // leave the tile and its hash slot clean for the next call (everything above has read them)
     __builtin_amdgcn_wave_barrier();
     if (lane < FT) {
-        T.reach[lane] = 0;
-        T.in[lane] = 0;
-        T.pass[lane] = 0;
-        T.cnt[lane] = 0;
-        T.hot[lane] = 0;
+        ast(&T.reach[lane], 0u);
+        ast(&T.in[lane], 0u);
+        ast(&T.pass[lane], 0u);
+        ast(&T.cnt[lane], 0u);
+        ast(&T.hot[lane], 0u);
     }
     if (lane == 0) {
-        if (T.slot1) B.hkeys[T.slot1 - 1] = HEMPTY;
-        T.slot1 = 0;
-        T.qstate = 0;
-        T.evald = 0;
+        const uint32_t slot1 = ald(&T.slot1);
+        if (slot1) ast(&B.hkeys[slot1 - 1], HEMPTY);
+        ast(&T.slot1, 0u);
+        ast(&T.qstate, 0u);
+        ast(&T.evald, 0u);
     }
 }
 
 __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBufs B, FloodRes *res, TileMask *masks,
                                                   uint32_t mcap) {
-    const uint32_t ntiles = min(*B.ntiles, B.tile_cap);
+    const uint32_t ntiles = min(ald(B.ntiles), B.tile_cap);
     for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) acc_tile(jobs, B, b, res, masks, &B.ctl[FLOOD_MASKS], mcap);
 }
 
@@ -489,15 +489,18 @@ __global__ __launch_bounds__(256) void k_flood_publish(const FloodRes *res, uint
                                                        FloodRes *h_res, uint32_t *h_ctl, TileMask *h_masks, uint32_t hmcap) {
     const uint32_t nt = blockDim.x * gridDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (t0 < CTL_WORDS) {
-        h_ctl[t0] = B.ctl[t0];
-        B.ctl_next[t0] = 0;
+        h_ctl[t0] = ald(&B.ctl[t0]);
+        ast(&B.ctl_next[t0], 0u);
     }
-    for (uint32_t q = t0; q < n; q += nt) h_res[q] = res[q];
-    const uint32_t nm = min(min(B.ctl[FLOOD_MASKS], hmcap), 0xffffffffu);
+    constexpr uint32_t RW = sizeof(FloodRes) / 4;
+    const uint32_t *rs = reinterpret_cast<const uint32_t *>(res);
+    uint32_t *rd = reinterpret_cast<uint32_t *>(h_res);
+    for (uint32_t q = t0; q < n * RW; q += nt) rd[q] = ald(&rs[q]);
+    const uint32_t nm = min(ald(&B.ctl[FLOOD_MASKS]), hmcap);
     constexpr uint32_t W = sizeof(TileMask) / 4;
     const uint32_t *src = reinterpret_cast<const uint32_t *>(masks);
     uint32_t *dst = reinterpret_cast<uint32_t *>(h_masks);
-    for (uint32_t q = t0; q < nm * W; q += nt) dst[q] = src[q];
+    for (uint32_t q = t0; q < nm * W; q += nt) dst[q] = ald(&src[q]);
 }
 
 // ------------------------------------------------------------------ launchers

@@ -442,53 +442,56 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     const dsx::FloodJob *d_jobs = reinterpret_cast<const dsx::FloodJob *>(d_up);
     uint32_t *d_small = reinterpret_cast<uint32_t *>(d_up + n * sizeof(dsx::FloodJob));
     dsx::FloodBufs &B = S->fb;
-    S->ctl_phase ^= 1u;
-    B.ctl = S->d_ctl + S->ctl_phase * dsx::CTL_WORDS;
-    B.ctl_next = S->d_ctl + (S->ctl_phase ^ 1u) * dsx::CTL_WORDS;
-    B.ntiles = B.ctl + dsx::CTL_NTILES;
-    // jobs and lists read from the pinned staging by the first kernel (no separate copy)
-    dsx::launch_flood_stage(c->st, S->h_up, uint32_t(n), n_small, n_big, d_up, S->d_res, B);
-    uint32_t off = 0;
-    for (int cls = 0; cls < 3; cls++) {
-        dsx::launch_flood_lds(c->st, cls, d_jobs, d_small + off, n_cls[cls], S->d_clim, S->grid, S->d_res, S->d_masks,
-                              S->mask_cap, B, S->prepass);
-        off += n_cls[cls];
-    }
-    // the tiled engine: the jobs listed now plus the small-window floods handed over on the device
+    // a batch that reaches the engine's time limit is run once more on a fully reset engine (tiles, hash
+    // table, work queue and both control blocks); its results are only taken from a complete run
+    const uint32_t *h = S->h_ctl;
+    const dsx::FloodRes *res = S->h_res;
     const bool prof = g_prof_on.load(std::memory_order_relaxed) != 0;
     const bool tiled = n_big > 0 || n_small > 0;
     uint32_t blocks_used = 0;
-    if (tiled) {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (prof) {
-            while (S->events.size() < 2) {
-                hipEvent_t e;
-                HIPCK(hipEventCreate(&e));
-                S->events.push_back(e);
-            }
-            e0 = S->events[0];
-            e1 = S->events[1];
+    for (int attempt = 0;; attempt++) {
+        S->ctl_phase ^= 1u;
+        B.ctl = S->d_ctl + S->ctl_phase * dsx::CTL_WORDS;
+        B.ctl_next = S->d_ctl + (S->ctl_phase ^ 1u) * dsx::CTL_WORDS;
+        B.ntiles = B.ctl + dsx::CTL_NTILES;
+        // jobs and lists read from the pinned staging by the first kernel (no separate copy)
+        dsx::launch_flood_stage(c->st, S->h_up, uint32_t(n), n_small, n_big, d_up, S->d_res, B);
+        uint32_t off = 0;
+        for (int cls = 0; cls < 3; cls++) {
+            dsx::launch_flood_lds(c->st, cls, d_jobs, d_small + off, n_cls[cls], S->d_clim, S->grid, S->d_res,
+                                  S->d_masks, S->mask_cap, B, S->prepass);
+            off += n_cls[cls];
         }
-        // workgroups: enough for the tiles in flight (every flood's frontier)
-        static const uint32_t per_job = uint32_t(env_u32("DSX_FLOOD_BLOCKS_PER_JOB", 4)),
-                              max_blocks = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS", 128));
-        const uint32_t blocks = uint32_t(std::clamp<size_t>(size_t(per_job) * n_big, 16, max_blocks));
-        blocks_used = blocks;
-        dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, S->prepass, e0, e1, prof ? 1 : 0);
-        dsx::launch_flood_acc(c->st, d_jobs, B, S->d_res, S->d_masks, S->mask_cap, n_big ? 1024 : 64);
-    }
-    dsx::launch_flood_publish(c->st, S->d_res, uint32_t(n), B, S->d_masks, S->h_res, S->h_ctl, S->h_masks, S->h_mask_cap);
-    HIPCK(hipGetLastError());
-    HIPCK(hipStreamSynchronize(c->st));
-    const uint32_t *h = S->h_ctl;
-    const dsx::FloodRes *res = S->h_res;
-    int status = DSX_OK;
-    if (h[dsx::FLOOD_OVF]) {
-        // acc ran on an incomplete flood state: results are void, and the tiles, hash slots and queue
-        // slots this call used are reset wholesale
-        status = DSX_E_CAPACITY;
+        // the tiled engine: the jobs listed now plus the small-window floods handed over on the device
+        if (tiled) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (prof) {
+                while (S->events.size() < 2) {
+                    hipEvent_t e;
+                    HIPCK(hipEventCreate(&e));
+                    S->events.push_back(e);
+                }
+                e0 = S->events[0];
+                e1 = S->events[1];
+            }
+            // workgroups: enough for the tiles in flight (every flood's frontier)
+            static const uint32_t per_job = uint32_t(env_u32("DSX_FLOOD_BLOCKS_PER_JOB", 4)),
+                                  max_blocks = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS", 128));
+            const uint32_t blocks = uint32_t(std::clamp<size_t>(size_t(per_job) * n_big, 16, max_blocks));
+            blocks_used = blocks;
+            dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, S->prepass, e0, e1, prof ? 1 : 0);
+            dsx::launch_flood_acc(c->st, d_jobs, B, S->d_res, S->d_masks, S->mask_cap, n_big ? 1024 : 64);
+        }
+        dsx::launch_flood_publish(c->st, S->d_res, uint32_t(n), B, S->d_masks, S->h_res, S->h_ctl, S->h_masks,
+                                  S->h_mask_cap);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(c->st));
+        if (!h[dsx::FLOOD_OVF]) break;
+        // acc ran on an incomplete flood state: results are void, and the engine state is reset
+        int status = DSX_E_CAPACITY;
         dsx::g_last_error = "flood tile pool or work queue exhausted: split the batch";
-        if (h[dsx::FLOOD_OVF] & dsx::OVF_TIME) {
+        const bool timed_out = (h[dsx::FLOOD_OVF] & dsx::OVF_TIME) != 0;
+        if (timed_out) {
             status = DSX_E_HIP;
             char msg[256];
             std::snprintf(msg, sizeof msg,
@@ -498,12 +501,15 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
                           h[dsx::TIME_ITEM], unsigned(n), n_big, blocks_used);
             dsx::g_last_error = msg;
         }
-        const uint32_t ntiles = std::min(h[dsx::CTL_NTILES], B.tile_cap), qtail = std::min(h[dsx::Q_TAIL], B.wq_cap);
+        const uint32_t ntiles = timed_out ? B.tile_cap : std::min(h[dsx::CTL_NTILES], B.tile_cap);
+        const uint32_t qtail = timed_out ? B.wq_cap : std::min(h[dsx::Q_TAIL], B.wq_cap);
         HIPCK(hipMemsetAsync(B.tiles, 0, size_t(ntiles) * sizeof(dsx::Tile), c->st));
         HIPCK(hipMemsetAsync(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8, c->st));
         if (qtail) HIPCK(hipMemsetAsync(B.wq, 0xff, size_t(qtail) * 4, c->st));
+        if (timed_out) HIPCK(hipMemsetAsync(S->d_ctl, 0, 2 * dsx::CTL_WORDS * 4, c->st));
         HIPCK(hipStreamSynchronize(c->st));
-        return status;
+        if (!timed_out || attempt > 0) return status;
+        std::fprintf(stderr, "dsx_flood: %s; running the batch again on a reset engine\n", dsx::g_last_error.c_str());
     }
     const uint32_t nm = h[dsx::FLOOD_MASKS];
     if (nm > S->mask_cap) {

@@ -751,11 +751,9 @@ void make_sites(const dsx_raw_cand *raw, size_t nraw, const dsx_search_params &P
 Group::Group() = default;
 Group::~Group() = default;
 
-// Tables, scan and sites of a group's seeds; then each scanned seed's coroutine starts in S and runs to
-// its first request.
-int group_start(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g, Sched &S, std::vector<Task> &tasks) {
+// Tables, scan and sites of a group's seeds (g.outs).
+int group_scan(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g) {
     const dsx_scan_params sp{float(P.ta), float(P.t), float(P.wc), float(P.we), P.sigma, P.sigmac};
-    if (const int rc = dsx_set_precision(w, P.prepass ? 1 : 0)) return rc;
     const size_t nb = g.seeds.size();
     auto tscan = std::chrono::steady_clock::now();
     std::vector<float> mta(nb), mtb(nb);
@@ -784,7 +782,13 @@ int group_start(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g, Sche
         g.outs.push_back(std::move(O));
     }
     prof_add(1, std::chrono::duration<double>(std::chrono::steady_clock::now() - tscan).count(), 0);
-    // hot zones, proxy floods, exact floods of every scanned seed as coroutines
+    return DSX_OK;
+}
+
+// Hot zones, proxy floods, exact floods of every scanned seed as coroutines in S: each starts and runs
+// to its first request.
+int group_tasks(Sched &S, const dsx_search_params &P, Group &g, std::vector<Task> &tasks) {
+    if (const int rc = dsx_set_precision(S.ctx, P.prepass ? 1 : 0)) return rc;
     for (auto &O : g.outs)
         if (O->scanned) tasks.push_back(seed_task(S, P, *O));
     return DSX_OK;
@@ -809,7 +813,8 @@ int group_finish(dsx_ctx *w, const dsx_search_params &P, Group &g) {
 int run_group(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g) {
     Sched S{w, {}, {}, DSX_OK, {}};
     std::vector<Task> tasks;
-    if (const int rc = group_start(w, P, mode, g, S, tasks)) return rc;
+    if (const int rc = group_scan(w, P, mode, g)) return rc;
+    if (const int rc = group_tasks(S, P, g, tasks)) return rc;
     auto tsites = std::chrono::steady_clock::now();
     while (S.step()) {
     }
@@ -901,7 +906,7 @@ class Pool {
             std::lock_guard<std::mutex> l(mu_);
             q_.push_back(g);
         }
-        cv_work_.notify_one();
+        cv_work_.notify_all();
     }
     void wait(Group *g) {
         std::unique_lock<std::mutex> l(mu_);
@@ -917,6 +922,15 @@ class Pool {
     }
 
   private:
+    void finish(Group *g, int rc, const std::string &err) {
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            g->rc = rc;
+            g->err = err;
+            g->done = true;
+        }
+        cv_done_.notify_all();
+    }
     // Continuous batching (DSX_ACTIVE > 1): the worker keeps the seeds of several groups in one
     // scheduler, so every flood and point call carries the requests of all seeds in flight, and a new
     // group is admitted whenever fewer than DSX_ACTIVE seeds are in flight. Results do not depend on the
@@ -932,15 +946,6 @@ class Pool {
         };
         std::deque<Active> act;
         size_t nact = 0;
-        auto finish = [this](Group *g, int rc, const std::string &err) {
-            {
-                std::lock_guard<std::mutex> l(mu_);
-                g->rc = rc;
-                g->err = err;
-                g->done = true;
-            }
-            cv_done_.notify_all();
-        };
         for (;;) {
             while (nact < active) {   // admit groups (wait for one only when nothing is in flight)
                 Group *g;
@@ -958,7 +963,8 @@ class Pool {
                     continue;
                 }
                 Active a{g, {}};
-                const int rc = group_start(w, P_, mode_, *g, S, a.tasks);
+                int rc = group_scan(w, P_, mode_, *g);
+                if (!rc) rc = group_tasks(S, P_, *g, a.tasks);
                 if (rc) {   // the scan failed: no coroutine started
                     finish(g, rc, dsx_last_error());
                     continue;

@@ -176,7 +176,8 @@ typedef struct {
  * (dense_flood.hpp:52-86) at every call site: hot zones (dsearch_desert.cpp:337, 352), proxy floods
  * (:387), exact floods (:512, :521) and tools/dflood.cpp:20. Results are identical to the reference's
  * depth-first flood (they do not depend on visiting order). DSX_E_CAPACITY if the batch needs more
- * tiles than the engine holds (split the batch). */
+ * tiles than the engine holds (split the batch). A batch whose tiled engine reaches its time limit
+ * (a queue item not taken for 5 s) runs once more on a reset engine; DSX_E_HIP if it does again. */
 int dsx_flood(dsx_ctx *ctx, const dsx_flood_job *jobs, uint64_t n, dsx_area *out);
 
 /* The counted cells (state 3 in dense_flood.hpp's window) of the floods of the last dsx_flood call, as
