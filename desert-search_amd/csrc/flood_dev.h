@@ -28,13 +28,18 @@ template <class T> __device__ __forceinline__ void ast(T *p, T v) {
 // (an agent-scope release fence would write back the whole L2 at every hand-over).
 __device__ __forceinline__ void mem_done() { __builtin_amdgcn_s_waitcnt(0); }
 
-// Flood results and exported masks are written and read through the same agent-scope accesses as the
-// rest of the engine's shared state, word by word.
+// Words that kernels update with atomic read-modify-writes (control counters, tile queue states and
+// `in` masks, hash keys, flood results) are also set and reset with atomics, never with stores: every
+// write to such a word then takes the same path to memory, so a reset cannot land after a later update.
+__device__ __forceinline__ void aset(uint32_t *p, uint32_t v) { (void)atomicExch(p, v); }
+__device__ __forceinline__ void aset(int *p, int v) { (void)atomicExch(p, v); }
+__device__ __forceinline__ void aset(unsigned long long *p, unsigned long long v) { (void)atomicExch(p, v); }
+// Flood results (acc adds to them with atomics) are initialised word by word with atomic exchanges.
 __device__ __forceinline__ void res_store(FloodRes *p, const FloodRes &r) {
     const uint32_t *s = reinterpret_cast<const uint32_t *>(&r);
     uint32_t *d = reinterpret_cast<uint32_t *>(p);
 #pragma unroll
-    for (int i = 0; i < int(sizeof(FloodRes) / 4); i++) ast(&d[i], s[i]);
+    for (int i = 0; i < int(sizeof(FloodRes) / 4); i++) aset(&d[i], s[i]);
 }
 __device__ __forceinline__ void mask_head_store(TileMask &m, uint32_t job, int32_t tx, int32_t tz) {
     ast(&m.job, job);

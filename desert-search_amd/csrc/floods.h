@@ -54,7 +54,8 @@ struct FloodBufs {
     uint32_t *wq;                 // work queue slots
     uint32_t wq_cap;
     uint32_t *ctl;                // this batch's control words (zero when it starts)
-    uint32_t *ctl_next;           // the next batch's control words (zeroed at the end of this one)
+    uint32_t *ctl_next;           // the control words of the batch two after this one (zeroed by this one)
+    uint32_t seq;                 // batch sequence number (for the time-limit diagnostics)
 };
 
 // ctl words: queue head/tail, pending items, overflow flags, exported masks, items processed, floods
@@ -62,11 +63,13 @@ struct FloodBufs {
 // workgroup that reached the time limit saw (for the error message): a flag, its slot, the queue head,
 // tail and pending count, and the slot's content; evaluation rounds of the tiled engine and the
 // sub-blocks they needed (before speculative ones are added); with profiling on, the workgroups' time
-// (100 MHz ticks) waiting for queue items, evaluating sub-blocks, loading octaves into LDS, and alive
+// (100 MHz ticks) waiting for queue items, evaluating sub-blocks, loading octaves into LDS, and alive;
+// then the batch number (plus one) the stage kernel wrote, the one the tiled engine's first workgroup
+// was given, the queue tail and the stage's batch number that workgroup read when it started
 constexpr int Q_HEAD = 0, Q_TAIL = 1, Q_PENDING = 2, FLOOD_OVF = 3, FLOOD_MASKS = 4, Q_ITEMS = 5, HANDED = 6,
               CTL_NTILES = 7, TIME_SEEN = 8, TIME_SLOT = 9, TIME_HEAD = 10, TIME_TAIL = 11, TIME_PENDING = 12,
               TIME_ITEM = 13, EVAL_ROUNDS = 14, EVAL_NEED = 15, PROF_WAIT = 16, PROF_EVAL = 17, PROF_LOAD = 18,
-              PROF_LIFE = 19, CTL_WORDS = 24;
+              PROF_LIFE = 19, STAGE_SEQ = 20, RUN_SEQ = 21, RUN_TAIL0 = 22, RUN_STAGE_SEQ = 23, CTL_WORDS = 24;
 constexpr uint32_t Q_EMPTY = 0xffffffffu;
 // FLOOD_OVF bits: 1 work queue full, 2 tile pool or hash table full, 4 time limit reached
 constexpr uint32_t OVF_QUEUE = 1, OVF_TILES = 2, OVF_TIME = 4;

@@ -62,6 +62,7 @@ __global__ __launch_bounds__(256) void k_flood_stage(const uint8_t *h_up, uint32
         dl[n + q] = k;
         flood_start_tile(hj[k], k, B);
     }
+    if (q == 0) aset(&B.ctl[STAGE_SEQ], B.seq + 1);
 }
 
 // Sub-blocks: a tile's 32x32 cells are evaluated lazily in 16 blocks of 8x8 (bit bz*4 + bx of the
@@ -202,6 +203,11 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                                                        int backoff, int spec, int tprof) {
     constexpr int NWAVE = NT / 64;
     const uint64_t t_born = tprof ? __builtin_amdgcn_s_memrealtime() : 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // what the first workgroup finds (time-limit diagnostics)
+        aset(&B.ctl[RUN_SEQ], B.seq + 1);
+        aset(&B.ctl[RUN_TAIL0], atomicAdd(&B.ctl[Q_TAIL], 0u));
+        aset(&B.ctl[RUN_STAGE_SEQ], atomicAdd(&B.ctl[STAGE_SEQ], 0u));
+    }
     __shared__ ClimSeed C;
     __shared__ uint32_t s_pass[FT], s_cnt[FT], s_hot[FT];
     __shared__ uint32_t s_item, s_again, s_need, s_evald;
@@ -229,12 +235,12 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                         }
                         if (__builtin_amdgcn_s_memrealtime() - t0 > 500000000ull) {   // 5 s at 100 MHz
                             if (atomicCAS(&B.ctl[TIME_SEEN], 0u, 1u) == 0u) {
-                                ast(&B.ctl[TIME_SLOT], k);
-                                ast(&B.ctl[TIME_HEAD], ald(&B.ctl[Q_HEAD]));
-                                ast(&B.ctl[TIME_TAIL], ald(&B.ctl[Q_TAIL]));
-                                ast(&B.ctl[TIME_PENDING], ald(&B.ctl[Q_PENDING]));
-                                ast(&B.ctl[TIME_ITEM], __hip_atomic_load(&B.wq[k], __ATOMIC_RELAXED,
-                                                                         __HIP_MEMORY_SCOPE_SYSTEM));
+                                aset(&B.ctl[TIME_SLOT], k);
+                                aset(&B.ctl[TIME_HEAD], atomicAdd(&B.ctl[Q_HEAD], 0u));
+                                aset(&B.ctl[TIME_TAIL], atomicAdd(&B.ctl[Q_TAIL], 0u));
+                                aset(&B.ctl[TIME_PENDING], atomicAdd(&B.ctl[Q_PENDING], 0u));
+                                aset(&B.ctl[TIME_ITEM], __hip_atomic_load(&B.wq[k], __ATOMIC_RELAXED,
+                                                                          __HIP_MEMORY_SCOPE_SYSTEM));
                             }
                             atomicOr(&B.ctl[FLOOD_OVF], OVF_TIME);
                             stop = 1;
@@ -247,7 +253,9 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                     else if (backoff == 1 || it < 512u) __builtin_amdgcn_s_sleep(32);
                     else __builtin_amdgcn_s_sleep(127);
                 }
-                if (item != Q_EMPTY) ast(&B.wq[k], Q_EMPTY);   // leave the slot clean for the next call
+                // leave the slot clean for the next call, with the same (system) scope as the pushes and
+                // polls of queue slots: a write of another scope may reach memory in another order
+                if (item != Q_EMPTY) __hip_atomic_store(&B.wq[k], Q_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             }
             s_item = item;
             s_stop = stop;
@@ -465,16 +473,16 @@ __device__ void acc_tile(const FloodJob *jobs, FloodBufs B, uint32_t b, FloodRes
     __builtin_amdgcn_wave_barrier();
     if (lane < FT) {
         ast(&T.reach[lane], 0u);
-        ast(&T.in[lane], 0u);
+        aset(&T.in[lane], 0u);
         ast(&T.pass[lane], 0u);
         ast(&T.cnt[lane], 0u);
         ast(&T.hot[lane], 0u);
     }
     if (lane == 0) {
         const uint32_t slot1 = ald(&T.slot1);
-        if (slot1) ast(&B.hkeys[slot1 - 1], HEMPTY);
+        if (slot1) aset(&B.hkeys[slot1 - 1], HEMPTY);
         ast(&T.slot1, 0u);
-        ast(&T.qstate, 0u);
+        aset(&T.qstate, 0u);
         ast(&T.evald, 0u);
     }
 }
@@ -490,7 +498,7 @@ __global__ __launch_bounds__(256) void k_flood_publish(const FloodRes *res, uint
     const uint32_t nt = blockDim.x * gridDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (t0 < CTL_WORDS) {
         h_ctl[t0] = ald(&B.ctl[t0]);
-        ast(&B.ctl_next[t0], 0u);
+        aset(&B.ctl_next[t0], 0u);
     }
     constexpr uint32_t RW = sizeof(FloodRes) / 4;
     const uint32_t *rs = reinterpret_cast<const uint32_t *>(res);

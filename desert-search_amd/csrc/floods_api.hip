@@ -74,6 +74,10 @@ struct HostGrid {
 }  // namespace
 
 // Device-resident climate state of a context (allocated on first use).
+// Control blocks in rotation: a block is zeroed two batches before its use, so the zeroing never
+// shares a batch boundary with the block's own batch.
+constexpr uint32_t CTL_BLOCKS = 4;
+
 struct ClimState {
     dsx::SurfGrid grid{};
     int32_t *d_cuts = nullptr;
@@ -99,8 +103,8 @@ struct ClimState {
     uint32_t masks_n = 0, masks_h = 0;  // masks of the last call; how many of them are in h_masks
     std::vector<hipEvent_t> events;   // event pair around the tiled engine's kernel (profiling on)
     int prepass = 0;                  // dsx_set_precision: 1 = FP32 pre-pass in floods and predicates
-    uint32_t *d_ctl = nullptr;        // two blocks of control words; batch i uses block i % 2
-    uint32_t ctl_phase = 0;
+    uint32_t *d_ctl = nullptr;        // CTL_BLOCKS blocks of control words; batch i uses block i % CTL_BLOCKS
+    uint32_t ctl_seq = 0;
     // flood batch I/O: jobs and job lists go up in one copy from pinned staging (device layout: n jobs,
     // n small-window indices, n tiled-engine indices); results come down through pinned memory the
     // publish kernel writes
@@ -125,13 +129,13 @@ static int clim_alloc(dsx_ctx *c, ClimState *&S) {
     HIPCK(hipMalloc(&S->d_cell, G.cell.size() * 2));
     size_t off = 0;
     for (int a = 0; a < 5; a++) {
-        HIPCK(hipMemcpy(S->d_cuts + off, G.cuts[a].data(), G.cuts[a].size() * 4, hipMemcpyHostToDevice));
+        HIPCK(hipMemcpyAsync(S->d_cuts + off, G.cuts[a].data(), G.cuts[a].size() * 4, hipMemcpyHostToDevice, c->st));
         S->grid.cuts[a] = S->d_cuts + off;
         S->grid.ncut[a] = int(G.cuts[a].size());
         S->grid.dim[a] = G.dim[a];
         off += G.cuts[a].size();
     }
-    HIPCK(hipMemcpy(S->d_cell, G.cell.data(), G.cell.size() * 2, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpyAsync(S->d_cell, G.cell.data(), G.cell.size() * 2, hipMemcpyHostToDevice, c->st));
     S->grid.cell = S->d_cell;
     S->clim_cap = 4096;
     HIPCK(hipMalloc(&S->d_clim, size_t(S->clim_cap) * sizeof(dsx::ClimSeed)));
@@ -141,20 +145,23 @@ static int clim_alloc(dsx_ctx *c, ClimState *&S) {
     B.tile_cap = 1u << 21;                    // 2M tiles x 528 B
     B.hmask = (1u << 23) - 1;                 // 8M hash slots
     HIPCK(hipMalloc(&B.tiles, size_t(B.tile_cap) * sizeof(dsx::Tile)));
-    HIPCK(hipMemset(B.tiles, 0, size_t(B.tile_cap) * sizeof(dsx::Tile)));
+    // every initialisation is ordered on the context's stream and completed before the state is used:
+    // hipMemset would run on the null stream, which the non-blocking context streams do not wait for
+    HIPCK(hipMemsetAsync(B.tiles, 0, size_t(B.tile_cap) * sizeof(dsx::Tile), c->st));
     HIPCK(hipMalloc(&B.hkeys, (size_t(B.hmask) + 1) * 8));
-    HIPCK(hipMemset(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8));
+    HIPCK(hipMemsetAsync(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8, c->st));
     B.wq_cap = 4 * B.tile_cap;               // queue slots (a tile is queued again when touched again)
     HIPCK(hipMalloc(&B.wq, size_t(B.wq_cap) * 4));
-    HIPCK(hipMemset(B.wq, 0xff, size_t(B.wq_cap) * 4));
-    HIPCK(hipMalloc(&S->d_ctl, 2 * dsx::CTL_WORDS * 4));   // two control blocks: batches alternate
-    HIPCK(hipMemset(S->d_ctl, 0, 2 * dsx::CTL_WORDS * 4));
+    HIPCK(hipMemsetAsync(B.wq, 0xff, size_t(B.wq_cap) * 4, c->st));
+    HIPCK(hipMalloc(&S->d_ctl, CTL_BLOCKS * dsx::CTL_WORDS * 4));
+    HIPCK(hipMemsetAsync(S->d_ctl, 0, CTL_BLOCKS * dsx::CTL_WORDS * 4, c->st));
     S->job_cap = 1u << 20;
     HIPCK(hipMalloc(&S->d_up, size_t(S->job_cap) * (sizeof(dsx::FloodJob) + 8)));
     HIPCK(hipMalloc(&S->d_res, size_t(S->job_cap) * sizeof(dsx::FloodRes)));
     S->mask_cap = 1u << 20;
     HIPCK(hipMalloc(&S->d_masks, size_t(S->mask_cap) * sizeof(dsx::TileMask)));
     HIPCK(hipHostMalloc(&S->h_ctl, dsx::CTL_WORDS * 4, hipHostMallocDefault));
+    HIPCK(hipStreamSynchronize(c->st));
     return DSX_OK;
 }
 
@@ -450,10 +457,13 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     const bool tiled = n_big > 0 || n_small > 0;
     uint32_t blocks_used = 0;
     for (int attempt = 0;; attempt++) {
-        S->ctl_phase ^= 1u;
-        B.ctl = S->d_ctl + S->ctl_phase * dsx::CTL_WORDS;
-        B.ctl_next = S->d_ctl + (S->ctl_phase ^ 1u) * dsx::CTL_WORDS;
+        // this batch's control block was zeroed by the publish kernel of the batch two before it; this
+        // batch's publish zeroes the block of the batch two after it
+        const uint32_t seq = S->ctl_seq++;
+        B.ctl = S->d_ctl + (seq % CTL_BLOCKS) * dsx::CTL_WORDS;
+        B.ctl_next = S->d_ctl + ((seq + 2) % CTL_BLOCKS) * dsx::CTL_WORDS;
         B.ntiles = B.ctl + dsx::CTL_NTILES;
+        B.seq = seq;
         // jobs and lists read from the pinned staging by the first kernel (no separate copy)
         dsx::launch_flood_stage(c->st, S->h_up, uint32_t(n), n_small, n_big, d_up, S->d_res, B);
         uint32_t off = 0;
@@ -493,12 +503,13 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         const bool timed_out = (h[dsx::FLOOD_OVF] & dsx::OVF_TIME) != 0;
         if (timed_out) {
             status = DSX_E_HIP;
-            char msg[256];
+            char msg[384];
             std::snprintf(msg, sizeof msg,
                           "flood engine time limit reached (slot %u: head %u tail %u pending %u item %08x; "
-                          "%u floods, %u tiled, %u workgroups)",
+                          "%u floods, %u tiled, %u workgroups; batch %u: stage %u, engine %u saw tail %u stage %u)",
                           h[dsx::TIME_SLOT], h[dsx::TIME_HEAD], h[dsx::TIME_TAIL], h[dsx::TIME_PENDING],
-                          h[dsx::TIME_ITEM], unsigned(n), n_big, blocks_used);
+                          h[dsx::TIME_ITEM], unsigned(n), n_big, blocks_used, seq + 1, h[dsx::STAGE_SEQ],
+                          h[dsx::RUN_SEQ], h[dsx::RUN_TAIL0], h[dsx::RUN_STAGE_SEQ]);
             dsx::g_last_error = msg;
         }
         const uint32_t ntiles = timed_out ? B.tile_cap : std::min(h[dsx::CTL_NTILES], B.tile_cap);
@@ -506,7 +517,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         HIPCK(hipMemsetAsync(B.tiles, 0, size_t(ntiles) * sizeof(dsx::Tile), c->st));
         HIPCK(hipMemsetAsync(B.hkeys, 0xff, (size_t(B.hmask) + 1) * 8, c->st));
         if (qtail) HIPCK(hipMemsetAsync(B.wq, 0xff, size_t(qtail) * 4, c->st));
-        if (timed_out) HIPCK(hipMemsetAsync(S->d_ctl, 0, 2 * dsx::CTL_WORDS * 4, c->st));
+        if (timed_out) HIPCK(hipMemsetAsync(S->d_ctl, 0, CTL_BLOCKS * dsx::CTL_WORDS * 4, c->st));
         HIPCK(hipStreamSynchronize(c->st));
         if (!timed_out || attempt > 0) return status;
         std::fprintf(stderr, "dsx_flood: %s; running the batch again on a reset engine\n", dsx::g_last_error.c_str());

@@ -78,10 +78,13 @@ int scan_alloc(dsx_ctx *c) {
     HIPCK(hipMalloc(&c->d_ibC, ibC.size() * 2));
     HIPCK(hipMalloc(&c->d_loT, 512));
     HIPCK(hipMalloc(&c->d_hiT, 512));
-    HIPCK(hipMemcpy(c->d_ibT, ibT.data(), ibT.size() * 2, hipMemcpyHostToDevice));
-    HIPCK(hipMemcpy(c->d_ibC, ibC.data(), ibC.size() * 2, hipMemcpyHostToDevice));
-    HIPCK(hipMemcpy(c->d_loT, loT.data(), 512, hipMemcpyHostToDevice));
-    HIPCK(hipMemcpy(c->d_hiT, hiT.data(), 512, hipMemcpyHostToDevice));
+    // ordered on the context's stream (a plain hipMemcpy goes to the null stream) and completed before
+    // the host buffers go out of scope
+    HIPCK(hipMemcpyAsync(c->d_ibT, ibT.data(), ibT.size() * 2, hipMemcpyHostToDevice, c->st));
+    HIPCK(hipMemcpyAsync(c->d_ibC, ibC.data(), ibC.size() * 2, hipMemcpyHostToDevice, c->st));
+    HIPCK(hipMemcpyAsync(c->d_loT, loT.data(), 512, hipMemcpyHostToDevice, c->st));
+    HIPCK(hipMemcpyAsync(c->d_hiT, hiT.data(), 512, hipMemcpyHostToDevice, c->st));
+    HIPCK(hipStreamSynchronize(c->st));
     return DSX_OK;
 }
 

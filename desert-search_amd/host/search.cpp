@@ -276,7 +276,13 @@ struct Sched {
             masks.resize(nm);
             if (nm && (rc = dsx_flood_masks(ctx, masks.data(), nm, &nm))) return rc;
             std::vector<uint32_t> cnt(hi - lo, 0);
-            for (const dsx_tile_mask &m : masks) cnt[m.job]++;
+            for (const dsx_tile_mask &m : masks) {
+                if (m.job >= hi - lo) {
+                    dsx::g_last_error = "flood mask of a job outside the batch";
+                    return DSX_E_HIP;
+                }
+                cnt[m.job]++;
+            }
             for (size_t k = lo; k < hi; k++)
                 if (cnt[k - lo] && who[k].first->want_cells) who[k].first->out[who[k].second].tiles.reserve(cnt[k - lo]);
             for (const dsx_tile_mask &m : masks) {
