@@ -17,6 +17,7 @@ struct Group {
     int rc = DSX_OK;
     std::string err;
     bool done = false;
+    bool scanned = false;   // tables, scan and sites done (outs filled) by a scan worker
     Group();
     ~Group();
 };

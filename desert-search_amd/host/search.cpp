@@ -894,10 +894,16 @@ int worker_contexts(dsx_ctx *ctx, int n, std::vector<dsx_ctx *> &out) {
 }
 
 // Runs groups on worker threads; the caller submits groups and collects them in submission order.
+// With scan workers, a group's tables, scan and sites run on a scan worker's context, then its floods
+// on a flood worker's, so the flood workers' device calls are floods and points only while the next
+// groups are scanned.
 class Pool {
   public:
-    Pool(const std::vector<dsx_ctx *> &ws, const dsx_search_params &P, int mode) : P_(P), mode_(mode) {
-        for (dsx_ctx *w : ws) th_.emplace_back([this, w] { loop(w); });
+    Pool(const std::vector<dsx_ctx *> &flood_ws, const std::vector<dsx_ctx *> &scan_ws, const dsx_search_params &P,
+         int mode)
+        : P_(P), mode_(mode), nscan_(scan_ws.size()) {
+        for (dsx_ctx *w : flood_ws) th_.emplace_back([this, w] { loop(w); });
+        for (dsx_ctx *w : scan_ws) th_.emplace_back([this, w] { scan_loop(w); });
     }
     ~Pool() {
         {
@@ -910,7 +916,7 @@ class Pool {
     void submit(Group *g) {
         {
             std::lock_guard<std::mutex> l(mu_);
-            q_.push_back(g);
+            (nscan_ ? qs_ : q_).push_back(g);
         }
         cv_work_.notify_all();
     }
@@ -928,6 +934,37 @@ class Pool {
     }
 
   private:
+    // no group can reach the flood queue any more (caller holds mu_)
+    bool drained() const { return closing_ && qs_.empty() && scanning_ == 0; }
+
+    void scan_loop(dsx_ctx *w) {
+        for (;;) {
+            Group *g;
+            bool skip;
+            {
+                std::unique_lock<std::mutex> l(mu_);
+                cv_work_.wait(l, [this] { return closing_ || !qs_.empty(); });
+                if (qs_.empty()) return;
+                g = qs_.front();
+                qs_.pop_front();
+                skip = stop_;
+                scanning_++;
+            }
+            const int rc = skip ? DSX_E_ARG : group_scan(w, P_, mode_, *g);
+            const std::string err = skip ? "search stopped after an earlier error" : (rc ? dsx_last_error() : "");
+            {
+                std::lock_guard<std::mutex> l(mu_);
+                scanning_--;
+                if (!rc) {
+                    g->scanned = true;
+                    q_.push_back(g);
+                }
+            }
+            if (rc) finish(g, rc, err);
+            cv_work_.notify_all();
+        }
+    }
+
     void finish(Group *g, int rc, const std::string &err) {
         {
             std::lock_guard<std::mutex> l(mu_);
@@ -958,7 +995,7 @@ class Pool {
                 bool skip;
                 {
                     std::unique_lock<std::mutex> l(mu_);
-                    if (act.empty()) cv_work_.wait(l, [this] { return closing_ || !q_.empty(); });
+                    if (act.empty()) cv_work_.wait(l, [this] { return !q_.empty() || drained(); });
                     if (q_.empty()) break;
                     g = q_.front();
                     q_.pop_front();
@@ -969,7 +1006,7 @@ class Pool {
                     continue;
                 }
                 Active a{g, {}};
-                int rc = group_scan(w, P_, mode_, *g);
+                int rc = g->scanned ? DSX_OK : group_scan(w, P_, mode_, *g);
                 if (!rc) rc = group_tasks(S, P_, *g, a.tasks);
                 if (rc) {   // the scan failed: no coroutine started
                     finish(g, rc, dsx_last_error());
@@ -980,7 +1017,7 @@ class Pool {
             }
             if (act.empty()) {
                 std::lock_guard<std::mutex> l(mu_);
-                if (q_.empty()) return;   // closing, nothing left
+                if (q_.empty() && drained()) return;   // closing, nothing left
                 continue;
             }
             S.step();
@@ -1014,9 +1051,11 @@ class Pool {
     }
     const dsx_search_params P_;
     const int mode_;
+    const size_t nscan_;
     std::mutex mu_;
     std::condition_variable cv_work_, cv_done_;
-    std::deque<Group *> q_;
+    std::deque<Group *> q_, qs_;   // groups ready for floods; groups to scan
+    size_t scanning_ = 0;          // groups a scan worker holds
     bool closing_ = false, stop_ = false;
     std::vector<std::thread> th_;
 };
@@ -1046,8 +1085,10 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
     if (P.study || P.nms <= 0 || P.max_sites < 0 || P.workers < 0 || P.workers > 64) return DSX_E_ARG;
     const int nw = P.workers ? P.workers : 16;
     std::vector<dsx_ctx *> wctx;
-    int rc = worker_contexts(ctx, nw, wctx);
+    const int ns = int(std::min<uint64_t>(env_u64("DSX_SCAN_WORKERS", 0), 8));
+    int rc = worker_contexts(ctx, nw + ns, wctx);
     if (rc) return rc;
+    const std::vector<dsx_ctx *> flood_ws(wctx.begin(), wctx.begin() + nw), scan_ws(wctx.begin() + nw, wctx.end());
     const std::string pre(prefix);
     FILE *out = std::fopen((pre + ".csv").c_str(), "a");
     FILE *sout = std::fopen((pre + ".seeds.csv").c_str(), "a");
@@ -1064,7 +1105,7 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
     std::deque<std::unique_ptr<Group>> pending;   // submitted, in seed order
     std::string first_err;
     {
-        Pool pool(wctx, P, mode);
+        Pool pool(flood_ws, scan_ws, P, mode);
         // write the finished groups at the front, waiting for them while more than `keep` are pending
         auto drain = [&](size_t keep) {
             while (!pending.empty()) {
