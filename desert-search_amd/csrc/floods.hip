@@ -141,58 +141,73 @@ __device__ uint32_t prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J
     const uint32_t NR = R & ~old;
     if (lane < FT && NR) ast(&T.reach[r], R);
     if (!__any(NR != 0)) return 0u;
-    // ---- frontier across the tile edges
+    // ---- frontier across the tile edges, all directions at once: the wave's lanes look up the
+    // neighbour tiles in parallel (lane d: direction d), OR the crossing rows into their `in` masks in
+    // two atomic instructions (lanes 0-31: the left neighbour's rows, 32-63: the right one's; lanes 0-5:
+    // the rows above, below and diagonal), then queue the neighbours that grew, again lane d for d.
+    // Directions d: 0 left, 1 right, 2 up, 3 down, 4 up-left, 5 up-right, 6 down-left, 7 down-right.
     const int nta = int(ntile_axis(J));
-    const uint32_t up0 = __shfl(NR, 0, 32), dn31 = __shfl(NR, 31, 32);
-    const uint32_t nr_m1 = __shfl(NR, (r + 31) & 31, 32), nr_p1 = __shfl(NR, (r + 1) & 31, 32);
-    const uint32_t prev = r > 0 ? nr_m1 : 0u, next = r < FT - 1 ? nr_p1 : 0u;
-    // left neighbour column 31 / right neighbour column 0, per row r
-    bool lbit = (NR & 1u) != 0, rbit = (NR >> 31) != 0;
-    if (c8) {
-        lbit = lbit || (prev & 1u) || (next & 1u);
-        rbit = rbit || (prev >> 31) || (next >> 31);
-    }
-    uint32_t vert_up = up0, vert_dn = dn31;
-    if (c8) {
-        vert_up |= (up0 << 1) | (up0 >> 1);
-        vert_dn |= (dn31 << 1) | (dn31 >> 1);
-    }
-    for (int d = 0; d < 8; d++) {
-        const int dxs[8] = {-1, 1, 0, 0, -1, 1, -1, 1}, dzs[8] = {0, 0, -1, 1, -1, -1, 1, 1};
-        const int dx = dxs[d], dz = dzs[d];
-        if (d >= 4 && !c8) break;
-        int row = -1;
-        uint32_t mask = 0;
-        if (d == 0) { row = r; mask = lbit ? (1u << 31) : 0u; }
-        else if (d == 1) { row = r; mask = rbit ? 1u : 0u; }
-        else if (d == 2) { row = FT - 1; mask = lane == 0 ? vert_up : 0u; }
-        else if (d == 3) { row = 0; mask = lane == 0 ? vert_dn : 0u; }
-        else if (d == 4) { row = FT - 1; mask = (lane == 0 && (up0 & 1u)) ? (1u << 31) : 0u; }
-        else if (d == 5) { row = FT - 1; mask = (lane == 0 && (up0 >> 31)) ? 1u : 0u; }
-        else if (d == 6) { row = 0; mask = (lane == 0 && (dn31 & 1u)) ? (1u << 31) : 0u; }
-        else { row = 0; mask = (lane == 0 && (dn31 >> 31)) ? 1u : 0u; }
-        if (lane >= FT) mask = 0;
-        const int ntx = tx + dx, ntz = tz + dz;
-        if (ntx < 0 || ntz < 0 || ntx >= nta || ntz >= nta) continue;     // outside the window: edge (acc)
-        mask &= col_mask(J, ntx);
-        if (ntz * FT + row >= nw) mask = 0;
-        if (!__any(mask != 0)) continue;
-        uint32_t nti = 0;
-        bool fresh = false;
-        if (lane == 0) nti = tile_get(B, job, ntx, ntz, &fresh);
-        nti = __shfl(nti, 0);
-        if (nti == 0xffffffffu) {
-            if (lane == 0) atomicOr(&B.ctl[FLOOD_OVF], OVF_TILES);
-            continue;
+    const int rowv = lane & 31;
+    // (every lane takes part in each shuffle: a lane reading an inactive lane's value gets nothing)
+    const uint32_t nr = __shfl(NR, rowv), nr_m1 = __shfl(NR, (rowv + FT - 1) & (FT - 1)),
+                   nr_p1 = __shfl(NR, (rowv + 1) & (FT - 1));
+    const uint32_t prv = rowv > 0 ? nr_m1 : 0u, nxt = rowv < FT - 1 ? nr_p1 : 0u;
+    const uint32_t up0 = __shfl(NR, 0), dn31 = __shfl(NR, FT - 1);
+    auto dxd = [](int d) { return d == 0 || d == 4 || d == 6 ? -1 : (d == 1 || d == 5 || d == 7 ? 1 : 0); };
+    auto dzd = [](int d) { return d == 2 || d == 4 || d == 5 ? -1 : (d == 3 || d == 6 || d == 7 ? 1 : 0); };
+    // the mask this lane ORs into row `row` of the neighbour in direction d (0 if none)
+    auto clip = [&](int d, int row, uint32_t m) -> uint32_t {
+        const int ntx = tx + dxd(d), ntz = tz + dzd(d);
+        if (ntx < 0 || ntz < 0 || ntx >= nta || ntz >= nta) return 0u;   // outside the window: edge (acc)
+        if (ntz * FT + row >= nw) return 0u;
+        return m & col_mask(J, ntx);
+    };
+    const int sd = lane < 32 ? 0 : 1;   // this lane's side direction, row rowv
+    uint32_t smask;
+    if (sd == 0) smask = ((nr & 1u) || (c8 && ((prv & 1u) || (nxt & 1u)))) ? (1u << 31) : 0u;
+    else smask = ((nr >> 31) || (c8 && ((prv >> 31) || (nxt >> 31)))) ? 1u : 0u;
+    smask = clip(sd, rowv, smask);
+    const int vd = lane + 2;            // lanes 0-5: direction lane + 2
+    int vrow = 0;
+    uint32_t vmask = 0;
+    if (lane < 6 && (lane < 2 || c8)) {
+        uint32_t vu = up0, vdn = dn31;
+        if (c8) {
+            vu |= (up0 << 1) | (up0 >> 1);
+            vdn |= (dn31 << 1) | (dn31 >> 1);
         }
-        Tile &N = B.tiles[nti];
-        bool grew = false;
-        if (mask) {
-            const uint32_t o = atomicOr(&N.in[row], mask);
-            grew = (o | mask) != o;
-        }
-        if (__any(grew) && lane == 0) tile_touch(B, nti);
+        if (vd == 2) { vrow = FT - 1; vmask = vu; }
+        else if (vd == 3) { vrow = 0; vmask = vdn; }
+        else if (vd == 4) { vrow = FT - 1; vmask = (up0 & 1u) ? (1u << 31) : 0u; }
+        else if (vd == 5) { vrow = FT - 1; vmask = (up0 >> 31) ? 1u : 0u; }
+        else if (vd == 6) { vrow = 0; vmask = (dn31 & 1u) ? (1u << 31) : 0u; }
+        else { vrow = 0; vmask = (dn31 >> 31) ? 1u : 0u; }
+        vmask = clip(vd, vrow, vmask);
     }
+    const uint64_t bs = __ballot(smask != 0u), bv = __ballot(vmask != 0u);
+    uint32_t want = (uint32_t(bs) ? 1u : 0u) | (uint32_t(bs >> 32) ? 2u : 0u) | (uint32_t(bv & 0x3fu) << 2);
+    if (!want) return 0u;
+    // neighbour tiles, looked up (or allocated) by lane d in parallel
+    uint32_t nti = 0xffffffffu;
+    bool fresh = false;
+    if (lane < 8 && ((want >> lane) & 1u)) {
+        nti = tile_get(B, job, tx + dxd(lane), tz + dzd(lane), &fresh);
+        if (nti == 0xffffffffu) atomicOr(&B.ctl[FLOOD_OVF], OVF_TILES);
+    }
+    want &= ~uint32_t(__ballot(lane < 8 && nti == 0xffffffffu));
+    const uint32_t nti_s = __shfl(nti, sd), nti_v = __shfl(nti, lane < 6 ? vd : 0);
+    bool gs = false, gv = false;
+    if (smask && ((want >> sd) & 1u)) {
+        const uint32_t o = atomicOr(&B.tiles[nti_s].in[rowv], smask);
+        gs = (o | smask) != o;
+    }
+    if (vmask && ((want >> vd) & 1u)) {
+        const uint32_t o = atomicOr(&B.tiles[nti_v].in[vrow], vmask);
+        gv = (o | vmask) != o;
+    }
+    const uint64_t gsb = __ballot(gs), gvb = __ballot(gv);
+    const uint32_t grew = (uint32_t(gsb) ? 1u : 0u) | (uint32_t(gsb >> 32) ? 2u : 0u) | (uint32_t(gvb & 0x3fu) << 2);
+    if (lane < 8 && ((grew >> lane) & 1u)) tile_touch(B, nti);
     return 0u;
 }
 
