@@ -131,6 +131,8 @@ __global__ __launch_bounds__(256) void k_points(const ClimSeed *cs, SurfGrid g, 
     const ClimSeed &C = cs[sidx[k]];
     if (what == 16) outd[k] = noise_sample(C, DSX_CH_T, xs[k], 0, zs[k]);
     else if (what == 17) outd[k] = proxy_prob(C, xs[k], zs[k]);
+    else if (what == 20) outd[k] = noise_octaves(C, DSX_CH_C, xs[k], zs[k], 3);   // desert_proxy.hpp:14-24
+    else if (what == 21) outd[k] = noise_octaves(C, DSX_CH_E, xs[k], zs[k], 2);
     else if (what == 19) {   // surface.hpp:79-83
         int64_t q[6];
         climate_q(C, xs[k], 64, zs[k], q);

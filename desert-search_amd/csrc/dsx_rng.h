@@ -79,6 +79,28 @@ DSX_HD double temperature_phase(uint64_t seed, double *fy) {
     return s;
 }
 
+// |fy - 1/2| summed over the two sides of octave j (the j-th octave of non-zero amplitude) of channel
+// ch: the initialisation of Perlin::init (standalone.hpp:55-61, 123-139) up to the y offset.
+DSX_HD double octave_phase(uint64_t seed, int ch, int j) {
+    const dsx_noise_cfg &cfg = DSX_NOISE[ch];
+    Xoro w = Xoro::seeded(seed);
+    const uint64_t a = w.next(), b = w.next();
+    Xoro r = Xoro::raw(a ^ cfg.fork_a, b ^ cfg.fork_b);
+    double s = 0;
+    for (int side = 0; side < 2; side++) {
+        const uint64_t ra = r.next(), rb = r.next();
+        int i = 0, cnt = -1;
+        for (; i < cfg.len; i++)
+            if (cfg.amp[i] != 0 && ++cnt == j) break;
+        const uint64_t *salt = DSX_OCT_SALT[cfg.first + i + 10];
+        Xoro q = Xoro::raw(ra ^ salt[0], rb ^ salt[1]);
+        q.uniform();
+        const double oy = q.uniform() * 256;
+        s += fabs((oy - floor(oy)) - .5);
+    }
+    return s;
+}
+
 DSX_HD double fade(double t) { return t * t * t * (t * (t * 6. - 15.) + 10.); }
 
 // Minecraft's 16 gradient vectors as (gx*x + gy*y) + gz*z with components in {0, +-1}

@@ -605,7 +605,8 @@ int dsx_flood_masks(dsx_ctx *c, dsx_tile_mask *out, uint64_t cap, uint64_t *n_ou
 int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_t *sidx, const int32_t *xs,
                const int32_t *zs, uint64_t n, int what, float cut, int8_t *out8, double *outd) {
     const bool hottest = what == DSX_POINT_HOTTEST;
-    const bool dbl = what == DSX_POINT_T || what == DSX_POINT_PROXYPROB || hottest;
+    const bool dbl = what == DSX_POINT_T || what == DSX_POINT_PROXYPROB || what == DSX_POINT_PROXY_C ||
+                     what == DSX_POINT_PROXY_E || hottest;
     if (!c || (n && (!sidx || !xs || !zs)) || (!dbl && (what < 0 || what > 3) && what != DSX_POINT_BIOME) || (dbl && n && !outd) ||
         ((!dbl || hottest) && n && !out8) || (nseeds && !seeds))
         return DSX_E_ARG;

@@ -51,7 +51,7 @@ FLOOD_JOB = np.dtype([("seed", "<u8"), ("kind", "<i4"), ("cut", "<f4"), ("qx", "
 AREA = np.dtype([("cells", "<u8"), ("sampled", "<u8"), ("minx", "<i4"), ("maxx", "<i4"), ("minz", "<i4"),
                  ("maxz", "<i4"), ("edge", "<i4"), ("border", "<i4")])
 PRED_DESERT_STATE, PRED_HOT, PRED_PROXY, PRED_DESERT = 0, 1, 2, 3
-POINT_T, POINT_PROXYPROB, POINT_HOTTEST, POINT_BIOME = 16, 17, 18, 19
+POINT_T, POINT_PROXYPROB, POINT_HOTTEST, POINT_BIOME, POINT_PROXY_C, POINT_PROXY_E = 16, 17, 18, 19, 20, 21
 TILE_MASK = np.dtype([("job", "<u4"), ("tx", "<i4"), ("tz", "<i4"), ("rows", "<u4", (32,))])
 AREA_QUERY = np.dtype([("seed", "<u8"), ("x", "<i4"), ("z", "<i4"), ("half", "<i4"), ("pad", "<i4")])
 
@@ -357,12 +357,17 @@ class Engine:
         return p
 
     def search(self, prefix, mode, start=0, count=0, seeds=None, params=None):
-        """Run the whole pipeline; writes PREFIX.csv / PREFIX.seeds.csv (with headers) and returns the counts."""
+        """Run the whole pipeline; writes PREFIX.csv / PREFIX.seeds.csv (and PREFIX.zones.csv for study=1),
+        with headers, and returns the counts."""
         p = params or self.search_params()
         with open(prefix + ".csv", "w") as f:
             f.write("seed,fyA,fyB,maxta,maxtb,rank,x,z,score,t0,c0,e0,proxy,s8,s4,edge,hot\n")
         with open(prefix + ".seeds.csv", "w") as f:
             f.write("seed,fyA,fyB,maxta,maxtb,candidates,sites,best\n")
+        if p.study:
+            with open(prefix + ".zones.csv", "w") as f:
+                f.write("seed,phT,amp,phE0,phE1,phC0,phC1,rank,x,z,t0,c0,e0,hot,proxy,f_ocean,f_badl,f_mang,f_good,"
+                        "mean_c,mean_e,sd_e\n")
         s = np.ascontiguousarray(seeds if seeds is not None else [], dtype=np.uint64)
         c = SearchCounts()
         self._check(self.lib.dsx_search(self.h, ctypes.byref(p), mode, start, count, s.ctypes.data if len(s) else None,

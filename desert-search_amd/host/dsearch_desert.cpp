@@ -69,7 +69,6 @@ int main(int argc, char **argv) {
         if (!P.gpu) { std::fprintf(stderr, "gate=1 needs gpu=1\n"); return 2; }
         P.from_stdin = 1;
     }
-    if (P.study) { std::fprintf(stderr, "study=1 is not supported by this build\n"); return 2; }
     FILE *out = std::fopen((prefix + ".csv").c_str(), "wx");
     FILE *sout = std::fopen((prefix + ".seeds.csv").c_str(), "wx");
     if (!out || !sout) {
@@ -80,6 +79,16 @@ int main(int argc, char **argv) {
     std::fprintf(sout, "seed,fyA,fyB,maxta,maxtb,candidates,sites,best\n");
     std::fclose(out);
     std::fclose(sout);
+    if (P.study) {   // dsearch_desert.cpp:147-150
+        FILE *zout = std::fopen((prefix + ".zones.csv").c_str(), "wx");
+        if (!zout) {
+            std::fprintf(stderr, "cannot create %s outputs (exist?)\n", prefix.c_str());
+            return 2;
+        }
+        std::fprintf(zout, "seed,phT,amp,phE0,phE1,phC0,phC1,rank,x,z,t0,c0,e0,hot,proxy,f_ocean,f_badl,f_mang,f_good,mean_c,"
+                           "mean_e,sd_e\n");
+        std::fclose(zout);
+    }
     std::signal(SIGINT, on_signal);
     std::signal(SIGTERM, on_signal);
 

@@ -26,7 +26,7 @@ int run_group(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g);
 // The group's output in the reference's formats: PREFIX.csv rows, PREFIX.seeds.csv rows, and the
 // stderr lines (CANDIDATE; "best" lines when `best` is given: it holds the running best).
 struct GroupText {
-    std::string csv, seeds, log;
+    std::string csv, seeds, log, zones;
     std::vector<std::string> verified;   // verified CANDIDATE lines (mcverify=1), without time and run
     uint64_t scanned = 0, sites = 0;
 };

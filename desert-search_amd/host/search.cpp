@@ -10,6 +10,7 @@
 // (hotcovered, covered, seen, std::sort of sites and proxies) is kept exactly as in the reference.
 #include "../../include/dsx.h"
 #include "../csrc/dsx_rng.h"
+#include "../data/mc_tables.inc"
 #include "pipeline.h"
 #include <algorithm>
 #include <chrono>
@@ -206,7 +207,8 @@ struct Sched {
         for (auto *r : rs) groups[{r->what, r->cut}].push_back(r);
         for (auto &[key, g] : groups) {
             const bool hottest = key.first == DSX_POINT_HOTTEST;
-            const bool dbl = key.first == DSX_POINT_T || key.first == DSX_POINT_PROXYPROB || hottest;
+            const bool dbl = key.first == DSX_POINT_T || key.first == DSX_POINT_PROXYPROB || key.first == DSX_POINT_PROXY_C ||
+                             key.first == DSX_POINT_PROXY_E || hottest;
             size_t i0 = 0;
             while (i0 < g.size()) {
                 std::vector<uint64_t> seeds;
@@ -389,6 +391,7 @@ struct SeedOut {
     std::vector<std::pair<int, int>> best_xy;
     std::vector<double> best_proxy;
     std::vector<int> best_step;
+    std::vector<std::string> zones;   // PREFIX.zones.csv rows (study=1)
     bool done = false;
 };
 
@@ -410,6 +413,89 @@ inline dsx_flood_job fjob(uint64_t seed, int kind, float cut, int qx, int qz, in
     j.half = half;
     j.conn8 = conn8;
     return j;
+}
+
+// Counted cells of a 4-connected flood in the order dense_flood_bridged (dense_flood.hpp:53-84) tests
+// them: its depth-first traversal replayed with the flood's own mask as the predicate. A cell outside
+// the mask that the traversal touches read 0 there, so the replay takes the same path.
+void flood_order(const FloodOut &f, std::vector<std::pair<int, int>> &cells) {
+    cells.clear();
+    if (!f.a.cells) return;
+    const int n = 2 * f.half + 1;
+    thread_local std::vector<uint8_t> seen;
+    thread_local std::vector<int> stack;
+    seen.assign(size_t(n) * n, 0);
+    stack.clear();
+    auto touch = [&](int i, int j) {
+        if (i < 0 || j < 0 || i >= n || j >= n) return;
+        uint8_t &s = seen[size_t(j) * n + i];
+        if (s) return;
+        s = 1;
+        if (!f.counted(i, j)) return;
+        cells.push_back({f.qx + (i - f.half) * f.step, f.qz + (j - f.half) * f.step});
+        stack.push_back(j * n + i);
+    };
+    touch(f.half, f.half);
+    while (!stack.empty()) {
+        const int k = stack.back();
+        stack.pop_back();
+        const int i = k % n, j = k / n;
+        touch(i - 1, j);
+        touch(i + 1, j);
+        touch(i, j - 1);
+        touch(i, j + 1);
+    }
+}
+
+// study=1 (dsearch_desert.cpp:401-420): for every proxy component of the seed, in the order found, the
+// hot zone (T >= 0.55 at step 64) flooded from the component's start, and the low-octave C/E census of
+// its cells. study_points lists the counted cells of every zone (off: first cell of each zone);
+// study_rows writes one PREFIX.zones.csv row per zone from their C and E values.
+void study_points(const std::vector<FloodOut> &hz, PointsReq &q, std::vector<size_t> &off) {
+    std::vector<std::pair<int, int>> cells;
+    off.assign(hz.size() + 1, 0);
+    for (size_t k = 0; k < hz.size(); k++) {
+        flood_order(hz[k], cells);
+        for (auto [x, z] : cells) {
+            q.xs.push_back(x);
+            q.zs.push_back(z);
+        }
+        off[k + 1] = q.xs.size();
+    }
+}
+void study_rows(const dsx_search_params &P, SeedOut &O, const std::vector<FloodOut> &hz, const std::vector<double> &cv,
+                const std::vector<double> &evv, const std::vector<size_t> &off) {
+    const uint64_t seed = O.seed;
+    const double phT = std::abs(O.fy[0] - .5) + std::abs(O.fy[1] - .5);
+    const double phE0 = dsx::octave_phase(seed, DSX_CH_E, 0), phE1 = dsx::octave_phase(seed, DSX_CH_E, 1);
+    const double phC0 = dsx::octave_phase(seed, DSX_CH_C, 0), phC1 = dsx::octave_phase(seed, DSX_CH_C, 1);
+    for (size_t k = 0; k < hz.size(); k++) {
+        double sc = 0, se = 0, se2 = 0;
+        long nh = 0, no = 0, nb = 0, nm = 0, ng = 0;
+        for (size_t p = off[k]; p < off[k + 1]; p++) {
+            const double c = cv[p], e = evv[p];
+            nh++;
+            sc += c;
+            se += e;
+            se2 += e * e;
+            no += c < -0.19;
+            nb += e < 0.05;
+            nm += e > 0.55;
+            const int i = std::clamp(int(std::floor((c + 0.8) / 0.05)), 0, 39);
+            const int j = std::clamp(int(std::floor((e + 1.0) / 0.05)), 0, 39);
+            ng += DSX_DPROB[i][j] >= P.pcut;
+        }
+        const double d = double(std::max(1L, nh)), me = se / d;
+        const Eval &z = O.ev[k];
+        const Site &st = O.sites[z.rank];
+        char line[512];
+        std::snprintf(line, sizeof line,
+                      "%lld,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f,%zu,%d,%d,%.4f,%.4f,%.4f,%.2f,%.2f,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f\n",
+                      (long long)seed, phT, O.maxta + O.maxtb, phE0, phE1, phC0, phC1, z.rank, z.x * 4, z.z * 4, st.t, st.c,
+                      st.e, hz[k].a.cells * 64.0 * 64 * 16 / 1e6, z.proxy, double(no) / d, double(nb) / d, double(nm) / d,
+                      double(ng) / d, sc / d, me, std::sqrt(std::max(0.0, se2 / d - me * me)));
+        O.zones.push_back(line);
+    }
 }
 
 // dsearch_desert.cpp:309-533 for one seed whose sites are known.
@@ -656,6 +742,18 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
         }
     }
     tl.reset();
+    if (P.study && !ev.empty()) {   // before the sort: rows in the order the components were found
+        FloodReq hz{&S, {}, true, {}, {}};
+        for (const Eval &e : ev) hz.jobs.push_back(fjob(seed, DSX_PRED_HOT, 0.f, e.x, e.z, 64, 24000 / 64, 0));
+        co_await hz;
+        PointsReq qc{&S, seed, DSX_POINT_PROXY_C, 0.f, {}, {}, {}, {}, {}};
+        std::vector<size_t> off;
+        study_points(hz.out, qc, off);
+        PointsReq qe{&S, seed, DSX_POINT_PROXY_E, 0.f, qc.xs, qc.zs, {}, {}, {}};
+        co_await qc;
+        co_await qe;
+        study_rows(P, O, hz.out, qc.vd, qe.vd, off);
+    }
     // ---- exact floods of the best proxies (dsearch_desert.cpp:494-533). The ev entries are
     // independent: ring searches of all of them in one batch, then their step-8 and step-4 floods.
     std::sort(ev.begin(), ev.end(), [](const Eval &a, const Eval &b) { return a.proxy > b.proxy; });
@@ -866,6 +964,7 @@ void format_group(const Group &g, const dsx_search_params &P, GroupText &t, doub
         std::snprintf(line, sizeof line, "%lld,%.4f,%.4f,%.4f,%.4f,%zu,%zu,%.3f\n", (long long)O->seed, O->fy[0], O->fy[1],
                       O->maxta, O->maxtb, O->ncand, O->sites.size(), O->best);
         t.seeds += line;
+        for (auto &z : O->zones) t.zones += z;
     }
 }
 
@@ -1082,7 +1181,7 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
     if (!ctx || !pp || !prefix || !counts || mode < 0 || mode > 2 || (mode == DSX_SEARCH_LIST && nseeds && !seeds))
         return DSX_E_ARG;
     const dsx_search_params P = *pp;
-    if (P.study || P.nms <= 0 || P.max_sites < 0 || P.workers < 0 || P.workers > 64) return DSX_E_ARG;
+    if (P.nms <= 0 || P.max_sites < 0 || P.workers < 0 || P.workers > 64) return DSX_E_ARG;
     const int nw = P.workers ? P.workers : 16;
     std::vector<dsx_ctx *> wctx;
     const int ns = int(std::min<uint64_t>(env_u64("DSX_SCAN_WORKERS", 0), 8));
@@ -1099,6 +1198,14 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
         return DSX_E_ARG;
     }
     FILE *vout = P.mcverify ? std::fopen((pre + ".verified").c_str(), "a") : nullptr;
+    FILE *zout = P.study ? std::fopen((pre + ".zones.csv").c_str(), "a") : nullptr;
+    if (P.study && !zout) {
+        std::fclose(out);
+        std::fclose(sout);
+        if (vout) std::fclose(vout);
+        dsx::g_last_error = "cannot open " + pre + ".zones.csv";
+        return DSX_E_ARG;
+    }
     const size_t slash = pre.find_last_of('/');
     const std::string run = slash == std::string::npos ? pre : pre.substr(slash + 1);
     static double global_best = 0;
@@ -1122,6 +1229,7 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
                     format_group(*g, P, t, &global_best);
                     std::fputs(t.csv.c_str(), out);
                     std::fputs(t.seeds.c_str(), sout);
+                    if (zout) std::fputs(t.zones.c_str(), zout);
                     std::fputs(t.log.c_str(), stderr);
                     if (vout && !t.verified.empty()) {
                         char ts[32];
@@ -1193,6 +1301,7 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
     std::fclose(out);
     std::fclose(sout);
     if (vout) std::fclose(vout);
+    if (zout) std::fclose(zout);
     if (rc) dsx::g_last_error = first_err;
     return rc;
 }

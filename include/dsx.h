@@ -149,6 +149,10 @@ enum { DSX_POINT_HOTTEST = 18 };
 /* what = DSX_POINT_BIOME: SurfaceSampler::biome (surface.hpp:79-83), the surface biome id at quart (x, 64, z)
  * from the exact climate (grid lookup, else the nearest-box search), in out8[k] read as uint8_t. */
 enum { DSX_POINT_BIOME = 19 };
+/* what = DSX_POINT_PROXY_C / _E: DesertProxy::octaves (desert_proxy.hpp:14-24) of continentalness over
+ * octaves 0-2 / erosion over octaves 0-1 at quart (x, z), in outd (the zone features of study=1,
+ * dsearch_desert.cpp:402-412). */
+enum { DSX_POINT_PROXY_C = 20, DSX_POINT_PROXY_E = 21 };
 int dsx_points(dsx_ctx *ctx, const uint64_t *seeds, uint64_t nseeds, const uint32_t *sidx, const int32_t *xs,
                const int32_t *zs, uint64_t n, int what, float cut, int8_t *out8, double *outd);
 
@@ -253,9 +257,11 @@ typedef struct {
 /* The per-seed pipeline of dsearch_desert.cpp:173-557 (gate -> tables/scan -> sites -> hot zones ->
  * proxy floods -> exact floods) with every numeric stage on the GPU. Appends rows to PREFIX.csv and
  * PREFIX.seeds.csv in the reference's formats (opened in append mode; the caller writes the headers),
- * and with mcverify=1 the verified CANDIDATE lines to PREFIX.verified,
+ * with mcverify=1 the verified CANDIDATE lines to PREFIX.verified, and with study=1 (no exact floods)
+ * the per-zone feature rows of dsearch_desert.cpp:401-420 to PREFIX.zones.csv (append mode, header by
+ * the caller),
  * prints CANDIDATE / best lines to stderr as the reference does, and accumulates counts.
- * Rows of one seed are contiguous; seeds appear in batch order. study=1 is not supported (DSX_E_ARG). */
+ * Rows of one seed are contiguous; seeds appear in batch order. */
 int dsx_search(dsx_ctx *ctx, const dsx_search_params *p, int mode, uint64_t start, uint64_t count, const uint64_t *seeds,
                uint64_t nseeds, const char *prefix, dsx_search_counts *counts);
 

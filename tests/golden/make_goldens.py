@@ -9,6 +9,8 @@ inputs and the reference's outputs.
   r3m.*           dsearch_desert 0 3000000 P 8          (larger range: exact step-8 floods)
   rg1m.*          dsearch_desert 0 0 P 8 stdin=1 < gated_1e12_1m.txt (seeds gated with the GPU-gate
                   rules over k in [1e12, 1e12 + 2^20); the list itself is an input fixture)
+  st1m.*          the same with study=1 (no exact floods; st1m.zones.csv.gz holds the sorted per-zone
+                  feature rows of PREFIX.zones.csv; `python make_goldens.py study` rewrites only these)
   dflood.jsonl    dflood SEED X Z STEP at the verified desert finds, steps 8 and 4 (all rows) and
                   step 1 (the first rows)
   biomemap.npz    biomemap SEED X Z HALF STEP at verified desert finds (n x n uint8 maps; `python
@@ -18,6 +20,7 @@ inputs and the reference's outputs.
 Sorted CSV rows are stored (the reference writes rows in thread completion order).
 """
 import concurrent.futures as cf
+import gzip
 import hashlib
 import json
 import os
@@ -39,6 +42,10 @@ def run_search(tag, args, stdin=None):
         rows = sorted(open(pre + ".csv").read().splitlines()[1:])
         srows = sorted(open(pre + ".seeds.csv").read().splitlines()[1:])
         js = json.load(open(pre + ".json"))
+        zrows = sorted(open(pre + ".zones.csv").read().splitlines()[1:]) if os.path.exists(pre + ".zones.csv") else None
+    if zrows is not None:
+        with gzip.GzipFile(os.path.join(HERE, tag + ".zones.csv.gz"), "wb", mtime=0) as f:
+            f.write(("\n".join(zrows) + "\n").encode())
     keep = {k: js[k] for k in ("start", "done", "passed", "scanned", "sites")}
     keep["seeds_rows"] = len(srows)
     keep["seeds_sha256"] = hashlib.sha256("\n".join(srows).encode()).hexdigest()
@@ -92,11 +99,17 @@ def dfunc_goldens():
         print("dfunc", seeds, pts, len(out.splitlines()))
 
 
+def study_goldens():
+    run_search("st1m", ["0", "0", "8", "stdin=1", "study=1"], stdin=os.path.join(HERE, "gated_1e12_1m.txt"))
+
+
 def main():
     if sys.argv[1:2] == ["biomemap"]:
         return biomemap_goldens()
     if sys.argv[1:2] == ["dfunc"]:
         return dfunc_goldens()
+    if sys.argv[1:2] == ["study"]:
+        return study_goldens()
     run_search("c1", ["0", "200000", "8"])
     run_search("r3m", ["0", "3000000", "8"])
     run_search("rg1m", ["0", "0", "8", "stdin=1"], stdin=os.path.join(HERE, "gated_1e12_1m.txt"))
@@ -113,6 +126,7 @@ def main():
     print("dflood", len(res))
     biomemap_goldens()
     dfunc_goldens()
+    study_goldens()
 
 
 if __name__ == "__main__":

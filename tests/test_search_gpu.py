@@ -43,6 +43,47 @@ def test_gated_list_pipeline(engine):
         _check("rg1m", pre, counts)
 
 
+def _check_zones(tag, prefix):
+    """PREFIX.zones.csv (study=1) against the reference's sorted zone rows."""
+    import gzip
+    rows = sorted(open(prefix + ".zones.csv").read().splitlines()[1:])
+    gold = gzip.open(os.path.join(GOLDEN, tag + ".zones.csv.gz"), "rt").read().splitlines()
+    assert len(rows) == len(gold)
+    bad = [(a, b) for a, b in zip(rows, gold) if a != b]
+    assert not bad, f"{len(bad)} zone rows differ, first: {bad[0]}"
+
+
+def test_study_zones(engine):
+    """study=1 on the 92 gated seeds: no exact floods (s8 = -1 in every row), and one PREFIX.zones.csv
+    row per proxy component with its step-64 hot zone and low-octave C/E census."""
+    dsx = load_package().dsx
+    seeds = [int(l.split()[0]) % (1 << 64) for l in open(os.path.join(GOLDEN, "gated_1e12_1m.txt"))]
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "st1m")
+        counts = engine.search(pre, dsx.SEARCH_LIST, seeds=seeds, params=engine.search_params(study=1))
+        _check("st1m", pre, counts)
+        _check_zones("st1m", pre)
+
+
+def test_cli_study(engine):
+    """dsearch_desert stdin=1 study=1: the zones file gets the reference's header and rows; an existing
+    zones file is refused like the other outputs."""
+    exe = os.path.join(REPO, "desert-search_amd", "dsearch_desert")
+    lines = open(os.path.join(GOLDEN, "gated_1e12_1m.txt")).read()
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "st")
+        r = subprocess.run([exe, "0", "0", pre, "8", "stdin=1", "study=1"], input=lines, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+        assert open(pre + ".zones.csv").readline().startswith("seed,phT,amp,phE0,phE1,phC0,phC1,rank,x,z,")
+        _check_zones("st1m", pre)
+        os.remove(pre + ".csv")
+        os.remove(pre + ".seeds.csv")
+        r2 = subprocess.run([exe, "0", "0", pre, "8", "stdin=1", "study=1"], input="", capture_output=True, text=True,
+                            timeout=60)
+        assert r2.returncode == 2 and "exist?" in r2.stderr
+
+
 def test_3m_pipeline(engine):
     dsx = load_package().dsx
     with tempfile.TemporaryDirectory() as td:
