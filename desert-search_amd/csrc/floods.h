@@ -83,9 +83,11 @@ void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_
 // The persistent flood kernel: `blocks` workgroups take queued tiles until no work is pending. A tile
 // is evaluated lazily, by 8x8 sub-blocks the flood touches; every processing grows its reached set to
 // a fixpoint and queues the neighbours it touched. ev0/ev1 (optional): HIP events around it.
-// pre: 1 = FP32 pre-pass on the predicates' temperature test (identical results)
+// pre: 1 = FP32 pre-pass on the predicates' temperature test (identical results); light: 1 when every
+// tiled job is a hot-zone or proxy flood (selects the instance without the biome predicates)
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
-                      uint32_t blocks, int pre, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, int tprof = 0);
+                      uint32_t blocks, int pre, int light, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                      int tprof = 0);
 // Reduce every allocated tile (count read on the device) into res, export counted masks, and return
 // the tiles and hash slots to the clean state.
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRes *res, struct TileMask *masks,

@@ -72,7 +72,7 @@ constexpr int SB = 8, NSB = FT / SB;
 // Evaluate sub-block sb of the tile at (tx, tz) with one wave (one cell per lane) and OR its bits into
 // the LDS row masks (pass = v != 0, cnt = v == 2, hot = the predicate's temperature test passed; cells
 // outside the window or the world are none of them).
-template <int PRE>
+template <int PRE, int LIGHT>
 __device__ __forceinline__ void eval_subblock(int sb, int tx, int tz, const FloodJob &J, const ClimSeed &C,
                                               const SurfGrid &g, uint32_t *s_pass, uint32_t *s_cnt, uint32_t *s_hot) {
     const int lane = threadIdx.x & 63;
@@ -85,7 +85,13 @@ __device__ __forceinline__ void eval_subblock(int sb, int tx, int tz, const Floo
     if (i < nw && j < nw) {
         const int x = J.qx + (i - J.half) * J.step, z = J.qz + (j - J.half) * J.step;
         if (x >= -WORLD_Q && x <= WORLD_Q && z >= -WORLD_Q && z <= WORLD_Q)
-            v = PRE ? predicate_pre(J.kind, C, g, x, z, J.cut, &hot) : predicate(J.kind, C, g, x, z, J.cut, &hot);
+        {
+            if constexpr (LIGHT)   // hot-zone and proxy floods only: no biome lookup in this instance
+                v = J.kind == PRED_HOT ? (PRE ? hot_state_pre(C, x, z) : hot_state(C, x, z))
+                                       : (PRE ? proxy_state_pre(C, x, z, J.cut, &hot) : proxy_state(C, x, z, J.cut, &hot));
+            else
+                v = PRE ? predicate_pre(J.kind, C, g, x, z, J.cut, &hot) : predicate(J.kind, C, g, x, z, J.cut, &hot);
+        }
     }
     const uint64_t mp = __ballot(v != 0), mc = __ballot(v == 2), mh = __ballot(hot);
     if (lane < SB) {   // lane rr: row bz*8 + rr of the sub-block
@@ -212,8 +218,10 @@ __device__ uint32_t prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J
 }
 
 // Persistent workgroups over the work queue (NT threads each; wave 0 propagates, all waves evaluate).
-// OCC: waves per SIMD the register allocation must allow.
-template <int NT, int OCC, int PRE>
+// OCC: waves per SIMD the register allocation must allow. LIGHT: an instance for batches whose tiled
+// floods are all hot-zone or proxy floods; it leaves out the biome predicates and so needs fewer
+// registers (more workgroups resident beside the other streams' kernels).
+template <int NT, int OCC, int PRE, int LIGHT>
 __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B,
                                                        int backoff, int spec, int tprof) {
     constexpr int NWAVE = NT / 64;
@@ -366,7 +374,7 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                 // the k-th needed sub-block goes to wave k % NWAVE
                 int k = 0;
                 for (uint32_t m = need; m; m &= m - 1, k++)
-                    if (k % NWAVE == wave) eval_subblock<PRE>(__ffs(m) - 1, tx, tz, J, C, g, s_pass, s_cnt, s_hot);
+                    if (k % NWAVE == wave) eval_subblock<PRE, LIGHT>(__ffs(m) - 1, tx, tz, J, C, g, s_pass, s_cnt, s_hot);
                 __syncthreads();
                 if (tid == 0) {
                     s_evald |= need;
@@ -533,7 +541,7 @@ void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_
     if (n) hipLaunchKernelGGL(k_flood_stage, dim3((n + 255) / 256), dim3(256), 0, st, h_up, n, n_small, n_big, d_up, res, B);
 }
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
-                      uint32_t blocks, int pre, hipEvent_t ev0, hipEvent_t ev1, int tprof) {
+                      uint32_t blocks, int pre, int light, hipEvent_t ev0, hipEvent_t ev1, int tprof) {
     // workgroup size (DSX_FLOOD_NT = 64 to 1024 threads); `blocks` is given for 256-thread
     // workgroups and scaled so that the launch holds the same number of waves
     static const int nt = [] {
@@ -553,13 +561,20 @@ void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, 
         const char *v = std::getenv("DSX_FLOOD_SPEC");   // 0 turns the speculative sub-blocks off
         return v && *v ? std::atoi(v) : 1;
     }();
-    decltype(&k_flood_run<256, 1, 0>) kern;
-    if (nt == 64) kern = pre ? k_flood_run<64, 1, 1> : k_flood_run<64, 1, 0>;
-    else if (nt == 128) kern = pre ? k_flood_run<128, 1, 1> : k_flood_run<128, 1, 0>;
-    else if (nt == 512) kern = pre ? k_flood_run<512, 1, 1> : k_flood_run<512, 1, 0>;
-    else if (nt == 1024) kern = pre ? k_flood_run<1024, 1, 1> : k_flood_run<1024, 1, 0>;
-    else kern = pre ? (occ4 ? k_flood_run<256, 4, 1> : k_flood_run<256, 1, 1>)
-                    : (occ4 ? k_flood_run<256, 4, 0> : k_flood_run<256, 1, 0>);
+    static const bool light_on = [] {
+        const char *v = std::getenv("DSX_FLOOD_LIGHT");   // 0: always the full instance
+        return !(v && *v == '0');
+    }();
+    light = light && light_on;
+    decltype(&k_flood_run<256, 1, 0, 0>) kern;
+#define DSX_FR(N, O) (pre ? (light ? k_flood_run<N, O, 1, 1> : k_flood_run<N, O, 1, 0>) \
+                          : (light ? k_flood_run<N, O, 0, 1> : k_flood_run<N, O, 0, 0>))
+    if (nt == 64) kern = DSX_FR(64, 1);
+    else if (nt == 128) kern = DSX_FR(128, 1);
+    else if (nt == 512) kern = DSX_FR(512, 1);
+    else if (nt == 1024) kern = DSX_FR(1024, 1);
+    else kern = occ4 ? DSX_FR(256, 4) : DSX_FR(256, 1);
+#undef DSX_FR
     const dim3 grid(nt <= 256 ? (blocks ? blocks : 1) * (256 / nt) : std::max(1u, blocks * 256 / nt));
     if (ev0) hipExtLaunchKernelGGL(kern, grid, dim3(nt), 0, st, ev0, ev1, 0u, jobs, cs, g, B, backoff, spec, tprof);
     else hipLaunchKernelGGL(kern, grid, dim3(nt), 0, st, jobs, cs, g, B, backoff, spec, tprof);

@@ -443,8 +443,12 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
                 h_small[n_small++] = uint32_t(k);
                 n_cls[cls]++;
             }
+    bool light = true;   // every tiled job a hot-zone or proxy flood
     for (uint64_t k = 0; k < n; k++)
-        if (cls_of(hj[k]) < 0) h_big[n_big++] = uint32_t(k);
+        if (cls_of(hj[k]) < 0) {
+            h_big[n_big++] = uint32_t(k);
+            light = light && (hj[k].kind == dsx::PRED_HOT || hj[k].kind == dsx::PRED_PROXY);
+        }
     uint8_t *d_up = S->d_up;
     const dsx::FloodJob *d_jobs = reinterpret_cast<const dsx::FloodJob *>(d_up);
     uint32_t *d_small = reinterpret_cast<uint32_t *>(d_up + n * sizeof(dsx::FloodJob));
@@ -489,7 +493,8 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
                                   max_blocks = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS", 128));
             const uint32_t blocks = uint32_t(std::clamp<size_t>(size_t(per_job) * n_big, 16, max_blocks));
             blocks_used = blocks;
-            dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, S->prepass, e0, e1, prof ? 1 : 0);
+            dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, S->prepass, light ? 1 : 0, e0, e1,
+                                  prof ? 1 : 0);
             dsx::launch_flood_acc(c->st, d_jobs, B, S->d_res, S->d_masks, S->mask_cap, n_big ? 1024 : 64);
         }
         dsx::launch_flood_publish(c->st, S->d_res, uint32_t(n), B, S->d_masks, S->h_res, S->h_ctl, S->h_masks,
