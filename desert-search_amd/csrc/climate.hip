@@ -16,6 +16,8 @@ __global__ __launch_bounds__(64) void k_climinit(const uint64_t *seeds, const ui
     const bool active = g < CLIM_NOCT * n;
     uint8_t *my = slots + lane * CSTRIDE;
     double ox = 0, oy = 0, oz = 0, freq = 0, weight = 0;
+    // seeds and dst may be in pinned host memory: each is read once per lane, dst passed on by shuffle
+    const uint32_t my_dst = active ? dst[g / CLIM_NOCT] : 0u;
     if (active) {
         const uint32_t s = g / CLIM_NOCT, slot = g % CLIM_NOCT;
         int ch = 5;
@@ -53,10 +55,11 @@ __global__ __launch_bounds__(64) void k_climinit(const uint64_t *seeds, const ui
     for (int k = 0; k < 64; k++) {
         const uint32_t gk = g0 + k;
         if (gk >= CLIM_NOCT * n) break;
-        reinterpret_cast<uint32_t *>(out[dst[gk / CLIM_NOCT]].oct[gk % CLIM_NOCT].perm)[lane] = slot32[k * (CSTRIDE / 4) + lane];
+        const uint32_t dk = __shfl(my_dst, k);
+        reinterpret_cast<uint32_t *>(out[dk].oct[gk % CLIM_NOCT].perm)[lane] = slot32[k * (CSTRIDE / 4) + lane];
     }
     if (active) {
-        ClimSeed &cs = out[dst[g / CLIM_NOCT]];
+        ClimSeed &cs = out[my_dst];
         ClimOct &o = cs.oct[g % CLIM_NOCT];
         o.ox = ox;
         o.oy = oy;

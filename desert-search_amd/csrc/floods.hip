@@ -549,10 +549,11 @@ void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, 
         const int n = v && *v ? std::atoi(v) : 256;
         return n == 64 || n == 128 || n == 512 || n == 1024 ? n : 256;
     }();
-    static const bool occ4 = [] {
+    static const int occ = [] {   // DSX_FLOOD_OCC: 4 (both instances), 5 or 6 (the light one)
         const char *v = std::getenv("DSX_FLOOD_OCC");
-        return v && std::atoi(v) >= 4;
+        return v && *v ? std::atoi(v) : 1;
     }();
+    const bool occ4 = occ >= 4;
     static const int backoff = [] {
         const char *v = std::getenv("DSX_FLOOD_BACKOFF");
         return v && *v ? std::atoi(v) : 0;
@@ -573,6 +574,8 @@ void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, 
     else if (nt == 128) kern = DSX_FR(128, 1);
     else if (nt == 512) kern = DSX_FR(512, 1);
     else if (nt == 1024) kern = DSX_FR(1024, 1);
+    else if (light && occ >= 5) kern = occ >= 6 ? (pre ? k_flood_run<256, 6, 1, 1> : k_flood_run<256, 6, 0, 1>)
+                                                 : (pre ? k_flood_run<256, 5, 1, 1> : k_flood_run<256, 5, 0, 1>);
     else kern = occ4 ? DSX_FR(256, 4) : DSX_FR(256, 1);
 #undef DSX_FR
     const dim3 grid(nt <= 256 ? (blocks ? blocks : 1) * (256 / nt) : std::max(1u, blocks * 256 / nt));

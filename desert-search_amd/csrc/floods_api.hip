@@ -84,10 +84,9 @@ struct ClimState {
     int16_t *d_cell = nullptr;
     dsx::ClimSeed *d_clim = nullptr;
     uint32_t clim_cap = 0;
-    uint64_t *d_useeds = nullptr;   // seeds to initialise, then their slots (u32)
+    uint64_t *h_useeds = nullptr;   // pinned: seeds to initialise, then their slots (u32), read by k_climinit
     std::unordered_map<uint64_t, uint32_t> cslot;   // seed -> climate slot (initialised on the device)
-    std::vector<uint64_t> miss;                      // upload sources of the last k_climinit (kept alive
-    std::vector<uint32_t> miss_slot;                 // until the call's stream synchronises)
+    std::vector<uint64_t> miss;
     // flood buffers
     dsx::FloodBufs fb{};
     dsx::FloodRes *d_res = nullptr;
@@ -139,7 +138,7 @@ static int clim_alloc(dsx_ctx *c, ClimState *&S) {
     S->grid.cell = S->d_cell;
     S->clim_cap = 4096;
     HIPCK(hipMalloc(&S->d_clim, size_t(S->clim_cap) * sizeof(dsx::ClimSeed)));
-    HIPCK(hipMalloc(&S->d_useeds, size_t(S->clim_cap) * 12));
+    HIPCK(hipHostMalloc(&S->h_useeds, size_t(S->clim_cap) * 12, hipHostMallocDefault));
     // flood engine
     dsx::FloodBufs &B = S->fb;
     B.tile_cap = 1u << 21;                    // 2M tiles x 528 B
@@ -228,15 +227,14 @@ static int clim_slots(dsx_ctx *c, ClimState *S, const uint64_t *seeds, uint32_t 
             return DSX_E_CAPACITY;
         }
     }
-    if (!miss.empty()) {
-        const uint32_t m = uint32_t(miss.size());
-        std::vector<uint32_t> &ms = S->miss_slot;
-        ms.resize(m);
-        for (uint32_t k = 0; k < m; k++) ms[k] = S->cslot[miss[k]];
-        uint32_t *d_sl = reinterpret_cast<uint32_t *>(S->d_useeds + S->clim_cap);
-        HIPCK(hipMemcpyAsync(S->d_useeds, miss.data(), size_t(m) * 8, hipMemcpyHostToDevice, c->st));
-        HIPCK(hipMemcpyAsync(d_sl, ms.data(), size_t(m) * 4, hipMemcpyHostToDevice, c->st));
-        dsx::launch_climinit(c->st, S->d_useeds, d_sl, m, S->d_clim);
+    if (!miss.empty()) {   // the kernel reads the pinned list in place (every API call ends synchronised,
+        const uint32_t m = uint32_t(miss.size());   // so no earlier k_climinit still reads it)
+        uint32_t *h_sl = reinterpret_cast<uint32_t *>(S->h_useeds + S->clim_cap);
+        for (uint32_t k = 0; k < m; k++) {
+            S->h_useeds[k] = miss[k];
+            h_sl[k] = S->cslot[miss[k]];
+        }
+        dsx::launch_climinit(c->st, S->h_useeds, h_sl, m, S->d_clim);
         HIPCK(hipGetLastError());
     }
     return DSX_OK;
@@ -303,11 +301,11 @@ void dsx_ctx_free_ext(dsx_ctx *c) {
         s = it->second;
         g_ext.erase(it);
     }
-    for (void *p : {(void *)s->d_cuts, (void *)s->d_cell, (void *)s->d_clim, (void *)s->d_useeds, (void *)s->fb.tiles,
+    for (void *p : {(void *)s->d_cuts, (void *)s->d_cell, (void *)s->d_clim, (void *)s->fb.tiles,
                     (void *)s->fb.hkeys, (void *)s->fb.wq, (void *)s->d_ctl, (void *)s->d_up,
                     (void *)s->d_res, s->d_io, (void *)s->d_masks})
         if (p) (void)hipFree(p);
-    for (void *p : {(void *)s->h_up, (void *)s->h_res, (void *)s->h_ctl, (void *)s->h_masks, s->h_io})
+    for (void *p : {(void *)s->h_up, (void *)s->h_res, (void *)s->h_ctl, (void *)s->h_masks, s->h_io, (void *)s->h_useeds})
         if (p) (void)hipHostFree(p);
     for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
     delete s;
@@ -630,23 +628,20 @@ int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_
     if ((rc = clim_slots(c, S, seeds, uint32_t(nseeds), sslot.data()))) return rc;
     for (uint64_t k = 0; k < n; k++) sidx2[k] = sslot[sidx[k]];
     if ((rc = io_reserve(c, S, n * 24 + 16))) return rc;
-    // inputs packed in pinned memory (one upload), results to their pinned twin (one download):
+    // inputs packed in pinned host memory, read by the kernel in place (each once, coalesced), and
+    // results written by the kernel straight after them (no copy dispatches):
     // [sidx | xs | zs | pad] then the doubles (or bytes), then for HOTTEST the index bytes
     const size_t in_bytes = size_t(n) * 12 + (n & 1) * 4;
     uint32_t *h_s = static_cast<uint32_t *>(S->h_io);
     std::memcpy(h_s, sidx2.data(), n * 4);
     std::memcpy(h_s + n, xs, n * 4);
     std::memcpy(h_s + 2 * n, zs, n * 4);
-    uint32_t *d_s = static_cast<uint32_t *>(S->d_io);
-    int32_t *d_x = reinterpret_cast<int32_t *>(d_s + n), *d_z = d_x + n;
-    double *d_d = reinterpret_cast<double *>(d_z + n + (n & 1));
-    int8_t *d_8 = hottest ? reinterpret_cast<int8_t *>(d_d + n) : reinterpret_cast<int8_t *>(d_d);
-    HIPCK(hipMemcpyAsync(d_s, h_s, in_bytes, hipMemcpyHostToDevice, c->st));
-    dsx::launch_points(c->st, S->d_clim, S->grid, d_s, d_x, d_z, uint32_t(n), what, cut, d_8, d_d, S->prepass);
-    HIPCK(hipGetLastError());
+    const int32_t *h_x = reinterpret_cast<const int32_t *>(h_s + n), *h_z = h_x + n;
     uint8_t *h_out = static_cast<uint8_t *>(S->h_io) + in_bytes;
-    const size_t out_bytes = dbl ? n * 8 + (hottest ? n : 0) : n;
-    HIPCK(hipMemcpyAsync(h_out, d_d, out_bytes, hipMemcpyDeviceToHost, c->st));
+    double *h_d = reinterpret_cast<double *>(h_out);
+    int8_t *h_8 = hottest ? reinterpret_cast<int8_t *>(h_d + n) : reinterpret_cast<int8_t *>(h_d);
+    dsx::launch_points(c->st, S->d_clim, S->grid, h_s, h_x, h_z, uint32_t(n), what, cut, h_8, h_d, S->prepass);
+    HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(c->st));
     if (dbl) std::memcpy(outd, h_out, n * 8);
     if (hottest) std::memcpy(out8, h_out + n * 8, n);

@@ -116,9 +116,10 @@ int tables_chunk(dsx_ctx *c, const uint64_t *seeds, uint32_t m, const dsx_scan_p
     A.w[0] = octave0_weight(DSX_CH_T);
     A.w[1] = octave0_weight(DSX_CH_C);
     A.w[2] = octave0_weight(DSX_CH_E);
+    // the init kernel reads the seeds from pinned memory in place (callers synchronise the stream before
+    // the next chunk overwrites them)
     std::memcpy(c->h_seeds, seeds, size_t(m) * 8);
-    HIPCK(hipMemcpyAsync(c->d_seeds, c->h_seeds, size_t(m) * 8, hipMemcpyHostToDevice, c->st));
-    dsx::launch_octinit(c->st, c->d_seeds, m, c->d_tsp, c->d_ncand, 2 * m);
+    dsx::launch_octinit(c->st, c->h_seeds, m, c->d_tsp, c->d_ncand, 2 * m);
     HIPCK(hipGetLastError());
     dsx::launch_tab(c->st, c->d_tsp, m, 0, 2, rT, A, c->d_tabs, c->d_bandmax);
     HIPCK(hipGetLastError());
