@@ -544,9 +544,17 @@ void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, 
                       uint32_t blocks, int pre, int light, hipEvent_t ev0, hipEvent_t ev1, int tprof) {
     // workgroup size (DSX_FLOOD_NT = 64 to 1024 threads); `blocks` is given for 256-thread
     // workgroups and scaled so that the launch holds the same number of waves
-    static const int nt = [] {
+    static const int nt_light = [] {
         const char *v = std::getenv("DSX_FLOOD_NT");
         const int n = v && *v ? std::atoi(v) : 256;
+        return n == 64 || n == 128 || n == 512 || n == 1024 ? n : 256;
+    }();
+    // the full instance (batches with exact floods: 52 octave samples per hot cell, a latency-bound
+    // chain of tiles) takes 512-thread workgroups, so that twice as many of a tile's sub-blocks are
+    // evaluated at once (DSX_FLOOD_NT_FULL overrides; 1024 needs register spills and measured slower)
+    static const int nt_full = [] {
+        const char *v = std::getenv("DSX_FLOOD_NT_FULL");
+        const int n = v && *v ? std::atoi(v) : 512;
         return n == 64 || n == 128 || n == 512 || n == 1024 ? n : 256;
     }();
     static const int occ = [] {   // DSX_FLOOD_OCC: 4 (both instances), 5 or 6 (the light one)
@@ -567,6 +575,7 @@ void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, 
         return !(v && *v == '0');
     }();
     light = light && light_on;
+    const int nt = light ? nt_light : nt_full;
     decltype(&k_flood_run<256, 1, 0, 0>) kern;
 #define DSX_FR(N, O) (pre ? (light ? k_flood_run<N, O, 1, 1> : k_flood_run<N, O, 1, 0>) \
                           : (light ? k_flood_run<N, O, 0, 1> : k_flood_run<N, O, 0, 0>))
