@@ -209,7 +209,21 @@ def main():
     ap.add_argument("--workers", type=int, default=0, help="pipeline workers (0: the library default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the gate-only, area-query and serving legs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device 3 times: the known-size reference for the FETCH_SIZE / "
+                         "WRITE_SIZE counters of the HBM traffic profiles")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        n = 256 << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copied {n} bytes x 3")
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (seeds shard by index range with no communication; "
               "no multi-GPU run is part of it)", file=sys.stderr)

@@ -113,7 +113,7 @@ struct ClimState {
     dsx::FloodRes *h_res = nullptr;
     uint32_t *h_ctl = nullptr;
     dsx::TileMask *h_masks = nullptr;
-    uint32_t h_res_cap = 0, h_mask_cap = 0;
+    uint32_t h_res_cap = 0, h_mask_cap = 0, mask_want = 0;
 };
 
 static ClimState *clim_state(dsx_ctx *c);
@@ -164,23 +164,28 @@ static int clim_alloc(dsx_ctx *c, ClimState *&S) {
     return DSX_OK;
 }
 
-// Pinned staging for a batch of n floods (grown between calls, when the stream is idle).
+// Pinned staging for a batch of n floods (grown between calls, when the stream is idle). The first
+// sizes hold a pipeline worker's batches (65,536 floods and masks, ~14 MB pinned) and buffers at least
+// double when they grow: hipFree / hipHostFree wait for the whole device, so every reallocation stalls
+// the calling worker behind all other streams' work.
 static int flood_io_reserve(ClimState *S, uint64_t n) {
     const size_t up = size_t(n) * (sizeof(dsx::FloodJob) + 8);
     if (S->h_up_cap < up) {
         if (S->h_up) HIPCK(hipHostFree(S->h_up));
         S->h_up = nullptr;
-        S->h_up_cap = std::max<size_t>(up, size_t(4096) * (sizeof(dsx::FloodJob) + 8));
+        S->h_up_cap = std::max({up, 2 * S->h_up_cap, size_t(1 << 16) * (sizeof(dsx::FloodJob) + 8)});
         HIPCK(hipHostMalloc(&S->h_up, S->h_up_cap, hipHostMallocDefault));
     }
     if (S->h_res_cap < n) {
         if (S->h_res) HIPCK(hipHostFree(S->h_res));
         S->h_res = nullptr;
-        S->h_res_cap = uint32_t(std::max<uint64_t>(n, 4096));
+        S->h_res_cap = uint32_t(std::max<uint64_t>({n, 2 * uint64_t(S->h_res_cap), 1u << 16}));
         HIPCK(hipHostMalloc(&S->h_res, size_t(S->h_res_cap) * sizeof(dsx::FloodRes), hipHostMallocDefault));
     }
-    if (!S->h_masks) {
-        S->h_mask_cap = 1u << 15;
+    if (!S->h_masks || S->h_mask_cap < S->mask_want) {   // mask_want: the last overflow of the pinned masks
+        if (S->h_masks) HIPCK(hipHostFree(S->h_masks));
+        S->h_masks = nullptr;
+        S->h_mask_cap = std::max(1u << 16, S->mask_want);
         HIPCK(hipHostMalloc(&S->h_masks, size_t(S->h_mask_cap) * sizeof(dsx::TileMask), hipHostMallocDefault));
     }
     return DSX_OK;
@@ -193,7 +198,7 @@ static int io_reserve(dsx_ctx *c, ClimState *S, size_t bytes) {
     if (S->h_io) HIPCK(hipHostFree(S->h_io));
     S->d_io = nullptr;
     S->h_io = nullptr;
-    S->io_cap = std::max<size_t>(bytes, 1 << 20);
+    S->io_cap = std::max({bytes, 2 * S->io_cap, size_t(8) << 20});   // at least doubles (see flood_io_reserve)
     HIPCK(hipMalloc(&S->d_io, S->io_cap));
     HIPCK(hipHostMalloc(&S->h_io, S->io_cap, hipHostMallocDefault));
     return DSX_OK;
@@ -534,7 +539,8 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     // those beyond its capacity are copied here
     const uint32_t nh = std::min(nm, S->h_mask_cap);
     S->masks.resize(nm - nh);
-    if (nm > nh) {
+    if (nm > nh) {   // the next call's staging holds this many pinned (twice the current capacity at least)
+        S->mask_want = std::min(S->mask_cap, std::max(nm, 2 * S->h_mask_cap));
         HIPCK(hipMemcpyAsync(S->masks.data(), S->d_masks + nh, size_t(nm - nh) * sizeof(dsx::TileMask),
                              hipMemcpyDeviceToHost, c->st));
         HIPCK(hipStreamSynchronize(c->st));

@@ -92,7 +92,7 @@ int cand_alloc(dsx_ctx *c, uint64_t per_seed) {
     if (c->d_cand && c->cand_cap >= per_seed) return DSX_OK;
     if (c->d_cand) HIPCK(hipFree(c->d_cand));
     c->d_cand = nullptr;
-    c->cand_cap = std::max<uint64_t>(per_seed, 1u << 12);
+    c->cand_cap = std::max<uint64_t>({per_seed, 2 * c->cand_cap, 1u << 12});   // doubles: hipFree waits for the device
     HIPCK(hipMalloc(&c->d_cand, size_t(SCAN_CHUNK) * c->cand_cap * sizeof(dsx::ScanCand)));
     if (c->h_cand) HIPCK(hipHostFree(c->h_cand));
     c->h_cand = nullptr;
