@@ -125,6 +125,22 @@ def measured_traffic():
         return None
 
 
+def measured_counters():
+    """VALU / FP64 / LDS utilisation of the flood engine and the pre-gate kernel from the committed
+    rocprofv3 --pmc summary (kernels run alone under the profiler), or None."""
+    path = os.path.join(REPO, "profiles", "kernel_counters.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    keep = {"valu": "valu_busy_pct", "p64": "fp64_frac_of_peak", "p32": "fp32_frac_of_peak", "plds": "lds_frac_of_peak",
+            "ldsutil": "lds_util_pct", "bc": "lds_bank_conflicts_per_access", "us": "avg_us_alone"}
+    out = {name: {v: d[name][k] for k, v in keep.items() if k in d[name]} for name in ("flood_run", "amp_pregate")
+           if name in d}
+    out["source"] = d.get("source")
+    return out
+
+
 def percentiles(v):
     import numpy as np
     if len(v) == 0:
@@ -273,6 +289,7 @@ def main():
         "roofline": {"bound": "fp64-valu", "kernel": "k_flood_run (persistent tiled flood engine)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
+                     "counters": measured_counters(),
                      "avg_launch_ms": avg_ms, "launches": int(st.eval_launches),
                      "flops_per_launch": flops / launches,
                      "octave_samples": int(samples), "sampled_cells": int(sampled_big),

@@ -491,10 +491,17 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
                 e0 = S->events[0];
                 e1 = S->events[1];
             }
-            // workgroups: enough for the tiles in flight (every flood's frontier)
+            // workgroups (in units of 256 threads): enough for the tiles in flight. Hot-zone and proxy
+            // floods are short chains of cheap tiles; exact floods (52 octave samples per hot cell) keep
+            // their workgroups evaluating most of the time, so a batch of them gets a wider launch
             static const uint32_t per_job = uint32_t(env_u32("DSX_FLOOD_BLOCKS_PER_JOB", 4)),
-                                  max_blocks = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS", 128));
-            const uint32_t blocks = uint32_t(std::clamp<size_t>(size_t(per_job) * n_big, 16, max_blocks));
+                                  max_blocks = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS", 128)),
+                                  min_full = uint32_t(env_u32("DSX_FLOOD_MIN_BLOCKS_FULL", 16)),
+                                  per_full = uint32_t(env_u32("DSX_FLOOD_BLOCKS_PER_EXACT", 4)),
+                                  max_full = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS_FULL", 128));
+            const uint32_t blocks =
+                light ? uint32_t(std::clamp<size_t>(size_t(per_job) * n_big, 16, max_blocks))
+                      : uint32_t(std::clamp<size_t>(size_t(per_full) * n_big, std::min(min_full, max_full), max_full));
             blocks_used = blocks;
             dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, S->prepass, light ? 1 : 0, e0, e1,
                                   prof ? 1 : 0);
