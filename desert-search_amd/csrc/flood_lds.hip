@@ -21,9 +21,11 @@
 
 namespace dsx {
 
-constexpr int LDS_N = 255;                                    // largest local sub-window edge
-constexpr int LDS_WORDS = (LDS_N * LDS_N + 31) / 32;           // 2033 words per bitmap
-constexpr int LDS_RING = 2048;                                 // pushed cells in flight
+// Local sub-window edge N (255, or 64 for threshold floods): bitmap words and pushed cells in flight.
+template <int N> struct LdsShape {
+    static constexpr int WORDS = (N * N + 31) / 32;   // 2033 words per bitmap at N = 255
+    static constexpr int RING = N >= 255 ? 2048 : 512;
+};
 
 template <class CS, int PRE>
 __device__ __forceinline__ int lds_pred(int kind, const CS &C, const SurfGrid &g, int x, int z, float cut, bool *hot) {
@@ -38,27 +40,28 @@ __device__ __forceinline__ int lds_pred(int kind, const CS &C, const SurfGrid &g
 
 // 32 bits of row lj of a local bitmap (row length L bits) starting at local column c0 (c0 may be
 // negative or run past L: those columns read as 0).
-__device__ __forceinline__ uint32_t bitmap_row32(const uint32_t *bm, int L, int lj, int c0) {
+__device__ __forceinline__ uint32_t bitmap_row32(const uint32_t *bm, int words, int L, int lj, int c0) {
     const int lo = c0 < 0 ? 0 : c0, hi = c0 + 32 > L ? L : c0 + 32;   // columns [lo, hi) exist
     if (hi <= lo) return 0u;
     const int b = lj * L + lo;
     const int w0 = b >> 5, sh = b & 31;
-    const uint64_t two = uint64_t(bm[w0]) | (uint64_t(w0 + 1 < LDS_WORDS ? bm[w0 + 1] : 0u) << 32);
+    const uint64_t two = uint64_t(bm[w0]) | (uint64_t(w0 + 1 < words ? bm[w0 + 1] : 0u) << 32);
     uint32_t v = uint32_t(two >> sh);
     const int n = hi - lo;
     if (n < 32) v &= (1u << n) - 1u;
     return v << (lo - c0);
 }
 
-template <class CS, int PRE>
+template <class CS, int PRE, int N>
 __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
                                                   SurfGrid g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B) {
     uint32_t *mcount = B.ctl + FLOOD_MASKS;
     __shared__ CS C;
+    constexpr int LDS_N = N, LDS_WORDS = LdsShape<N>::WORDS, LDS_RING = LdsShape<N>::RING;
     __shared__ uint32_t touched[LDS_WORDS], counted[LDS_WORDS];
     __shared__ uint32_t ring[LDS_RING];
     __shared__ uint32_t evl[512];
-    __shared__ uint32_t qhead, qtail, nev, ovf;
+    __shared__ uint32_t qhead, qtail, nev, ovf, ncnt;
     const uint32_t b = blockIdx.x;
     if (b >= n) return;
     const uint32_t job = list[b];
@@ -80,11 +83,14 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
             qtail = 0;
             nev = 0;
             ovf = 0;
+            ncnt = 0;
         }
     }
     __syncthreads();
     unsigned long long cells = 0, sampled = 0, hotc = 0;
     int minx = J.qx, maxx = J.qx, minz = J.qz, maxz = J.qz;
+    const uint32_t stop_at = uint32_t(J.conn8) >> 8;
+    bool capped = false;
     bool edge = false, border = false;
     // the centre is touched first (dense_flood.hpp:80)
     if (lane == 0) {
@@ -111,6 +117,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
             if (v == 0) continue;
             if (v == 2) {
                 atomicOr(&counted[c >> 5], 1u << (c & 31));
+                atomicAdd(&ncnt, 1u);
                 cells++;
                 minx = min(minx, x);
                 maxx = max(maxx, x);
@@ -124,13 +131,18 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         __syncthreads();
         if (lane == 0) nev = 0;
         const uint32_t h = qhead, tl = qtail;
+        // a threshold flood (job stop_cells) is done once it has counted that many cells
+        const bool stop = stop_at != 0u && ncnt >= stop_at;
         __syncthreads();
-        if (ovf || h == tl) break;
+        if (ovf || h == tl || stop) {
+            capped = stop && !ovf && h != tl;
+            break;
+        }
         // ---- expand up to 64 pushed cells: claim their untouched neighbours
         const uint32_t np = min(tl - h, 64u);
         if (uint32_t(lane) < np) {
             const int c = int(ring[(h + lane) % LDS_RING]), li = c % L, lj = c / L;
-            const int nn = J.conn8 ? 8 : 4;
+            const int nn = (J.conn8 & 1) ? 8 : 4;
             for (int d = 0; d < nn; d++) {
                 const int di = d == 0 ? -1 : d == 1 ? 1 : d == 2 ? 0 : d == 3 ? 0 : d == 4 ? -1 : d == 5 ? 1 : d == 6 ? -1 : 1;
                 const int dj = d == 0 ? 0 : d == 1 ? 0 : d == 2 ? -1 : d == 3 ? 1 : d == 4 ? -1 : d == 5 ? -1 : d == 6 ? 1 : 1;
@@ -192,7 +204,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
         r.evald = sampled;
         res_store(&res[job], r);
     }
-    if (cells == 0) return;
+    if (cells == 0 || capped) return;   // a stopped flood's cells are not exported
     // ---- counted cells as 32x32 tile masks (window tile coordinates) over the counted bounding box
     const int i0 = (minx - J.qx) / J.step + J.half, i1 = (maxx - J.qx) / J.step + J.half;
     const int j0 = (minz - J.qz) / J.step + J.half, j1 = (maxz - J.qz) / J.step + J.half;
@@ -202,7 +214,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
             if (lane < 32) {
                 const int j = tz * 32 + lane;                 // window row
                 if (j < nw && j - o >= 0 && j - o < L) {
-                    row = bitmap_row32(counted, L, j - o, tx * 32 - o);
+                    row = bitmap_row32(counted, LDS_WORDS, L, j - o, tx * 32 - o);
                     const int k = nw - tx * 32;               // valid columns of this row
                     if (k < 32) row &= (1u << k) - 1u;
                 }
@@ -220,12 +232,16 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
 template <int PRE>
 static void lds_launch(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
                        const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B) {
-    if (cls == LDS_HOT)
-        hipLaunchKernelGGL((k_flood_lds<ClimT, PRE>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
+    if (cls == LDS_HOT_SMALL)
+        hipLaunchKernelGGL((k_flood_lds<ClimT, PRE, 64>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
+    else if (cls == LDS_HOT)
+        hipLaunchKernelGGL((k_flood_lds<ClimT, PRE, 255>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
     else if (cls == LDS_PROXY)
-        hipLaunchKernelGGL((k_flood_lds<ClimP, PRE>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
+        hipLaunchKernelGGL((k_flood_lds<ClimP, PRE, 255>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap,
+                           B);
     else
-        hipLaunchKernelGGL((k_flood_lds<ClimSeed, PRE>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
+        hipLaunchKernelGGL((k_flood_lds<ClimSeed, PRE, 255>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks,
+                           mcap, B);
 }
 
 void launch_flood_lds(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,

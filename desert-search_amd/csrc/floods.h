@@ -12,7 +12,8 @@ struct FloodJob {
     int32_t seed;   // index into the batch's ClimSeed array
     int32_t kind;
     float cut;
-    int32_t qx, qz, step, half, conn8;
+    int32_t qx, qz, step, half;
+    int32_t conn8;  // bit 0: 8-connectivity; bits 8-31: stop after this many counted cells (0: never)
 };
 struct FloodRes {
     unsigned long long cells, sampled;
@@ -101,7 +102,7 @@ void launch_flood_publish(hipStream_t st, const FloodRes *res, uint32_t n, Flood
 // start tile queued). cls: LDS_HOT (the T test, any window), LDS_PROXY (proxy predicate, any window),
 // LDS_FULL (any kind; the host sends only windows that fit whole).
 constexpr int LDS_MAX_N = 255;
-enum { LDS_HOT = 0, LDS_PROXY = 1, LDS_FULL = 2 };
+enum { LDS_HOT = 0, LDS_PROXY = 1, LDS_FULL = 2, LDS_HOT_SMALL = 3 };   // LDS_HOT_SMALL: 64 x 64 sub-window
 void launch_flood_lds(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
                       const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B, int pre);
 

@@ -113,7 +113,7 @@ __device__ uint32_t prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J
     Tile &T = B.tiles[ti];
     const int lane = threadIdx.x & 63;
     const int r = lane & 31;
-    const bool c8 = J.conn8 != 0;
+    const bool c8 = (J.conn8 & 1) != 0;
     const uint32_t job = ald(&T.job);
     const int tx = ald(&T.tx), tz = ald(&T.tz);
     const uint32_t in = lane < FT ? ald(&T.in[r]) : 0u;
@@ -417,7 +417,7 @@ __device__ void acc_tile(const FloodJob *jobs, FloodBufs B, uint32_t b, FloodRes
     const int tx = ald(&T.tx), tz = ald(&T.tz);
     const FloodJob J = jobs[job];
     const int lane = threadIdx.x, r = lane & 31;
-    const bool c8 = J.conn8 != 0;
+    const bool c8 = (J.conn8 & 1) != 0;
     const uint32_t R = lane < FT ? ald(&T.reach[r]) : 0u, I = lane < FT ? ald(&T.in[r]) : 0u;
     const uint32_t up = __shfl(R, (r + 31) & 31, 32), dn = __shfl(R, (r + 1) & 31, 32);
     const uint32_t upv = r > 0 ? up : 0u, dnv = r < FT - 1 ? dn : 0u;

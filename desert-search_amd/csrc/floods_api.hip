@@ -396,7 +396,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     for (uint64_t k = 0; k < n; k++) {
         const dsx_flood_job &j = jobs[k];
         if (j.kind < 0 || j.kind > 3 || j.step < 1 || j.half < 0 || 2 * int64_t(j.half) + 1 > 65535 ||
-            int64_t(j.half) * j.step > 1000000)
+            int64_t(j.half) * j.step > 1000000 || j.stop_cells < 0 || j.stop_cells > (1 << 23))
             return DSX_E_ARG;
     }
     if (n == 0) {
@@ -423,7 +423,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         auto [it, fresh] = slot.try_emplace(jobs[k].seed, int32_t(useeds.size()));
         if (fresh) useeds.push_back(jobs[k].seed);
         hj[k] = dsx::FloodJob{it->second, jobs[k].kind, jobs[k].cut, jobs[k].qx, jobs[k].qz, jobs[k].step, jobs[k].half,
-                              jobs[k].conn8};
+                              (jobs[k].conn8 != 0 ? 1 : 0) | (jobs[k].stop_cells << 8)};
     }
     std::vector<uint32_t> uslot(useeds.size());
     if ((rc = clim_slots(c, S, useeds.data(), uint32_t(useeds.size()), uslot.data()))) return rc;
@@ -433,20 +433,27 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     // BFS layer per step, which measured slower in the pipeline than the tiled engine); the rest, and
     // the floods that outgrow the sub-window (handed over on the device), on the tiled engine.
     static const bool lds_route = env_flag("DSX_LDS_ROUTE", false);
-    uint32_t n_cls[3] = {0, 0, 0}, n_small = 0, n_big = 0;
+    uint32_t n_cls[4] = {0, 0, 0, 0}, n_small = 0, n_big = 0;
+    static const bool hot_small = env_flag("DSX_LDS_HOT_SMALL", true);
     auto cls_of = [&](const dsx::FloodJob &j) {
         const bool fits = 2 * j.half + 1 <= dsx::LDS_MAX_N;
+        // threshold floods (a stop count: the hot-zone pre-floods) almost always end in a few dozen
+        // cells: a 64 x 64 sub-window (7 KB of LDS instead of 28) holds them; the rest are handed over
+        if (j.kind == dsx::PRED_HOT && (uint32_t(j.conn8) >> 8) != 0u && hot_small && (fits || lds_route))
+            return int(dsx::LDS_HOT_SMALL);
         if (j.kind == dsx::PRED_HOT && (fits || lds_route)) return int(dsx::LDS_HOT);
         if (j.kind == dsx::PRED_PROXY && (fits || lds_route)) return int(dsx::LDS_PROXY);
         return fits ? int(dsx::LDS_FULL) : -1;
     };
-    for (int cls = 0; cls < 3; cls++)
+    for (int cls = 0; cls < 4; cls++)
         for (uint64_t k = 0; k < n; k++)
             if (cls_of(hj[k]) == cls) {
                 h_small[n_small++] = uint32_t(k);
                 n_cls[cls]++;
             }
-    bool light = true;   // every tiled job a hot-zone or proxy flood
+    // every tiled job a hot-zone or proxy flood, including those the small-window engine may hand over
+    // (any of its floods whose ring overflows): so no small-window biome-predicate floods either
+    bool light = n_cls[dsx::LDS_FULL] == 0;
     for (uint64_t k = 0; k < n; k++)
         if (cls_of(hj[k]) < 0) {
             h_big[n_big++] = uint32_t(k);
@@ -474,7 +481,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         // jobs and lists read from the pinned staging by the first kernel (no separate copy)
         dsx::launch_flood_stage(c->st, S->h_up, uint32_t(n), n_small, n_big, d_up, S->d_res, B);
         uint32_t off = 0;
-        for (int cls = 0; cls < 3; cls++) {
+        for (int cls = 0; cls < 4; cls++) {
             dsx::launch_flood_lds(c->st, cls, d_jobs, d_small + off, n_cls[cls], S->d_clim, S->grid, S->d_res,
                                   S->d_masks, S->mask_cap, B, S->prepass);
             off += n_cls[cls];

@@ -47,7 +47,7 @@ RAW_CAND = np.dtype([("s", "<f4"), ("t", "<f4"), ("c", "<f4"), ("e", "<f4"), ("x
 
 
 FLOOD_JOB = np.dtype([("seed", "<u8"), ("kind", "<i4"), ("cut", "<f4"), ("qx", "<i4"), ("qz", "<i4"), ("step", "<i4"),
-                      ("half", "<i4"), ("conn8", "<i4"), ("pad", "<i4")])
+                      ("half", "<i4"), ("conn8", "<i4"), ("stop_cells", "<i4")])
 AREA = np.dtype([("cells", "<u8"), ("sampled", "<u8"), ("minx", "<i4"), ("maxx", "<i4"), ("minz", "<i4"),
                  ("maxz", "<i4"), ("edge", "<i4"), ("border", "<i4")])
 PRED_DESERT_STATE, PRED_HOT, PRED_PROXY, PRED_DESERT = 0, 1, 2, 3

@@ -560,12 +560,22 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
     using XY = std::pair<int, int>;
     std::map<XY, size_t> pre_idx, h128_idx;
     FloodReq pre{&S, {}, true, {}, {}};          // step-256 pre-floods of every distinct hot point
+    // A pre-flood only decides cells * 256^2 * 16 / 1e6 < hot_pre * hot_min (dsearch_desert.cpp:337-339)
+    // and its cells are used only when it does; it may stop at the smallest count that fails the test.
+    int pre_stop = 0;
+    if (P.hot_pre > 0) {
+        const double thr = P.hot_pre * P.hot_min;
+        int64_t c = std::max<int64_t>(0, int64_t(thr / (256.0 * 256 * 16 / 1e6)) - 2);
+        while (c * 256.0 * 256 * 16 / 1e6 < thr) c++;
+        pre_stop = c >= 1 && c <= (1 << 23) ? int(c) : 0;
+    }
     FloodReq h128{&S, {}, true, {}, {}};         // step-128 floods of every big one
     if (P.hot_min > 0) {
         for (size_t r = 0; r < ns; r++) {
             if (bt[r] < 0.55 || pre_idx.count({hx[r], hz[r]})) continue;
             pre_idx[{hx[r], hz[r]}] = pre.jobs.size();
             pre.jobs.push_back(fjob(seed, DSX_PRED_HOT, 0.f, hx[r], hz[r], 256, 32000 / 256, 0));
+            pre.jobs.back().stop_cells = pre_stop;
         }
         if (P.hot_pre > 0) co_await pre;
         for (auto &[xy, k] : pre_idx)

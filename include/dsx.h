@@ -165,6 +165,10 @@ typedef struct {
     int32_t qx, qz;   /* centre, quarts */
     int32_t step, half;
     int32_t conn8;    /* 1: also step to diagonal neighbours */
+    int32_t stop_cells; /* > 0: the flood may stop once it has counted this many cells; its cells are then
+                         * >= stop_cells and its other outputs and masks unspecified (a threshold test
+                         * such as the hot-zone pre-flood's, dsearch_desert.cpp:337-339, needs no more).
+                         * 0: complete flood. At most 2^23. */
 } dsx_flood_job;
 
 /* standalone::DenseArea (dense_flood.hpp:9-13). */

@@ -117,6 +117,36 @@ def test_flood_small_windows_mixed_batch(engine, oracle):
     _compare(engine, oracle, rows)
 
 
+def test_flood_stop_cells(engine, oracle):
+    """stop_cells (the hot-zone pre-flood's threshold test): a flood that counts fewer cells than the stop
+    is complete and equals the oracle; one that reaches it reports at least that many cells, and so does
+    the oracle's complete flood. 138 is the pipeline's stop (0.8 x 180 M at step 256)."""
+    dsx = load_package().dsx
+    rows = []
+    for r in FINDS[:10]:
+        seed, qx, qz = u64(r["seed"]), r["x"] >> 2, r["z"] >> 2
+        rows += [(seed, 1, 0.35, qx, qz, 256, 125, 0), (seed, 1, 0.35, qx + 3000, qz - 7000, 256, 125, 0),
+                 (seed, 1, 0.35, qx - 20000, qz + 9000, 256, 125, 0), (seed, 1, 0.35, qx, qz, 64, 100, 0)]
+    with cf.ThreadPoolExecutor(8) as ex:
+        want = list(ex.map(lambda r: oracle.flood(*r), rows))
+    for stop in (138, 1, 40):
+        jobs = dsx.flood_jobs(rows)
+        jobs["stop_cells"] = stop
+        got = engine.flood(jobs)
+        n_capped = 0
+        for k, (row, w) in enumerate(zip(rows, want)):
+            if w[0] < stop:
+                assert _area(got[k]) == w, (stop, k, row)
+            else:
+                assert int(got[k]["cells"]) >= stop, (stop, k, row)
+                n_capped += int(got[k]["cells"]) < w[0]
+        assert n_capped > 0, stop   # some floods really stopped early
+    bad = dsx.flood_jobs(rows[:1])
+    bad["stop_cells"] = -1
+    with pytest.raises(dsx.DsxError):
+        engine.flood(bad)
+
+
 def test_hottest_point_search(engine, oracle):
     """The device search for the hottest of 7 x 7 points around each site (dsearch_desert.cpp:328-332:
     dz then dx from -3 to 3, step 256, strict greater, so the first maximum wins) equals that scan over
