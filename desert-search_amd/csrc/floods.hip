@@ -35,6 +35,7 @@
 #include "floods.h"
 #include "flood_dev.h"
 #include <hip/hip_ext.h>
+#include <type_traits>
 #include <algorithm>
 #include <climits>
 #include <cstdlib>
@@ -72,8 +73,8 @@ constexpr int SB = 8, NSB = FT / SB;
 // Evaluate sub-block sb of the tile at (tx, tz) with one wave (one cell per lane) and OR its bits into
 // the LDS row masks (pass = v != 0, cnt = v == 2, hot = the predicate's temperature test passed; cells
 // outside the window or the world are none of them).
-template <int PRE, int LIGHT>
-__device__ __forceinline__ void eval_subblock(int sb, int tx, int tz, const FloodJob &J, const ClimSeed &C,
+template <int PRE, int LIGHT, class CS>
+__device__ __forceinline__ void eval_subblock(int sb, int tx, int tz, const FloodJob &J, const CS &C,
                                               const SurfGrid &g, uint32_t *s_pass, uint32_t *s_cnt, uint32_t *s_hot) {
     const int lane = threadIdx.x & 63;
     const int bx = sb % NSB, bz = sb / NSB;
@@ -231,7 +232,8 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
         aset(&B.ctl[RUN_TAIL0], atomicAdd(&B.ctl[Q_TAIL], 0u));
         aset(&B.ctl[RUN_STAGE_SEQ], atomicAdd(&B.ctl[STAGE_SEQ], 0u));
     }
-    __shared__ ClimSeed C;
+    // the octaves the predicates read: the proxy's 14 (ClimP, 5 KB of LDS) in the light instance
+    __shared__ std::conditional_t<LIGHT != 0, ClimP, ClimSeed> C;
     __shared__ uint32_t s_pass[FT], s_cnt[FT], s_hot[FT];
     __shared__ uint32_t s_item, s_again, s_need, s_evald;
     __shared__ int s_stop;
@@ -354,14 +356,14 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                     if (cls < 2) {
                         const int nslot = cls == 0 ? 4 : 14;
                         for (int k = tid; k < nslot * OW; k += NT) {
-                            const int slot = PROXY_SLOTS[k / OW];
-                            reinterpret_cast<uint4 *>(&C.oct[slot])[k % OW] =
+                            const int slot = PROXY_SLOTS[k / OW];   // ClimP holds them in this order
+                            reinterpret_cast<uint4 *>(&C.oct[LIGHT ? k / OW : slot])[k % OW] =
                                 reinterpret_cast<const uint4 *>(&G.oct[slot])[k % OW];
                         }
                         constexpr int GW = sizeof(GradTab) / 16;
                         if (tid < GW) reinterpret_cast<uint4 *>(&C.g)[tid] = reinterpret_cast<const uint4 *>(&G.g)[tid];
                         if (tid < 6) C.factor[tid] = G.factor[tid];
-                    } else {
+                    } else if constexpr (!LIGHT) {   // light batches hold no biome-predicate jobs
                         const uint4 *src = reinterpret_cast<const uint4 *>(&G);
                         uint4 *dst = reinterpret_cast<uint4 *>(&C);
                         for (int k = tid; k < int(sizeof(ClimSeed) / 16); k += NT) dst[k] = src[k];
