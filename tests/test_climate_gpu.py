@@ -129,6 +129,14 @@ def test_flood_stop_cells(engine, oracle):
                  (seed, 1, 0.35, qx - 20000, qz + 9000, 256, 125, 0), (seed, 1, 0.35, qx, qz, 64, 100, 0)]
     with cf.ThreadPoolExecutor(8) as ex:
         want = list(ex.map(lambda r: oracle.flood(*r), rows))
+    # a stop never reached: every flood completes; the large hot zones leave the 64 x 64 sub-window of
+    # the threshold-flood instance and are finished by the tiled engine, exactly
+    jobs = dsx.flood_jobs(rows)
+    jobs["stop_cells"] = 1 << 23
+    got = engine.flood(jobs)
+    for k, (row, w) in enumerate(zip(rows, want)):
+        assert _area(got[k]) == w, ("no stop", k, row)
+    assert max(w[0] for w in want) > 64 * 64   # some of them cannot fit the sub-window
     for stop in (138, 1, 40):
         jobs = dsx.flood_jobs(rows)
         jobs["stop_cells"] = stop
