@@ -226,6 +226,10 @@ template <int NT, int OCC, int PRE, int LIGHT>
 __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B,
                                                        int backoff, int spec, int tprof) {
     constexpr int NWAVE = NT / 64;
+    // spec bit 1: raise this kernel's waves' issue priority (s_setprio) above the default of other
+    // kernels sharing the SIMDs, e.g. the VALU-bound gate; bit 0: speculative sub-blocks
+    if (spec & 2) __builtin_amdgcn_s_setprio(2);
+    spec &= 1;
     const uint64_t t_born = tprof ? __builtin_amdgcn_s_memrealtime() : 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {   // what the first workgroup finds (time-limit diagnostics)
         aset(&B.ctl[RUN_SEQ], B.seq + 1);
@@ -570,7 +574,8 @@ void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, 
     }();
     static const int spec = [] {
         const char *v = std::getenv("DSX_FLOOD_SPEC");   // 0 turns the speculative sub-blocks off
-        return v && *v ? std::atoi(v) : 1;
+        const char *p = std::getenv("DSX_FLOOD_PRIO");   // 1: flood waves at raised issue priority
+        return (v && *v ? (std::atoi(v) != 0 ? 1 : 0) : 1) | (p && std::atoi(p) ? 2 : 0);
     }();
     static const bool light_on = [] {
         const char *v = std::getenv("DSX_FLOOD_LIGHT");   // 0: always the full instance
