@@ -232,7 +232,9 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
 template <int PRE>
 static void lds_launch(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
                        const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B) {
-    if (cls == LDS_HOT_SMALL)
+    if (cls == LDS_PROXY_SMALL)
+        hipLaunchKernelGGL((k_flood_lds<ClimP, PRE, 64>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
+    else if (cls == LDS_HOT_SMALL)
         hipLaunchKernelGGL((k_flood_lds<ClimT, PRE, 64>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);
     else if (cls == LDS_HOT)
         hipLaunchKernelGGL((k_flood_lds<ClimT, PRE, 255>), dim3(n), dim3(64), 0, st, jobs, list, n, cs, g, res, masks, mcap, B);

@@ -102,7 +102,8 @@ void launch_flood_publish(hipStream_t st, const FloodRes *res, uint32_t n, Flood
 // start tile queued). cls: LDS_HOT (the T test, any window), LDS_PROXY (proxy predicate, any window),
 // LDS_FULL (any kind; the host sends only windows that fit whole).
 constexpr int LDS_MAX_N = 255;
-enum { LDS_HOT = 0, LDS_PROXY = 1, LDS_FULL = 2, LDS_HOT_SMALL = 3 };   // LDS_HOT_SMALL: 64 x 64 sub-window
+// LDS_HOT_SMALL / LDS_PROXY_SMALL: 64 x 64 sub-window (floods that leave it are handed to the tiled engine)
+enum { LDS_HOT = 0, LDS_PROXY = 1, LDS_FULL = 2, LDS_HOT_SMALL = 3, LDS_PROXY_SMALL = 4, LDS_NCLS = 5 };
 void launch_flood_lds(hipStream_t st, int cls, const FloodJob *jobs, const uint32_t *list, uint32_t n, const ClimSeed *cs,
                       const SurfGrid &g, FloodRes *res, TileMask *masks, uint32_t mcap, FloodBufs B, int pre);
 

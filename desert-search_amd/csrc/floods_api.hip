@@ -433,8 +433,10 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     // BFS layer per step, which measured slower in the pipeline than the tiled engine); the rest, and
     // the floods that outgrow the sub-window (handed over on the device), on the tiled engine.
     static const bool lds_route = env_flag("DSX_LDS_ROUTE", false);
-    uint32_t n_cls[4] = {0, 0, 0, 0}, n_small = 0, n_big = 0;
+    uint32_t n_cls[dsx::LDS_NCLS] = {}, n_small = 0, n_big = 0;
     static const bool hot_small = env_flag("DSX_LDS_HOT_SMALL", true);
+    // proxy floods of any window on a 64 x 64 sub-window first (most touch 1-4 tiles)
+    static const bool proxy_small = env_flag("DSX_LDS_PROXY_SMALL", false);
     auto cls_of = [&](const dsx::FloodJob &j) {
         const bool fits = 2 * j.half + 1 <= dsx::LDS_MAX_N;
         // threshold floods (a stop count: the hot-zone pre-floods) almost always end in a few dozen
@@ -442,10 +444,11 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         if (j.kind == dsx::PRED_HOT && (uint32_t(j.conn8) >> 8) != 0u && hot_small && (fits || lds_route))
             return int(dsx::LDS_HOT_SMALL);
         if (j.kind == dsx::PRED_HOT && (fits || lds_route)) return int(dsx::LDS_HOT);
+        if (j.kind == dsx::PRED_PROXY && proxy_small) return int(dsx::LDS_PROXY_SMALL);
         if (j.kind == dsx::PRED_PROXY && (fits || lds_route)) return int(dsx::LDS_PROXY);
         return fits ? int(dsx::LDS_FULL) : -1;
     };
-    for (int cls = 0; cls < 4; cls++)
+    for (int cls = 0; cls < dsx::LDS_NCLS; cls++)
         for (uint64_t k = 0; k < n; k++)
             if (cls_of(hj[k]) == cls) {
                 h_small[n_small++] = uint32_t(k);
@@ -481,7 +484,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         // jobs and lists read from the pinned staging by the first kernel (no separate copy)
         dsx::launch_flood_stage(c->st, S->h_up, uint32_t(n), n_small, n_big, d_up, S->d_res, B);
         uint32_t off = 0;
-        for (int cls = 0; cls < 4; cls++) {
+        for (int cls = 0; cls < dsx::LDS_NCLS; cls++) {
             dsx::launch_flood_lds(c->st, cls, d_jobs, d_small + off, n_cls[cls], S->d_clim, S->grid, S->d_res,
                                   S->d_masks, S->mask_cap, B, S->prepass);
             off += n_cls[cls];
