@@ -291,15 +291,19 @@ void or_flood(uint64_t seed, int kind, float pcut, int qx, int qz, int step, int
 
 // Whole pipeline over seed indices [start, start+count) (gated=0, seeds = mix(start+k)) or over an
 // explicit seed list (gated=1). Writes PREFIX.csv, PREFIX.seeds.csv and PREFIX.json in the reference's
-// formats (rows in completion order). counters = {done, passed, scanned, sites}; seconds = wall time.
+// formats (rows in completion order) and the CANDIDATE lines to PREFIX.cand. counters = {done, passed,
+// scanned, sites}; seconds = wall time. params (optional): {max_phase, amp, amp_margin} and, with
+// nparams >= 6, {exact_min, refine_mb, verify_mb} (the `exact_min`, `refine`, `verify` keys).
 int or_run(uint64_t start, uint64_t count, const uint64_t *seeds, int64_t nseeds, int gated, int threads,
            const char *prefix, uint64_t *counters, double *seconds, const double *params, int nparams) {
     Params P;
     if (params && nparams >= 3) { P.max_phase = params[0]; P.amp = params[1]; P.amp_margin = params[2]; }
+    if (params && nparams >= 6) { P.exact_min = params[3]; P.refine_mb = params[4]; P.verify_mb = params[5]; }
     std::string pre(prefix);
     FILE *out = std::fopen((pre + ".csv").c_str(), "w");
     FILE *sout = std::fopen((pre + ".seeds.csv").c_str(), "w");
-    if (!out || !sout) return 2;
+    FILE *cout = std::fopen((pre + ".cand").c_str(), "w");
+    if (!out || !sout || !cout) return 2;
     std::fprintf(out, "seed,fyA,fyB,maxta,maxtb,rank,x,z,score,t0,c0,e0,proxy,s8,s4,edge,hot\n");
     std::fprintf(sout, "seed,fyA,fyB,maxta,maxtb,candidates,sites,best\n");
     std::atomic<uint64_t> next(0), done(0), passed(0), scanned(0), nsites(0);
@@ -322,7 +326,7 @@ int or_run(uint64_t start, uint64_t count, const uint64_t *seeds, int64_t nseeds
             std::lock_guard<std::mutex> g(mu);
             std::fputs(rows.c_str(), out);
             std::fputs(srow.c_str(), sout);
-            for (auto &c : r.candidates) std::fprintf(stderr, "%s\n", c.c_str());
+            for (auto &c : r.candidates) std::fprintf(cout, "%s\n", c.c_str());
         }
     };
     std::vector<std::thread> pool;
@@ -331,6 +335,7 @@ int or_run(uint64_t start, uint64_t count, const uint64_t *seeds, int64_t nseeds
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::fclose(out);
     std::fclose(sout);
+    std::fclose(cout);
     counters[0] = done; counters[1] = passed; counters[2] = scanned; counters[3] = nsites;
     *seconds = sec;
     return 0;

@@ -11,6 +11,12 @@ inputs and the reference's outputs.
                   rules over k in [1e12, 1e12 + 2^20); the list itself is an input fixture)
   st1m.*          the same with study=1 (no exact floods; st1m.zones.csv.gz holds the sorted per-zone
                   feature rows of PREFIX.zones.csv; `python make_goldens.py study` rewrites only these)
+  rgx.*           the 92 gated seeds with exact_min=0 refine=0 verify=0: every proxy start of the two best
+                  zones gets the step-8 and the step-4 flood and a CANDIDATE line (dsearch_desert.cpp:498-531)
+  fx.*            production Params on the seeds of fx_seeds.txt (the seeds of desert_finds.jsonl, then two
+                  seeds whose CANDIDATE lines a GPU run printed): step-4 floods and CANDIDATE lines at the
+                  production thresholds. `python make_goldens.py exact` rewrites only rgx.* and fx.*
+  TAG.cand        the sorted CANDIDATE lines the run printed on stderr (one per line)
   dflood.jsonl    dflood SEED X Z STEP at the verified desert finds, steps 8 and 4 (all rows) and
                   step 1 (the first rows)
   biomemap.npz    biomemap SEED X Z HALF STEP at verified desert finds (n x n uint8 maps; `python
@@ -37,8 +43,9 @@ REF = os.path.join(REPO, "oracle", "_ref")
 def run_search(tag, args, stdin=None):
     with tempfile.TemporaryDirectory() as td:
         pre = os.path.join(td, tag)
-        subprocess.run([os.path.join(REF, "dsearch_desert")] + [args[0], args[1], pre] + args[2:], check=True,
-                       stdin=open(stdin) if stdin else None, stderr=subprocess.DEVNULL)
+        err = subprocess.run([os.path.join(REF, "dsearch_desert")] + [args[0], args[1], pre] + args[2:], check=True,
+                             stdin=open(stdin) if stdin else None, stderr=subprocess.PIPE, text=True).stderr
+        cand = sorted(l for l in err.splitlines() if l.startswith("CANDIDATE "))
         rows = sorted(open(pre + ".csv").read().splitlines()[1:])
         srows = sorted(open(pre + ".seeds.csv").read().splitlines()[1:])
         js = json.load(open(pre + ".json"))
@@ -48,12 +55,16 @@ def run_search(tag, args, stdin=None):
             f.write(("\n".join(zrows) + "\n").encode())
     keep = {k: js[k] for k in ("start", "done", "passed", "scanned", "sites")}
     keep["seeds_rows"] = len(srows)
+    keep["candidates"] = len(cand)
     keep["seeds_sha256"] = hashlib.sha256("\n".join(srows).encode()).hexdigest()
     with open(os.path.join(HERE, tag + ".csv"), "w") as f:
         f.write("\n".join(rows) + "\n")
     if len(srows) <= 1000:
         with open(os.path.join(HERE, tag + ".seeds.csv"), "w") as f:
             f.write("\n".join(srows) + "\n")
+    if cand:
+        with open(os.path.join(HERE, tag + ".cand"), "w") as f:
+            f.write("\n".join(cand) + "\n")
     with open(os.path.join(HERE, tag + ".json"), "w") as f:
         json.dump(keep, f, indent=1)
     print(tag, keep)
@@ -103,7 +114,33 @@ def study_goldens():
     run_search("st1m", ["0", "0", "8", "stdin=1", "study=1"], stdin=os.path.join(HERE, "gated_1e12_1m.txt"))
 
 
+def fx_seed_list():
+    """The seeds of the verified finds (signed, as desert_finds.jsonl holds them), in file order, then the
+    two extra seeds; duplicates dropped."""
+    seeds = []
+    for line in open(os.path.join(HERE, "desert_finds.jsonl")):
+        r = json.loads(line)
+        if "seed" in r and "x" in r:
+            seeds.append(int(r["seed"]))
+    seeds += [1368900756516249850, 2136669149758385146]
+    out = []
+    for s in seeds:
+        if s not in out:
+            out.append(s)
+    return out
+
+
+def exact_goldens():
+    run_search("rgx", ["0", "0", "8", "stdin=1", "exact_min=0", "refine=0", "verify=0"],
+               stdin=os.path.join(HERE, "gated_1e12_1m.txt"))
+    with open(os.path.join(HERE, "fx_seeds.txt"), "w") as f:
+        f.write("".join("%d\n" % s for s in fx_seed_list()))
+    run_search("fx", ["0", "0", "8", "stdin=1"], stdin=os.path.join(HERE, "fx_seeds.txt"))
+
+
 def main():
+    if sys.argv[1:2] == ["exact"]:
+        return exact_goldens()
     if sys.argv[1:2] == ["biomemap"]:
         return biomemap_goldens()
     if sys.argv[1:2] == ["dfunc"]:
@@ -127,6 +164,7 @@ def main():
     biomemap_goldens()
     dfunc_goldens()
     study_goldens()
+    exact_goldens()
 
 
 if __name__ == "__main__":

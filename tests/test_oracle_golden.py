@@ -13,36 +13,45 @@ import pytest
 from conftest import GOLDEN
 
 
-def _run(oracle, tmp, tag, start=0, count=0, seeds=None):
+def _run(oracle, tmp, tag, start=0, count=0, seeds=None, exact=None):
+    """exact = (exact_min, refine, verify) overrides the exact-flood thresholds."""
     pre = os.path.join(tmp, tag)
     cnt = np.zeros(4, np.uint64)
     sec = ctypes.c_double()
+    par = np.array([0.10, 0.70, 0.01] + list(exact or (75, 70, 80)), np.float64)
     if seeds is not None:
         arr = np.ascontiguousarray(seeds, dtype=np.uint64)
         rc = oracle.or_run(0, 0, arr.ctypes.data, len(arr), 1, 8, pre.encode(), cnt.ctypes.data, ctypes.byref(sec),
-                           None, 0)
+                           par.ctypes.data, len(par))
     else:
-        rc = oracle.or_run(start, count, None, 0, 0, 8, pre.encode(), cnt.ctypes.data, ctypes.byref(sec), None, 0)
+        rc = oracle.or_run(start, count, None, 0, 0, 8, pre.encode(), cnt.ctypes.data, ctypes.byref(sec),
+                           par.ctypes.data, len(par))
     assert rc == 0
     rows = sorted(open(pre + ".csv").read().splitlines()[1:])
     srows = sorted(open(pre + ".seeds.csv").read().splitlines()[1:])
-    return rows, srows, [int(c) for c in cnt]
+    cand = sorted(open(pre + ".cand").read().splitlines())
+    return rows, srows, [int(c) for c in cnt], cand
 
 
-def _check(tag, rows, srows, cnt):
+def _check(tag, rows, srows, cnt, cand=None):
     gold = json.load(open(os.path.join(GOLDEN, tag + ".json")))
     grows = open(os.path.join(GOLDEN, tag + ".csv")).read().splitlines()
     assert rows == grows
     assert len(srows) == gold["seeds_rows"]
     assert hashlib.sha256("\n".join(srows).encode()).hexdigest() == gold["seeds_sha256"]
     assert cnt == [gold["done"], gold["passed"], gold["scanned"], gold["sites"]]
+    if cand is not None:
+        path = os.path.join(GOLDEN, tag + ".cand")
+        want = open(path).read().splitlines() if os.path.exists(path) else []
+        assert len(want) == gold.get("candidates", 0)
+        assert cand == want
 
 
 def test_config1_matches_reference(oracle):
     """BASELINE config 1 (seeds mix(k), k < 200000, production Params): CSV, seeds CSV and counters."""
     with tempfile.TemporaryDirectory() as td:
-        rows, srows, cnt = _run(oracle, td, "c1", 0, 200000)
-    _check("c1", rows, srows, cnt)
+        rows, srows, cnt, cand = _run(oracle, td, "c1", 0, 200000)
+    _check("c1", rows, srows, cnt, cand)
     assert srows == open(os.path.join(GOLDEN, "c1.seeds.csv")).read().splitlines()
 
 
@@ -50,15 +59,35 @@ def test_gated_list_matches_reference(oracle):
     """stdin=1 mode on seeds gated with the GPU-gate rules (92 seeds, k in [1e12, 1e12 + 2^20))."""
     seeds = [int(l.split()[0]) % (1 << 64) for l in open(os.path.join(GOLDEN, "gated_1e12_1m.txt"))]
     with tempfile.TemporaryDirectory() as td:
-        rows, srows, cnt = _run(oracle, td, "rg1m", seeds=seeds)
-    _check("rg1m", rows, srows, cnt)
+        rows, srows, cnt, cand = _run(oracle, td, "rg1m", seeds=seeds)
+    _check("rg1m", rows, srows, cnt, cand)
 
 
 def test_3m_range_matches_reference(oracle):
     """3M seed indices: 392 scanned seeds including exact step-8 floods."""
     with tempfile.TemporaryDirectory() as td:
-        rows, srows, cnt = _run(oracle, td, "r3m", 0, 3000000)
-    _check("r3m", rows, srows, cnt)
+        rows, srows, cnt, cand = _run(oracle, td, "r3m", 0, 3000000)
+    _check("r3m", rows, srows, cnt, cand)
+
+
+def test_exact_branch_matches_reference(oracle):
+    """The exact-flood driver (dsearch_desert.cpp:498-531) with exact_min=0 refine=0 verify=0 on the 92
+    gated seeds: every step-8 result gets the step-4 flood (s4, edge flags ORed) and a CANDIDATE line."""
+    seeds = [int(l.split()[0]) % (1 << 64) for l in open(os.path.join(GOLDEN, "gated_1e12_1m.txt"))]
+    with tempfile.TemporaryDirectory() as td:
+        rows, srows, cnt, cand = _run(oracle, td, "rgx", seeds=seeds, exact=(0, 0, 0))
+    assert sum(1 for r in rows if float(r.split(",")[14]) > 0) > 100
+    _check("rgx", rows, srows, cnt, cand)
+
+
+def test_production_candidates_match_reference(oracle):
+    """Production Params on the seeds of the verified finds (stdin=1): step-4 floods at the production
+    thresholds and the CANDIDATE list, byte for byte."""
+    seeds = [int(l) % (1 << 64) for l in open(os.path.join(GOLDEN, "fx_seeds.txt"))]
+    with tempfile.TemporaryDirectory() as td:
+        rows, srows, cnt, cand = _run(oracle, td, "fx", seeds=seeds)
+    assert len(cand) > 150
+    _check("fx", rows, srows, cnt, cand)
 
 
 def test_fixture_rows_consistent():

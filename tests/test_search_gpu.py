@@ -13,15 +13,28 @@ from conftest import GOLDEN, REPO, load_package
 pytestmark = pytest.mark.gpu
 
 
-def _check(tag, prefix, counts=None):
+def _check(tag, prefix, counts=None, stderr=None):
+    """stderr: the text the run printed; its CANDIDATE lines, sorted, must equal the reference's."""
     gold = json.load(open(os.path.join(GOLDEN, tag + ".json")))
     rows = sorted(open(prefix + ".csv").read().splitlines()[1:])
     srows = sorted(open(prefix + ".seeds.csv").read().splitlines()[1:])
-    assert rows == open(os.path.join(GOLDEN, tag + ".csv")).read().splitlines()
+    want = open(os.path.join(GOLDEN, tag + ".csv")).read().splitlines()
+    bad = [(a, b) for a, b in zip(rows, want) if a != b]
+    assert len(rows) == len(want) and not bad, f"{len(rows)} vs {len(want)} rows; first difference {bad[:1]}"
     assert len(srows) == gold["seeds_rows"]
     assert hashlib.sha256("\n".join(srows).encode()).hexdigest() == gold["seeds_sha256"]
     if counts is not None:
         assert list(counts) == [gold["done"], gold["passed"], gold["scanned"], gold["sites"]]
+    if stderr is not None:
+        cand = sorted(l for l in stderr.splitlines() if l.startswith("CANDIDATE "))
+        path = os.path.join(GOLDEN, tag + ".cand")
+        gcand = open(path).read().splitlines() if os.path.exists(path) else []
+        assert len(gcand) == gold.get("candidates", 0)
+        assert cand == gcand, f"{len(cand)} vs {len(gcand)} CANDIDATE lines"
+
+
+def _seeds(name):
+    return [int(l.split()[0]) % (1 << 64) for l in open(os.path.join(GOLDEN, name))]
 
 
 def test_config1_pipeline(engine):
@@ -82,6 +95,47 @@ def test_cli_study(engine):
         r2 = subprocess.run([exe, "0", "0", pre, "8", "stdin=1", "study=1"], input="", capture_output=True, text=True,
                             timeout=60)
         assert r2.returncode == 2 and "exist?" in r2.stderr
+
+
+@pytest.mark.parametrize("prepass", [0, 1])
+def test_exact_branch_pipeline(engine, capfd, prepass):
+    """The exact-flood driver (dsearch_desert.cpp:498-531) with exact_min=0 refine=0 verify=0 on the 92
+    gated seeds: every step-8 result is refined by the 4-connected step-4 flood (s4 and the ORed edge flag
+    in the rows) and printed as a CANDIDATE; rows and CANDIDATE lines equal the reference's."""
+    dsx = load_package().dsx
+    p = engine.search_params(exact_min=0, refine_mb=0, verify_mb=0, prepass=prepass)
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "rgx")
+        capfd.readouterr()
+        counts = engine.search(pre, dsx.SEARCH_LIST, seeds=_seeds("gated_1e12_1m.txt"), params=p)
+        _check("rgx", pre, counts, stderr=capfd.readouterr().err)
+
+
+def test_production_candidates_pipeline(engine, capfd):
+    """Production Params (stdin=1) on the seeds of the verified finds and two seeds whose CANDIDATE lines a
+    GPU run printed, with the FP32 pre-pass: step-4 floods at the production thresholds and the
+    CANDIDATE list equal the reference's."""
+    dsx = load_package().dsx
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "fx")
+        capfd.readouterr()
+        counts = engine.search(pre, dsx.SEARCH_LIST, seeds=_seeds("fx_seeds.txt"),
+                               params=engine.search_params(prepass=1))
+        _check("fx", pre, counts, stderr=capfd.readouterr().err)
+
+
+def test_cli_production_candidates(engine):
+    """The same through the dsearch_desert command line (stdin=1), plain FP64: rows, seeds CSV, JSON
+    counters and the CANDIDATE lines on stderr."""
+    exe = os.path.join(REPO, "desert-search_amd", "dsearch_desert")
+    lines = open(os.path.join(GOLDEN, "fx_seeds.txt")).read()
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "fx")
+        r = subprocess.run([exe, "0", "0", pre, "8", "stdin=1"], input=lines, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+        js = json.load(open(pre + ".json"))
+        _check("fx", pre, (js["done"], js["passed"], js["scanned"], js["sites"]), stderr=r.stderr)
 
 
 def test_3m_pipeline(engine):
