@@ -19,7 +19,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared", "
              "-fno-gpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
 HIP_SOURCES = ["csrc/gate.hip", "csrc/tables.hip", "csrc/climate.hip", "csrc/floods.hip", "csrc/flood_lds.hip",
-               "csrc/dsx_api.hip",
+               "csrc/dsx_api.hip", "csrc/gate_noalloc.hip",
                "csrc/scan_api.hip", "csrc/floods_api.hip", "host/search.cpp", "host/serve.cpp"]
 HOST_TOOLS = ["host/dsearch_desert.cpp", "host/tgate.cpp", "host/biomemap.cpp", "host/dfunc.cpp"]
 

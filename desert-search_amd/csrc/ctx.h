@@ -9,6 +9,8 @@
 namespace dsx {
 extern thread_local std::string g_last_error;
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
+GateKernel gate_kernel(int R, float sig);            // GPU-gate blur weights (gate.cuh:216, 248)
+bool valid_gate_params(const dsx_gate_params *p);
 }  // namespace dsx
 
 #define HIPCK(x)                                                                    \

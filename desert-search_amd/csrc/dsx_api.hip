@@ -21,10 +21,9 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
 }  // namespace dsx
 using dsx::g_last_error;
 
-namespace {
 // Gate blur weights: expf(-0.5 d^2 / sig^2) in float, as gate.cuh:216 forms them, evaluated here
 // with the host libm; ksum is their float sum in index order and norm = 1 / (ksum * ksum).
-dsx::GateKernel gate_kernel(int R, float sig) {
+dsx::GateKernel dsx::gate_kernel(int R, float sig) {
     dsx::GateKernel K{};
     for (int t = 0; t <= 2 * R; t++) {
         const float d = float(t - R);
@@ -36,6 +35,14 @@ dsx::GateKernel gate_kernel(int R, float sig) {
     K.wd = double(float(1.25 * (32.0 / 63.0) * 1.5));
     return K;
 }
+bool dsx::valid_gate_params(const dsx_gate_params *p) {
+    return p && p->box >= 0 && p->box <= 1 && std::isfinite(p->max_phase) && std::isfinite(p->amp) &&
+           std::isfinite(p->pthr) && std::isfinite(p->margin) && p->sig > 0 && std::isfinite(p->sig);
+}
+
+namespace {
+using dsx::gate_kernel;
+using dsx::valid_gate_params;
 // CPU-default pre-gate kernel (lowfield.hpp:92-96): r = 4, float(exp(-d^2/2)) normalised by their
 // double sum; octave weight factor * weight of temperature octave 0 (dsearch_desert.cpp:217).
 dsx::GateKernel cpu_gate_kernel() {
@@ -149,11 +156,6 @@ int dsx_destroy(dsx_ctx *c) {
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return DSX_OK;
-}
-
-static bool valid_gate_params(const dsx_gate_params *p) {
-    return p && p->box >= 0 && p->box <= 1 && std::isfinite(p->max_phase) && std::isfinite(p->amp) &&
-           std::isfinite(p->pthr) && std::isfinite(p->margin) && p->sig > 0 && std::isfinite(p->sig);
 }
 
 int dsx_gate(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *p, uint64_t *seeds, float *amps,

@@ -26,9 +26,12 @@ struct GateKernel {
 };
 
 void launch_phase(hipStream_t st, uint64_t base, uint32_t n, double maxph, uint32_t *out, uint32_t *count, uint32_t cap);
-void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64_t *seeds, uint32_t n, SideParams *out);
+// n_dev (optional): a device word holding the item count, read by the kernels (min(n, *n_dev)); the grid
+// still covers n items, so a caller that does not know the count on the host launches with an upper bound.
+void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64_t *seeds, uint32_t n, SideParams *out,
+                 const uint32_t *n_dev = nullptr);
 void launch_amp(hipStream_t st, int mode, const SideParams *sides, const uint32_t *sel, uint32_t n, const GateKernel &K,
                 double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count, uint32_t cap,
-                float *all_amp);
+                float *all_amp, const uint32_t *n_dev = nullptr);
 
 }  // namespace dsx

@@ -41,10 +41,13 @@ __global__ __launch_bounds__(256) void k_phase(uint64_t base, uint32_t n, double
 // ------------------------------------------------------------------ Perlin init, one lane per side
 constexpr int PERM_STRIDE = 260;   // 65 dwords: lane-private slots start in different LDS banks
 
+// n_dev (optional): the number of seeds is min(n, *n_dev), read on the device (the grid covers n).
 __global__ __launch_bounds__(64) void k_init(uint64_t base, const uint32_t *ks, const uint64_t *seeds, uint32_t n,
-                                             SideParams *out) {
+                                             SideParams *out, const uint32_t *n_dev) {
     __shared__ uint32_t slot32[64 * PERM_STRIDE / 4];
     uint8_t *slots = reinterpret_cast<uint8_t *>(slot32);
+    if (n_dev) n = min(n, *n_dev);
+    if (blockIdx.x * 64u >= 2u * n) return;
     const int lane = threadIdx.x;
     const uint32_t g = blockIdx.x * 64u + lane;
     const bool active = g < 2u * n;
@@ -106,14 +109,16 @@ __device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K)
 
 // One 128-thread block per seed: wave 0 = side A, wave 1 = side B. sel (optional) maps block -> side
 // pair index. Passing seeds are appended as (kmap ? kmap[pair] : pair, amplitude); all_amp (optional)
-// gets every amplitude, indexed by block.
+// gets every amplitude, indexed by block. n_dev (optional): the number of seeds is min(n, *n_dev), read
+// on the device (the grid covers n).
 template <int MODE>
 __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint32_t *sel, uint32_t n, GateKernel K,
                                              double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp,
-                                             uint32_t *count, uint32_t cap, float *all_amp) {
+                                             uint32_t *count, uint32_t cap, float *all_amp, const uint32_t *n_dev) {
     __shared__ WaveLds L[2];
     __shared__ float red[2];
     const uint32_t item = blockIdx.x;
+    if (n_dev) n = min(n, *n_dev);
     if (item >= n) return;
     const uint32_t pidx = sel ? sel[item] : item;
     const int wv = threadIdx.x >> 6;
@@ -139,23 +144,24 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
 void launch_phase(hipStream_t st, uint64_t base, uint32_t n, double maxph, uint32_t *out, uint32_t *count, uint32_t cap) {
     hipLaunchKernelGGL(k_phase, dim3((n + 255) / 256), dim3(256), 0, st, base, n, maxph, out, count, cap);
 }
-void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64_t *seeds, uint32_t n, SideParams *out) {
+void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64_t *seeds, uint32_t n, SideParams *out,
+                 const uint32_t *n_dev) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_init, dim3((2 * n + 63) / 64), dim3(64), 0, st, base, ks, seeds, n, out);
+    hipLaunchKernelGGL(k_init, dim3((2 * n + 63) / 64), dim3(64), 0, st, base, ks, seeds, n, out, n_dev);
 }
 void launch_amp(hipStream_t st, int mode, const SideParams *sides, const uint32_t *sel, uint32_t n, const GateKernel &K,
                 double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count, uint32_t cap,
-                float *all_amp) {
+                float *all_amp, const uint32_t *n_dev) {
     if (n == 0) return;
     if (mode == 0)
         hipLaunchKernelGGL(k_amp<0>, dim3(n), dim3(128), 0, st, sides, sel, n, K, thr, kmap, out_idx, out_amp, count, cap,
-                           all_amp);
+                           all_amp, n_dev);
     else if (mode == 1)
         hipLaunchKernelGGL(k_amp<1>, dim3(n), dim3(128), 0, st, sides, sel, n, K, thr, kmap, out_idx, out_amp, count, cap,
-                           all_amp);
+                           all_amp, n_dev);
     else
         hipLaunchKernelGGL(k_amp<2>, dim3(n), dim3(128), 0, st, sides, sel, n, K, thr, kmap, out_idx, out_amp, count, cap,
-                           all_amp);
+                           all_amp, n_dev);
 }
 
 }  // namespace dsx

@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdsx.so")
 
 DSX_OK, DSX_E_ARG, DSX_E_HIP, DSX_E_CAPACITY, DSX_E_NODEVICE, DSX_E_BUSY = 0, 1, 2, 3, 4, 5
+DSX_NOALLOC_MAX_COUNT, DSX_NOALLOC_MAX_PHASE = 1 << 22, 1 << 16   # dsx_gate_noalloc limits per call
 HW_QUEUES = 20   # hardware queues requested from the HIP runtime (16 pipeline workers, the gate, spares)
 
 
@@ -146,6 +147,9 @@ def load(path=LIB_PATH):
     lib.dsx_destroy.argtypes = [ctypes.c_void_p]
     lib.dsx_gate.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(GateParams), _u64p, _f32p,
                              _u64p, ctypes.c_uint64, _u64p, ctypes.POINTER(GateCounts)]
+    lib.dsx_gate_noalloc.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                     ctypes.POINTER(GateParams), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_uint64, ctypes.c_void_p]
     lib.dsx_gate_amps.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, ctypes.POINTER(GateParams), _f32p, _f32p]
     lib.dsx_gate_profile.argtypes = [ctypes.c_void_p, _f64p, _u64p]
     lib.dsx_scan.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, ctypes.POINTER(ScanParams), _f32p, _f32p,

@@ -77,6 +77,26 @@ int dsx_gate_cpu(dsx_ctx *ctx, uint64_t start, uint64_t count, double max_phase,
  * Replaces the calibration mode of the reference's gate tool (tgate.cu:53-85, `calib=N`). */
 int dsx_gate_amps(dsx_ctx *ctx, const uint64_t *seeds, uint64_t n, const dsx_gate_params *p, float *pre, float *full);
 
+/* The gate of dsx_gate without any allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It takes no context; every intermediate lives in a static device array of the library
+ * (DSX_NOALLOC_SCRATCH_BYTES, about 40 MB, held on each device the call runs on until the process ends).
+ * Seed indices [start, start+count) with count <= DSX_NOALLOC_MAX_COUNT (larger: DSX_E_ARG); the call
+ * enqueues its kernels on `stream` (a hipStream_t of `device`; null: the null stream) and returns without
+ * waiting. Outputs are DEVICE memory written by those kernels: survivors sorted by k into seeds / amps /
+ * kidx (each may be null; at most cap entries), and counts[0..3] = {phase survivors, pre-gate survivors,
+ * full-gate survivors, entries written = min(full, cap)}. If counts[0] > DSX_NOALLOC_MAX_PHASE the phase
+ * survivors overflowed the scratch list: no survivors are written (counts[1..3] = 0); call again on a
+ * smaller range. Results equal dsx_gate's on the same range. All calls on one device share its scratch
+ * array: calls that overlap in time on one device (other streams, other threads) race. */
+enum {
+    DSX_NOALLOC_MAX_COUNT = 1 << 22,    /* seed indices per call */
+    DSX_NOALLOC_MAX_PHASE = 1 << 16,    /* phase survivors per call (0.72% of 2^22 at max_phase 0.06 is ~30k) */
+    DSX_NOALLOC_SCRATCH_BYTES = 39846144
+};
+int dsx_gate_noalloc(int device, void *stream, uint64_t start, uint64_t count, const dsx_gate_params *p, uint64_t *seeds,
+                     float *amps, uint64_t *kidx, uint64_t cap, uint64_t *counts);
+
 /* Device time (ms) of the kernels of the last dsx_gate call, summed over its batches:
  * ms[0] phase filter, ms[1] Perlin init, ms[2] pre-gate, ms[3] full gate; work[0..3] the number of
  * items each processed (seed indices, sides, pre-gate seeds, full-gate seeds). */
