@@ -47,8 +47,9 @@ import threading
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
-# hardware queues for the 16 pipeline worker streams, the gate and spares (DSX_HW_QUEUES overrides)
-os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("DSX_HW_QUEUES", "20")
+# hardware queues for the 16 pipeline worker streams, the gate and spares (DSX_HW_QUEUES overrides, clamped to
+# [1, 28] as desert-search_amd/host/hwq.h does)
+os.environ["GPU_MAX_HW_QUEUES"] = str(max(1, min(28, int(os.environ.get("DSX_HW_QUEUES") or 20))))
 START = 10 ** 12
 N_STEP = 10 ** 8
 FP64_PEAK_TFLOPS = 78.6
@@ -352,6 +353,12 @@ def main():
                                 "phase_survivors": int(work[2])}
         out["area_queries"] = area_burst(dsx)
         out["serving"] = serving(dsx)
+    # flood batches that hit the engine's time limit (run again on a reset engine) or exhausted its tile pool,
+    # over every leg of this run: 0 in a healthy run
+    st2 = dsx.FloodStats()
+    lib.dsx_flood_stats_get(ctypes.byref(st2), 1)
+    out["flood_health"] = {"time_retries": int(st.time_retries + st2.time_retries),
+                           "capacity_resets": int(st.capacity_resets + st2.capacity_resets)}
     out["cpu_baseline"] = None if a.no_cpu else cpu_baseline()
     print(json.dumps(out))
 

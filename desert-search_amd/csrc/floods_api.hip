@@ -141,7 +141,7 @@ static int clim_alloc(dsx_ctx *c, ClimState *&S) {
     HIPCK(hipHostMalloc(&S->h_useeds, size_t(S->clim_cap) * 12, hipHostMallocDefault));
     // flood engine
     dsx::FloodBufs &B = S->fb;
-    B.tile_cap = 1u << 21;                    // 2M tiles x 528 B
+    B.tile_cap = 1u << 21;                    // 2M tiles x sizeof(Tile) (672 B): 1.4 GB
     B.hmask = (1u << 23) - 1;                 // 8M hash slots
     HIPCK(hipMalloc(&B.tiles, size_t(B.tile_cap) * sizeof(dsx::Tile)));
     // every initialisation is ordered on the context's stream and completed before the state is used:
@@ -395,7 +395,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     if (!c || (n && (!jobs || !out))) return DSX_E_ARG;
     for (uint64_t k = 0; k < n; k++) {
         const dsx_flood_job &j = jobs[k];
-        if (j.kind < 0 || j.kind > 3 || j.step < 1 || j.half < 0 || 2 * int64_t(j.half) + 1 > 65535 ||
+        if (j.kind < 0 || j.kind > 3 || j.step < 1 || j.half < 0 || 2 * int64_t(j.half) + 1 > DSX_FLOOD_MAX_WIDTH ||
             int64_t(j.half) * j.step > 1000000 || j.stop_cells < 0 || j.stop_cells > (1 << 23))
             return DSX_E_ARG;
     }
@@ -536,6 +536,11 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
                           h[dsx::TIME_ITEM], unsigned(n), n_big, blocks_used, seq + 1, h[dsx::STAGE_SEQ],
                           h[dsx::RUN_SEQ], h[dsx::RUN_TAIL0], h[dsx::RUN_STAGE_SEQ]);
             dsx::g_last_error = msg;
+        }
+        {
+            std::lock_guard<std::mutex> l(g_stats_mu);
+            if (timed_out) g_stats.time_retries++;
+            else g_stats.capacity_resets++;
         }
         const uint32_t ntiles = timed_out ? B.tile_cap : std::min(h[dsx::CTL_NTILES], B.tile_cap);
         const uint32_t qtail = timed_out ? B.wq_cap : std::min(h[dsx::Q_TAIL], B.wq_cap);

@@ -17,6 +17,14 @@ LIB_PATH = os.path.join(HERE, "libdsx.so")
 DSX_OK, DSX_E_ARG, DSX_E_HIP, DSX_E_CAPACITY, DSX_E_NODEVICE, DSX_E_BUSY = 0, 1, 2, 3, 4, 5
 DSX_NOALLOC_MAX_COUNT, DSX_NOALLOC_MAX_PHASE = 1 << 22, 1 << 16   # dsx_gate_noalloc limits per call
 HW_QUEUES = 20   # hardware queues requested from the HIP runtime (16 pipeline workers, the gate, spares)
+HW_QUEUES_MAX = 28   # the most measured since the flood engine's stream-ordering fix (host/hwq.h)
+
+
+def hw_queues():
+    """GPU_MAX_HW_QUEUES for this process: DSX_HW_QUEUES if set, clamped to [1, HW_QUEUES_MAX], else 20."""
+    v = os.environ.get("DSX_HW_QUEUES", "")
+    q = int(v) if v.strip() else HW_QUEUES
+    return max(1, min(HW_QUEUES_MAX, q))
 
 
 class DsxError(RuntimeError):
@@ -113,7 +121,8 @@ class FloodStats(ctypes.Structure):
                 ("size_floods", (ctypes.c_uint64 * 6) * 4), ("size_tiles", (ctypes.c_uint64 * 6) * 4),
                 ("size_sampled", (ctypes.c_uint64 * 6) * 4), ("eval_rounds", ctypes.c_uint64),
                 ("eval_needed", ctypes.c_uint64), ("wg_wait", ctypes.c_uint64), ("wg_eval", ctypes.c_uint64),
-                ("wg_load", ctypes.c_uint64), ("wg_life", ctypes.c_uint64)]
+                ("wg_load", ctypes.c_uint64), ("wg_life", ctypes.c_uint64), ("time_retries", ctypes.c_uint64),
+                ("capacity_resets", ctypes.c_uint64)]
 
 
 class GateCounts(ctypes.Structure):
@@ -135,9 +144,7 @@ def load(path=LIB_PATH):
     if not os.path.exists(path):
         raise DsxError(DSX_E_NODEVICE, "libdsx.so not built", path)
     # hardware queues for the pipeline worker streams, the gate and spares, read by the HIP runtime when it starts
-    # (DSX_HW_QUEUES overrides the count). Streams beyond it share queues; with 32 queues the persistent flood kernel
-    # has run into its time limit; 20 measured as fast as 24
-    os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("DSX_HW_QUEUES", str(HW_QUEUES))
+    os.environ["GPU_MAX_HW_QUEUES"] = str(hw_queues())
     lib = ctypes.CDLL(path)
     lib.dsx_strerror.restype = ctypes.c_char_p
     lib.dsx_strerror.argtypes = [ctypes.c_int]

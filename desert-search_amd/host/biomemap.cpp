@@ -7,6 +7,7 @@
 // Biome ids and desert states come from the GPU (dsx_points: DSX_POINT_BIOME, DSX_PRED_DESERT_STATE); the
 // component walk over the map's bytes stays on the host.
 #include "../../include/dsx.h"
+#include "hwq.h"
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -19,8 +20,7 @@ static int fail(const char *what, int rc) {
 }
 
 int main(int argc, char **argv) {
-    const char *hwq = std::getenv("DSX_HW_QUEUES");
-    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "20", 1);
+    dsx_set_hw_queues();
     if (argc < 4) {
         std::fprintf(stderr, "biomemap SEED X Z [HALF_BLOCKS] [STEP_BLOCKS]\n");
         return 2;

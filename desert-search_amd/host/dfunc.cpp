@@ -5,14 +5,14 @@
 // E bin (set 0: C and E octave 0; set 1: C octaves 0-2, E octaves 0-1). gen_dtable.py turns them into the
 // 40 x 40 proxy table.
 #include "../../include/dsx.h"
+#include "hwq.h"
 #include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
 
 int main(int argc, char **argv) {
-    const char *hwq = std::getenv("DSX_HW_QUEUES");
-    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "20", 1);
+    dsx_set_hw_queues();
     if (argc < 4) {
         std::fprintf(stderr, "dfunc SEEDS POINTS THREADS\n");
         return 2;

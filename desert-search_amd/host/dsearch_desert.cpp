@@ -7,6 +7,7 @@
 // unbounded); stdin=1 reads gated seeds ("seed [amp]" lines) from stdin; otherwise the CPU-default gate
 // semantics (phase <= `phase`, amplitude_sub >= amp - amp_margin, maxta + maxtb >= amp) apply.
 #include "../../include/dsx.h"
+#include "hwq.h"
 #include <chrono>
 #include <csignal>
 #include <cstdio>
@@ -19,11 +20,7 @@ static volatile sig_atomic_t interrupted = 0;
 static void on_signal(int) { interrupted = 1; }
 
 int main(int argc, char **argv) {
-    // hardware queues for the pipeline worker streams, the gate and spares, read by the HIP runtime when it starts
-    // (DSX_HW_QUEUES overrides the count). Streams beyond it share queues; with 32 queues the persistent flood
-    // kernel has run into its time limit; 20 measured as fast as 24
-    const char *hwq = std::getenv("DSX_HW_QUEUES");
-    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "20", 1);
+    dsx_set_hw_queues();
     if (argc < 4) {
         std::fprintf(stderr, "dsearch_desert START COUNT OUT_PREFIX [threads] [key=value...]\n");
         return 2;

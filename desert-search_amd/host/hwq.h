@@ -1,0 +1,16 @@
+// Hardware queues of the process for the HIP runtime, set before the runtime starts.
+#pragma once
+#include <cstdlib>
+#include <string>
+
+// 20 queues serve the 16 pipeline worker streams, the gate and spares (20 measured as fast as 24 or 28).
+// DSX_HW_QUEUES overrides the count, clamped to [1, 28]: 28 is the most measured since the stream-ordering
+// fix of the flood engine's initialisation; the time-limit stalls once seen at 32 queues came before that
+// fix, and 32 has not been measured since.
+inline void dsx_set_hw_queues() {
+    int q = 20;
+    if (const char *e = std::getenv("DSX_HW_QUEUES"))
+        if (*e) q = std::atoi(e);
+    q = q < 1 ? 1 : (q > 28 ? 28 : q);
+    setenv("GPU_MAX_HW_QUEUES", std::to_string(q).c_str(), 1);
+}

@@ -8,6 +8,7 @@
 //   check=1  the first 4 passing seeds are re-evaluated from an explicit seed list (dsx_gate_amps), and
 //            both amplitudes are printed to stderr
 #include "../../include/dsx.h"
+#include "hwq.h"
 #include "../csrc/dsx_rng.h"
 #include <algorithm>
 #include <chrono>
@@ -23,8 +24,7 @@ static int fail(const char *what, int rc) {
 }
 
 int main(int argc, char **argv) {
-    const char *hwq = std::getenv("DSX_HW_QUEUES");
-    setenv("GPU_MAX_HW_QUEUES", hwq && *hwq ? hwq : "20", 1);
+    dsx_set_hw_queues();
     if (argc < 3) {
         std::fprintf(stderr,
                      "tgate START COUNT [phase=0.10] [amp=0.70] [box=1 pthr=0.645 | box=0 margin=0.01] [check=0] [calib=N]\n");

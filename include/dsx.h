@@ -203,9 +203,13 @@ typedef struct {
 /* A batch of floods, evaluated together on the device. Replaces dense_flood_bridged
  * (dense_flood.hpp:52-86) at every call site: hot zones (dsearch_desert.cpp:337, 352), proxy floods
  * (:387), exact floods (:512, :521) and tools/dflood.cpp:20. Results are identical to the reference's
- * depth-first flood (they do not depend on visiting order). DSX_E_CAPACITY if the batch needs more
- * tiles than the engine holds (split the batch). A batch whose tiled engine reaches its time limit
- * (a queue item not taken for 5 s) runs once more on a reset engine; DSX_E_HIP if it does again. */
+ * depth-first flood (they do not depend on visiting order). Windows: 2*half+1 <= DSX_FLOOD_MAX_WIDTH
+ * lattice points per side (so one flood's window holds at most the engine's 2^21 tiles of 32 x 32) and
+ * half*step <= 10^6 quarts; otherwise DSX_E_ARG. DSX_E_CAPACITY if the batch needs more tiles or queue
+ * slots than the engine holds (split the batch). A batch whose tiled engine reaches its time limit (a queue
+ * item not taken for 5 s) runs once more on a reset engine; DSX_E_HIP if it does again. Both events are
+ * counted in dsx_flood_stats (time_retries, capacity_resets). */
+enum { DSX_FLOOD_MAX_WIDTH = 46335 };
 int dsx_flood(dsx_ctx *ctx, const dsx_flood_job *jobs, uint64_t n, dsx_area *out);
 
 /* The counted cells (state 3 in dense_flood.hpp's window) of the floods of the last dsx_flood call, as
@@ -242,6 +246,11 @@ typedef struct {
      * workgroups: waiting for a queue item, evaluating sub-blocks (octave loads included), loading octaves
      * into LDS, and from start to exit */
     uint64_t wg_wait, wg_eval, wg_load, wg_life;
+    /* batches that failed and were cleaned up, counted whether or not the call then succeeded:
+     * time_retries = batches whose tiled engine reached its time limit (a queue item not taken for 5 s) and
+     * were run again on a reset engine; capacity_resets = batches that exhausted the tile pool or the work
+     * queue (the call returns DSX_E_CAPACITY). Both are 0 in a healthy run. */
+    uint64_t time_retries, capacity_resets;
 } dsx_flood_stats;
 /* 1: bracket every launch of the tiled engine's kernel with HIP events on its stream (eval_ms). */
 int dsx_profile(int on);

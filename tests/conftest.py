@@ -123,6 +123,22 @@ class Oracle:
         return out[:n]
 
 
+@pytest.fixture(autouse=True, scope="module")
+def flood_health(request):
+    """After every GPU test module: no flood batch reached the engine's time limit (and was run again) or
+    exhausted its tile pool, counted process-wide by dsx_flood_stats."""
+    yield
+    marks = getattr(request.module, "pytestmark", [])
+    marks = marks if isinstance(marks, list) else [marks]
+    pkg = sys.modules.get("desert_search_amd")
+    if not any(m.name == "gpu" for m in marks) or pkg is None or pkg.dsx._lib is None:
+        return
+    st = pkg.dsx.FloodStats()
+    assert pkg.dsx._lib.dsx_flood_stats_get(ctypes.byref(st), 0) == 0
+    assert (st.time_retries, st.capacity_resets) == (0, 0), \
+        f"flood engine: {st.time_retries} time-limit retries, {st.capacity_resets} capacity resets"
+
+
 @pytest.fixture(scope="session")
 def oracle():
     return Oracle()

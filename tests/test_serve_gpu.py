@@ -75,6 +75,31 @@ def test_area_workload_batch_matches_per_query(engine):
         assert [_t(a) for a in part] == [_t(a) for a in big[lo:lo + 128]]
 
 
+def test_area_workload_4096_vs_oracle(engine, oracle):
+    """The whole synthetic area workload (4096 queries: seed splitmix64(j), X and Z uniform in +-3e7,
+    windows from caps log-uniform over 1e3..1e7 cells) in one dsx_area_queries batch, every field against
+    the oracle: the step-1, 4-connected, river-bridged flood dense_flood_bridged(desert_state, qx, qz, 1,
+    half) where the start is desert (dflood.cpp:17-23, dense_flood.hpp:52-86), else no flood."""
+    import concurrent.futures as cf
+    import os as _os
+    dsx = load_package().dsx
+    q, caps = dsx.area_workload(4096)
+    got = engine.area_queries(q)
+    qx = (q["x"].astype(np.int64) >> 2).astype(np.int32)
+    qz = (q["z"].astype(np.int64) >> 2).astype(np.int32)
+    state = np.array([oracle.pred_batch(int(q[j]["seed"]), 0, [qx[j]], [qz[j]])[0] for j in range(len(q))])
+    starts = np.flatnonzero(state == 2)
+    for j in np.flatnonzero(state != 2):
+        assert _t(got[j]) == (0, 1, qx[j], qx[j], qz[j], qz[j], False, False), j
+    with cf.ThreadPoolExecutor(min(16, _os.cpu_count() or 1)) as ex:
+        want = list(ex.map(lambda j: oracle.flood(int(q[j]["seed"]), 0, 0.35, int(qx[j]), int(qz[j]), 1,
+                                                  int(q[j]["half"]), 0), starts))
+    bad = [(int(j), _t(got[j]), w) for j, w in zip(starts, want) if _t(got[j]) != w]
+    assert not bad, f"{len(bad)} of {len(starts)} floods differ, first {bad[0]}"
+    assert len(starts) >= 50 and sum(w[6] for w in want) >= 20   # floods that reach their window's edge
+    test_area_workload_4096_vs_oracle.result = [_t(a) for a in got]
+
+
 @pytest.fixture(scope="module")
 def server():
     dsx = load_package().dsx
@@ -94,6 +119,21 @@ def test_server_scan_equals_reference(server):
     assert [r.counts.done, r.counts.passed, r.counts.scanned, r.counts.sites] == \
         [gold["done"], gold["passed"], gold["scanned"], gold["sites"]]
     server.release(rid)
+
+
+def test_server_area_workload_4096(server, engine):
+    """The 4096 workload queries submitted to the server at once (continuous batching across its workers)
+    equal the single-batch results, which the oracle pins (test above)."""
+    dsx = load_package().dsx
+    q, caps = dsx.area_workload(4096)
+    want = getattr(test_area_workload_4096_vs_oracle, "result", None) or [_t(a) for a in engine.area_queries(q)]
+    ids = [server.submit_area(int(r["seed"]), int(r["x"]), int(r["z"]), int(r["half"])) for r in q]
+    for k, rid in enumerate(ids):
+        r = server.wait(rid)
+        assert r.status == 0
+        a = r.area
+        assert (a.cells, a.sampled, a.minx, a.maxx, a.minz, a.maxz, bool(a.edge), bool(a.border)) == want[k], k
+        server.release(rid)
 
 
 def test_server_alone_equals_mix(server):
