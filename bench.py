@@ -27,6 +27,8 @@ Also reported:
   cpu_baseline  the reference's own CPU dsearch_desert (built from its sources into oracle/_ref, gpu=0)
                 over a bounded sample of seed indices on this host's cores: scanned / seconds. Falls
                 back to the oracle's C++ restatement of the same pipeline when oracle/_ref is absent.
+                post_gate: the same CPU pipeline in stdin=1 mode on exactly the seeds the GPU gate passes
+                over config 3's 1e7-index range, so both sides run the same gated seeds after the gate.
   gate_only     the GPU gate alone over 2^24 seed indices (the gate kernel workload).
   area_queries  4096 exact step-1 area queries (seed splitmix64(j), X, Z uniform in +-3e7, window
                 half = floor((sqrt(cap)-1)/2) for caps log-uniform over 1e3..1e7 cells) submitted at
@@ -88,33 +90,57 @@ def host_cores():
     return n
 
 
-def cpu_baseline(sample=10 ** 7):
-    """The reference CPU pipeline (gpu=0) over seed indices [0, sample) on this host's cores."""
+def _cpu_run(args, pre, stdin_text=None, seeds=None, sample=0):
+    """One CPU pipeline run: the reference's dsearch_desert from oracle/_ref when built, else the oracle's
+    restatement. Returns (kind, scanned, seconds, passed)."""
     cores = host_cores()
     ref = os.path.join(REPO, "oracle", "_ref", "dsearch_desert")
+    if os.access(ref, os.X_OK):
+        subprocess.run([ref] + args[:2] + [pre, str(cores)] + args[2:], check=True, input=stdin_text, text=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        m = json.load(open(pre + ".json"))
+        return "reference", m["scanned"], m["seconds"], m["passed"]
+    import numpy as np
+    L = ctypes.CDLL(os.path.join(REPO, "oracle", "liboracle.so"))
+    L.or_run.restype = ctypes.c_int
+    L.or_run.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                         ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    cnt = (ctypes.c_uint64 * 4)()
+    secs = ctypes.c_double()
+    if seeds is not None:
+        arr = np.ascontiguousarray(seeds, dtype=np.uint64)
+        L.or_run(0, 0, arr.ctypes.data, len(arr), 1, cores, pre.encode(), cnt, ctypes.byref(secs), None, 0)
+    else:
+        L.or_run(0, sample, None, 0, 0, cores, pre.encode(), cnt, ctypes.byref(secs), None, 0)
+    return "port", cnt[2], secs.value, cnt[1]
+
+
+def cpu_baseline(gated_seeds, gpu_value, sample=10 ** 7):
+    """Two CPU legs on this host's cores:
+    - the reference CPU pipeline with its own CPU gate (gpu=0) over seed indices [0, sample): the whole job
+      the reference does without a GPU (value, the reported baseline);
+    - post_gate: the reference in stdin=1 mode (dsearch_desert.cpp:571-581) on exactly the seeds the GPU gate
+      passes over config 3's range [1e12, 1e12 + 1e7): the same gated seeds the GPU pipeline processes,
+      so the ratio compares the post-gate pipeline like for like."""
+    cores = host_cores()
     with tempfile.TemporaryDirectory() as td:
-        pre = os.path.join(td, "cpu")
-        if os.access(ref, os.X_OK):
-            subprocess.run([ref, "0", str(sample), pre, str(cores)], check=True, stdout=subprocess.DEVNULL,
-                           stderr=subprocess.DEVNULL)
-            m = json.load(open(pre + ".json"))
-            kind, what = "reference", "the reference's dsearch_desert built from its sources"
-            scanned, sec, passed = m["scanned"], m["seconds"], m["passed"]
-        else:
-            L = ctypes.CDLL(os.path.join(REPO, "oracle", "liboracle.so"))
-            L.or_run.restype = ctypes.c_int
-            L.or_run.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
-                                 ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                 ctypes.c_int]
-            cnt = (ctypes.c_uint64 * 4)()
-            secs = ctypes.c_double()
-            L.or_run(0, sample, None, 0, 0, cores, pre.encode(), cnt, ctypes.byref(secs), None, 0)
-            kind, what = "port", "the oracle's C++ restatement of the reference pipeline"
-            scanned, sec, passed = cnt[2], secs.value, cnt[1]
-    return {"value": scanned / sec, "unit": "gated seeds/s", "cores": cores, "kind": kind,
+        kind, scanned, sec, passed = _cpu_run(["0", str(sample)], os.path.join(td, "cpu"), sample=sample)
+        text = "".join(f"{int(x)}\n" for x in gated_seeds)
+        kind2, scanned2, sec2, _ = _cpu_run(["0", "0", "stdin=1"], os.path.join(td, "post"), stdin_text=text,
+                                            seeds=gated_seeds)
+    what = ("the reference's dsearch_desert built from its sources" if kind == "reference"
+            else "the oracle's C++ restatement of the reference pipeline")
+    value = scanned / sec
+    post = scanned2 / sec2
+    return {"value": value, "unit": "gated seeds/s", "cores": cores, "kind": kind,
             "sample": f"seed indices [0, {sample:.0e}) through {what} with its CPU gate (gpu=0, production "
                       f"Params): {passed} phase survivors, {scanned} gated seeds through the pipeline in "
-                      f"{sec:.1f} s on {cores} threads"}
+                      f"{sec:.1f} s on {cores} threads",
+            "gpu_vs_cpu": gpu_value / value,
+            "post_gate": {"value": post, "unit": "gated seeds/s", "cores": cores, "kind": kind2,
+                          "sample": f"{scanned2} seeds the GPU gate passes over [1e12, 1e12 + 1e7), through the "
+                                    f"same CPU pipeline in stdin=1 mode (no CPU gate) in {sec2:.1f} s",
+                          "gpu_vs_cpu": gpu_value / post}}
 
 
 def measured_traffic():
@@ -336,6 +362,8 @@ def main():
     out["prepass"] = {"value": gated_pp / sec_pp, "ms_per_step": 1e3 * sec_pp / a.steps, "gated_seeds": int(gated_pp),
                       "outputs_identical": bool(same and gated_pp == gated), "vs_fp64": (gated_pp / sec_pp) / value,
                       "fp32_error": err, "bound_T": 2e-4}
+    # the seeds the GPU gate passes over config 3's range: the CPU baseline's post-gate leg runs exactly these
+    cfg3_gated = eng.gate(START, 10 ** 7)[0] if not a.no_cpu else None
     eng.close()
     if not a.no_extra:
         g = dsx.Engine(0)
@@ -359,7 +387,7 @@ def main():
     lib.dsx_flood_stats_get(ctypes.byref(st2), 1)
     out["flood_health"] = {"time_retries": int(st.time_retries + st2.time_retries),
                            "capacity_resets": int(st.capacity_resets + st2.capacity_resets)}
-    out["cpu_baseline"] = None if a.no_cpu else cpu_baseline()
+    out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(cfg3_gated, value)
     print(json.dumps(out))
 
 
