@@ -80,6 +80,7 @@ const char *dsx_strerror(int code) {
     case DSX_E_CAPACITY: return "capacity exceeded";
     case DSX_E_NODEVICE: return "no usable gfx950 device";
     case DSX_E_BUSY: return "request not finished";
+    case DSX_E_INTERRUPTED: return "interrupted";
     default: return "unknown error";
     }
 }

@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdsx.so")
 
-DSX_OK, DSX_E_ARG, DSX_E_HIP, DSX_E_CAPACITY, DSX_E_NODEVICE, DSX_E_BUSY = 0, 1, 2, 3, 4, 5
+DSX_OK, DSX_E_ARG, DSX_E_HIP, DSX_E_CAPACITY, DSX_E_NODEVICE, DSX_E_BUSY, DSX_E_INTERRUPTED = 0, 1, 2, 3, 4, 5, 6
 DSX_NOALLOC_MAX_COUNT, DSX_NOALLOC_MAX_PHASE = 1 << 22, 1 << 16   # dsx_gate_noalloc limits per call
 HW_QUEUES = 20   # hardware queues requested from the HIP runtime (16 pipeline workers, the gate, spares)
 HW_QUEUES_MAX = 28   # the most measured since the flood engine's stream-ordering fix (host/hwq.h)
@@ -81,7 +81,9 @@ class SearchParams(ctypes.Structure):
 
 class SearchCounts(ctypes.Structure):
     _fields_ = [("done", ctypes.c_uint64), ("passed", ctypes.c_uint64), ("scanned", ctypes.c_uint64),
-                ("sites", ctypes.c_uint64)]
+                ("sites", ctypes.c_uint64), ("hot_zones", ctypes.c_uint64), ("big_zones", ctypes.c_uint64),
+                ("sec_tables", ctypes.c_double), ("sec_scan", ctypes.c_double), ("sec_sites", ctypes.c_double),
+                ("sec_proxy", ctypes.c_double), ("sec_hot", ctypes.c_double), ("sec_gpu", ctypes.c_double)]
 
 
 SEARCH_ENUM, SEARCH_GATE, SEARCH_LIST = 0, 1, 2

@@ -2,10 +2,14 @@
 //   dsearch_desert START COUNT OUT_PREFIX [THREADS] [key=value ...]     seeds = splitmix64(START + k)
 // Same keys, defaults (struct Params, dsearch_desert.cpp:50-78), checks, outputs (PREFIX.csv,
 // PREFIX.seeds.csv, PREFIX.json) and stderr lines (CANDIDATE, best, progress, done) as the reference.
-// Every stage runs on the GPU (gate, tables, scan, hot zones, proxy and exact floods); gpu/gscan keys
-// are accepted for compatibility. gate=1 runs the GPU seed gate over [START, START+COUNT) (COUNT 0 =
+// Every stage runs on the GPU (gate, tables, scan, hot zones, proxy and exact floods); the gpu and gscan
+// keys are accepted for compatibility. gate=1 runs the GPU seed gate over [START, START+COUNT) (COUNT 0 =
 // unbounded); stdin=1 reads gated seeds ("seed [amp]" lines) from stdin; otherwise the CPU-default gate
 // semantics (phase <= `phase`, amplitude_sub >= amp - amp_margin, maxta + maxtb >= amp) apply.
+// gscancheck=N and gpucheck=N (with gpu=1) compare the first N seeds' tables and scan, and the first N big
+// hot zones' proxy predicates, with the reference's CPU arithmetic and print the reference's lines.
+// SIGINT / SIGTERM stop the search as the reference's handler does: seeds in flight are finished and
+// written, then the JSON is written (dsearch_desert.cpp:40-41, 586-598).
 #include "../../include/dsx.h"
 #include "hwq.h"
 #include <chrono>
@@ -17,7 +21,10 @@
 #include <vector>
 
 static volatile sig_atomic_t interrupted = 0;
-static void on_signal(int) { interrupted = 1; }
+static void on_signal(int) {
+    interrupted = 1;
+    dsx_interrupt(1);
+}
 
 int main(int argc, char **argv) {
     dsx_set_hw_queues();
@@ -102,9 +109,14 @@ int main(int argc, char **argv) {
         const auto now = std::chrono::steady_clock::now();
         if (!force && std::chrono::duration<double>(now - last).count() <= 30) return;
         last = now;
-        std::fprintf(stderr, "%llu seeds, %llu passed, %llu scanned, %llu sites; %.2f passed/s\n",
+        // dsearch_desert.cpp:552-554; core-s are the pipeline workers' seconds per stage (dsx_search_counts)
+        std::fprintf(stderr,
+                     "%llu seeds, %llu passed, %llu scanned, %llu sites; %.2f passed/s; core-s tables %.0f scan %.0f "
+                     "sites %.0f (proxy %.0f hot %.0f gpu %.0f); hot zones %llu, big %llu\n",
                      (unsigned long long)C.done, (unsigned long long)C.passed, (unsigned long long)C.scanned,
-                     (unsigned long long)C.sites, C.passed / std::chrono::duration<double>(now - begin).count());
+                     (unsigned long long)C.sites, C.passed / std::chrono::duration<double>(now - begin).count(),
+                     C.sec_tables, C.sec_scan, C.sec_sites, C.sec_proxy, C.sec_hot, C.sec_gpu,
+                     (unsigned long long)C.hot_zones, (unsigned long long)C.big_zones);
     };
     if (P.from_stdin && !P.gate) {
         char line[256];
@@ -131,6 +143,7 @@ int main(int argc, char **argv) {
             progress(false);
         }
     }
+    if (rc == DSX_E_INTERRUPTED) rc = 0;   // the reference leaves with status 0 after an interrupt
     if (rc) std::fprintf(stderr, "dsx_search: %s %s\n", dsx_strerror(rc), dsx_last_error());
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - begin).count();
     FILE *m = std::fopen((prefix + ".json").c_str(), "w");
@@ -140,7 +153,7 @@ int main(int argc, char **argv) {
                  "\"cpu_tables\":%.3f,\"cpu_scan\":%.3f,\"cpu_sites\":%.3f}\n",
                  (unsigned long long)start, (unsigned long long)C.done, (unsigned long long)C.passed,
                  (unsigned long long)C.scanned, P.amp, (unsigned long long)C.sites, sec, threads, P.max_phase, P.sigma, P.wc,
-                 P.we, P.t, P.ta, P.max_sites, P.nms, 0.0, 0.0, 0.0);
+                 P.we, P.t, P.ta, P.max_sites, P.nms, C.sec_tables, C.sec_scan, C.sec_sites);
     std::fclose(m);
     std::fprintf(stderr, "done: %llu seeds (%llu passed) in %.1fs\n", (unsigned long long)C.done,
                  (unsigned long long)C.passed, sec);

@@ -28,7 +28,7 @@ int run_group(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g);
 struct GroupText {
     std::string csv, seeds, log, zones;
     std::vector<std::string> verified;   // verified CANDIDATE lines (mcverify=1), without time and run
-    uint64_t scanned = 0, sites = 0;
+    uint64_t scanned = 0, sites = 0, hot_zones = 0, big_zones = 0;
 };
 void format_group(const Group &g, const dsx_search_params &P, GroupText &t, double *best);
 

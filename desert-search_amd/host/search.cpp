@@ -11,7 +11,9 @@
 #include "../../include/dsx.h"
 #include "../csrc/dsx_rng.h"
 #include "../data/mc_tables.inc"
+#include "cpu_ref.h"
 #include "pipeline.h"
+#include <atomic>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -41,6 +43,13 @@ int dsx_ctx_device(const dsx_ctx *c);
 void dsx_ctx_free_workers(dsx_ctx *c);
 
 namespace dsxh {
+namespace cpu = dsx::cpu;
+
+// gscancheck=N (dsearch_desert.cpp:228-242, 289-296): the first N seeds this process scans also get the
+// reference's CPU tables and scan (cpu_ref.h); the line compares the CPU and GPU maxima and the buckets'
+// best scores. gpucheck=N (:445-450): the first N big hot zones compare the GPU proxy predicate with the
+// CPU DesertProxy at every 7th cell of their step-32 grids.
+static std::atomic<int64_t> g_gscan_checked{0}, g_gpu_checked{0};
 
 struct Site { float s, t, c, e; int x, z; };
 struct Eval {
@@ -182,6 +191,7 @@ struct Prof {
 };
 Prof g_prof;
 std::mutex g_prof_mu;
+std::atomic<int> g_interrupt{0};   // dsx_interrupt
 void prof_add(int k, double t, uint64_t n) {
     std::lock_guard<std::mutex> l(g_prof_mu);
     g_prof.t[k] += t;
@@ -392,6 +402,7 @@ struct SeedOut {
     std::vector<double> best_proxy;
     std::vector<int> best_step;
     std::vector<std::string> zones;   // PREFIX.zones.csv rows (study=1)
+    uint64_t nhot = 0, nbig = 0;      // hot-zone floods and big zones (dsearch_desert.cpp:348, 369)
     bool done = false;
 };
 
@@ -697,6 +708,7 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
                 const FloodOut &h2 = pre.out[pre_idx[hp]];
                 if (h2.a.cells * 256.0 * 256 * 16 / 1e6 < P.hot_pre * P.hot_min) {
                     hotcovered.add(h2, 7, 2);
+                    O.nhot++;
                     continue;
                 }
             }
@@ -704,7 +716,31 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
             const FloodOut &hzn = h128.out[z];
             hot_area = hzn.a.cells * 128.0 * 128 * 16 / 1e6;
             hotcovered.add(hzn, 7, 1);
+            O.nhot++;
             if (hot_area < P.hot_min) continue;
+            O.nbig++;
+            if (P.gpucheck > 0 && g_gpu_checked.fetch_add(1) < P.gpucheck) {
+                // the zone's step-32 grid as dsearch_desert.cpp:367-371 forms it; every 7th cell (:447)
+                const int x0 = int(std::floor((hzn.a.minx - 128) / 32.0)) * 32;
+                const int z0 = int(std::floor((hzn.a.minz - 128) / 32.0)) * 32;
+                const int gnx = (hzn.a.maxx + 128 - x0) / 32 + 1, gnz = (hzn.a.maxz + 128 - z0) / 32 + 1;
+                PointsReq q{&S, seed, DSX_PRED_PROXY, pcut, {}, {}, {}, {}, {}};
+                for (long c = 0; c < long(gnx) * gnz; c += 7) {
+                    q.xs.push_back(x0 + int(c % gnx) * 32);
+                    q.zs.push_back(z0 + int(c / gnx) * 32);
+                }
+                tl.reset();
+                co_await q;
+                tl.emplace(18);
+                auto N = std::make_unique<cpu::Noise[]>(3);
+                cpu::noise_init(seed, DSX_CH_T, N[0]);
+                cpu::noise_init(seed, DSX_CH_C, N[1]);
+                cpu::noise_init(seed, DSX_CH_E, N[2]);
+                long diff = 0;
+                for (size_t k = 0; k < q.xs.size(); k++)
+                    diff += (cpu::proxy_pred(N[0], N[1], N[2], DSX_DPROB, pcut, q.xs[k], q.zs[k]) == 2) != (q.v8[k] == 2);
+                std::fprintf(stderr, "gpucheck zone %dx%d: %ld/%zu cells differ\n", gnx, gnz, diff, q.xs.size());
+            }
             starts = zone_starts[z];
             start_px.assign(pxq.v8.begin() + px_off[z], pxq.v8.begin() + px_off[z + 1]);
         } else {
@@ -861,6 +897,28 @@ void make_sites(const dsx_raw_cand *raw, size_t nraw, const dsx_search_params &P
     }
 }
 
+static void gscan_check(uint64_t seed, const dsx_search_params &P, float gmaxta, float gmaxtb, const dsx_raw_cand *raw,
+                        size_t nraw) {
+    const cpu::ScanResult R = cpu::scan_seed(seed, P.sigma, P.sigmac, float(P.ta), float(P.t), float(P.wc), float(P.we),
+                                             P.nms);
+    std::vector<cpu::Cand> gc;
+    std::unordered_map<uint64_t, uint32_t> gb;
+    for (size_t i = 0; i < nraw; i++) {
+        const dsx_raw_cand &r = raw[i];
+        auto [it, fresh] = gb.try_emplace(cpu::bucket_key(r.x, r.z, P.nms), uint32_t(gc.size()));
+        if (fresh) gc.push_back({r.s, r.t, r.c, r.e, r.x, r.z});
+        else if (r.s > gc[it->second].s) gc[it->second] = {r.s, r.t, r.c, r.e, r.x, r.z};
+    }
+    long miss = 0;
+    for (const cpu::Cand &c : R.cand) {
+        auto it = gb.find(cpu::bucket_key(c.x, c.z, P.nms));
+        if (it == gb.end() || gc[it->second].s != c.s) miss++;
+    }
+    std::fprintf(stderr,
+                 "gscancheck seed %lld: maxta %.6f/%.6f maxtb %.6f/%.6f, buckets cpu %zu gpu %zu, mismatched %ld, raw %zu\n",
+                 (long long)seed, R.maxta, gmaxta, R.maxtb, gmaxtb, R.cand.size(), gc.size(), miss, nraw);
+}
+
 // ------------------------------------------------------------------ groups of seeds on worker contexts
 Group::Group() = default;
 Group::~Group() = default;
@@ -888,6 +946,8 @@ int group_scan(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g) {
         dsx::temperature_phase(O->seed, O->fy);   // fyA, fyB (dsearch_desert.cpp:80-94)
         O->maxta = mta[k];
         O->maxtb = mtb[k];
+        if (P.gscancheck > 0 && g_gscan_checked.fetch_add(1) < P.gscancheck)
+            gscan_check(O->seed, P, mta[k], mtb[k], raw.data() + k * cap, ncand[k]);
         if (!(mode == DSX_SEARCH_ENUM && O->maxta + O->maxtb < P.amp)) {
             O->scanned = true;
             Timer tm(8);
@@ -954,6 +1014,8 @@ void format_group(const Group &g, const dsx_search_params &P, GroupText &t, doub
         }
         t.scanned++;
         t.sites += O->sites.size();
+        t.hot_zones += O->nhot;
+        t.big_zones += O->nbig;
         for (auto &l : O->lines) t.log += l + "\n";
         for (size_t k = 0; k < O->bests.size(); k++)
             if (best && O->bests[k] > *best) {
@@ -1129,7 +1191,11 @@ class Pool {
                 if (q_.empty() && drained()) return;   // closing, nothing left
                 continue;
             }
-            S.step();
+            {
+                const auto ts = std::chrono::steady_clock::now();   // per-seed site stages (hot, proxy, exact)
+                S.step();
+                prof_add(2, std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count(), 0);
+            }
             if (S.err) {   // a device call failed: every group in flight fails with it
                 const std::string err = dsx_last_error();
                 for (auto &a : act) {
@@ -1186,10 +1252,21 @@ void dsx_search_defaults(dsx_search_params *p) {
     *p = d;
 }
 
+int dsx_interrupt(int on) {
+    g_interrupt.store(on ? 1 : 0, std::memory_order_relaxed);
+    return DSX_OK;
+}
+
 int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t start, uint64_t count, const uint64_t *seeds,
                uint64_t nseeds, const char *prefix, dsx_search_counts *counts) {
     if (!ctx || !pp || !prefix || !counts || mode < 0 || mode > 2 || (mode == DSX_SEARCH_LIST && nseeds && !seeds))
         return DSX_E_ARG;
+    if (g_interrupt.load(std::memory_order_relaxed)) return DSX_E_INTERRUPTED;
+    double prof0[20];
+    {
+        std::lock_guard<std::mutex> l(g_prof_mu);
+        std::copy(g_prof.t, g_prof.t + 20, prof0);
+    }
     const dsx_search_params P = *pp;
     if (P.nms <= 0 || P.max_sites < 0 || P.workers < 0 || P.workers > 64) return DSX_E_ARG;
     const int nw = P.workers ? P.workers : 16;
@@ -1250,6 +1327,12 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
                     }
                     counts->scanned += t.scanned;
                     counts->sites += t.sites;
+                    counts->hot_zones += t.hot_zones;
+                    counts->big_zones += t.big_zones;
+                    if (mode != DSX_SEARCH_ENUM) {   // gated seeds count as they are written
+                        counts->done += g->seeds.size();
+                        counts->passed += g->seeds.size();
+                    }
                 }
                 pending.pop_front();
             }
@@ -1259,14 +1342,17 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
         const size_t group = size_t(std::clamp<uint64_t>(env_u64("DSX_GROUP", 64), 1, 4096));
         const uint64_t total = mode == DSX_SEARCH_LIST ? nseeds : count;
         std::vector<uint64_t> batch;
-        for (uint64_t pos = 0; pos < total && rc == DSX_OK;) {
+        bool stopped = false;
+        for (uint64_t pos = 0; pos < total && rc == DSX_OK && !stopped;) {
             const uint64_t m = std::min(CHUNK, total - pos);
             batch.clear();
             auto tgate = std::chrono::steady_clock::now();
+            if (g_interrupt.load(std::memory_order_relaxed)) {
+                stopped = true;
+                break;
+            }
             if (mode == DSX_SEARCH_LIST) {
                 batch.assign(seeds + pos, seeds + pos + m);
-                counts->done += m;
-                counts->passed += m;
             } else if (mode == DSX_SEARCH_GATE) {
                 dsx_gate_params gp{float(P.gate_phase), float(P.gate_amp), float(P.gate_pthr), 0.01f, 1.0f, 1};
                 uint64_t n = 0;
@@ -1279,8 +1365,6 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
                 }
                 if (r) { rc = r; first_err = dsx_last_error(); break; }
                 batch.resize(n);
-                counts->done += n;
-                counts->passed += n;
             } else {
                 uint64_t n = 0, nph = 0;
                 batch.resize(1 << 18);
@@ -1299,6 +1383,12 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
             pos += m;
             prof_add(0, std::chrono::duration<double>(std::chrono::steady_clock::now() - tgate).count(), 0);
             for (size_t b0 = 0; b0 < batch.size(); b0 += group) {
+                // stop between groups: those in flight finish and are written (the CPU-gate mode finishes its
+                // chunk, whose counts are already taken)
+                if (mode != DSX_SEARCH_ENUM && g_interrupt.load(std::memory_order_relaxed)) {
+                    stopped = true;
+                    break;
+                }
                 auto g = std::make_unique<Group>();
                 g->seeds.assign(batch.begin() + b0, batch.begin() + std::min(batch.size(), b0 + group));
                 pool.submit(g.get());
@@ -1307,11 +1397,27 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
             drain(size_t(8 * nw));   // bounded backlog; rows of finished groups go out early
         }
         drain(0);
+        if (rc == DSX_OK && stopped) {
+            rc = DSX_E_INTERRUPTED;
+            first_err = "interrupted";
+        }
     }
     std::fclose(out);
     std::fclose(sout);
     if (vout) std::fclose(vout);
     if (zout) std::fclose(zout);
+    {
+        std::lock_guard<std::mutex> l(g_prof_mu);
+        const double *t = g_prof.t;
+        // Timer categories: 1 tables + scan stage, 8 make_sites, 2 per-seed site stages, 3 point batches,
+        // 4 flood batches, 16 hot-zone starts, 17 proxy-flood rounds
+        counts->sec_tables += (t[1] - prof0[1]) - (t[8] - prof0[8]);
+        counts->sec_scan += t[8] - prof0[8];
+        counts->sec_sites += t[2] - prof0[2];
+        counts->sec_proxy += t[17] - prof0[17];
+        counts->sec_hot += t[16] - prof0[16];
+        counts->sec_gpu += (t[3] - prof0[3]) + (t[4] - prof0[4]);
+    }
     if (rc) dsx::g_last_error = first_err;
     return rc;
 }

@@ -5,11 +5,14 @@
 // `dsearch_desert ... stdin=1`) for the seeds passing the gate, in seed-index order.
 //   calib=N  "seed pre full phase" for the first N (by seed value) phase-passed seeds of the first 2^26
 //            indices (tgate.cu:53-85; the pre-gate threshold calibration), sorted by seed
-//   check=1  the first 4 passing seeds are re-evaluated from an explicit seed list (dsx_gate_amps), and
-//            both amplitudes are printed to stderr
+//   check=1  for the first 4 passing seeds, the GPU amplitude beside the full amplitude in the CPU
+//            arithmetic of tgate.cu:14-39 (cpu_ref.h gate_amp), on stderr (tgate.cu:87-91); the two use
+//            different arithmetic (float corner-term tables on the GPU, double Perlin then float on the
+//            CPU), so they agree to a few 1e-6, not bit for bit
 #include "../../include/dsx.h"
 #include "hwq.h"
 #include "../csrc/dsx_rng.h"
+#include "cpu_ref.h"
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -102,10 +105,8 @@ int main(int argc, char **argv) {
         for (uint64_t i = 0; i < n; i++) {
             std::printf("%llu %.5f\n", (unsigned long long)seeds[i], amps[i]);
             if (check && checked < 4) {
-                float pre = 0, full = 0;
-                if ((rc = dsx_gate_amps(ctx, &seeds[i], 1, &g, &pre, &full))) return fail("dsx_gate_amps", rc);
-                std::fprintf(stderr, "  check seed %llu: gate amp %.5f explicit-list amp %.5f (pre-gate %.5f)\n",
-                             (unsigned long long)seeds[i], amps[i], full, pre);
+                std::fprintf(stderr, "  check seed %llu: gpu amp %.5f cpu amp %.5f\n", (unsigned long long)seeds[i],
+                             amps[i], dsx::cpu::gate_amp(seeds[i], 512, 7, 2.0f));
                 checked++;
             }
         }

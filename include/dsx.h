@@ -29,6 +29,7 @@ enum {
     DSX_E_CAPACITY = 3,  /* caller buffer or internal queue too small; *n_out holds the true count */
     DSX_E_NODEVICE = 4,  /* no usable gfx950 device */
     DSX_E_BUSY = 5,      /* the request has not finished yet (dsx_poll) */
+    DSX_E_INTERRUPTED = 6, /* dsx_search stopped early after dsx_interrupt(1); its counts cover the seeds it wrote */
 };
 const char *dsx_strerror(int code);
 /* Text of the last HIP failure on this thread (empty if none). */
@@ -285,6 +286,13 @@ enum {
 };
 typedef struct {
     uint64_t done, passed, scanned, sites;
+    uint64_t hot_zones, big_zones; /* hot-zone floods run and those reaching hot_min (dsearch_desert.cpp:348, 369) */
+    /* worker-seconds of the pipeline workers during the call, per stage (the reference's per-stage core-seconds,
+     * dsearch_desert.cpp:552-554, 590-593, for this pipeline's stages; process-wide if searches overlap):
+     * tables = tables, blur and scan on the device; scan = the host's bucket reduction, sort and NMS; sites =
+     * hot zones, proxy and exact floods per seed; of these, proxy = proxy-flood rounds, hot = hot-zone starts,
+     * gpu = waiting for device batches (points and floods) */
+    double sec_tables, sec_scan, sec_sites, sec_proxy, sec_hot, sec_gpu;
 } dsx_search_counts;
 
 /* The per-seed pipeline of dsearch_desert.cpp:173-557 (gate -> tables/scan -> sites -> hot zones ->
@@ -297,6 +305,13 @@ typedef struct {
  * Rows of one seed are contiguous; seeds appear in batch order. */
 int dsx_search(dsx_ctx *ctx, const dsx_search_params *p, int mode, uint64_t start, uint64_t count, const uint64_t *seeds,
                uint64_t nseeds, const char *prefix, dsx_search_counts *counts);
+
+/* Stop (on = 1) or allow (on = 0) the searches of this process, as the reference's SIGINT / SIGTERM handler
+ * does (dsearch_desert.cpp:40-41, 151, 586-598): running dsx_search calls start no new seeds, finish and write
+ * the seeds already in flight, and return DSX_E_INTERRUPTED with counts of what they wrote; later calls return
+ * it at once until dsx_interrupt(0). Async-signal-safe (one atomic store): call it from a signal handler. In
+ * the CPU-gate mode (DSX_SEARCH_ENUM) the seed-index chunk in flight (2^22 indices) is finished. */
+int dsx_interrupt(int on);
 
 /* ---------------------------------------------------------------- area queries (tools/dflood.cpp) */
 typedef struct {

@@ -97,8 +97,9 @@ def test_gate_capacity_and_argument_errors(engine):
 
 def test_tgate_tool(engine, oracle):
     """The tgate command line (tgate.cu:1-7): "seed amp" lines equal to dsx_gate's survivors, check=1
-    re-evaluates from an explicit list, calib=N gives "seed pre full phase" of the first N phase
-    survivors (by seed value) of 2^26 indices."""
+    prints the GPU amplitude beside tgate.cu:14-39's CPU amplitude for the first 4 (different arithmetic:
+    equal to a few 1e-6), calib=N gives "seed pre full phase" of the first N phase survivors (by seed
+    value) of 2^26 indices."""
     import ctypes
     import os
     import subprocess
@@ -114,8 +115,9 @@ def test_tgate_tool(engine, oracle):
     checks = [l for l in r.stderr.splitlines() if "check seed" in l]
     assert len(checks) == min(4, len(seeds))
     for l in checks:
-        w = l.split()
-        assert w[5] == w[8]   # gate amp == explicit-list amp
+        w = l.split()   # "check seed S: gpu amp A cpu amp B"
+        assert w[3:5] == ["gpu", "amp"] and w[6:8] == ["cpu", "amp"]
+        assert abs(float(w[5]) - float(w[8])) <= 2e-5, l
     assert "%d seed indices, %d phase, %d pre-gate, %d gated" % ((count,) + tuple(cnt)) in r.stderr
     n = 12
     r = subprocess.run([exe, "5", "1", "calib=%d" % n, "phase=0.10"], capture_output=True, text=True, timeout=300)

@@ -138,6 +138,65 @@ def test_cli_production_candidates(engine):
         _check("fx", pre, (js["done"], js["passed"], js["scanned"], js["sites"]), stderr=r.stderr)
 
 
+def test_cli_validation_switches(engine):
+    """gscancheck=N and gpucheck=N (dsearch_desert.cpp:228-242, 289-296, 445-450): the first seeds' tables and
+    scan recomputed on the CPU agree with the GPU (maxima, every bucket's best score), and the CPU
+    DesertProxy agrees with the GPU proxy at every 7th cell of the first big hot zones' grids; the outputs
+    are unchanged."""
+    import re
+    exe = os.path.join(REPO, "desert-search_amd", "dsearch_desert")
+    lines = open(os.path.join(GOLDEN, "gated_1e12_1m.txt")).read()
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "chk")
+        r = subprocess.run([exe, "0", "0", pre, "8", "stdin=1", "gpu=1", "gscan=1", "gscancheck=3", "gpucheck=2"],
+                           input=lines, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        js = json.load(open(pre + ".json"))
+        _check("rg1m", pre, (js["done"], js["passed"], js["scanned"], js["sites"]))
+    gs = re.findall(r"gscancheck seed (-?\d+): maxta ([\d.-]+)/([\d.-]+) maxtb ([\d.-]+)/([\d.-]+), buckets cpu (\d+) "
+                    r"gpu (\d+), mismatched (\d+), raw (\d+)", r.stderr)
+    assert len(gs) == 3, r.stderr[-2000:]
+    for seed, a1, a2, b1, b2, nc, ng, miss, raw in gs:
+        assert a1 == a2 and b1 == b2 and nc == ng and miss == "0" and int(nc) > 0, (seed, a1, a2, b1, b2, nc, ng, miss)
+    gp = re.findall(r"gpucheck zone (\d+)x(\d+): (\d+)/(\d+) cells differ", r.stderr)
+    assert len(gp) == 2
+    for nx, nz, diff, tot in gp:
+        assert diff == "0" and int(tot) > 100
+    bad = subprocess.run([exe, "0", "10", os.path.join(td, "x"), "1", "gscancheck=1"], capture_output=True, text=True)
+    assert bad.returncode == 2 and "gscan=1 needs gpu=1" in bad.stderr
+
+
+def test_cli_interrupt(engine):
+    """SIGINT mid-run (dsearch_desert.cpp:40-41, 586-598): the seeds in flight are finished and written, the JSON
+    is written and the exit status is 0; every row is whole and belongs to a seed of the seeds CSV, and the
+    counters equal the rows written (gate=1: done = passed = scanned = gated seeds processed)."""
+    import signal
+    import time
+    exe = os.path.join(REPO, "desert-search_amd", "dsearch_desert")
+    total = 1 << 34
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "int")
+        p = subprocess.Popen([exe, str(10 ** 12), str(total), pre, "8", "gpu=1", "gate=1"], stderr=subprocess.PIPE,
+                             stdout=subprocess.DEVNULL, text=True)
+        t0 = time.time()
+        while time.time() - t0 < 120 and p.poll() is None:
+            if os.path.exists(pre + ".seeds.csv") and os.path.getsize(pre + ".seeds.csv") > 4096:
+                break
+            time.sleep(0.2)
+        p.send_signal(signal.SIGINT)
+        err = p.communicate(timeout=120)[1]
+        assert p.returncode == 0, err[-2000:]
+        js = json.load(open(pre + ".json"))
+        srows = open(pre + ".seeds.csv").read().splitlines()[1:]
+        rows = open(pre + ".csv").read().splitlines()[1:]
+    assert 0 < js["done"] == js["passed"] == js["scanned"] == len(srows) < total // 20000
+    assert all(len(r.split(",")) == 8 for r in srows)
+    seeds = {r.split(",")[0] for r in srows}
+    assert all(len(r.split(",")) == 17 and r.split(",")[0] in seeds for r in rows)
+    assert ("done: %d seeds (%d passed)" % (js["done"], js["passed"])) in err
+    assert js["cpu_tables"] > 0 and js["cpu_sites"] > 0
+
+
 def test_3m_pipeline(engine):
     dsx = load_package().dsx
     with tempfile.TemporaryDirectory() as td:
