@@ -30,7 +30,8 @@ struct Tile {
     uint32_t qstate;        // 0 idle, 1 queued or being processed, 2 being processed and touched again
     uint32_t slot1;         // hash slot + 1 of this tile's entry (0: not in the table)
     uint32_t evald;         // 8x8 sub-blocks evaluated so far (bit bz*4 + bx)
-    uint32_t pad_[2];
+    uint32_t hint;          // sub-blocks to evaluate at the tile's first visit, touched or not (a flood hint)
+    uint32_t pad_;
     uint32_t pass[FT];      // predicate != 0 (inside window and world)
     uint32_t cnt[FT];       // predicate == 2
     uint32_t reach[FT];     // cells the flood has entered (pushed)
@@ -75,12 +76,19 @@ constexpr uint32_t Q_EMPTY = 0xffffffffu;
 // FLOOD_OVF bits: 1 work queue full, 2 tile pool or hash table full, 4 time limit reached
 constexpr uint32_t OVF_QUEUE = 1, OVF_TILES = 2, OVF_TIME = 4;
 
+// Evaluation hint of a tiled flood: the 8x8 sub-blocks `blocks` (bit bz*4 + bx) of the tile (tx, tz) of
+// job `job` are evaluated when the tile is first visited, whether or not the flood has touched them yet.
+struct FloodHint {
+    uint32_t job;
+    int32_t tx, tz;
+    uint32_t blocks;
+};
 // First kernel of a batch of n floods, reading the host's pinned staging directly: jobs and job lists
-// (layout: n jobs, n small-window indices, then the tiled engine's indices) to the device, every
-// result initialised (an empty area at the centre), and the start tile of each of the n_big tiled
-// floods touched and queued.
-void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big, uint8_t *d_up,
-                        FloodRes *res, FloodBufs B);
+// (layout: n jobs, n small-window indices, n tiled-engine indices, then nh hints) to the device, every
+// result initialised (an empty area at the centre), the start tile of each of the n_big tiled floods
+// touched and queued, and every hinted tile allocated, given its hint and queued.
+void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big, uint32_t nh,
+                        uint8_t *d_up, FloodRes *res, FloodBufs B);
 // The persistent flood kernel: `blocks` workgroups take queued tiles until no work is pending. A tile
 // is evaluated lazily, by 8x8 sub-blocks the flood touches; every processing grows its reached set to
 // a fixpoint and queues the neighbours it touched. ev0/ev1 (optional): HIP events around it.

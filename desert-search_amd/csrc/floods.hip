@@ -43,7 +43,7 @@
 namespace dsx {
 
 __global__ __launch_bounds__(256) void k_flood_stage(const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big,
-                                                     uint8_t *d_up, FloodRes *res, FloodBufs B) {
+                                                     uint32_t nh, uint8_t *d_up, FloodRes *res, FloodBufs B) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     const FloodJob *hj = reinterpret_cast<const FloodJob *>(h_up);
     FloodJob *dj = reinterpret_cast<FloodJob *>(d_up);
@@ -62,6 +62,21 @@ __global__ __launch_bounds__(256) void k_flood_stage(const uint8_t *h_up, uint32
         const uint32_t k = hl[n + q];
         dl[n + q] = k;
         flood_start_tile(hj[k], k, B);
+    }
+    if (q < nh) {   // hinted tiles: allocated, hinted and queued; their jobs are tiled jobs of this batch
+        const FloodHint hnt = reinterpret_cast<const FloodHint *>(hl + 2 * n)[q];
+        const FloodJob J = hj[hnt.job];
+        const int nta = int(ntile_axis(J));
+        if (hnt.blocks && hnt.tx >= 0 && hnt.tz >= 0 && hnt.tx < nta && hnt.tz < nta) {
+            bool fresh = false;
+            const uint32_t ti = tile_get(B, hnt.job, hnt.tx, hnt.tz, &fresh);
+            if (ti == 0xffffffffu) {
+                atomicOr(&B.ctl[FLOOD_OVF], OVF_TILES);
+            } else {
+                atomicOr(&B.tiles[ti].hint, hnt.blocks & 0xffffu);
+                tile_touch(B, ti);
+            }
+        }
     }
     if (q == 0) aset(&B.ctl[STAGE_SEQ], B.seq + 1);
 }
@@ -110,7 +125,8 @@ __device__ __forceinline__ void eval_subblock(int sb, int tx, int tz, const Floo
 // the sub-blocks that hold touched cells not yet evaluated (reached cells' neighbours, and cells
 // touched from neighbouring tiles). When none are left (0), the reached set is final: it is stored
 // and its new frontier is handed to the neighbouring tiles.
-__device__ uint32_t prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J, const uint32_t *s_pass, uint32_t evald) {
+__device__ uint32_t prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J, const uint32_t *s_pass, uint32_t evald,
+                              uint32_t hint) {
     Tile &T = B.tiles[ti];
     const int lane = threadIdx.x & 63;
     const int r = lane & 31;
@@ -143,7 +159,7 @@ __device__ uint32_t prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J
         for (int bz = 0; bz < NSB; bz++)
             if ((rows >> (bz * SB)) & 0xffu) need |= 1u << (bz * NSB + bx);
     }
-    need &= ~evald;
+    need = (need | hint) & ~evald;
     if (need) return need;
     const uint32_t NR = R & ~old;
     if (lane < FT && NR) ast(&T.reach[r], R);
@@ -239,7 +255,7 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
     // the octaves the predicates read: the proxy's 14 (ClimP, 5 KB of LDS) in the light instance
     __shared__ std::conditional_t<LIGHT != 0, ClimP, ClimSeed> C;
     __shared__ uint32_t s_pass[FT], s_cnt[FT], s_hot[FT];
-    __shared__ uint32_t s_item, s_again, s_need, s_evald;
+    __shared__ uint32_t s_item, s_again, s_need, s_evald, s_hint;
     __shared__ int s_stop;
     const int tid = threadIdx.x, wave = tid >> 6;
     int loaded_seed = -1, loaded_cls = -1;
@@ -302,6 +318,7 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
             atomicExch(&T.qstate, 1u);   // in process: touches from now on ask for another pass
             atomicAdd(&B.ctl[Q_ITEMS], 1u);
             s_evald = ald(&T.evald);
+            s_hint = ald(&T.hint);
         }
         if (tid < FT) {
             s_pass[tid] = ald(&T.pass[tid]);
@@ -316,7 +333,7 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
             // ---- evaluate the touched sub-blocks until the reached set is final, then propagate
             for (;;) {
                 if (tid < 64) {
-                    const uint32_t need = prop_tile(B, ti, J, s_pass, s_evald);
+                    const uint32_t need = prop_tile(B, ti, J, s_pass, s_evald, s_hint);
                     if (tid == 0) s_need = need;
                 }
                 __syncthreads();
@@ -513,6 +530,7 @@ __device__ void acc_tile(const FloodJob *jobs, FloodBufs B, uint32_t b, FloodRes
         ast(&T.slot1, 0u);
         aset(&T.qstate, 0u);
         ast(&T.evald, 0u);
+        aset(&T.hint, 0u);
     }
 }
 
@@ -542,9 +560,10 @@ __global__ __launch_bounds__(256) void k_flood_publish(const FloodRes *res, uint
 
 // ------------------------------------------------------------------ launchers
 
-void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big, uint8_t *d_up,
-                        FloodRes *res, FloodBufs B) {
-    if (n) hipLaunchKernelGGL(k_flood_stage, dim3((n + 255) / 256), dim3(256), 0, st, h_up, n, n_small, n_big, d_up, res, B);
+void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big, uint32_t nh,
+                        uint8_t *d_up, FloodRes *res, FloodBufs B) {
+    const uint32_t m = std::max(n, nh);
+    if (m) hipLaunchKernelGGL(k_flood_stage, dim3((m + 255) / 256), dim3(256), 0, st, h_up, n, n_small, n_big, nh, d_up, res, B);
 }
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
                       uint32_t blocks, int pre, int light, hipEvent_t ev0, hipEvent_t ev1, int tprof) {

@@ -168,8 +168,8 @@ static int clim_alloc(dsx_ctx *c, ClimState *&S) {
 // sizes hold a pipeline worker's batches (65,536 floods and masks, ~14 MB pinned) and buffers at least
 // double when they grow: hipFree / hipHostFree wait for the whole device, so every reallocation stalls
 // the calling worker behind all other streams' work.
-static int flood_io_reserve(ClimState *S, uint64_t n) {
-    const size_t up = size_t(n) * (sizeof(dsx::FloodJob) + 8);
+static int flood_io_reserve(ClimState *S, uint64_t n, uint64_t nh) {
+    const size_t up = size_t(n) * (sizeof(dsx::FloodJob) + 8) + size_t(nh) * sizeof(dsx::FloodHint);
     if (S->h_up_cap < up) {
         if (S->h_up) HIPCK(hipHostFree(S->h_up));
         S->h_up = nullptr;
@@ -392,7 +392,14 @@ int dsx_predicates(dsx_ctx *c, uint64_t seed, int kind, float cut, const int32_t
 }
 
 int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) {
-    if (!c || (n && (!jobs || !out))) return DSX_E_ARG;
+    return dsx_flood_hinted(c, jobs, n, nullptr, 0, out);
+}
+
+int dsx_flood_hinted(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, const dsx_flood_hint *hints, uint64_t nh,
+                     dsx_area *out) {
+    if (!c || (n && (!jobs || !out)) || (nh && !hints) || nh > (1u << 24)) return DSX_E_ARG;
+    for (uint64_t k = 0; k < nh; k++)
+        if (hints[k].job >= n) return DSX_E_ARG;
     for (uint64_t k = 0; k < n; k++) {
         const dsx_flood_job &j = jobs[k];
         if (j.kind < 0 || j.kind > 3 || j.step < 1 || j.half < 0 || 2 * int64_t(j.half) + 1 > DSX_FLOOD_MAX_WIDTH ||
@@ -413,7 +420,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         dsx::g_last_error = "too many floods in one call";
         return DSX_E_CAPACITY;
     }
-    if ((rc = flood_io_reserve(S, n))) return rc;
+    if ((rc = flood_io_reserve(S, n, nh))) return rc;
     // jobs (seed -> climate slot) and job lists straight into pinned staging, in the device layout
     dsx::FloodJob *hj = reinterpret_cast<dsx::FloodJob *>(S->h_up);
     uint32_t *h_small = reinterpret_cast<uint32_t *>(hj + n), *h_big = h_small + n;
@@ -462,6 +469,12 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
             h_big[n_big++] = uint32_t(k);
             light = light && (hj[k].kind == dsx::PRED_HOT || hj[k].kind == dsx::PRED_PROXY);
         }
+    // hints of the tiled engine's floods, after the job lists (the small-window engine takes no hints)
+    dsx::FloodHint *h_hint = reinterpret_cast<dsx::FloodHint *>(h_small + 2 * n);
+    uint32_t n_hint = 0;
+    for (uint64_t k = 0; k < nh; k++)
+        if (cls_of(hj[hints[k].job]) < 0 && (hints[k].blocks & 0xffffu))
+            h_hint[n_hint++] = dsx::FloodHint{hints[k].job, hints[k].tx, hints[k].tz, hints[k].blocks & 0xffffu};
     uint8_t *d_up = S->d_up;
     const dsx::FloodJob *d_jobs = reinterpret_cast<const dsx::FloodJob *>(d_up);
     uint32_t *d_small = reinterpret_cast<uint32_t *>(d_up + n * sizeof(dsx::FloodJob));
@@ -482,7 +495,7 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
         B.ntiles = B.ctl + dsx::CTL_NTILES;
         B.seq = seq;
         // jobs and lists read from the pinned staging by the first kernel (no separate copy)
-        dsx::launch_flood_stage(c->st, S->h_up, uint32_t(n), n_small, n_big, d_up, S->d_res, B);
+        dsx::launch_flood_stage(c->st, S->h_up, uint32_t(n), n_small, n_big, n_hint, d_up, S->d_res, B);
         uint32_t off = 0;
         for (int cls = 0; cls < dsx::LDS_NCLS; cls++) {
             dsx::launch_flood_lds(c->st, cls, d_jobs, d_small + off, n_cls[cls], S->d_clim, S->grid, S->d_res,
@@ -509,9 +522,12 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
                                   min_full = uint32_t(env_u32("DSX_FLOOD_MIN_BLOCKS_FULL", 16)),
                                   per_full = uint32_t(env_u32("DSX_FLOOD_BLOCKS_PER_EXACT", 4)),
                                   max_full = uint32_t(env_u32("DSX_FLOOD_MAX_BLOCKS_FULL", 128));
+            // hinted tiles are evaluated in bulk at the start: one more unit per `per_hint` of them
+            static const uint32_t per_hint = std::max(1u, uint32_t(env_u32("DSX_FLOOD_TILES_PER_UNIT", 8)));
+            const size_t want = (light ? size_t(per_job) : size_t(per_full)) * n_big + n_hint / per_hint;
             const uint32_t blocks =
-                light ? uint32_t(std::clamp<size_t>(size_t(per_job) * n_big, 16, max_blocks))
-                      : uint32_t(std::clamp<size_t>(size_t(per_full) * n_big, std::min(min_full, max_full), max_full));
+                light ? uint32_t(std::clamp<size_t>(want, 16, max_blocks))
+                      : uint32_t(std::clamp<size_t>(want, std::min(min_full, max_full), max_full));
             blocks_used = blocks;
             dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, S->prepass, light ? 1 : 0, e0, e1,
                                   prof ? 1 : 0);
@@ -559,16 +575,16 @@ int dsx_flood(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, dsx_area *out) 
     }
     // the masks stay in the pinned buffer the publish kernel wrote (dsx_flood_masks copies them out); only
     // those beyond its capacity are copied here
-    const uint32_t nh = std::min(nm, S->h_mask_cap);
-    S->masks.resize(nm - nh);
-    if (nm > nh) {   // the next call's staging holds this many pinned (twice the current capacity at least)
+    const uint32_t nmh = std::min(nm, S->h_mask_cap);
+    S->masks.resize(nm - nmh);
+    if (nm > nmh) {   // the next call's staging holds this many pinned (twice the current capacity at least)
         S->mask_want = std::min(S->mask_cap, std::max(nm, 2 * S->h_mask_cap));
-        HIPCK(hipMemcpyAsync(S->masks.data(), S->d_masks + nh, size_t(nm - nh) * sizeof(dsx::TileMask),
+        HIPCK(hipMemcpyAsync(S->masks.data(), S->d_masks + nmh, size_t(nm - nmh) * sizeof(dsx::TileMask),
                              hipMemcpyDeviceToHost, c->st));
         HIPCK(hipStreamSynchronize(c->st));
     }
     S->masks_n = nm;
-    S->masks_h = nh;
+    S->masks_h = nmh;
     dsx_flood_stats st{};
     st.calls = 1;
     st.rounds = h[dsx::Q_ITEMS];

@@ -63,6 +63,7 @@ PRED_DESERT_STATE, PRED_HOT, PRED_PROXY, PRED_DESERT = 0, 1, 2, 3
 POINT_T, POINT_PROXYPROB, POINT_HOTTEST, POINT_BIOME, POINT_PROXY_C, POINT_PROXY_E = 16, 17, 18, 19, 20, 21
 TILE_MASK = np.dtype([("job", "<u4"), ("tx", "<i4"), ("tz", "<i4"), ("rows", "<u4", (32,))])
 AREA_QUERY = np.dtype([("seed", "<u8"), ("x", "<i4"), ("z", "<i4"), ("half", "<i4"), ("pad", "<i4")])
+FLOOD_HINT = np.dtype([("job", "<u4"), ("tx", "<i4"), ("tz", "<i4"), ("blocks", "<u4")])
 
 
 class SearchParams(ctypes.Structure):
@@ -171,6 +172,8 @@ def load(path=LIB_PATH):
     lib.dsx_predicates.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
     lib.dsx_flood.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+    lib.dsx_flood_hinted.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
+                                     ctypes.c_void_p]
     lib.dsx_flood_masks.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, _u64p]
     lib.dsx_points.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
                                ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
@@ -328,6 +331,15 @@ class Engine:
         j = np.ascontiguousarray(jobs, dtype=FLOOD_JOB)
         out = np.zeros(len(j), AREA)
         self._check(self.lib.dsx_flood(self.h, j.ctypes.data, len(j), out.ctypes.data), "dsx_flood")
+        return out
+
+    def flood_hinted(self, jobs, hints):
+        """dsx_flood_hinted: jobs (FLOOD_JOB array) with evaluation hints (FLOOD_HINT array) -> AREA array"""
+        j = np.ascontiguousarray(jobs, dtype=FLOOD_JOB)
+        h = np.ascontiguousarray(hints, dtype=FLOOD_HINT)
+        out = np.zeros(len(j), AREA)
+        self._check(self.lib.dsx_flood_hinted(self.h, j.ctypes.data, len(j), h.ctypes.data if len(h) else None, len(h),
+                                              out.ctypes.data), "dsx_flood_hinted")
         return out
 
     def set_precision(self, mode):

@@ -16,6 +16,7 @@
 #include <atomic>
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <condition_variable>
 #include <coroutine>
@@ -52,12 +53,14 @@ namespace cpu = dsx::cpu;
 static std::atomic<int64_t> g_gscan_checked{0}, g_gpu_checked{0};
 
 struct Site { float s, t, c, e; int x, z; };
+struct FloodOut;
 struct Eval {
     size_t rank;
     int x, z;
     double proxy, s8 = -1, s4 = 0;
     bool edge = false;
     double hot = 0;
+    const FloodOut *px = nullptr;   // the proxy flood of this component (its counted cells hint the exact floods)
 };
 
 // ------------------------------------------------------------------ coroutine plumbing
@@ -152,6 +155,7 @@ struct FloodReq {
     bool want_cells;
     std::vector<FloodOut> out;
     std::coroutine_handle<> h;
+    std::vector<dsx_flood_hint> hints;   // evaluation hints (job = index into jobs)
     bool await_ready() const noexcept { return jobs.empty(); }
     void await_suspend(std::coroutine_handle<> hh);
     void await_resume() const noexcept {}
@@ -259,18 +263,21 @@ struct Sched {
 
     // One dsx_flood call over jobs [lo, hi) of the flattened list; split in halves on capacity errors.
     int flood_range(std::vector<dsx_flood_job> &jobs, std::vector<std::pair<FloodReq *, size_t>> &who, size_t lo,
-                    size_t hi, bool cells) {
+                    size_t hi, bool cells, const std::vector<dsx_flood_hint> &hints) {
         if (lo >= hi) return DSX_OK;
         std::vector<dsx_area> res(hi - lo);
+        std::vector<dsx_flood_hint> sub;   // the hints of jobs [lo, hi), re-based
+        for (const dsx_flood_hint &h : hints)
+            if (h.job >= lo && h.job < hi) sub.push_back({uint32_t(h.job - lo), h.tx, h.tz, h.blocks});
         int rc;
         {
             Timer tf(10);
-            rc = dsx_flood(ctx, jobs.data() + lo, hi - lo, res.data());
+            rc = dsx_flood_hinted(ctx, jobs.data() + lo, hi - lo, sub.data(), sub.size(), res.data());
         }
         if (rc == DSX_E_CAPACITY && hi - lo > 1) {
             const size_t mid = lo + (hi - lo) / 2;
-            if ((rc = flood_range(jobs, who, lo, mid, cells))) return rc;
-            return flood_range(jobs, who, mid, hi, cells);
+            if ((rc = flood_range(jobs, who, lo, mid, cells, hints))) return rc;
+            return flood_range(jobs, who, mid, hi, cells, hints);
         }
         if (rc) return rc;
         for (size_t k = lo; k < hi; k++) {
@@ -316,10 +323,13 @@ struct Sched {
     int run_floods(std::vector<FloodReq *> &rs) {
         std::vector<dsx_flood_job> jobs;
         std::vector<std::pair<FloodReq *, size_t>> who;
+        std::vector<dsx_flood_hint> hints;
         bool cells = false;
         for (auto *r : rs) {
             r->out.assign(r->jobs.size(), FloodOut{});
             cells |= r->want_cells;
+            for (const dsx_flood_hint &h : r->hints)
+                hints.push_back({uint32_t(jobs.size() + h.job), h.tx, h.tz, h.blocks});
             for (size_t k = 0; k < r->jobs.size(); k++) {
                 jobs.push_back(r->jobs[k]);
                 who.push_back({r, k});
@@ -337,7 +347,7 @@ struct Sched {
                 }
                 hi++;
             }
-            const int rc = flood_range(jobs, who, lo, hi, cells);
+            const int rc = flood_range(jobs, who, lo, hi, cells, hints);
             if (rc) return rc;
             lo = hi;
         }
@@ -510,6 +520,86 @@ void study_rows(const dsx_search_params &P, SeedOut &O, const std::vector<FloodO
 }
 
 // dsearch_desert.cpp:309-533 for one seed whose sites are known.
+// Hints from a grid of marked sub-blocks (b, c) in [b0, b0 + w) x [c0, c0 + h): one hint per 32 x 32 tile
+// (4 x 4 sub-blocks) holding a marked one, in tile order.
+static void emit_hints(const std::vector<uint8_t> &mark, int b0, int c0, int w, int h, uint32_t job,
+                       std::vector<dsx_flood_hint> &out) {
+    const int t0x = b0 / 4, t0z = c0 / 4, ntx = (b0 + w - 1) / 4 - t0x + 1, ntz = (c0 + h - 1) / 4 - t0z + 1;
+    std::vector<uint32_t> bits(size_t(ntx) * ntz, 0);
+    for (int cc = 0; cc < h; cc++)
+        for (int bb = 0; bb < w; bb++)
+            if (mark[size_t(cc) * w + bb]) {
+                const int b = b0 + bb, c = c0 + cc;
+                bits[size_t(c / 4 - t0z) * ntx + (b / 4 - t0x)] |= 1u << ((c % 4) * 4 + b % 4);
+            }
+    for (int tz = 0; tz < ntz; tz++)
+        for (int tx = 0; tx < ntx; tx++)
+            if (const uint32_t m = bits[size_t(tz) * ntx + tx]) out.push_back({job, t0x + tx, t0z + tz, m});
+}
+
+// Evaluation hints for a step-4 flood that refines the step-8 flood f8 from the same centre
+// (dsearch_desert.cpp:512-523): step-4 window cell (2i, 2j) is step-8 cell (i, j), so step-4 sub-block
+// (b, c) (8 x 8 cells) covers step-8 cells [4b, 4b + 4) x [4c, 4c + 4). Every sub-block holding a counted
+// step-8 cell, and its 8 neighbours, is hinted: the step-4 component lies within about one step-8 cell of
+// the step-8 component, and a sub-block the hints miss is still evaluated when the flood touches it.
+static void refine_hints(const FloodOut &f8, uint32_t job, int half4, std::vector<dsx_flood_hint> &out) {
+    if (f8.tiles.empty()) return;
+    int b0 = INT_MAX, b1 = INT_MIN, c0 = INT_MAX, c1 = INT_MIN;
+    for (const dsx_tile_mask &m : f8.tiles) {
+        b0 = std::min(b0, m.tx * 8);
+        b1 = std::max(b1, m.tx * 8 + 7);
+        c0 = std::min(c0, m.tz * 8);
+        c1 = std::max(c1, m.tz * 8 + 7);
+    }
+    const int nb = (2 * half4 + 1 + 7) / 8;   // sub-blocks per axis of the step-4 window
+    b0 = std::max(b0 - 1, 0), c0 = std::max(c0 - 1, 0);
+    b1 = std::min(b1 + 1, nb - 1), c1 = std::min(c1 + 1, nb - 1);
+    if (b0 > b1 || c0 > c1) return;
+    const int w = b1 - b0 + 1, h = c1 - c0 + 1;
+    std::vector<uint8_t> mark(size_t(w) * h, 0);
+    for (const dsx_tile_mask &m : f8.tiles)
+        for (int r = 0; r < 32; r++) {
+            const uint32_t bits = m.rows[r];
+            if (!bits) continue;
+            const int c = m.tz * 8 + r / 4;
+            for (int g = 0; g < 8; g++) {
+                if (!((bits >> (4 * g)) & 0xfu)) continue;
+                const int b = m.tx * 8 + g;
+                for (int dc = -1; dc <= 1; dc++)
+                    for (int db = -1; db <= 1; db++) {
+                        const int bb = b + db, cc = c + dc;
+                        if (bb >= b0 && bb <= b1 && cc >= c0 && cc <= c1) mark[size_t(cc - c0) * w + (bb - b0)] = 1;
+                    }
+            }
+        }
+    emit_hints(mark, b0, c0, w, h, job, out);
+}
+
+// Evaluation hints for a step-8 exact flood (window centre (sx, sz), half `half8`) from the proxy flood px
+// whose component selected it (dsearch_desert.cpp:498-512): each counted proxy point stands for the 32 x 32
+// quarts around it; every step-8 sub-block (8 x 8 cells, 64 x 64 quarts) within 16 + 8 quarts of a counted
+// proxy point is hinted. The exact component mostly lies inside the proxy component; cells outside the hints
+// are evaluated when the flood touches them.
+static void proxy_hints(const FloodOut &px, uint32_t job, int sx, int sz, int half8, std::vector<dsx_flood_hint> &out) {
+    if (px.tiles.empty() || px.a.cells == 0) return;
+    const int n8 = 2 * half8 + 1, nb = (n8 + 7) / 8;
+    auto fdiv = [](int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); };
+    auto sub = [&](int q, int s0) { return fdiv(fdiv(q - s0, 8) + half8, 8); };   // step-8 sub-block of quart q
+    // the sub-blocks within reach of the component's bounding box
+    const int b0 = std::max(sub(px.a.minx - 24, sx), 0), b1 = std::min(sub(px.a.maxx + 24, sx), nb - 1);
+    const int c0 = std::max(sub(px.a.minz - 24, sz), 0), c1 = std::min(sub(px.a.maxz + 24, sz), nb - 1);
+    if (b0 > b1 || c0 > c1) return;
+    const int w = b1 - b0 + 1, h = c1 - c0 + 1;
+    std::vector<uint8_t> mark(size_t(w) * h, 0);
+    px.each_counted([&](int i, int j) {
+        const int x = px.qx + (i - px.half) * px.step, z = px.qz + (j - px.half) * px.step;
+        for (int c = std::max(sub(z - 24, sz), c0); c <= std::min(sub(z + 24, sz), c1); c++)
+            for (int b = std::max(sub(x - 24, sx), b0); b <= std::min(sub(x + 24, sx), b1); b++)
+                mark[size_t(c - c0) * w + (b - b0)] = 1;
+    });
+    emit_hints(mark, b0, c0, w, h, job, out);
+}
+
 Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
     const uint64_t seed = O.seed;
     const float pcut = float(P.pcut);
@@ -784,6 +874,7 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
             seen.push_back(sig);
             Eval e{rank, bx, bz, a->a.cells * 32.0 * 32 * 16 / 1e6};
             e.hot = hot_area;
+            e.px = a;
             ev.push_back(e);
         }
     }
@@ -805,6 +896,9 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
     std::sort(ev.begin(), ev.end(), [](const Eval &a, const Eval &b) { return a.proxy > b.proxy; });
     size_t nx = 0;
     while (!P.study && nx < ev.size() && int(nx) < P.exact_n && ev[nx].proxy >= P.exact_min) nx++;
+    // exact floods get evaluation hints (DSX_HINT_EXACT=0 turns them off): step 8 from the proxy component,
+    // step 4 from the step-8 component
+    static const bool hint_exact = env_u64("DSX_HINT_EXACT", 1) != 0;
     PointsReq ring{&S, seed, DSX_PRED_DESERT, 0.f, {}, {}, {}, {}, {}};
     for (size_t k2 = 0; k2 < nx; k2++)
         for (int r = 0; r <= 8; r++)
@@ -816,12 +910,14 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
                 }
     co_await ring;
     const size_t per = ring.xs.size() / std::max<size_t>(nx, 1);
-    FloodReq f8{&S, {}, false, {}, {}};
+    FloodReq f8{&S, {}, hint_exact, {}, {}};   // counted cells: the step-4 floods' hints
     std::vector<int> f8_of(nx, -1);
     for (size_t k2 = 0; k2 < nx; k2++) {
         for (size_t k = k2 * per; k < (k2 + 1) * per; k++)
             if (ring.v8[k] == 2) {
                 f8_of[k2] = int(f8.jobs.size());
+                if (hint_exact && ev[k2].px)
+                    proxy_hints(*ev[k2].px, uint32_t(f8.jobs.size()), ring.xs[k], ring.zs[k], 16000 / 8, f8.hints);
                 f8.jobs.push_back(fjob(seed, DSX_PRED_DESERT_STATE, 0.f, ring.xs[k], ring.zs[k], 8, 16000 / 8, 1));
                 break;
             }
@@ -834,6 +930,7 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
         const FloodOut &a8 = f8.out[f8_of[k2]];
         if (a8.a.cells * 64.0 * 16 / 1e6 >= P.refine_mb) {
             f4_of[k2] = int(f4.jobs.size());
+            if (hint_exact) refine_hints(a8, uint32_t(f4.jobs.size()), 16000 / 4, f4.hints);
             f4.jobs.push_back(fjob(seed, DSX_PRED_DESERT_STATE, 0.f, a8.qx, a8.qz, 4, 16000 / 4, 0));
         }
     }

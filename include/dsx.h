@@ -213,6 +213,22 @@ typedef struct {
 enum { DSX_FLOOD_MAX_WIDTH = 46335 };
 int dsx_flood(dsx_ctx *ctx, const dsx_flood_job *jobs, uint64_t n, dsx_area *out);
 
+/* An evaluation hint for flood `job` of a dsx_flood_hinted batch: the 8x8 sub-blocks `blocks` (bit bz*4 + bx,
+ * 16 bits) of its 32x32 window tile (tx, tz) are evaluated up front, in parallel with every other hinted
+ * tile, instead of when the flood first touches them. Hints change only which cells are evaluated and when:
+ * the results are those of dsx_flood (a flood's cells, sampled cells, bounds and flags follow the cells it
+ * touches, never the evaluated ones). Hints for floods the small-window engine runs are ignored. */
+typedef struct {
+    uint32_t job;
+    int32_t tx, tz;
+    uint32_t blocks;
+} dsx_flood_hint;
+/* dsx_flood with evaluation hints (no counterpart in the reference): the pipeline hints each step-4 exact
+ * flood with the region of the step-8 flood it refines (dsearch_desert.cpp:520-523), so the exact flood's
+ * predicate is evaluated in one parallel pass rather than along its frontier. At most 2^24 hints. */
+int dsx_flood_hinted(dsx_ctx *ctx, const dsx_flood_job *jobs, uint64_t n, const dsx_flood_hint *hints, uint64_t nh,
+                     dsx_area *out);
+
 /* The counted cells (state 3 in dense_flood.hpp's window) of the floods of the last dsx_flood call, as
  * 32x32 tile masks: window cell (i, j) = (tx*32 + c, tz*32 + r) is counted iff bit c of rows[r].
  * Only tiles with at least one counted cell; any order. The hot-zone and proxy stages need them for

@@ -231,3 +231,49 @@ def test_flood_golden_step4_all_finds(engine):
     rows = [(u64(g["seed"]), 0, 0.35, g["x"] >> 2, g["z"] >> 2, 4, 5000, 0) for g in gold]
     got = engine.flood(dsx.flood_jobs(rows))
     assert [int(a["cells"]) for a in got] == [g["cells"] for g in gold]
+
+
+def test_flood_hints_change_nothing(engine):
+    """dsx_flood_hinted: evaluation hints only change which cells are evaluated and when. Step-4 floods hinted
+    with their step-8 flood's region (as the pipeline hints them), plus hints that help nothing (every
+    sub-block of random tiles, tiles outside the window, duplicates, hints of small-window floods) give
+    results and counted-cell masks identical to dsx_flood."""
+    dsx = load_package().dsx
+    rng = np.random.default_rng(11)
+    pts = [(u64(r["seed"]), r["x"] >> 2, r["z"] >> 2) for r in FINDS[20:26]]
+    f8 = engine.flood(dsx.flood_jobs([(s, 0, 0.35, qx, qz, 8, 2000, 1) for s, qx, qz in pts]))
+    m8 = engine.flood_masks()
+    rows = [(s, 0, 0.35, qx, qz, 4, 4000, 0) for s, qx, qz in pts]
+    rows += [(pts[0][0], 2, 0.35, pts[0][1], pts[0][2], 32, 500, 1), (pts[1][0], 1, 0.35, pts[1][1], pts[1][2], 128, 250, 0),
+             (pts[2][0], 0, 0.35, pts[2][1], pts[2][2], 4, 60, 0)]   # the last fits the small-window engine
+    jobs = dsx.flood_jobs(rows)
+    want = engine.flood(jobs)
+    wmask = engine.flood_masks()
+    hints = []
+    for k in range(len(pts)):   # step-4 sub-block (b, c) covers step-8 cells [4b, 4b+4) x [4c, 4c+4)
+        marked = set()
+        for m in m8[m8["job"] == k]:
+            for r in range(32):
+                bits = int(m["rows"][r])
+                for g in range(8):
+                    if (bits >> (4 * g)) & 0xF:
+                        b, c = int(m["tx"]) * 8 + g, int(m["tz"]) * 8 + r // 4
+                        marked |= {(b + db, c + dc) for db in (-1, 0, 1) for dc in (-1, 0, 1)}
+        tiles = {}
+        for b, c in marked:
+            if 0 <= b < 1001 and 0 <= c < 1001:
+                tiles[(c // 4, b // 4)] = tiles.get((c // 4, b // 4), 0) | (1 << ((c % 4) * 4 + b % 4))
+        hints += [(k, tx, tz, bits) for (tz, tx), bits in tiles.items()]
+    assert len(hints) > 100
+    nta = [((2 * h + 1) + 31) // 32 for h in (4000,) * len(pts) + (500, 250, 60)]
+    for _ in range(300):
+        k = int(rng.integers(len(rows)))
+        hints.append((k, int(rng.integers(-2, nta[k] + 2)), int(rng.integers(-2, nta[k] + 2)), 0xFFFF))
+    hints += hints[:50]
+    h = np.array(hints, dtype=dsx.FLOOD_HINT)
+    got = engine.flood_hinted(jobs, h)
+    gmask = engine.flood_masks()
+    assert [tuple(a) for a in got] == [tuple(a) for a in want]
+    key = lambda m: (int(m["job"]), int(m["tx"]), int(m["tz"]), tuple(int(x) for x in m["rows"]))
+    assert sorted(map(key, gmask)) == sorted(map(key, wmask))
+    assert [int(a["cells"]) for a in f8] and all(int(a["cells"]) > 0 for a in got[:len(pts)])
