@@ -28,8 +28,11 @@ Also reported:
                 over a bounded sample of seed indices on this host's cores: scanned / seconds. Falls
                 back to the oracle's C++ restatement of the same pipeline when oracle/_ref is absent.
                 post_gate: the same CPU pipeline in stdin=1 mode on exactly the seeds the GPU gate passes
-                over config 3's 1e7-index range, so both sides run the same gated seeds after the gate.
-  gate_only     the GPU gate alone over 2^24 seed indices (the gate kernel workload).
+                over the 1e7-index range [1e12, 1e12 + 1e7), so both sides run the same gated seeds after the gate.
+  gate_only     the GPU gate alone over 2^24 seed indices (the gate kernel workload), median of 3 calls,
+                with the gate replayed as one hipGraph per 2^22-index chunk (graph_on_s, the default) and
+                stage by stage with a survivor-count read-back after each stage (graph_off_s), on the
+                same rows (rows_identical).
   area_queries  4096 exact step-1 area queries (seed splitmix64(j), X, Z uniform in +-3e7, window
                 half = floor((sqrt(cap)-1)/2) for caps log-uniform over 1e3..1e7 cells) submitted at
                 once to the request server: p50/p99 latency per query and queries/s.
@@ -120,7 +123,7 @@ def cpu_baseline(gated_seeds, gpu_value, sample=10 ** 7):
     - the reference CPU pipeline with its own CPU gate (gpu=0) over seed indices [0, sample): the whole job
       the reference does without a GPU (value, the reported baseline);
     - post_gate: the reference in stdin=1 mode (dsearch_desert.cpp:571-581) on exactly the seeds the GPU gate
-      passes over config 3's range [1e12, 1e12 + 1e7): the same gated seeds the GPU pipeline processes,
+      passes over the range [1e12, 1e12 + 1e7): the same gated seeds the GPU pipeline processes,
       so the ratio compares the post-gate pipeline like for like."""
     cores = host_cores()
     with tempfile.TemporaryDirectory() as td:
@@ -362,19 +365,28 @@ def main():
     out["prepass"] = {"value": gated_pp / sec_pp, "ms_per_step": 1e3 * sec_pp / a.steps, "gated_seeds": int(gated_pp),
                       "outputs_identical": bool(same and gated_pp == gated), "vs_fp64": (gated_pp / sec_pp) / value,
                       "fp32_error": err, "bound_T": 2e-4}
-    # the seeds the GPU gate passes over config 3's range: the CPU baseline's post-gate leg runs exactly these
+    # the seeds the GPU gate passes over [1e12, 1e12 + 1e7): the CPU baseline's post-gate leg runs exactly these
     cfg3_gated = eng.gate(START, 10 ** 7)[0] if not a.no_cpu else None
     eng.close()
     if not a.no_extra:
         g = dsx.Engine(0)
-        g.gate(0, 1 << 24)
-        t1 = time.perf_counter()
-        _, _, _, cnt = g.gate(0, 1 << 24)
-        gsec = time.perf_counter() - t1
-        ms, work = g.gate_profile()
+        gsec, rows = {}, {}
+        for mode in (1, 0, 1, 0):   # graph replay, then stage by stage, twice each; the same 2^24 rows
+            g.set_gate_graph(mode)
+            g.gate(0, 1 << 24)
+            reps = []
+            for _ in range(3):
+                t1 = time.perf_counter()
+                sd, am, _, cnt = g.gate(0, 1 << 24)
+                reps.append(time.perf_counter() - t1)
+            gsec[mode] = min(gsec.get(mode, 1e9), float(np.median(reps)))
+            rows[mode] = (sd.tobytes(), am.tobytes(), tuple(cnt))
+        ms, work = g.gate_profile()   # per-kernel times of the last (stage-by-stage) call
+        g.set_gate_graph(1)
         g.close()
-        out["gate_only"] = {"seed_indices": 1 << 24, "seconds": gsec, "gated_seeds_per_s": cnt[2] / gsec,
-                            "counts": list(cnt)}
+        out["gate_only"] = {"seed_indices": 1 << 24, "seconds": gsec[1], "gated_seeds_per_s": cnt[2] / gsec[1],
+                            "counts": list(cnt), "graph_on_s": gsec[1], "graph_off_s": gsec[0],
+                            "graph_speedup": gsec[0] / gsec[1], "rows_identical": rows[0] == rows[1]}
         gach = 7077888 * work[2] / (ms[2] * 1e-3) / 1e12
         out["roofline_gate"] = {"bound": "fp32-valu", "kernel": "k_amp<0> (4-parity pre-gate)", "achieved": gach,
                                 "peak": 157.3, "unit": "TFLOP/s", "frac": gach / 157.3, "avg_launch_ms": ms[2],

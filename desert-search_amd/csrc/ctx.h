@@ -37,6 +37,11 @@ struct dsx_ctx {
     hipEvent_t ev[8] = {};
     double prof_ms[4] = {0, 0, 0, 0};
     uint64_t prof_work[4] = {0, 0, 0, 0};
+    // the gate as a replayed hipGraph (dsx_set_gate_graph): its configuration and the pinned outputs its
+    // sort kernel writes (4 counts, then cap1 indices and cap1 amplitudes)
+    int gate_graph = -1;                 // -1: from DSX_GATE_GRAPH (default on)
+    dsx::GateGraph ggraph;
+    uint64_t *h_gout = nullptr;
     // period tables + scan (allocated on first use)
     uint32_t scan_seeds = 0;             // seeds per chunk
     dsx::SideParams *d_tsp = nullptr;    // 6 octave sides per seed

@@ -154,26 +154,36 @@ int dsx_destroy(dsx_ctx *c) {
         if (p) (void)hipHostFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    dsx::gate_graph_free(c->ggraph);
+    if (c->h_gout) (void)hipHostFree(c->h_gout);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return DSX_OK;
 }
 
-int dsx_gate(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *p, uint64_t *seeds, float *amps,
-             uint64_t *kidx, uint64_t cap, uint64_t *n_out, dsx_gate_counts *counts) {
-    if (!c || !valid_gate_params(p) || !n_out) return DSX_E_ARG;
-    HIPCK(hipSetDevice(c->device));
+int dsx_set_gate_graph(dsx_ctx *c, int on) {
+    if (!c) return DSX_E_ARG;
+    c->gate_graph = on ? 1 : 0;
+    return DSX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+using Pass = std::vector<std::pair<uint64_t, float>>;
+
+// The gate over [start + off, start + off + count) with a host round trip per stage (survivor counts read
+// back before each stage's launch); survivors appended as (off + k, amplitude). Per-kernel times in prof_*.
+int gate_eager(dsx_ctx *c, uint64_t start, uint64_t off, uint64_t count, const dsx_gate_params *p, Pass &pass,
+               uint64_t tot[3]) {
     const dsx::GateKernel kpre = gate_kernel(4, p->box ? p->sig : 1.0f);
     const dsx::GateKernel kfull = gate_kernel(7, 2.0f);
     const double thr_pre = p->box ? double(p->pthr) : double(p->amp - p->margin);
     const int mode_pre = p->box ? 0 : 2;
     const double maxph = double(p->max_phase);
-    std::vector<std::pair<uint64_t, float>> pass;
-    uint64_t tot[3] = {0, 0, 0};
-    for (int i = 0; i < 4; i++) { c->prof_ms[i] = 0; c->prof_work[i] = 0; }
     uint64_t B = 1ull << 24;   // seed indices per batch (expected phase survivors ~0.72% << cap1)
-    for (uint64_t base = 0; base < count;) {
-        const uint32_t nb = uint32_t(std::min<uint64_t>(B, count - base));
+    for (uint64_t base = off; base < off + count;) {
+        const uint32_t nb = uint32_t(std::min<uint64_t>(B, off + count - base));
         HIPCK(hipMemsetAsync(c->d_cnt, 0, 16, c->st));
         HIPCK(hipEventRecord(c->ev[0], c->st));
         dsx::launch_phase(c->st, start + base, nb, maxph, c->d_k1, c->d_cnt, c->cap1);
@@ -234,6 +244,73 @@ int dsx_gate(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *
         tot[2] += n3;
         base += nb;
     }
+    return DSX_OK;
+}
+
+// Gate graph configuration for a chunk of up to 2^22 indices: phase-survivor grid from the expected pass
+// fraction 2 s^2 (s = max_phase <= 0.5), with room to spare; a chunk whose survivors exceed it is redone
+// by gate_eager.
+constexpr uint64_t GATE_GRAPH_CHUNK = 1ull << 22;
+uint32_t gate_graph_grid(const dsx_ctx *c, double maxph) {
+    const double s = std::min(std::max(maxph, 0.0), 0.5);
+    const double expect = 2.0 * s * s * double(GATE_GRAPH_CHUNK);
+    const uint64_t g = (uint64_t(1.25 * expect) + 4096 + 255) & ~uint64_t(255);
+    return uint32_t(std::min<uint64_t>(g, c->cap1));
+}
+
+int gate_graph_range(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *p, Pass &pass, uint64_t tot[3]) {
+    dsx::GateStages S;
+    std::memset(&S, 0, sizeof S);   // compared bytewise with the graph's configuration (padding included)
+    S.maxph = double(p->max_phase);
+    S.thr_pre = p->box ? double(p->pthr) : double(p->amp - p->margin);
+    S.amp = double(p->amp);
+    S.mode_pre = p->box ? 0 : 2;
+    S.kpre = gate_kernel(4, p->box ? p->sig : 1.0f);
+    S.kfull = gate_kernel(7, 2.0f);
+    const uint32_t grid = gate_graph_grid(c, S.maxph);
+    if (!c->h_gout) HIPCK(hipHostMalloc(&c->h_gout, (4 + size_t(c->cap1)) * 8 + size_t(c->cap1) * 4, hipHostMallocDefault));
+    uint64_t *h_counts = c->h_gout, *h_kidx = c->h_gout + 4;
+    float *h_amps = reinterpret_cast<float *>(h_kidx + c->cap1);
+    dsx::GateGraph &G = c->ggraph;
+    const bool same = G.exec && G.C.grid == grid && G.nmax == uint32_t(GATE_GRAPH_CHUNK) &&
+                      std::memcmp(&G.S, &S, sizeof S) == 0;
+    if (!same) {
+        const dsx::GateChain C{c->d_cnt, c->d_k1, c->d_i2, c->d_k3, c->d_a3, c->d_sides, grid};
+        HIPCK(dsx::gate_graph_build(G, C, S, uint32_t(GATE_GRAPH_CHUNK), dsx::GateOut{nullptr, h_amps, h_kidx, c->cap1, h_counts}));
+    }
+    for (uint64_t base = 0; base < count; base += GATE_GRAPH_CHUNK) {
+        const uint32_t m = uint32_t(std::min<uint64_t>(GATE_GRAPH_CHUNK, count - base));
+        HIPCK(dsx::gate_graph_launch(G, c->st, start + base, m));
+        HIPCK(hipStreamSynchronize(c->st));
+        if (h_counts[0] > grid) {   // more phase survivors than the graph holds: this chunk the eager way
+            if (const int rc = gate_eager(c, start, base, m, p, pass, tot)) return rc;
+            continue;
+        }
+        const uint64_t n3 = h_counts[3];
+        for (uint64_t i = 0; i < n3; i++) pass.push_back({base + h_kidx[i], h_amps[i]});
+        tot[0] += h_counts[0];
+        tot[1] += h_counts[1];
+        tot[2] += h_counts[2];
+    }
+    return DSX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int dsx_gate(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *p, uint64_t *seeds, float *amps,
+             uint64_t *kidx, uint64_t cap, uint64_t *n_out, dsx_gate_counts *counts) {
+    if (!c || !valid_gate_params(p) || !n_out) return DSX_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    if (c->gate_graph < 0) {
+        const char *e = std::getenv("DSX_GATE_GRAPH");
+        c->gate_graph = (e && *e == '0') ? 0 : 1;
+    }
+    Pass pass;
+    uint64_t tot[3] = {0, 0, 0};
+    for (int i = 0; i < 4; i++) { c->prof_ms[i] = 0; c->prof_work[i] = 0; }
+    const int rc = c->gate_graph ? gate_graph_range(c, start, count, p, pass, tot) : gate_eager(c, start, 0, count, p, pass, tot);
+    if (rc) return rc;
     std::sort(pass.begin(), pass.end());
     if (counts) { counts->phase = tot[0]; counts->pre = tot[1]; counts->full = tot[2]; }
     *n_out = pass.size();

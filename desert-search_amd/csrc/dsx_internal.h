@@ -25,6 +25,47 @@ struct GateKernel {
     double wd;   // octave weight the corner tables are scaled by
 };
 
+// The gate as one chain of kernels that needs no host round trip (no counts read back between stages):
+// counters cleared -> phase filter -> the count the later stages take (all phase survivors, or none when they
+// exceed `grid`) -> Perlin init -> pre-gate -> full gate -> survivors rank-sorted by seed index into `out`.
+// The grids of the later stages are sized for `grid` phase survivors; their kernels read the live counts.
+struct GateChain {
+    uint32_t *cnt;        // 4 counters: phase, pre, full, taken
+    uint32_t *k1, *i2, *k3;
+    float *a3;
+    SideParams *sides;    // 2 * grid
+    uint32_t grid;        // phase survivors the chain holds
+};
+struct GateOut {          // device or pinned host memory the sort kernel writes
+    uint64_t *seeds;      // may be null
+    float *amps;          // may be null
+    uint64_t *kidx;       // may be null
+    uint64_t cap;         // entries written at most
+    uint64_t *counts;     // {phase, pre, full (0, 0 if the phase survivors overflowed), min(full, cap)}
+};
+struct GateStages {       // thresholds and kernels of one gate configuration
+    double maxph, thr_pre, amp;
+    int mode_pre;         // 0 anti-aliased pre-gate, 2 legacy even-parity pre-gate
+    GateKernel kpre, kfull;
+};
+// Enqueues the chain on st for seed indices [start, start + count) (count <= 2^31).
+hipError_t enqueue_gate_chain(hipStream_t st, const GateChain &C, const GateStages &S, uint64_t start, uint32_t count,
+                              const GateOut &O);
+// The same chain as a hipGraph of kernel nodes (no memset nodes), built once per context and configuration and
+// replayed per chunk with only its start and count changed (count <= nmax, the size it was built for).
+struct GateGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t n_phase = nullptr, n_init = nullptr, n_sort = nullptr;
+    GateChain C{};
+    GateStages S{};
+    GateOut O{};
+    uint32_t nmax = 0;
+};
+hipError_t gate_graph_build(GateGraph &G, const GateChain &C, const GateStages &S, uint32_t nmax, const GateOut &O);
+hipError_t gate_graph_launch(GateGraph &G, hipStream_t st, uint64_t start, uint32_t count);
+void gate_graph_free(GateGraph &G);
+
 void launch_phase(hipStream_t st, uint64_t base, uint32_t n, double maxph, uint32_t *out, uint32_t *count, uint32_t cap);
 // n_dev (optional): a device word holding the item count, read by the kernels (min(n, *n_dev)); the grid
 // still covers n items, so a caller that does not know the count on the host launches with an upper bound.

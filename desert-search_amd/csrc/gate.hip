@@ -140,6 +140,197 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
     }
 }
 
+// ------------------------------------------------------------------ the gate as one chain without host round trips
+__global__ void k_gate_clear(uint32_t *cnt) {
+    if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+}
+
+// The number of phase survivors the later stages take: all, or none if they overflow the chain's grid.
+__global__ void k_gate_clamp(uint32_t *cnt, uint32_t grid) {
+    if (threadIdx.x == 0) cnt[3] = cnt[0] <= grid ? cnt[0] : 0u;
+}
+
+// One thread per full-gate survivor: its rank among the survivors' (unique) seed indices is its output slot.
+__global__ __launch_bounds__(256) void k_gate_sort(const uint32_t *cnt, const uint32_t *k3, const float *a3, uint64_t start,
+                                                   uint32_t grid, GateOut O) {
+    const uint32_t n = cnt[3] ? min(cnt[2], grid) : 0u;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i == 0) {
+        O.counts[0] = cnt[0];
+        O.counts[1] = cnt[3] ? cnt[1] : 0u;
+        O.counts[2] = n;
+        O.counts[3] = uint64_t(n) < O.cap ? uint64_t(n) : O.cap;
+    }
+    if (i >= n) return;
+    const uint32_t key = k3[i];
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < n; j++) rank += k3[j] < key ? 1u : 0u;
+    if (rank >= O.cap) return;
+    if (O.seeds) O.seeds[rank] = mix64(start + key);
+    if (O.amps) O.amps[rank] = a3[i];
+    if (O.kidx) O.kidx[rank] = key;
+}
+
+hipError_t enqueue_gate_chain(hipStream_t st, const GateChain &C, const GateStages &S, uint64_t start, uint32_t count,
+                              const GateOut &O) {
+    hipLaunchKernelGGL(k_gate_clear, dim3(1), dim3(64), 0, st, C.cnt);
+    if (count) {
+        launch_phase(st, start, count, S.maxph, C.k1, C.cnt, C.grid);
+        hipLaunchKernelGGL(k_gate_clamp, dim3(1), dim3(64), 0, st, C.cnt, C.grid);
+        launch_init(st, start, C.k1, nullptr, C.grid, C.sides, C.cnt + 3);
+        launch_amp(st, S.mode_pre, C.sides, nullptr, C.grid, S.kpre, S.thr_pre, nullptr, C.i2, nullptr, C.cnt + 1, C.grid,
+                   nullptr, C.cnt + 3);
+        launch_amp(st, 1, C.sides, C.i2, C.grid, S.kfull, S.amp, C.k1, C.k3, C.a3, C.cnt + 2, C.grid, nullptr, C.cnt + 1);
+    }
+    hipLaunchKernelGGL(k_gate_sort, dim3((C.grid + 255) / 256), dim3(256), 0, st, C.cnt, C.k3, C.a3, start, C.grid, O);
+    return hipGetLastError();
+}
+
+namespace {
+hipKernelNodeParams kparams(const void *func, dim3 grid, dim3 block, void **args) {
+    hipKernelNodeParams p{};
+    p.func = const_cast<void *>(func);
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.kernelParams = args;
+    p.sharedMemBytes = 0;
+    p.extra = nullptr;
+    return p;
+}
+// The argument lists of the nodes whose arguments change per chunk (start, count).
+hipKernelNodeParams phase_params(const GateGraph &G, uint64_t &base, uint32_t &n, double &maxph, uint32_t *&out,
+                                 uint32_t *&cnt, uint32_t &cap, void **a) {
+    a[0] = &base; a[1] = &n; a[2] = &maxph; a[3] = &out; a[4] = &cnt; a[5] = &cap;
+    return kparams(reinterpret_cast<const void *>(&k_phase), dim3((G.nmax + 255) / 256), dim3(256), a);
+}
+}  // namespace
+
+hipError_t gate_graph_build(GateGraph &G, const GateChain &C, const GateStages &S, uint32_t nmax, const GateOut &O) {
+    gate_graph_free(G);
+    G.C = C;
+    G.S = S;
+    G.O = O;
+    G.nmax = nmax;
+    hipError_t e = hipGraphCreate(&G.graph, 0);
+    if (e != hipSuccess) return e;
+    hipGraphNode_t prev = nullptr, node = nullptr;
+    auto add = [&](const hipKernelNodeParams &p, hipGraphNode_t *out) -> hipError_t {
+        const hipError_t r = hipGraphAddKernelNode(&node, G.graph, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+        if (r == hipSuccess) {
+            prev = node;
+            if (out) *out = node;
+        }
+        return r;
+    };
+    uint32_t *cnt = G.C.cnt;
+    {   // clear
+        void *a[] = {&cnt};
+        if ((e = add(kparams(reinterpret_cast<const void *>(&k_gate_clear), dim3(1), dim3(64), a), nullptr))) return e;
+    }
+    {   // phase filter (grid for nmax indices; the kernel checks k < n)
+        uint64_t base = 0;
+        uint32_t n = nmax, cap = G.C.grid;
+        double maxph = S.maxph;
+        uint32_t *out = G.C.k1;
+        void *a[6];
+        if ((e = add(phase_params(G, base, n, maxph, out, cnt, cap, a), &G.n_phase))) return e;
+    }
+    {   // clamp
+        uint32_t grid = G.C.grid;
+        void *a[] = {&cnt, &grid};
+        if ((e = add(kparams(reinterpret_cast<const void *>(&k_gate_clamp), dim3(1), dim3(64), a), nullptr))) return e;
+    }
+    {   // Perlin init of both sides of each phase survivor
+        uint64_t base = 0;
+        const uint32_t *ks = G.C.k1;
+        const uint64_t *seeds = nullptr;
+        uint32_t n = G.C.grid;
+        SideParams *out = G.C.sides;
+        const uint32_t *ndev = cnt + 3;
+        void *a[] = {&base, &ks, &seeds, &n, &out, &ndev};
+        if ((e = add(kparams(reinterpret_cast<const void *>(&k_init), dim3((2 * G.C.grid + 63) / 64), dim3(64), a),
+                     &G.n_init)))
+            return e;
+    }
+    for (int stage = 0; stage < 2; stage++) {   // pre-gate, then full gate
+        const SideParams *sides = G.C.sides;
+        const uint32_t *sel = stage ? G.C.i2 : nullptr;
+        uint32_t n = G.C.grid, cap = G.C.grid;
+        GateKernel K = stage ? S.kfull : S.kpre;
+        double thr = stage ? S.amp : S.thr_pre;
+        const uint32_t *kmap = stage ? G.C.k1 : nullptr;
+        uint32_t *oidx = stage ? G.C.k3 : G.C.i2;
+        float *oamp = stage ? G.C.a3 : nullptr;
+        uint32_t *count = cnt + 1 + stage;
+        float *all = nullptr;
+        const uint32_t *ndev = stage ? cnt + 1 : cnt + 3;
+        void *a[] = {&sides, &sel, &n, &K, &thr, &kmap, &oidx, &oamp, &count, &cap, &all, &ndev};
+        const int mode = stage ? 1 : S.mode_pre;
+        const void *f = mode == 0 ? reinterpret_cast<const void *>(&k_amp<0>)
+                        : mode == 1 ? reinterpret_cast<const void *>(&k_amp<1>)
+                                    : reinterpret_cast<const void *>(&k_amp<2>);
+        if ((e = add(kparams(f, dim3(G.C.grid), dim3(128), a), nullptr))) return e;
+    }
+    {   // rank sort into the outputs
+        const uint32_t *cc = cnt, *k3 = G.C.k3;
+        const float *a3 = G.C.a3;
+        uint64_t start = 0;
+        uint32_t grid = G.C.grid;
+        GateOut o = G.O;
+        void *a[] = {&cc, &k3, &a3, &start, &grid, &o};
+        if ((e = add(kparams(reinterpret_cast<const void *>(&k_gate_sort), dim3((G.C.grid + 255) / 256), dim3(256), a),
+                     &G.n_sort)))
+            return e;
+    }
+    return hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0);
+}
+
+hipError_t gate_graph_launch(GateGraph &G, hipStream_t st, uint64_t start, uint32_t count) {
+    if (!G.exec || count > G.nmax) return hipErrorInvalidValue;
+    hipError_t e;
+    {
+        uint64_t base = start;
+        uint32_t n = count, cap = G.C.grid;
+        double maxph = G.S.maxph;
+        uint32_t *out = G.C.k1, *cnt = G.C.cnt;
+        void *a[6];
+        const hipKernelNodeParams p = phase_params(G, base, n, maxph, out, cnt, cap, a);
+        if ((e = hipGraphExecKernelNodeSetParams(G.exec, G.n_phase, &p))) return e;
+    }
+    {
+        uint64_t base = start;
+        const uint32_t *ks = G.C.k1;
+        const uint64_t *seeds = nullptr;
+        uint32_t n = G.C.grid;
+        SideParams *out = G.C.sides;
+        const uint32_t *ndev = G.C.cnt + 3;
+        void *a[] = {&base, &ks, &seeds, &n, &out, &ndev};
+        const hipKernelNodeParams p =
+            kparams(reinterpret_cast<const void *>(&k_init), dim3((2 * G.C.grid + 63) / 64), dim3(64), a);
+        if ((e = hipGraphExecKernelNodeSetParams(G.exec, G.n_init, &p))) return e;
+    }
+    {
+        const uint32_t *cc = G.C.cnt, *k3 = G.C.k3;
+        const float *a3 = G.C.a3;
+        uint64_t s0 = start;
+        uint32_t grid = G.C.grid;
+        GateOut o = G.O;
+        void *a[] = {&cc, &k3, &a3, &s0, &grid, &o};
+        const hipKernelNodeParams p =
+            kparams(reinterpret_cast<const void *>(&k_gate_sort), dim3((G.C.grid + 255) / 256), dim3(256), a);
+        if ((e = hipGraphExecKernelNodeSetParams(G.exec, G.n_sort, &p))) return e;
+    }
+    return hipGraphLaunch(G.exec, st);
+}
+
+void gate_graph_free(GateGraph &G) {
+    if (G.exec) (void)hipGraphExecDestroy(G.exec);
+    if (G.graph) (void)hipGraphDestroy(G.graph);
+    G.exec = nullptr;
+    G.graph = nullptr;
+    G.n_phase = G.n_init = G.n_sort = nullptr;
+}
+
 // ------------------------------------------------------------------ launchers
 void launch_phase(hipStream_t st, uint64_t base, uint32_t n, double maxph, uint32_t *out, uint32_t *count, uint32_t cap) {
     hipLaunchKernelGGL(k_phase, dim3((n + 255) / 256), dim3(256), 0, st, base, n, maxph, out, count, cap);

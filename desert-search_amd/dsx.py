@@ -160,6 +160,7 @@ def load(path=LIB_PATH):
     lib.dsx_gate_noalloc.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                      ctypes.POINTER(GateParams), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_uint64, ctypes.c_void_p]
+    lib.dsx_set_gate_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.dsx_gate_amps.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, ctypes.POINTER(GateParams), _f32p, _f32p]
     lib.dsx_gate_profile.argtypes = [ctypes.c_void_p, _f64p, _u64p]
     lib.dsx_scan.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, ctypes.POINTER(ScanParams), _f32p, _f32p,
@@ -253,6 +254,10 @@ class Engine:
         self._check(rc, "dsx_gate")
         m = n.value
         return seeds[:m], amps[:m], kidx[:m], (cnt.phase, cnt.pre, cnt.full)
+
+    def set_gate_graph(self, on):
+        """dsx_set_gate_graph: 1 replays the gate as a hipGraph per chunk, 0 launches it stage by stage."""
+        self._check(self.lib.dsx_set_gate_graph(self.h, 1 if on else 0), "dsx_set_gate_graph")
 
     def gate_amps(self, seeds, params=None):
         """(pre-gate amplitude, full amplitude) per seed, no thresholds (tgate calib mode)."""

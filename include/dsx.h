@@ -98,7 +98,14 @@ enum {
 int dsx_gate_noalloc(int device, void *stream, uint64_t start, uint64_t count, const dsx_gate_params *p, uint64_t *seeds,
                      float *amps, uint64_t *kidx, uint64_t cap, uint64_t *counts);
 
-/* Device time (ms) of the kernels of the last dsx_gate call, summed over its batches:
+/* How dsx_gate runs (no counterpart in the reference, whose run_gate synchronises after each stage,
+ * gate.cuh:384-407): on = 1 (the default; DSX_GATE_GRAPH=0 changes the default) replays, per 2^22-index
+ * chunk, a hipGraph of the whole gate built once per context (the stages read their survivor counts on the
+ * device; one synchronisation per chunk); on = 0 launches the stages one by one and reads each stage's
+ * survivor count back first (dsx_gate_profile times each kernel only then). Results are identical. */
+int dsx_set_gate_graph(dsx_ctx *ctx, int on);
+
+/* Device time (ms) of the kernels of the last dsx_gate call, summed over its batches (with the graph off):
  * ms[0] phase filter, ms[1] Perlin init, ms[2] pre-gate, ms[3] full gate; work[0..3] the number of
  * items each processed (seed indices, sides, pre-gate seeds, full-gate seeds). */
 int dsx_gate_profile(dsx_ctx *ctx, double ms[4], uint64_t work[4]);

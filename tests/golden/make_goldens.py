@@ -5,7 +5,7 @@ The tools are built from the reference sources by ``make -f oracle/Makefile.ref`
 (this container only; /root/reference is not needed afterwards). Every fixture written here is data:
 inputs and the reference's outputs.
 
-  c1.*            dsearch_desert 0 200000 P 8           (config 1, CPU-default gate)
+  c1.*            dsearch_desert 0 200000 P 8           (200,000 seed indices, CPU-default gate)
   r3m.*           dsearch_desert 0 3000000 P 8          (larger range: exact step-8 floods)
   rg1m.*          dsearch_desert 0 0 P 8 stdin=1 < gated_1e12_1m.txt (seeds gated with the GPU-gate
                   rules over k in [1e12, 1e12 + 2^20); the list itself is an input fixture)

@@ -54,7 +54,7 @@ def test_gate_legacy_mode_matches_oracle(engine, oracle):
 
 
 def test_gate_full_size_batch_invariance(engine):
-    """BASELINE config 2 (2^24 seeds): one call equals 16 calls of 2^20 (a checksum of checksums),
+    """The 2^24-index gate workload: one call equals 16 calls of 2^20 (a checksum of checksums),
     survivors sorted and above threshold."""
     seeds, amps, k, cnt = engine.gate(0, 1 << 24)
     assert np.all(np.diff(k.astype(np.int64)) > 0)
@@ -134,3 +134,30 @@ def test_tgate_tool(engine, oracle):
         assert ph == "%.5f" % oracle.or_phase(int(s), ctypes.byref(fy))
     r = subprocess.run([exe, "0", "10", "bogus=1"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "unknown key bogus" in r.stderr
+
+
+def test_gate_graph_equals_eager(engine):
+    """The gate replayed as a hipGraph per 2^22-index chunk (dsx_set_gate_graph(1), the default) gives the
+    survivors, amplitudes and counts of the stage-by-stage launches, bit for bit: across chunk boundaries and
+    a partial last chunk, the 64-bit index wrap, another phase cut (graph rebuilt), a phase cut whose survivors
+    overflow the graph's grid (those chunks redone stage by stage), and the legacy pre-gate; graph replays
+    alternate with eager calls on the same context."""
+    from conftest import load_package
+    dsx = load_package().dsx
+    cases = [(10 ** 12, (1 << 24) + 777, dsx.GateParams()), (0, 1 << 22, dsx.GateParams()),
+             ((1 << 64) - (1 << 20), 1 << 21, dsx.GateParams()), (5 * 10 ** 11, 3 << 20, dsx.GateParams(max_phase=0.10)),
+             (7, 1 << 21, dsx.GateParams(max_phase=0.3)), (10 ** 12, 1 << 22, dsx.GateParams(box=0))]
+    try:
+        for rep in range(2):
+            for start, count, p in cases:
+                engine.set_gate_graph(1)
+                g = engine.gate(start % (1 << 64), count, p)
+                engine.set_gate_graph(0)
+                e = engine.gate(start % (1 << 64), count, p)
+                print(f"gate graph vs eager: start={start} count={count} counts {g[3]} / {e[3]}", flush=True)
+                assert g[3] == e[3]
+                assert np.array_equal(g[0], e[0]) and np.array_equal(g[2], e[2])
+                assert np.array_equal(g[1].view(np.uint32), e[1].view(np.uint32))
+                assert len(g[0]) > 0
+    finally:
+        engine.set_gate_graph(1)

@@ -48,7 +48,7 @@ def _check(tag, rows, srows, cnt, cand=None):
 
 
 def test_config1_matches_reference(oracle):
-    """BASELINE config 1 (seeds mix(k), k < 200000, production Params): CSV, seeds CSV and counters."""
+    """The 200,000-index workload (seeds mix(k), k < 200000, production Params): CSV, seeds CSV and counters."""
     with tempfile.TemporaryDirectory() as td:
         rows, srows, cnt, cand = _run(oracle, td, "c1", 0, 200000)
     _check("c1", rows, srows, cnt, cand)

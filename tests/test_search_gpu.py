@@ -38,7 +38,7 @@ def _seeds(name):
 
 
 def test_config1_pipeline(engine):
-    """seeds mix(k), k < 200000, CPU-default gate semantics, production Params."""
+    """The 200,000-index workload: seeds mix(k), k < 200000, CPU-default gate semantics, production Params."""
     dsx = load_package().dsx
     with tempfile.TemporaryDirectory() as td:
         pre = os.path.join(td, "c1")
