@@ -30,9 +30,12 @@ Also reported:
                 post_gate: the same CPU pipeline in stdin=1 mode on exactly the seeds the GPU gate passes
                 over the 1e7-index range [1e12, 1e12 + 1e7), so both sides run the same gated seeds after the gate.
   gate_only     the GPU gate alone over 2^24 seed indices (the gate kernel workload), median of 3 calls,
-                with the gate replayed as one hipGraph per 2^22-index chunk (graph_on_s, the default) and
+                with the gate replayed as one hipGraph per chunk (graph_on_s, the default; 2^24 indices here) and
                 stage by stage with a survivor-count read-back after each stage (graph_off_s), on the
                 same rows (rows_identical).
+  tuning        the autotuner's settings (desert-search_amd/tuned.json, applied by the library) against the
+                library defaults on the same timed steps, each in fresh processes, alternating, median of 2:
+                gated seeds/s of each and whether their output rows are identical.
   area_queries  4096 exact step-1 area queries (seed splitmix64(j), X, Z uniform in +-3e7, window
                 half = floor((sqrt(cap)-1)/2) for caps log-uniform over 1e3..1e7 cells) submitted at
                 once to the request server: p50/p99 latency per query and queries/s.
@@ -45,6 +48,7 @@ import ctypes
 import importlib.util
 import json
 import os
+import statistics
 import subprocess
 import sys
 import tempfile
@@ -247,6 +251,34 @@ def serving(dsx, n_req=240, rate=24.0, seed=20261203, workers=16):
     return out
 
 
+def tuning(steps, reps=2):
+    """desert-search_amd/tuned.json (the autotuner's settings, applied by dsx_create) against the library
+    defaults (DSX_TUNED=0): the workload's timed steps in fresh processes (the settings are read once per
+    process), alternating, median of `reps` each; rows_identical: the sorted output rows of every step
+    hash the same."""
+    path = os.path.join(REPO, "desert-search_amd", "tuned.json")
+    env_t = json.load(open(path))["env"] if os.path.exists(path) else {}
+    script = os.path.join(REPO, "desert-search_amd", "autotune.py")
+
+    def run(off):
+        e = {k: v for k, v in os.environ.items() if k != "DSX_TUNED"}
+        if off:
+            e["DSX_TUNED"] = "0"
+        r = subprocess.run([sys.executable, script, "--trial", "--steps", str(steps)], env=e, capture_output=True,
+                           text=True, timeout=300, check=True)
+        o = json.loads(r.stdout.strip().splitlines()[-1])
+        return o["value"], o["rows"]
+
+    t, d = [], []
+    for _ in range(reps):
+        t.append(run(False))
+        d.append(run(True))
+    tv = statistics.median(v for v, _ in t)
+    dv = statistics.median(v for v, _ in d)
+    return {"env": env_t, "tuned": tv, "default": dv, "speedup": tv / dv, "steps": steps, "reps": reps,
+            "rows_identical": len({h for _, h in t + d}) == 1}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,6 +306,8 @@ def main():
         print("bench.py: this build measures one GPU only (seeds shard by index range with no communication; "
               "no multi-GPU run is part of it)", file=sys.stderr)
         sys.exit(2)
+    # first, while this process holds no GPU queues of its own (held ones would compete with the trials')
+    tun = None if a.no_extra else tuning(a.steps)
     import torch
     pkg = load_package()
     dsx = pkg.dsx
@@ -391,6 +425,7 @@ def main():
         out["roofline_gate"] = {"bound": "fp32-valu", "kernel": "k_amp<0> (4-parity pre-gate)", "achieved": gach,
                                 "peak": 157.3, "unit": "TFLOP/s", "frac": gach / 157.3, "avg_launch_ms": ms[2],
                                 "phase_survivors": int(work[2])}
+        out["tuning"] = tun
         out["area_queries"] = area_burst(dsx)
         out["serving"] = serving(dsx)
     # flood batches that hit the engine's time limit (run again on a reset engine) or exhausted its tile pool,

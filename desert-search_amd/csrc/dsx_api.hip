@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <dlfcn.h>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -64,6 +65,59 @@ dsx::GateKernel cpu_gate_kernel() {
     K.wd = factor * (w0 * c.amp[0]);
     return K;
 }
+// Settings chosen by the autotuner for this device: tuned.json beside the library holds an "env" object of
+// "DSX_*": "value" pairs, each of which becomes the default of that environment variable (a variable that
+// is already set wins; DSX_TUNED=0 skips the file). Applied once, at the first dsx_create and before its
+// first HIP call, so every setting the library reads later (launch shapes, workers, batching) sees it; the
+// settings are all result-invariant. A malformed file is ignored with a message on stderr.
+void apply_tuned() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const char *off = std::getenv("DSX_TUNED");
+        if (off && *off == '0') return;
+        Dl_info info;
+        if (!dladdr(reinterpret_cast<void *>(&apply_tuned), &info) || !info.dli_fname) return;
+        std::string path(info.dli_fname);
+        path = path.substr(0, path.rfind('/') + 1) + "tuned.json";
+        FILE *f = std::fopen(path.c_str(), "rb");
+        if (!f) return;
+        std::string txt;
+        char buf[4096];
+        for (size_t n; (n = std::fread(buf, 1, sizeof buf, f)) > 0;) txt.append(buf, n);
+        std::fclose(f);
+        // the "env" object: a flat list of string keys and string or number values
+        size_t p = txt.find("\"env\"");
+        p = p == std::string::npos ? p : txt.find('{', p);
+        const size_t e = p == std::string::npos ? p : txt.find('}', p);
+        if (e == std::string::npos) {
+            std::fprintf(stderr, "dsx: %s has no \"env\" object; ignored\n", path.c_str());
+            return;
+        }
+        std::vector<std::pair<std::string, std::string>> kv;
+        const auto parse = [&] {
+            for (size_t i = p + 1; i < e;) {
+                const size_t k0 = txt.find('"', i);
+                if (k0 == std::string::npos || k0 >= e) return true;
+                const size_t k1 = txt.find('"', k0 + 1), c = k1 == std::string::npos ? k1 : txt.find(':', k1);
+                if (c == std::string::npos || c >= e) return false;
+                const size_t v0 = txt.find_first_not_of(" \t\r\n", c + 1);
+                if (v0 == std::string::npos || v0 >= e) return false;
+                const bool quoted = txt[v0] == '"';
+                const size_t v1 = quoted ? txt.find('"', v0 + 1) : txt.find_first_of(",} \t\r\n", v0);
+                if (v1 == std::string::npos || v1 > e || (quoted && v1 == e)) return false;
+                kv.emplace_back(txt.substr(k0 + 1, k1 - k0 - 1), txt.substr(v0 + quoted, v1 - v0 - quoted));
+                i = v1 + quoted;
+            }
+            return true;
+        };
+        if (!parse()) {
+            std::fprintf(stderr, "dsx: malformed \"env\" object in %s; ignored\n", path.c_str());
+            return;
+        }
+        for (auto &[k, v] : kv)
+            if (k.rfind("DSX_", 0) == 0 && k != "DSX_TUNED") setenv(k.c_str(), v.c_str(), 0);
+    });
+}
 }  // namespace
 
 static int ctx_alloc(dsx_ctx *c);
@@ -89,6 +143,7 @@ const char *dsx_last_error(void) { return g_last_error.c_str(); }
 int dsx_create(int device, dsx_ctx **out) {
     if (!out) return DSX_E_ARG;
     *out = nullptr;
+    apply_tuned();
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
         g_last_error = "no HIP device";
@@ -247,15 +302,18 @@ int gate_eager(dsx_ctx *c, uint64_t start, uint64_t off, uint64_t count, const d
     return DSX_OK;
 }
 
-// Gate graph configuration for a chunk of up to 2^22 indices: phase-survivor grid from the expected pass
-// fraction 2 s^2 (s = max_phase <= 0.5), with room to spare; a chunk whose survivors exceed it is redone
-// by gate_eager.
-constexpr uint64_t GATE_GRAPH_CHUNK = 1ull << 22;
-uint32_t gate_graph_grid(const dsx_ctx *c, double maxph) {
-    const double s = std::min(std::max(maxph, 0.0), 0.5);
-    const double expect = 2.0 * s * s * double(GATE_GRAPH_CHUNK);
-    const uint64_t g = (uint64_t(1.25 * expect) + 4096 + 255) & ~uint64_t(255);
-    return uint32_t(std::min<uint64_t>(g, c->cap1));
+// Gate graph configuration: chunks of nmax indices, the largest power of two up to 2^26 whose expected
+// phase survivors 2 s^2 nmax (s = max_phase <= 0.5), with a quarter and 4096 to spare, fit the survivor
+// buffers (cap1), shrunk to the smallest power of two that holds the whole call; grid = that survivor
+// bound. A chunk whose survivors exceed the grid is redone by gate_eager.
+void gate_graph_shape(const dsx_ctx *c, double maxph, uint64_t count, uint32_t &nmax, uint32_t &grid) {
+    const double s = std::min(std::max(maxph, 0.0), 0.5), f = 2.0 * s * s;
+    uint64_t n = 1ull << 26;
+    while (n > 256 && 1.25 * f * double(n) + 4096 > double(c->cap1)) n >>= 1;
+    while (n > 256 && n / 2 >= count) n >>= 1;
+    nmax = uint32_t(n);
+    const uint64_t g = (uint64_t(1.25 * f * double(n)) + 4096 + 255) & ~uint64_t(255);
+    grid = uint32_t(std::min<uint64_t>(g, c->cap1));
 }
 
 int gate_graph_range(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *p, Pass &pass, uint64_t tot[3]) {
@@ -267,19 +325,19 @@ int gate_graph_range(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_
     S.mode_pre = p->box ? 0 : 2;
     S.kpre = gate_kernel(4, p->box ? p->sig : 1.0f);
     S.kfull = gate_kernel(7, 2.0f);
-    const uint32_t grid = gate_graph_grid(c, S.maxph);
+    uint32_t nmax, grid;
+    gate_graph_shape(c, S.maxph, count, nmax, grid);
     if (!c->h_gout) HIPCK(hipHostMalloc(&c->h_gout, (4 + size_t(c->cap1)) * 8 + size_t(c->cap1) * 4, hipHostMallocDefault));
     uint64_t *h_counts = c->h_gout, *h_kidx = c->h_gout + 4;
     float *h_amps = reinterpret_cast<float *>(h_kidx + c->cap1);
     dsx::GateGraph &G = c->ggraph;
-    const bool same = G.exec && G.C.grid == grid && G.nmax == uint32_t(GATE_GRAPH_CHUNK) &&
-                      std::memcmp(&G.S, &S, sizeof S) == 0;
+    const bool same = G.exec && G.C.grid == grid && G.nmax == nmax && std::memcmp(&G.S, &S, sizeof S) == 0;
     if (!same) {
         const dsx::GateChain C{c->d_cnt, c->d_k1, c->d_i2, c->d_k3, c->d_a3, c->d_sides, grid};
-        HIPCK(dsx::gate_graph_build(G, C, S, uint32_t(GATE_GRAPH_CHUNK), dsx::GateOut{nullptr, h_amps, h_kidx, c->cap1, h_counts}));
+        HIPCK(dsx::gate_graph_build(G, C, S, nmax, dsx::GateOut{nullptr, h_amps, h_kidx, c->cap1, h_counts}));
     }
-    for (uint64_t base = 0; base < count; base += GATE_GRAPH_CHUNK) {
-        const uint32_t m = uint32_t(std::min<uint64_t>(GATE_GRAPH_CHUNK, count - base));
+    for (uint64_t base = 0; base < count; base += nmax) {
+        const uint32_t m = uint32_t(std::min<uint64_t>(nmax, count - base));
         HIPCK(dsx::gate_graph_launch(G, c->st, start + base, m));
         HIPCK(hipStreamSynchronize(c->st));
         if (h_counts[0] > grid) {   // more phase survivors than the graph holds: this chunk the eager way

@@ -1366,7 +1366,8 @@ int dsx_search(dsx_ctx *ctx, const dsx_search_params *pp, int mode, uint64_t sta
     }
     const dsx_search_params P = *pp;
     if (P.nms <= 0 || P.max_sites < 0 || P.workers < 0 || P.workers > 64) return DSX_E_ARG;
-    const int nw = P.workers ? P.workers : 16;
+    // workers = 0: DSX_WORKERS, else 16 (one per host core of a 16-core share)
+    const int nw = P.workers ? P.workers : int(std::clamp<uint64_t>(env_u64("DSX_WORKERS", 16), 1, 64));
     std::vector<dsx_ctx *> wctx;
     const int ns = int(std::min<uint64_t>(env_u64("DSX_SCAN_WORKERS", 0), 8));
     int rc = worker_contexts(ctx, nw + ns, wctx);

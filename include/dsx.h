@@ -99,10 +99,12 @@ int dsx_gate_noalloc(int device, void *stream, uint64_t start, uint64_t count, c
                      float *amps, uint64_t *kidx, uint64_t cap, uint64_t *counts);
 
 /* How dsx_gate runs (no counterpart in the reference, whose run_gate synchronises after each stage,
- * gate.cuh:384-407): on = 1 (the default; DSX_GATE_GRAPH=0 changes the default) replays, per 2^22-index
- * chunk, a hipGraph of the whole gate built once per context (the stages read their survivor counts on the
- * device; one synchronisation per chunk); on = 0 launches the stages one by one and reads each stage's
- * survivor count back first (dsx_gate_profile times each kernel only then). Results are identical. */
+ * gate.cuh:384-407): on = 1 (the default; DSX_GATE_GRAPH=0 changes the default) replays, per chunk, a
+ * hipGraph of the whole gate (the stages read their survivor counts on the device; one synchronisation per
+ * chunk), built once per context and shape: chunks are the largest power of two up to 2^26 indices whose
+ * expected phase survivors fit the survivor buffers (2^24 at max_phase 0.06), or the smallest holding the
+ * call; on = 0 launches the stages one by one and reads each stage's survivor count back first
+ * (dsx_gate_profile times each kernel only then). Results are identical. */
 int dsx_set_gate_graph(dsx_ctx *ctx, int on);
 
 /* Device time (ms) of the kernels of the last dsx_gate call, summed over its batches (with the graph off):

@@ -137,16 +137,19 @@ def test_tgate_tool(engine, oracle):
 
 
 def test_gate_graph_equals_eager(engine):
-    """The gate replayed as a hipGraph per 2^22-index chunk (dsx_set_gate_graph(1), the default) gives the
-    survivors, amplitudes and counts of the stage-by-stage launches, bit for bit: across chunk boundaries and
-    a partial last chunk, the 64-bit index wrap, another phase cut (graph rebuilt), a phase cut whose survivors
-    overflow the graph's grid (those chunks redone stage by stage), and the legacy pre-gate; graph replays
-    alternate with eager calls on the same context."""
+    """The gate replayed as a hipGraph per chunk (dsx_set_gate_graph(1), the default) gives the survivors,
+    amplitudes and counts of the stage-by-stage launches, bit for bit: across chunk boundaries and a partial
+    last chunk (2^24 + 777 at max_phase 0.06: chunks of 2^24), the 64-bit index wrap, calls of other sizes
+    and phase cuts (graph rebuilt; 0.3 gives 2^20-index chunks), a phase cut whose survivors overflow the
+    graph's grid (max_phase 0.9: about 98 % survive, the grid holds 0.625 of a chunk + 4096, so every chunk
+    is redone stage by stage), and the legacy pre-gate; graph replays alternate with eager calls on the same
+    context."""
     from conftest import load_package
     dsx = load_package().dsx
     cases = [(10 ** 12, (1 << 24) + 777, dsx.GateParams()), (0, 1 << 22, dsx.GateParams()),
              ((1 << 64) - (1 << 20), 1 << 21, dsx.GateParams()), (5 * 10 ** 11, 3 << 20, dsx.GateParams(max_phase=0.10)),
-             (7, 1 << 21, dsx.GateParams(max_phase=0.3)), (10 ** 12, 1 << 22, dsx.GateParams(box=0))]
+             (7, 1 << 21, dsx.GateParams(max_phase=0.3)), (10 ** 12, 1 << 22, dsx.GateParams(box=0)),
+             (3, 1 << 20, dsx.GateParams(max_phase=0.9))]
     try:
         for rep in range(2):
             for start, count, p in cases:
@@ -159,5 +162,7 @@ def test_gate_graph_equals_eager(engine):
                 assert np.array_equal(g[0], e[0]) and np.array_equal(g[2], e[2])
                 assert np.array_equal(g[1].view(np.uint32), e[1].view(np.uint32))
                 assert len(g[0]) > 0
+                if p.max_phase > 0.5:
+                    assert g[3][0] > 0.7 * count   # beyond the graph grid's 0.625 of a chunk + 4096
     finally:
         engine.set_gate_graph(1)
