@@ -12,7 +12,8 @@ second over the timed steps. Inputs are seed index ranges generated on the devic
 CSV files the pipeline writes.
 
 Also reported:
-  roofline      the dominant kernel, the persistent tiled flood kernel (k_flood_run, FP64 VALU):
+  roofline      the dominant kernel, the persistent tiled flood kernel (k_flood_run, FP64 VALU), per launch
+                and (device_level) over the timed region's wall time:
                 algorithmic FP64 operations = the reference's sampled cells of the floods it ran x octave
                 samples per predicate call of their kind and cost class x 89 FP64 operations per octave
                 sample, divided by its device time from HIP events recorded around each launch on the
@@ -358,6 +359,10 @@ def main():
                      "flops_per_launch": flops / launches,
                      "octave_samples": int(samples), "sampled_cells": int(sampled_big),
                      "cells_evaluated": int(cells_eval),
+                     # the same algorithmic FP64 operations over the timed region's wall time: the engine's
+                     # rate at device level, with the launches of the 16 worker streams overlapping
+                     "device_level": {"achieved": flops / sec / 1e12, "frac": flops / sec / 1e12 / FP64_PEAK_TFLOPS,
+                                      "launch_overlap": st.eval_ms * 1e-3 / sec},
                      "by_kind": {KINDS[k]: {"floods": st.floods[k], "tiles": st.tiles[k], "evaluated": st.evaluated[k],
                                             "sampled_cold": st.sampled_cold[k], "sampled_hot": st.sampled_hot[k],
                                             "by_size": [[st.size_floods[k][b], st.size_tiles[k][b],
