@@ -232,9 +232,12 @@ typedef struct {
     int32_t tx, tz;
     uint32_t blocks;
 } dsx_flood_hint;
-/* dsx_flood with evaluation hints (no counterpart in the reference): the pipeline hints each step-4 exact
- * flood with the region of the step-8 flood it refines (dsearch_desert.cpp:520-523), so the exact flood's
- * predicate is evaluated in one parallel pass rather than along its frontier. At most 2^24 hints. */
+/* dsx_flood with evaluation hints (no counterpart in the reference): the pipeline hints each step-8 exact
+ * flood with the proxy component that selected it (dsearch_desert.cpp:498-512) and each step-4 flood with
+ * the region of the step-8 flood it refines (:520-523), so the exact floods' predicate is evaluated in one
+ * parallel pass rather than along their frontiers. Tile coordinates are relative to the job's window
+ * corner; hints outside the window are ignored; a hint's job must be < n (else DSX_E_ARG). At most 2^24
+ * hints. */
 int dsx_flood_hinted(dsx_ctx *ctx, const dsx_flood_job *jobs, uint64_t n, const dsx_flood_hint *hints, uint64_t nh,
                      dsx_area *out);
 
