@@ -97,7 +97,8 @@ __device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K)
     grid_setup<MODE>(sp, L, K.wd);
     float best = -1e9f;
     constexpr int NC = Shape<MODE>::G / 64;
-    grid_blur_rows<MODE, GateR<MODE>::R>(sp, L, K.k, 0, Shape<MODE>::G - 1, [&](int, const float (&o)[NC]) {
+    // only the maximum is kept, so sums may start from their first term (ZSIGN off)
+    grid_blur_rows<MODE, GateR<MODE>::R, false>(sp, L, K.k, 0, Shape<MODE>::G - 1, [&](int, const float (&o)[NC]) {
 #pragma unroll
         for (int q = 0; q < NC; q++) best = fmaxf(best, o[q]);
     });

@@ -12,9 +12,12 @@
 //
 // Layout: each lane owns NC = G/64 adjacent columns and walks the rows in pairs; the corner tables
 // (16-float sub-tables, one per corner term, indexed by gradient hash) and the packed hash streams
-// (Q4[m] = byte offsets 4*(P[(m+t)&255] & 15), t = 0..3) sit in LDS; the horizontal halo comes from
-// the neighbouring lanes by shuffle (lane 63 <-> lane 0 is the periodic wrap); the vertical window of
-// 2R+1 rows stays in registers.
+// (Q4[QPAD + m] = byte offsets 4*(P[(m+t)&255] & 15), t = 0..3, replicated for m in [-QPAD, QN - QPAD)
+// so that a stream's row offset needs no wrap) sit in LDS; the horizontal halo comes from the
+// neighbouring lanes by shuffle (lane 63 <-> lane 0 is the periodic wrap); the vertical window of
+// 2R+1 rows stays in registers. The corner tables come first in WaveLds: a table read is its hash
+// byte's address plus an immediate sub-table offset, and paired reads (ds_read2, 8-bit dword offsets)
+// reach 1 KB from one address.
 #pragma once
 #include "dsx_internal.h"
 
@@ -25,17 +28,35 @@ template <> struct Shape<0> { static constexpr int G = 256, NSTREAM = 12; };
 template <> struct Shape<1> { static constexpr int G = 512, NSTREAM = 20; };
 template <> struct Shape<2> { static constexpr int G = 256, NSTREAM = 10; };
 
+// Hash-stream replication: a stream's base column is reduced to [0, 256) and the rows walked add offsets
+// of at least -4 and at most 259 for every grid and blur radius used (checked in grid_blur_rows).
+constexpr int QPAD = 8, QN = 528;
+
 struct WaveLds {
+    float tab[32 * 16];
+    uint32_t Q4[QN];
     uint8_t P[256];
     uint8_t col[2][256];
-    uint32_t Q4[256];
-    float tab[32 * 16];
 };
+
+// LDS byte address (offset in the workgroup's LDS) of a __shared__ object, and a 32-bit load from one
+typedef __attribute__((address_space(3))) const uint32_t lds_u32;
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+    return uint32_t(reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) const void *)p));
+}
+__device__ __forceinline__ uint32_t lds_load(uint32_t addr) {
+    return *reinterpret_cast<lds_u32 *>(uintptr_t(addr));
+}
 
 __device__ __forceinline__ float tab_at(const float *tab, int sub, uint32_t off) {
     return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(tab) + sub * 64 + off);
 }
 __device__ __forceinline__ uint32_t byte_of(uint32_t dw, int b) { return (dw >> (8 * b)) & 0xffu; }
+// One step of a sum that starts from 0: s + x, or x itself for the sum's first term when ZSIGN is off
+// (the two differ only when x is -0; see grid_blur_rows)
+template <bool ZSIGN> __device__ __forceinline__ float sum_step(float s, float x, bool first) {
+    return (!ZSIGN && first) ? x : s + x;
+}
 
 // Fills L (permutation, hashed columns, Q4, corner tables) for one side. wd: octave weight (double).
 // Ends with a block barrier: every wave of the block must call it.
@@ -49,8 +70,10 @@ __device__ void grid_setup(const SideParams &sp, WaveLds &L, double wd) {
         const int pc = L.P[c];
         L.col[0][c] = L.P[(pc + iy) & 255];
         L.col[1][c] = L.P[(pc + iy + 1) & 255];
+    }
+    for (int c = lane; c < QN; c += 64) {
         uint32_t q = 0;
-        for (int t = 0; t < 4; t++) q |= uint32_t((L.P[(c + t) & 255] & 15) * 4) << (8 * t);
+        for (int t = 0; t < 4; t++) q |= uint32_t((L.P[(c - QPAD + t) & 255] & 15) * 4) << (8 * t);
         L.Q4[c] = q;
     }
     const double ty = sp.vy;
@@ -93,13 +116,21 @@ __device__ void grid_setup(const SideParams &sp, WaveLds &L, double wd) {
     __syncthreads();
 }
 
-// Walks output rows [o_lo, o_hi] (o_hi - o_lo < G); epi(o, s[NC]) receives each blurred row segment
-// of this lane (columns NC*lane .. NC*lane+NC-1), unnormalised.
-template <int MODE, int R, class Epi>
+// Walks output rows [o_lo, o_hi] (0 <= o_lo <= o_hi < G); epi(o, s[NC]) receives each blurred row
+// segment of this lane (columns NC*lane .. NC*lane+NC-1), unnormalised.
+// ZSIGN: every sum starts from 0 as the reference's loops do. Without it (for callers that keep only the
+// maximum) a sum starts from its first term: 0 + x == x except that 0 + (-0) == +0, so values differ at
+// most in the sign of a zero, and a maximum does not.
+template <int MODE, int R, bool ZSIGN = true, class Epi>
 __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const float *kern, int o_lo, int o_hi, Epi &&epi) {
     using S = Shape<MODE>;
     constexpr int G = S::G, NC = G / 64, W = 2 * R + 1;
     static_assert(R <= NC, "halo must come from the adjacent lane only");
+    // rows j0 run from (o_lo - R) & ~1 >= (-R) & ~1 up to o_hi + R <= G - 1 + R; a stream reads row
+    // offset zoff = j0 (j0 >> 1 for the half-lattice grid) past a base column in [0, 256)
+    constexpr int ZMIN = (MODE == 1) ? (((-R) & ~1) >> 1) : ((-R) & ~1);
+    constexpr int ZMAX = (MODE == 1) ? ((G - 1 + R) >> 1) : (G - 1 + R);
+    static_assert(QPAD + ZMIN >= 0 && 255 + QPAD + ZMAX < QN, "stream rows beyond the replicated hash stream");
     const int lane = threadIdx.x & 63;
     constexpr bool HALFO = (G == 512) || (MODE == 0);
     int ix0[2], iz0[2];
@@ -107,16 +138,18 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
         ix0[par] = int(floor((HALFO ? par * 0.5 : 0.0) + sp.ox));
         iz0[par] = int(floor((HALFO ? par * 0.5 : 0.0) + sp.oz));
     }
-    int sbase[S::NSTREAM];
+    // per stream: the LDS address of its row-0 hash dword; row offset zoff adds 4 * zoff
+    uint32_t sbase[S::NSTREAM];
+    const auto q4_addr = [&](int c) { return lds_addr(&L.Q4[QPAD + ((c + iz0[0]) & 255)]); };
     if (MODE == 0) {
         for (int x = 0; x < 6; x++)
-            for (int dy = 0; dy < 2; dy++) sbase[x * 2 + dy] = int(L.col[dy][(4 * lane + x + ix0[0]) & 255]) + iz0[0];
+            for (int dy = 0; dy < 2; dy++) sbase[x * 2 + dy] = q4_addr(L.col[dy][(4 * lane + x + ix0[0]) & 255]);
     } else {
         constexpr int NP = (G == 512) ? 2 : 1;
         for (int px = 0; px < NP; px++)
             for (int m = 0; m < 5; m++)
                 for (int dy = 0; dy < 2; dy++)
-                    sbase[(px * 5 + m) * 2 + dy] = int(L.col[dy][(4 * lane + ix0[px] + m) & 255]) + iz0[0];
+                    sbase[(px * 5 + m) * 2 + dy] = q4_addr(L.col[dy][(4 * lane + ix0[px] + m) & 255]);
     }
     const uint32_t diz = uint32_t(iz0[1] - iz0[0]);
     float kw[W];
@@ -132,7 +165,8 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
         const int zoff = (MODE == 1) ? (j0 >> 1) : j0;
         uint32_t dw[S::NSTREAM];
 #pragma unroll
-        for (int s = 0; s < S::NSTREAM; s++) dw[s] = L.Q4[(sbase[s] + zoff) & 255];
+        for (int s = 0; s < S::NSTREAM; s++)
+            dw[s] = lds_load(sbase[s] + uint32_t(4 * zoff));
 #pragma unroll
         for (int t = 0; t < 2; t++) {
             const int j = j0 + t;
@@ -147,7 +181,8 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
                         for (int dy = 0; dy < 2; dy++)
 #pragma unroll
                             for (int u = 0; u < 3; u++)
-                                acc += tab_at(tab, (u * 3 + v) * 2 + dy, byte_of(dw[(q + u) * 2 + dy], t + v));
+                                acc = sum_step<ZSIGN>(acc, tab_at(tab, (u * 3 + v) * 2 + dy, byte_of(dw[(q + u) * 2 + dy], t + v)),
+                                                      v == 0 && dy == 0 && u == 0);
                     V[q] = acc;
                 }
             } else if (MODE == 1) {
@@ -162,7 +197,7 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
                         const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
                         const uint32_t d = dw[(px * 5 + mq + dx) * 2 + dy];
                         const uint32_t off = (d >> (8u * (uint32_t(dz) + zb))) & 0xffu;
-                        acc += tab_at(tab, (px * 2 + pz) * 8 + c, off);
+                        acc = sum_step<ZSIGN>(acc, tab_at(tab, (px * 2 + pz) * 8 + c, off), c == 0);
                     }
                     V[q] = acc;
                 }
@@ -173,7 +208,7 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
 #pragma unroll
                     for (int c = 0; c < 8; c++) {
                         const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-                        acc += tab_at(tab, c, byte_of(dw[(q + dx) * 2 + dy], t + dz));
+                        acc = sum_step<ZSIGN>(acc, tab_at(tab, c, byte_of(dw[(q + dx) * 2 + dy], t + dz)), c == 0);
                     }
                     V[q] = acc;
                 }
@@ -194,7 +229,7 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
             for (int q = 0; q < NC; q++) {
                 float s = 0;
 #pragma unroll
-                for (int d = 0; d < W; d++) s += kw[d] * Wv[q + d];
+                for (int d = 0; d < W; d++) s = sum_step<ZSIGN>(s, kw[d] * Wv[q + d], d == 0);
                 ring[W - 1][q] = s;
             }
             const int o = j - R;
@@ -204,7 +239,7 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
                 for (int q = 0; q < NC; q++) {
                     float s = 0;
 #pragma unroll
-                    for (int d = 0; d < W; d++) s += kw[d] * ring[d][q];
+                    for (int d = 0; d < W; d++) s = sum_step<ZSIGN>(s, kw[d] * ring[d][q], d == 0);
                     out[q] = s;
                 }
                 epi(o, out);
