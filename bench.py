@@ -31,9 +31,10 @@ Also reported:
                 post_gate: the same CPU pipeline in stdin=1 mode on exactly the seeds the GPU gate passes
                 over the 1e7-index range [1e12, 1e12 + 1e7), so both sides run the same gated seeds after the gate.
   gate_only     the GPU gate alone over 2^24 seed indices (the gate kernel workload), median of 3 calls,
-                with the gate replayed as one hipGraph per chunk (graph_on_s, the default; 2^24 indices here) and
-                stage by stage with a survivor-count read-back after each stage (graph_off_s), on the
-                same rows (rows_identical).
+                with the gate replayed as one hipGraph per chunk (graph_on_s, the default; 2^24 indices here;
+                its pre-gate decides from the fused-blur amplitude where that is provably the exact
+                decision) and stage by stage in the reference's arithmetic with a survivor-count read-back
+                after each stage (graph_off_s), on the same rows (rows_identical).
   tuning        the autotuner's settings (desert-search_amd/tuned.json, applied by the library) against the
                 library defaults on the same timed steps, each in fresh processes, alternating, median of 2:
                 gated seeds/s of each and whether their output rows are identical.

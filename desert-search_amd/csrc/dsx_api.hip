@@ -183,7 +183,7 @@ static int ctx_alloc(dsx_ctx *c) {
     HIPCK(hipMalloc(&c->d_k3, n1 * 4));
     HIPCK(hipMalloc(&c->d_a3, n1 * 4));
     HIPCK(hipMalloc(&c->d_all, n1 * 4));
-    HIPCK(hipMalloc(&c->d_cnt, 16));
+    HIPCK(hipMalloc(&c->d_cnt, dsx::GATE_NCNT * 4));
     HIPCK(hipMalloc(&c->d_sides, 2 * n1 * sizeof(dsx::SideParams)));
     HIPCK(hipMalloc(&c->d_seeds, n1 * 8));
     HIPCK(hipHostMalloc(&c->h_cnt, 256, hipHostMallocDefault));
@@ -261,7 +261,7 @@ int gate_eager(dsx_ctx *c, uint64_t start, uint64_t off, uint64_t count, const d
             dsx::launch_init(c->st, start + base, c->d_k1, nullptr, n1, c->d_sides);
             HIPCK(hipGetLastError());
             HIPCK(hipEventRecord(c->ev[3], c->st));
-            dsx::launch_amp(c->st, mode_pre, c->d_sides, nullptr, n1, kpre, thr_pre, nullptr, c->d_i2, nullptr,
+            dsx::launch_amp(c->st, mode_pre, dsx::GATE_EXACT, c->d_sides, nullptr, n1, kpre, thr_pre, nullptr, c->d_i2, nullptr,
                             c->d_cnt + 1, c->cap1, nullptr);
             HIPCK(hipGetLastError());
             HIPCK(hipEventRecord(c->ev[4], c->st));
@@ -276,7 +276,7 @@ int gate_eager(dsx_ctx *c, uint64_t start, uint64_t off, uint64_t count, const d
             c->prof_work[2] += n1;
             if (n2) {
                 HIPCK(hipEventRecord(c->ev[5], c->st));
-                dsx::launch_amp(c->st, 1, c->d_sides, c->d_i2, n2, kfull, double(p->amp), c->d_k1, c->d_k3, c->d_a3, c->d_cnt + 2,
+                dsx::launch_amp(c->st, 1, dsx::GATE_EXACT, c->d_sides, c->d_i2, n2, kfull, double(p->amp), c->d_k1, c->d_k3, c->d_a3, c->d_cnt + 2,
                                 c->cap1, nullptr);
                 HIPCK(hipGetLastError());
                 HIPCK(hipEventRecord(c->ev[6], c->st));
@@ -333,7 +333,9 @@ int gate_graph_range(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_
     dsx::GateGraph &G = c->ggraph;
     const bool same = G.exec && G.C.grid == grid && G.nmax == nmax && std::memcmp(&G.S, &S, sizeof S) == 0;
     if (!same) {
-        const dsx::GateChain C{c->d_cnt, c->d_k1, c->d_i2, c->d_k3, c->d_a3, c->d_sides, grid};
+        // the fast chain: decisions from the fast amplitude where provably equal, the reference's arithmetic
+        // for the rest and for every amplitude written
+        const dsx::GateChain C{c->d_cnt, c->d_k1, c->d_i2, c->d_k3, c->d_a3, c->d_sides, grid, true};
         HIPCK(dsx::gate_graph_build(G, C, S, nmax, dsx::GateOut{nullptr, h_amps, h_kidx, c->cap1, h_counts}));
     }
     for (uint64_t base = 0; base < count; base += nmax) {
@@ -404,7 +406,7 @@ int dsx_gate_cpu(dsx_ctx *c, uint64_t start, uint64_t count, double max_phase, d
         }
         if (n1) {
             dsx::launch_init(c->st, start + base, c->d_k1, nullptr, n1, c->d_sides);
-            dsx::launch_amp(c->st, 2, c->d_sides, nullptr, n1, K, thr, c->d_k1, c->d_k3, c->d_a3, c->d_cnt + 1, c->cap1,
+            dsx::launch_amp(c->st, 2, dsx::GATE_EXACT, c->d_sides, nullptr, n1, K, thr, c->d_k1, c->d_k3, c->d_a3, c->d_cnt + 1, c->cap1,
                             nullptr);
             HIPCK(hipGetLastError());
             HIPCK(hipMemcpyAsync(c->h_cnt + 1, c->d_cnt + 1, 4, hipMemcpyDeviceToHost, c->st));
@@ -445,13 +447,13 @@ int dsx_gate_amps(dsx_ctx *c, const uint64_t *seeds, uint64_t n, const dsx_gate_
         dsx::launch_init(c->st, 0, nullptr, c->d_seeds, m, c->d_sides);
         HIPCK(hipGetLastError());
         if (pre) {
-            dsx::launch_amp(c->st, mode_pre, c->d_sides, nullptr, m, kpre, 1e300, nullptr, c->d_i2, nullptr, c->d_cnt,
+            dsx::launch_amp(c->st, mode_pre, dsx::GATE_EXACT, c->d_sides, nullptr, m, kpre, 1e300, nullptr, c->d_i2, nullptr, c->d_cnt,
                             c->cap1, c->d_all);
             HIPCK(hipGetLastError());
             HIPCK(hipMemcpyAsync(pre + b, c->d_all, size_t(m) * 4, hipMemcpyDeviceToHost, c->st));
         }
         if (full) {
-            dsx::launch_amp(c->st, 1, c->d_sides, nullptr, m, kfull, 1e300, nullptr, c->d_i2, nullptr, c->d_cnt + 1,
+            dsx::launch_amp(c->st, 1, dsx::GATE_EXACT, c->d_sides, nullptr, m, kfull, 1e300, nullptr, c->d_i2, nullptr, c->d_cnt + 1,
                             c->cap1, c->d_a3);
             HIPCK(hipGetLastError());
             HIPCK(hipMemcpyAsync(full + b, c->d_a3, size_t(m) * 4, hipMemcpyDeviceToHost, c->st));

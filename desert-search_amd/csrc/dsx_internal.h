@@ -29,12 +29,17 @@ struct GateKernel {
 // counters cleared -> phase filter -> the count the later stages take (all phase survivors, or none when they
 // exceed `grid`) -> Perlin init -> pre-gate -> full gate -> survivors rank-sorted by seed index into `out`.
 // The grids of the later stages are sized for `grid` phase survivors; their kernels read the live counts.
+// With fast set, the pre-gate first decides with the fast amplitude (FMA blurs, provably the same decisions)
+// and runs the reference's arithmetic only where a decision is within its error bound (gate_launches in
+// gate.hip); every amplitude output is the reference's.
+constexpr int GATE_NCNT = 8;   // counters: phase, pre, full, taken, pre undecided
 struct GateChain {
-    uint32_t *cnt;        // 4 counters: phase, pre, full, taken
+    uint32_t *cnt;        // GATE_NCNT counters
     uint32_t *k1, *i2, *k3;
     float *a3;
     SideParams *sides;    // 2 * grid
     uint32_t grid;        // phase survivors the chain holds
+    bool fast = false;
 };
 struct GateOut {          // device or pinned host memory the sort kernel writes
     uint64_t *seeds;      // may be null
@@ -71,8 +76,24 @@ void launch_phase(hipStream_t st, uint64_t base, uint32_t n, double maxph, uint3
 // still covers n items, so a caller that does not know the count on the host launches with an upper bound.
 void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64_t *seeds, uint32_t n, SideParams *out,
                  const uint32_t *n_dev = nullptr);
-void launch_amp(hipStream_t st, int mode, const SideParams *sides, const uint32_t *sel, uint32_t n, const GateKernel &K,
-                double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count, uint32_t cap,
-                float *all_amp, const uint32_t *n_dev = nullptr);
+// Amplitude kernel k_amp<mode, dec> (gate.hip): dec = GATE_EXACT, or GATE_SPLIT (modes 0 and 2), whose
+// undecided list is und_idx / und_count.
+enum { GATE_EXACT = 0, GATE_SPLIT = 1 };
+void launch_amp(hipStream_t st, int mode, int dec, const SideParams *sides, const uint32_t *sel, uint32_t n,
+                const GateKernel &K, double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count,
+                uint32_t cap, float *all_amp, const uint32_t *n_dev = nullptr, uint32_t *und_idx = nullptr,
+                uint32_t *und_count = nullptr);
+// One amplitude stage of a chain: k_amp<mode, dec> over sel (null: all phase survivors), its count in *ndev,
+// with the pre-gate (pre) or full-gate kernel and threshold.
+struct GateLaunch {
+    int mode, dec;
+    bool pre;
+    const uint32_t *sel, *kmap;
+    uint32_t *out;
+    float *amp;
+    uint32_t *count;
+    const uint32_t *ndev;
+    uint32_t *und, *und_count;
+};
 
 }  // namespace dsx

@@ -17,6 +17,7 @@
 //     and hash streams live in LDS, laid out so every table read of a wave hits one 16-dword
 //     sub-table (conflict-free).
 #include "grid_blur.h"
+#include <vector>
 
 namespace dsx {
 
@@ -92,13 +93,13 @@ __global__ __launch_bounds__(64) void k_init(uint64_t base, const uint32_t *ks, 
 // 256^2 even-parity pre-gate (R=4). Grid arithmetic and blur: grid_blur.h.
 template <int MODE> struct GateR { static constexpr int R = (MODE == 1) ? 7 : 4; };
 
-template <int MODE>
+template <int MODE, bool FAST>
 __device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K) {
     grid_setup<MODE>(sp, L, K.wd);
     float best = -1e9f;
     constexpr int NC = Shape<MODE>::G / 64;
     // only the maximum is kept, so sums may start from their first term (ZSIGN off)
-    grid_blur_rows<MODE, GateR<MODE>::R, false>(sp, L, K.k, 0, Shape<MODE>::G - 1, [&](int, const float (&o)[NC]) {
+    grid_blur_rows<MODE, GateR<MODE>::R, false, FAST>(sp, L, K.k, 0, Shape<MODE>::G - 1, [&](int, const float (&o)[NC]) {
 #pragma unroll
         for (int q = 0; q < NC; q++) best = fmaxf(best, o[q]);
     });
@@ -108,34 +109,83 @@ __device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K)
     return best * K.norm;
 }
 
-// One 128-thread block per seed: wave 0 = side A, wave 1 = side B. sel (optional) maps block -> side
-// pair index. Passing seeds are appended as (kmap ? kmap[pair] : pair, amplitude); all_amp (optional)
-// gets every amplitude, indexed by block. n_dev (optional): the number of seeds is min(n, *n_dev), read
-// on the device (the grid covers n).
+// A bound on |grid value| of one side (the corner tables must be set up): a value sums one entry of each
+// sub-table of its cell's set (MODE 0: all 18; MODE 1: the 8 of one of the 4 parities; MODE 2: all 8), so
+// the sum of those sub-tables' largest |entry| bounds it, up to the float roundings of both sums (covered
+// by the margin of gate_eps).
 template <int MODE>
+__device__ float side_bound(const WaveLds &L) {
+    constexpr int NSUB = MODE == 0 ? 18 : MODE == 1 ? 32 : 8;
+    const int lane = threadIdx.x & 63;
+    float m = 0.f;
+    if (lane < NSUB)
+        for (int h = 0; h < 16; h++) m = fmaxf(m, fabsf(L.tab[lane * 16 + h]));
+    // sum over groups of 8 lanes (one parity's corner terms in MODE 1), then over (MODE 0, 2) or the
+    // maximum of (MODE 1) the groups; lanes >= NSUB hold 0
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) m += __shfl_xor(m, o);
+#pragma unroll
+    for (int o = 8; o < 64; o <<= 1) m = MODE == 1 ? fmaxf(m, __shfl_xor(m, o)) : m + __shfl_xor(m, o);
+    return m;
+}
+
+// Decision margin of the fast amplitude (FMA blurs) for sides bounded by bA, bB. Each blur pass of W <= 15
+// taps, fused or not, is within gamma_15 * sum_d k_d |input| of its exact value (gamma_15 = 15u / (1 - 15u)
+// <= 9e-7, u = 2^-24), so the fused and the reference result of the two passes differ by at most
+// 4 gamma_15 * ksum^2 * b before the factor norm = 1 / ksum^2, i.e. 3.6e-6 * b per side; the maximum over
+// cells moves by no more than the largest difference, and the roundings of `* norm` and of the sum of the
+// two sides add at most 4u |amp| < 1e-6 for |amp| < 4. 2e-5 * (bA + bB) + 1e-5 covers this 5 times over.
+__device__ __forceinline__ double gate_eps(float bA, float bB) { return 2e-5 * (double(bA) + double(bB)) + 1e-5; }
+
+// One 128-thread block per seed: wave 0 = side A, wave 1 = side B. sel (optional) maps block -> side
+// pair index. n_dev (optional): the number of seeds is min(n, *n_dev), read on the device (the grid covers n).
+// DEC selects what is decided and written:
+//   GATE_EXACT  the reference's arithmetic: seeds with amplitude >= thr are appended as
+//               (kmap ? kmap[pair] : pair, amplitude); all_amp (optional) gets every amplitude, by block
+//   GATE_SPLIT  the fast amplitude (FMA blurs): pairs of seeds certainly at or above thr (amp - eps >= thr)
+//               to out_idx, those within eps of thr to und_idx (to be decided by GATE_EXACT)
+// The fast decisions equal the exact ones: the two amplitudes differ by less than eps (gate_eps).
+template <int MODE, int DEC>
 __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint32_t *sel, uint32_t n, GateKernel K,
                                              double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp,
-                                             uint32_t *count, uint32_t cap, float *all_amp, const uint32_t *n_dev) {
+                                             uint32_t *count, uint32_t cap, float *all_amp, const uint32_t *n_dev,
+                                             uint32_t *und_idx, uint32_t *und_count) {
     __shared__ WaveLds L[2];
-    __shared__ float red[2];
+    __shared__ float red[2], bnd[2];
     const uint32_t item = blockIdx.x;
     if (n_dev) n = min(n, *n_dev);
     if (item >= n) return;
     const uint32_t pidx = sel ? sel[item] : item;
     const int wv = threadIdx.x >> 6;
-    const float m = side_max<MODE>(sides[2 * size_t(pidx) + wv], L[wv], K);
-    if ((threadIdx.x & 63) == 0) red[wv] = m;
+    const float m = side_max<MODE, DEC != GATE_EXACT>(sides[2 * size_t(pidx) + wv], L[wv], K);
+    float b = 0.f;
+    if (DEC != GATE_EXACT) b = side_bound<MODE>(L[wv]);
+    if ((threadIdx.x & 63) == 0) {
+        red[wv] = m;
+        bnd[wv] = b;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         float amp = 0;
         amp += red[0];
         amp += red[1];
-        if (all_amp) all_amp[item] = amp;
-        if (double(amp) >= thr) {
-            const uint32_t i = atomicAdd(count, 1u);
-            if (i < cap) {
-                out_idx[i] = kmap ? kmap[pidx] : pidx;
-                if (out_amp) out_amp[i] = amp;
+        if (DEC == GATE_EXACT) {
+            if (all_amp) all_amp[item] = amp;
+            if (double(amp) >= thr) {
+                const uint32_t i = atomicAdd(count, 1u);
+                if (i < cap) {
+                    out_idx[i] = kmap ? kmap[pidx] : pidx;
+                    if (out_amp) out_amp[i] = amp;
+                }
+            }
+        } else {
+            const double eps = gate_eps(bnd[0], bnd[1]);
+            if (double(amp) - eps >= thr) {
+                const uint32_t i = atomicAdd(count, 1u);
+                if (i < cap) out_idx[i] = pidx;
+            } else if (double(amp) + eps >= thr) {
+                const uint32_t i = atomicAdd(und_count, 1u);
+                if (i < cap) und_idx[i] = pidx;
             }
         }
     }
@@ -143,7 +193,7 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
 
 // ------------------------------------------------------------------ the gate as one chain without host round trips
 __global__ void k_gate_clear(uint32_t *cnt) {
-    if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+    if (threadIdx.x < GATE_NCNT) cnt[threadIdx.x] = 0;
 }
 
 // The number of phase survivors the later stages take: all, or none if they overflow the chain's grid.
@@ -172,6 +222,33 @@ __global__ __launch_bounds__(256) void k_gate_sort(const uint32_t *cnt, const ui
     if (O.kidx) O.kidx[rank] = key;
 }
 
+// The amplitude launches of a chain: pre-gate over the phase survivors, full gate over its survivors. In the
+// fast chain (C.fast) the pre-gate decides with the fast amplitude (GATE_SPLIT) and the exact pre-gate
+// decides the undecided pairs (listed in k3, which only the full gate writes later). The full gate always
+// runs the reference's arithmetic: it outputs the amplitudes, and a fast filter before it (measured: 15 %
+// faster than the exact kernel, whose re-run of the ~16 % kept seeds then costs more) does not pay.
+std::vector<GateLaunch> gate_launches(const GateChain &C, const GateStages &S) {
+    const int mp = S.mode_pre;
+    std::vector<GateLaunch> v;
+    if (C.fast) {
+        v.push_back({mp, GATE_SPLIT, true, nullptr, nullptr, C.i2, nullptr, C.cnt + 1, C.cnt + 3, C.k3, C.cnt + 4});
+        v.push_back({mp, GATE_EXACT, true, C.k3, nullptr, C.i2, nullptr, C.cnt + 1, C.cnt + 4, nullptr, nullptr});
+    } else {
+        v.push_back({mp, GATE_EXACT, true, nullptr, nullptr, C.i2, nullptr, C.cnt + 1, C.cnt + 3, nullptr, nullptr});
+    }
+    v.push_back({1, GATE_EXACT, false, C.i2, C.k1, C.k3, C.a3, C.cnt + 2, C.cnt + 1, nullptr, nullptr});
+    return v;
+}
+
+const void *amp_kernel(int mode, int dec) {
+    if (dec == GATE_SPLIT)
+        return mode == 0 ? reinterpret_cast<const void *>(&k_amp<0, GATE_SPLIT>)
+                         : reinterpret_cast<const void *>(&k_amp<2, GATE_SPLIT>);
+    return mode == 0 ? reinterpret_cast<const void *>(&k_amp<0, GATE_EXACT>)
+           : mode == 1 ? reinterpret_cast<const void *>(&k_amp<1, GATE_EXACT>)
+                       : reinterpret_cast<const void *>(&k_amp<2, GATE_EXACT>);
+}
+
 hipError_t enqueue_gate_chain(hipStream_t st, const GateChain &C, const GateStages &S, uint64_t start, uint32_t count,
                               const GateOut &O) {
     hipLaunchKernelGGL(k_gate_clear, dim3(1), dim3(64), 0, st, C.cnt);
@@ -179,9 +256,9 @@ hipError_t enqueue_gate_chain(hipStream_t st, const GateChain &C, const GateStag
         launch_phase(st, start, count, S.maxph, C.k1, C.cnt, C.grid);
         hipLaunchKernelGGL(k_gate_clamp, dim3(1), dim3(64), 0, st, C.cnt, C.grid);
         launch_init(st, start, C.k1, nullptr, C.grid, C.sides, C.cnt + 3);
-        launch_amp(st, S.mode_pre, C.sides, nullptr, C.grid, S.kpre, S.thr_pre, nullptr, C.i2, nullptr, C.cnt + 1, C.grid,
-                   nullptr, C.cnt + 3);
-        launch_amp(st, 1, C.sides, C.i2, C.grid, S.kfull, S.amp, C.k1, C.k3, C.a3, C.cnt + 2, C.grid, nullptr, C.cnt + 1);
+        for (const GateLaunch &g : gate_launches(C, S))
+            launch_amp(st, g.mode, g.dec, C.sides, g.sel, C.grid, g.pre ? S.kpre : S.kfull, g.pre ? S.thr_pre : S.amp,
+                       g.kmap, g.out, g.amp, g.count, C.grid, nullptr, g.ndev, g.und, g.und_count);
     }
     hipLaunchKernelGGL(k_gate_sort, dim3((C.grid + 255) / 256), dim3(256), 0, st, C.cnt, C.k3, C.a3, start, C.grid, O);
     return hipGetLastError();
@@ -253,24 +330,16 @@ hipError_t gate_graph_build(GateGraph &G, const GateChain &C, const GateStages &
                      &G.n_init)))
             return e;
     }
-    for (int stage = 0; stage < 2; stage++) {   // pre-gate, then full gate
+    for (const GateLaunch &g : gate_launches(G.C, S)) {   // pre-gate, then full gate
         const SideParams *sides = G.C.sides;
-        const uint32_t *sel = stage ? G.C.i2 : nullptr;
+        const uint32_t *sel = g.sel, *kmap = g.kmap, *ndev = g.ndev;
         uint32_t n = G.C.grid, cap = G.C.grid;
-        GateKernel K = stage ? S.kfull : S.kpre;
-        double thr = stage ? S.amp : S.thr_pre;
-        const uint32_t *kmap = stage ? G.C.k1 : nullptr;
-        uint32_t *oidx = stage ? G.C.k3 : G.C.i2;
-        float *oamp = stage ? G.C.a3 : nullptr;
-        uint32_t *count = cnt + 1 + stage;
-        float *all = nullptr;
-        const uint32_t *ndev = stage ? cnt + 1 : cnt + 3;
-        void *a[] = {&sides, &sel, &n, &K, &thr, &kmap, &oidx, &oamp, &count, &cap, &all, &ndev};
-        const int mode = stage ? 1 : S.mode_pre;
-        const void *f = mode == 0 ? reinterpret_cast<const void *>(&k_amp<0>)
-                        : mode == 1 ? reinterpret_cast<const void *>(&k_amp<1>)
-                                    : reinterpret_cast<const void *>(&k_amp<2>);
-        if ((e = add(kparams(f, dim3(G.C.grid), dim3(128), a), nullptr))) return e;
+        GateKernel K = g.pre ? S.kpre : S.kfull;
+        double thr = g.pre ? S.thr_pre : S.amp;
+        uint32_t *oidx = g.out, *count = g.count, *und = g.und, *und_count = g.und_count;
+        float *oamp = g.amp, *all = nullptr;
+        void *a[] = {&sides, &sel, &n, &K, &thr, &kmap, &oidx, &oamp, &count, &cap, &all, &ndev, &und, &und_count};
+        if ((e = add(kparams(amp_kernel(g.mode, g.dec), dim3(G.C.grid), dim3(128), a), nullptr))) return e;
     }
     {   // rank sort into the outputs
         const uint32_t *cc = cnt, *k3 = G.C.k3;
@@ -341,19 +410,13 @@ void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64
     if (n == 0) return;
     hipLaunchKernelGGL(k_init, dim3((2 * n + 63) / 64), dim3(64), 0, st, base, ks, seeds, n, out, n_dev);
 }
-void launch_amp(hipStream_t st, int mode, const SideParams *sides, const uint32_t *sel, uint32_t n, const GateKernel &K,
-                double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count, uint32_t cap,
-                float *all_amp, const uint32_t *n_dev) {
+void launch_amp(hipStream_t st, int mode, int dec, const SideParams *sides, const uint32_t *sel, uint32_t n,
+                const GateKernel &K, double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count,
+                uint32_t cap, float *all_amp, const uint32_t *n_dev, uint32_t *und_idx, uint32_t *und_count) {
     if (n == 0) return;
-    if (mode == 0)
-        hipLaunchKernelGGL(k_amp<0>, dim3(n), dim3(128), 0, st, sides, sel, n, K, thr, kmap, out_idx, out_amp, count, cap,
-                           all_amp, n_dev);
-    else if (mode == 1)
-        hipLaunchKernelGGL(k_amp<1>, dim3(n), dim3(128), 0, st, sides, sel, n, K, thr, kmap, out_idx, out_amp, count, cap,
-                           all_amp, n_dev);
-    else
-        hipLaunchKernelGGL(k_amp<2>, dim3(n), dim3(128), 0, st, sides, sel, n, K, thr, kmap, out_idx, out_amp, count, cap,
-                           all_amp, n_dev);
+    void *a[] = {&sides, &sel, &n, const_cast<GateKernel *>(&K), &thr, &kmap, &out_idx, &out_amp, &count, &cap, &all_amp,
+                 &n_dev, &und_idx, &und_count};
+    (void)hipLaunchKernel(amp_kernel(mode, dec), dim3(n), dim3(128), a, 0, st);
 }
 
 }  // namespace dsx

@@ -103,8 +103,11 @@ int dsx_gate_noalloc(int device, void *stream, uint64_t start, uint64_t count, c
  * hipGraph of the whole gate (the stages read their survivor counts on the device; one synchronisation per
  * chunk), built once per context and shape: chunks are the largest power of two up to 2^26 indices whose
  * expected phase survivors fit the survivor buffers (2^24 at max_phase 0.06), or the smallest holding the
- * call; on = 0 launches the stages one by one and reads each stage's survivor count back first
- * (dsx_gate_profile times each kernel only then). Results are identical. */
+ * call. The graph's pre-gate decides a seed from a faster amplitude (fused multiply-add blurs) when that
+ * amplitude is farther from the threshold than its proven error bound, and from the reference's arithmetic
+ * otherwise, so every decision is the reference's; the full gate and every amplitude returned use the
+ * reference's arithmetic. on = 0 launches the stages one by one in the reference's arithmetic and reads each
+ * stage's survivor count back first (dsx_gate_profile times each kernel only then). Results are identical. */
 int dsx_set_gate_graph(dsx_ctx *ctx, int on);
 
 /* Device time (ms) of the kernels of the last dsx_gate call, summed over its batches (with the graph off):

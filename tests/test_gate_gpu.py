@@ -166,3 +166,49 @@ def test_gate_graph_equals_eager(engine):
                     assert g[3][0] > 0.7 * count   # beyond the graph grid's 0.625 of a chunk + 4096
     finally:
         engine.set_gate_graph(1)
+
+
+def test_fast_decisions_at_exact_thresholds(engine):
+    """The graph chain decides a seed with the fast amplitude (FMA blurs) only when that amplitude is
+    farther from the threshold than its proven error bound, and runs the reference's arithmetic for the
+    rest and for every amplitude it outputs. Thresholds set exactly to a surviving seed's pre-gate or full
+    amplitude (amp - thr == 0, which the fast amplitude can never decide) keep that seed; the next float
+    above removes it. Each case: the graph chain equals the stage-by-stage exact launches bit for bit, and
+    the boundary seed is in or out as its explicit amplitudes (dsx_gate_amps) say. Pre-gate boundaries in
+    the anti-aliased (box=1, pthr) and legacy (box=0, amp - margin: a Sterbenz-exact margin) modes; full
+    gate boundaries through amp (box=1)."""
+    from conftest import load_package
+    dsx = load_package().dsx
+    start, count = 10 ** 12, 1 << 21
+    up = lambda x: float(np.nextafter(np.float32(x), np.float32(np.inf)))
+    try:
+        for box in (1, 0):
+            base = dsx.GateParams(box=box)
+            engine.set_gate_graph(0)
+            seeds, amps, _, _ = engine.gate(start, count, base)
+            pre, full = engine.gate_amps(seeds, base)
+            assert np.array_equal(full.view(np.uint32), amps.view(np.uint32))
+            i, j = int(np.argmin(pre)), int(np.argmin(full))
+            cases = []
+            for t, keep in ((float(pre[i]), True), (up(pre[i]), False)):
+                if box:
+                    cases.append((dsx.GateParams(box=1, pthr=t), seeds[i], keep))
+                else:
+                    m = np.float32(base.amp) - np.float32(t)   # exact (Sterbenz): amp - margin == t in float
+                    assert np.float32(np.float32(base.amp) - m) == np.float32(t)
+                    cases.append((dsx.GateParams(box=0, margin=float(m)), seeds[i], keep))
+            if box:
+                cases += [(dsx.GateParams(box=1, amp=float(full[j])), seeds[j], True),
+                          (dsx.GateParams(box=1, amp=up(full[j])), seeds[j], False)]
+            for p, s, keep in cases:
+                engine.set_gate_graph(1)
+                g = engine.gate(start, count, p)
+                engine.set_gate_graph(0)
+                e = engine.gate(start, count, p)
+                print(f"boundary box={box} pthr={p.pthr!r} margin={p.margin!r} amp={p.amp!r}: counts {g[3]} / {e[3]}",
+                      flush=True)
+                assert g[3] == e[3]
+                assert np.array_equal(g[0], e[0]) and np.array_equal(g[1].view(np.uint32), e[1].view(np.uint32))
+                assert (int(s) in set(g[0].tolist())) == keep
+    finally:
+        engine.set_gate_graph(1)
