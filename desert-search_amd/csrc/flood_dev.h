@@ -46,6 +46,24 @@ __device__ __forceinline__ void mask_head_store(TileMask &m, uint32_t job, int32
     ast(&m.tx, tx);
     ast(&m.tz, tz);
 }
+// Exported mask k of a batch, written by one wave (lane < 32: its row): into the device list (k < mcap) and,
+// when the batch asks for it, into pinned host memory as well (k < B.h_mcap).
+__device__ __forceinline__ void mask_export(const FloodBufs &B, TileMask *masks, uint32_t mcap, uint32_t k, int lane,
+                                            uint32_t row, uint32_t job, int32_t tx, int32_t tz) {
+    if (k < mcap) {
+        if (lane < 32) ast(&masks[k].rows[lane], row);
+        if (lane == 0) mask_head_store(masks[k], job, tx, tz);
+    }
+    if (B.h_masks && k < B.h_mcap) {
+        TileMask &h = B.h_masks[k];
+        if (lane < 32) h.rows[lane] = row;
+        if (lane == 0) {
+            h.job = job;
+            h.tx = tx;
+            h.tz = tz;
+        }
+    }
+}
 
 constexpr unsigned long long HEMPTY = ~0ull;
 __device__ __forceinline__ unsigned long long hkey(uint32_t job, int tx, int tz) {

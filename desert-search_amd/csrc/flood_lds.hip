@@ -223,9 +223,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(const FloodJob *jobs, const ui
             uint32_t k = 0;
             if (lane == 0) k = atomicAdd(mcount, 1u);
             k = __shfl(k, 0);
-            if (k >= mcap) continue;
-            if (lane < 32) ast(&masks[k].rows[lane], row);
-            if (lane == 0) mask_head_store(masks[k], job, tx, tz);
+            mask_export(B, masks, mcap, k, lane, row, job, tx, tz);
         }
 }
 

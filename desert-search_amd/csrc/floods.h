@@ -58,6 +58,8 @@ struct FloodBufs {
     uint32_t *ctl;                // this batch's control words (zero when it starts)
     uint32_t *ctl_next;           // the control words of the batch two after this one (zeroed by this one)
     uint32_t seq;                 // batch sequence number (for the time-limit diagnostics)
+    struct TileMask *h_masks;     // when set: exported masks are also written here as they are made (pinned,
+    uint32_t h_mcap;              // at most h_mcap), for batches whose publishing copies no masks
 };
 
 // ctl words: queue head/tail, pending items, overflow flags, exported masks, items processed, floods
@@ -67,11 +69,13 @@ struct FloodBufs {
 // sub-blocks they needed (before speculative ones are added); with profiling on, the workgroups' time
 // (100 MHz ticks) waiting for queue items, evaluating sub-blocks, loading octaves into LDS, and alive;
 // then the batch number (plus one) the stage kernel wrote, the one the tiled engine's first workgroup
-// was given, the queue tail and the stage's batch number that workgroup read when it started
+// was given, the queue tail and the stage's batch number that workgroup read when it started; the
+// accumulate kernel's workgroups that have finished
 constexpr int Q_HEAD = 0, Q_TAIL = 1, Q_PENDING = 2, FLOOD_OVF = 3, FLOOD_MASKS = 4, Q_ITEMS = 5, HANDED = 6,
               CTL_NTILES = 7, TIME_SEEN = 8, TIME_SLOT = 9, TIME_HEAD = 10, TIME_TAIL = 11, TIME_PENDING = 12,
               TIME_ITEM = 13, EVAL_ROUNDS = 14, EVAL_NEED = 15, PROF_WAIT = 16, PROF_EVAL = 17, PROF_LOAD = 18,
-              PROF_LIFE = 19, STAGE_SEQ = 20, RUN_SEQ = 21, RUN_TAIL0 = 22, RUN_STAGE_SEQ = 23, CTL_WORDS = 24;
+              PROF_LIFE = 19, STAGE_SEQ = 20, RUN_SEQ = 21, RUN_TAIL0 = 22, RUN_STAGE_SEQ = 23, ACC_DONE = 24,
+              CTL_WORDS = 25;
 constexpr uint32_t Q_EMPTY = 0xffffffffu;
 // FLOOD_OVF bits: 1 work queue full, 2 tile pool or hash table full, 4 time limit reached
 constexpr uint32_t OVF_QUEUE = 1, OVF_TILES = 2, OVF_TIME = 4;
@@ -105,6 +109,10 @@ void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRe
 // stream synchronisation ends the batch, and zero the next batch's control words.
 void launch_flood_publish(hipStream_t st, const FloodRes *res, uint32_t n, FloodBufs B, const struct TileMask *masks,
                           FloodRes *h_res, uint32_t *h_ctl, struct TileMask *h_masks, uint32_t hmcap);
+// Both in one kernel: the accumulation's last workgroup to finish publishes.
+void launch_flood_acc_publish(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRes *res, struct TileMask *masks,
+                              uint32_t mcap, uint32_t blocks, uint32_t n, FloodRes *h_res, uint32_t *h_ctl,
+                              struct TileMask *h_masks, uint32_t hmcap);
 // One wave per flood with its state in LDS over a local sub-window of at most LDS_MAX_N^2 points
 // around the start; a flood that outgrows it is handed to the tiled engine (its result reset and its
 // start tile queued). cls: LDS_HOT (the T test, any window), LDS_PROXY (proxy predicate, any window),

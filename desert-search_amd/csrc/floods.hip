@@ -510,10 +510,7 @@ __device__ void acc_tile(const FloodJob *jobs, FloodBufs B, uint32_t b, FloodRes
         uint32_t k = 0;
         if (lane == 0) k = atomicAdd(mcount, 1u);
         k = __shfl(k, 0);
-        if (k < mcap) {
-            if (lane < FT) ast(&masks[k].rows[lane], RC);
-            if (lane == 0) mask_head_store(masks[k], job, tx, tz);
-        }
+        mask_export(B, masks, mcap, k, lane, RC, job, tx, tz);
     }
     // leave the tile and its hash slot clean for the next call (everything above has read them)
     __builtin_amdgcn_wave_barrier();
@@ -540,9 +537,11 @@ __global__ __launch_bounds__(64) void k_flood_acc(const FloodJob *jobs, FloodBuf
     for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) acc_tile(jobs, B, b, res, masks, &B.ctl[FLOOD_MASKS], mcap);
 }
 
-__global__ __launch_bounds__(256) void k_flood_publish(const FloodRes *res, uint32_t n, FloodBufs B, const TileMask *masks,
-                                                       FloodRes *h_res, uint32_t *h_ctl, TileMask *h_masks, uint32_t hmcap) {
-    const uint32_t nt = blockDim.x * gridDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
+// Results, control words and (masks non-null) masks into pinned memory (t0 of nt threads), the control
+// block of the batch two after this one zeroed. Reads only agent-scope state that the accumulation has
+// completed.
+__device__ void publish(const FloodRes *res, uint32_t n, const FloodBufs &B, const TileMask *masks, FloodRes *h_res,
+                        uint32_t *h_ctl, TileMask *h_masks, uint32_t hmcap, uint32_t t0, uint32_t nt) {
     if (t0 < CTL_WORDS) {
         h_ctl[t0] = ald(&B.ctl[t0]);
         aset(&B.ctl_next[t0], 0u);
@@ -551,11 +550,34 @@ __global__ __launch_bounds__(256) void k_flood_publish(const FloodRes *res, uint
     const uint32_t *rs = reinterpret_cast<const uint32_t *>(res);
     uint32_t *rd = reinterpret_cast<uint32_t *>(h_res);
     for (uint32_t q = t0; q < n * RW; q += nt) rd[q] = ald(&rs[q]);
+    if (!masks) return;
     const uint32_t nm = min(ald(&B.ctl[FLOOD_MASKS]), hmcap);
     constexpr uint32_t W = sizeof(TileMask) / 4;
     const uint32_t *src = reinterpret_cast<const uint32_t *>(masks);
     uint32_t *dst = reinterpret_cast<uint32_t *>(h_masks);
     for (uint32_t q = t0; q < nm * W; q += nt) dst[q] = ald(&src[q]);
+}
+
+__global__ __launch_bounds__(256) void k_flood_publish(const FloodRes *res, uint32_t n, FloodBufs B, const TileMask *masks,
+                                                       FloodRes *h_res, uint32_t *h_ctl, TileMask *h_masks, uint32_t hmcap) {
+    publish(res, n, B, masks, h_res, h_ctl, h_masks, hmcap, blockIdx.x * blockDim.x + threadIdx.x, blockDim.x * gridDim.x);
+}
+
+// k_flood_acc, then the results and control words published by the last workgroup to finish (one launch
+// fewer per batch); the batch's masks go to pinned memory as they are exported (B.h_masks): each workgroup's
+// accumulation (agent-scope atomics and stores) has completed (mem_done) before it counts itself out with
+// an agent-scope atomic, so the workgroup that counts last reads every final result. No workgroup waits
+// for another.
+__global__ __launch_bounds__(64) void k_flood_acc_publish(const FloodJob *jobs, FloodBufs B, FloodRes *res, TileMask *masks,
+                                                          uint32_t mcap, uint32_t n, FloodRes *h_res, uint32_t *h_ctl,
+                                                          TileMask *h_masks, uint32_t hmcap) {
+    const uint32_t ntiles = min(ald(B.ntiles), B.tile_cap);
+    for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) acc_tile(jobs, B, b, res, masks, &B.ctl[FLOOD_MASKS], mcap);
+    mem_done();
+    uint32_t last = 0;
+    if (threadIdx.x == 0) last = atomicAdd(&B.ctl[ACC_DONE], 1u) == gridDim.x - 1 ? 1u : 0u;
+    if (!__shfl(last, 0)) return;
+    publish(res, n, B, nullptr, h_res, h_ctl, nullptr, 0, threadIdx.x, blockDim.x);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -624,6 +646,12 @@ void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRe
 void launch_flood_publish(hipStream_t st, const FloodRes *res, uint32_t n, FloodBufs B, const TileMask *masks,
                           FloodRes *h_res, uint32_t *h_ctl, TileMask *h_masks, uint32_t hmcap) {
     hipLaunchKernelGGL(k_flood_publish, dim3(64), dim3(256), 0, st, res, n, B, masks, h_res, h_ctl, h_masks, hmcap);
+}
+void launch_flood_acc_publish(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRes *res, TileMask *masks,
+                              uint32_t mcap, uint32_t blocks, uint32_t n, FloodRes *h_res, uint32_t *h_ctl,
+                              TileMask *h_masks, uint32_t hmcap) {
+    hipLaunchKernelGGL(k_flood_acc_publish, dim3(blocks ? blocks : 1), dim3(64), 0, st, jobs, B, res, masks, mcap, n, h_res,
+                       h_ctl, h_masks, hmcap);
 }
 
 }  // namespace dsx

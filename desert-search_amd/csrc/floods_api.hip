@@ -494,6 +494,11 @@ int dsx_flood_hinted(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, const ds
         B.ctl_next = S->d_ctl + ((seq + 2) % CTL_BLOCKS) * dsx::CTL_WORDS;
         B.ntiles = B.ctl + dsx::CTL_NTILES;
         B.seq = seq;
+        // with the fused accumulate-and-publish kernel, masks reach pinned memory as they are exported
+        static const bool acc_publish = env_flag("DSX_FLOOD_ACC_PUBLISH", true);
+        const bool fused = tiled && acc_publish;
+        B.h_masks = fused ? S->h_masks : nullptr;
+        B.h_mcap = fused ? S->h_mask_cap : 0u;
         // jobs and lists read from the pinned staging by the first kernel (no separate copy)
         dsx::launch_flood_stage(c->st, S->h_up, uint32_t(n), n_small, n_big, n_hint, d_up, S->d_res, B);
         uint32_t off = 0;
@@ -531,10 +536,16 @@ int dsx_flood_hinted(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, const ds
             blocks_used = blocks;
             dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, S->prepass, light ? 1 : 0, e0, e1,
                                   prof ? 1 : 0);
-            dsx::launch_flood_acc(c->st, d_jobs, B, S->d_res, S->d_masks, S->mask_cap, n_big ? 1024 : 64);
         }
-        dsx::launch_flood_publish(c->st, S->d_res, uint32_t(n), B, S->d_masks, S->h_res, S->h_ctl, S->h_masks,
-                                  S->h_mask_cap);
+        // accumulate, then publish: one kernel whose last workgroup publishes (DSX_FLOOD_ACC_PUBLISH=0: two)
+        if (fused) {
+            dsx::launch_flood_acc_publish(c->st, d_jobs, B, S->d_res, S->d_masks, S->mask_cap, n_big ? 1024 : 64,
+                                          uint32_t(n), S->h_res, S->h_ctl, S->h_masks, S->h_mask_cap);
+        } else {
+            if (tiled) dsx::launch_flood_acc(c->st, d_jobs, B, S->d_res, S->d_masks, S->mask_cap, n_big ? 1024 : 64);
+            dsx::launch_flood_publish(c->st, S->d_res, uint32_t(n), B, S->d_masks, S->h_res, S->h_ctl, S->h_masks,
+                                      S->h_mask_cap);
+        }
         HIPCK(hipGetLastError());
         HIPCK(hipStreamSynchronize(c->st));
         if (!h[dsx::FLOOD_OVF]) break;
