@@ -136,6 +136,48 @@ def test_server_area_workload_4096(server, engine):
         server.release(rid)
 
 
+def _free_device_bytes():
+    import ctypes
+    L = ctypes.CDLL("libamdhip64.so.7")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert L.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value
+
+
+def test_server_hbm_budget():
+    """dsx_server_config.hbm_budget admits as many workers as the budget holds of the first worker's measured
+    device footprint (at least one, at most the workers asked for): a 1-byte budget gives one worker, no
+    budget all of them, and a budget of 2.5 one-worker servers (the gate context plus one worker, so at least
+    one worker's footprint each) two or three. A budget-limited server still gives the reference's rows."""
+    dsx = load_package().dsx
+    free0 = _free_device_bytes()
+    s1 = dsx.Server(0, workers=6, hbm_budget=1)
+    try:
+        assert s1.workers == 1
+        one = free0 - _free_device_bytes()
+        assert one > 1 << 28
+        rid = s1.submit_scan(10 ** 12, 1 << 20)
+        r = s1.wait(rid)
+        assert r.status == 0
+        csv, _ = s1.text(rid)
+        assert sorted(csv.splitlines()) == open(os.path.join(GOLDEN, "rg1m.csv")).read().splitlines()
+        s1.release(rid)
+    finally:
+        s1.close()
+    s2 = dsx.Server(0, workers=6, hbm_budget=int(2.5 * one))
+    try:
+        print(f"one-worker server {one / 2 ** 30:.2f} GiB; budget {2.5 * one / 2 ** 30:.2f} GiB -> {s2.workers} workers",
+              flush=True)
+        assert 2 <= s2.workers <= 3
+    finally:
+        s2.close()
+    s3 = dsx.Server(0, workers=3, hbm_budget=0)
+    try:
+        assert s3.workers == 3
+    finally:
+        s3.close()
+
+
 def test_server_alone_equals_mix(server):
     """Every request gives the same result alone and when submitted in a mix with others in flight."""
     dsx = load_package().dsx
