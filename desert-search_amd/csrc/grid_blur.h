@@ -14,7 +14,7 @@
 // (16-float sub-tables, one per corner term, indexed by gradient hash) and the packed hash streams
 // (Q4[QPAD + m] = byte offsets 4*(P[(m+t)&255] & 15), t = 0..3, replicated for m in [-QPAD, QN - QPAD)
 // so that a stream's row offset needs no wrap) sit in LDS; the horizontal halo comes from the
-// neighbouring lanes by shuffle (lane 63 <-> lane 0 is the periodic wrap); the vertical window of
+// neighbouring lanes by DPP wave rotates or LDS permutes (lane 63 <-> lane 0 is the periodic wrap); the vertical window of
 // 2R+1 rows stays in registers. The corner tables come first in WaveLds: a table read is its hash
 // byte's address plus an immediate sub-table offset, and paired reads (ds_read2, 8-bit dword offsets)
 // reach 1 KB from one address.
@@ -52,6 +52,14 @@ __device__ __forceinline__ float tab_at(const float *tab, int sub, uint32_t off)
     return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(tab) + sub * 64 + off);
 }
 __device__ __forceinline__ uint32_t byte_of(uint32_t dw, int b) { return (dw >> (8 * b)) & 0xffu; }
+// The value of the lane to the left (lane - 1, lane 0 takes lane 63's) and to the right (lane + 1, lane 63
+// takes lane 0's): DPP wavefront rotates (wave_ror:1, wave_rol:1), which use no LDS bandwidth
+__device__ __forceinline__ float from_left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, false));
+}
 // One step of a sum that starts from 0: s + x, or x itself for the sum's first term when ZSIGN is off
 // (the two differ only when x is -0; see grid_blur_rows)
 template <bool ZSIGN> __device__ __forceinline__ float sum_step(float s, float x, bool first) {
@@ -217,14 +225,16 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
                     V[q] = acc;
                 }
             }
+            // the halo: DPP rotates for the 256^2 grids (their kernels wait on LDS issue: measured -6 %), LDS
+            // permutes for the 512^2 grid (its 2 x 7 halo values per row as rotates measured +8 %)
             float Wv[NC + 2 * R];
             const int left = (lane + 63) & 63, right = (lane + 1) & 63;
 #pragma unroll
-            for (int q = 0; q < R; q++) Wv[q] = __shfl(V[NC - R + q], left);
+            for (int q = 0; q < R; q++) Wv[q] = G == 256 ? from_left(V[NC - R + q]) : __shfl(V[NC - R + q], left);
 #pragma unroll
             for (int q = 0; q < NC; q++) Wv[R + q] = V[q];
 #pragma unroll
-            for (int q = 0; q < R; q++) Wv[R + NC + q] = __shfl(V[q], right);
+            for (int q = 0; q < R; q++) Wv[R + NC + q] = G == 256 ? from_right(V[q]) : __shfl(V[q], right);
 #pragma unroll
             for (int k = 0; k < W - 1; k++)
 #pragma unroll
