@@ -138,6 +138,32 @@ def test_cli_production_candidates(engine):
         _check("fx", pre, (js["done"], js["passed"], js["scanned"], js["sites"]), stderr=r.stderr)
 
 
+def test_cli_alternative_paths(engine):
+    """The paths the defaults do not take, through the command line: the gate launched stage by stage
+    (DSX_GATE_GRAPH=0, no fast pre-gate decisions), accumulate and publish as two kernels
+    (DSX_FLOOD_ACC_PUBLISH=0), exact floods without evaluation hints (DSX_HINT_EXACT=0) and the library's
+    default launch settings (DSX_TUNED=0). gate=1 over [10^12, 10^12 + 2^20) gives the reference's rows
+    and counters (rg1m), and stdin=1 on the production-Params seeds its rows and CANDIDATE lines (fx)."""
+    exe = os.path.join(REPO, "desert-search_amd", "dsearch_desert")
+    env = dict(os.environ, DSX_GATE_GRAPH="0", DSX_FLOOD_ACC_PUBLISH="0", DSX_HINT_EXACT="0", DSX_TUNED="0")
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "g")
+        r = subprocess.run([exe, str(10 ** 12), str(1 << 20), pre, "8", "gpu=1", "gate=1"], capture_output=True,
+                           text=True, timeout=300, env=env)
+        assert r.returncode == 0, r.stderr
+        rows = sorted(open(pre + ".csv").read().splitlines()[1:])
+        assert rows == open(os.path.join(GOLDEN, "rg1m.csv")).read().splitlines()
+        js, gold = json.load(open(pre + ".json")), json.load(open(os.path.join(GOLDEN, "rg1m.json")))
+        assert [js[k] for k in ("done", "passed", "scanned", "sites")] == \
+            [gold[k] for k in ("done", "passed", "scanned", "sites")]
+        pre = os.path.join(td, "fx")
+        r = subprocess.run([exe, "0", "0", pre, "8", "stdin=1"], input=open(os.path.join(GOLDEN, "fx_seeds.txt")).read(),
+                           capture_output=True, text=True, timeout=300, env=env)
+        assert r.returncode == 0, r.stderr
+        js = json.load(open(pre + ".json"))
+        _check("fx", pre, (js["done"], js["passed"], js["scanned"], js["sites"]), stderr=r.stderr)
+
+
 def test_cli_validation_switches(engine):
     """gscancheck=N and gpucheck=N (dsearch_desert.cpp:228-242, 289-296, 445-450): the first seeds' tables and
     scan recomputed on the CPU agree with the GPU (maxima, every bucket's best score), and the CPU
