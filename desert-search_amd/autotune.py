@@ -1,7 +1,7 @@
 """Autotuner of libdsx's launch and batching settings on the whole per-seed pipeline.
 
     python desert-search_amd/autotune.py [--steps 4] [--min-gain 0.01] [--passes 2] [--confirm 3] [--dry-run]
-                                         [--out PATH]
+                                         [--out PATH] [--knobs NAME,...] [--from-tuned]
 
 Workload: the search in gate mode (gate=1: GPU gate -> tables -> scan -> hot zones -> proxy floods ->
 exact floods) over seed indices [1e12 + s*1e8, +1e8) for s < steps, after one warm-up step on the range
@@ -12,12 +12,16 @@ every timed step) must hash to the default run's, and a setting whose rows diffe
 reported.
 
 The settings are environment variables read once per process, so each trial is a fresh process. Search:
-three runs of the defaults give the run-to-run spread; then coordinate descent, each knob in turn over its
-candidates with the others at the best so far, --passes times or until a pass changes nothing; a change is
-kept when it is faster than the best so far by more than the threshold max(--min-gain, spread). The
-winner and the library defaults are then timed in alternation (--confirm pairs), and tuned.json (beside
-libdsx.so, read by dsx_create; an environment variable that is set wins, DSX_TUNED=0 ignores the file)
-holds the winner only when its median beats the defaults' median by the threshold, else an empty "env".
+three runs of the starting point (the library defaults, or with --from-tuned the settings tuned.json holds
+now) give the run-to-run spread; then coordinate descent, each knob in turn over its candidates with the
+others at the best so far, --passes times or until a pass changes nothing; a change is kept when it is
+faster than the best so far by more than the threshold max(--min-gain, spread). A knob named "A+B" moves
+the variables A and B together (candidate "x+y" sets A=x, B=y): settings that only pay together, such as
+more pipeline workers once the host threads block instead of spinning while they wait for the device, are
+out of reach of one-variable moves. --knobs restricts the sweep to the named knobs. The winner and the
+library defaults are then timed in alternation (--confirm pairs), and tuned.json (beside libdsx.so, read
+by dsx_create; an environment variable that is set wins, DSX_TUNED=0 ignores the file) holds the winner
+only when its median beats the defaults' median by the threshold, else an empty "env".
 """
 import argparse
 import hashlib
@@ -38,6 +42,7 @@ N_STEP = 10 ** 8
 KNOBS = [
     ("DSX_WORKERS", ["", "12", "20", "24"]),                    # pipeline workers (default 16)
     ("DSX_SYNC", ["", "spin", "yield", "block"]),               # host wait mode of stream synchronisations
+    ("DSX_SYNC+DSX_WORKERS", ["block+20", "block+24", "block+28"]),   # blocking waits free host cores for workers
     ("DSX_FLOOD_NT", ["", "128", "512"]),                       # light flood instance workgroup (256)
     ("DSX_FLOOD_NT_FULL", ["", "256", "128"]),                  # full (exact) instance workgroup (512)
     ("DSX_FLOOD_OCC", ["", "4", "5", "6"]),                     # occupancy hint of the 256-thread instances
@@ -53,6 +58,19 @@ KNOBS = [
     ("DSX_ACTIVE", ["", "2"]),                                  # continuous batching: seeds in flight (1)
     ("DSX_CHUNK", ["", "2097152", "8388608"]),                  # seed indices per gate call (2^22)
 ]
+
+
+def knob_vars(k):
+    return k.split("+")
+
+
+def knob_vals(k, v):
+    """The values candidate v of knob k gives its variables ("" = all unset)."""
+    n = len(knob_vars(k))
+    vals = v.split("+") if v != "" else [""] * n
+    if len(vals) != n:
+        raise ValueError(f"candidate {v!r} of knob {k} sets {len(vals)} values for {n} variables")
+    return dict(zip(knob_vars(k), vals))
 
 
 def load_dsx():
@@ -111,6 +129,8 @@ def main():
     ap.add_argument("--confirm", type=int, default=3)
     ap.add_argument("--dry-run", action="store_true", help="search, but do not write tuned.json")
     ap.add_argument("--out", default=TUNED, help="where to write the result (default: tuned.json beside libdsx.so)")
+    ap.add_argument("--knobs", default="", help="comma-separated knob names to sweep (default: all)")
+    ap.add_argument("--from-tuned", action="store_true", help="start the descent from the settings tuned.json holds")
     ap.add_argument("--trial", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.trial:
@@ -118,27 +138,39 @@ def main():
             v, rows = run_steps(load_dsx(), a.steps, td)
         print(json.dumps({"value": v, "rows": rows}))
         return
-    base = {k: "" for k, _ in KNOBS}
-    # run-to-run spread of the defaults: a change must beat the best so far by more than it
-    reps = [trial(base, a.steps) for _ in range(3)]
+    base = {x: "" for k, _ in KNOBS for x in knob_vars(k)}
+    knobs = KNOBS
+    if a.knobs:
+        names = a.knobs.split(",")
+        unknown = set(names) - {k for k, _ in KNOBS}
+        if unknown:
+            raise SystemExit(f"unknown knobs: {sorted(unknown)}")
+        knobs = [(k, c) for k, c in KNOBS if k in names]
+    start = dict(base)
+    if a.from_tuned:
+        with open(TUNED) as f:
+            start.update(json.load(f).get("env", {}))
+    # run-to-run spread of the starting point: a change must beat the best so far by more than it
+    reps = [trial(start, a.steps) for _ in range(3)]
     rows0 = reps[0][1]
     if any(r[1] != rows0 for r in reps):
-        raise RuntimeError("the default runs' rows differ from each other")
+        raise RuntimeError("the starting point's runs' rows differ from each other")
     vals = sorted(r[0] for r in reps)
     best_v = statistics.median(vals)
     spread = (vals[-1] - vals[0]) / best_v
     gain = max(a.min_gain, spread)
-    print(f"default: {best_v:.1f} gated seeds/s (spread {100 * spread:.1f} %, threshold {100 * gain:.1f} %)",
+    print(f"start: {best_v:.1f} gated seeds/s (spread {100 * spread:.1f} %, threshold {100 * gain:.1f} %)",
           flush=True)
-    best = dict(base)
-    log, rejected = [{"env": {}, "value": v} for v in vals], []
+    best = dict(start)
+    log, rejected = [{"env": {x: y for x, y in start.items() if y != ""}, "value": v} for v in vals], []
     for p in range(a.passes):
         changed_pass = False
-        for k, cands in KNOBS:
+        for k, cands in knobs:
             for v in cands:
-                if v == best[k]:
+                kv = knob_vals(k, v)
+                if all(best[x] == y for x, y in kv.items()):
                     continue
-                env = dict(best, **{k: v})
+                env = dict(best, **kv)
                 val, rows = trial(env, a.steps)
                 changed = {x: y for x, y in env.items() if y != ""}
                 log.append({"env": changed, "value": val})
@@ -165,8 +197,9 @@ def main():
             tuned = {}
     doc = {"env": tuned,
            "device": "MI355X (gfx950), one GPU",
+           "start": {x: y for x, y in start.items() if y != ""}, "knobs": [k for k, _ in knobs],
            "workload": f"search gate=1 over seed indices [1e12 + s*1e8, +1e8), s < {a.steps}, after 1 warm-up step",
-           "unit": "gated seeds/s", "min_gain": gain, "default_spread": spread,
+           "unit": "gated seeds/s", "min_gain": gain, "start_spread": spread,
            "default_value": statistics.median(dv) if dv else best_v if not tuned else None,
            "tuned_value": statistics.median(tv) if tv else None,
            "confirm": {"default": dv, "tuned": tv}, "rejected_rows_differ": rejected, "trials": log}
