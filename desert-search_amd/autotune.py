@@ -57,6 +57,8 @@ KNOBS = [
     ("DSX_GROUP", ["", "32", "128"]),                           # seeds per worker group (64)
     ("DSX_ACTIVE", ["", "2"]),                                  # continuous batching: seeds in flight (1)
     ("DSX_CHUNK", ["", "2097152", "8388608"]),                  # seed indices per gate call (2^22)
+    ("DSX_FLOOD_SPEC", ["", "0"]),                              # speculative sub-blocks in idle waves (1)
+    ("DSX_GATE_TRUNC", ["", "0"]),                              # truncated-blur fast pre-gate decisions (1)
 ]
 
 

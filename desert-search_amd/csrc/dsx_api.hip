@@ -34,6 +34,16 @@ dsx::GateKernel dsx::gate_kernel(int R, float sig) {
     for (int t = 0; t <= 2 * R; t++) ks += K.k[t];
     K.norm = 1.f / (ks * ks);
     K.wd = double(float(1.25 * (32.0 / 63.0) * 1.5));
+    // the normalised 2D weights the truncated blur (taps 1 .. 2R - 1) leaves out, in double, with a margin
+    // of 1e-3 for this evaluation's own roundings; the weights are positive
+    if (R >= 2) {
+        double s = 0, st = 0;
+        for (int t = 0; t <= 2 * R; t++) {
+            s += K.k[t];
+            if (t > 0 && t < 2 * R) st += K.k[t];
+        }
+        K.trunc = std::nextafter(float((s * s - st * st) * double(K.norm) * 1.001), INFINITY);
+    }
     return K;
 }
 bool dsx::valid_gate_params(const dsx_gate_params *p) {

@@ -19,9 +19,13 @@ struct SideParams {
 // maximum, and the octave weight. GPU gate (gate.cuh:216, 248): unnormalised expf weights, factor
 // 1/(ksum*ksum), weight float(1.25 * 32/63 * 1.5). CPU-default pre-gate (lowfield.hpp:92-96):
 // normalised float(exp / sum) weights, factor 1, weight factor * octave weight in double.
+// trunc: a bound, per unit of |grid value|, on how far the normalised blur moves when its outermost taps
+// (d = +-R) are left out of both passes: norm * (ksum^2 - ksum_t^2), ksum_t the sum without them, rounded up
+// (0: no truncated fast pre-gate for this kernel).
 struct GateKernel {
     float k[15];
     float norm;
+    float trunc;
     double wd;   // octave weight the corner tables are scaled by
 };
 
@@ -77,8 +81,9 @@ void launch_phase(hipStream_t st, uint64_t base, uint32_t n, double maxph, uint3
 void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64_t *seeds, uint32_t n, SideParams *out,
                  const uint32_t *n_dev = nullptr);
 // Amplitude kernel k_amp<mode, dec> (gate.hip): dec = GATE_EXACT, or GATE_SPLIT (modes 0 and 2), whose
-// undecided list is und_idx / und_count.
-enum { GATE_EXACT = 0, GATE_SPLIT = 1 };
+// undecided list is und_idx / und_count, or GATE_SPLIT_T (mode 0): GATE_SPLIT with the blur's outermost taps
+// left out of the fast amplitude, its decision margin widened by GateKernel::trunc.
+enum { GATE_EXACT = 0, GATE_SPLIT = 1, GATE_SPLIT_T = 2 };
 void launch_amp(hipStream_t st, int mode, int dec, const SideParams *sides, const uint32_t *sel, uint32_t n,
                 const GateKernel &K, double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count,
                 uint32_t cap, float *all_amp, const uint32_t *n_dev = nullptr, uint32_t *und_idx = nullptr,

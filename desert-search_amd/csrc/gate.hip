@@ -17,6 +17,7 @@
 //     and hash streams live in LDS, laid out so every table read of a wave hits one 16-dword
 //     sub-table (conflict-free).
 #include "grid_blur.h"
+#include <cstdlib>
 #include <vector>
 
 namespace dsx {
@@ -93,13 +94,16 @@ __global__ __launch_bounds__(64) void k_init(uint64_t base, const uint32_t *ks, 
 // 256^2 even-parity pre-gate (R=4). Grid arithmetic and blur: grid_blur.h.
 template <int MODE> struct GateR { static constexpr int R = (MODE == 1) ? 7 : 4; };
 
-template <int MODE, bool FAST>
+// FAST: fused multiply-add blurs; TRUNC (with FAST): the blur's outermost taps left out (radius R - 1, taps
+// 1 .. 2R - 1 of K.k), which moves the normalised result by at most K.trunc per unit of |grid value|.
+template <int MODE, bool FAST, bool TRUNC = false>
 __device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K) {
+    static_assert(FAST || !TRUNC, "only the fast amplitude may leave taps out");
     grid_setup<MODE>(sp, L, K.wd);
     float best = -1e9f;
-    constexpr int NC = Shape<MODE>::G / 64;
+    constexpr int NC = Shape<MODE>::G / 64, R = GateR<MODE>::R - (TRUNC ? 1 : 0);
     // only the maximum is kept, so sums may start from their first term (ZSIGN off)
-    grid_blur_rows<MODE, GateR<MODE>::R, false, FAST>(sp, L, K.k, 0, Shape<MODE>::G - 1, [&](int, const float (&o)[NC]) {
+    grid_blur_rows<MODE, R, false, FAST>(sp, L, K.k + (TRUNC ? 1 : 0), 0, Shape<MODE>::G - 1, [&](int, const float (&o)[NC]) {
 #pragma unroll
         for (int q = 0; q < NC; q++) best = fmaxf(best, o[q]);
     });
@@ -144,7 +148,11 @@ __device__ __forceinline__ double gate_eps(float bA, float bB) { return 2e-5 * (
 //               (kmap ? kmap[pair] : pair, amplitude); all_amp (optional) gets every amplitude, by block
 //   GATE_SPLIT  the fast amplitude (FMA blurs): pairs of seeds certainly at or above thr (amp - eps >= thr)
 //               to out_idx, those within eps of thr to und_idx (to be decided by GATE_EXACT)
-// The fast decisions equal the exact ones: the two amplitudes differ by less than eps (gate_eps).
+//   GATE_SPLIT_T the same with the blur's outermost taps left out (radius R - 1 in both passes: a quarter
+//               fewer blur operations at R = 4), eps widened by K.trunc * (bA + bB)
+// The fast decisions equal the exact ones: the two amplitudes differ by less than eps (gate_eps, plus the
+// truncation bound: dropping taps changes the exact blurred value by at most norm * (ksum^2 - ksum_t^2) times
+// the largest |grid value|, itself at most the side's bound b).
 template <int MODE, int DEC>
 __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint32_t *sel, uint32_t n, GateKernel K,
                                              double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp,
@@ -157,7 +165,7 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
     if (item >= n) return;
     const uint32_t pidx = sel ? sel[item] : item;
     const int wv = threadIdx.x >> 6;
-    const float m = side_max<MODE, DEC != GATE_EXACT>(sides[2 * size_t(pidx) + wv], L[wv], K);
+    const float m = side_max<MODE, DEC != GATE_EXACT, DEC == GATE_SPLIT_T>(sides[2 * size_t(pidx) + wv], L[wv], K);
     float b = 0.f;
     if (DEC != GATE_EXACT) b = side_bound<MODE>(L[wv]);
     if ((threadIdx.x & 63) == 0) {
@@ -179,7 +187,9 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
                 }
             }
         } else {
-            const double eps = gate_eps(bnd[0], bnd[1]);
+            // a truncated blur moves each side's maximum by at most K.trunc times its grid bound
+            const double eps = gate_eps(bnd[0], bnd[1]) +
+                               (DEC == GATE_SPLIT_T ? double(K.trunc) * (double(bnd[0]) + double(bnd[1])) : 0.0);
             if (double(amp) - eps >= thr) {
                 const uint32_t i = atomicAdd(count, 1u);
                 if (i < cap) out_idx[i] = pidx;
@@ -231,7 +241,14 @@ std::vector<GateLaunch> gate_launches(const GateChain &C, const GateStages &S) {
     const int mp = S.mode_pre;
     std::vector<GateLaunch> v;
     if (C.fast) {
-        v.push_back({mp, GATE_SPLIT, true, nullptr, nullptr, C.i2, nullptr, C.cnt + 1, C.cnt + 3, C.k3, C.cnt + 4});
+        // the truncated fast amplitude where its wider margin stays small (the production kernel: 5.4e-4 per
+        // unit of grid bound); DSX_GATE_TRUNC=0 keeps every tap
+        static const bool trunc_on = [] {
+            const char *e = std::getenv("DSX_GATE_TRUNC");
+            return !(e && *e == '0');
+        }();
+        const int dec = (trunc_on && mp == 0 && S.kpre.trunc > 0.f && S.kpre.trunc <= 2e-3f) ? GATE_SPLIT_T : GATE_SPLIT;
+        v.push_back({mp, dec, true, nullptr, nullptr, C.i2, nullptr, C.cnt + 1, C.cnt + 3, C.k3, C.cnt + 4});
         v.push_back({mp, GATE_EXACT, true, C.k3, nullptr, C.i2, nullptr, C.cnt + 1, C.cnt + 4, nullptr, nullptr});
     } else {
         v.push_back({mp, GATE_EXACT, true, nullptr, nullptr, C.i2, nullptr, C.cnt + 1, C.cnt + 3, nullptr, nullptr});
@@ -241,7 +258,8 @@ std::vector<GateLaunch> gate_launches(const GateChain &C, const GateStages &S) {
 }
 
 const void *amp_kernel(int mode, int dec) {
-    if (dec == GATE_SPLIT)
+    if (dec == GATE_SPLIT_T && mode == 0) return reinterpret_cast<const void *>(&k_amp<0, GATE_SPLIT_T>);
+    if (dec != GATE_EXACT)
         return mode == 0 ? reinterpret_cast<const void *>(&k_amp<0, GATE_SPLIT>)
                          : reinterpret_cast<const void *>(&k_amp<2, GATE_SPLIT>);
     return mode == 0 ? reinterpret_cast<const void *>(&k_amp<0, GATE_EXACT>)
