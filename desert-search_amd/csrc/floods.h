@@ -70,12 +70,13 @@ struct FloodBufs {
 // (100 MHz ticks) waiting for queue items, evaluating sub-blocks, loading octaves into LDS, and alive;
 // then the batch number (plus one) the stage kernel wrote, the one the tiled engine's first workgroup
 // was given, the queue tail and the stage's batch number that workgroup read when it started; the
-// accumulate kernel's workgroups that have finished
+// accumulate kernel's workgroups that have finished; when the tiled engine stages the batch itself, the
+// tickets its workgroups drew on arrival and the staging workgroups that have finished
 constexpr int Q_HEAD = 0, Q_TAIL = 1, Q_PENDING = 2, FLOOD_OVF = 3, FLOOD_MASKS = 4, Q_ITEMS = 5, HANDED = 6,
               CTL_NTILES = 7, TIME_SEEN = 8, TIME_SLOT = 9, TIME_HEAD = 10, TIME_TAIL = 11, TIME_PENDING = 12,
               TIME_ITEM = 13, EVAL_ROUNDS = 14, EVAL_NEED = 15, PROF_WAIT = 16, PROF_EVAL = 17, PROF_LOAD = 18,
               PROF_LIFE = 19, STAGE_SEQ = 20, RUN_SEQ = 21, RUN_TAIL0 = 22, RUN_STAGE_SEQ = 23, ACC_DONE = 24,
-              CTL_WORDS = 25;
+              STAGE_TICKET = 25, STAGE_DONE = 26, CTL_WORDS = 27;
 constexpr uint32_t Q_EMPTY = 0xffffffffu;
 // FLOOD_OVF bits: 1 work queue full, 2 tile pool or hash table full, 4 time limit reached
 constexpr uint32_t OVF_QUEUE = 1, OVF_TILES = 2, OVF_TIME = 4;
@@ -93,14 +94,24 @@ struct FloodHint {
 // touched and queued, and every hinted tile allocated, given its hint and queued.
 void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big, uint32_t nh,
                         uint8_t *d_up, FloodRes *res, FloodBufs B);
+// A batch staged by the tiled engine's own first workgroups instead of k_flood_stage (batches without
+// small-window floods): the stage kernel's arguments; nst = the number of staging workgroups (set by
+// launch_flood_run; 0 = the batch is already staged).
+struct StageArgs {
+    const uint8_t *h_up;
+    uint8_t *d_up;
+    FloodRes *res;
+    uint32_t n, n_big, nh, nst;
+};
 // The persistent flood kernel: `blocks` workgroups take queued tiles until no work is pending. A tile
 // is evaluated lazily, by 8x8 sub-blocks the flood touches; every processing grows its reached set to
 // a fixpoint and queues the neighbours it touched. ev0/ev1 (optional): HIP events around it.
 // pre: 1 = FP32 pre-pass on the predicates' temperature test (identical results); light: 1 when every
-// tiled job is a hot-zone or proxy flood (selects the instance without the biome predicates)
+// tiled job is a hot-zone or proxy flood (selects the instance without the biome predicates);
+// stage (optional): stage the batch in this kernel (no k_flood_stage launch before it).
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
                       uint32_t blocks, int pre, int light, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
-                      int tprof = 0);
+                      int tprof = 0, const StageArgs *stage = nullptr);
 // Reduce every allocated tile (count read on the device) into res, export counted masks, and return
 // the tiles and hash slots to the clean state.
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRes *res, struct TileMask *masks,

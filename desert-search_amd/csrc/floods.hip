@@ -42,16 +42,35 @@
 
 namespace dsx {
 
-__global__ __launch_bounds__(256) void k_flood_stage(const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big,
-                                                     uint32_t nh, uint8_t *d_up, FloodRes *res, FloodBufs B) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+// Jobs are written and read word by word with agent-scope accesses where the same kernel both stages
+// and floods (a plain store is not visible to workgroups on other XCDs before the kernel ends).
+__device__ __forceinline__ void job_store(FloodJob *p, const FloodJob &J) {
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(&J);
+    uint32_t *d = reinterpret_cast<uint32_t *>(p);
+#pragma unroll
+    for (int i = 0; i < int(sizeof(FloodJob) / 4); i++) ast(&d[i], s[i]);
+}
+__device__ __forceinline__ FloodJob job_load(const FloodJob *p) {
+    FloodJob J;
+    uint32_t *d = reinterpret_cast<uint32_t *>(&J);
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(p);
+#pragma unroll
+    for (int i = 0; i < int(sizeof(FloodJob) / 4); i++) d[i] = ald(&s[i]);
+    return J;
+}
+
+// Item q of a batch's staging: job q copied from the pinned staging to the device and its result
+// initialised (q < n), small-window list entry q copied (q < n_small), tiled flood q's start tile touched
+// and queued (q < n_big), hint q applied (q < nh).
+__device__ __forceinline__ void stage_item(uint32_t q, const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big,
+                                           uint32_t nh, uint8_t *d_up, FloodRes *res, const FloodBufs &B) {
     const FloodJob *hj = reinterpret_cast<const FloodJob *>(h_up);
     FloodJob *dj = reinterpret_cast<FloodJob *>(d_up);
     const uint32_t *hl = reinterpret_cast<const uint32_t *>(hj + n);
     uint32_t *dl = reinterpret_cast<uint32_t *>(dj + n);
     if (q < n) {
         const FloodJob J = hj[q];
-        dj[q] = J;
+        job_store(&dj[q], J);
         FloodRes r{};
         r.minx = r.maxx = J.qx;
         r.minz = r.maxz = J.qz;
@@ -78,6 +97,12 @@ __global__ __launch_bounds__(256) void k_flood_stage(const uint8_t *h_up, uint32
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_flood_stage(const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big,
+                                                     uint32_t nh, uint8_t *d_up, FloodRes *res, FloodBufs B) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    stage_item(q, h_up, n, n_small, n_big, nh, d_up, res, B);
     if (q == 0) aset(&B.ctl[STAGE_SEQ], B.seq + 1);
 }
 
@@ -238,9 +263,13 @@ __device__ uint32_t prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J
 // OCC: waves per SIMD the register allocation must allow. LIGHT: an instance for batches whose tiled
 // floods are all hot-zone or proxy floods; it leaves out the biome predicates and so needs fewer
 // registers (more workgroups resident beside the other streams' kernels).
+// With sa.nst > 0 the kernel stages its batch first: the first sa.nst workgroups to arrive (by a ticket
+// drawn on arrival, so each of them is running) stage the batch's items in a grid-stride loop and count
+// themselves done; no workgroup leaves before all of them have, since the items they queue are counted as
+// pending before that.
 template <int NT, int OCC, int PRE, int LIGHT>
 __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B,
-                                                       int backoff, int spec, int tprof) {
+                                                       int backoff, int spec, int tprof, StageArgs sa) {
     constexpr int NWAVE = NT / 64;
     // spec bit 1: raise this kernel's waves' issue priority (s_setprio) above the default of other
     // kernels sharing the SIMDs, e.g. the VALU-bound gate; bit 0: speculative sub-blocks
@@ -259,6 +288,23 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
     __shared__ int s_stop;
     const int tid = threadIdx.x, wave = tid >> 6;
     int loaded_seed = -1, loaded_cls = -1;
+    if (sa.nst) {
+        if (tid == 0) s_item = atomicAdd(&B.ctl[STAGE_TICKET], 1u);
+        __syncthreads();
+        const uint32_t t = s_item;
+        if (t < sa.nst) {
+            const uint32_t m = max(sa.n, max(sa.n_big, sa.nh));
+            for (uint32_t q = t * NT + tid; q < m; q += sa.nst * NT)
+                stage_item(q, sa.h_up, sa.n, 0u, sa.n_big, sa.nh, sa.d_up, sa.res, B);
+            mem_done();   // this wave's stores, queue pushes and pending counts have completed
+            __syncthreads();
+            if (tid == 0) {
+                if (t == 0) aset(&B.ctl[STAGE_SEQ], B.seq + 1);
+                atomicAdd(&B.ctl[STAGE_DONE], 1u);
+            }
+        }
+        __syncthreads();   // s_item is taken again below
+    }
     for (;;) {
         // ---- take the next queue slot and wait for its tile
         if (tid == 0) {
@@ -274,7 +320,14 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                     item = __hip_atomic_load(&B.wq[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                     if (item != Q_EMPTY) break;
                     if ((it & 7u) == 7u) {   // the exit conditions, every 8th poll
-                        if (ald(&B.ctl[Q_PENDING]) == 0u || ald(&B.ctl[FLOOD_OVF]) != 0u) {
+                        // staging complete (read first: its queued items are counted as pending by then),
+                        // then nothing pending
+                        bool staged = true;
+                        if (sa.nst) {
+                            staged = ald(&B.ctl[STAGE_DONE]) == sa.nst;
+                            mem_done();
+                        }
+                        if ((staged && ald(&B.ctl[Q_PENDING]) == 0u) || ald(&B.ctl[FLOOD_OVF]) != 0u) {
                             stop = 1;
                             break;
                         }
@@ -326,7 +379,7 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
             s_hot[tid] = ald(&T.hot[tid]);
         }
         __syncthreads();
-        const FloodJob J = jobs[ald(&T.job)];
+        const FloodJob J = sa.nst ? job_load(&jobs[ald(&T.job)]) : jobs[ald(&T.job)];
         const int tx = ald(&T.tx), tz = ald(&T.tz);
         const uint32_t evald0 = s_evald;
         for (;;) {
@@ -588,7 +641,8 @@ void launch_flood_stage(hipStream_t st, const uint8_t *h_up, uint32_t n, uint32_
     if (m) hipLaunchKernelGGL(k_flood_stage, dim3((m + 255) / 256), dim3(256), 0, st, h_up, n, n_small, n_big, nh, d_up, res, B);
 }
 void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, const SurfGrid &g, FloodBufs B,
-                      uint32_t blocks, int pre, int light, hipEvent_t ev0, hipEvent_t ev1, int tprof) {
+                      uint32_t blocks, int pre, int light, hipEvent_t ev0, hipEvent_t ev1, int tprof,
+                      const StageArgs *stage) {
     // workgroup size (DSX_FLOOD_NT = 64 to 1024 threads); `blocks` is given for 256-thread
     // workgroups and scaled so that the launch holds the same number of waves
     static const int nt_light = [] {
@@ -636,8 +690,14 @@ void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, 
     else kern = occ4 ? DSX_FR(256, 4) : DSX_FR(256, 1);
 #undef DSX_FR
     const dim3 grid(nt <= 256 ? (blocks ? blocks : 1) * (256 / nt) : std::max(1u, blocks * 256 / nt));
-    if (ev0) hipExtLaunchKernelGGL(kern, grid, dim3(nt), 0, st, ev0, ev1, 0u, jobs, cs, g, B, backoff, spec, tprof);
-    else hipLaunchKernelGGL(kern, grid, dim3(nt), 0, st, jobs, cs, g, B, backoff, spec, tprof);
+    StageArgs sa{};
+    if (stage) {   // one staging workgroup per nt items, at most the whole grid
+        sa = *stage;
+        const uint32_t m = std::max(sa.n, std::max(sa.n_big, sa.nh));
+        sa.nst = std::max(1u, std::min(grid.x, (m + uint32_t(nt) - 1) / uint32_t(nt)));
+    }
+    if (ev0) hipExtLaunchKernelGGL(kern, grid, dim3(nt), 0, st, ev0, ev1, 0u, jobs, cs, g, B, backoff, spec, tprof, sa);
+    else hipLaunchKernelGGL(kern, grid, dim3(nt), 0, st, jobs, cs, g, B, backoff, spec, tprof, sa);
 }
 void launch_flood_acc(hipStream_t st, const FloodJob *jobs, FloodBufs B, FloodRes *res, TileMask *masks, uint32_t mcap,
                       uint32_t blocks) {

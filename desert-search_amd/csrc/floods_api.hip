@@ -499,8 +499,12 @@ int dsx_flood_hinted(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, const ds
         const bool fused = tiled && acc_publish;
         B.h_masks = fused ? S->h_masks : nullptr;
         B.h_mcap = fused ? S->h_mask_cap : 0u;
-        // jobs and lists read from the pinned staging by the first kernel (no separate copy)
-        dsx::launch_flood_stage(c->st, S->h_up, uint32_t(n), n_small, n_big, n_hint, d_up, S->d_res, B);
+        // jobs and lists read from the pinned staging by the first kernel (no separate copy): the stage
+        // kernel, or (a batch without small-window floods) the tiled engine's own first workgroups
+        static const bool self_stage = env_flag("DSX_FLOOD_SELF_STAGE", true);
+        const bool staged_by_run = self_stage && n_small == 0 && n_big > 0;
+        if (!staged_by_run)
+            dsx::launch_flood_stage(c->st, S->h_up, uint32_t(n), n_small, n_big, n_hint, d_up, S->d_res, B);
         uint32_t off = 0;
         for (int cls = 0; cls < dsx::LDS_NCLS; cls++) {
             dsx::launch_flood_lds(c->st, cls, d_jobs, d_small + off, n_cls[cls], S->d_clim, S->grid, S->d_res,
@@ -534,8 +538,9 @@ int dsx_flood_hinted(dsx_ctx *c, const dsx_flood_job *jobs, uint64_t n, const ds
                 light ? uint32_t(std::clamp<size_t>(want, 16, max_blocks))
                       : uint32_t(std::clamp<size_t>(want, std::min(min_full, max_full), max_full));
             blocks_used = blocks;
+            const dsx::StageArgs sa{S->h_up, d_up, S->d_res, uint32_t(n), n_big, n_hint, 0u};
             dsx::launch_flood_run(c->st, d_jobs, S->d_clim, S->grid, B, blocks, S->prepass, light ? 1 : 0, e0, e1,
-                                  prof ? 1 : 0);
+                                  prof ? 1 : 0, staged_by_run ? &sa : nullptr);
         }
         // accumulate, then publish: one kernel whose last workgroup publishes (DSX_FLOOD_ACC_PUBLISH=0: two)
         if (fused) {
