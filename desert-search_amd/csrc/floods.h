@@ -71,12 +71,13 @@ struct FloodBufs {
 // then the batch number (plus one) the stage kernel wrote, the one the tiled engine's first workgroup
 // was given, the queue tail and the stage's batch number that workgroup read when it started; the
 // accumulate kernel's workgroups that have finished; when the tiled engine stages the batch itself, the
-// tickets its workgroups drew on arrival and the staging workgroups that have finished
+// tickets its workgroups drew on arrival, the staging workgroups that have finished, and those that have
+// stored their jobs
 constexpr int Q_HEAD = 0, Q_TAIL = 1, Q_PENDING = 2, FLOOD_OVF = 3, FLOOD_MASKS = 4, Q_ITEMS = 5, HANDED = 6,
               CTL_NTILES = 7, TIME_SEEN = 8, TIME_SLOT = 9, TIME_HEAD = 10, TIME_TAIL = 11, TIME_PENDING = 12,
               TIME_ITEM = 13, EVAL_ROUNDS = 14, EVAL_NEED = 15, PROF_WAIT = 16, PROF_EVAL = 17, PROF_LOAD = 18,
               PROF_LIFE = 19, STAGE_SEQ = 20, RUN_SEQ = 21, RUN_TAIL0 = 22, RUN_STAGE_SEQ = 23, ACC_DONE = 24,
-              STAGE_TICKET = 25, STAGE_DONE = 26, CTL_WORDS = 27;
+              STAGE_TICKET = 25, STAGE_DONE = 26, STAGE_JOBS = 27, CTL_WORDS = 28;
 constexpr uint32_t Q_EMPTY = 0xffffffffu;
 // FLOOD_OVF bits: 1 work queue full, 2 tile pool or hash table full, 4 time limit reached
 constexpr uint32_t OVF_QUEUE = 1, OVF_TILES = 2, OVF_TIME = 4;

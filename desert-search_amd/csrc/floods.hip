@@ -59,11 +59,13 @@ __device__ __forceinline__ FloodJob job_load(const FloodJob *p) {
     return J;
 }
 
-// Item q of a batch's staging: job q copied from the pinned staging to the device and its result
-// initialised (q < n), small-window list entry q copied (q < n_small), tiled flood q's start tile touched
-// and queued (q < n_big), hint q applied (q < nh).
-__device__ __forceinline__ void stage_item(uint32_t q, const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big,
-                                           uint32_t nh, uint8_t *d_up, FloodRes *res, const FloodBufs &B) {
+// Item q of a batch's staging, in two parts: stage_job copies job q from the pinned staging to the device
+// and initialises its result (q < n) and copies small-window list entry q (q < n_small); stage_tiles touches
+// and queues tiled flood q's start tile (q < n_big) and applies hint q (q < nh). A queued tile's job must be
+// on the device before any workgroup can take the tile: where the same kernel stages and floods, every job
+// is stored before any tile is queued.
+__device__ __forceinline__ void stage_job(uint32_t q, const uint8_t *h_up, uint32_t n, uint32_t n_small, uint8_t *d_up,
+                                          FloodRes *res) {
     const FloodJob *hj = reinterpret_cast<const FloodJob *>(h_up);
     FloodJob *dj = reinterpret_cast<FloodJob *>(d_up);
     const uint32_t *hl = reinterpret_cast<const uint32_t *>(hj + n);
@@ -77,6 +79,12 @@ __device__ __forceinline__ void stage_item(uint32_t q, const uint8_t *h_up, uint
         res_store(&res[q], r);
         if (q < n_small) dl[q] = hl[q];
     }
+}
+__device__ __forceinline__ void stage_tiles(uint32_t q, const uint8_t *h_up, uint32_t n, uint32_t n_big, uint32_t nh,
+                                            uint8_t *d_up, const FloodBufs &B) {
+    const FloodJob *hj = reinterpret_cast<const FloodJob *>(h_up);
+    const uint32_t *hl = reinterpret_cast<const uint32_t *>(hj + n);
+    uint32_t *dl = reinterpret_cast<uint32_t *>(reinterpret_cast<FloodJob *>(d_up) + n);
     if (q < n_big) {
         const uint32_t k = hl[n + q];
         dl[n + q] = k;
@@ -99,10 +107,12 @@ __device__ __forceinline__ void stage_item(uint32_t q, const uint8_t *h_up, uint
     }
 }
 
+// (the tiled engine that consumes the queued tiles is a later kernel, so its jobs are all in place)
 __global__ __launch_bounds__(256) void k_flood_stage(const uint8_t *h_up, uint32_t n, uint32_t n_small, uint32_t n_big,
                                                      uint32_t nh, uint8_t *d_up, FloodRes *res, FloodBufs B) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    stage_item(q, h_up, n, n_small, n_big, nh, d_up, res, B);
+    stage_job(q, h_up, n, n_small, d_up, res);
+    stage_tiles(q, h_up, n, n_big, nh, d_up, B);
     if (q == 0) aset(&B.ctl[STAGE_SEQ], B.seq + 1);
 }
 
@@ -264,9 +274,9 @@ __device__ uint32_t prop_tile(const FloodBufs &B, uint32_t ti, const FloodJob &J
 // floods are all hot-zone or proxy floods; it leaves out the biome predicates and so needs fewer
 // registers (more workgroups resident beside the other streams' kernels).
 // With sa.nst > 0 the kernel stages its batch first: the first sa.nst workgroups to arrive (by a ticket
-// drawn on arrival, so each of them is running) stage the batch's items in a grid-stride loop and count
-// themselves done; no workgroup leaves before all of them have, since the items they queue are counted as
-// pending before that.
+// drawn on arrival, so each of them is running) store every job and result, wait for each other, queue
+// the start and hinted tiles (grid-stride loops) and count themselves done; no workgroup leaves before all
+// of them have, since the items they queue are counted as pending before that.
 template <int NT, int OCC, int PRE, int LIGHT>
 __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, const ClimSeed *cs, SurfGrid g, FloodBufs B,
                                                        int backoff, int spec, int tprof, StageArgs sa) {
@@ -293,10 +303,26 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
         __syncthreads();
         const uint32_t t = s_item;
         if (t < sa.nst) {
-            const uint32_t m = max(sa.n, max(sa.n_big, sa.nh));
-            for (uint32_t q = t * NT + tid; q < m; q += sa.nst * NT)
-                stage_item(q, sa.h_up, sa.n, 0u, sa.n_big, sa.nh, sa.d_up, sa.res, B);
-            mem_done();   // this wave's stores, queue pushes and pending counts have completed
+            // every job and result first, in all staging workgroups (each of them is running, so this wait
+            // ends), then the start and hinted tiles: no tile is queued before its job is on the device
+            for (uint32_t q = t * NT + tid; q < sa.n; q += sa.nst * NT) stage_job(q, sa.h_up, sa.n, 0u, sa.d_up, sa.res);
+            mem_done();
+            __syncthreads();
+            if (tid == 0) {
+                atomicAdd(&B.ctl[STAGE_JOBS], 1u);
+                const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+                while (ald(&B.ctl[STAGE_JOBS]) < sa.nst) {
+                    if (__builtin_amdgcn_s_memrealtime() - t0 > 500000000ull) {   // 5 s at 100 MHz: flagged
+                        atomicOr(&B.ctl[FLOOD_OVF], OVF_TIME);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            __syncthreads();
+            const uint32_t m = max(sa.n_big, sa.nh);
+            for (uint32_t q = t * NT + tid; q < m; q += sa.nst * NT) stage_tiles(q, sa.h_up, sa.n, sa.n_big, sa.nh, sa.d_up, B);
+            mem_done();   // this wave's queue pushes and pending counts have completed
             __syncthreads();
             if (tid == 0) {
                 if (t == 0) aset(&B.ctl[STAGE_SEQ], B.seq + 1);
