@@ -277,3 +277,28 @@ def test_flood_hints_change_nothing(engine):
     key = lambda m: (int(m["job"]), int(m["tx"]), int(m["tz"]), tuple(int(x) for x in m["rows"]))
     assert sorted(map(key, gmask)) == sorted(map(key, wmask))
     assert [int(a["cells"]) for a in f8] and all(int(a["cells"]) > 0 for a in got[:len(pts)])
+
+
+def test_flood_hints_self_staged_batch(engine):
+    """A batch without small-window floods is staged by the tiled engine's own first workgroups, and a hinted
+    tile can be queued by another workgroup than the one that stores its job: every job is stored before any
+    tile is queued. 440 step-4 exact floods (301^2 windows, tiled) with hints in reverse job order (so the
+    hints of the last jobs are staged by the first staging workgroup, their jobs by the second), run three
+    times with the hints in a new order each time: results and counted-cell masks equal to dsx_flood's."""
+    dsx = load_package().dsx
+    rng = np.random.default_rng(5)
+    rows = [(u64(r["seed"]), 0, 0.35, (r["x"] >> 2) + dx, (r["z"] >> 2) + dz, 4, 150, 0)
+            for r in FINDS for dx, dz in ((0, 0), (37, -23))]
+    assert len(rows) >= 400
+    jobs = dsx.flood_jobs(rows)
+    want = engine.flood(jobs)
+    key = lambda m: (int(m["job"]), int(m["tx"]), int(m["tz"]), tuple(int(x) for x in m["rows"]))
+    wmask = sorted(map(key, engine.flood_masks()))
+    t = 150 // 32   # the start tile (150, 150) and its neighbours in the 10 x 10 tile window
+    hints = [(k, t + dx, t + dz, 0xFFFF) for k in reversed(range(len(rows))) for dx in (-1, 0, 1) for dz in (-1, 0, 1)]
+    for rep in range(3):
+        h = np.array(hints if rep == 0 else [hints[i] for i in rng.permutation(len(hints))], dtype=dsx.FLOOD_HINT)
+        got = engine.flood_hinted(jobs, h)
+        assert [tuple(a) for a in got] == [tuple(a) for a in want], rep
+        assert sorted(map(key, engine.flood_masks())) == wmask, rep
+    assert sum(int(a["cells"]) > 0 for a in want) > 200
