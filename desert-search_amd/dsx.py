@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libdsx.so")
 
 DSX_OK, DSX_E_ARG, DSX_E_HIP, DSX_E_CAPACITY, DSX_E_NODEVICE, DSX_E_BUSY, DSX_E_INTERRUPTED = 0, 1, 2, 3, 4, 5, 6
 DSX_NOALLOC_MAX_COUNT, DSX_NOALLOC_MAX_PHASE = 1 << 22, 1 << 16   # dsx_gate_noalloc limits per call
-HW_QUEUES = 20   # hardware queues requested from the HIP runtime (16 pipeline workers, the gate, spares)
+HW_QUEUES = 20   # hardware queues requested from the HIP runtime (pipeline workers, the gate; more measured slower)
 HW_QUEUES_MAX = 28   # the most measured since the flood engine's stream-ordering fix (host/hwq.h)
 
 

@@ -3,7 +3,9 @@
 #include <cstdlib>
 #include <string>
 
-// 20 queues serve the 16 pipeline worker streams, the gate and spares (20 measured as fast as 24 or 28).
+// 20 queues serve the pipeline worker streams (20 with the tuned settings), the gate and the server's streams;
+// streams beyond the queue count share queues. With blocking host waits and 20-24 workers, 16 or 20 queues
+// measured the same and 24 or 28 queues far slower (about two thirds of the rate): keep 20.
 // DSX_HW_QUEUES overrides the count, clamped to [1, 28]: 28 is the most measured since the stream-ordering
 // fix of the flood engine's initialisation; the time-limit stalls once seen at 32 queues came before that
 // fix, and 32 has not been measured since.
