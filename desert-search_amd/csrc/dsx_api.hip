@@ -450,6 +450,14 @@ int dsx_gate_amps(dsx_ctx *c, const uint64_t *seeds, uint64_t n, const dsx_gate_
     const dsx::GateKernel kpre = gate_kernel(4, p->box ? p->sig : 1.0f);
     const dsx::GateKernel kfull = gate_kernel(7, 2.0f);
     const int mode_pre = p->box ? 0 : 2;
+    // (tests) DSX_GATE_AMPS_FAST = 1, 2 or 3: pre gets the fast pre-gate amplitudes of that decision kernel
+    // (GATE_SPLIT, GATE_SPLIT_T, GATE_SPLIT_Q; the last two for box = 1 only) and full their margins
+    int fast_dec = dsx::GATE_EXACT;
+    if (const char *e = std::getenv("DSX_GATE_AMPS_FAST"); e && *e) {
+        fast_dec = std::atoi(e);
+        if (fast_dec < dsx::GATE_SPLIT || fast_dec > dsx::GATE_SPLIT_Q || (fast_dec != dsx::GATE_SPLIT && !p->box))
+            return DSX_E_ARG;
+    }
     for (uint64_t b = 0; b < n; b += c->cap1) {
         const uint32_t m = uint32_t(std::min<uint64_t>(c->cap1, n - b));
         HIPCK(hipMemcpyAsync(c->d_seeds, seeds + b, size_t(m) * 8, hipMemcpyHostToDevice, c->st));
@@ -457,12 +465,13 @@ int dsx_gate_amps(dsx_ctx *c, const uint64_t *seeds, uint64_t n, const dsx_gate_
         dsx::launch_init(c->st, 0, nullptr, c->d_seeds, m, c->d_sides);
         HIPCK(hipGetLastError());
         if (pre) {
-            dsx::launch_amp(c->st, mode_pre, dsx::GATE_EXACT, c->d_sides, nullptr, m, kpre, 1e300, nullptr, c->d_i2, nullptr, c->d_cnt,
-                            c->cap1, c->d_all);
+            dsx::launch_amp(c->st, mode_pre, fast_dec, c->d_sides, nullptr, m, kpre, 1e300, nullptr, c->d_i2,
+                            fast_dec ? c->d_a3 : nullptr, c->d_cnt, c->cap1, c->d_all, nullptr, c->d_k3, c->d_cnt + 2);
             HIPCK(hipGetLastError());
             HIPCK(hipMemcpyAsync(pre + b, c->d_all, size_t(m) * 4, hipMemcpyDeviceToHost, c->st));
+            if (fast_dec && full) HIPCK(hipMemcpyAsync(full + b, c->d_a3, size_t(m) * 4, hipMemcpyDeviceToHost, c->st));
         }
-        if (full) {
+        if (full && !fast_dec) {
             dsx::launch_amp(c->st, 1, dsx::GATE_EXACT, c->d_sides, nullptr, m, kfull, 1e300, nullptr, c->d_i2, nullptr, c->d_cnt + 1,
                             c->cap1, c->d_a3);
             HIPCK(hipGetLastError());

@@ -82,8 +82,9 @@ void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64
                  const uint32_t *n_dev = nullptr);
 // Amplitude kernel k_amp<mode, dec> (gate.hip): dec = GATE_EXACT, or GATE_SPLIT (modes 0 and 2), whose
 // undecided list is und_idx / und_count, or GATE_SPLIT_T (mode 0): GATE_SPLIT with the blur's outermost taps
-// left out of the fast amplitude, its decision margin widened by GateKernel::trunc.
-enum { GATE_EXACT = 0, GATE_SPLIT = 1, GATE_SPLIT_T = 2 };
+// left out of the fast amplitude, its decision margin widened by GateKernel::trunc, or GATE_SPLIT_Q (mode 0):
+// GATE_SPLIT_T with the grid summed in 16-bit fixed point, the margin widened by 2.9e-4 per unit of grid bound.
+enum { GATE_EXACT = 0, GATE_SPLIT = 1, GATE_SPLIT_T = 2, GATE_SPLIT_Q = 3 };
 void launch_amp(hipStream_t st, int mode, int dec, const SideParams *sides, const uint32_t *sel, uint32_t n,
                 const GateKernel &K, double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count,
                 uint32_t cap, float *all_amp, const uint32_t *n_dev = nullptr, uint32_t *und_idx = nullptr,

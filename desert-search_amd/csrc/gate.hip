@@ -94,25 +94,6 @@ __global__ __launch_bounds__(64) void k_init(uint64_t base, const uint32_t *ks, 
 // 256^2 even-parity pre-gate (R=4). Grid arithmetic and blur: grid_blur.h.
 template <int MODE> struct GateR { static constexpr int R = (MODE == 1) ? 7 : 4; };
 
-// FAST: fused multiply-add blurs; TRUNC (with FAST): the blur's outermost taps left out (radius R - 1, taps
-// 1 .. 2R - 1 of K.k), which moves the normalised result by at most K.trunc per unit of |grid value|.
-template <int MODE, bool FAST, bool TRUNC = false>
-__device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K) {
-    static_assert(FAST || !TRUNC, "only the fast amplitude may leave taps out");
-    grid_setup<MODE>(sp, L, K.wd);
-    float best = -1e9f;
-    constexpr int NC = Shape<MODE>::G / 64, R = GateR<MODE>::R - (TRUNC ? 1 : 0);
-    // only the maximum is kept, so sums may start from their first term (ZSIGN off)
-    grid_blur_rows<MODE, R, false, FAST>(sp, L, K.k + (TRUNC ? 1 : 0), 0, Shape<MODE>::G - 1, [&](int, const float (&o)[NC]) {
-#pragma unroll
-        for (int q = 0; q < NC; q++) best = fmaxf(best, o[q]);
-    });
-    // wave maximum; x -> fl(x * norm) is monotone, so max(s) * norm == max(s * norm)
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
-    return best * K.norm;
-}
-
 // A bound on |grid value| of one side (the corner tables must be set up): a value sums one entry of each
 // sub-table of its cell's set (MODE 0: all 18; MODE 1: the 8 of one of the 4 parities; MODE 2: all 8), so
 // the sum of those sub-tables' largest |entry| bounds it, up to the float roundings of both sums (covered
@@ -133,6 +114,36 @@ __device__ float side_bound(const WaveLds &L) {
     return m;
 }
 
+// FAST: fused multiply-add blurs; TRUNC (with FAST): the blur's outermost taps left out (radius R - 1, taps
+// 1 .. 2R - 1 of K.k), which moves the normalised result by at most K.trunc per unit of |grid value|.
+// Q16 (with FAST, MODE 0): the grid in 16-bit fixed point (grid_setup_q16), each value within 2.83e-4 * b of
+// the float one, b = the side's grid bound, which it stores in *bound.
+template <int MODE, bool FAST, bool TRUNC = false, bool Q16 = false>
+__device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K, float *bound = nullptr) {
+    static_assert(FAST || !TRUNC, "only the fast amplitude may leave taps out");
+    static_assert(!Q16 || (FAST && MODE == 0), "the fixed-point grid is a fast-amplitude form of MODE 0");
+    grid_setup<MODE>(sp, L, K.wd);
+    float inv_s = 1.f;
+    if constexpr (Q16) {
+        const float b = side_bound<MODE>(L);
+        *bound = b;
+        inv_s = 1.f / grid_setup_q16(L, b);
+    }
+    float best = -1e9f;
+    constexpr int NC = Shape<MODE>::G / 64, R = GateR<MODE>::R - (TRUNC ? 1 : 0);
+    // only the maximum is kept, so sums may start from their first term (ZSIGN off)
+    grid_blur_rows<MODE, R, false, FAST, Q16>(sp, L, K.k + (TRUNC ? 1 : 0), 0, Shape<MODE>::G - 1,
+                                              [&](int, const float (&o)[NC]) {
+#pragma unroll
+        for (int q = 0; q < NC; q++) best = fmaxf(best, o[q]);
+    });
+    // wave maximum; x -> fl(x * norm) is monotone, so max(s) * norm == max(s * norm)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
+    // fixed point: the maximum in units of 1 / S (two more roundings, a few ulps of |amp|: inside gate_eps)
+    return Q16 ? best * (K.norm * inv_s) : best * K.norm;
+}
+
 // Decision margin of the fast amplitude (FMA blurs) for sides bounded by bA, bB. Each blur pass of W <= 15
 // taps, fused or not, is within gamma_15 * sum_d k_d |input| of its exact value (gamma_15 = 15u / (1 - 15u)
 // <= 9e-7, u = 2^-24), so the fused and the reference result of the two passes differ by at most
@@ -147,9 +158,12 @@ __device__ __forceinline__ double gate_eps(float bA, float bB) { return 2e-5 * (
 //   GATE_EXACT  the reference's arithmetic: seeds with amplitude >= thr are appended as
 //               (kmap ? kmap[pair] : pair, amplitude); all_amp (optional) gets every amplitude, by block
 //   GATE_SPLIT  the fast amplitude (FMA blurs): pairs of seeds certainly at or above thr (amp - eps >= thr)
-//               to out_idx, those within eps of thr to und_idx (to be decided by GATE_EXACT)
+//               to out_idx, those within eps of thr to und_idx (to be decided by GATE_EXACT); all_amp
+//               (optional) gets every fast amplitude and out_amp (optional, with all_amp) its margin eps
 //   GATE_SPLIT_T the same with the blur's outermost taps left out (radius R - 1 in both passes: a quarter
 //               fewer blur operations at R = 4), eps widened by K.trunc * (bA + bB)
+//   GATE_SPLIT_Q GATE_SPLIT_T with the grid summed in 16-bit fixed point, two cells per add (MODE 0),
+//               eps widened by a further 2.9e-4 * (bA + bB)
 // The fast decisions equal the exact ones: the two amplitudes differ by less than eps (gate_eps, plus the
 // truncation bound: dropping taps changes the exact blurred value by at most norm * (ksum^2 - ksum_t^2) times
 // the largest |grid value|, itself at most the side's bound b).
@@ -165,9 +179,10 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
     if (item >= n) return;
     const uint32_t pidx = sel ? sel[item] : item;
     const int wv = threadIdx.x >> 6;
-    const float m = side_max<MODE, DEC != GATE_EXACT, DEC == GATE_SPLIT_T>(sides[2 * size_t(pidx) + wv], L[wv], K);
     float b = 0.f;
-    if (DEC != GATE_EXACT) b = side_bound<MODE>(L[wv]);
+    const float m = side_max<MODE, DEC != GATE_EXACT, DEC == GATE_SPLIT_T || DEC == GATE_SPLIT_Q, DEC == GATE_SPLIT_Q>(
+        sides[2 * size_t(pidx) + wv], L[wv], K, &b);
+    if (DEC == GATE_SPLIT || DEC == GATE_SPLIT_T) b = side_bound<MODE>(L[wv]);
     if ((threadIdx.x & 63) == 0) {
         red[wv] = m;
         bnd[wv] = b;
@@ -187,9 +202,15 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
                 }
             }
         } else {
-            // a truncated blur moves each side's maximum by at most K.trunc times its grid bound
-            const double eps = gate_eps(bnd[0], bnd[1]) +
-                               (DEC == GATE_SPLIT_T ? double(K.trunc) * (double(bnd[0]) + double(bnd[1])) : 0.0);
+            // a truncated blur moves each side's maximum by at most K.trunc times its grid bound, the
+            // fixed-point grid by at most 2.83e-4 times it
+            const double bs = double(bnd[0]) + double(bnd[1]);
+            const double eps = gate_eps(bnd[0], bnd[1]) + (DEC != GATE_SPLIT ? double(K.trunc) * bs : 0.0) +
+                               (DEC == GATE_SPLIT_Q ? 2.9e-4 * bs : 0.0);
+            if (all_amp) {   // (tests) the fast amplitude and its margin, by block
+                all_amp[item] = amp;
+                if (out_amp) out_amp[item] = float(eps);
+            }
             if (double(amp) - eps >= thr) {
                 const uint32_t i = atomicAdd(count, 1u);
                 if (i < cap) out_idx[i] = pidx;
@@ -247,7 +268,13 @@ std::vector<GateLaunch> gate_launches(const GateChain &C, const GateStages &S) {
             const char *e = std::getenv("DSX_GATE_TRUNC");
             return !(e && *e == '0');
         }();
-        const int dec = (trunc_on && mp == 0 && S.kpre.trunc > 0.f && S.kpre.trunc <= 2e-3f) ? GATE_SPLIT_T : GATE_SPLIT;
+        // and the grid in 16-bit fixed point (2.9e-4 more); DSX_GATE_Q16=0 keeps the float grid sums
+        static const bool q16_on = [] {
+            const char *e = std::getenv("DSX_GATE_Q16");
+            return !(e && *e == '0');
+        }();
+        const bool tr = trunc_on && mp == 0 && S.kpre.trunc > 0.f && S.kpre.trunc <= 2e-3f;
+        const int dec = tr ? (q16_on ? GATE_SPLIT_Q : GATE_SPLIT_T) : GATE_SPLIT;
         v.push_back({mp, dec, true, nullptr, nullptr, C.i2, nullptr, C.cnt + 1, C.cnt + 3, C.k3, C.cnt + 4});
         v.push_back({mp, GATE_EXACT, true, C.k3, nullptr, C.i2, nullptr, C.cnt + 1, C.cnt + 4, nullptr, nullptr});
     } else {
@@ -259,6 +286,7 @@ std::vector<GateLaunch> gate_launches(const GateChain &C, const GateStages &S) {
 
 const void *amp_kernel(int mode, int dec) {
     if (dec == GATE_SPLIT_T && mode == 0) return reinterpret_cast<const void *>(&k_amp<0, GATE_SPLIT_T>);
+    if (dec == GATE_SPLIT_Q && mode == 0) return reinterpret_cast<const void *>(&k_amp<0, GATE_SPLIT_Q>);
     if (dec != GATE_EXACT)
         return mode == 0 ? reinterpret_cast<const void *>(&k_amp<0, GATE_SPLIT>)
                          : reinterpret_cast<const void *>(&k_amp<2, GATE_SPLIT>);

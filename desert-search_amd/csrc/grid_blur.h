@@ -52,6 +52,11 @@ __device__ __forceinline__ float tab_at(const float *tab, int sub, uint32_t off)
     return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(tab) + sub * 64 + off);
 }
 __device__ __forceinline__ uint32_t byte_of(uint32_t dw, int b) { return (dw >> (8 * b)) & 0xffu; }
+// Two 16-bit integers in one register: sums of them are packed 16-bit adds (two cells per instruction)
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ s16x2 tabq_at(const float *tab, int sub, uint32_t off) {
+    return *reinterpret_cast<const s16x2 *>(reinterpret_cast<const char *>(tab) + sub * 64 + off);
+}
 // The value of the lane to the left (lane - 1, lane 0 takes lane 63's) and to the right (lane + 1, lane 63
 // takes lane 0's): DPP wavefront rotates (wave_ror:1, wave_rol:1), which use no LDS bandwidth
 __device__ __forceinline__ float from_left(float v) {
@@ -124,6 +129,34 @@ __device__ void grid_setup(const SideParams &sp, WaveLds &L, double wd) {
     __syncthreads();
 }
 
+// MODE 0 grid in 16-bit fixed point, for the fast amplitude only. The 18 float sub-tables T(u, v, dy) set up
+// by grid_setup<0> are scaled by S = 32000 / b (b: the side's grid bound, sum of the sub-tables' largest
+// |entry|) and rounded to integers, then packed for a pair of adjacent cells (c, c + 1): table
+// (v * 2 + dy) * 4 + p holds, for lattice column c + p, the low half round(S T(p, v, dy)) (cell c, p <= 2)
+// and the high half round(S T(p - 1, v, dy)) (cell c + 1, p >= 1), 0 where a cell reads no such term. A
+// cell's 18 terms then add up exactly in 16 bits: every partial sum is at most S b + 18 * 0.502 < 32767 in
+// magnitude. Each term is within 0.502 of S times its float (the rounding to an integer, plus the float
+// product's error of at most 32000 * 2^-24), so the integer sum is within 9.04 of S times the exact sum,
+// i.e. within 2.83e-4 * b after the division by S. Every wave of the block must call it (block barriers);
+// returns S (1 when b is below 1e-20: then every term rounds to 0, off by less than b).
+__device__ float grid_setup_q16(WaveLds &L, float b) {
+    const int lane = threadIdx.x & 63;
+    const float S = b > 1e-20f ? 32000.f / b : 1.f;
+    uint32_t w[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int e = lane + 64 * k, sub = e >> 4, h = e & 15, vd = sub >> 2, p = sub & 3;
+        const int lo = p <= 2 ? int(rintf(L.tab[(p * 6 + vd) * 16 + h] * S)) : 0;
+        const int hi = p >= 1 ? int(rintf(L.tab[((p - 1) * 6 + vd) * 16 + h] * S)) : 0;
+        w[k] = (uint32_t(lo) & 0xffffu) | (uint32_t(hi) << 16);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 6; k++) reinterpret_cast<uint32_t *>(L.tab)[lane + 64 * k] = w[k];
+    __syncthreads();
+    return S;
+}
+
 // Walks output rows [o_lo, o_hi] (0 <= o_lo <= o_hi < G); epi(o, s[NC]) receives each blurred row
 // segment of this lane (columns NC*lane .. NC*lane+NC-1), unnormalised.
 // ZSIGN: every sum starts from 0 as the reference's loops do. Without it (for callers that keep only the
@@ -133,7 +166,8 @@ __device__ void grid_setup(const SideParams &sp, WaveLds &L, double wd) {
 // 15 fewer instructions per pass and cell). Not the reference's arithmetic: each pass is then within
 // gamma_W = W u / (1 - W u) (u = 2^-24) of the exact dot product times sum |k_d| |input|, as the
 // unfused pass is, so the two results differ by at most 2 gamma_W of that per pass (see gate.hip).
-template <int MODE, int R, bool ZSIGN = true, bool FMA = false, class Epi>
+// Q16 (MODE 0 only, after grid_setup_q16): the grid values are the 16-bit fixed-point sums, in units of 1/S.
+template <int MODE, int R, bool ZSIGN = true, bool FMA = false, bool Q16 = false, class Epi>
 __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const float *kern, int o_lo, int o_hi, Epi &&epi) {
     using S = Shape<MODE>;
     constexpr int G = S::G, NC = G / 64, W = 2 * R + 1;
@@ -183,7 +217,24 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
         for (int t = 0; t < 2; t++) {
             const int j = j0 + t;
             float V[NC];
-            if (MODE == 0) {
+            if (MODE == 0 && Q16) {
+                // cells (2 pq, 2 pq + 1) together: lattice column 2 pq + p of each (v, dy) term
+#pragma unroll
+                for (int pq = 0; pq < NC / 2; pq++) {
+                    s16x2 acc = {0, 0};
+#pragma unroll
+                    for (int v = 0; v < 3; v++)
+#pragma unroll
+                        for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+                            for (int p = 0; p < 4; p++) {
+                                const s16x2 e = tabq_at(tab, (v * 2 + dy) * 4 + p, byte_of(dw[(2 * pq + p) * 2 + dy], t + v));
+                                acc = (v == 0 && dy == 0 && p == 0) ? e : acc + e;
+                            }
+                    V[2 * pq] = float(int(acc.x));
+                    V[2 * pq + 1] = float(int(acc.y));
+                }
+            } else if (MODE == 0) {
 #pragma unroll
                 for (int q = 0; q < NC; q++) {
                     float acc = 0;

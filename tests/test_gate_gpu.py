@@ -168,6 +168,33 @@ def test_gate_graph_equals_eager(engine):
         engine.set_gate_graph(1)
 
 
+def test_fast_amplitude_within_margin(engine):
+    """Each fast pre-gate amplitude (1: FMA blurs; 2: FMA blurs of radius 3; 3: the same over the grid summed
+    in 16-bit fixed point, two cells per add) lies within its decision margin of the reference amplitude, on
+    2^18 random seeds (any seed has a pre-gate amplitude); the fast kernels only decide seeds farther than
+    that margin from the threshold, so this is what keeps their decisions the reference's. The margins grow
+    with each approximation and the fixed-point amplitudes differ from the float ones."""
+    import os
+    seeds = np.random.default_rng(11).integers(0, 1 << 63, 1 << 18, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+    exact, _ = engine.gate_amps(seeds)
+    fast, eps = {}, {}
+    try:
+        for dec in (1, 2, 3):
+            os.environ["DSX_GATE_AMPS_FAST"] = str(dec)
+            fast[dec], eps[dec] = engine.gate_amps(seeds)
+    finally:
+        os.environ.pop("DSX_GATE_AMPS_FAST", None)
+    for dec in (1, 2, 3):
+        d = np.abs(fast[dec].astype(np.float64) - exact.astype(np.float64))
+        r = d / eps[dec].astype(np.float64)
+        print(f"fast amplitude {dec}: max |fast - exact| {d.max():.3e}, max ratio to margin {r.max():.3f}, "
+              f"median margin {np.median(eps[dec]):.3e}", flush=True)
+        assert np.all(np.isfinite(fast[dec])) and np.all(eps[dec] > 0)
+        assert np.all(d <= eps[dec])
+    assert np.all(eps[2] >= eps[1]) and np.all(eps[3] > eps[2])
+    assert not np.array_equal(fast[2].view(np.uint32), fast[3].view(np.uint32))
+
+
 def test_fast_decisions_at_exact_thresholds(engine):
     """The graph chain decides a seed with the fast amplitude (FMA blurs) only when that amplitude is
     farther from the threshold than its proven error bound, and runs the reference's arithmetic for the
