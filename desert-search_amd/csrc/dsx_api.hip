@@ -44,6 +44,25 @@ dsx::GateKernel dsx::gate_kernel(int R, float sig) {
         }
         K.trunc = std::nextafter(float((s * s - st * st) * double(K.norm) * 1.001), INFINITY);
     }
+    // integer weights of the truncated blur's horizontal pass (the 16-bit fast pre-gate, R = 4): rounding
+    // them moves a horizontally blurred value by at most sum_d |kq_d 2^-14 - k_d| times the largest |grid
+    // value|, and the normalised output by that times the vertical taps' sum times norm (with 1e-3 for this
+    // evaluation's roundings and for the fixed-point grid's own excess over the grid bound)
+    if (R == 4) {
+        int kq[9] = {0};
+        double dk = 0, st = 0;
+        for (int t = 1; t < 2 * R; t++) {
+            kq[t] = int(std::lrint(double(K.k[t]) * 16384.0));
+            dk += std::fabs(kq[t] / 16384.0 - double(K.k[t]));
+            st += K.k[t];
+        }
+        const auto pk = [](int lo, int hi) { return (uint32_t(lo) & 0xffffu) | (uint32_t(hi) << 16); };
+        for (int m = 0; m < 4; m++) {
+            K.kq2[m] = pk(m == 0 ? 0 : kq[2 * m], kq[2 * m + 1]);
+            K.kq2[4 + m] = pk(kq[2 * m + 1], m == 3 ? 0 : kq[2 * m + 2]);
+        }
+        K.wq = std::nextafter(float(dk * st * double(K.norm) * 1.001), INFINITY);
+    }
     return K;
 }
 bool dsx::valid_gate_params(const dsx_gate_params *p) {

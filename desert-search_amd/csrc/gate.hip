@@ -136,12 +136,13 @@ __device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K,
                                               [&](int, const float (&o)[NC]) {
 #pragma unroll
         for (int q = 0; q < NC; q++) best = fmaxf(best, o[q]);
-    });
+    }, K.kq2);
     // wave maximum; x -> fl(x * norm) is monotone, so max(s) * norm == max(s * norm)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
-    // fixed point: the maximum in units of 1 / S (two more roundings, a few ulps of |amp|: inside gate_eps)
-    return Q16 ? best * (K.norm * inv_s) : best * K.norm;
+    // fixed point: the maximum in units of 1 / S, or 2^-14 / S after the integer horizontal pass (R = 3; two
+    // more roundings, a few ulps of |amp|: inside gate_eps)
+    return Q16 ? best * (K.norm * (R == 3 ? inv_s * 0x1p-14f : inv_s)) : best * K.norm;
 }
 
 // Decision margin of the fast amplitude (FMA blurs) for sides bounded by bA, bB. Each blur pass of W <= 15
@@ -162,8 +163,9 @@ __device__ __forceinline__ double gate_eps(float bA, float bB) { return 2e-5 * (
 //               (optional) gets every fast amplitude and out_amp (optional, with all_amp) its margin eps
 //   GATE_SPLIT_T the same with the blur's outermost taps left out (radius R - 1 in both passes: a quarter
 //               fewer blur operations at R = 4), eps widened by K.trunc * (bA + bB)
-//   GATE_SPLIT_Q GATE_SPLIT_T with the grid summed in 16-bit fixed point, two cells per add (MODE 0),
-//               eps widened by a further 2.9e-4 * (bA + bB)
+//   GATE_SPLIT_Q GATE_SPLIT_T with the grid summed in 16-bit fixed point, two cells per add, and its
+//               horizontal pass in 16-bit integer dot products (MODE 0), eps widened by a further
+//               (2.9e-4 + K.wq) * (bA + bB)
 // The fast decisions equal the exact ones: the two amplitudes differ by less than eps (gate_eps, plus the
 // truncation bound: dropping taps changes the exact blurred value by at most norm * (ksum^2 - ksum_t^2) times
 // the largest |grid value|, itself at most the side's bound b).
@@ -203,10 +205,11 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
             }
         } else {
             // a truncated blur moves each side's maximum by at most K.trunc times its grid bound, the
-            // fixed-point grid by at most 2.83e-4 times it
+            // fixed-point grid by at most 2.83e-4 times it, the integer weights of its horizontal pass by at
+            // most K.wq times it
             const double bs = double(bnd[0]) + double(bnd[1]);
             const double eps = gate_eps(bnd[0], bnd[1]) + (DEC != GATE_SPLIT ? double(K.trunc) * bs : 0.0) +
-                               (DEC == GATE_SPLIT_Q ? 2.9e-4 * bs : 0.0);
+                               (DEC == GATE_SPLIT_Q ? (2.9e-4 + double(K.wq)) * bs : 0.0);
             if (all_amp) {   // (tests) the fast amplitude and its margin, by block
                 all_amp[item] = amp;
                 if (out_amp) out_amp[item] = float(eps);

@@ -65,6 +65,12 @@ __device__ __forceinline__ float from_left(float v) {
 __device__ __forceinline__ float from_right(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, false));
 }
+__device__ __forceinline__ s16x2 from_left(s16x2 v) {
+    return __builtin_bit_cast(s16x2, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, false));
+}
+__device__ __forceinline__ s16x2 from_right(s16x2 v) {
+    return __builtin_bit_cast(s16x2, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x134, 0xF, 0xF, false));
+}
 // One step of a sum that starts from 0: s + x, or x itself for the sum's first term when ZSIGN is off
 // (the two differ only when x is -0; see grid_blur_rows)
 template <bool ZSIGN> __device__ __forceinline__ float sum_step(float s, float x, bool first) {
@@ -167,8 +173,13 @@ __device__ float grid_setup_q16(WaveLds &L, float b) {
 // gamma_W = W u / (1 - W u) (u = 2^-24) of the exact dot product times sum |k_d| |input|, as the
 // unfused pass is, so the two results differ by at most 2 gamma_W of that per pass (see gate.hip).
 // Q16 (MODE 0 only, after grid_setup_q16): the grid values are the 16-bit fixed-point sums, in units of 1/S.
+// At R = 3 (with FMA) the horizontal pass then runs on them in integer arithmetic: 16-bit dot products
+// (v_dot2_i32_i16) of two adjacent cells with the weight pairs kq2 (the W weights times 2^14, rounded;
+// kq2[0..3] = (0, k0), (k1, k2), (k3, k4), (k5, k6) for a pair's first cell, kq2[4..7] = (k0, k1), (k2, k3),
+// (k4, k5), (k6, 0) for its second), so the blurred values come out in units of 2^-14 / S.
 template <int MODE, int R, bool ZSIGN = true, bool FMA = false, bool Q16 = false, class Epi>
-__device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const float *kern, int o_lo, int o_hi, Epi &&epi) {
+__device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const float *kern, int o_lo, int o_hi, Epi &&epi,
+                               const uint32_t *kq2 = nullptr) {
     using S = Shape<MODE>;
     constexpr int G = S::G, NC = G / 64, W = 2 * R + 1;
     static_assert(R <= NC, "halo must come from the adjacent lane only");
@@ -201,12 +212,29 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
     float kw[W];
 #pragma unroll
     for (int d = 0; d < W; d++) kw[d] = kern[d];
-    float ring[W][NC];
+    constexpr bool QH = Q16 && FMA && R == 3;
+    static_assert(!QH || NC == 4, "the integer horizontal pass takes two cell pairs per lane");
+    s16x2 kq[8];
+    if constexpr (QH) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) kq[m] = __builtin_bit_cast(s16x2, kq2[m]);
+    }
+    // The vertical window: with UNR (the fast 256^2 grid at radius 3 or less) row j sits in slot
+    // (j - js) % WR of a ring of WR = W + 1 slots and the loop runs WR / 2 row pairs per iteration, so every
+    // slot index is a constant and nothing moves; otherwise the W rows shift down one slot per row. (At
+    // radius 4 the unrolled loop needs 153 VGPRs: 3 waves per SIMD instead of 4.)
+    constexpr bool UNR = FMA && MODE == 0 && R <= 3;
+    constexpr int WR = UNR ? W + 1 : W, PU = UNR ? WR / 2 : 1;
+    float ring[WR][NC];
     const int js = (o_lo - R) & ~1;                        // first row (even)
     const int npair = (o_hi + R - js + 2) / 2;
     const float *tab = L.tab;
 #pragma unroll 1
-    for (int pr = 0; pr < npair; pr++) {
+    for (int pr0 = 0; pr0 < npair; pr0 += PU)
+#pragma unroll
+    for (int u = 0; u < PU; u++) {
+        const int pr = pr0 + u;
+        if (UNR && pr >= npair) break;
         const int j0 = js + 2 * pr;
         const int zoff = (MODE == 1) ? (j0 >> 1) : j0;
         uint32_t dw[S::NSTREAM];
@@ -217,6 +245,7 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
         for (int t = 0; t < 2; t++) {
             const int j = j0 + t;
             float V[NC];
+            s16x2 P[NC / 2];   // (Q16) the cell pairs' packed sums
             if (MODE == 0 && Q16) {
                 // cells (2 pq, 2 pq + 1) together: lattice column 2 pq + p of each (v, dy) term
 #pragma unroll
@@ -231,8 +260,11 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
                                 const s16x2 e = tabq_at(tab, (v * 2 + dy) * 4 + p, byte_of(dw[(2 * pq + p) * 2 + dy], t + v));
                                 acc = (v == 0 && dy == 0 && p == 0) ? e : acc + e;
                             }
-                    V[2 * pq] = float(int(acc.x));
-                    V[2 * pq + 1] = float(int(acc.y));
+                    P[pq] = acc;
+                    if constexpr (!QH) {
+                        V[2 * pq] = float(int(acc.x));
+                        V[2 * pq + 1] = float(int(acc.y));
+                    }
                 }
             } else if (MODE == 0) {
 #pragma unroll
@@ -280,24 +312,46 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
             // permutes for the 512^2 grid (its 2 x 7 halo values per row as rotates measured +8 %)
             float Wv[NC + 2 * R];
             const int left = (lane + 63) & 63, right = (lane + 1) & 63;
+            float Hq[NC];   // (QH) this row's horizontal pass
+            if constexpr (QH) {
+                // pairs X[0..5] = the left lane's two, this lane's two, the right lane's two; cell c's 7 taps
+                // are the 4 pairs from X[(c + 1) / 2] with the weight pairs of its parity
+                const s16x2 X[6] = {from_left(P[0]), from_left(P[1]), P[0], P[1], from_right(P[0]), from_right(P[1])};
 #pragma unroll
-            for (int q = 0; q < R; q++) Wv[q] = G == 256 ? from_left(V[NC - R + q]) : __shfl(V[NC - R + q], left);
+                for (int c = 0; c < NC; c++) {
+                    int h = 0;
 #pragma unroll
-            for (int q = 0; q < NC; q++) Wv[R + q] = V[q];
+                    for (int m = 0; m < 4; m++) h = __builtin_amdgcn_sdot2(X[(c + 1) / 2 + m], kq[(c & 1) * 4 + m], h, false);
+                    Hq[c] = float(h);
+                }
+            } else {
 #pragma unroll
-            for (int q = 0; q < R; q++) Wv[R + NC + q] = G == 256 ? from_right(V[q]) : __shfl(V[q], right);
+                for (int q = 0; q < R; q++) Wv[q] = G == 256 ? from_left(V[NC - R + q]) : __shfl(V[NC - R + q], left);
 #pragma unroll
-            for (int k = 0; k < W - 1; k++)
+                for (int q = 0; q < NC; q++) Wv[R + q] = V[q];
 #pragma unroll
-                for (int q = 0; q < NC; q++) ring[k][q] = ring[k + 1][q];
+                for (int q = 0; q < R; q++) Wv[R + NC + q] = G == 256 ? from_right(V[q]) : __shfl(V[q], right);
+            }
+            if constexpr (!UNR) {
+#pragma unroll
+                for (int k = 0; k < W - 1; k++)
+#pragma unroll
+                    for (int q = 0; q < NC; q++) ring[k][q] = ring[k + 1][q];
+            }
+            // slot of row j - W + 1 + d, the d-th row of the window ending at row j (d = W - 1: row j)
+            const auto slot = [&](int d) { return UNR ? (2 * u + t - (W - 1) + d + 2 * WR) % WR : d; };
 #pragma unroll
             for (int q = 0; q < NC; q++) {
                 float s = 0;
+                if constexpr (QH) {
+                    s = Hq[q];
+                } else {
 #pragma unroll
-                for (int d = 0; d < W; d++)
-                    s = FMA ? (d == 0 ? kw[0] * Wv[q] : __builtin_fmaf(kw[d], Wv[q + d], s))
-                            : sum_step<ZSIGN>(s, kw[d] * Wv[q + d], d == 0);
-                ring[W - 1][q] = s;
+                    for (int d = 0; d < W; d++)
+                        s = FMA ? (d == 0 ? kw[0] * Wv[q] : __builtin_fmaf(kw[d], Wv[q + d], s))
+                                : sum_step<ZSIGN>(s, kw[d] * Wv[q + d], d == 0);
+                }
+                ring[slot(W - 1)][q] = s;
             }
             const int o = j - R;
             if (o >= o_lo && o <= o_hi) {
@@ -307,8 +361,8 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
                     float s = 0;
 #pragma unroll
                     for (int d = 0; d < W; d++)
-                        s = FMA ? (d == 0 ? kw[0] * ring[0][q] : __builtin_fmaf(kw[d], ring[d][q], s))
-                                : sum_step<ZSIGN>(s, kw[d] * ring[d][q], d == 0);
+                        s = FMA ? (d == 0 ? kw[0] * ring[slot(0)][q] : __builtin_fmaf(kw[d], ring[slot(d)][q], s))
+                                : sum_step<ZSIGN>(s, kw[d] * ring[slot(d)][q], d == 0);
                     out[q] = s;
                 }
                 epi(o, out);
