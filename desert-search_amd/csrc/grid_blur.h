@@ -65,11 +65,12 @@ __device__ __forceinline__ float from_left(float v) {
 __device__ __forceinline__ float from_right(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, false));
 }
+// (every lane receives a value, so no old value needs setting: mov_dpp)
 __device__ __forceinline__ s16x2 from_left(s16x2 v) {
-    return __builtin_bit_cast(s16x2, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, false));
+    return __builtin_bit_cast(s16x2, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, false));
 }
 __device__ __forceinline__ s16x2 from_right(s16x2 v) {
-    return __builtin_bit_cast(s16x2, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x134, 0xF, 0xF, false));
+    return __builtin_bit_cast(s16x2, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x134, 0xF, 0xF, false));
 }
 // One step of a sum that starts from 0: s + x, or x itself for the sum's first term when ZSIGN is off
 // (the two differ only when x is -0; see grid_blur_rows)
