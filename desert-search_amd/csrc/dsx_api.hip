@@ -44,10 +44,12 @@ dsx::GateKernel dsx::gate_kernel(int R, float sig) {
         }
         K.trunc = std::nextafter(float((s * s - st * st) * double(K.norm) * 1.001), INFINITY);
     }
-    // integer weights of the truncated blur's horizontal pass (the 16-bit fast pre-gate, R = 4): rounding
-    // them moves a horizontally blurred value by at most sum_d |kq_d 2^-14 - k_d| times the largest |grid
-    // value|, and the normalised output by that times the vertical taps' sum times norm (with 1e-3 for this
-    // evaluation's roundings and for the fixed-point grid's own excess over the grid bound)
+    // integer weights of the truncated blur (the 16-bit fast pre-gate, R = 4; both passes): rounding them
+    // moves a pass's output by at most sum_d |kq_d 2^-14 - k_d| times the largest |input|, so the normalised
+    // output by that times the other pass's taps' sum times norm, per pass and per unit of |grid value|; the
+    // horizontal sums, rounded to units of 2^16 (2^15 / (2^14 S) = 6.25e-5 b of a grid value each, S =
+    // 32000 / b), move it by 6.25e-5 times the vertical taps' sum times norm. wq is the sum of the three, with
+    // 1e-3 for this evaluation's roundings and for the fixed-point grid's own excess over the grid bound.
     if (R == 4) {
         int kq[9] = {0};
         double dk = 0, st = 0;
@@ -61,7 +63,7 @@ dsx::GateKernel dsx::gate_kernel(int R, float sig) {
             K.kq2[m] = pk(m == 0 ? 0 : kq[2 * m], kq[2 * m + 1]);
             K.kq2[4 + m] = pk(kq[2 * m + 1], m == 3 ? 0 : kq[2 * m + 2]);
         }
-        K.wq = std::nextafter(float(dk * st * double(K.norm) * 1.001), INFINITY);
+        K.wq = std::nextafter(float((2.0 * dk + 6.25e-5) * st * double(K.norm) * 1.001), INFINITY);
     }
     return K;
 }

@@ -27,8 +27,8 @@ struct GateKernel {
     float norm;
     float trunc;
     // R = 4 only: taps 1 .. 7 times 2^14, rounded, as 16-bit pairs (0, k1), (k2, k3), (k4, k5), (k6, k7),
-    // (k1, k2), (k3, k4), (k5, k6), (k7, 0); wq bounds the normalised change their rounding makes per unit of
-    // |grid value| in the truncated blur
+    // (k1, k2), (k3, k4), (k5, k6), (k7, 0); wq bounds the normalised change that the integer blur passes
+    // (these weights, the intermediate sums rounded to 16 bits) make per unit of |grid value|
     uint32_t kq2[8];
     float wq;
     double wd;   // octave weight the corner tables are scaled by

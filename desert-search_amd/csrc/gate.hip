@@ -17,6 +17,7 @@
 //     and hash streams live in LDS, laid out so every table read of a wave hits one 16-dword
 //     sub-table (conflict-free).
 #include "grid_blur.h"
+#include <climits>
 #include <cstdlib>
 #include <vector>
 
@@ -130,19 +131,26 @@ __device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K,
         inv_s = 1.f / grid_setup_q16(L, b);
     }
     float best = -1e9f;
+    int ibest = INT_MIN;
     constexpr int NC = Shape<MODE>::G / 64, R = GateR<MODE>::R - (TRUNC ? 1 : 0);
+    constexpr bool INTB = Q16 && R == 3;   // both blur passes in integers (grid_blur_rows, QH)
     // only the maximum is kept, so sums may start from their first term (ZSIGN off)
     grid_blur_rows<MODE, R, false, FAST, Q16>(sp, L, K.k + (TRUNC ? 1 : 0), 0, Shape<MODE>::G - 1,
-                                              [&](int, const float (&o)[NC]) {
+                                              [&](int, const auto &o) {
 #pragma unroll
-        for (int q = 0; q < NC; q++) best = fmaxf(best, o[q]);
+        for (int q = 0; q < NC; q++) {
+            if constexpr (INTB) ibest = max(ibest, o[q]);
+            else best = fmaxf(best, o[q]);
+        }
     }, K.kq2);
+    if constexpr (INTB) best = float(ibest);   // one rounding (inside gate_eps)
     // wave maximum; x -> fl(x * norm) is monotone, so max(s) * norm == max(s * norm)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
-    // fixed point: the maximum in units of 1 / S, or 2^-14 / S after the integer horizontal pass (R = 3; two
-    // more roundings, a few ulps of |amp|: inside gate_eps)
-    return Q16 ? best * (K.norm * (R == 3 ? inv_s * 0x1p-14f : inv_s)) : best * K.norm;
+    // fixed point: the maximum in units of 1 / S, or 2^-12 / S after the integer blur passes (2^14 per
+    // pass's weights, 2^-16 for the rounding between them; two more roundings, a few ulps of |amp|: inside
+    // gate_eps)
+    return Q16 ? best * (K.norm * (INTB ? inv_s * 0x1p-12f : inv_s)) : best * K.norm;
 }
 
 // Decision margin of the fast amplitude (FMA blurs) for sides bounded by bA, bB. Each blur pass of W <= 15
@@ -163,8 +171,8 @@ __device__ __forceinline__ double gate_eps(float bA, float bB) { return 2e-5 * (
 //               (optional) gets every fast amplitude and out_amp (optional, with all_amp) its margin eps
 //   GATE_SPLIT_T the same with the blur's outermost taps left out (radius R - 1 in both passes: a quarter
 //               fewer blur operations at R = 4), eps widened by K.trunc * (bA + bB)
-//   GATE_SPLIT_Q GATE_SPLIT_T with the grid summed in 16-bit fixed point, two cells per add, and its
-//               horizontal pass in 16-bit integer dot products (MODE 0), eps widened by a further
+//   GATE_SPLIT_Q GATE_SPLIT_T with the grid summed in 16-bit fixed point, two cells per add, and both blur
+//               passes in 16-bit integer dot products (MODE 0), eps widened by a further
 //               (2.9e-4 + K.wq) * (bA + bB)
 // The fast decisions equal the exact ones: the two amplitudes differ by less than eps (gate_eps, plus the
 // truncation bound: dropping taps changes the exact blurred value by at most norm * (ksum^2 - ksum_t^2) times
@@ -204,9 +212,9 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
                 }
             }
         } else {
-            // a truncated blur moves each side's maximum by at most K.trunc times its grid bound, the
-            // fixed-point grid by at most 2.83e-4 times it, the integer weights of its horizontal pass by at
-            // most K.wq times it
+            // a truncated blur moves each side's maximum by at most K.trunc times its grid bound; the
+            // fixed-point grid by at most 2.83e-4 times it, and the integer blur passes (their rounded weights,
+            // the horizontal sums rounded to 16 bits) by at most K.wq times it
             const double bs = double(bnd[0]) + double(bnd[1]);
             const double eps = gate_eps(bnd[0], bnd[1]) + (DEC != GATE_SPLIT ? double(K.trunc) * bs : 0.0) +
                                (DEC == GATE_SPLIT_Q ? (2.9e-4 + double(K.wq)) * bs : 0.0);

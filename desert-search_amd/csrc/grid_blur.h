@@ -226,7 +226,12 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
     // radius 4 the unrolled loop needs 153 VGPRs: 3 waves per SIMD instead of 4.)
     constexpr bool UNR = FMA && MODE == 0 && R <= 3;
     constexpr int WR = UNR ? W + 1 : W, PU = UNR ? WR / 2 : 1;
+    static_assert(!QH || (UNR && PU == 4), "the integer vertical pass keeps 4 row pairs");
     float ring[WR][NC];
+    // (QH) the vertical window as 16-bit row pairs: slot u % 4 holds rows (j0, j0 + 1) of pair u, each value
+    // the horizontal sum rounded to units of 2^16 (its high half after adding 2^15)
+    s16x2 hring[QH ? 4 : 1][NC] = {};
+    int hrow[2][NC];
     const int js = (o_lo - R) & ~1;                        // first row (even)
     const int npair = (o_hi + R - js + 2) / 2;
     const float *tab = L.tab;
@@ -313,17 +318,16 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
             // permutes for the 512^2 grid (its 2 x 7 halo values per row as rotates measured +8 %)
             float Wv[NC + 2 * R];
             const int left = (lane + 63) & 63, right = (lane + 1) & 63;
-            float Hq[NC];   // (QH) this row's horizontal pass
             if constexpr (QH) {
                 // pairs X[0..5] = the left lane's two, this lane's two, the right lane's two; cell c's 7 taps
                 // are the 4 pairs from X[(c + 1) / 2] with the weight pairs of its parity
                 const s16x2 X[6] = {from_left(P[0]), from_left(P[1]), P[0], P[1], from_right(P[0]), from_right(P[1])};
 #pragma unroll
                 for (int c = 0; c < NC; c++) {
-                    int h = 0;
+                    int h = 1 << 15;
 #pragma unroll
                     for (int m = 0; m < 4; m++) h = __builtin_amdgcn_sdot2(X[(c + 1) / 2 + m], kq[(c & 1) * 4 + m], h, false);
-                    Hq[c] = float(h);
+                    hrow[t][c] = h;
                 }
             } else {
 #pragma unroll
@@ -333,6 +337,7 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
 #pragma unroll
                 for (int q = 0; q < R; q++) Wv[R + NC + q] = G == 256 ? from_right(V[q]) : __shfl(V[q], right);
             }
+            if constexpr (QH) continue;   // both rows of the pair first (below)
             if constexpr (!UNR) {
 #pragma unroll
                 for (int k = 0; k < W - 1; k++)
@@ -344,14 +349,10 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
 #pragma unroll
             for (int q = 0; q < NC; q++) {
                 float s = 0;
-                if constexpr (QH) {
-                    s = Hq[q];
-                } else {
 #pragma unroll
-                    for (int d = 0; d < W; d++)
-                        s = FMA ? (d == 0 ? kw[0] * Wv[q] : __builtin_fmaf(kw[d], Wv[q + d], s))
-                                : sum_step<ZSIGN>(s, kw[d] * Wv[q + d], d == 0);
-                }
+                for (int d = 0; d < W; d++)
+                    s = FMA ? (d == 0 ? kw[0] * Wv[q] : __builtin_fmaf(kw[d], Wv[q + d], s))
+                            : sum_step<ZSIGN>(s, kw[d] * Wv[q + d], d == 0);
                 ring[slot(W - 1)][q] = s;
             }
             const int o = j - R;
@@ -367,6 +368,29 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
                     out[q] = s;
                 }
                 epi(o, out);
+            }
+        }
+        if constexpr (QH) {
+            // vertical pass over row pairs u - 3 .. u (rows j0 - 6 .. j0 + 1): output j0 - 3 takes the odd
+            // weight pairs, output j0 - 2 the even ones, as the horizontal pass does for a cell pair
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                hring[u][c] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(uint32_t(hrow[1][c]), uint32_t(hrow[0][c]), 0x07060302u));
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const int o = j0 - 3 + e;
+                if (o >= o_lo && o <= o_hi) {
+                    int out[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; c++) {
+                        int v = 0;
+#pragma unroll
+                        for (int m = 0; m < 4; m++)
+                            v = __builtin_amdgcn_sdot2(hring[(u + 1 + m) % 4][c], kq[(1 - e) * 4 + m], v, false);
+                        out[c] = v;
+                    }
+                    epi(o, out);
+                }
             }
         }
     }
