@@ -44,26 +44,30 @@ dsx::GateKernel dsx::gate_kernel(int R, float sig) {
         }
         K.trunc = std::nextafter(float((s * s - st * st) * double(K.norm) * 1.001), INFINITY);
     }
-    // integer weights of the truncated blur (the 16-bit fast pre-gate, R = 4; both passes): rounding them
-    // moves a pass's output by at most sum_d |kq_d 2^-14 - k_d| times the largest |input|, so the normalised
-    // output by that times the other pass's taps' sum times norm, per pass and per unit of |grid value|; the
-    // horizontal sums, rounded to units of 2^16 (2^15 / (2^14 S) = 6.25e-5 b of a grid value each, S =
-    // 32000 / b), move it by 6.25e-5 times the vertical taps' sum times norm. wq is the sum of the three, with
-    // 1e-3 for this evaluation's roundings and for the fixed-point grid's own excess over the grid bound.
-    if (R == 4) {
-        int kq[9] = {0};
+    // integer weights of the 16-bit fast amplitudes (R = 4: the truncated taps 1 .. 7 at SK = 2^14; R = 7: all
+    // 15 taps at SK = 2^13, so that the sums of 15 products stay below 2^31): rounding them moves a pass's
+    // output by at most sum_d |kq_d / SK - w_d| times the largest |input|, so the normalised output by that
+    // times the other pass's taps' sum times norm, per pass and per unit of |grid value|; the horizontal sums,
+    // rounded to units of 2^16 (2^15 / (SK S) = 2^15 / (32000 SK) b of a grid value each, S = 32000 / b), move
+    // it by that times the vertical taps' sum times norm. wq is the sum of the three, with 1e-3 for this
+    // evaluation's roundings and for the fixed-point grid's own excess over the grid bound.
+    if (R == 4 || R == 7) {
+        const int t0 = R == 4 ? 1 : 0, W = R == 4 ? 7 : 15, NP = (W + 1) / 2;
+        const double SK = R == 4 ? 16384.0 : 8192.0;
+        int kq[16] = {0};
         double dk = 0, st = 0;
-        for (int t = 1; t < 2 * R; t++) {
-            kq[t] = int(std::lrint(double(K.k[t]) * 16384.0));
-            dk += std::fabs(kq[t] / 16384.0 - double(K.k[t]));
-            st += K.k[t];
+        for (int d = 0; d < W; d++) {
+            kq[d] = int(std::lrint(double(K.k[t0 + d]) * SK));
+            dk += std::fabs(kq[d] / SK - double(K.k[t0 + d]));
+            st += K.k[t0 + d];
         }
+        const auto w = [&](int d) { return d < 0 || d >= W ? 0 : kq[d]; };
         const auto pk = [](int lo, int hi) { return (uint32_t(lo) & 0xffffu) | (uint32_t(hi) << 16); };
-        for (int m = 0; m < 4; m++) {
-            K.kq2[m] = pk(m == 0 ? 0 : kq[2 * m], kq[2 * m + 1]);
-            K.kq2[4 + m] = pk(kq[2 * m + 1], m == 3 ? 0 : kq[2 * m + 2]);
+        for (int m = 0; m < NP; m++) {
+            K.kq2[m] = pk(w(2 * m - 1), w(2 * m));
+            K.kq2[NP + m] = pk(w(2 * m), w(2 * m + 1));
         }
-        K.wq = std::nextafter(float((2.0 * dk + 6.25e-5) * st * double(K.norm) * 1.001), INFINITY);
+        K.wq = std::nextafter(float((2.0 * dk + 32768.0 / (32000.0 * SK)) * st * double(K.norm) * 1.001), INFINITY);
     }
     return K;
 }
@@ -347,9 +351,17 @@ void gate_graph_shape(const dsx_ctx *c, double maxph, uint64_t count, uint32_t &
     grid = uint32_t(std::min<uint64_t>(g, c->cap1));
 }
 
-int gate_graph_range(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *p, Pass &pass, uint64_t tot[3]) {
+// want_amps false: the caller takes no amplitudes, so the full gate may decide from its fast amplitude too
+// (DSX_GATE_FAST_FULL=0 keeps the reference's arithmetic for every full-gate amplitude).
+int gate_graph_range(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *p, Pass &pass, uint64_t tot[3],
+                     bool want_amps) {
+    static const bool fast_full_on = [] {
+        const char *e = std::getenv("DSX_GATE_FAST_FULL");
+        return !(e && *e == '0');
+    }();
     dsx::GateStages S;
     std::memset(&S, 0, sizeof S);   // compared bytewise with the graph's configuration (padding included)
+    S.full_fast = (!want_amps && fast_full_on) ? 1 : 0;
     S.maxph = double(p->max_phase);
     S.thr_pre = p->box ? double(p->pthr) : double(p->amp - p->margin);
     S.amp = double(p->amp);
@@ -366,7 +378,8 @@ int gate_graph_range(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_
     if (!same) {
         // the fast chain: decisions from the fast amplitude where provably equal, the reference's arithmetic
         // for the rest and for every amplitude written
-        const dsx::GateChain C{c->d_cnt, c->d_k1, c->d_i2, c->d_k3, c->d_a3, c->d_sides, grid, true};
+        const dsx::GateChain C{c->d_cnt, c->d_k1, c->d_i2, c->d_k3, c->d_a3, c->d_sides, grid, true,
+                               reinterpret_cast<uint32_t *>(c->d_all)};
         HIPCK(dsx::gate_graph_build(G, C, S, nmax, dsx::GateOut{nullptr, h_amps, h_kidx, c->cap1, h_counts}));
     }
     for (uint64_t base = 0; base < count; base += nmax) {
@@ -400,7 +413,8 @@ int dsx_gate(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_params *
     Pass pass;
     uint64_t tot[3] = {0, 0, 0};
     for (int i = 0; i < 4; i++) { c->prof_ms[i] = 0; c->prof_work[i] = 0; }
-    const int rc = c->gate_graph ? gate_graph_range(c, start, count, p, pass, tot) : gate_eager(c, start, 0, count, p, pass, tot);
+    const int rc = c->gate_graph ? gate_graph_range(c, start, count, p, pass, tot, amps != nullptr)
+                                 : gate_eager(c, start, 0, count, p, pass, tot);
     if (rc) return rc;
     std::sort(pass.begin(), pass.end());
     if (counts) { counts->phase = tot[0]; counts->pre = tot[1]; counts->full = tot[2]; }
@@ -472,12 +486,31 @@ int dsx_gate_amps(dsx_ctx *c, const uint64_t *seeds, uint64_t n, const dsx_gate_
     const dsx::GateKernel kfull = gate_kernel(7, 2.0f);
     const int mode_pre = p->box ? 0 : 2;
     // (tests) DSX_GATE_AMPS_FAST = 1, 2 or 3: pre gets the fast pre-gate amplitudes of that decision kernel
-    // (GATE_SPLIT, GATE_SPLIT_T, GATE_SPLIT_Q; the last two for box = 1 only) and full their margins
+    // (GATE_SPLIT, GATE_SPLIT_T, GATE_SPLIT_Q; the last two for box = 1 only) and full their margins;
+    // DSX_GATE_AMPS_FAST_FULL = 1: full gets the full gate's fast amplitudes (GATE_SPLIT_Q) and pre their margins
     int fast_dec = dsx::GATE_EXACT;
     if (const char *e = std::getenv("DSX_GATE_AMPS_FAST"); e && *e) {
         fast_dec = std::atoi(e);
         if (fast_dec < dsx::GATE_SPLIT || fast_dec > dsx::GATE_SPLIT_Q || (fast_dec != dsx::GATE_SPLIT && !p->box))
             return DSX_E_ARG;
+    }
+    const char *ef = std::getenv("DSX_GATE_AMPS_FAST_FULL");
+    const bool fast_full = ef && *ef == '1';
+    if (fast_full) {
+        if (fast_dec != dsx::GATE_EXACT || !pre || !full) return DSX_E_ARG;
+        for (uint64_t b = 0; b < n; b += c->cap1) {
+            const uint32_t m = uint32_t(std::min<uint64_t>(c->cap1, n - b));
+            HIPCK(hipMemcpyAsync(c->d_seeds, seeds + b, size_t(m) * 8, hipMemcpyHostToDevice, c->st));
+            HIPCK(hipMemsetAsync(c->d_cnt, 0, 32, c->st));
+            dsx::launch_init(c->st, 0, nullptr, c->d_seeds, m, c->d_sides);
+            dsx::launch_amp(c->st, 1, dsx::GATE_SPLIT_Q, c->d_sides, nullptr, m, kfull, 1e300, nullptr, c->d_i2, c->d_a3,
+                            c->d_cnt, c->cap1, c->d_all, nullptr, c->d_k3, c->d_cnt + 2);
+            HIPCK(hipGetLastError());
+            HIPCK(hipMemcpyAsync(full + b, c->d_all, size_t(m) * 4, hipMemcpyDeviceToHost, c->st));
+            HIPCK(hipMemcpyAsync(pre + b, c->d_a3, size_t(m) * 4, hipMemcpyDeviceToHost, c->st));
+            HIPCK(hipStreamSynchronize(c->st));
+        }
+        return DSX_OK;
     }
     for (uint64_t b = 0; b < n; b += c->cap1) {
         const uint32_t m = uint32_t(std::min<uint64_t>(c->cap1, n - b));

@@ -26,10 +26,12 @@ struct GateKernel {
     float k[15];
     float norm;
     float trunc;
-    // R = 4 only: taps 1 .. 7 times 2^14, rounded, as 16-bit pairs (0, k1), (k2, k3), (k4, k5), (k6, k7),
-    // (k1, k2), (k3, k4), (k5, k6), (k7, 0); wq bounds the normalised change that the integer blur passes
-    // (these weights, the intermediate sums rounded to 16 bits) make per unit of |grid value|
-    uint32_t kq2[8];
+    // The integer blur passes of the 16-bit fast amplitudes: the taps w they use (R = 4: taps 1 .. 7, the
+    // truncated blur; R = 7: all 15) times SK (2^14 and 2^13), rounded, as 16-bit pairs: with NP = 4 or 8,
+    // kq2[m] = (w[2m - 1], w[2m]) and kq2[NP + m] = (w[2m], w[2m + 1]), w[-1] = w[W] = 0. wq bounds the
+    // normalised change that the integer passes (these weights, the horizontal sums rounded to 16 bits)
+    // make per unit of |grid value|.
+    uint32_t kq2[16];
     float wq;
     double wd;   // octave weight the corner tables are scaled by
 };
@@ -41,7 +43,7 @@ struct GateKernel {
 // With fast set, the pre-gate first decides with the fast amplitude (FMA blurs, provably the same decisions)
 // and runs the reference's arithmetic only where a decision is within its error bound (gate_launches in
 // gate.hip); every amplitude output is the reference's.
-constexpr int GATE_NCNT = 8;   // counters: phase, pre, full, taken, pre undecided
+constexpr int GATE_NCNT = 8;   // counters: phase, pre, full, taken, pre undecided, full undecided
 struct GateChain {
     uint32_t *cnt;        // GATE_NCNT counters
     uint32_t *k1, *i2, *k3;
@@ -49,6 +51,7 @@ struct GateChain {
     SideParams *sides;    // 2 * grid
     uint32_t grid;        // phase survivors the chain holds
     bool fast = false;
+    uint32_t *u2 = nullptr;   // (GateStages::full_fast) the full gate's undecided pairs, grid entries
 };
 struct GateOut {          // device or pinned host memory the sort kernel writes
     uint64_t *seeds;      // may be null
@@ -60,6 +63,8 @@ struct GateOut {          // device or pinned host memory the sort kernel writes
 struct GateStages {       // thresholds and kernels of one gate configuration
     double maxph, thr_pre, amp;
     int mode_pre;         // 0 anti-aliased pre-gate, 2 legacy even-parity pre-gate
+    int full_fast;        // (fast chains) the full gate decided from its fast amplitude: survivors only, no
+                          // amplitudes for the seeds it decides
     GateKernel kpre, kfull;
 };
 // Enqueues the chain on st for seed indices [start, start + count) (count <= 2^31).
@@ -87,8 +92,9 @@ void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64
                  const uint32_t *n_dev = nullptr);
 // Amplitude kernel k_amp<mode, dec> (gate.hip): dec = GATE_EXACT, or GATE_SPLIT (modes 0 and 2), whose
 // undecided list is und_idx / und_count, or GATE_SPLIT_T (mode 0): GATE_SPLIT with the blur's outermost taps
-// left out of the fast amplitude, its decision margin widened by GateKernel::trunc, or GATE_SPLIT_Q (mode 0):
-// GATE_SPLIT_T with the grid summed in 16-bit fixed point, the margin widened by 2.9e-4 per unit of grid bound.
+// left out of the fast amplitude, its decision margin widened by GateKernel::trunc, or GATE_SPLIT_Q (modes 0
+// and 1): the grid summed in 16-bit fixed point and blurred in 16-bit integer dot products (mode 0 also
+// truncated as GATE_SPLIT_T), the margin widened by the bounds of both.
 enum { GATE_EXACT = 0, GATE_SPLIT = 1, GATE_SPLIT_T = 2, GATE_SPLIT_Q = 3 };
 void launch_amp(hipStream_t st, int mode, int dec, const SideParams *sides, const uint32_t *sel, uint32_t n,
                 const GateKernel &K, double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count,

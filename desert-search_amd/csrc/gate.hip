@@ -117,23 +117,27 @@ __device__ float side_bound(const WaveLds &L) {
 
 // FAST: fused multiply-add blurs; TRUNC (with FAST): the blur's outermost taps left out (radius R - 1, taps
 // 1 .. 2R - 1 of K.k), which moves the normalised result by at most K.trunc per unit of |grid value|.
-// Q16 (with FAST, MODE 0): the grid in 16-bit fixed point (grid_setup_q16), each value within 2.83e-4 * b of
-// the float one, b = the side's grid bound, which it stores in *bound.
+// Q16 (with FAST; MODE 0 truncated to radius 3, or MODE 1 at radius 7): the grid in 16-bit fixed point
+// (grid_setup_q16 into tq: MODE 0 packs it into L.tab, MODE 1 needs 768 words of its own) and both blur
+// passes in integer dot products; b = the side's grid bound, which it stores in *bound.
 template <int MODE, bool FAST, bool TRUNC = false, bool Q16 = false>
-__device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K, float *bound = nullptr) {
+__device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K, float *bound = nullptr,
+                          uint32_t *tq = nullptr) {
     static_assert(FAST || !TRUNC, "only the fast amplitude may leave taps out");
-    static_assert(!Q16 || (FAST && MODE == 0), "the fixed-point grid is a fast-amplitude form of MODE 0");
+    static_assert(!Q16 || (FAST && ((MODE == 0 && TRUNC) || (MODE == 1 && !TRUNC))),
+                  "the 16-bit forms: the truncated 256^2 grid, the full 512^2 grid");
     grid_setup<MODE>(sp, L, K.wd);
     float inv_s = 1.f;
     if constexpr (Q16) {
         const float b = side_bound<MODE>(L);
         *bound = b;
-        inv_s = 1.f / grid_setup_q16(L, b);
+        if (MODE == 0) tq = reinterpret_cast<uint32_t *>(L.tab);
+        inv_s = 1.f / grid_setup_q16<MODE>(L, b, tq);
     }
     float best = -1e9f;
     int ibest = INT_MIN;
     constexpr int NC = Shape<MODE>::G / 64, R = GateR<MODE>::R - (TRUNC ? 1 : 0);
-    constexpr bool INTB = Q16 && R == 3;   // both blur passes in integers (grid_blur_rows, QH)
+    constexpr bool INTB = Q16;   // both blur passes in integers (grid_blur_rows, QH)
     // only the maximum is kept, so sums may start from their first term (ZSIGN off)
     grid_blur_rows<MODE, R, false, FAST, Q16>(sp, L, K.k + (TRUNC ? 1 : 0), 0, Shape<MODE>::G - 1,
                                               [&](int, const auto &o) {
@@ -142,15 +146,14 @@ __device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K,
             if constexpr (INTB) ibest = max(ibest, o[q]);
             else best = fmaxf(best, o[q]);
         }
-    }, K.kq2);
+    }, K.kq2, tq);
     if constexpr (INTB) best = float(ibest);   // one rounding (inside gate_eps)
     // wave maximum; x -> fl(x * norm) is monotone, so max(s) * norm == max(s * norm)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
-    // fixed point: the maximum in units of 1 / S, or 2^-12 / S after the integer blur passes (2^14 per
-    // pass's weights, 2^-16 for the rounding between them; two more roundings, a few ulps of |amp|: inside
-    // gate_eps)
-    return Q16 ? best * (K.norm * (INTB ? inv_s * 0x1p-12f : inv_s)) : best * K.norm;
+    // fixed point: the maximum in units of 2^16 / (SK^2 S) after the integer blur passes (SK = 2^14 at
+    // radius 3, 2^13 at radius 7; two more roundings, a few ulps of |amp|: inside gate_eps)
+    return Q16 ? best * (K.norm * (inv_s * (R == 3 ? 0x1p-12f : 0x1p-10f))) : best * K.norm;
 }
 
 // Decision margin of the fast amplitude (FMA blurs) for sides bounded by bA, bB. Each blur pass of W <= 15
@@ -171,9 +174,11 @@ __device__ __forceinline__ double gate_eps(float bA, float bB) { return 2e-5 * (
 //               (optional) gets every fast amplitude and out_amp (optional, with all_amp) its margin eps
 //   GATE_SPLIT_T the same with the blur's outermost taps left out (radius R - 1 in both passes: a quarter
 //               fewer blur operations at R = 4), eps widened by K.trunc * (bA + bB)
-//   GATE_SPLIT_Q GATE_SPLIT_T with the grid summed in 16-bit fixed point, two cells per add, and both blur
-//               passes in 16-bit integer dot products (MODE 0), eps widened by a further
-//               (2.9e-4 + K.wq) * (bA + bB)
+//   GATE_SPLIT_Q the grid summed in 16-bit fixed point, two cells per add, and both blur passes in 16-bit
+//               integer dot products: MODE 0 with GATE_SPLIT_T's truncation, eps widened by a further
+//               (2.9e-4 + K.wq) * (bA + bB); MODE 1 at the full radius, eps widened by (1.3e-4 + K.wq) *
+//               (bA + bB). Certain passes go out as GATE_EXACT's do (kmap ? kmap[pair] : pair), undecided
+//               pairs to und_idx
 // The fast decisions equal the exact ones: the two amplitudes differ by less than eps (gate_eps, plus the
 // truncation bound: dropping taps changes the exact blurred value by at most norm * (ksum^2 - ksum_t^2) times
 // the largest |grid value|, itself at most the side's bound b).
@@ -184,14 +189,17 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
                                              uint32_t *und_idx, uint32_t *und_count) {
     __shared__ WaveLds L[2];
     __shared__ float red[2], bnd[2];
+    constexpr bool Q1 = MODE == 1 && DEC == GATE_SPLIT_Q;   // the 16-bit tables of the 512^2 grid
+    __shared__ uint32_t TQ[Q1 ? 2 : 1][Q1 ? 768 : 1];
     const uint32_t item = blockIdx.x;
     if (n_dev) n = min(n, *n_dev);
     if (item >= n) return;
     const uint32_t pidx = sel ? sel[item] : item;
     const int wv = threadIdx.x >> 6;
     float b = 0.f;
-    const float m = side_max<MODE, DEC != GATE_EXACT, DEC == GATE_SPLIT_T || DEC == GATE_SPLIT_Q, DEC == GATE_SPLIT_Q>(
-        sides[2 * size_t(pidx) + wv], L[wv], K, &b);
+    constexpr bool TR = DEC == GATE_SPLIT_T || (DEC == GATE_SPLIT_Q && MODE == 0);   // truncated blur
+    const float m = side_max<MODE, DEC != GATE_EXACT, TR, DEC == GATE_SPLIT_Q>(sides[2 * size_t(pidx) + wv], L[wv], K, &b,
+                                                                               Q1 ? TQ[Q1 ? wv : 0] : nullptr);
     if (DEC == GATE_SPLIT || DEC == GATE_SPLIT_T) b = side_bound<MODE>(L[wv]);
     if ((threadIdx.x & 63) == 0) {
         red[wv] = m;
@@ -216,15 +224,15 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
             // fixed-point grid by at most 2.83e-4 times it, and the integer blur passes (their rounded weights,
             // the horizontal sums rounded to 16 bits) by at most K.wq times it
             const double bs = double(bnd[0]) + double(bnd[1]);
-            const double eps = gate_eps(bnd[0], bnd[1]) + (DEC != GATE_SPLIT ? double(K.trunc) * bs : 0.0) +
-                               (DEC == GATE_SPLIT_Q ? (2.9e-4 + double(K.wq)) * bs : 0.0);
+            const double eps = gate_eps(bnd[0], bnd[1]) + (TR ? double(K.trunc) * bs : 0.0) +
+                               (DEC == GATE_SPLIT_Q ? ((MODE == 0 ? 2.9e-4 : 1.3e-4) + double(K.wq)) * bs : 0.0);
             if (all_amp) {   // (tests) the fast amplitude and its margin, by block
                 all_amp[item] = amp;
                 if (out_amp) out_amp[item] = float(eps);
             }
             if (double(amp) - eps >= thr) {
                 const uint32_t i = atomicAdd(count, 1u);
-                if (i < cap) out_idx[i] = pidx;
+                if (i < cap) out_idx[i] = kmap ? kmap[pidx] : pidx;
             } else if (double(amp) + eps >= thr) {
                 const uint32_t i = atomicAdd(und_count, 1u);
                 if (i < cap) und_idx[i] = pidx;
@@ -291,13 +299,21 @@ std::vector<GateLaunch> gate_launches(const GateChain &C, const GateStages &S) {
     } else {
         v.push_back({mp, GATE_EXACT, true, nullptr, nullptr, C.i2, nullptr, C.cnt + 1, C.cnt + 3, nullptr, nullptr});
     }
-    v.push_back({1, GATE_EXACT, false, C.i2, C.k1, C.k3, C.a3, C.cnt + 2, C.cnt + 1, nullptr, nullptr});
+    if (C.fast && S.full_fast && C.u2) {
+        // survivors only: the full gate decided from its 16-bit fast amplitude, the reference's arithmetic
+        // for the undecided pairs (the amplitudes of the seeds decided fast are not computed)
+        v.push_back({1, GATE_SPLIT_Q, false, C.i2, C.k1, C.k3, nullptr, C.cnt + 2, C.cnt + 1, C.u2, C.cnt + 5});
+        v.push_back({1, GATE_EXACT, false, C.u2, C.k1, C.k3, C.a3, C.cnt + 2, C.cnt + 5, nullptr, nullptr});
+    } else {
+        v.push_back({1, GATE_EXACT, false, C.i2, C.k1, C.k3, C.a3, C.cnt + 2, C.cnt + 1, nullptr, nullptr});
+    }
     return v;
 }
 
 const void *amp_kernel(int mode, int dec) {
     if (dec == GATE_SPLIT_T && mode == 0) return reinterpret_cast<const void *>(&k_amp<0, GATE_SPLIT_T>);
     if (dec == GATE_SPLIT_Q && mode == 0) return reinterpret_cast<const void *>(&k_amp<0, GATE_SPLIT_Q>);
+    if (dec == GATE_SPLIT_Q && mode == 1) return reinterpret_cast<const void *>(&k_amp<1, GATE_SPLIT_Q>);
     if (dec != GATE_EXACT)
         return mode == 0 ? reinterpret_cast<const void *>(&k_amp<0, GATE_SPLIT>)
                          : reinterpret_cast<const void *>(&k_amp<2, GATE_SPLIT>);

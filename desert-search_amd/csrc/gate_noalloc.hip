@@ -85,6 +85,7 @@ int gate_noalloc(int device, hipStream_t st, uint64_t start, uint64_t count, con
     S.thr_pre = p->box ? double(p->pthr) : double(p->amp - p->margin);
     S.amp = double(p->amp);
     S.mode_pre = p->box ? 0 : 2;
+    S.full_fast = 0;
     S.kpre = dsx::gate_kernel(4, p->box ? p->sig : 1.0f);
     S.kfull = dsx::gate_kernel(7, 2.0f);
     HIPCK(dsx::enqueue_gate_chain(st, C, S, start, uint32_t(count), dsx::GateOut{seeds, amps, kidx, cap, counts}));

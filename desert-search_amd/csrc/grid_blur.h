@@ -136,30 +136,48 @@ __device__ void grid_setup(const SideParams &sp, WaveLds &L, double wd) {
     __syncthreads();
 }
 
-// MODE 0 grid in 16-bit fixed point, for the fast amplitude only. The 18 float sub-tables T(u, v, dy) set up
-// by grid_setup<0> are scaled by S = 32000 / b (b: the side's grid bound, sum of the sub-tables' largest
-// |entry|) and rounded to integers, then packed for a pair of adjacent cells (c, c + 1): table
-// (v * 2 + dy) * 4 + p holds, for lattice column c + p, the low half round(S T(p, v, dy)) (cell c, p <= 2)
-// and the high half round(S T(p - 1, v, dy)) (cell c + 1, p >= 1), 0 where a cell reads no such term. A
-// cell's 18 terms then add up exactly in 16 bits: every partial sum is at most S b + 18 * 0.502 < 32767 in
+// The grid in 16-bit fixed point, for the fast amplitude only. The float sub-tables set up by grid_setup are
+// scaled by S = 32000 / b (b: the side's grid bound, the sum of the largest |entry| of the sub-tables one
+// cell reads) and rounded to integers, then packed so that one 32-bit read gives the terms of a pair of
+// cells that read the same lattice point (low half: the pair's first cell, high half: its second, 0 where
+// a cell reads no such term), into tq:
+//   MODE 0 (18 sub-tables T(u, v, dy), 384 words in L.tab): adjacent cells (c, c + 1); table
+//          (v * 2 + dy) * 4 + p is lattice column c + p: low round(S T(p, v, dy)) (p <= 2), high
+//          round(S T(p - 1, v, dy)) (p >= 1)
+//   MODE 1 (8 sub-tables T(par, dx + 2 dy + 4 dz) per parity par = px * 2 + pz, 768 words): cells (q, q + 2)
+//          of one parity; table ((par * 2 + dy) * 2 + dz) * 3 + p is lattice column mq + p of the pair's
+//          first cell: low round(S T(par, p, dy, dz)) (p <= 1), high round(S T(par, p - 1, dy, dz)) (p >= 1)
+// A cell's terms then add up exactly in 16 bits: every partial sum is at most S b + 18 * 0.502 < 32767 in
 // magnitude. Each term is within 0.502 of S times its float (the rounding to an integer, plus the float
-// product's error of at most 32000 * 2^-24), so the integer sum is within 9.04 of S times the exact sum,
-// i.e. within 2.83e-4 * b after the division by S. Every wave of the block must call it (block barriers);
-// returns S (1 when b is below 1e-20: then every term rounds to 0, off by less than b).
-__device__ float grid_setup_q16(WaveLds &L, float b) {
+// product's error of at most 32000 * 2^-24), so the integer sum of n terms is within 0.502 n of S times the
+// exact sum: 2.83e-4 * b (MODE 0, 18 terms) or 1.26e-4 * b (MODE 1, 8 terms) after the division by S.
+// Every wave of the block must call it (block barriers); returns S (1 when b is below 1e-20: then every term
+// rounds to 0, off by less than b).
+template <int MODE>
+__device__ float grid_setup_q16(WaveLds &L, float b, uint32_t *tq) {
+    static_assert(MODE == 0 || MODE == 1, "16-bit grids of the 256^2 and 512^2 gates");
     const int lane = threadIdx.x & 63;
     const float S = b > 1e-20f ? 32000.f / b : 1.f;
-    uint32_t w[6];
+    constexpr int NW = MODE == 0 ? 6 : 12;   // words per lane
+    uint32_t w[NW];
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const int e = lane + 64 * k, sub = e >> 4, h = e & 15, vd = sub >> 2, p = sub & 3;
-        const int lo = p <= 2 ? int(rintf(L.tab[(p * 6 + vd) * 16 + h] * S)) : 0;
-        const int hi = p >= 1 ? int(rintf(L.tab[((p - 1) * 6 + vd) * 16 + h] * S)) : 0;
+    for (int k = 0; k < NW; k++) {
+        const int e = lane + 64 * k, sub = e >> 4, h = e & 15;
+        int lo, hi;
+        if (MODE == 0) {
+            const int vd = sub >> 2, p = sub & 3;
+            lo = p <= 2 ? int(rintf(L.tab[(p * 6 + vd) * 16 + h] * S)) : 0;
+            hi = p >= 1 ? int(rintf(L.tab[((p - 1) * 6 + vd) * 16 + h] * S)) : 0;
+        } else {
+            const int p = sub % 3, pdd = sub / 3, dz = pdd & 1, dy = (pdd >> 1) & 1, par = pdd >> 2;
+            lo = p <= 1 ? int(rintf(L.tab[(par * 8 + p + 2 * dy + 4 * dz) * 16 + h] * S)) : 0;
+            hi = p >= 1 ? int(rintf(L.tab[(par * 8 + (p - 1) + 2 * dy + 4 * dz) * 16 + h] * S)) : 0;
+        }
         w[k] = (uint32_t(lo) & 0xffffu) | (uint32_t(hi) << 16);
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 6; k++) reinterpret_cast<uint32_t *>(L.tab)[lane + 64 * k] = w[k];
+    for (int k = 0; k < NW; k++) tq[lane + 64 * k] = w[k];
     __syncthreads();
     return S;
 }
@@ -173,14 +191,16 @@ __device__ float grid_setup_q16(WaveLds &L, float b) {
 // 15 fewer instructions per pass and cell). Not the reference's arithmetic: each pass is then within
 // gamma_W = W u / (1 - W u) (u = 2^-24) of the exact dot product times sum |k_d| |input|, as the
 // unfused pass is, so the two results differ by at most 2 gamma_W of that per pass (see gate.hip).
-// Q16 (MODE 0 only, after grid_setup_q16): the grid values are the 16-bit fixed-point sums, in units of 1/S.
-// At R = 3 (with FMA) the horizontal pass then runs on them in integer arithmetic: 16-bit dot products
-// (v_dot2_i32_i16) of two adjacent cells with the weight pairs kq2 (the W weights times 2^14, rounded;
-// kq2[0..3] = (0, k0), (k1, k2), (k3, k4), (k5, k6) for a pair's first cell, kq2[4..7] = (k0, k1), (k2, k3),
-// (k4, k5), (k6, 0) for its second), so the blurred values come out in units of 2^-14 / S.
+// Q16 (with FMA; MODE 0 at R = 3, MODE 1 at R = 7, after grid_setup_q16 into tq): the grid values are the
+// 16-bit fixed-point sums, in units of 1/S, and both blur passes run on them in integer arithmetic: 16-bit
+// dot products (v_dot2_i32_i16) of two adjacent cells, or of two adjacent rows, with the weight pairs kq2
+// (the W taps w times SK, rounded: SK = 2^14 at R = 3, 2^13 at R = 7; with NP = R + 1 pairs, kq2[m] =
+// (w[2m - 1], w[2m]) for a pair's first cell or row and kq2[NP + m] = (w[2m], w[2m + 1]) for its second,
+// w[-1] = w[W] = 0). The horizontal sums, biased by 2^15, are rounded to their high 16 bits between the
+// passes, so epi receives int values in units of 2^16 / (SK^2 S).
 template <int MODE, int R, bool ZSIGN = true, bool FMA = false, bool Q16 = false, class Epi>
 __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const float *kern, int o_lo, int o_hi, Epi &&epi,
-                               const uint32_t *kq2 = nullptr) {
+                               const uint32_t *kq2 = nullptr, const uint32_t *tq = nullptr) {
     using S = Shape<MODE>;
     constexpr int G = S::G, NC = G / 64, W = 2 * R + 1;
     static_assert(R <= NC, "halo must come from the adjacent lane only");
@@ -213,24 +233,26 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
     float kw[W];
 #pragma unroll
     for (int d = 0; d < W; d++) kw[d] = kern[d];
-    constexpr bool QH = Q16 && FMA && R == 3;
-    static_assert(!QH || NC == 4, "the integer horizontal pass takes two cell pairs per lane");
-    s16x2 kq[8];
+    constexpr bool QH = Q16;
+    static_assert(!Q16 || (FMA && NC == R + 1 && (MODE == 0 || MODE == 1)),
+                  "the integer passes take NC / 2 = (R + 1) / 2 cell pairs per lane");
+    constexpr int NP = R + 1;   // (QH) weight pairs per pass
+    s16x2 kq[QH ? 2 * NP : 1];
     if constexpr (QH) {
 #pragma unroll
-        for (int m = 0; m < 8; m++) kq[m] = __builtin_bit_cast(s16x2, kq2[m]);
+        for (int m = 0; m < 2 * NP; m++) kq[m] = __builtin_bit_cast(s16x2, kq2[m]);
     }
     // The vertical window: with UNR (the fast 256^2 grid at radius 3 or less) row j sits in slot
     // (j - js) % WR of a ring of WR = W + 1 slots and the loop runs WR / 2 row pairs per iteration, so every
     // slot index is a constant and nothing moves; otherwise the W rows shift down one slot per row. (At
     // radius 4 the unrolled loop needs 153 VGPRs: 3 waves per SIMD instead of 4.)
-    constexpr bool UNR = FMA && MODE == 0 && R <= 3;
+    constexpr bool UNR = FMA && ((MODE == 0 && R <= 3) || QH);
     constexpr int WR = UNR ? W + 1 : W, PU = UNR ? WR / 2 : 1;
-    static_assert(!QH || (UNR && PU == 4), "the integer vertical pass keeps 4 row pairs");
-    float ring[WR][NC];
-    // (QH) the vertical window as 16-bit row pairs: slot u % 4 holds rows (j0, j0 + 1) of pair u, each value
+    static_assert(!QH || PU == NP, "the integer vertical pass keeps R + 1 row pairs");
+    float ring[QH ? 1 : WR][NC];   // (float passes)
+    // (QH) the vertical window as 16-bit row pairs: slot u holds rows (j0, j0 + 1) of pair u, each value
     // the horizontal sum rounded to units of 2^16 (its high half after adding 2^15)
-    s16x2 hring[QH ? 4 : 1][NC] = {};
+    s16x2 hring[QH ? NP : 1][NC] = {};
     int hrow[2][NC];
     const int js = (o_lo - R) & ~1;                        // first row (even)
     const int npair = (o_hi + R - js + 2) / 2;
@@ -250,143 +272,177 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
 #pragma unroll
         for (int t = 0; t < 2; t++) {
             const int j = j0 + t;
-            float V[NC];
-            s16x2 P[NC / 2];   // (Q16) the cell pairs' packed sums
-            if (MODE == 0 && Q16) {
-                // cells (2 pq, 2 pq + 1) together: lattice column 2 pq + p of each (v, dy) term
-#pragma unroll
-                for (int pq = 0; pq < NC / 2; pq++) {
-                    s16x2 acc = {0, 0};
-#pragma unroll
-                    for (int v = 0; v < 3; v++)
-#pragma unroll
-                        for (int dy = 0; dy < 2; dy++)
-#pragma unroll
-                            for (int p = 0; p < 4; p++) {
-                                const s16x2 e = tabq_at(tab, (v * 2 + dy) * 4 + p, byte_of(dw[(2 * pq + p) * 2 + dy], t + v));
-                                acc = (v == 0 && dy == 0 && p == 0) ? e : acc + e;
-                            }
-                    P[pq] = acc;
-                    if constexpr (!QH) {
-                        V[2 * pq] = float(int(acc.x));
-                        V[2 * pq + 1] = float(int(acc.y));
-                    }
-                }
-            } else if (MODE == 0) {
-#pragma unroll
-                for (int q = 0; q < NC; q++) {
-                    float acc = 0;
-#pragma unroll
-                    for (int v = 0; v < 3; v++)
-#pragma unroll
-                        for (int dy = 0; dy < 2; dy++)
-#pragma unroll
-                            for (int u = 0; u < 3; u++)
-                                acc = sum_step<ZSIGN>(acc, tab_at(tab, (u * 3 + v) * 2 + dy, byte_of(dw[(q + u) * 2 + dy], t + v)),
-                                                      v == 0 && dy == 0 && u == 0);
-                    V[q] = acc;
-                }
-            } else if (MODE == 1) {
-                const int pz = t;
-                const uint32_t zb = pz ? diz : 0u;
-#pragma unroll
-                for (int q = 0; q < NC; q++) {
-                    const int px = q & 1, mq = q >> 1;
-                    float acc = 0;
-#pragma unroll
-                    for (int c = 0; c < 8; c++) {
-                        const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-                        const uint32_t d = dw[(px * 5 + mq + dx) * 2 + dy];
-                        const uint32_t off = (d >> (8u * (uint32_t(dz) + zb))) & 0xffu;
-                        acc = sum_step<ZSIGN>(acc, tab_at(tab, (px * 2 + pz) * 8 + c, off), c == 0);
-                    }
-                    V[q] = acc;
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < NC; q++) {
-                    float acc = 0;
-#pragma unroll
-                    for (int c = 0; c < 8; c++) {
-                        const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-                        acc = sum_step<ZSIGN>(acc, tab_at(tab, c, byte_of(dw[(q + dx) * 2 + dy], t + dz)), c == 0);
-                    }
-                    V[q] = acc;
-                }
-            }
-            // the halo: DPP rotates for the 256^2 grids (their kernels wait on LDS issue: measured -6 %), LDS
-            // permutes for the 512^2 grid (its 2 x 7 halo values per row as rotates measured +8 %)
-            float Wv[NC + 2 * R];
-            const int left = (lane + 63) & 63, right = (lane + 1) & 63;
             if constexpr (QH) {
-                // pairs X[0..5] = the left lane's two, this lane's two, the right lane's two; cell c's 7 taps
-                // are the 4 pairs from X[(c + 1) / 2] with the weight pairs of its parity
-                const s16x2 X[6] = {from_left(P[0]), from_left(P[1]), P[0], P[1], from_right(P[0]), from_right(P[1])};
+                // the row's cells as 16-bit adjacent pairs A[a] = (cell 2a, cell 2a + 1)
+                s16x2 A[NC / 2];
+                if constexpr (MODE == 0) {
+                    // cells (2 pq, 2 pq + 1) together: lattice column 2 pq + p of each (v, dy) term
+#pragma unroll
+                    for (int pq = 0; pq < NC / 2; pq++) {
+                        s16x2 acc = {0, 0};
+#pragma unroll
+                        for (int v = 0; v < 3; v++)
+#pragma unroll
+                            for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+                                for (int p = 0; p < 4; p++) {
+                                    const s16x2 e = tabq_at(reinterpret_cast<const float *>(tq), (v * 2 + dy) * 4 + p,
+                                                            byte_of(dw[(2 * pq + p) * 2 + dy], t + v));
+                                    acc = (v == 0 && dy == 0 && p == 0) ? e : acc + e;
+                                }
+                        A[pq] = acc;
+                    }
+                } else {
+                    // cells (px + 4k, px + 4k + 2) of parity (px, t) together: lattice column 2k + p of each
+                    // (dy, dz) term; row t reads hash byte dz + t * diz (the stream shifted once per row)
+                    s16x2 B[2][2];
+#pragma unroll
+                    for (int px = 0; px < 2; px++)
+#pragma unroll
+                        for (int k = 0; k < 2; k++) {
+                            s16x2 acc = {0, 0};
+#pragma unroll
+                            for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+                                for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                                    for (int p = 0; p < 3; p++) {
+                                        const uint32_t d = dw[(px * 5 + 2 * k + p) * 2 + dy];
+                                        const uint32_t ds = t ? (d >> (8u * diz)) : d;
+                                        const s16x2 e = tabq_at(reinterpret_cast<const float *>(tq),
+                                                                (((px * 2 + t) * 2 + dy) * 2 + dz) * 3 + p, byte_of(ds, dz));
+                                        acc = (dy == 0 && dz == 0 && p == 0) ? e : acc + e;
+                                    }
+                            B[px][k] = acc;
+                        }
+                    // (q, q + 2) pairs of both parities into adjacent pairs: (4k, 4k + 1), (4k + 2, 4k + 3)
+#pragma unroll
+                    for (int k = 0; k < 2; k++) {
+                        const uint32_t b0 = __builtin_bit_cast(uint32_t, B[0][k]), b1 = __builtin_bit_cast(uint32_t, B[1][k]);
+                        A[2 * k] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(b1, b0, 0x05040100u));
+                        A[2 * k + 1] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(b1, b0, 0x07060302u));
+                    }
+                }
+                // horizontal pass: pairs X = the left lane's NC / 2, this lane's, the right lane's; cell c's W
+                // taps are the NP pairs from X[(c + 1) / 2] with the weight pairs of its parity
+                s16x2 X[3 * (NC / 2)];
+#pragma unroll
+                for (int a = 0; a < NC / 2; a++) {
+                    X[a] = from_left(A[a]);
+                    X[NC / 2 + a] = A[a];
+                    X[NC + a] = from_right(A[a]);
+                }
 #pragma unroll
                 for (int c = 0; c < NC; c++) {
                     int h = 1 << 15;
 #pragma unroll
-                    for (int m = 0; m < 4; m++) h = __builtin_amdgcn_sdot2(X[(c + 1) / 2 + m], kq[(c & 1) * 4 + m], h, false);
+                    for (int m = 0; m < NP; m++) h = __builtin_amdgcn_sdot2(X[(c + 1) / 2 + m], kq[(c & 1) * NP + m], h, false);
                     hrow[t][c] = h;
                 }
             } else {
+                float V[NC];
+                if (MODE == 0) {
+#pragma unroll
+                    for (int q = 0; q < NC; q++) {
+                        float acc = 0;
+#pragma unroll
+                        for (int v = 0; v < 3; v++)
+#pragma unroll
+                            for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+                                for (int u = 0; u < 3; u++)
+                                    acc = sum_step<ZSIGN>(acc, tab_at(tab, (u * 3 + v) * 2 + dy, byte_of(dw[(q + u) * 2 + dy], t + v)),
+                                                          v == 0 && dy == 0 && u == 0);
+                        V[q] = acc;
+                    }
+                } else if (MODE == 1) {
+                    const int pz = t;
+                    const uint32_t zb = pz ? diz : 0u;
+#pragma unroll
+                    for (int q = 0; q < NC; q++) {
+                        const int px = q & 1, mq = q >> 1;
+                        float acc = 0;
+#pragma unroll
+                        for (int c = 0; c < 8; c++) {
+                            const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+                            const uint32_t d = dw[(px * 5 + mq + dx) * 2 + dy];
+                            const uint32_t off = (d >> (8u * (uint32_t(dz) + zb))) & 0xffu;
+                            acc = sum_step<ZSIGN>(acc, tab_at(tab, (px * 2 + pz) * 8 + c, off), c == 0);
+                        }
+                        V[q] = acc;
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NC; q++) {
+                        float acc = 0;
+#pragma unroll
+                        for (int c = 0; c < 8; c++) {
+                            const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+                            acc = sum_step<ZSIGN>(acc, tab_at(tab, c, byte_of(dw[(q + dx) * 2 + dy], t + dz)), c == 0);
+                        }
+                        V[q] = acc;
+                    }
+                }
+                // the halo: DPP rotates for the 256^2 grids (their kernels wait on LDS issue: measured -6 %), LDS
+                // permutes for the 512^2 grid (its 2 x 7 halo values per row as rotates measured +8 %)
+                float Wv[NC + 2 * R];
+                const int left = (lane + 63) & 63, right = (lane + 1) & 63;
 #pragma unroll
                 for (int q = 0; q < R; q++) Wv[q] = G == 256 ? from_left(V[NC - R + q]) : __shfl(V[NC - R + q], left);
 #pragma unroll
                 for (int q = 0; q < NC; q++) Wv[R + q] = V[q];
 #pragma unroll
                 for (int q = 0; q < R; q++) Wv[R + NC + q] = G == 256 ? from_right(V[q]) : __shfl(V[q], right);
-            }
-            if constexpr (QH) continue;   // both rows of the pair first (below)
-            if constexpr (!UNR) {
+                if constexpr (!UNR) {
 #pragma unroll
-                for (int k = 0; k < W - 1; k++)
+                    for (int k = 0; k < W - 1; k++)
 #pragma unroll
-                    for (int q = 0; q < NC; q++) ring[k][q] = ring[k + 1][q];
-            }
-            // slot of row j - W + 1 + d, the d-th row of the window ending at row j (d = W - 1: row j)
-            const auto slot = [&](int d) { return UNR ? (2 * u + t - (W - 1) + d + 2 * WR) % WR : d; };
-#pragma unroll
-            for (int q = 0; q < NC; q++) {
-                float s = 0;
-#pragma unroll
-                for (int d = 0; d < W; d++)
-                    s = FMA ? (d == 0 ? kw[0] * Wv[q] : __builtin_fmaf(kw[d], Wv[q + d], s))
-                            : sum_step<ZSIGN>(s, kw[d] * Wv[q + d], d == 0);
-                ring[slot(W - 1)][q] = s;
-            }
-            const int o = j - R;
-            if (o >= o_lo && o <= o_hi) {
-                float out[NC];
+                        for (int q = 0; q < NC; q++) ring[k][q] = ring[k + 1][q];
+                }
+                // slot of row j - W + 1 + d, the d-th row of the window ending at row j (d = W - 1: row j)
+                const auto slot = [&](int d) { return UNR ? (2 * u + t - (W - 1) + d + 2 * WR) % WR : d; };
 #pragma unroll
                 for (int q = 0; q < NC; q++) {
                     float s = 0;
 #pragma unroll
                     for (int d = 0; d < W; d++)
-                        s = FMA ? (d == 0 ? kw[0] * ring[slot(0)][q] : __builtin_fmaf(kw[d], ring[slot(d)][q], s))
-                                : sum_step<ZSIGN>(s, kw[d] * ring[slot(d)][q], d == 0);
-                    out[q] = s;
+                        s = FMA ? (d == 0 ? kw[0] * Wv[q] : __builtin_fmaf(kw[d], Wv[q + d], s))
+                                : sum_step<ZSIGN>(s, kw[d] * Wv[q + d], d == 0);
+                    ring[slot(W - 1)][q] = s;
                 }
-                epi(o, out);
+                const int o = j - R;
+                if (o >= o_lo && o <= o_hi) {
+                    float out[NC];
+#pragma unroll
+                    for (int q = 0; q < NC; q++) {
+                        float s = 0;
+#pragma unroll
+                        for (int d = 0; d < W; d++)
+                            s = FMA ? (d == 0 ? kw[0] * ring[slot(0)][q] : __builtin_fmaf(kw[d], ring[slot(d)][q], s))
+                                    : sum_step<ZSIGN>(s, kw[d] * ring[slot(d)][q], d == 0);
+                        out[q] = s;
+                    }
+                    epi(o, out);
+                }
             }
         }
         if constexpr (QH) {
-            // vertical pass over row pairs u - 3 .. u (rows j0 - 6 .. j0 + 1): output j0 - 3 takes the odd
-            // weight pairs, output j0 - 2 the even ones, as the horizontal pass does for a cell pair
+            // vertical pass over row pairs u - R .. u (rows j0 - 2R .. j0 + 1): output j0 - R takes the odd
+            // weight pairs, output j0 - R + 1 the even ones, as the horizontal pass does for a cell pair
 #pragma unroll
             for (int c = 0; c < NC; c++)
                 hring[u][c] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(uint32_t(hrow[1][c]), uint32_t(hrow[0][c]), 0x07060302u));
 #pragma unroll
             for (int e = 0; e < 2; e++) {
-                const int o = j0 - 3 + e;
+                const int o = j0 - R + e;
                 if (o >= o_lo && o <= o_hi) {
                     int out[NC];
 #pragma unroll
                     for (int c = 0; c < NC; c++) {
                         int v = 0;
 #pragma unroll
-                        for (int m = 0; m < 4; m++)
-                            v = __builtin_amdgcn_sdot2(hring[(u + 1 + m) % 4][c], kq[(1 - e) * 4 + m], v, false);
+                        for (int m = 0; m < NP; m++)
+                            v = __builtin_amdgcn_sdot2(hring[(u + 1 + m) % NP][c], kq[(1 - e) * NP + m], v, false);
                         out[c] = v;
                     }
                     epi(o, out);

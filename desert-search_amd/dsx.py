@@ -240,20 +240,22 @@ class Engine:
         except Exception:
             pass
 
-    def gate(self, start, count, params=None, cap=1 << 20):
-        """Seeds passing the GPU gate over indices [start, start+count): (seeds u64, amps f32, k u64, counts)."""
+    def gate(self, start, count, params=None, cap=1 << 20, want_amps=True):
+        """Seeds passing the GPU gate over indices [start, start+count): (seeds u64, amps f32, k u64, counts).
+        want_amps=False passes no amplitude buffer (amps is then None), as the pipeline does: the gate may
+        then decide the full gate from its fast amplitude."""
         p = params or GateParams()
         seeds = np.zeros(cap, np.uint64)
-        amps = np.zeros(cap, np.float32)
+        amps = np.zeros(cap, np.float32) if want_amps else None
         kidx = np.zeros(cap, np.uint64)
         n = ctypes.c_uint64()
         cnt = GateCounts()
         rc = self.lib.dsx_gate(self.h, start, count, ctypes.byref(p), _ptr(seeds, ctypes.c_uint64),
-                               _ptr(amps, ctypes.c_float), _ptr(kidx, ctypes.c_uint64), cap, ctypes.byref(n),
-                               ctypes.byref(cnt))
+                               _ptr(amps, ctypes.c_float) if want_amps else None, _ptr(kidx, ctypes.c_uint64), cap,
+                               ctypes.byref(n), ctypes.byref(cnt))
         self._check(rc, "dsx_gate")
         m = n.value
-        return seeds[:m], amps[:m], kidx[:m], (cnt.phase, cnt.pre, cnt.full)
+        return seeds[:m], (amps[:m] if want_amps else None), kidx[:m], (cnt.phase, cnt.pre, cnt.full)
 
     def set_gate_graph(self, on):
         """dsx_set_gate_graph: 1 replays the gate as a hipGraph per chunk, 0 launches it stage by stage."""

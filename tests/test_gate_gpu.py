@@ -157,10 +157,14 @@ def test_gate_graph_equals_eager(engine):
                 g = engine.gate(start % (1 << 64), count, p)
                 engine.set_gate_graph(0)
                 e = engine.gate(start % (1 << 64), count, p)
-                print(f"gate graph vs eager: start={start} count={count} counts {g[3]} / {e[3]}", flush=True)
-                assert g[3] == e[3]
+                engine.set_gate_graph(1)
+                f = engine.gate(start % (1 << 64), count, p, want_amps=False)
+                print(f"gate graph vs eager: start={start} count={count} counts {g[3]} / {e[3]} / survivors only {f[3]}",
+                      flush=True)
+                assert g[3] == e[3] == f[3]
                 assert np.array_equal(g[0], e[0]) and np.array_equal(g[2], e[2])
                 assert np.array_equal(g[1].view(np.uint32), e[1].view(np.uint32))
+                assert np.array_equal(f[0], e[0]) and np.array_equal(f[2], e[2]) and f[1] is None
                 assert len(g[0]) > 0
                 if p.max_phase > 0.5:
                     assert g[3][0] > 0.7 * count   # beyond the graph grid's 0.625 of a chunk + 4096
@@ -169,11 +173,12 @@ def test_gate_graph_equals_eager(engine):
 
 
 def test_fast_amplitude_within_margin(engine):
-    """Each fast pre-gate amplitude (1: FMA blurs; 2: FMA blurs of radius 3; 3: the same over the grid summed
-    in 16-bit fixed point, two cells per add) lies within its decision margin of the reference amplitude, on
-    2^18 random seeds (any seed has a pre-gate amplitude); the fast kernels only decide seeds farther than
-    that margin from the threshold, so this is what keeps their decisions the reference's. The margins grow
-    with each approximation and the fixed-point amplitudes differ from the float ones."""
+    """Each fast pre-gate amplitude (1: FMA blurs; 2: FMA blurs of radius 3; 3: radius 3 over the grid summed
+    in 16-bit fixed point, blurred in integer dot products), and the full gate's fast amplitude (the 512^2
+    grid in 16 bits, integer passes at radius 7), lies within its decision margin of the reference amplitude,
+    on 2^18 random seeds (any seed has both amplitudes); the fast kernels only decide seeds farther than that
+    margin from the threshold, so this is what keeps their decisions the reference's. The margins grow with
+    each approximation and the fixed-point amplitudes differ from the float ones."""
     import os
     seeds = np.random.default_rng(11).integers(0, 1 << 63, 1 << 18, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
     exact, _ = engine.gate_amps(seeds)
@@ -193,6 +198,19 @@ def test_fast_amplitude_within_margin(engine):
         assert np.all(d <= eps[dec])
     assert np.all(eps[2] >= eps[1]) and np.all(eps[3] > eps[2])
     assert not np.array_equal(fast[2].view(np.uint32), fast[3].view(np.uint32))
+    # the full gate's fast amplitude (16-bit grid, integer passes at radius 7) on the same seeds
+    _, exact_full = engine.gate_amps(seeds)
+    try:
+        os.environ["DSX_GATE_AMPS_FAST_FULL"] = "1"
+        eps_full, fast_full = engine.gate_amps(seeds)
+    finally:
+        os.environ.pop("DSX_GATE_AMPS_FAST_FULL", None)
+    d = np.abs(fast_full.astype(np.float64) - exact_full.astype(np.float64))
+    print(f"fast full amplitude: max |fast - exact| {d.max():.3e}, max ratio to margin "
+          f"{(d / eps_full.astype(np.float64)).max():.3f}, median margin {np.median(eps_full):.3e}", flush=True)
+    assert np.all(np.isfinite(fast_full)) and np.all(eps_full > 0)
+    assert np.all(d <= eps_full)
+    assert not np.array_equal(fast_full.view(np.uint32), exact_full.view(np.uint32))
 
 
 def test_fast_decisions_at_exact_thresholds(engine):
@@ -232,10 +250,13 @@ def test_fast_decisions_at_exact_thresholds(engine):
                 g = engine.gate(start, count, p)
                 engine.set_gate_graph(0)
                 e = engine.gate(start, count, p)
-                print(f"boundary box={box} pthr={p.pthr!r} margin={p.margin!r} amp={p.amp!r}: counts {g[3]} / {e[3]}",
-                      flush=True)
-                assert g[3] == e[3]
+                engine.set_gate_graph(1)
+                f = engine.gate(start, count, p, want_amps=False)   # survivors only: the full gate decided fast
+                print(f"boundary box={box} pthr={p.pthr!r} margin={p.margin!r} amp={p.amp!r}: counts {g[3]} / {e[3]}"
+                      f" / {f[3]}", flush=True)
+                assert g[3] == e[3] == f[3]
                 assert np.array_equal(g[0], e[0]) and np.array_equal(g[1].view(np.uint32), e[1].view(np.uint32))
+                assert np.array_equal(f[0], e[0]) and np.array_equal(f[2], e[2])
                 assert (int(s) in set(g[0].tolist())) == keep
     finally:
         engine.set_gate_graph(1)
