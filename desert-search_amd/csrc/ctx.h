@@ -30,6 +30,7 @@ struct dsx_ctx {
     uint32_t cap1 = 0;
     uint32_t *d_k1 = nullptr, *d_i2 = nullptr, *d_k3 = nullptr, *d_cnt = nullptr;
     float *d_a3 = nullptr, *d_all = nullptr;
+    float *d_band = nullptr;   // gate graph: band maxima of undecided pairs (cap1 * 16), allocated with the graph
     dsx::SideParams *d_sides = nullptr;
     uint64_t *d_seeds = nullptr;
     uint32_t *h_cnt = nullptr, *h_k3 = nullptr;

@@ -236,7 +236,7 @@ int dsx_destroy(dsx_ctx *c) {
     dsx_ctx_free_ext(c);
     for (void *p : {(void *)c->d_k1, (void *)c->d_i2, (void *)c->d_k3, (void *)c->d_a3, (void *)c->d_all, (void *)c->d_cnt,
                     (void *)c->d_sides, (void *)c->d_seeds, (void *)c->d_tsp, (void *)c->d_tabs, (void *)c->d_bandmax,
-                    (void *)c->d_act, (void *)c->d_ncand, (void *)c->d_cand, (void *)c->d_ibT, (void *)c->d_ibC,
+                    (void *)c->d_act, (void *)c->d_ncand, (void *)c->d_cand, (void *)c->d_ibT, (void *)c->d_ibC, (void *)c->d_band,
                     (void *)c->d_loT, (void *)c->d_hiT})
         if (p) (void)hipFree(p);
     for (void *p : {(void *)c->h_cnt, (void *)c->h_k3, (void *)c->h_a3, (void *)c->h_seeds, (void *)c->h_smax,
@@ -378,8 +378,9 @@ int gate_graph_range(dsx_ctx *c, uint64_t start, uint64_t count, const dsx_gate_
     if (!same) {
         // the fast chain: decisions from the fast amplitude where provably equal, the reference's arithmetic
         // for the rest and for every amplitude written
+        if (!c->d_band) HIPCK(hipMalloc(&c->d_band, size_t(c->cap1) * 16 * sizeof(float)));
         const dsx::GateChain C{c->d_cnt, c->d_k1, c->d_i2, c->d_k3, c->d_a3, c->d_sides, grid, true,
-                               reinterpret_cast<uint32_t *>(c->d_all)};
+                               reinterpret_cast<uint32_t *>(c->d_all), c->d_band};
         HIPCK(dsx::gate_graph_build(G, C, S, nmax, dsx::GateOut{nullptr, h_amps, h_kidx, c->cap1, h_counts}));
     }
     for (uint64_t base = 0; base < count; base += nmax) {

@@ -52,6 +52,7 @@ struct GateChain {
     uint32_t grid;        // phase survivors the chain holds
     bool fast = false;
     uint32_t *u2 = nullptr;   // (GateStages::full_fast) the full gate's undecided pairs, grid entries
+    float *band = nullptr;    // (fast chains) band maxima of the undecided pairs, grid * 16 entries
 };
 struct GateOut {          // device or pinned host memory the sort kernel writes
     uint64_t *seeds;      // may be null
@@ -96,6 +97,11 @@ void launch_init(hipStream_t st, uint64_t base, const uint32_t *ks, const uint64
 // and 1): the grid summed in 16-bit fixed point and blurred in 16-bit integer dot products (mode 0 also
 // truncated as GATE_SPLIT_T), the margin widened by the bounds of both.
 enum { GATE_EXACT = 0, GATE_SPLIT = 1, GATE_SPLIT_T = 2, GATE_SPLIT_Q = 3 };
+// In a chain's launch list (GateLaunch::dec) only: the reference's arithmetic over a short list of undecided
+// pairs, one wave per (pair, side, band of 64 output rows) and then a kernel that takes each side's maximum
+// over its bands and decides (the same values and the same decisions as GATE_EXACT, whose one block per
+// pair would keep a CU busy for the whole 256^2 or 512^2 grid); needs GateChain::band.
+enum { GATE_EXACT_BANDS = 4 };
 void launch_amp(hipStream_t st, int mode, int dec, const SideParams *sides, const uint32_t *sel, uint32_t n,
                 const GateKernel &K, double thr, const uint32_t *kmap, uint32_t *out_idx, float *out_amp, uint32_t *count,
                 uint32_t cap, float *all_amp, const uint32_t *n_dev = nullptr, uint32_t *und_idx = nullptr,

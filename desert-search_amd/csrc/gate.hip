@@ -241,6 +241,64 @@ __global__ __launch_bounds__(128) void k_amp(const SideParams *sides, const uint
     }
 }
 
+// The reference's arithmetic for the pairs sel[0 .. min(*n_dev, grid)) in bands: work item w = (pair, side,
+// band), one 64-thread workgroup (one wave) each, grid-stride over the items. A band's output rows take their
+// values from the same rows of the horizontal pass as the whole grid's do, so each value, and a side's
+// maximum over its bands, is the whole-grid kernel's; bandmax[w] gets the band's maximum (unnormalised).
+template <int MODE> struct GateBands { static constexpr int NB = Shape<MODE>::G / 64; };
+template <int MODE>
+__global__ __launch_bounds__(64) void k_amp_band(const SideParams *sides, const uint32_t *sel, GateKernel K,
+                                                 const uint32_t *n_dev, uint32_t grid, float *bandmax) {
+    __shared__ WaveLds L;
+    constexpr int NB = GateBands<MODE>::NB, NC = Shape<MODE>::G / 64, R = GateR<MODE>::R;
+    const uint32_t n = min(*n_dev, grid);
+    for (uint32_t w = blockIdx.x; w < n * 2u * NB; w += gridDim.x) {
+        const uint32_t item = w / (2u * NB), side = (w / NB) & 1u, band = w % NB;
+        const SideParams &sp = sides[2 * size_t(sel[item]) + side];
+        __syncthreads();   // the previous item's reads of L are done
+        grid_setup<MODE>(sp, L, K.wd);
+        float best = -1e9f;
+        grid_blur_rows<MODE, R, false, false>(sp, L, K.k, int(band) * 64, int(band) * 64 + 63, [&](int, const float (&o)[NC]) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) best = fmaxf(best, o[q]);
+        });
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
+        if (threadIdx.x == 0) bandmax[w] = best;
+    }
+}
+
+// Per pair of the banded list: each side's maximum over its bands times norm, the amplitude as k_amp sums it
+// (0 + A + B), and k_amp's GATE_EXACT decision and output.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_amp_band_decide(const uint32_t *sel, const uint32_t *n_dev, uint32_t grid,
+                                                         const float *bandmax, float norm, double thr, const uint32_t *kmap,
+                                                         uint32_t *out_idx, float *out_amp, uint32_t *count, uint32_t cap) {
+    constexpr int NB = GateBands<MODE>::NB;
+    const uint32_t n = min(*n_dev, grid);
+    for (uint32_t item = blockIdx.x * 256u + threadIdx.x; item < n; item += gridDim.x * 256u) {
+        float m[2];
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            float best = -1e9f;
+            for (int b = 0; b < NB; b++) best = fmaxf(best, bandmax[(item * 2u + side) * NB + b]);
+            m[side] = best * norm;
+        }
+        float amp = 0;
+        amp += m[0];
+        amp += m[1];
+        if (double(amp) >= thr) {
+            const uint32_t pidx = sel[item];
+            const uint32_t i = atomicAdd(count, 1u);
+            if (i < cap) {
+                out_idx[i] = kmap ? kmap[pidx] : pidx;
+                if (out_amp) out_amp[i] = amp;
+            }
+        }
+    }
+}
+constexpr uint32_t BAND_GRID = 2048, BAND_DECIDE_GRID = 16;
+
 // ------------------------------------------------------------------ the gate as one chain without host round trips
 __global__ void k_gate_clear(uint32_t *cnt) {
     if (threadIdx.x < GATE_NCNT) cnt[threadIdx.x] = 0;
@@ -294,6 +352,8 @@ std::vector<GateLaunch> gate_launches(const GateChain &C, const GateStages &S) {
         }();
         const bool tr = trunc_on && mp == 0 && S.kpre.trunc > 0.f && S.kpre.trunc <= 2e-3f;
         const int dec = tr ? (q16_on ? GATE_SPLIT_Q : GATE_SPLIT_T) : GATE_SPLIT;
+        // (the pre-gate's undecided pairs, some 2 % of the phase survivors, keep a CU each busy for a 256^2
+        // grid just as long as in bands, which measured 0.44 against 0.42 ms per 2^24 indices)
         v.push_back({mp, dec, true, nullptr, nullptr, C.i2, nullptr, C.cnt + 1, C.cnt + 3, C.k3, C.cnt + 4});
         v.push_back({mp, GATE_EXACT, true, C.k3, nullptr, C.i2, nullptr, C.cnt + 1, C.cnt + 4, nullptr, nullptr});
     } else {
@@ -301,9 +361,11 @@ std::vector<GateLaunch> gate_launches(const GateChain &C, const GateStages &S) {
     }
     if (C.fast && S.full_fast && C.u2) {
         // survivors only: the full gate decided from its 16-bit fast amplitude, the reference's arithmetic
-        // for the undecided pairs (the amplitudes of the seeds decided fast are not computed)
+        // for the undecided pairs (the amplitudes of the seeds decided fast are not computed), in bands: about
+        // 1 % of the pre-gate survivors, whose 512^2 grids would each keep one CU busy for 0.6 ms
         v.push_back({1, GATE_SPLIT_Q, false, C.i2, C.k1, C.k3, nullptr, C.cnt + 2, C.cnt + 1, C.u2, C.cnt + 5});
-        v.push_back({1, GATE_EXACT, false, C.u2, C.k1, C.k3, C.a3, C.cnt + 2, C.cnt + 5, nullptr, nullptr});
+        v.push_back({1, C.band ? GATE_EXACT_BANDS : GATE_EXACT, false, C.u2, C.k1, C.k3, C.a3, C.cnt + 2, C.cnt + 5,
+                     nullptr, nullptr});
     } else {
         v.push_back({1, GATE_EXACT, false, C.i2, C.k1, C.k3, C.a3, C.cnt + 2, C.cnt + 1, nullptr, nullptr});
     }
@@ -329,9 +391,24 @@ hipError_t enqueue_gate_chain(hipStream_t st, const GateChain &C, const GateStag
         launch_phase(st, start, count, S.maxph, C.k1, C.cnt, C.grid);
         hipLaunchKernelGGL(k_gate_clamp, dim3(1), dim3(64), 0, st, C.cnt, C.grid);
         launch_init(st, start, C.k1, nullptr, C.grid, C.sides, C.cnt + 3);
-        for (const GateLaunch &g : gate_launches(C, S))
-            launch_amp(st, g.mode, g.dec, C.sides, g.sel, C.grid, g.pre ? S.kpre : S.kfull, g.pre ? S.thr_pre : S.amp,
-                       g.kmap, g.out, g.amp, g.count, C.grid, nullptr, g.ndev, g.und, g.und_count);
+        for (const GateLaunch &g : gate_launches(C, S)) {
+            const GateKernel &K = g.pre ? S.kpre : S.kfull;
+            const double thr = g.pre ? S.thr_pre : S.amp;
+            if (g.dec == GATE_EXACT_BANDS) {
+                if (g.mode == 0) {
+                    hipLaunchKernelGGL(k_amp_band<0>, dim3(BAND_GRID), dim3(64), 0, st, C.sides, g.sel, K, g.ndev, C.grid, C.band);
+                    hipLaunchKernelGGL(k_amp_band_decide<0>, dim3(BAND_DECIDE_GRID), dim3(256), 0, st, g.sel, g.ndev, C.grid,
+                                       C.band, K.norm, thr, g.kmap, g.out, g.amp, g.count, C.grid);
+                } else {
+                    hipLaunchKernelGGL(k_amp_band<1>, dim3(BAND_GRID), dim3(64), 0, st, C.sides, g.sel, K, g.ndev, C.grid, C.band);
+                    hipLaunchKernelGGL(k_amp_band_decide<1>, dim3(BAND_DECIDE_GRID), dim3(256), 0, st, g.sel, g.ndev, C.grid,
+                                       C.band, K.norm, thr, g.kmap, g.out, g.amp, g.count, C.grid);
+                }
+                continue;
+            }
+            launch_amp(st, g.mode, g.dec, C.sides, g.sel, C.grid, K, thr, g.kmap, g.out, g.amp, g.count, C.grid, nullptr,
+                       g.ndev, g.und, g.und_count);
+        }
     }
     hipLaunchKernelGGL(k_gate_sort, dim3((C.grid + 255) / 256), dim3(256), 0, st, C.cnt, C.k3, C.a3, start, C.grid, O);
     return hipGetLastError();
@@ -411,6 +488,20 @@ hipError_t gate_graph_build(GateGraph &G, const GateChain &C, const GateStages &
         double thr = g.pre ? S.thr_pre : S.amp;
         uint32_t *oidx = g.out, *count = g.count, *und = g.und, *und_count = g.und_count;
         float *oamp = g.amp, *all = nullptr;
+        if (g.dec == GATE_EXACT_BANDS) {
+            float *band = G.C.band;
+            const float *cband = G.C.band;
+            float norm = K.norm;
+            void *a1[] = {&sides, &sel, &K, &ndev, &n, &band};
+            void *a2[] = {&sel, &ndev, &n, &cband, &norm, &thr, &kmap, &oidx, &oamp, &count, &cap};
+            const void *f1 = g.mode == 0 ? reinterpret_cast<const void *>(&k_amp_band<0>)
+                                         : reinterpret_cast<const void *>(&k_amp_band<1>);
+            const void *f2 = g.mode == 0 ? reinterpret_cast<const void *>(&k_amp_band_decide<0>)
+                                         : reinterpret_cast<const void *>(&k_amp_band_decide<1>);
+            if ((e = add(kparams(f1, dim3(BAND_GRID), dim3(64), a1), nullptr))) return e;
+            if ((e = add(kparams(f2, dim3(BAND_DECIDE_GRID), dim3(256), a2), nullptr))) return e;
+            continue;
+        }
         void *a[] = {&sides, &sel, &n, &K, &thr, &kmap, &oidx, &oamp, &count, &cap, &all, &ndev, &und, &und_count};
         if ((e = add(kparams(amp_kernel(g.mode, g.dec), dim3(G.C.grid), dim3(128), a), nullptr))) return e;
     }
