@@ -128,11 +128,12 @@ __device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K,
                   "the 16-bit forms: the truncated 256^2 grid, the full 512^2 grid");
     grid_setup<MODE>(sp, L, K.wd);
     float inv_s = 1.f;
+    uint32_t qbias[4] = {0, 0, 0, 0};
     if constexpr (Q16) {
         const float b = side_bound<MODE>(L);
         *bound = b;
         if (MODE == 0) tq = reinterpret_cast<uint32_t *>(L.tab);
-        inv_s = 1.f / grid_setup_q16<MODE>(L, b, tq);
+        inv_s = 1.f / grid_setup_q16<MODE>(L, b, tq, qbias);
     }
     float best = -1e9f;
     int ibest = INT_MIN;
@@ -146,7 +147,7 @@ __device__ float side_max(const SideParams &sp, WaveLds &L, const GateKernel &K,
             if constexpr (INTB) ibest = max(ibest, o[q]);
             else best = fmaxf(best, o[q]);
         }
-    }, K.kq2, tq);
+    }, K.kq2, tq, qbias);
     if constexpr (INTB) best = float(ibest);   // one rounding (inside gate_eps)
     // wave maximum; x -> fl(x * norm) is monotone, so max(s) * norm == max(s * norm)
 #pragma unroll

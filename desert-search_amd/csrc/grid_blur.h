@@ -54,8 +54,8 @@ __device__ __forceinline__ float tab_at(const float *tab, int sub, uint32_t off)
 __device__ __forceinline__ uint32_t byte_of(uint32_t dw, int b) { return (dw >> (8 * b)) & 0xffu; }
 // Two 16-bit integers in one register: sums of them are packed 16-bit adds (two cells per instruction)
 typedef short s16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ s16x2 tabq_at(const float *tab, int sub, uint32_t off) {
-    return *reinterpret_cast<const s16x2 *>(reinterpret_cast<const char *>(tab) + sub * 64 + off);
+__device__ __forceinline__ uint32_t tabq_u32(const uint32_t *tq, int sub, uint32_t off) {
+    return *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(tq) + sub * 64 + off);
 }
 // The value of the lane to the left (lane - 1, lane 0 takes lane 63's) and to the right (lane + 1, lane 63
 // takes lane 0's): DPP wavefront rotates (wave_ror:1, wave_rol:1), which use no LDS bandwidth
@@ -147,33 +147,54 @@ __device__ void grid_setup(const SideParams &sp, WaveLds &L, double wd) {
 //   MODE 1 (8 sub-tables T(par, dx + 2 dy + 4 dz) per parity par = px * 2 + pz, 768 words): cells (q, q + 2)
 //          of one parity; table ((par * 2 + dy) * 2 + dz) * 3 + p is lattice column mq + p of the pair's
 //          first cell: low round(S T(par, p, dy, dz)) (p <= 1), high round(S T(par, p - 1, dy, dz)) (p >= 1)
-// A cell's terms then add up exactly in 16 bits: every partial sum is at most S b + 18 * 0.502 < 32767 in
-// magnitude. Each term is within 0.502 of S times its float (the rounding to an integer, plus the float
-// product's error of at most 32000 * 2^-24), so the integer sum of n terms is within 0.502 n of S times the
-// exact sum: 2.83e-4 * b (MODE 0, 18 terms) or 1.26e-4 * b (MODE 1, 8 terms) after the division by S.
-// Every wave of the block must call it (block barriers); returns S (1 when b is below 1e-20: then every term
-// rounds to 0, off by less than b).
+// Each entry is biased by its sub-table's largest |rounded entry| OFF, so that both halves hold values in
+// [0, 2 OFF]: a cell's terms then add up exactly as 32-bit integers, both halves at once (three per
+// v_add3_u32), with no carry from the low half into the high one: a half's sum lies in [0, 2 B], B = the sum
+// of its cell's OFFs <= S b + 18 * 0.502, and 2 B < 65536. Subtracting B from each half on its own (a packed
+// 16-bit subtraction; qbias: B in both halves, of MODE 0 or per parity of MODE 1) leaves the signed sum,
+// |sum| <= B < 32767. Each term is within 0.502 of S times
+// its float (the rounding to an integer, plus the float product's error of at most 32000 * 2^-24), so the
+// integer sum of n terms is within 0.502 n of S times the exact sum: 2.83e-4 * b (MODE 0, 18 terms) or
+// 1.26e-4 * b (MODE 1, 8 terms) after the division by S. Every wave of the block must call it (block
+// barriers); returns S (1 when b is below 1e-20: then every term rounds to 0, off by less than b).
 template <int MODE>
-__device__ float grid_setup_q16(WaveLds &L, float b, uint32_t *tq) {
+__device__ float grid_setup_q16(WaveLds &L, float b, uint32_t *tq, uint32_t (&qbias)[4]) {
     static_assert(MODE == 0 || MODE == 1, "16-bit grids of the 256^2 and 512^2 gates");
     const int lane = threadIdx.x & 63;
     const float S = b > 1e-20f ? 32000.f / b : 1.f;
-    constexpr int NW = MODE == 0 ? 6 : 12;   // words per lane
+    constexpr int NSUB = MODE == 0 ? 18 : 32, NW = MODE == 0 ? 6 : 12;   // float sub-tables; words per lane
+    const auto q = [&](int sub, int h) { return int(rintf(L.tab[sub * 16 + h] * S)); };
+    // lane i < NSUB: sub-table i's offset
+    int off = 0;
+    if (lane < NSUB)
+        for (int h = 0; h < 16; h++) off = max(off, abs(q(lane, h)));
+    // B: the sum over all 18 sub-tables (MODE 0) or over the 8 of each parity (MODE 1, lanes par * 8 ..)
+    int bs = off;
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) bs += __shfl_xor(bs, o);
+    if (MODE == 0) {
+#pragma unroll
+        for (int o = 8; o < 64; o <<= 1) bs += __shfl_xor(bs, o);
+    }
+#pragma unroll
+    for (int par = 0; par < 4; par++) qbias[par] = uint32_t(__shfl(bs, MODE == 0 ? 0 : par * 8)) * 0x10001u;
     uint32_t w[NW];
 #pragma unroll
     for (int k = 0; k < NW; k++) {
         const int e = lane + 64 * k, sub = e >> 4, h = e & 15;
-        int lo, hi;
+        int slo = -1, shi = -1;   // float sub-tables of the low and high half (-1: no term)
         if (MODE == 0) {
             const int vd = sub >> 2, p = sub & 3;
-            lo = p <= 2 ? int(rintf(L.tab[(p * 6 + vd) * 16 + h] * S)) : 0;
-            hi = p >= 1 ? int(rintf(L.tab[((p - 1) * 6 + vd) * 16 + h] * S)) : 0;
+            if (p <= 2) slo = p * 6 + vd;
+            if (p >= 1) shi = (p - 1) * 6 + vd;
         } else {
             const int p = sub % 3, pdd = sub / 3, dz = pdd & 1, dy = (pdd >> 1) & 1, par = pdd >> 2;
-            lo = p <= 1 ? int(rintf(L.tab[(par * 8 + p + 2 * dy + 4 * dz) * 16 + h] * S)) : 0;
-            hi = p >= 1 ? int(rintf(L.tab[(par * 8 + (p - 1) + 2 * dy + 4 * dz) * 16 + h] * S)) : 0;
+            if (p <= 1) slo = par * 8 + p + 2 * dy + 4 * dz;
+            if (p >= 1) shi = par * 8 + (p - 1) + 2 * dy + 4 * dz;
         }
-        w[k] = (uint32_t(lo) & 0xffffu) | (uint32_t(hi) << 16);
+        const int olo = __shfl(off, slo < 0 ? 0 : slo), ohi = __shfl(off, shi < 0 ? 0 : shi);
+        const int lo = slo >= 0 ? q(slo, h) + olo : 0, hi = shi >= 0 ? q(shi, h) + ohi : 0;
+        w[k] = uint32_t(lo) | (uint32_t(hi) << 16);
     }
     __syncthreads();
 #pragma unroll
@@ -200,7 +221,8 @@ __device__ float grid_setup_q16(WaveLds &L, float b, uint32_t *tq) {
 // passes, so epi receives int values in units of 2^16 / (SK^2 S).
 template <int MODE, int R, bool ZSIGN = true, bool FMA = false, bool Q16 = false, class Epi>
 __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const float *kern, int o_lo, int o_hi, Epi &&epi,
-                               const uint32_t *kq2 = nullptr, const uint32_t *tq = nullptr) {
+                               const uint32_t *kq2 = nullptr, const uint32_t *tq = nullptr,
+                               const uint32_t *qbias = nullptr) {
     using S = Shape<MODE>;
     constexpr int G = S::G, NC = G / 64, W = 2 * R + 1;
     static_assert(R <= NC, "halo must come from the adjacent lane only");
@@ -279,18 +301,17 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
                     // cells (2 pq, 2 pq + 1) together: lattice column 2 pq + p of each (v, dy) term
 #pragma unroll
                     for (int pq = 0; pq < NC / 2; pq++) {
-                        s16x2 acc = {0, 0};
+                        uint32_t acc = 0;
 #pragma unroll
                         for (int v = 0; v < 3; v++)
 #pragma unroll
                             for (int dy = 0; dy < 2; dy++)
 #pragma unroll
                                 for (int p = 0; p < 4; p++) {
-                                    const s16x2 e = tabq_at(reinterpret_cast<const float *>(tq), (v * 2 + dy) * 4 + p,
-                                                            byte_of(dw[(2 * pq + p) * 2 + dy], t + v));
+                                    const uint32_t e = tabq_u32(tq, (v * 2 + dy) * 4 + p, byte_of(dw[(2 * pq + p) * 2 + dy], t + v));
                                     acc = (v == 0 && dy == 0 && p == 0) ? e : acc + e;
                                 }
-                        A[pq] = acc;
+                        A[pq] = __builtin_bit_cast(s16x2, acc) - __builtin_bit_cast(s16x2, qbias[0]);   // per half
                     }
                 } else {
                     // cells (px + 4k, px + 4k + 2) of parity (px, t) together: lattice column 2k + p of each
@@ -300,7 +321,7 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
                     for (int px = 0; px < 2; px++)
 #pragma unroll
                         for (int k = 0; k < 2; k++) {
-                            s16x2 acc = {0, 0};
+                            uint32_t acc = 0;
 #pragma unroll
                             for (int dy = 0; dy < 2; dy++)
 #pragma unroll
@@ -309,11 +330,10 @@ __device__ void grid_blur_rows(const SideParams &sp, const WaveLds &L, const flo
                                     for (int p = 0; p < 3; p++) {
                                         const uint32_t d = dw[(px * 5 + 2 * k + p) * 2 + dy];
                                         const uint32_t ds = t ? (d >> (8u * diz)) : d;
-                                        const s16x2 e = tabq_at(reinterpret_cast<const float *>(tq),
-                                                                (((px * 2 + t) * 2 + dy) * 2 + dz) * 3 + p, byte_of(ds, dz));
+                                        const uint32_t e = tabq_u32(tq, (((px * 2 + t) * 2 + dy) * 2 + dz) * 3 + p, byte_of(ds, dz));
                                         acc = (dy == 0 && dz == 0 && p == 0) ? e : acc + e;
                                     }
-                            B[px][k] = acc;
+                            B[px][k] = __builtin_bit_cast(s16x2, acc) - __builtin_bit_cast(s16x2, qbias[px * 2 + t]);
                         }
                     // (q, q + 2) pairs of both parities into adjacent pairs: (4k, 4k + 1), (4k + 2, 4k + 3)
 #pragma unroll
