@@ -34,7 +34,9 @@ Also reported:
                 with the gate replayed as one hipGraph per chunk (graph_on_s, the default; 2^24 indices here;
                 its pre-gate decides from the fused-blur amplitude where that is provably the exact
                 decision) and stage by stage in the reference's arithmetic with a survivor-count read-back
-                after each stage (graph_off_s), on the same rows (rows_identical).
+                after each stage (graph_off_s), on the same rows (rows_identical); survivors_only_s: the
+                graph without amplitudes, as the pipeline calls it (its full gate also decides from a fast
+                amplitude where provably exact), and whether its survivors are the same.
   tuning        the autotuner's settings (desert-search_amd/tuned.json, applied by the library) against the
                 library defaults on the same timed steps, each in fresh processes, alternating, median of 2:
                 gated seeds/s of each and whether their output rows are identical.
@@ -423,10 +425,21 @@ def main():
             rows[mode] = (sd.tobytes(), am.tobytes(), tuple(cnt))
         ms, work = g.gate_profile()   # per-kernel times of the last (stage-by-stage) call
         g.set_gate_graph(1)
+        # survivors only, as the pipeline calls it (no amplitudes: the full gate decides from its fast
+        # amplitude too); the same survivors
+        g.gate(0, 1 << 24, want_amps=False)
+        reps = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            so = g.gate(0, 1 << 24, want_amps=False)
+            reps.append(time.perf_counter() - t1)
+        so_s = float(np.median(reps))
         g.close()
         out["gate_only"] = {"seed_indices": 1 << 24, "seconds": gsec[1], "gated_seeds_per_s": cnt[2] / gsec[1],
                             "counts": list(cnt), "graph_on_s": gsec[1], "graph_off_s": gsec[0],
-                            "graph_speedup": gsec[0] / gsec[1], "rows_identical": rows[0] == rows[1]}
+                            "graph_speedup": gsec[0] / gsec[1], "rows_identical": rows[0] == rows[1],
+                            "survivors_only_s": so_s, "survivors_only_gated_seeds_per_s": so[3][2] / so_s,
+                            "survivors_only_identical": so[0].tobytes() == rows[1][0] and tuple(so[3]) == rows[1][2]}
         gach = 7077888 * work[2] / (ms[2] * 1e-3) / 1e12
         out["roofline_gate"] = {"bound": "fp32-valu", "kernel": "k_amp<0> (4-parity pre-gate)", "achieved": gach,
                                 "peak": 157.3, "unit": "TFLOP/s", "frac": gach / 157.3, "avg_launch_ms": ms[2],
