@@ -98,29 +98,66 @@ __global__ __launch_bounds__(64) void k_tab(const SideParams *sp, uint32_t n, in
     if (lane == 0) bandmax[(size_t(s) * 6 + t) * NB + band] = best;
 }
 
-// Side-A cells with TA >= ta, as (seed, cell) pairs; one atomic per wave.
-// Active cells (TA >= ta) of seed s = blockIdx.y into its own segment act[s * seg ..], counted in
-// count[s] (cells past the segment are counted, not stored).
+// Active cells (TA >= ta) of seed s = blockIdx.y into its own segment act[s * seg ..], counted in count[s]
+// (cells past the segment are counted, not stored), in any order. A workgroup takes 2048 cells (8 per
+// thread, coalesced) and reserves their slots with one atomic: one per wave with an active cell kept the
+// kernel far above its 60 MB of reads, the atomics on each seed's counter serialising.
+constexpr uint32_t ACT_PER_THREAD = 8, ACT_PER_WG = 256 * ACT_PER_THREAD;
 __global__ __launch_bounds__(256) void k_active(const float *tabs, uint32_t n, float ta, uint32_t *act, uint32_t *count,
                                                 uint32_t seg) {
+    __shared__ uint32_t s_wtot[4], s_base;
     const uint32_t s = blockIdx.y;
-    const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
-    const bool on = s < n && tabs[size_t(s) * 6 * 512 * 512 + cell] >= ta;
-    const uint64_t mask = __ballot(on);
-    if (mask == 0) return;
-    const int lane = threadIdx.x & 63;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&count[s], uint32_t(__popcll(mask)));
-    base = __shfl(base, 0);
-    if (on) {
-        const uint32_t k = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-        if (k < seg) act[size_t(s) * seg + k] = cell;
+    if (s >= n) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t c0 = blockIdx.x * ACT_PER_WG + threadIdx.x;
+    const float *ta_tab = tabs + size_t(s) * 6 * 512 * 512;
+    uint64_t bal[ACT_PER_THREAD];
+    uint32_t wtot = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < ACT_PER_THREAD; i++) {
+        bal[i] = __ballot(ta_tab[c0 + i * 256u] >= ta);
+        wtot += uint32_t(__popcll(bal[i]));
+    }
+    if (lane == 0) s_wtot[wave] = wtot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = s_wtot[0] + s_wtot[1] + s_wtot[2] + s_wtot[3];
+        s_base = tot ? atomicAdd(&count[s], tot) : 0u;
+    }
+    __syncthreads();
+    if (wtot == 0) return;
+    uint32_t slot = s_base;
+    for (int w = 0; w < wave; w++) slot += s_wtot[w];
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (uint32_t i = 0; i < ACT_PER_THREAD; i++) {
+        if ((bal[i] >> lane) & 1ull) {
+            const uint32_t k = slot + uint32_t(__popcll(bal[i] & lt));
+            if (k < seg) act[size_t(s) * seg + k] = c0 + i * 256u;
+        }
+        slot += uint32_t(__popcll(bal[i]));
     }
 }
 
 __device__ __forceinline__ void scan_cell(const float *tabs, uint32_t s, uint32_t cell, const int16_t *ibT,
                                           const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A,
-                                          ScanCand *cand, uint32_t *ncand, uint32_t cap);
+                                          ScanCand *cand, uint32_t *ncand, uint32_t cap, ScanCand *stage, uint32_t &nst);
+// A wave's candidates are staged in LDS (SCAN_STAGE of them) and appended to seed s's list with one atomic
+// per flush: one atomic per translation step with a hit, on the counter of the one seed an XCD scans at a
+// time, serialised. The list is in any order either way (the host sorts it by scan position).
+constexpr uint32_t SCAN_STAGE = 64;
+__device__ __forceinline__ void scan_flush(uint32_t s, ScanCand *stage, uint32_t &nst, ScanCand *cand, uint32_t *ncand,
+                                           uint32_t cap) {
+    if (nst == 0) return;
+    const uint32_t lane = threadIdx.x;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&ncand[s], nst);
+    base = __shfl(base, 0);
+    __syncthreads();   // (one wave per workgroup) the staged entries are written
+    if (lane < nst && base + lane < cap) cand[size_t(s) * cap + base + lane] = stage[lane];
+    __syncthreads();   // and read before the stage is reused
+    nst = 0;
+}
 // XCD-aware: workgroups are dispatched to the 8 XCDs round-robin (XCD = blockIdx.x % 8), and the
 // workgroups of XCD x scan seeds x, x + 8, ... one after another, striding over each seed's active
 // cells, so each XCD's L2 holds the B-side tables (TB, CB, EB: 3 MB) of one seed at a time instead of
@@ -129,11 +166,15 @@ __global__ __launch_bounds__(64) void k_scan(const float *tabs, const uint32_t *
                                              uint32_t seg, const int16_t *ibT, const int8_t *loT, const int8_t *hiT,
                                              const int16_t *ibC, ScanArgs A, ScanCand *cand, uint32_t *ncand,
                                              uint32_t cap) {
+    __shared__ ScanCand stage[SCAN_STAGE];
+    uint32_t nst = 0;
     constexpr uint32_t NXCD = 8;
     const uint32_t xcd = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, per = gridDim.x / NXCD;
     for (uint32_t s = xcd; s < m; s += NXCD) {
         const uint32_t na = min(count[s], seg);
-        for (uint32_t a = lb; a < na; a += per) scan_cell(tabs, s, act[size_t(s) * seg + a], ibT, loT, hiT, ibC, A, cand, ncand, cap);
+        for (uint32_t a = lb; a < na; a += per)
+            scan_cell(tabs, s, act[size_t(s) * seg + a], ibT, loT, hiT, ibC, A, cand, ncand, cap, stage, nst);
+        scan_flush(s, stage, nst, cand, ncand, cap);
     }
 }
 
@@ -165,7 +206,7 @@ __global__ __launch_bounds__(64) void k_scan_publish(const float *bandmax, const
 
 __device__ __forceinline__ void scan_cell(const float *tabs, uint32_t s, uint32_t cell, const int16_t *ibT,
                                           const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A,
-                                          ScanCand *cand, uint32_t *ncand, uint32_t cap) {
+                                          ScanCand *cand, uint32_t *ncand, uint32_t cap, ScanCand *stage, uint32_t &nst) {
     const int j = int(cell >> 9), i = int(cell & 511), lane = threadIdx.x;
     const float *TA = tabs + size_t(s) * 6 * 512 * 512;
     const float *TB = TA + 512 * 512, *CA = TB + 512 * 512, *CB = CA + 512 * 512, *EA = CB + 512 * 512,
@@ -192,13 +233,12 @@ __device__ __forceinline__ void scan_cell(const float *tabs, uint32_t s, uint32_
         }
         const uint64_t mask = __ballot(hit);
         if (mask == 0) continue;
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&ncand[s], uint32_t(__popcll(mask)));
-        base = __shfl(base, 0);
+        const uint32_t nh = uint32_t(__popcll(mask));
+        if (nst + nh > SCAN_STAGE) scan_flush(s, stage, nst, cand, ncand, cap);
         if (hit) {
-            const uint32_t k = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-            if (k < cap) {
-                ScanCand &o = cand[size_t(s) * cap + k];
+            const uint32_t k = nst + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+            {
+                ScanCand &o = stage[k];
                 o.s = sc;
                 o.t = tv;
                 o.c = cv;
@@ -208,6 +248,7 @@ __device__ __forceinline__ void scan_cell(const float *tabs, uint32_t s, uint32_
                 o.pos = ((uint32_t(j) * 512u + uint32_t(i)) * 64u + uint32_t(tz)) * 64u + uint32_t(lane);
             }
         }
+        nst += nh;
     }
 }
 
@@ -233,7 +274,7 @@ bool launch_tab(hipStream_t st, const SideParams *sp, uint32_t n, int t0, int nt
     }
 }
 void launch_active(hipStream_t st, const float *tabs, uint32_t n, float ta, uint32_t *act, uint32_t *count, uint32_t seg) {
-    if (n) hipLaunchKernelGGL(k_active, dim3(512 * 512 / 256, n), dim3(256), 0, st, tabs, n, ta, act, count, seg);
+    if (n) hipLaunchKernelGGL(k_active, dim3(512 * 512 / ACT_PER_WG, n), dim3(256), 0, st, tabs, n, ta, act, count, seg);
 }
 void launch_scan(hipStream_t st, const float *tabs, const uint32_t *act, const uint32_t *count, uint32_t m, uint32_t seg,
                  const int16_t *ibT, const int8_t *loT, const int8_t *hiT, const int16_t *ibC, const ScanArgs &A,
