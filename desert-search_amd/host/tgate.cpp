@@ -1,6 +1,7 @@
 // GPU seed gate tool on top of libdsx.so, with the command line of the reference's gate tool
 // (src/gpu/tgate.cu:1-7):
 //   tgate START COUNT [phase=0.10] [amp=0.70] [box=1 pthr=0.645 | box=0 margin=0.01] [sig=1.0] [check=0] [calib=N]
+//         [amps=1]
 // Seeds are splitmix64(START + k), k in [0, COUNT). Prints "seed amp" lines (the input of
 // `dsearch_desert ... stdin=1`) for the seeds passing the gate, in seed-index order.
 //   calib=N  "seed pre full phase" for the first N (by seed value) phase-passed seeds of the first 2^26
@@ -9,6 +10,8 @@
 //            arithmetic of tgate.cu:14-39 (cpu_ref.h gate_amp), on stderr (tgate.cu:87-91); the two use
 //            different arithmetic (float corner-term tables on the GPU, double Perlin then float on the
 //            CPU), so they agree to a few 1e-6, not bit for bit
+//   amps=0   (this tool only) print the seeds alone, asking the library for no amplitudes, as the search
+//            pipeline does: the full gate may then decide from its fast amplitude too (same seeds)
 #include "../../include/dsx.h"
 #include "hwq.h"
 #include "../csrc/dsx_rng.h"
@@ -30,12 +33,12 @@ int main(int argc, char **argv) {
     dsx_set_hw_queues();
     if (argc < 3) {
         std::fprintf(stderr,
-                     "tgate START COUNT [phase=0.10] [amp=0.70] [box=1 pthr=0.645 | box=0 margin=0.01] [check=0] [calib=N]\n");
+                     "tgate START COUNT [phase=0.10] [amp=0.70] [box=1 pthr=0.645 | box=0 margin=0.01] [check=0] [calib=N] [amps=1]\n");
         return 2;
     }
     const uint64_t start = std::stoull(argv[1]), count = std::stoull(argv[2]);
     dsx_gate_params g{0.10f, 0.70f, 0.645f, 0.01f, 1.0f, 1};   // GateCfg defaults (gate.cuh:349-355)
-    int check = 0, calib = 0;
+    int check = 0, calib = 0, want_amps = 1;
     for (int i = 3; i < argc; i++) {
         const std::string kv = argv[i];
         const auto e = kv.find('=');
@@ -46,6 +49,7 @@ int main(int argc, char **argv) {
         else if (k == "margin") g.margin = float(v);
         else if (k == "check") check = int(v);
         else if (k == "calib") calib = int(v);
+        else if (k == "amps") want_amps = v != 0 ? 1 : 0;
         else if (k == "box") g.box = int(v);
         else if (k == "sig") g.sig = float(v);
         else if (k == "pthr") g.pthr = float(v);
@@ -95,16 +99,19 @@ int main(int argc, char **argv) {
         const uint64_t m = std::min(B, count - pos);
         uint64_t n = 0;
         dsx_gate_counts c;
-        rc = dsx_gate(ctx, start + pos, m, &g, seeds.data(), amps.data(), nullptr, seeds.size(), &n, &c);
+        float *ap = want_amps ? amps.data() : nullptr;
+        rc = dsx_gate(ctx, start + pos, m, &g, seeds.data(), ap, nullptr, seeds.size(), &n, &c);
         if (rc == DSX_E_CAPACITY) {
             seeds.resize(n);
             amps.resize(n);
-            rc = dsx_gate(ctx, start + pos, m, &g, seeds.data(), amps.data(), nullptr, seeds.size(), &n, &c);
+            ap = want_amps ? amps.data() : nullptr;
+            rc = dsx_gate(ctx, start + pos, m, &g, seeds.data(), ap, nullptr, seeds.size(), &n, &c);
         }
         if (rc) return fail("dsx_gate", rc);
         for (uint64_t i = 0; i < n; i++) {
-            std::printf("%llu %.5f\n", (unsigned long long)seeds[i], amps[i]);
-            if (check && checked < 4) {
+            if (want_amps) std::printf("%llu %.5f\n", (unsigned long long)seeds[i], amps[i]);
+            else std::printf("%llu\n", (unsigned long long)seeds[i]);
+            if (want_amps && check && checked < 4) {
                 std::fprintf(stderr, "  check seed %llu: gpu amp %.5f cpu amp %.5f\n", (unsigned long long)seeds[i],
                              amps[i], dsx::cpu::gate_amp(seeds[i], 512, 7, 2.0f));
                 checked++;

@@ -119,6 +119,10 @@ def test_tgate_tool(engine, oracle):
         assert w[3:5] == ["gpu", "amp"] and w[6:8] == ["cpu", "amp"]
         assert abs(float(w[5]) - float(w[8])) <= 2e-5, l
     assert "%d seed indices, %d phase, %d pre-gate, %d gated" % ((count,) + tuple(cnt)) in r.stderr
+    # amps=0: the seeds alone (the library is asked for no amplitudes), the same seeds
+    r0 = subprocess.run([exe, str(start), str(count), "amps=0"], capture_output=True, text=True, timeout=120)
+    assert r0.returncode == 0, r0.stderr
+    assert r0.stdout.splitlines() == ["%d" % int(s) for s in seeds]
     n = 12
     r = subprocess.run([exe, "5", "1", "calib=%d" % n, "phase=0.10"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
