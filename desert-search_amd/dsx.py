@@ -190,6 +190,7 @@ def load(path=LIB_PATH):
     lib.dsx_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerConfig), ctypes.POINTER(ctypes.c_void_p)]
     lib.dsx_server_destroy.argtypes = [ctypes.c_void_p]
     lib.dsx_server_workers.argtypes = [ctypes.c_void_p]
+    lib.dsx_server_pool_clear.argtypes = [ctypes.c_int]
     lib.dsx_submit_scan.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, _u64p]
     lib.dsx_submit_area.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p]
     lib.dsx_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(Result)]
@@ -205,7 +206,7 @@ def load(path=LIB_PATH):
     lib.dsx_dfunc.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
     for f in ("dsx_create", "dsx_destroy", "dsx_gate", "dsx_gate_amps", "dsx_gate_profile", "dsx_scan", "dsx_tables",
               "dsx_noise", "dsx_climate", "dsx_predicates", "dsx_flood", "dsx_flood_masks", "dsx_points", "dsx_search", "dsx_area_queries",
-              "dsx_server_create", "dsx_server_destroy", "dsx_server_workers", "dsx_submit_scan", "dsx_submit_area",
+              "dsx_server_create", "dsx_server_destroy", "dsx_server_workers", "dsx_server_pool_clear", "dsx_submit_scan", "dsx_submit_area",
               "dsx_wait", "dsx_poll", "dsx_result_text", "dsx_release", "dsx_profile", "dsx_flood_stats_get", "dsx_set_precision", "dsx_noise_f32", "dsx_dfunc"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib

@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -30,6 +31,55 @@ extern thread_local std::string g_last_error;
 }
 
 namespace {
+
+// Contexts of destroyed servers, kept for the next server on the same device (DSX_SERVER_POOL=0: none
+// kept). Creating and destroying a server's 17 contexts (a stream, the flood engine's buffers, pinned
+// staging each) one server after another in one process made later servers' bursts stall now and then;
+// a context keeps no request state between calls, so a pooled one serves the next server as a new one
+// would. dsx_server_pool_clear frees them. The pool is never destroyed: its contexts may outlive the
+// runtime's static teardown.
+struct CtxPool {
+    std::mutex mu;
+    std::map<int, std::vector<dsx_ctx *>> free;
+    std::map<int, uint64_t> footprint;   // one worker's measured device bytes, per device
+};
+CtxPool &ctx_pool() {
+    static CtxPool *p = new CtxPool;
+    return *p;
+}
+bool pool_on() {
+    static const bool on = [] {
+        const char *e = std::getenv("DSX_SERVER_POOL");
+        return !(e && *e == '0');
+    }();
+    return on;
+}
+// *fresh (optional): whether the context was created now rather than taken from the pool
+int pool_get(int device, dsx_ctx **out, bool *fresh = nullptr) {
+    if (fresh) *fresh = false;
+    if (pool_on()) {
+        CtxPool &P = ctx_pool();
+        std::lock_guard<std::mutex> l(P.mu);
+        auto &v = P.free[device];
+        if (!v.empty()) {
+            *out = v.back();
+            v.pop_back();
+            return DSX_OK;
+        }
+    }
+    if (fresh) *fresh = true;
+    return dsx_create(device, out);
+}
+void pool_put(int device, dsx_ctx *c) {
+    if (!c) return;
+    if (!pool_on()) {
+        dsx_destroy(c);
+        return;
+    }
+    CtxPool &P = ctx_pool();
+    std::lock_guard<std::mutex> l(P.mu);
+    P.free[device].push_back(c);
+}
 
 constexpr uint64_t TILE_CELLS = 32 * 32;
 constexpr uint64_t POOL_TILES = 1ull << 21;   // tiles of a context's flood engine (floods_api.hip)
@@ -314,7 +364,7 @@ int dsx_server_create(int device, const dsx_server_config *cfg, dsx_server **out
     auto s = std::make_unique<dsx_server>();
     s->device = device;
     s->cfg = *cfg;
-    int rc = dsx_create(device, &s->gate_ctx);
+    int rc = pool_get(device, &s->gate_ctx);
     if (rc) return rc;
     const int want = cfg->workers ? cfg->workers : 16;
     int limit = want;
@@ -322,9 +372,21 @@ int dsx_server_create(int device, const dsx_server_config *cfg, dsx_server **out
     if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&free0, &total) != hipSuccess) rc = DSX_E_HIP;
     for (int k = 0; k < limit && rc == DSX_OK; k++) {
         dsx_ctx *w = nullptr;
-        if ((rc = dsx_create(device, &w))) break;
+        bool fresh = false;
+        if ((rc = pool_get(device, &w, &fresh))) break;
         s->wctx.push_back(w);
         if (k > 0) continue;
+        if (!fresh) {
+            // a pooled first worker: its buffers are in place, and the server that created it measured them
+            uint64_t per = 0;
+            {
+                CtxPool &P = ctx_pool();
+                std::lock_guard<std::mutex> l(P.mu);
+                per = P.footprint[device];
+            }
+            if (cfg->hbm_budget && per) limit = int(std::clamp<uint64_t>(cfg->hbm_budget / per, 1, uint64_t(want)));
+            continue;
+        }
         // the first worker's device footprint (buffers are allocated on first use: touch them) sets
         // how many workers the HBM budget admits
         dsx_area_query q{};
@@ -345,6 +407,11 @@ int dsx_server_create(int device, const dsx_server_config *cfg, dsx_server **out
             break;
         }
         const uint64_t per = free0 > free1 ? uint64_t(free0 - free1) : 1;
+        {
+            CtxPool &P = ctx_pool();
+            std::lock_guard<std::mutex> l(P.mu);
+            P.footprint[device] = per;
+        }
         if (cfg->hbm_budget) limit = int(std::clamp<uint64_t>(cfg->hbm_budget / per, 1, uint64_t(want)));
     }
     if (rc) {
@@ -375,9 +442,20 @@ int dsx_server_destroy(dsx_server *s) {
     s->cv_work.notify_all();
     s->cv_gate.notify_all();
     for (auto &t : s->threads) t.join();
-    for (dsx_ctx *w : s->wctx) dsx_destroy(w);
-    dsx_destroy(s->gate_ctx);
+    for (dsx_ctx *w : s->wctx) pool_put(s->device, w);
+    pool_put(s->device, s->gate_ctx);
     delete s;
+    return DSX_OK;
+}
+
+int dsx_server_pool_clear(int device) {
+    std::vector<dsx_ctx *> v;
+    {
+        CtxPool &P = ctx_pool();
+        std::lock_guard<std::mutex> l(P.mu);
+        v.swap(P.free[device]);
+    }
+    for (dsx_ctx *c : v) dsx_destroy(c);
     return DSX_OK;
 }
 

@@ -399,8 +399,12 @@ typedef struct {
     uint64_t csv_bytes, seeds_bytes;  /* scan rows: PREFIX.csv and PREFIX.seeds.csv text sizes */
 } dsx_result;
 int dsx_server_create(int device, const dsx_server_config *cfg, dsx_server **out);
-/* Finishes the requests in flight, then frees the workers. */
+/* Finishes the requests in flight, then stops the workers; their contexts (and their device memory) are
+ * kept for the next server on the device (DSX_SERVER_POOL=0: freed here). */
 int dsx_server_destroy(dsx_server *s);
+/* No counterpart in the reference (serving only): frees the contexts that destroyed servers left for reuse
+ * on the device. */
+int dsx_server_pool_clear(int device);
 int dsx_server_workers(const dsx_server *s);
 /* GPU-gate scan of seeds splitmix64(k), k in [start, start + count). */
 int dsx_submit_scan(dsx_server *s, uint64_t start, uint64_t count, uint64_t *id);

@@ -150,6 +150,7 @@ def test_server_hbm_budget():
     budget all of them, and a budget of 2.5 one-worker servers (the gate context plus one worker, so at least
     one worker's footprint each) two or three. A budget-limited server still gives the reference's rows."""
     dsx = load_package().dsx
+    dsx.load().dsx_server_pool_clear(0)   # the contexts of servers closed before: s1 starts from fresh ones
     free0 = _free_device_bytes()
     s1 = dsx.Server(0, workers=6, hbm_budget=1)
     try:
@@ -176,6 +177,41 @@ def test_server_hbm_budget():
         assert s3.workers == 3
     finally:
         s3.close()
+
+
+def test_server_pool_reuse():
+    """A destroyed server's contexts serve the next server on the device (no new device memory), with the same
+    results; dsx_server_pool_clear returns their memory."""
+    dsx = load_package().dsx
+    lib = dsx.load()
+    q, _ = dsx.area_workload(256)
+    def run():
+        ids = [s.submit_area(int(r["seed"]), int(r["x"]), int(r["z"]), int(r["half"])) for r in q]
+        out = []
+        for rid in ids:
+            r = s.wait(rid)
+            assert r.status == 0
+            a = r.area
+            out.append((a.cells, a.sampled, a.minx, a.maxx, a.minz, a.maxz, a.edge, a.border))
+            s.release(rid)
+        return out
+    lib.dsx_server_pool_clear(0)
+    s = dsx.Server(0, workers=3)
+    try:
+        first = run()
+    finally:
+        s.close()
+    free_pooled = _free_device_bytes()
+    s = dsx.Server(0, workers=3)
+    try:
+        assert _free_device_bytes() >= free_pooled - (64 << 20)   # reused: no new contexts
+        assert run() == first
+    finally:
+        s.close()
+    assert lib.dsx_server_pool_clear(0) == 0
+    freed = _free_device_bytes() - free_pooled
+    print(f"pool of 4 contexts held {freed / 2 ** 30:.2f} GiB", flush=True)
+    assert freed > 1 << 28
 
 
 def test_server_alone_equals_mix(server):
