@@ -354,6 +354,38 @@ __device__ int desert_state(const ClimSeed &C, const SurfGrid &g, int x, int z, 
     return pick() == 2 ? 1 : 0;
 }
 
+// The two sides of a noise computed apart (by two waves), in the operations of noise_sample and
+// noise_octaves: side 0 and side 1 of noise_sample's sums at y = 0 (the caller forms (sum0 + sum1) *
+// factor); side 0's running sum of noise_octaves from 0, or side 1's weighted terms, which the caller adds
+// to side 0's sum in order before multiplying by factor.
+template <class CS>
+__device__ __forceinline__ double noise_side(const CS &C, int ch, int side, double x, double z) {
+    const int n = clim_n(ch);
+    const double sc = 337. / 331.;
+    const double px = side ? x * sc : x, py = side ? 0. * sc : 0., pz = side ? z * sc : z;
+    double sum = 0;
+#pragma unroll 1
+    for (int j = 0; j < n; j++) {
+        const ClimOct &o = C.oct[oct_slot<CS>(ch, side, j)];
+        sum += o.weight * perlin_sample(C.g, o, px * o.freq, py * o.freq, pz * o.freq);
+    }
+    return sum;
+}
+template <class CS>
+__device__ __forceinline__ double octaves_side(const CS &C, int ch, int side, double x, double z, int hi, double *terms) {
+    const int n = clim_n(ch);
+    const double sc = side ? 337.0 / 331.0 : 1.0;
+    double s = 0;
+#pragma unroll 1
+    for (int k = 0; k < hi && k < n; k++) {
+        const ClimOct &o = C.oct[oct_slot<CS>(ch, side, k)];
+        const double t = o.weight * perlin_sample(C.g, o, x * sc * o.freq, 0, z * sc * o.freq);
+        if (side) terms[k] = t;
+        else s += t;
+    }
+    return s;
+}
+
 // dsearch_desert.cpp:334
 template <class CS>
 __device__ __forceinline__ int hot_state(const CS &C, int x, int z) {

@@ -156,6 +156,95 @@ __device__ __forceinline__ void eval_subblock(int sb, int tx, int tz, const Floo
     }
 }
 
+// The light instance's FP64 evaluation by pairs of waves: the pair's first wave takes side 0 of every noise,
+// its second side 1, for the same 64 cells (one per lane), which halves the octave samples per lane (T: 2 of
+// 4; a hot proxy cell's C and E: 5 of 10). Side 1's sums and terms reach side 0 through X, and side 0
+// finishes each value in noise_sample's and noise_octaves' operation order (so the states are those of
+// hot_state / proxy_state). sb < 0: the pair has no sub-block this pass but takes part in the block's
+// barriers, which every wave of the block must reach the same number of times.
+struct PairX {
+    double t1[64];       // side 1's T sum
+    double c1[3][64];    // side 1's weighted C terms (octaves 0-2)
+    double e1[2][64];    // side 1's weighted E terms (octaves 0-1)
+    uint32_t hot[64];    // side 0's temperature test, for side 1
+};
+template <class CS>
+__device__ __forceinline__ void eval_pair(int sb, int side, int tx, int tz, const FloodJob &J, const CS &C, PairX &X,
+                                          uint32_t *s_pass, uint32_t *s_cnt, uint32_t *s_hot) {
+    const int lane = threadIdx.x & 63;
+    const int bx = sb < 0 ? 0 : sb % NSB, bz = sb < 0 ? 0 : sb / NSB;
+    const int r = bz * SB + (lane >> 3), col = bx * SB + (lane & 7);
+    const int i = tx * FT + col, j = tz * FT + r;
+    const int nw = int(nwin(J));
+    int x = 0, z = 0;
+    bool in = false;
+    if (sb >= 0 && i < nw && j < nw) {
+        x = J.qx + (i - J.half) * J.step;
+        z = J.qz + (j - J.half) * J.step;
+        in = x >= -WORLD_Q && x <= WORLD_Q && z >= -WORLD_Q && z <= WORLD_Q;
+    }
+    // T: each side's sum, side 1's to side 0
+    const double ts = in ? noise_side(C, DSX_CH_T, side, x, z) : 0.0;
+    if (side == 1) X.t1[lane] = ts;
+    __syncthreads();
+    bool hot = false;
+    int v = 0;
+    if (side == 0 && in) {
+        const double T = (ts + X.t1[lane]) * C.factor[DSX_CH_T];
+        hot = T >= 0.55;
+        if (J.kind == PRED_HOT) v = hot ? 2 : 0;
+    }
+    if (J.kind == PRED_PROXY) {   // (uniform: one tile's job)
+        if (side == 0) X.hot[lane] = hot ? 1u : 0u;
+        __syncthreads();
+        const bool h = side == 0 ? hot : X.hot[lane] != 0u;
+        double cs = 0, es = 0;   // side 0: C's and E's running sums over its octaves
+        if (in && h) {
+            if (side == 0) {
+                cs = octaves_side(C, DSX_CH_C, 0, x, z, 3, nullptr);
+                es = octaves_side(C, DSX_CH_E, 0, x, z, 2, nullptr);
+            } else {
+                double tc[3], te[2];
+                octaves_side(C, DSX_CH_C, 1, x, z, 3, tc);
+                octaves_side(C, DSX_CH_E, 1, x, z, 2, te);
+#pragma unroll
+                for (int k = 0; k < 3; k++) X.c1[k][lane] = tc[k];
+#pragma unroll
+                for (int k = 0; k < 2; k++) X.e1[k][lane] = te[k];
+            }
+        }
+        __syncthreads();
+        if (side == 0 && in) {
+            float prob = 0.f;
+            if (hot) {
+                double c = cs, e = es;   // noise_octaves: side 0's sum, side 1's terms in order, times factor
+                for (int k = 0; k < 3; k++) c += X.c1[k][lane];
+                for (int k = 0; k < 2; k++) e += X.e1[k][lane];
+                c = c * C.factor[DSX_CH_C];
+                e = e * C.factor[DSX_CH_E];
+                int ii = int(floor((c + 0.8) / 0.05)), jj = int(floor((e + 1.0) / 0.05));
+                ii = ii < 0 ? 0 : (ii > 39 ? 39 : ii);
+                jj = jj < 0 ? 0 : (jj > 39 ? 39 : jj);
+                prob = DSX_DPROB[ii][jj];
+            }
+            v = prob >= J.cut ? 2 : 0;
+        }
+    }
+    __syncthreads();   // X is free for the pair's next sub-block
+    if (side == 0 && sb >= 0) {
+        const uint64_t mp = __ballot(v != 0), mc = __ballot(v == 2), mh = __ballot(J.kind == PRED_PROXY && hot);
+        if (lane < SB) {   // lane rr: row bz*8 + rr of the sub-block
+            const int sh = lane * SB;
+            const int row = bz * SB + lane;
+            const uint32_t p = uint32_t((mp >> sh) & 0xffu) << (bx * SB), c = uint32_t((mc >> sh) & 0xffu) << (bx * SB),
+                           h = uint32_t((mh >> sh) & 0xffu) << (bx * SB);
+            if (p) atomicOr(&s_pass[row], p);
+            if (c) atomicOr(&s_cnt[row], c);
+            if (h) atomicOr(&s_hot[row], h);
+        }
+    }
+}
+
 // One wave: grow the reached set of tile T through its evaluated passable cells to a fixpoint. Returns
 // the sub-blocks that hold touched cells not yet evaluated (reached cells' neighbours, and cells
 // touched from neighbouring tiles). When none are left (0), the reached set is final: it is stored
@@ -284,6 +373,7 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
     // spec bit 1: raise this kernel's waves' issue priority (s_setprio) above the default of other
     // kernels sharing the SIMDs, e.g. the VALU-bound gate; bit 0: speculative sub-blocks
     if (spec & 2) __builtin_amdgcn_s_setprio(2);
+    const int spec_pairs = spec & 4;
     spec &= 1;
     const uint64_t t_born = tprof ? __builtin_amdgcn_s_memrealtime() : 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {   // what the first workgroup finds (time-limit diagnostics)
@@ -296,6 +386,11 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
     __shared__ uint32_t s_pass[FT], s_cnt[FT], s_hot[FT];
     __shared__ uint32_t s_item, s_again, s_need, s_evald, s_hint;
     __shared__ int s_stop;
+    // (light instance, FP64) sub-blocks evaluated by pairs of waves, one noise side each (eval_pair);
+    // DSX_FLOOD_PAIR=0 keeps one wave per sub-block (spec bit 2 off)
+    constexpr bool PAIRS = LIGHT && PRE == 0 && NWAVE >= 2 && NWAVE % 2 == 0;
+    __shared__ PairX s_x[PAIRS ? NWAVE / 2 : 1];
+    const bool pairs = PAIRS && spec_pairs != 0;
     const int tid = threadIdx.x, wave = tid >> 6;
     int loaded_seed = -1, loaded_cls = -1;
     if (sa.nst) {
@@ -427,7 +522,8 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                     // sub-blocks next to the needed ones that are not evaluated yet, inside the window: the
                     // flood usually grows into them next, and evaluated cells are only a cache of the
                     // predicate (what is reached and counted still follows the touched cells alone)
-                    int idle = (NWAVE - __popc(need) % NWAVE) % NWAVE;
+                    const int slots = pairs ? NWAVE / 2 : NWAVE;   // sub-blocks one evaluation pass takes
+                    int idle = (slots - __popc(need) % slots) % slots;
                     if (idle) {
                         const int nw = int(nwin(J));
                         uint32_t nb = 0;
@@ -473,10 +569,26 @@ __global__ __launch_bounds__(NT, OCC) void k_flood_run(const FloodJob *jobs, con
                     loaded_cls = cls;
                     if (tprof && tid == 0) atomicAdd(&B.ctl[PROF_LOAD], uint32_t(__builtin_amdgcn_s_memrealtime() - te0));
                 }
+                if constexpr (PAIRS) {
+                    if (pairs) {
+                        // pass q: the (2q + p)-th needed sub-block to pair p (both of its waves), every pair
+                        // through the same number of passes
+                        const int np = NWAVE / 2, pair = wave >> 1, side = wave & 1, nn = __popc(need);
+                        uint32_t m = need;
+                        for (int q = 0; q * np < nn; q++) {
+                            int sb = -1;
+                            for (int k = 0; k < np && m; k++, m &= m - 1)
+                                if (k == pair) sb = __ffs(m) - 1;
+                            eval_pair(sb, side, tx, tz, J, C, s_x[pair], s_pass, s_cnt, s_hot);
+                        }
+                    }
+                }
                 // the k-th needed sub-block goes to wave k % NWAVE
-                int k = 0;
-                for (uint32_t m = need; m; m &= m - 1, k++)
-                    if (k % NWAVE == wave) eval_subblock<PRE, LIGHT>(__ffs(m) - 1, tx, tz, J, C, g, s_pass, s_cnt, s_hot);
+                if (!pairs) {
+                    int k = 0;
+                    for (uint32_t m = need; m; m &= m - 1, k++)
+                        if (k % NWAVE == wave) eval_subblock<PRE, LIGHT>(__ffs(m) - 1, tx, tz, J, C, g, s_pass, s_cnt, s_hot);
+                }
                 __syncthreads();
                 if (tid == 0) {
                     s_evald |= need;
@@ -696,7 +808,8 @@ void launch_flood_run(hipStream_t st, const FloodJob *jobs, const ClimSeed *cs, 
     static const int spec = [] {
         const char *v = std::getenv("DSX_FLOOD_SPEC");   // 0 turns the speculative sub-blocks off
         const char *p = std::getenv("DSX_FLOOD_PRIO");   // 1: flood waves at raised issue priority
-        return (v && *v ? (std::atoi(v) != 0 ? 1 : 0) : 1) | (p && std::atoi(p) ? 2 : 0);
+        const char *w = std::getenv("DSX_FLOOD_PAIR");   // 0: one wave per sub-block in the light instance
+        return (v && *v ? (std::atoi(v) != 0 ? 1 : 0) : 1) | (p && std::atoi(p) ? 2 : 0) | (w && *w == '0' ? 0 : 4);
     }();
     static const bool light_on = [] {
         const char *v = std::getenv("DSX_FLOOD_LIGHT");   // 0: always the full instance
