@@ -4,6 +4,7 @@
 #include "dsx_rng.h"
 #include "ctx.h"
 #include "floods.h"
+#include "../host/pipeline.h"
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -664,6 +665,25 @@ int dsx_flood_masks(dsx_ctx *c, dsx_tile_mask *out, uint64_t cap, uint64_t *n_ou
     if (w > wh) std::memcpy(out + wh, S->masks.data(), (w - wh) * sizeof(dsx_tile_mask));
     return S->masks_n > cap ? DSX_E_CAPACITY : DSX_OK;
 }
+
+}  // extern "C"
+
+namespace dsxh {
+// The last dsx_flood call's masks where they lie (no copy): a (pinned memory), then b (the overflow
+// beyond the pinned buffer). Valid until the context's next flood call.
+int flood_masks_view(dsx_ctx *c, const dsx_tile_mask **a, uint64_t *na, const dsx_tile_mask **b, uint64_t *nb) {
+    if (!c || !a || !na || !b || !nb) return DSX_E_ARG;
+    ClimState *S = clim_state(c);
+    static_assert(sizeof(dsx_tile_mask) == sizeof(dsx::TileMask), "mask layout");
+    *na = S->masks_h;
+    *a = reinterpret_cast<const dsx_tile_mask *>(S->h_masks);
+    *nb = S->masks_n - S->masks_h;
+    *b = reinterpret_cast<const dsx_tile_mask *>(S->masks.data());
+    return DSX_OK;
+}
+}  // namespace dsxh
+
+extern "C" {
 
 int dsx_points(dsx_ctx *c, const uint64_t *seeds, uint64_t nseeds, const uint32_t *sidx, const int32_t *xs,
                const int32_t *zs, uint64_t n, int what, float cut, int8_t *out8, double *outd) {

@@ -33,6 +33,9 @@ struct GroupText {
 void format_group(const Group &g, const dsx_search_params &P, GroupText &t, double *best);
 
 int worker_contexts(dsx_ctx *ctx, int n, std::vector<dsx_ctx *> &out);
+// The last dsx_flood call's exported masks in place (dsx_flood_masks without the copy): na of them at a,
+// then nb at b; valid until the context's next flood call.
+int flood_masks_view(dsx_ctx *c, const dsx_tile_mask **a, uint64_t *na, const dsx_tile_mask **b, uint64_t *nb);
 uint64_t env_u64(const char *name, uint64_t def);
 
 }  // namespace dsxh

@@ -213,7 +213,7 @@ struct Sched {
     std::vector<PointsReq *> points;
     std::vector<FloodReq *> floods;
     int err = DSX_OK;
-    std::vector<dsx_tile_mask> masks;
+    std::vector<uint32_t> moff, mpos, midx;   // flood_range's scratch: masks grouped by job
 
     int run_points(std::vector<PointsReq *> &rs) {
         // group by (what, cut); each group one dsx_points call over many seeds
@@ -289,32 +289,39 @@ struct Sched {
             o.half = jobs[k].half;
         }
         if (cells) {
+            // each wanted job's masks, sorted by (tz, tx), read once from where the engine left them:
+            // mask indices grouped by job (counting sort), each group's indices sorted by tile
             Timer tmk(11);
-            uint64_t nm = 0;
-            dsx_flood_masks(ctx, nullptr, 0, &nm);
-            masks.resize(nm);
-            if (nm && (rc = dsx_flood_masks(ctx, masks.data(), nm, &nm))) return rc;
-            std::vector<uint32_t> cnt(hi - lo, 0);
-            for (const dsx_tile_mask &m : masks) {
-                if (m.job >= hi - lo) {
+            const dsx_tile_mask *ma = nullptr, *mb = nullptr;
+            uint64_t na = 0, nb = 0;
+            if ((rc = flood_masks_view(ctx, &ma, &na, &mb, &nb))) return rc;
+            auto mask = [&](uint32_t i) -> const dsx_tile_mask & { return i < na ? ma[i] : mb[i - na]; };
+            const uint32_t nm = uint32_t(na + nb), nj = uint32_t(hi - lo);
+            moff.assign(size_t(nj) + 1, 0);
+            for (uint32_t i = 0; i < nm; i++) {
+                const uint32_t j = mask(i).job;
+                if (j >= nj) {
                     dsx::g_last_error = "flood mask of a job outside the batch";
                     return DSX_E_HIP;
                 }
-                cnt[m.job]++;
+                moff[j + 1]++;
             }
-            for (size_t k = lo; k < hi; k++)
-                if (cnt[k - lo] && who[k].first->want_cells) who[k].first->out[who[k].second].tiles.reserve(cnt[k - lo]);
-            for (const dsx_tile_mask &m : masks) {
-                auto [r, k] = who[lo + m.job];
-                if (r->want_cells) r->out[k].tiles.push_back(m);
-            }
-            for (size_t k = lo; k < hi; k++) {
-                auto [r, q] = who[k];
-                auto &t = r->out[q].tiles;
-                if (t.size() > 1)
-                    std::sort(t.begin(), t.end(), [](const dsx_tile_mask &a, const dsx_tile_mask &b) {
-                        return a.tz != b.tz ? a.tz < b.tz : a.tx < b.tx;
+            for (uint32_t k = 0; k < nj; k++) moff[k + 1] += moff[k];
+            mpos.assign(moff.begin(), moff.end() - 1);
+            midx.resize(nm);
+            for (uint32_t i = 0; i < nm; i++) midx[mpos[mask(i).job]++] = i;
+            for (uint32_t k = 0; k < nj; k++) {
+                auto [r, q] = who[lo + k];
+                if (!r->want_cells || moff[k + 1] == moff[k]) continue;
+                const auto b = midx.begin() + moff[k], e = midx.begin() + moff[k + 1];
+                if (e - b > 1)
+                    std::sort(b, e, [&](uint32_t x, uint32_t y) {
+                        const dsx_tile_mask &p = mask(x), &o = mask(y);
+                        return p.tz != o.tz ? p.tz < o.tz : p.tx < o.tx;
                     });
+                auto &t = r->out[q].tiles;
+                t.resize(size_t(e - b));
+                for (size_t u = 0; u < t.size(); u++) t[u] = mask(b[u]);
             }
         }
         return DSX_OK;
