@@ -1028,18 +1028,25 @@ Group::Group() = default;
 Group::~Group() = default;
 
 // Tables, scan and sites of a group's seeds (g.outs).
+std::atomic<uint64_t> g_cand_cap{1u << 12};
 int group_scan(dsx_ctx *w, const dsx_search_params &P, int mode, Group &g) {
     const dsx_scan_params sp{float(P.ta), float(P.t), float(P.wc), float(P.we), P.sigma, P.sigmac};
     const size_t nb = g.seeds.size();
     auto tscan = std::chrono::steady_clock::now();
     std::vector<float> mta(nb), mtb(nb);
     std::vector<uint64_t> ncand(nb);
-    uint64_t cap = 1 << 12;
+    // room per seed: the most candidates any seed of the process has had, up to 2^15 (a seed with more
+    // makes dsx_scan report the capacity, and the group is scanned again with room for it)
+    uint64_t cap = g_cand_cap.load(std::memory_order_relaxed);
     thread_local std::vector<dsx_raw_cand> raw;   // reused: dsx_scan writes the ncand[k] entries read below
     if (raw.size() < nb * cap) raw.resize(nb * cap);
     int rc = dsx_scan(w, g.seeds.data(), nb, &sp, mta.data(), mtb.data(), raw.data(), cap, ncand.data());
     if (rc == DSX_E_CAPACITY) {
-        cap = *std::max_element(ncand.begin(), ncand.end());
+        cap = std::max(cap, *std::max_element(ncand.begin(), ncand.end()));
+        const uint64_t keep = std::min<uint64_t>(cap, 1u << 15);
+        uint64_t seen = g_cand_cap.load(std::memory_order_relaxed);
+        while (seen < keep && !g_cand_cap.compare_exchange_weak(seen, keep, std::memory_order_relaxed)) {
+        }
         if (raw.size() < nb * cap) raw.resize(nb * cap);
         rc = dsx_scan(w, g.seeds.data(), nb, &sp, mta.data(), mtb.data(), raw.data(), cap, ncand.data());
     }
