@@ -112,17 +112,35 @@ struct KeySet {
     // kx0..kx1, kz0..kz1: the key range of the flood's counted cells (its bbox, plus the sub-points)
     struct Cover { const FloodOut *f; int shift, sub, bx, bz, kx0, kx1, kz0, kz1; };
     std::vector<Cover> covers;
+    // Covers by 8192-quart cell of the offset quart plane: a cover is listed in every cell its key range
+    // overlaps, so a query reads one cell's list instead of every cover (membership is an OR over the
+    // covers, so the order they are tried in does not matter).
+    static constexpr int CELL = 13;
+    std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
+    static uint64_t cell_key(int64_t cx, int64_t cz) { return (uint64_t(uint32_t(cx)) << 32) | uint32_t(cz); }
     // insert every counted cell of f, lattice point x0 + (i - half) * step plus `sub` sub-points per axis
     // spaced 2^shift (sub = 2: the step-256 pre-flood's 4 keys per cell, dsearch_desert.cpp:345)
     void add(const FloodOut &f, int shift, int sub) {
-        covers.push_back({&f, shift, sub, (f.qx - f.half * f.step + 8000000) >> shift,
-                          (f.qz - f.half * f.step + 8000000) >> shift, (f.a.minx + 8000000) >> shift,
-                          ((f.a.maxx + 8000000) >> shift) + sub - 1, (f.a.minz + 8000000) >> shift,
-                          ((f.a.maxz + 8000000) >> shift) + sub - 1});
+        const Cover c{&f, shift, sub, (f.qx - f.half * f.step + 8000000) >> shift,
+                      (f.qz - f.half * f.step + 8000000) >> shift, (f.a.minx + 8000000) >> shift,
+                      ((f.a.maxx + 8000000) >> shift) + sub - 1, (f.a.minz + 8000000) >> shift,
+                      ((f.a.maxz + 8000000) >> shift) + sub - 1};
+        covers.push_back(c);
+        if (f.a.cells == 0) return;   // never matches
+        // offset quarts k with k >> shift in [kx0, kx1]: [kx0 << shift, ((kx1 + 1) << shift) - 1]
+        const int64_t x0 = (int64_t(c.kx0) * (int64_t(1) << shift)) >> CELL,
+                      x1 = ((int64_t(c.kx1) + 1) * (int64_t(1) << shift) - 1) >> CELL,
+                      z0 = (int64_t(c.kz0) * (int64_t(1) << shift)) >> CELL,
+                      z1 = ((int64_t(c.kz1) + 1) * (int64_t(1) << shift) - 1) >> CELL;
+        for (int64_t gz = z0; gz <= z1; gz++)
+            for (int64_t gx = x0; gx <= x1; gx++) grid[cell_key(gx, gz)].push_back(uint32_t(covers.size() - 1));
     }
     bool has(int x, int z) const {
         const int kx = (x + 8000000), kz = (z + 8000000);
-        for (const Cover &c : covers) {
+        const auto it = grid.find(cell_key(int64_t(kx) >> CELL, int64_t(kz) >> CELL));
+        if (it == grid.end()) return false;
+        for (const uint32_t ci : it->second) {
+            const Cover &c = covers[ci];
             if (c.f->a.cells == 0) continue;
             const int qx = kx >> c.shift, qz = kz >> c.shift;
             if (qx < c.kx0 || qx > c.kx1 || qz < c.kz0 || qz > c.kz1) continue;
@@ -131,6 +149,13 @@ struct KeySet {
             if (c.f->counted(mx / c.sub, mz / c.sub)) return true;
         }
         return false;
+    }
+};
+
+// hash of a quart coordinate pair
+struct XYHash {
+    size_t operator()(const std::pair<int, int> &p) const {
+        return std::hash<uint64_t>()((uint64_t(uint32_t(p.first)) << 32) | uint32_t(p.second));
     }
 };
 
@@ -667,6 +692,7 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
     // computed ahead are exactly what the sequential loop would compute; the loop decides which to use.
     using XY = std::pair<int, int>;
     std::map<XY, size_t> pre_idx, h128_idx;
+    std::vector<size_t> pre_of(ns, 0), h128_of(ns, 0);   // per site: its hot point's pre-flood and step-128 flood
     FloodReq pre{&S, {}, true, {}, {}};          // step-256 pre-floods of every distinct hot point
     // A pre-flood only decides cells * 256^2 * 16 / 1e6 < hot_pre * hot_min (dsearch_desert.cpp:337-339)
     // and its cells are used only when it does; it may stop at the smallest count that fails the test.
@@ -680,8 +706,10 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
     FloodReq h128{&S, {}, true, {}, {}};         // step-128 floods of every big one
     if (P.hot_min > 0) {
         for (size_t r = 0; r < ns; r++) {
-            if (bt[r] < 0.55 || pre_idx.count({hx[r], hz[r]})) continue;
-            pre_idx[{hx[r], hz[r]}] = pre.jobs.size();
+            if (bt[r] < 0.55) continue;
+            const auto [it, fresh] = pre_idx.try_emplace({hx[r], hz[r]}, pre.jobs.size());
+            pre_of[r] = it->second;
+            if (!fresh) continue;
             pre.jobs.push_back(fjob(seed, DSX_PRED_HOT, 0.f, hx[r], hz[r], 256, 32000 / 256, 0));
             pre.jobs.back().stop_cells = pre_stop;
         }
@@ -690,6 +718,11 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
             if (P.hot_pre <= 0 || pre.out[k].a.cells * 256.0 * 256 * 16 / 1e6 >= P.hot_pre * P.hot_min) {
                 h128_idx[xy] = h128.jobs.size();
                 h128.jobs.push_back(fjob(seed, DSX_PRED_HOT, 0.f, xy.first, xy.second, 128, 32000 / 128, 0));
+            }
+        for (size_t r = 0; r < ns; r++)
+            if (bt[r] >= 0.55) {
+                const auto it = h128_idx.find({hx[r], hz[r]});
+                h128_of[r] = it == h128_idx.end() ? 0 : it->second;
             }
         co_await h128;
     }
@@ -720,7 +753,7 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
     // Proxy floods ahead of the loop, in rounds: each round floods, per zone, the first passable start
     // that no flood so far covers. A start the loop will flood is one that no earlier flood covers, so
     // after the rounds the loop almost never has to wait (the covered set it sees is a subset).
-    std::map<XY, const FloodOut *> prox_idx;
+    std::unordered_map<XY, const FloodOut *, XYHash> prox_idx;
     std::vector<std::unique_ptr<FloodReq>> prox_rounds;
     KeySet spec_cov;
     // Each zone's primary sequence is what the loop below floods: its first passable start that no
@@ -733,16 +766,14 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
     std::vector<size_t> cursor(zone_starts.size(), 0);
     std::vector<int> zrounds(zone_starts.size(), 0);
     std::vector<uint8_t> is_primary;   // per request of the current round
+    std::unordered_set<XY, XYHash> req_set;   // the starts the current round requests
     static const int extra_cap = int(env_u64("DSX_PROXY_EXTRA", 7)), extra_sep = int(env_u64("DSX_PROXY_SEP", 1024));
     for (;;) {
         std::optional<Timer> tr(17);
         auto req = std::make_unique<FloodReq>(FloodReq{&S, {}, true, {}, {}});
         is_primary.clear();
-        auto requested = [&](const XY &st) {
-            for (auto &j : req->jobs)
-                if (j.qx == st.first && j.qz == st.second) return true;
-            return false;
-        };
+        req_set.clear();
+        auto requested = [&](const XY &st) { return req_set.count(st) != 0; };
         for (size_t z = 0; z < zone_starts.size(); z++) {
             const size_t nz = zone_starts[z].size();
             size_t &k = cursor[z];
@@ -762,6 +793,7 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
             std::vector<XY> picks{st};
             if (!requested(st)) {
                 req->jobs.push_back(fjob(seed, DSX_PRED_PROXY, pcut, st.first, st.second, 32, 16000 / 32, 1));
+                req_set.insert(st);
                 is_primary.push_back(1);
             } else {
                 for (size_t j = 0; j < req->jobs.size(); j++)
@@ -777,6 +809,7 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
                 if (near) continue;
                 picks.push_back(sj);
                 req->jobs.push_back(fjob(seed, DSX_PRED_PROXY, pcut, sj.first, sj.second, 32, 16000 / 32, 1));
+                req_set.insert(sj);
                 is_primary.push_back(0);
             }
             zrounds[z]++;
@@ -800,16 +833,15 @@ Task seed_task(Sched &S, const dsx_search_params &P, SeedOut &O) {
         double hot_area = 0;
         if (P.hot_min > 0) {
             if (bt[rank] < 0.55 || hotcovered.has(hx[rank], hz[rank])) continue;
-            const XY hp{hx[rank], hz[rank]};
             if (P.hot_pre > 0) {
-                const FloodOut &h2 = pre.out[pre_idx[hp]];
+                const FloodOut &h2 = pre.out[pre_of[rank]];
                 if (h2.a.cells * 256.0 * 256 * 16 / 1e6 < P.hot_pre * P.hot_min) {
                     hotcovered.add(h2, 7, 2);
                     O.nhot++;
                     continue;
                 }
             }
-            const size_t z = h128_idx[hp];
+            const size_t z = h128_of[rank];
             const FloodOut &hzn = h128.out[z];
             hot_area = hzn.a.cells * 128.0 * 128 * 16 / 1e6;
             hotcovered.add(hzn, 7, 1);
@@ -992,11 +1024,26 @@ void make_sites(const dsx_raw_cand *raw, size_t nraw, const dsx_search_params &P
     }
     O.ncand = cand.size();
     std::sort(cand.begin(), cand.end(), [](const Site &a, const Site &b) { return a.s > b.s; });
+    // a candidate is dropped when a kept site lies within nms on both axes; kept sites are found by
+    // nms-sized cell, so only the 3 x 3 cells around the candidate can hold one
+    std::unordered_map<uint64_t, std::vector<uint32_t>> kept;
+    auto cell = [&](int v) { return uint32_t((int64_t(v) + 8000000) / P.nms); };
     for (auto &c : cand) {
         bool near = false;
-        for (auto &q : O.sites)
-            if (std::abs(q.x - c.x) < P.nms && std::abs(q.z - c.z) < P.nms) { near = true; break; }
-        if (!near) O.sites.push_back(c);
+        const uint32_t cx = cell(c.x), cz = cell(c.z);
+        for (uint32_t gz = cz - 1; gz != cz + 2 && !near; gz++)
+            for (uint32_t gx = cx - 1; gx != cx + 2 && !near; gx++) {
+                const auto it = kept.find((uint64_t(gx) << 32) | gz);
+                if (it == kept.end()) continue;
+                for (const uint32_t k : it->second) {
+                    const Site &q = O.sites[k];
+                    if (std::abs(q.x - c.x) < P.nms && std::abs(q.z - c.z) < P.nms) { near = true; break; }
+                }
+            }
+        if (!near) {
+            kept[(uint64_t(cx) << 32) | cz].push_back(uint32_t(O.sites.size()));
+            O.sites.push_back(c);
+        }
         if (int(O.sites.size()) >= P.max_sites) break;
     }
 }
